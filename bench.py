@@ -1,0 +1,156 @@
+"""Throughput of the explicit time step: 2-D compressible Euler, MUSCL(2) + Venkatakrishnan + Rusanov + MOOD
+(EPD1, u1), SSP-RK3, CFL 0.4, outflow walls, on the jittered h = 1e-3 cloud of the unit square (about 1.01 M nodes
+including the ghost band), four seeded constant quadrants.
+
+One step = get_time_step (CFL reduction on the device) + one RK3 step with its MOOD fixed points, exactly as the
+reference's driver loop (test/time_integration.jl:107-111). The timed region holds K steps with the state resident
+in HBM; the metric is node-stage updates per second = N * 3 * K / seconds.
+
+Prints one JSON line (rank 0). --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "hypercloudcore.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "node-stage updates/s, 2D Euler MUSCL+MOOD @1M nodes; % of HBM roofline"
+
+
+def flux_pass_bytes(N, N_int, S_int, D, M):
+    """Compulsory HBM bytes of one full flux pass of this implementation (neighbour data assumed re-used from
+    cache): per interior node its SELL indices (4 B per slot), count, order, state, gradients, position, the
+    derivative rows of the divergence inverse and the output; boundary nodes write a zero K."""
+    per_node = 4 + 1 + 1 + 8 * M + 8 * D * M + 8 * D + 8 * D * (D + 1) + 8 * M + 1
+    return N_int * per_node + 4 * S_int + (N - N_int) * (8 * M + 1)
+
+
+def gather_pass_bytes(N_int, S_int, D, M):
+    per_node = 4 + 1 + 1 + 8 * M + 8 * D + 8 * D * D + 8 * D * M + 1
+    return N_int * per_node + 4 * S_int
+
+
+def cpu_baseline(wl, steps, threads):
+    """The oracle (C++ restatement of the reference CPU path, OpenMP over host cores) on a bounded sample: a few
+    steps of the same cloud and state."""
+    from oracle import oracle as O
+    c = wl["cloud"]
+    o = O.OracleSim(wl["domain"], c["pos"], c["is_boundary"], c["tags"], wl["scheme"], threads=threads)
+    o.set_state(wl["U0"])
+    t0 = time.perf_counter()
+    for k in range(steps):
+        dt = wl["cfl"] * o.time_step()
+        o.step(0.0, dt)
+    t = time.perf_counter() - t0
+    return {"value": c["N"] * 3 * steps / t, "unit": "node-stage updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (CFL + RK3 + MOOD) of the same {c['N']}-node cloud and initial state, "
+                      f"{t:.1f} s on {threads} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1000, help="nodes per axis (h = 1/n)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import hcc_hip
+    from hcc_hip import Context, Grid, Stepper
+    from hcc_hip import workloads as W
+
+    t_setup = time.perf_counter()
+    wl = W.euler_2d(h=1.0 / args.n, seed=args.seed)
+    c = wl["cloud"]
+    ctx = Context(0)
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    t_setup = time.perf_counter() - t_setup
+    print(f"# setup {t_setup:.1f} s: N={g.N} nodes, S={g.S} slots, max_nb={g.max_nb}", file=sys.stderr, flush=True)
+
+    t = 0.0
+    for _ in range(args.warmup):
+        dt = wl["cfl"] * st.time_step()
+        st.step(t, dt)
+        t += dt
+    hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+    g_ms, f_ms, step_ms, flags, iters = [], [], [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = wl["cfl"] * st.time_step()
+        st.step(t, dt)
+        t += dt
+        a, b, cc = st.timing()
+        step_ms.append(a); g_ms.append(b); f_ms.append(cc)
+        iters += st.mood_iterations()
+    hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+    wall = time.perf_counter() - t0
+    flags = int(st.mood_flags().sum())
+    U = st.download()
+    assert np.all(np.isfinite(U)), "non-finite state"
+
+    N = g.N
+    isb = c["is_boundary"].astype(bool)
+    N_int = int((~isb).sum())
+    ranges, _, _, _ = g.neighbors(weights=False, distances=False)
+    counts = np.diff(ranges)
+    S_int = int(counts[~isb].sum())
+    nnb = S_int / max(N_int, 1)
+    D, M, stages = 2, 4, 3
+    value = N * stages * args.steps / wall
+    flux_avg_ms = float(np.mean(f_ms)) / stages
+    gather_avg_ms = float(np.mean(g_ms)) / stages
+    fb = flux_pass_bytes(N, N_int, S_int, D, M)
+    gb = gather_pass_bytes(N_int, S_int, D, M)
+    achieved = fb / (flux_avg_ms * 1e-3) / 1e9
+    model_b = W.bytes_per_node_stage(D, M, nnb)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2-D Euler four-quadrant, jittered h=1/{args.n} unit square with outflow ghost band, "
+                               f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4",
+                   "nodes": N, "interior_nodes": N_int, "mean_neighbours": round(nnb, 3), "stages": stages,
+                   "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_flux<2,4,Euler> (full pass)", "bytes_per_launch": fb,
+                     "avg_launch_ms": flux_avg_ms},
+        "detail": {"gather_avg_ms": gather_avg_ms, "gather_bytes": gb, "flux_avg_ms": flux_avg_ms,
+                   "step_event_ms": float(np.mean(step_ms)), "mood_iterations": iters, "flags_last_step": flags,
+                   "model_bytes_per_node_stage": model_b,
+                   "model_roofline_frac": value * model_b / (HBM_PEAK_GBS * 1e9)},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+// Internal declarations of libhcc (not part of the C ABI).
+#pragma once
+#include "../../include/hcc.h"
+#include "hcc_math.h"
+#include <string>
+#include <vector>
+
+struct hcc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    unsigned long long* h_pinned = nullptr;   // pinned host mirror of the device counters
+};
+
+namespace hcc {
+
+struct Xoshiro {
+    uint64_t s[4];
+    explicit Xoshiro(uint64_t seed);
+    uint64_t next();
+    double uniform();
+};
+
+// Geometry of a rectangular domain with its ghost canvas (ComputationalDomain) and the derived constants
+// the kernels use.
+struct DomainGeom {
+    hcc_domain_desc desc{};
+    int D = 0;
+    double irf = 0;                    // interp_range_factor
+    double cmin[3], cmax[3];           // canvas
+    double L[3], Lw[3], invLw[3];      // physical lengths; wrap lengths (0 on non-periodic axes)
+    double invL = 1, R_sq = 0, inv_range_sq = 0;
+    void init(const hcc_domain_desc& d);
+};
+
+struct HostCloud {
+    int64_t N = 0, morton_ties = 0;
+    std::vector<double> pos;
+    std::vector<uint8_t> isb;
+    std::vector<int32_t> tags;
+};
+
+void generate_cloud(const DomainGeom& g, HostCloud& out);
+void bin_window_table(int D, const int64_t* dims, const int32_t* periodic, std::vector<int32_t>& table);
+void outflow_roots(const DomainGeom& g, int64_t N, const double* pos, const uint8_t* isb, const int32_t* tags,
+                   const std::vector<int64_t>& rstart, const std::vector<int32_t>& nbr, int n_bc, const int32_t* bc_tag,
+                   const int32_t* bc_kind, std::vector<int32_t>& root);
+
+// ---- device-side argument blocks (passed by value to kernels) -------------------------------------------
+constexpr int kChunk = 64;             // SELL chunk = one wavefront of nodes
+
+struct GeomArgs {
+    int64_t N;
+    const double* px;
+    const double* py;
+    const double* pz;
+    const uint8_t* isb;
+    const int32_t* cnt;        // neighbours per node
+    const int64_t* chunk_off;  // SELL-64: first slot of each chunk of 64 nodes
+    const int32_t* nbr;        // SELL-64 neighbour indices: slot k of node i at chunk_off[i/64] + 64 k + i%64
+    double Lw[3], invLw[3];
+    double invL, alpha, inv_range_sq;
+};
+
+struct PdeArgs {
+    double gamma;
+    double c[3][8];
+    double amax[3];
+};
+
+struct StepArgs {
+    GeomArgs g;
+    PdeArgs pde;
+    int limiter, mode, strategy, criterion;
+    double delta;
+    const double* invg;          // reconstruction inverse moments, D*D per node: [q * N + i]
+    const double* invd;          // divergence inverse moments, derivative rows 1..D: [(r * (D+1) + c) * N + i]
+    const uint8_t* okg;
+    const uint8_t* okd;
+    double* grads;               // limited gradients: [(a * M + m) * N + i]
+    int8_t* orders;
+    uint8_t* changed;
+    uint8_t* dirty;
+    uint8_t* trig;               // mood_flags of the step
+    uint8_t* inadm;              // last check: candidate not admissible
+    unsigned long long* counters;// [0] orders changed in the last check, [1] inadmissible accepted, [2] dt bits
+};
+
+}  // namespace hcc
+
+struct hcc_grid {
+    hcc_ctx* ctx = nullptr;
+    hcc::DomainGeom geom;
+    int64_t N = 0;
+    double* d_pos[3] = {nullptr, nullptr, nullptr};
+    uint8_t* d_isb = nullptr;
+    int32_t* d_tags = nullptr;
+    // cell list
+    int64_t dims[3] = {1, 1, 1}, nbins = 0;
+    double bin_size[3] = {0, 0, 0};
+    int32_t* d_bin_of = nullptr;
+    int32_t* d_cell_start = nullptr;   // nbins + 2
+    int32_t* d_cell_cursor = nullptr;  // nbins + 1
+    int32_t* d_cell_particles = nullptr;
+    int32_t* d_bin_window = nullptr;   // nbins * 3^D
+    // neighbours (SELL-64)
+    int32_t* d_cnt = nullptr;
+    int64_t nchunks = 0, S = 0, S_sell = 0, max_nb = 0;
+    int64_t* d_chunk_off = nullptr;
+    int32_t* d_nbr = nullptr;
+    std::vector<int32_t> h_tags;
+    std::vector<uint8_t> h_isb;
+    hcc::GeomArgs args() const;
+};

@@ -1,0 +1,127 @@
+// Scalar building blocks of the hot path, compiled for gfx950 and for the host set-up code.
+//
+// Every function reproduces the Julia operation order of the reference so that the kernels give the
+// same IEEE-754 results (the library is built with -ffp-contract=off; FP64 division and sqrt are
+// correctly rounded on gfx950, as on the host).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+#define HCC_HD __host__ __device__ __forceinline__
+
+namespace hcc {
+
+constexpr double kEps = 2.220446049250313e-16;          // eps(Float64)
+constexpr double kFloatMin = 2.2250738585072014e-308;   // floatmin(Float64)
+constexpr int kMaxStages = 5;
+
+// Julia Base.max / Base.min on IEEE floats: NaN-propagating, -0 < +0 (julia/base/math.jl).
+HCC_HD double jmax(double x, double y) {
+    double diff = x - y;
+    double am = __builtin_signbit(diff) ? y : x;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? diff : am;
+}
+HCC_HD double jmin(double x, double y) {
+    double diff = x - y;
+    double am = __builtin_signbit(diff) ? x : y;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? diff : am;
+}
+// Julia Base.mod for floats: r = rem(x, y) (exact), copysign at zero, r + y on a sign mismatch.
+HCC_HD double jmod(double x, double y) {
+    double r = fmod(x, y);
+    if (r == 0.0) return copysign(r, y);
+    if ((r > 0) != (y > 0)) return r + y;
+    return r;
+}
+
+// ExponentialWeightFunction: arg = (-alpha * d2) * (1/R^2), 1 / P4(-arg)
+// (src/Grid/MLSWeightFunctions.jl:68-82)
+HCC_HD double mls_weight(double alpha, double inv_range_sq, double d2) {
+    double arg = -alpha * d2 * inv_range_sq;
+    double y = -arg;
+    double den = 1.0 + y * (1.0 + y * (0.5 + y * (0.16666666666666666 + y * 0.041666666666666664)));
+    return 1.0 / den;
+}
+
+// Venkatakrishnan / Barth-Jespersen phi (src/Interpolation/Limiter.jl:35-45)
+HCC_HD double vk_phi(double r) {
+    double r_pos = jmax(0.0, r);
+    double s = 1.0 / jmax(1.0, r_pos);
+    if (r_pos > 1.0) return (1.0 + 2.0 * s) / (1.0 + s + 2.0 * (s * s));
+    return (r_pos * r_pos + 2.0 * r_pos) / (r_pos * r_pos + r_pos + 2.0);
+}
+HCC_HD double bj_phi(double r) { return jmin(1.0, r); }
+
+// Compressible Euler (build-defined; the reference ships no Euler PDE): state (rho, m_1..m_D, E),
+// u = m * (1/rho), p = (gamma - 1) (E - 0.5 sum m_d u_d), c = sqrt((gamma p) * (1/rho)).
+template <int D>
+struct EulerPrim {
+    double inv, u[D], p;
+};
+template <int D>
+HCC_HD EulerPrim<D> euler_prim(double gamma, const double* U) {
+    EulerPrim<D> q;
+    q.inv = 1.0 / U[0];
+    for (int d = 0; d < D; ++d) q.u[d] = U[1 + d] * q.inv;
+    double ke = U[1] * q.u[0];
+    for (int d = 1; d < D; ++d) ke = ke + U[1 + d] * q.u[d];
+    ke = 0.5 * ke;
+    q.p = (gamma - 1.0) * (U[D + 1] - ke);
+    return q;
+}
+
+// Cholesky of a small SPD moment matrix, column by column with a multiply-add per update and a
+// reciprocal of each pivot, and the conditional Tikhonov retry of _cholesky_moment
+// (src/Interpolation/Interpolators.jl:46-64): the plain factorisation is kept if every pivot U[k,k]^2
+// exceeds 10 eps N[k,k]; otherwise sqrt(eps) tr(N)/B is added to the diagonal and the factorisation
+// repeated; false if that fails too.
+template <int B>
+HCC_HD bool chol_plain(const double (&A)[B][B], double (&U)[B][B]) {
+    double L[B][B];
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) L[r][c] = 0.0;
+    for (int n = 0; n < B; ++n) {
+        for (int m = n; m < B; ++m) L[m][n] = A[n][m];
+        for (int k = 0; k < n; ++k)
+            for (int m = n; m < B; ++m) L[m][n] = fma(-L[m][k], L[n][k], L[m][n]);
+        if (!(L[n][n] > 0.0)) return false;
+        L[n][n] = sqrt(L[n][n]);
+        double linv = 1.0 / L[n][n];
+        for (int m = n + 1; m < B; ++m) L[m][n] *= linv;
+    }
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) U[r][c] = (c >= r) ? L[c][r] : 0.0;
+    return true;
+}
+template <int B>
+HCC_HD bool chol_moment(const double (&A)[B][B], double (&U)[B][B]) {
+    if (chol_plain<B>(A, U)) {
+        bool ok = true;
+        for (int k = 0; k < B; ++k) if (!(U[k][k] * U[k][k] > (10 * kEps) * A[k][k])) ok = false;
+        if (ok) return true;
+    }
+    double tr = A[0][0];
+    for (int k = 1; k < B; ++k) tr = tr + A[k][k];
+    double reg = 1.4901161193847656e-08 * tr / (double)B;     // sqrt(eps(Float64)) tr / B
+    double A2[B][B];
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) A2[r][c] = A[r][c] + ((r == c) ? reg : 0.0);
+    return chol_plain<B>(A2, U);
+}
+// inv(C) = U \ (U' \ I), one column at a time: forward then back substitution.
+template <int B>
+HCC_HD void chol_inverse(const double (&U)[B][B], double (&X)[B][B]) {
+    for (int j = 0; j < B; ++j) {
+        double y[B];
+        for (int i = 0; i < B; ++i) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) s = s - U[k][i] * y[k];
+            y[i] = s / U[i][i];
+        }
+        for (int i = B - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < B; ++k) s = s - U[i][k] * X[k][j];
+            X[i][j] = s / U[i][i];
+        }
+    }
+}
+
+}  // namespace hcc

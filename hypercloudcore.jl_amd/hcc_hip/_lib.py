@@ -1,0 +1,329 @@
+"""ctypes binding of libhcc.so (the C ABI declared in include/hcc.h).
+
+The library is the only compute path: if it is missing, or no GPU is present when a device call is made, the
+call raises. There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libhcc.so")
+
+HCC_PDE_LINADV, HCC_PDE_EULER = 1, 2
+HCC_LIM_NONE, HCC_LIM_BJ, HCC_LIM_VK = 0, 1, 2
+HCC_MODE_SOFT, HCC_MODE_HARD = 0, 1
+HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1 = 0, 1, 2
+HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY = 0, 1, 2, 3
+HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
+HCC_E_CAPACITY = -4
+
+# every symbol include/hcc.h declares
+EXPORTS = [
+    "hcc_ctx_create", "hcc_ctx_destroy", "hcc_last_error", "hcc_ctx_synchronize", "hcc_cloud_generate",
+    "hcc_rng_uniform", "hcc_grid_create", "hcc_grid_destroy", "hcc_grid_num_slots", "hcc_grid_get_neighbors",
+    "hcc_grid_get_bins", "hcc_grid_get_positions", "hcc_stepper_create", "hcc_stepper_destroy", "hcc_state_upload",
+    "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
+    "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing",
+]
+
+
+class DomainDesc(C.Structure):
+    _fields_ = [("D", C.c_int32), ("mins", C.c_double * 3), ("maxs", C.c_double * 3), ("periodic", C.c_int32 * 3),
+                ("dx", C.c_double * 3), ("R", C.c_double), ("alpha", C.c_double), ("jitter", C.c_double * 3),
+                ("seed", C.c_uint64)]
+
+
+class StepperDesc(C.Structure):
+    _fields_ = [("M", C.c_int32), ("pde", C.c_int32), ("gamma", C.c_double), ("adv_speed", (C.c_double * 8) * 3),
+                ("muscl_order", C.c_int32), ("limiter", C.c_int32), ("limiter_mode", C.c_int32),
+                ("mood_strategy", C.c_int32), ("mood_criterion", C.c_int32), ("mood_delta", C.c_double),
+                ("stages", C.c_int32), ("a_t", (C.c_double * 5) * 5), ("b_t", C.c_double * 5), ("c_t", C.c_double * 5),
+                ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
+                ("bc_kind", C.c_int32 * 16)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libhcc.so (raises if it has not been built: there is no other implementation)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libhcc.so not found at {LIB_PATH}: build it with __graft_entry__.build() "
+                           "(the HIP library is the only implementation; there is no fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+    P = C.POINTER
+    sig = {
+        "hcc_ctx_create": (C.c_int, [C.c_int, P(vp)]),
+        "hcc_ctx_destroy": (C.c_int, [vp]),
+        "hcc_last_error": (C.c_char_p, [vp]),
+        "hcc_ctx_synchronize": (C.c_int, [vp]),
+        "hcc_cloud_generate": (C.c_int, [P(DomainDesc), i64, vp, vp, vp, P(i64), vp, P(i64)]),
+        "hcc_rng_uniform": (C.c_int, [C.c_uint64, i64, vp]),
+        "hcc_grid_create": (C.c_int, [vp, P(DomainDesc), i64, vp, vp, vp, P(vp)]),
+        "hcc_grid_destroy": (C.c_int, [vp]),
+        "hcc_grid_num_slots": (C.c_int, [vp, P(i64), P(i64)]),
+        "hcc_grid_get_neighbors": (C.c_int, [vp, vp, vp, vp, vp]),
+        "hcc_grid_get_bins": (C.c_int, [vp, vp, vp, vp, vp]),
+        "hcc_grid_get_positions": (C.c_int, [vp, vp]),
+        "hcc_stepper_create": (C.c_int, [vp, vp, P(StepperDesc), P(vp)]),
+        "hcc_stepper_destroy": (C.c_int, [vp]),
+        "hcc_state_upload": (C.c_int, [vp, vp]),
+        "hcc_state_download": (C.c_int, [vp, vp]),
+        "hcc_time_step": (C.c_int, [vp, P(d)]),
+        "hcc_step": (C.c_int, [vp, d, d]),
+        "hcc_get_orders": (C.c_int, [vp, vp]),
+        "hcc_get_mood_flags": (C.c_int, [vp, vp]),
+        "hcc_get_inadmissible": (C.c_int, [vp, P(i64)]),
+        "hcc_get_mood_iterations": (C.c_int, [vp, P(i64)]),
+        "hcc_get_moments": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "hcc_rhs": (C.c_int, [vp, vp, vp, vp]),
+        "hcc_last_step_timing": (C.c_int, [vp, P(d), P(d), P(d)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class HccError(RuntimeError):
+    pass
+
+
+def check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = lib().hcc_last_error(ctx)
+        raise HccError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def domain_desc(D, mins, maxs, periodic, dx, R, alpha=1.0, jitter=None, seed=0):
+    dd = DomainDesc()
+    dd.D = D
+    for a in range(D):
+        dd.mins[a], dd.maxs[a] = float(mins[a]), float(maxs[a])
+        dd.periodic[a] = int(bool(periodic[a]))
+        dd.dx[a] = float(dx[a])
+        dd.jitter[a] = float(jitter[a]) if jitter is not None else 0.0
+    dd.R, dd.alpha, dd.seed = float(R), float(alpha), int(seed)
+    return dd
+
+
+def cloud_generate(dd):
+    """Host point cloud (lattice + jitter + ghost band + tags, Morton-ordered)."""
+    L = lib()
+    n = C.c_int64(0)
+    ties = C.c_int64(0)
+    canvas = np.zeros(2 * dd.D)
+    rc = L.hcc_cloud_generate(C.byref(dd), 0, None, None, None, C.byref(n), ptr(canvas), C.byref(ties))
+    if rc not in (0, HCC_E_CAPACITY):
+        check(rc, None, "hcc_cloud_generate")
+    N = n.value
+    pos = np.zeros((N, dd.D))
+    isb = np.zeros(N, np.uint8)
+    tags = np.zeros(N, np.int32)
+    check(L.hcc_cloud_generate(C.byref(dd), N, ptr(pos), ptr(isb), ptr(tags), C.byref(n), ptr(canvas), C.byref(ties)),
+          None, "hcc_cloud_generate")
+    return {"pos": pos, "is_boundary": isb, "tags": tags, "canvas": canvas, "morton_ties": ties.value, "N": N}
+
+
+def rng_uniform(seed, n):
+    out = np.zeros(int(n))
+    check(lib().hcc_rng_uniform(int(seed), int(n), ptr(out)), None, "hcc_rng_uniform")
+    return out
+
+
+class Context:
+    """One GPU context (device + stream)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        check(L.hcc_ctx_create(int(device), C.byref(h)), None, "hcc_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hcc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid:
+    """Device particle grid: nodes uploaded and the neighbour search run on the GPU."""
+
+    def __init__(self, ctx, dd, pos, is_boundary, tags):
+        L = lib()
+        self.ctx, self.dd = ctx, dd
+        self.D = dd.D
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.N = pos.shape[0]
+        self._keep = (pos, np.ascontiguousarray(is_boundary, np.uint8), np.ascontiguousarray(tags, np.int32))
+        h = C.c_void_p()
+        check(L.hcc_grid_create(ctx.h, C.byref(dd), self.N, ptr(self._keep[0]), ptr(self._keep[1]), ptr(self._keep[2]),
+                                C.byref(h)), ctx.h, "hcc_grid_create")
+        self.h = h
+        S, mx = C.c_int64(), C.c_int64()
+        check(L.hcc_grid_num_slots(self.h, C.byref(S), C.byref(mx)), ctx.h, "hcc_grid_num_slots")
+        self.S, self.max_nb = S.value, mx.value
+
+    def neighbors(self, weights=True, distances=True):
+        ranges = np.zeros(self.N + 1, np.int64)
+        idx = np.zeros(max(self.S, 1), np.int32)
+        w = np.zeros(max(self.S, 1)) if weights else None
+        dist = np.zeros((max(self.S, 1), self.D)) if distances else None
+        check(lib().hcc_grid_get_neighbors(self.h, ptr(ranges), ptr(idx), ptr(w), ptr(dist)), self.ctx.h,
+              "hcc_grid_get_neighbors")
+        S = self.S
+        return ranges, idx[:S], (w[:S] if w is not None else None), (dist[:S] if dist is not None else None)
+
+    def bins(self):
+        dims = np.zeros(4, np.int64)
+        check(lib().hcc_grid_get_bins(self.h, None, None, None, ptr(dims)), self.ctx.h, "hcc_grid_get_bins")
+        nb = int(dims[3])
+        bin_of = np.zeros(self.N, np.int32)
+        cs = np.zeros(nb + 2, np.int64)
+        cp = np.zeros(self.N, np.int32)
+        check(lib().hcc_grid_get_bins(self.h, ptr(bin_of), ptr(cs), ptr(cp), ptr(dims)), self.ctx.h, "hcc_grid_get_bins")
+        return bin_of, cs, cp, dims
+
+    def positions(self):
+        out = np.zeros((self.N, self.D))
+        check(lib().hcc_grid_get_positions(self.h, ptr(out)), self.ctx.h, "hcc_grid_get_positions")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hcc_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stepper_desc(scheme):
+    """hcc_stepper_desc from a plain scheme dict (see hcc_hip.workloads)."""
+    sd = StepperDesc()
+    sd.M = scheme["M"]
+    sd.pde = scheme["pde"]
+    sd.gamma = scheme.get("gamma", 1.4)
+    for d, row in enumerate(scheme.get("adv_speed", [])):
+        for m, v in enumerate(row):
+            sd.adv_speed[d][m] = v
+    sd.muscl_order = scheme.get("muscl_order", 2)
+    sd.limiter, sd.limiter_mode = scheme["limiter"], scheme.get("limiter_mode", HCC_MODE_HARD)
+    sd.mood_strategy, sd.mood_criterion = scheme["strategy"], scheme["criterion"]
+    sd.mood_delta = scheme.get("delta", 0.0)
+    a_t, b_t, c_t = scheme["a_t"], scheme["b_t"], scheme["c_t"]
+    s = len(b_t)
+    sd.stages = s
+    for i in range(s):
+        sd.b_t[i], sd.c_t[i] = b_t[i], c_t[i]
+        for j in range(s):
+            sd.a_t[i][j] = a_t[i][j]
+    sd.substep_ratio = scheme.get("substep_ratio", 0.0)
+    bcs = scheme.get("bcs", [])
+    sd.n_bc = len(bcs)
+    for k, (tag, kind) in enumerate(bcs):
+        sd.bc_tag[k], sd.bc_kind[k] = tag, kind
+    return sd
+
+
+class Stepper:
+    """Device time stepper (GeneralRKTimeStepper with MUSCL-2 + MOOD) bound to one grid."""
+
+    def __init__(self, ctx, grid, scheme):
+        self.ctx, self.grid, self.scheme = ctx, grid, scheme
+        self.N, self.M, self.D = grid.N, scheme["M"], grid.D
+        self.sd = stepper_desc(scheme)
+        h = C.c_void_p()
+        check(lib().hcc_stepper_create(ctx.h, grid.h, C.byref(self.sd), C.byref(h)), ctx.h, "hcc_stepper_create")
+        self.h = h
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)
+        check(lib().hcc_state_upload(self.h, ptr(U)), self.ctx.h, "hcc_state_upload")
+
+    def download(self):
+        U = np.zeros((self.N, self.M))
+        check(lib().hcc_state_download(self.h, ptr(U)), self.ctx.h, "hcc_state_download")
+        return U
+
+    def time_step(self):
+        v = C.c_double()
+        check(lib().hcc_time_step(self.h, C.byref(v)), self.ctx.h, "hcc_time_step")
+        return v.value
+
+    def step(self, t, dt):
+        check(lib().hcc_step(self.h, float(t), float(dt)), self.ctx.h, "hcc_step")
+
+    def orders(self):
+        o = np.zeros(self.N, np.int32)
+        check(lib().hcc_get_orders(self.h, ptr(o)), self.ctx.h, "hcc_get_orders")
+        return o
+
+    def mood_flags(self):
+        f = np.zeros(self.N, np.uint8)
+        check(lib().hcc_get_mood_flags(self.h, ptr(f)), self.ctx.h, "hcc_get_mood_flags")
+        return f
+
+    def inadmissible(self):
+        v = C.c_int64()
+        check(lib().hcc_get_inadmissible(self.h, C.byref(v)), self.ctx.h, "hcc_get_inadmissible")
+        return v.value
+
+    def mood_iterations(self):
+        v = C.c_int64()
+        check(lib().hcc_get_mood_iterations(self.h, C.byref(v)), self.ctx.h, "hcc_get_mood_iterations")
+        return v.value
+
+    def moments(self):
+        D, N, S = self.D, self.N, self.grid.S
+        ig = np.zeros((N, D, D))
+        og = np.zeros(N, np.uint8)
+        idv = np.zeros((N, D + 1, D + 1))
+        od = np.zeros(N, np.uint8)
+        dw = np.zeros((max(S, 1), D))
+        check(lib().hcc_get_moments(self.h, ptr(ig), ptr(og), ptr(idv), ptr(od), ptr(dw)), self.ctx.h, "hcc_get_moments")
+        return ig, og, idv, od, dw[:S]
+
+    def rhs(self, Y):
+        Y = np.ascontiguousarray(Y, np.float64).reshape(self.N, self.M)
+        K = np.zeros((self.N, self.M))
+        G = np.zeros((self.N, self.D, self.M))
+        check(lib().hcc_rhs(self.h, ptr(Y), ptr(K), ptr(G)), self.ctx.h, "hcc_rhs")
+        return K, G
+
+    def timing(self):
+        """(step ms, summed full gather-pass ms, summed full flux-pass ms) of the last step"""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(lib().hcc_last_step_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.ctx.h, "hcc_last_step_timing")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hcc_stepper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,145 @@
+/*
+ * hcc.h -- C ABI of the MI355X (gfx950) implementation of HyperCloudCore.jl's explicit time-step hot path.
+ *
+ * The reference's user-facing seam for this path is three calls per run (README.md:108-150,
+ * test/time_integration.jl:106-111): `ParticleGrid(...)` (src/Grid/ParticleGrids.jl:221-279),
+ * `get_time_step(pg, eq, interp)` (src/Grid/ParticleGrids.jl:386-415) and the stepper functor
+ * `ts(eq, pg, t, dt)` (src/TimeStepping/TimeSteppers.jl:531-577). Kernel bodies there are Julia closures over
+ * user physics, so the replacement plugs in at the operator level: a grid object, a stepper object and a
+ * PDE descriptor that selects a compiled kernel variant. Each entry point below names the reference
+ * function it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on error; hcc_last_error() gives the text.
+ *     No C++ exception crosses this boundary.
+ *   - Host pointers are borrowed for the duration of the call only. Device memory is owned by the library.
+ *   - States cross the boundary as AoS: M contiguous doubles per node (Julia's Vector{SVector{M,Float64}}).
+ *     Positions likewise D doubles per node. Node and slot indices are 0-based here (a Julia binding adds 1).
+ *   - One HIP stream per context; calls on one context are not re-entrant (callers serialise per context).
+ *   - Only one GPU (the context's device) is used.
+ */
+#ifndef HCC_H
+#define HCC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hcc_ctx hcc_ctx;
+typedef struct hcc_grid hcc_grid;
+typedef struct hcc_stepper hcc_stepper;
+
+enum {
+    HCC_OK = 0,
+    HCC_E_ARG = -1,       /* invalid argument */
+    HCC_E_HIP = -2,       /* HIP runtime error */
+    HCC_E_UNSUPPORTED = -3,/* PDE / dimension / scheme not compiled in (no fallback path exists) */
+    HCC_E_CAPACITY = -4   /* caller buffer too small; required size returned through the size argument */
+};
+
+/* ---- context ------------------------------------------------------------------------------------------ */
+/* Selects `device`, creates the context's stream. */
+int hcc_ctx_create(int device, hcc_ctx** out);
+int hcc_ctx_destroy(hcc_ctx* ctx);
+/* Text of the last error on this context (or of a failed hcc_ctx_create when ctx is NULL). */
+const char* hcc_last_error(const hcc_ctx* ctx);
+int hcc_ctx_synchronize(hcc_ctx* ctx);
+
+/* ---- point cloud (host set-up) ------------------------------------------------------------------------ */
+/* A rectangular domain as get_rectangular_domain (src/Grid/Domains.jl:46-97) with the ghost canvas of
+ * ComputationalDomain (Domains.jl:159-191) for cutoff R = weight-function range. */
+typedef struct {
+    int32_t D;              /* 1, 2 or 3 */
+    double mins[3], maxs[3];
+    int32_t periodic[3];
+    double dx[3];           /* nominal spacing per axis */
+    double R;               /* support radius (ExponentialWeightFunction range) */
+    double alpha;           /* ExponentialWeightFunction alpha */
+    double jitter[3];       /* `randomness`: uniform jitter amplitude per axis */
+    uint64_t seed;          /* seed of the jitter stream (xoshiro256**, splitmix64-seeded) */
+} hcc_domain_desc;
+
+/* Replaces get_points + _narrow_band + tags (Domains.jl:220-347) and the Morton reorder
+ * (src/Grid/Reordering.jl:55-101). Host code. Writes N nodes (pos: N*D doubles) if cap >= N, else returns
+ * HCC_E_CAPACITY. *n_out = N always; canvas (optional, 2*D doubles): canvas mins then maxs;
+ * morton_ties (optional): number of equal Morton keys (their order is by original index). */
+int hcc_cloud_generate(const hcc_domain_desc* desc, int64_t cap, double* pos, uint8_t* is_boundary, int32_t* tags,
+                       int64_t* n_out, double* canvas, int64_t* morton_ties);
+
+/* Seeded uniform doubles in [0,1) from the same generator (initial conditions of the synthetic workloads). */
+int hcc_rng_uniform(uint64_t seed, int64_t n, double* out);
+
+/* ---- grid + neighbour search (device) ----------------------------------------------------------------- */
+/* Replaces ParticleGrid's device part: uploads the nodes and runs the cell-list neighbour search on the
+ * GPU -- periodic wrap, counting-sort bins with descending in-bin order, count, scan, write
+ * (src/Grid/NeighborLogic.jl:22-167, 235-329). Positions are wrapped on periodic axes. */
+int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_t N, const double* pos, const uint8_t* is_boundary,
+                    const int32_t* tags, hcc_grid** out);
+int hcc_grid_destroy(hcc_grid* g);
+/* total neighbour slots S and the largest count max_nb (pg.meta.max_nb) */
+int hcc_grid_num_slots(const hcc_grid* g, int64_t* S, int64_t* max_nb);
+/* Row-major neighbour lists as the reference's host layout: ranges N+1 offsets, indices S entries, weights S,
+ * distances S*D (any output may be NULL). Same lists in the same order as NeighborLogic.jl:309-329. */
+int hcc_grid_get_neighbors(hcc_grid* g, int64_t* ranges, int32_t* indices, double* weights, double* distances);
+/* Cell list: bin_of N, cell_start nbins+2, cell_particles N, dims[4] = (dims_x, dims_y, dims_z, nbins). */
+int hcc_grid_get_bins(hcc_grid* g, int32_t* bin_of, int64_t* cell_start, int32_t* cell_particles, int64_t* dims);
+/* Positions after the periodic wrap (N*D AoS). */
+int hcc_grid_get_positions(hcc_grid* g, double* pos);
+
+/* ---- stepper ------------------------------------------------------------------------------------------ */
+enum { HCC_PDE_LINADV = 1, HCC_PDE_EULER = 2 };
+enum { HCC_LIM_NONE = 0, HCC_LIM_BJ = 1, HCC_LIM_VK = 2 };
+enum { HCC_MODE_SOFT = 0, HCC_MODE_HARD = 1 };
+enum { HCC_STRAT_NONE = 0, HCC_STRAT_EPD0 = 1, HCC_STRAT_EPD1 = 2 };
+enum { HCC_CRIT_NONE = 0, HCC_CRIT_U1 = 1, HCC_CRIT_PAD = 2, HCC_CRIT_ONLY = 3 };
+enum { HCC_BC_NONE = 0, HCC_BC_OUTFLOW = 1, HCC_BC_DIRICHLET = 2 };
+
+typedef struct {
+    int32_t M;                 /* state components */
+    int32_t pde;               /* HCC_PDE_* */
+    double gamma;              /* Euler: ratio of specific heats */
+    double adv_speed[3][8];    /* linear advection: speed of component m along axis d (diagonal system) */
+    int32_t muscl_order;       /* MUSCL max_order (2 is compiled) */
+    int32_t limiter, limiter_mode;
+    int32_t mood_strategy, mood_criterion;
+    double mood_delta;         /* MOODu1 d */
+    int32_t stages;            /* explicit tableau, s <= 5 */
+    double a_t[5][5], b_t[5], c_t[5];
+    double substep_ratio;      /* Δt/C per Δt of the MOOD Euler substeps (TimeSteppers.jl:218); 0: check stage candidates */
+    int32_t n_bc;
+    int32_t bc_tag[16], bc_kind[16];
+} hcc_stepper_desc;
+
+/* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL
+ * geometry caches of prepare_geometry! (src/Interpolation/MUSCL.jl:172-223): inverse moment matrices of the
+ * reconstruction and divergence fits, per-slot derivative weights. */
+int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_desc* desc, hcc_stepper** out);
+int hcc_stepper_destroy(hcc_stepper* st);
+/* pg.rhos up/download (N*M AoS doubles) */
+int hcc_state_upload(hcc_stepper* st, const double* U);
+int hcc_state_download(hcc_stepper* st, double* U);
+/* Replaces get_time_step(pg, eq, interp) with cached weights (ParticleGrids.jl:386-450): the CFL-1 step. */
+int hcc_time_step(hcc_stepper* st, double* dt);
+/* Replaces ts(eq, pg, t, dt) (TimeSteppers.jl:531-577): one time step in place on the device state. */
+int hcc_step(hcc_stepper* st, double t, double dt);
+/* Diagnostics of the last step: particle_orders (N), mood_flags (TimeSteppers.jl:258, N bytes),
+ * mood_inadmissible (:272), and the number of MOOD recompute iterations. */
+int hcc_get_orders(hcc_stepper* st, int32_t* orders);
+int hcc_get_mood_flags(hcc_stepper* st, uint8_t* flags);
+int hcc_get_inadmissible(hcc_stepper* st, int64_t* count);
+int hcc_get_mood_iterations(hcc_stepper* st, int64_t* count);
+/* Geometry caches (row-major B*B per node; ok bytes; dweights S*D in row-major slot order). */
+int hcc_get_moments(hcc_stepper* st, double* inv_grad, uint8_t* ok_grad, double* inv_div, uint8_t* ok_div,
+                    double* dweights);
+/* One full right-hand side (gather + flux pass, orders reset to the maximum) on state Y (N*M AoS):
+ * K (N*M) and the limited gradients (N*D*M, node-major then basis term then component). */
+int hcc_rhs(hcc_stepper* st, const double* Y, double* K, double* gradients);
+/* Timing of the last hcc_step on the context stream (HIP events), milliseconds: the whole step, and the summed
+ * durations of its full-domain gather (gradient + limiter) and flux passes, one of each per stage. */
+int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, double* ms_flux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCC_H */

@@ -1,0 +1,1201 @@
+// CPU oracle of HyperCloudCore.jl's explicit time-step hot path (Float64).
+//
+// TEST INFRASTRUCTURE ONLY. This file restates, loop for loop, the reference's CPU path for one
+// explicit SSP-RK step with MUSCL (order 2) + slope limiter + Rusanov flux + cached MLS/GFD
+// divergence + MOOD, and the set-up it depends on (point cloud, Morton order, cell-list neighbour
+// search, MLS moment caches, CFL estimate, outflow boundary condition). It exists to check the HIP
+// implementation: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+// and never as the thing measured or shipped.
+//
+// Operation order follows the Julia sources cited at each function (paths relative to the
+// reference repository root) so that the GPU kernels, which follow the same order, agree bit for
+// bit. Build with -ffp-contract=off: Julia does not contract a*b+c into an FMA unless asked to.
+//
+// Parity status per part (what pins it is listed in the test files):
+//   * neighbour lists, weights, distances, bins, Morton keys, tags: pinned (brute force, closed forms,
+//     reference unit-test expectations).
+//   * weight function, VK limiter phi, Rusanov, RK3 tableau, DMP, outflow: pinned (closed forms and
+//     the reference's unit-test values).
+//   * MLS Cholesky / inverse: pinned to tolerance only (exact linear reproduction, zero-sum
+//     weights); the bit-level rounding of StaticArrays' small-matrix cholesky/inv cannot be imported
+//     here -- "parity unpinned" at the bit level for that part.
+//   * Point-cloud jitter: Julia's MersenneTwister stream is not reproducible here; the cloud uses a
+//     documented xoshiro256** stream instead (build-defined, "parity unpinned" against Julia's RNG).
+//   * Euler PDE: not part of the reference (build-defined closed form).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <array>
+#include <algorithm>
+#include <numeric>
+#include <unordered_map>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+static constexpr double EPS = 2.220446049250313e-16;   // eps(Float64)
+
+// ---------------------------------------------------------------------------------------------
+// Julia scalar semantics
+// ---------------------------------------------------------------------------------------------
+// Base.max / Base.min for IEEE floats (julia/base/math.jl): NaN-propagating, -0 < +0.
+static inline double jmax(double x, double y) {
+    double diff = x - y;
+    double am = std::signbit(diff) ? y : x;
+    return (std::isnan(x) || std::isnan(y)) ? diff : am;
+}
+static inline double jmin(double x, double y) {
+    double diff = x - y;
+    double am = std::signbit(diff) ? x : y;
+    return (std::isnan(x) || std::isnan(y)) ? diff : am;
+}
+// Base.mod(x, y) for floats: r = rem(x, y); r == 0 -> copysign(r, y); sign mismatch -> r + y.
+static inline double jmod(double x, double y) {
+    double r = std::fmod(x, y);
+    if (r == 0.0) return std::copysign(r, y);
+    if ((r > 0) != (y > 0)) return r + y;
+    return r;
+}
+// Base.eps(x): the spacing of floats at |x| (2^-1074 at zero and below the normal range).
+static inline double eps_of(double x) {
+    x = std::fabs(x);
+    if (!(x >= 2.2250738585072014e-308)) return 4.9406564584124654e-324;
+    return std::ldexp(1.0, std::ilogb(x) - 52);
+}
+// round(Int, x): RoundNearest, ties to even.
+static inline int64_t jround_int(double x) { return (int64_t)std::nearbyint(x); }
+
+// ---------------------------------------------------------------------------------------------
+// Seeded generator of the point-cloud jitter and the initial conditions (build-defined):
+// splitmix64 seeding of xoshiro256**, doubles in [0, 1) from the top 53 bits.
+// ---------------------------------------------------------------------------------------------
+struct Rng {
+    uint64_t s[4];
+    static uint64_t splitmix(uint64_t& x) {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    explicit Rng(uint64_t seed) { uint64_t x = seed; for (int i = 0; i < 4; ++i) s[i] = splitmix(x); }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Domain + point generation: src/Grid/Domains.jl
+// ---------------------------------------------------------------------------------------------
+struct Domain {
+    int D;
+    double mins[3], maxs[3];
+    int periodic[3];
+    double dx[3];
+    double R;            // cutoff range of the weight function
+    double canvas_min[3], canvas_max[3];
+    double L[3], L_wrap[3], invL_wrap[3];
+    double irf;          // interp_range_factor = R / max_dx
+};
+
+// ComputationalDomain (Domains.jl:159-191) with interp_range_factor = R / max(dx) (ParticleGrids.jl:232-237)
+static void computational_domain(Domain& g) {
+    double max_dx = g.dx[0];
+    for (int d = 1; d < g.D; ++d) max_dx = std::max(max_dx, g.dx[d]);
+    g.irf = g.R / max_dx;
+    for (int d = 0; d < g.D; ++d) {
+        int64_t ng;
+        if (g.dx[d] == max_dx) ng = (int64_t)std::ceil(g.irf);
+        else ng = (int64_t)std::ceil(g.irf * (max_dx / g.dx[d]) * (1 - 4 * EPS));
+        g.canvas_min[d] = g.periodic[d] ? g.mins[d] : g.mins[d] - (double)ng * g.dx[d];
+        g.canvas_max[d] = g.periodic[d] ? g.maxs[d] : g.maxs[d] + (double)ng * g.dx[d];
+        g.L[d] = g.maxs[d] - g.mins[d];
+        g.L_wrap[d] = g.periodic[d] ? g.L[d] : 0.0;
+        g.invL_wrap[d] = g.periodic[d] ? 1.0 / g.L[d] : 0.0;
+    }
+}
+
+// _wall_index (Domains.jl:202-207)
+static int64_t wall_index(double w, double cmin, double dx) {
+    double r = (w - cmin) / dx;
+    int64_t k = jround_int(r);
+    double tol = (2 * (eps_of(w) + eps_of(cmin)) + EPS * std::fabs(w - cmin)) / dx + 4 * EPS * std::fabs(r);
+    return std::fabs(r - (double)k) <= std::min(0.25, tol) ? k : -1;
+}
+
+// _wrap_periodic (Domains.jl:314)
+static double wrap_periodic(double x, double lo, double L) {
+    double y = lo + jmod(x - lo, L);
+    return y >= lo + L ? lo : y;
+}
+
+// get_tag of a rectangular domain (Domains.jl:59-94)
+static int rect_tag(const Domain& g, const double* p) {
+    bool interior = true;
+    for (int d = 0; d < g.D; ++d) if (p[d] < g.mins[d] || p[d] > g.maxs[d]) { interior = false; break; }
+    if (interior) return 0;
+    int best_tag = 0; double best_out = 0.0;
+    for (int d = 0; d < g.D; ++d) {
+        if (g.periodic[d]) continue;
+        double o_lo = g.mins[d] - p[d], o_hi = p[d] - g.maxs[d];
+        if (o_lo > best_out) { best_tag = 2 * (d + 1) - 1; best_out = o_lo; }
+        if (o_hi > best_out) { best_tag = 2 * (d + 1); best_out = o_hi; }
+    }
+    if (best_tag > 0) return best_tag;
+    best_tag = 1; double best_d = INFINITY;
+    for (int d = 0; d < g.D; ++d) {
+        if (g.periodic[d]) continue;
+        double d_lo = std::fabs(p[d] - g.mins[d]), d_hi = std::fabs(p[d] - g.maxs[d]);
+        if (d_lo < best_d) { best_tag = 2 * (d + 1) - 1; best_d = d_lo; }
+        if (d_hi < best_d) { best_tag = 2 * (d + 1); best_d = d_hi; }
+    }
+    return best_tag;
+}
+
+// get_points (Domains.jl:220-287) + _narrow_band (:319-347) + tags; then the Morton reorder
+// (Reordering.jl:55-101). Positions come back AoS (N x D).
+struct Cloud {
+    std::vector<double> pos;
+    std::vector<uint8_t> is_boundary;
+    std::vector<int32_t> tags;
+    int64_t N = 0, n_interior = 0, morton_ties = 0;
+};
+
+static void get_points(const Domain& g, const double* jitter, uint64_t seed, Cloud& out) {
+    const int D = g.D;
+    int64_t Ns[3] = {1, 1, 1};
+    for (int d = 0; d < D; ++d) {
+        int64_t n = std::max<int64_t>(1, jround_int((g.canvas_max[d] - g.canvas_min[d]) / g.dx[d]));
+        Ns[d] = g.periodic[d] ? n : n + 1;
+    }
+    int64_t wlo[3], whi[3];
+    for (int d = 0; d < D; ++d) {
+        wlo[d] = g.periodic[d] ? -1 : wall_index(g.mins[d], g.canvas_min[d], g.dx[d]);
+        whi[d] = g.periodic[d] ? -1 : wall_index(g.maxs[d], g.canvas_min[d], g.dx[d]);
+    }
+    Rng rng(seed);
+    std::vector<double> inner, ghosts;
+    int64_t total = Ns[0] * Ns[1] * Ns[2];
+    for (int64_t lin = 0; lin < total; ++lin) {
+        int64_t I[3] = {lin % Ns[0], (lin / Ns[0]) % Ns[1], lin / (Ns[0] * Ns[1])};   // CartesianIndices: first axis fastest
+        double p[3];
+        for (int d = 0; d < D; ++d) {
+            int64_t j = I[d];
+            double x = (j == wlo[d]) ? g.mins[d] : (j == whi[d]) ? g.maxs[d] : g.canvas_min[d] + (double)j * g.dx[d];
+            x += jitter[d] * (rng.uniform() * 2 - 1);
+            p[d] = g.periodic[d] ? wrap_periodic(x, g.mins[d], g.L_wrap[d]) : x;
+        }
+        bool interior = true;
+        for (int d = 0; d < D; ++d) if (p[d] < g.mins[d] || p[d] > g.maxs[d]) interior = false;
+        std::vector<double>& dst = interior ? inner : ghosts;
+        for (int d = 0; d < D; ++d) dst.push_back(p[d]);
+    }
+    int64_t ni = (int64_t)inner.size() / D, ng = (int64_t)ghosts.size() / D;
+    double max_dx = g.dx[0];
+    for (int d = 1; d < D; ++d) max_dx = std::max(max_dx, g.dx[d]);
+    double cut = g.irf * max_dx + max_dx;
+    double cutoff_sq = cut * cut;
+    // _narrow_band: interior points hashed into cells of width sqrt(cutoff_sq)
+    std::vector<char> keep(ng, 0);
+    if (ni > 0 && ng > 0) {
+        double h = std::sqrt(cutoff_sq);
+        double origin[3];
+        for (int d = 0; d < D; ++d) { origin[d] = inner[d]; }
+        for (int64_t i = 1; i < ni; ++i) for (int d = 0; d < D; ++d) origin[d] = jmin(origin[d], inner[i * D + d]);
+        auto cell_of = [&](const double* p, int64_t* c) { for (int d = 0; d < D; ++d) c[d] = (int64_t)std::floor((p[d] - origin[d]) / h); };
+        auto key = [&](const int64_t* c) { uint64_t k = 1469598103934665603ull; for (int d = 0; d < D; ++d) { k ^= (uint64_t)(c[d] + (1ll << 40)); k *= 1099511628211ull; } return k; };
+        std::unordered_map<uint64_t, std::vector<int64_t>> cells;
+        cells.reserve((size_t)ni);
+        for (int64_t i = 0; i < ni; ++i) { int64_t c[3]; cell_of(&inner[i * D], c); cells[key(c)].push_back(i); }
+        for (int64_t gi = 0; gi < ng; ++gi) {
+            const double* gp = &ghosts[gi * D];
+            int64_t c[3]; cell_of(gp, c);
+            bool found = false;
+            int nw = D == 1 ? 3 : D == 2 ? 9 : 27;
+            for (int w = 0; w < nw && !found; ++w) {
+                int64_t cc[3];
+                int rem = w;
+                for (int d = 0; d < D; ++d) { cc[d] = c[d] + (rem % 3) - 1; rem /= 3; }
+                auto it = cells.find(key(cc));
+                if (it == cells.end()) continue;
+                for (int64_t i : it->second) {
+                    double s = 0.0;
+                    for (int d = 0; d < D; ++d) { double t = gp[d] - inner[i * D + d]; s = (d == 0) ? t * t : s + t * t; }
+                    if (s <= cutoff_sq) { found = true; break; }
+                }
+            }
+            keep[gi] = found;
+        }
+    }
+    out.pos = inner;
+    for (int64_t gi = 0; gi < ng; ++gi) if (keep[gi]) for (int d = 0; d < D; ++d) out.pos.push_back(ghosts[gi * D + d]);
+    out.N = (int64_t)out.pos.size() / D;
+    out.n_interior = ni;
+    out.tags.resize(out.N); out.is_boundary.resize(out.N);
+    for (int64_t i = 0; i < out.N; ++i) {
+        int t = rect_tag(g, &out.pos[i * D]);
+        out.tags[i] = t; out.is_boundary[i] = (t != 0);
+    }
+}
+
+// Morton keys (Reordering.jl:6-35)
+static uint32_t expand2(uint32_t w) {
+    w &= 0x0000ffff; w = (w | (w << 8)) & 0x00FF00FF; w = (w | (w << 4)) & 0x0F0F0F0F;
+    w = (w | (w << 2)) & 0x33333333; w = (w | (w << 1)) & 0x55555555; return w;
+}
+static uint32_t expand3(uint32_t w) {
+    w &= 0x000003ff; w = (w | (w << 16)) & 0xFF0000FF; w = (w | (w << 8)) & 0x0300F00F;
+    w = (w | (w << 4)) & 0x030C30C3; w = (w | (w << 2)) & 0x09249249; return w;
+}
+// key(x, d, top) = UInt32(clamp(floor((x - mins[d]) / extents[d] * top), 0, top)) (Reordering.jl:72)
+static uint32_t axis_key(double x, double mn, double ext, double top) {
+    double v = std::floor((x - mn) / ext * top);
+    v = std::min(std::max(v, 0.0), top);
+    return (uint32_t)v;
+}
+
+// Reordering.jl:55-101. Julia sorts with an unstable QuickSort; equal keys would keep an order this
+// restatement cannot reproduce, so ties are ordered by the original index and counted.
+static void morton_reorder(const Domain& g, Cloud& c) {
+    const int D = g.D; const int64_t N = c.N;
+    if (N <= 1) return;
+    std::vector<uint64_t> keyv(N);
+    std::vector<double> keyx(N);
+    for (int64_t i = 0; i < N; ++i) {
+        const double* p = &c.pos[i * D];
+        double ext[3];
+        for (int d = 0; d < D; ++d) ext[d] = g.canvas_max[d] - g.canvas_min[d];
+        if (D == 1) keyx[i] = p[0];
+        else if (D == 2) keyv[i] = expand2(axis_key(p[0], g.canvas_min[0], ext[0], 65535.0)) |
+                                   (expand2(axis_key(p[1], g.canvas_min[1], ext[1], 65535.0)) << 1);
+        else keyv[i] = expand3(axis_key(p[0], g.canvas_min[0], ext[0], 1023.0)) |
+                       (expand3(axis_key(p[1], g.canvas_min[1], ext[1], 1023.0)) << 1) |
+                       (expand3(axis_key(p[2], g.canvas_min[2], ext[2], 1023.0)) << 2);
+    }
+    std::vector<int64_t> perm(N);
+    std::iota(perm.begin(), perm.end(), 0);
+    if (D == 1) std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return keyx[a] < keyx[b]; });
+    else std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return keyv[a] < keyv[b]; });
+    int64_t ties = 0;
+    for (int64_t k = 1; k < N; ++k) {
+        bool eq = (D == 1) ? keyx[perm[k]] == keyx[perm[k - 1]] : keyv[perm[k]] == keyv[perm[k - 1]];
+        ties += eq;
+    }
+    c.morton_ties = ties;
+    bool sorted = true;
+    for (int64_t k = 0; k < N; ++k) if (perm[k] != k) { sorted = false; break; }
+    if (sorted) return;
+    std::vector<double> pos(c.pos.size()); std::vector<uint8_t> b(N); std::vector<int32_t> t(N);
+    for (int64_t k = 0; k < N; ++k) {
+        for (int d = 0; d < D; ++d) pos[k * D + d] = c.pos[perm[k] * D + d];
+        b[k] = c.is_boundary[perm[k]]; t[k] = c.tags[perm[k]];
+    }
+    c.pos.swap(pos); c.is_boundary.swap(b); c.tags.swap(t);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weight function: src/Grid/MLSWeightFunctions.jl:68-82
+// ---------------------------------------------------------------------------------------------
+static inline double fast_exp_accurate(double x) {
+    double y = -x;
+    double den = 1.0 + y * (1.0 + y * (0.5 + y * (0.16666666666666666 + y * 0.041666666666666664)));
+    return 1.0 / den;
+}
+static inline double weight_fn(double alpha, double inv_range_sq, double d2) {
+    double arg = -alpha * d2 * inv_range_sq;
+    return fast_exp_accurate(arg);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Small symmetric positive definite matrices: Cholesky with conditional Tikhonov retry
+// (Interpolators.jl:46-64) and the inverse U \ (U' \ I).
+// The factorisation is the column-by-column form with a multiply-add per update and a
+// reciprocal of each pivot (the form of StaticArrays' unrolled small-matrix cholesky).
+// ---------------------------------------------------------------------------------------------
+template <int B>
+static bool chol_plain(const double (&A)[B][B], double (&U)[B][B], double (&piv)[B]) {
+    double L[B][B] = {};
+    for (int n = 0; n < B; ++n) {
+        for (int m = n; m < B; ++m) L[m][n] = A[n][m];
+        for (int k = 0; k < n; ++k)
+            for (int m = n; m < B; ++m) L[m][n] = std::fma(-L[m][k], L[n][k], L[m][n]);
+        piv[n] = L[n][n];
+        if (!(L[n][n] > 0.0)) return false;
+        L[n][n] = std::sqrt(L[n][n]);
+        double linv = 1.0 / L[n][n];
+        for (int m = n + 1; m < B; ++m) L[m][n] *= linv;
+    }
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) U[r][c] = (c >= r) ? L[c][r] : 0.0;
+    return true;
+}
+template <int B>
+static bool chol_moment(const double (&A)[B][B], double (&U)[B][B]) {
+    double piv[B];
+    bool ok = chol_plain<B>(A, U, piv);
+    if (ok) {
+        const double tol = 10 * EPS;
+        for (int k = 0; k < B; ++k) if (!(U[k][k] * U[k][k] > tol * A[k][k])) { ok = false; break; }
+        if (ok) return true;
+    }
+    double tr = A[0][0];
+    for (int k = 1; k < B; ++k) tr = tr + A[k][k];
+    double reg = std::sqrt(EPS) * tr / (double)B;
+    double A2[B][B];
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) A2[r][c] = A[r][c] + ((r == c) ? reg : 0.0);
+    return chol_plain<B>(A2, U, piv);
+}
+// inv(C) = U \ (U' \ I): forward substitution with L = U', then back substitution, column by column.
+template <int B>
+static void chol_inverse(const double (&U)[B][B], double (&X)[B][B]) {
+    double Y[B][B];
+    for (int j = 0; j < B; ++j) {
+        for (int i = 0; i < B; ++i) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) s = s - U[k][i] * Y[k][j];
+            Y[i][j] = s / U[i][i];
+        }
+        for (int i = B - 1; i >= 0; --i) {
+            double s = Y[i][j];
+            for (int k = i + 1; k < B; ++k) s = s - U[i][k] * X[k][j];
+            X[i][j] = s / U[i][i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// PDEs: linear advection (diagonal system, test/PDEs/linear_advection.jl, benchmark/step.jl:41-54)
+// and the compressible Euler equations (build-defined; no Euler PDE exists in the reference).
+// ---------------------------------------------------------------------------------------------
+enum { PDE_LINADV = 1, PDE_EULER = 2 };
+struct PDE {
+    int kind = PDE_LINADV;
+    double gamma = 1.4;
+    double c[3][8] = {};     // advection speeds per axis and component
+    double amax[3] = {};     // max_m |c[d][m]|
+};
+
+template <int D, int M>
+struct Phys {
+    // flux along every axis
+    static inline void flux(const PDE& e, const double* U, double (&F)[D][M]) {
+        if (e.kind == PDE_LINADV || M != D + 2) {
+            for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m) F[d][m] = e.c[d][m] * U[m];
+        } else if constexpr (M == D + 2) {
+            double inv = 1.0 / U[0];
+            double u[D];
+            for (int d = 0; d < D; ++d) u[d] = U[1 + d] * inv;
+            double ke = U[1] * u[0];
+            for (int d = 1; d < D; ++d) ke = ke + U[1 + d] * u[d];
+            ke = 0.5 * ke;
+            double p = (e.gamma - 1.0) * (U[D + 1] - ke);
+            for (int d = 0; d < D; ++d) {
+                F[d][0] = U[1 + d];
+                for (int k = 0; k < D; ++k) F[d][1 + k] = (k == d) ? U[1 + d] * u[k] + p : U[1 + d] * u[k];
+                F[d][D + 1] = (U[D + 1] + p) * u[d];
+            }
+        }
+    }
+    // max_eigenvalue(eq, U, d) for every axis
+    static inline void maxeig(const PDE& e, const double* U, double (&lam)[D]) {
+        if (e.kind == PDE_LINADV || M != D + 2) {
+            for (int d = 0; d < D; ++d) lam[d] = e.amax[d];
+        } else if constexpr (M == D + 2) {
+            double inv = 1.0 / U[0];
+            double u[D];
+            for (int d = 0; d < D; ++d) u[d] = U[1 + d] * inv;
+            double ke = U[1] * u[0];
+            for (int d = 1; d < D; ++d) ke = ke + U[1 + d] * u[d];
+            ke = 0.5 * ke;
+            double p = (e.gamma - 1.0) * (U[D + 1] - ke);
+            double cs = std::sqrt((e.gamma * p) * inv);
+            for (int d = 0; d < D; ++d) lam[d] = std::fabs(u[d]) + cs;
+        }
+    }
+    static inline bool admissible(const PDE& e, const double* U) {
+        for (int m = 0; m < M; ++m) if (!std::isfinite(U[m])) return false;
+        if constexpr (M == D + 2) if (e.kind == PDE_EULER) {
+            double inv = 1.0 / U[0];
+            double ke = 0.0;
+            {
+                double u0 = U[1] * inv; ke = U[1] * u0;
+                for (int d = 1; d < D; ++d) { double ud = U[1 + d] * inv; ke = ke + U[1 + d] * ud; }
+                ke = 0.5 * ke;
+            }
+            double p = (e.gamma - 1.0) * (U[D + 1] - ke);
+            return U[0] > 0.0 && p > 0.0;
+        }
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Scheme / stepper configuration
+// ---------------------------------------------------------------------------------------------
+enum { LIM_NONE = 0, LIM_BJ = 1, LIM_VK = 2 };
+enum { MODE_SOFT = 0, MODE_HARD = 1 };
+enum { STRAT_NONE = 0, STRAT_EPD0 = 1, STRAT_EPD1 = 2 };
+enum { CRIT_NONE = 0, CRIT_U1 = 1, CRIT_PAD = 2, CRIT_ONLY = 3 };
+enum { BC_NONE = 0, BC_OUTFLOW = 1, BC_DIRICHLET = 2 };
+
+struct Scheme {
+    int limiter = LIM_VK, mode = MODE_HARD;
+    int strategy = STRAT_EPD1, criterion = CRIT_U1;
+    double delta = 0.0;
+    int s = 3;
+    double a_t[5][5] = {}, b_t[5] = {}, c_t[5] = {};
+    double substep_ratio = 0.0;
+    int n_bc = 0;
+    int bc_tag[16] = {}, bc_kind[16] = {};
+};
+
+// ---------------------------------------------------------------------------------------------
+// The simulation object: a grid (src/Grid/ParticleGrids.jl), a MUSCL order-2 scheme
+// (src/Interpolation/MUSCL.jl) and a GeneralRKTimeStepper (src/TimeStepping/TimeSteppers.jl).
+// Neighbour slots are stored row-major (the host layout of the reference).
+// ---------------------------------------------------------------------------------------------
+struct SimBase {
+    virtual ~SimBase() {}
+    int dim = 0, ncomp = 0;
+    int64_t N = 0;
+    virtual void set_scheme(const PDE&, const Scheme&) = 0;
+    virtual void set_state(const double* U) = 0;
+    virtual void get_state(double* U) const = 0;
+    virtual double time_step() = 0;
+    virtual void step(double t, double dt) = 0;
+    virtual void rhs(const double* Y, double* K, double* grads) = 0;
+    virtual int64_t num_slots() const = 0;
+    virtual void get_neighbors(int64_t* ranges, int32_t* idx, double* w, double* dist) const = 0;
+    virtual void get_moments(double* invg, uint8_t* okg, double* invd, uint8_t* okd, double* dweights) const = 0;
+    virtual void get_orders(int32_t* o) const = 0;
+    virtual void get_flags(uint8_t* f) const = 0;
+    virtual int64_t inadmissible() const = 0;
+    virtual void get_bins(int32_t* bin_of, int64_t* cell_start, int32_t* cell_particles, int64_t* dims) const = 0;
+    virtual void set_threads(int n) = 0;
+    virtual int64_t mood_iterations() const = 0;
+};
+
+template <int D, int M>
+struct Sim : SimBase {
+    static constexpr int BG = D;       // reconstruction basis (degree 1): x, y, z
+    static constexpr int BD = D + 1;   // divergence basis with intercept
+    using State = std::array<double, M>;
+    using P = Phys<D, M>;
+
+    Domain g;
+    double alpha = 1.0, inv_range_sq = 0.0, R_sq = 0.0, invL = 1.0;
+    std::vector<std::array<double, D>> pos;
+    std::vector<uint8_t> isb;
+    std::vector<int32_t> tags;
+    // bins (NeighborLogic.jl:22-40), 0-based
+    int64_t dims[3] = {1, 1, 1};
+    double bin_size[3] = {};
+    int64_t nbins = 0;
+    std::vector<int32_t> bin_of;
+    std::vector<int64_t> cell_start;     // nbins + 2
+    std::vector<int32_t> cell_particles;
+    std::vector<std::array<int64_t, 27>> bin_neighbors;
+    int n_off = 0;
+    // neighbour lists (row-major)
+    std::vector<int64_t> rstart;           // N + 1
+    std::vector<int32_t> nbr;
+    std::vector<double> wts;
+    std::vector<std::array<double, D>> dist;
+    // caches
+    std::vector<std::array<double, BG * BG>> invg;
+    std::vector<uint8_t> okg;
+    std::vector<std::array<double, BD * BD>> invd;
+    std::vector<uint8_t> okd;
+    std::vector<std::array<double, D>> dweights;
+    // scheme + stepper state
+    PDE pde;
+    Scheme sc;
+    std::vector<State> rhos, rho_n, rho_cand, rho_sub;
+    std::vector<State> Y[5], K[5];
+    std::vector<std::array<State, BG>> grads;
+    std::vector<int32_t> orders;
+    std::vector<uint8_t> triggered, changed, dirty, scratch, triggered_step;
+    std::vector<int64_t> status;
+    std::vector<std::array<State, D>> dflux;   // per-slot flux differences (buffered divergence)
+    int64_t n_inadmissible = 0;
+    int64_t n_mood_iter = 0;
+    int threads = 1;
+
+    Sim(const Domain& dom, int64_t n, const double* p, const uint8_t* b, const int32_t* t, double alpha_) : g(dom), alpha(alpha_) {
+        this->dim = D; this->ncomp = M; this->N = n;
+        pos.resize(N); isb.assign(b, b + N); tags.assign(t, t + N);
+        for (int64_t i = 0; i < N; ++i) for (int d = 0; d < D; ++d) pos[i][d] = p[i * D + d];
+        inv_range_sq = 1.0 / (g.R * g.R);
+        R_sq = g.R * g.R;
+        double mindx = g.dx[0];
+        for (int d = 1; d < D; ++d) mindx = std::min(mindx, g.dx[d]);
+        invL = 1.0 / mindx;
+        search();
+        prepare_geometry();
+    }
+
+    void set_threads(int n) override { threads = n < 1 ? 1 : n; }
+
+    // get_distance (ParticleGrids.jl:209-219)
+    inline void get_distance(int64_t i, int64_t j, std::array<double, D>& out) const {
+        for (int d = 0; d < D; ++d) {
+            double dx = pos[j][d] - pos[i][d];
+            out[d] = dx - g.L_wrap[d] * std::nearbyint(dx * g.invL_wrap[d]);
+        }
+    }
+    static inline double sumsq(const std::array<double, D>& v) {
+        double s = v[0] * v[0];
+        for (int d = 1; d < D; ++d) s = s + v[d] * v[d];
+        return s;
+    }
+
+    // GlobalBins + _fill_bin_neighbors! + build_global_bins! (NeighborLogic.jl:22-167)
+    void build_bins() {
+        nbins = 1;
+        for (int d = 0; d < D; ++d) {
+            double size = g.canvas_max[d] - g.canvas_min[d];
+            dims[d] = std::max<int64_t>(1, (int64_t)std::floor(size / g.R));
+            bin_size[d] = size / (double)dims[d];
+            nbins *= dims[d];
+        }
+        n_off = D == 1 ? 3 : D == 2 ? 9 : 27;
+        bin_neighbors.assign(nbins, {});
+        const int64_t dummy = nbins;    // 0-based dummy bin
+        for (int64_t b = 0; b < nbins; ++b) {
+            int64_t c[3] = {b % dims[0], (b / dims[0]) % (D > 1 ? dims[1] : 1), D > 2 ? b / (dims[0] * dims[1]) : 0};
+            std::array<int64_t, 27> raw{};
+            for (int w = 0; w < n_off; ++w) {
+                int rem = w; bool valid = true; int64_t f[3] = {0, 0, 0};
+                for (int d = 0; d < D; ++d) {
+                    int64_t nc = c[d] + (rem % 3) - 1; rem /= 3;
+                    if (nc < 0 || nc >= dims[d]) {
+                        if (g.periodic[d]) nc = ((nc % dims[d]) + dims[d]) % dims[d];
+                        else valid = false;
+                    }
+                    f[d] = nc;
+                }
+                raw[w] = valid ? f[0] + f[1] * dims[0] + f[2] * dims[0] * (D > 1 ? dims[1] : 1) : dummy;
+            }
+            for (int w = 0; w < n_off; ++w) {
+                int64_t v = raw[w];
+                bool rep = false;
+                for (int k = 0; k < w; ++k) if (raw[k] == v) rep = true;
+                bin_neighbors[b][w] = rep ? dummy : v;
+            }
+        }
+        // counting sort
+        bin_of.assign(N, 0);
+        std::vector<int64_t> cnt(nbins + 1, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            int64_t lin = 0, stride = 1;
+            for (int d = 0; d < D; ++d) {
+                int64_t ix = (int64_t)std::floor((pos[i][d] - g.canvas_min[d]) / bin_size[d]) + 1;
+                ix = std::min<int64_t>(std::max<int64_t>(ix, 1), dims[d]);
+                lin += (ix - 1) * stride; stride *= dims[d];
+            }
+            bin_of[i] = (int32_t)lin; cnt[lin]++;
+        }
+        cell_start.assign(nbins + 2, 0);
+        for (int64_t b = 0; b < nbins; ++b) cell_start[b + 1] = cell_start[b] + cnt[b];
+        cell_start[nbins + 1] = N;      // dummy bin [N, N) is empty
+        cell_particles.assign(N, 0);
+        std::vector<int64_t> cur(nbins, 0);
+        // scatter in ascending index, then every bin in descending index order (the result of _bin_sort!)
+        for (int64_t i = 0; i < N; ++i) { int64_t b = bin_of[i]; cell_particles[cell_start[b] + cur[b]++] = (int32_t)i; }
+        for (int64_t b = 0; b < nbins; ++b) std::reverse(cell_particles.begin() + cell_start[b], cell_particles.begin() + cell_start[b + 1]);
+    }
+
+    // neighbour functor (NeighborLogic.jl:235-274, kernel bodies :288-329), row-major slots
+    void search() {
+        // _wrap_position! on periodic axes (NeighborLogic.jl:279-286)
+        for (int64_t i = 0; i < N; ++i)
+            for (int d = 0; d < D; ++d)
+                if (g.periodic[d]) pos[i][d] = g.canvas_min[d] + jmod(pos[i][d] - g.canvas_min[d], g.L[d]);
+        build_bins();
+        std::vector<int64_t> counts(N, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            int64_t c = 0;
+            for (int w = 0; w < n_off; ++w) {
+                int64_t b = bin_neighbors[bin_of[i]][w];
+                for (int64_t s = cell_start[b]; s < cell_start[b + 1]; ++s) {
+                    int64_t j = cell_particles[s];
+                    if (i == j) continue;
+                    std::array<double, D> dd; get_distance(i, j, dd);
+                    if (sumsq(dd) <= R_sq) ++c;
+                }
+            }
+            counts[i] = c;
+        }
+        rstart.assign(N + 1, 0);
+        for (int64_t i = 0; i < N; ++i) rstart[i + 1] = rstart[i] + counts[i];
+        int64_t S = rstart[N];
+        nbr.assign(S, 0); wts.assign(S, 0.0); dist.assign(S, {});
+        for (int64_t i = 0; i < N; ++i) {
+            int64_t k = rstart[i];
+            for (int w = 0; w < n_off; ++w) {
+                int64_t b = bin_neighbors[bin_of[i]][w];
+                for (int64_t s = cell_start[b]; s < cell_start[b + 1]; ++s) {
+                    int64_t j = cell_particles[s];
+                    if (i == j) continue;
+                    std::array<double, D> dd; get_distance(i, j, dd);
+                    double d2 = sumsq(dd);
+                    if (d2 <= R_sq) { nbr[k] = (int32_t)j; wts[k] = weight_fn(alpha, inv_range_sq, d2); dist[k] = dd; ++k; }
+                }
+            }
+        }
+    }
+
+    // prepare_geometry! (MUSCL.jl:172-187): grad cache (update_weights!, Interpolators.jl:517-577),
+    // divergence cache with intercept (MUSCL.jl:169-179) and per-slot dweights (MUSCL.jl:197-223)
+    void prepare_geometry() {
+        invg.assign(N, {}); okg.assign(N, 0); invd.assign(N, {}); okd.assign(N, 0);
+        dweights.assign(nbr.size(), {});
+        for (int64_t i = 0; i < N; ++i) {
+            {   // reconstruction fit: basis (d * invL), B = D
+                double A[BG][BG] = {};
+                for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                    double p[BG];
+                    for (int d = 0; d < D; ++d) p[d] = dist[k][d] * invL;
+                    for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) A[r][c] = A[r][c] + wts[k] * (p[r] * p[c]);
+                }
+                double U[BG][BG], X[BG][BG];
+                bool ok = chol_moment<BG>(A, U);
+                if (ok) { chol_inverse<BG>(U, X); for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) invg[i][r * BG + c] = X[r][c]; }
+                okg[i] = ok;
+            }
+            {   // divergence fit: [1; d * invL], B = D + 1 (MUSCL order 2: degree 1 for every stencil)
+                double A[BD][BD] = {};
+                for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                    double p[BD]; p[0] = 1.0;
+                    for (int d = 0; d < D; ++d) p[1 + d] = dist[k][d] * invL;
+                    for (int r = 0; r < BD; ++r) for (int c = 0; c < BD; ++c) A[r][c] = A[r][c] + wts[k] * (p[r] * p[c]);
+                }
+                double U[BD][BD], X[BD][BD];
+                bool ok = chol_moment<BD>(A, U);
+                if (ok) { chol_inverse<BD>(U, X); for (int r = 0; r < BD; ++r) for (int c = 0; c < BD; ++c) invd[i][r * BD + c] = X[r][c]; }
+                okd[i] = ok;
+                for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                    if (!ok) { dweights[k] = {}; continue; }
+                    double v[BD]; v[0] = wts[k] * 1.0;
+                    for (int d = 0; d < D; ++d) v[1 + d] = wts[k] * (dist[k][d] * invL);
+                    for (int r = 0; r < D; ++r) {
+                        double acc = invd[i][(1 + r) * BD + 0] * v[0];
+                        for (int c = 1; c < BD; ++c) acc = acc + invd[i][(1 + r) * BD + c] * v[c];
+                        dweights[k][r] = acc * invL;
+                    }
+                }
+            }
+        }
+    }
+
+    void set_scheme(const PDE& e, const Scheme& s) override {
+        pde = e; sc = s;
+        rhos.assign(N, State{}); rho_n.assign(N, State{}); rho_cand.assign(N, State{}); rho_sub.assign(N, State{});
+        for (int i = 0; i < 5; ++i) { Y[i].assign(N, State{}); K[i].assign(N, State{}); }
+        grads.assign(N, {});
+        orders.assign(N, 2);
+        triggered.assign(N, 0); changed.assign(N, 0); dirty.assign(N, 0); scratch.assign(N, 0); triggered_step.assign(N, 0);
+        status.assign(N, 0);
+        dflux.assign(nbr.size(), {});
+    }
+    void set_state(const double* U) override { for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) rhos[i][m] = U[i * M + m]; }
+    void get_state(double* U) const override { for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) U[i * M + m] = rhos[i][m]; }
+
+    // get_time_step with cached weights (ParticleGrids.jl:386-415, _cfl_weights_body! :439-450)
+    double time_step() override {
+        if (N == 0) return INFINITY;
+        double dtmin = INFINITY;
+        const double tol = 2.2250738585072014e-308, inv_dim = 1.0 / (double)D;
+#pragma omp parallel for num_threads(threads) reduction(min : dtmin) schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            if (isb[i]) continue;
+            if (rstart[i + 1] == rstart[i]) continue;
+            double lam_i[D]; P::maxeig(pde, rhos[i].data(), lam_i);
+            double sum_c = 0.0;
+            for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                double lam_j[D]; P::maxeig(pde, rhos[nbr[k]].data(), lam_j);
+                double dotv = std::fabs(dweights[k][0]) * jmax(lam_i[0], lam_j[0]);
+                for (int d = 1; d < D; ++d) dotv = dotv + std::fabs(dweights[k][d]) * jmax(lam_i[d], lam_j[d]);
+                sum_c += dotv;
+            }
+            if (sum_c > tol) dtmin = std::min(dtmin, inv_dim / sum_c);
+        }
+        return dtmin;
+    }
+
+    // VK / BJ limiter_phi (Limiter.jl:35-45)
+    inline double limiter_phi(double r) const {
+        if (sc.limiter == LIM_BJ) return jmin(1.0, r);
+        double r_pos = jmax(0.0, r);
+        double s = 1.0 / jmax(1.0, r_pos);
+        double small = (r_pos * r_pos + 2.0 * r_pos) / (r_pos * r_pos + r_pos + 2.0);
+        double large = (1.0 + 2.0 * s) / (1.0 + s + 2.0 * (s * s));
+        return (r_pos > 1.0) ? large : small;
+    }
+
+    // update_content! (MUSCL.jl:351-384) with apply_weights (Interpolators.jl:585-602) and
+    // _limit_slopes (Limiter.jl:120-143)
+    void gather_node(int64_t i, const std::vector<State>& U) {
+        const State& fi = U[i];
+        int64_t k0 = rstart[i], k1 = rstart[i + 1];
+        int64_t num_nb = k1 - k0;
+        int p_order = orders[i];
+        while (p_order > 1 && num_nb < D) --p_order;       // basis_count(D, p_order - 1) for MUSCL2
+        if (p_order < orders[i]) orders[i] = p_order;
+        std::array<State, BG> raw{};
+        if (p_order == 2 && okg[i]) {
+            double b[BG][M] = {};
+            for (int64_t k = k0; k < k1; ++k) {
+                double p[BG];
+                for (int d = 0; d < D; ++d) p[d] = dist[k][d] * invL;
+                const State& fj = U[nbr[k]];
+                for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) b[a][m] = b[a][m] + wts[k] * (p[a] * (fj[m] - fi[m]));
+            }
+            for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) {
+                double c = invg[i][a * BG + 0] * b[0][m];
+                for (int q = 1; q < BG; ++q) c = c + invg[i][a * BG + q] * b[q][m];
+                raw[a][m] = c * invL;          // build_scale_factors: invL for degree-1 terms
+            }
+        }
+        // (order 2 without a factorisation: the direct solve fails the same way and returns zero;
+        //  order 1: ConstantReconstruction, zero)
+        if (p_order == 1) { grads[i] = raw; return; }
+        if (sc.limiter == LIM_NONE) { grads[i] = raw; return; }
+        State umax = fi, umin = fi;
+        for (int64_t k = k0; k < k1; ++k) {
+            const State& fj = U[nbr[k]];
+            for (int m = 0; m < M; ++m) { umax[m] = jmax(umax[m], fj[m]); umin[m] = jmin(umin[m], fj[m]); }
+        }
+        State phi; phi.fill(1.0);
+        for (int64_t k = k0; k < k1; ++k) {
+            double pint[BG];
+            for (int d = 0; d < D; ++d) pint[d] = 0.5 * dist[k][d];
+            for (int m = 0; m < M; ++m) {
+                double rec = 0.0;
+                for (int a = 0; a < BG; ++a) rec = rec + raw[a][m] * pint[a];
+                double bound = rec > 0.0 ? umax[m] : umin[m];
+                double tolm = 16 * EPS * jmax(std::fabs(fi[m]), std::fabs(bound));
+                if (std::fabs(rec) > tolm) {
+                    double r = (bound - fi[m]) / rec;
+                    phi[m] = jmin(phi[m], limiter_phi(r));
+                }
+            }
+        }
+        for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) raw[a][m] = raw[a][m] * phi[m];   // :hard, or k <= D
+        grads[i] = raw;
+    }
+
+    // MUSCL functor (MUSCL.jl:390-424) with _flux_loop (:463-508) and apply_weights_divergence
+    // (Interpolators.jl:610-626); returns K_E = -2 div + 0 (TimeSteppers.jl:354)
+    void flux_node(int64_t i, const std::vector<State>& U, State& KE) {
+        int64_t k0 = rstart[i], k1 = rstart[i + 1];
+        if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + 0.0; return; }
+        const State& fi = U[i];
+        double Fi[D][M]; P::flux(pde, fi.data(), Fi);
+        const auto& gi = grads[i];
+        int oi = orders[i];
+        for (int64_t k = k0; k < k1; ++k) {
+            int64_t j = nbr[k];
+            const State& fj = U[j];
+            const auto& gj = grads[j];
+            double pi_[BG], pj_[BG];
+            for (int d = 0; d < D; ++d) { pi_[d] = 0.5 * dist[k][d]; pj_[d] = -0.5 * dist[k][d]; }
+            int oj = orders[j];
+            int o_i, o_j;
+            if (sc.strategy == STRAT_EPD1) { o_i = o_j = std::min(oi, oj); } else { o_i = oi; o_j = oj; }
+            if (o_i != 2) for (int d = 0; d < D; ++d) pi_[d] = 0.0;       // mask_basis
+            if (o_j != 2) for (int d = 0; d < D; ++d) pj_[d] = 0.0;
+            State fij, fji;
+            for (int m = 0; m < M; ++m) {
+                double si = gi[0][m] * pi_[0], sj = gj[0][m] * pj_[0];
+                for (int a = 1; a < BG; ++a) { si = si + gi[a][m] * pi_[a]; sj = sj + gj[a][m] * pj_[a]; }
+                fij[m] = fi[m] + si; fji[m] = fj[m] + sj;
+            }
+            double Fij[D][M], Fji[D][M];
+            P::flux(pde, fij.data(), Fij); P::flux(pde, fji.data(), Fji);
+            double lij[D], lji[D];
+            P::maxeig(pde, fij.data(), lij); P::maxeig(pde, fji.data(), lji);
+            for (int d = 0; d < D; ++d) {
+                bool pos_ = dist[k][d] > 0.0;           // sort_flux (Utils.jl:95-103)
+                const double* fL = pos_ ? fij.data() : fji.data();
+                const double* fR = pos_ ? fji.data() : fij.data();
+                const double* FL = pos_ ? Fij[d] : Fji[d];
+                const double* FR = pos_ ? Fji[d] : Fij[d];
+                double lL = pos_ ? lij[d] : lji[d], lR = pos_ ? lji[d] : lij[d];
+                double sd = jmax(lL, lR);               // max_eigenvalues (FluxFunctions.jl:16-22)
+                for (int m = 0; m < M; ++m) {
+                    double diss = sd * (fR[m] - fL[m]);
+                    double Fnum = 0.5 * (FL[m] + FR[m] - diss);      // RusanovFlux (FluxFunctions.jl:66-76)
+                    dflux[k][d][m] = (Fnum - Fi[d][m]) + 0.0;         // + evaluate_nc_jump (zero)
+                }
+            }
+        }
+        double div[M];
+        if (okd[i]) {
+            double b[BD][D * M] = {};
+            for (int64_t k = k0; k < k1; ++k) {
+                double p[BD]; p[0] = 1.0;
+                for (int d = 0; d < D; ++d) p[1 + d] = dist[k][d] * invL;
+                for (int a = 0; a < BD; ++a)
+                    for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m)
+                        b[a][d * M + m] = b[a][d * M + m] + wts[k] * (p[a] * dflux[k][d][m]);
+            }
+            for (int m = 0; m < M; ++m) {
+                double acc = 0.0;
+                for (int d = 0; d < D; ++d) {
+                    double r = invd[i][(1 + d) * BD + 0] * b[0][d * M + m];
+                    for (int a = 1; a < BD; ++a) r = r + invd[i][(1 + d) * BD + a] * b[a][d * M + m];
+                    acc = acc + r;
+                }
+                div[m] = acc * invL;
+            }
+        } else {
+            for (int m = 0; m < M; ++m) div[m] = 0.0;   // singular stencil: compute_dynamic_divergence returns zero
+        }
+        for (int m = 0; m < M; ++m) KE[m] = -(2.0 * div[m]) + 0.0;
+    }
+
+    // compute_explicit_rhs! (TimeSteppers.jl:312-356)
+    void compute_rhs(std::vector<State>& KE, const std::vector<State>& Yv, bool full) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t p = 0; p < N; ++p) {
+            if (isb[p] || !(full || changed[p])) continue;
+            gather_node(p, Yv);
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t p = 0; p < N; ++p) {
+            if (isb[p]) { if (full) KE[p] = State{}; continue; }
+            if (!(full || dirty[p])) continue;
+            flux_node(p, Yv, KE[p]);
+        }
+    }
+
+    // OutflowBC (BoundaryConditions.jl:82-131)
+    void outflow(std::vector<State>& buf, int tag) {
+        for (int64_t i = 0; i < N; ++i) status[i] = !isb[i] ? 1 : (tags[i] == tag ? 0 : -1);
+        for (int pass = 1; pass <= 5; ++pass) {
+            for (int64_t i = 0; i < N; ++i) {
+                if (status[i] != 0) continue;
+                int64_t closest = -1; double min_d2 = INFINITY;
+                for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                    int64_t j = nbr[k]; int64_t s = status[j];
+                    if (0 < s && s <= pass) {
+                        double d2 = sumsq(dist[k]);
+                        if (d2 < min_d2) { min_d2 = d2; closest = j; }
+                    }
+                }
+                if (closest != -1) { buf[i] = buf[closest]; status[i] = pass + 1; }
+            }
+            bool any0 = false;
+            for (int64_t i = 0; i < N; ++i) if (status[i] == 0) { any0 = true; break; }
+            if (!any0) break;
+        }
+        for (int64_t i = 0; i < N; ++i) if (status[i] == 0) buf[i] = rhos[i];
+    }
+    void apply_bcs(std::vector<State>& buf) {
+        for (int b = 0; b < sc.n_bc; ++b) {
+            if (sc.bc_kind[b] == BC_OUTFLOW) outflow(buf, sc.bc_tag[b]);
+            else if (sc.bc_kind[b] == BC_DIRICHLET)
+                for (int64_t i = 0; i < N; ++i) if (isb[i] && tags[i] == sc.bc_tag[b]) buf[i] = rhos[i];
+        }
+    }
+
+    // form_candidate! (TimeSteppers.jl:365-421), explicit tableau, no sources
+    std::vector<State>& form_candidate(int i, double dt) {
+        bool fin = i > sc.s;
+        std::vector<State>& target = fin ? rho_cand : Y[i - 1];
+        int n_prev = fin ? sc.s : i - 1;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t p = 0; p < N; ++p) {
+            if (isb[p]) { target[p] = rhos[p]; continue; }
+            State y = rho_n[p];
+            for (int j = 1; j <= n_prev; ++j) {
+                double at = fin ? sc.b_t[j - 1] : sc.a_t[i - 1][j - 1];
+                if (at != 0.0) {
+                    double c = dt * at;
+                    for (int m = 0; m < M; ++m) y[m] = y[m] + c * K[j - 1][p][m];
+                }
+            }
+            target[p] = y;
+        }
+        apply_bcs(target);
+        return target;
+    }
+
+    bool checks_substeps() const { return sc.substep_ratio > 0.0 && sc.criterion != CRIT_NONE; }
+
+    // _dmp_fails (MOOD.jl:175-184)
+    bool dmp_fails(int64_t p, const std::vector<State>& Up, const State& cand) const {
+        State mn = Up[p], mx = Up[p];
+        for (int64_t k = rstart[p]; k < rstart[p + 1]; ++k) {
+            const State& fj = Up[nbr[k]];
+            for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], fj[m]); mx[m] = jmax(mx[m], fj[m]); }
+        }
+        double d = sc.delta, d3 = d * d * d;
+        for (int m = 0; m < M; ++m)
+            if (std::fabs(mx[m] - mn[m]) >= d3 && (cand[m] < mn[m] - d || cand[m] > mx[m] + d)) return true;
+        return false;
+    }
+
+    // detect_and_drop! (MOOD.jl:375-405) for EPD strategies (no forcing)
+    bool detect_and_drop(const std::vector<State>& Up, const std::vector<State>& cand, bool full) {
+        if (sc.criterion == CRIT_NONE) return false;
+        std::fill(triggered.begin(), triggered.end(), 0);
+        std::fill(changed.begin(), changed.end(), 0);
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t p = 0; p < N; ++p) {
+            if (isb[p] || !(full || dirty[p])) continue;
+            bool flag = !P::admissible(pde, cand[p].data());
+            if (!flag) {
+                if (sc.criterion == CRIT_ONLY) flag = true;
+                else if (sc.criterion == CRIT_U1) flag = dmp_fails(p, Up, cand[p]);
+            }
+            if (flag) triggered[p] = 1;
+        }
+        bool any = false;
+        for (int64_t p = 0; p < N; ++p) {
+            if (!triggered[p]) continue;
+            if (orders[p] > 1) { orders[p] -= 1; changed[p] = 1; triggered_step[p] = 1; any = true; }
+            else triggered[p] = 0;
+        }
+        return any;
+    }
+
+    // expand_mask! one hop (MOOD.jl:323-351)
+    void expand_dirty() {
+        for (int64_t j = 0; j < N; ++j) {
+            uint8_t m = dirty[j];
+            if (!m) for (int64_t k = rstart[j]; k < rstart[j + 1]; ++k) if (dirty[nbr[k]]) { m = 1; break; }
+            scratch[j] = m;
+        }
+        dirty = scratch;
+    }
+
+    // mood_checkpoint! (TimeSteppers.jl:454-477) and form_substep! (:501-518)
+    void mood_checkpoint(int i, double dt) {
+        std::vector<State>& Yp = Y[i - 2];
+        std::vector<State>& Kp = K[i - 2];
+        bool full = true;
+        while (true) {
+            std::vector<State>& target = form_candidate(i, dt);
+            const std::vector<State>* cand = &target;
+            if (checks_substeps()) {
+                double h = sc.substep_ratio * dt;
+                for (int64_t p = 0; p < N; ++p) {
+                    if (isb[p]) { rho_sub[p] = Yp[p]; continue; }
+                    for (int m = 0; m < M; ++m) rho_sub[p][m] = Yp[p][m] + h * Kp[p][m];
+                }
+                cand = &rho_sub;
+            }
+            if (!detect_and_drop(Yp, *cand, full)) {
+                if (sc.criterion != CRIT_NONE)
+                    for (int64_t p = 0; p < N; ++p) n_inadmissible += (!isb[p] && !P::admissible(pde, (*cand)[p].data()));
+                return;
+            }
+            ++n_mood_iter;
+            dirty = changed;
+            expand_dirty();
+            compute_rhs(Kp, Yp, false);
+            full = false;
+        }
+    }
+
+    // the step functor (TimeSteppers.jl:531-577), static grid
+    void step(double t, double dt) override {
+        (void)t;
+        for (auto& gr : grads) gr = {};
+        rho_n = rhos;
+        std::fill(orders.begin(), orders.end(), 2);
+        std::fill(triggered_step.begin(), triggered_step.end(), 0);
+        n_inadmissible = 0;
+        n_mood_iter = 0;
+        for (int i = 1; i <= sc.s; ++i) {
+            if (i == 1) form_candidate(1, dt);
+            else mood_checkpoint(i, dt);
+            compute_rhs(K[i - 1], Y[i - 1], true);
+        }
+        mood_checkpoint(sc.s + 1, dt);
+        rhos = rho_cand;
+    }
+
+    // one full right-hand side at orders = 2 on the given state (a gather + flux pass)
+    void rhs(const double* Yin, double* Kout, double* gout) override {
+        std::vector<State> Yv(N), Kv(N);
+        for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Yv[i][m] = Yin[i * M + m];
+        for (auto& gr : grads) gr = {};
+        std::fill(orders.begin(), orders.end(), 2);
+        compute_rhs(Kv, Yv, true);
+        for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Kout[i * M + m] = Kv[i][m];
+        if (gout) for (int64_t i = 0; i < N; ++i) for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) gout[(i * BG + a) * M + m] = grads[i][a][m];
+    }
+
+    int64_t num_slots() const override { return (int64_t)nbr.size(); }
+    void get_neighbors(int64_t* ranges, int32_t* idx, double* w, double* dd) const override {
+        if (ranges) for (int64_t i = 0; i <= N; ++i) ranges[i] = rstart[i];
+        for (size_t k = 0; k < nbr.size(); ++k) {
+            if (idx) idx[k] = nbr[k];
+            if (w) w[k] = wts[k];
+            if (dd) for (int d = 0; d < D; ++d) dd[k * D + d] = dist[k][d];
+        }
+    }
+    void get_moments(double* ig, uint8_t* og, double* id, uint8_t* od, double* dw) const override {
+        for (int64_t i = 0; i < N; ++i) {
+            if (ig) for (int q = 0; q < BG * BG; ++q) ig[i * BG * BG + q] = invg[i][q];
+            if (og) og[i] = okg[i];
+            if (id) for (int q = 0; q < BD * BD; ++q) id[i * BD * BD + q] = invd[i][q];
+            if (od) od[i] = okd[i];
+        }
+        if (dw) for (size_t k = 0; k < dweights.size(); ++k) for (int d = 0; d < D; ++d) dw[k * D + d] = dweights[k][d];
+    }
+    void get_orders(int32_t* o) const override { for (int64_t i = 0; i < N; ++i) o[i] = orders[i]; }
+    void get_flags(uint8_t* f) const override { for (int64_t i = 0; i < N; ++i) f[i] = triggered_step[i]; }
+    int64_t inadmissible() const override { return n_inadmissible; }
+    int64_t mood_iterations() const override { return n_mood_iter; }
+    void get_bins(int32_t* bo, int64_t* cs, int32_t* cp, int64_t* dm) const override {
+        if (bo) for (int64_t i = 0; i < N; ++i) bo[i] = bin_of[i];
+        if (cs) for (int64_t b = 0; b < nbins + 2; ++b) cs[b] = cell_start[b];
+        if (cp) for (int64_t i = 0; i < N; ++i) cp[i] = cell_particles[i];
+        if (dm) { dm[0] = dims[0]; dm[1] = dims[1]; dm[2] = dims[2]; dm[3] = nbins; }
+    }
+};
+
+}  // namespace orc
+
+// =============================================================================================
+// C-ABI of the oracle (ctypes, tests and the CPU baseline only)
+// =============================================================================================
+using namespace orc;
+
+template <int B> static int spd_inverse_b(const double* A, double* X) {
+    double a[B][B], u[B][B], x[B][B];
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) a[r][c] = A[r * B + c];
+    if (!chol_moment<B>(a, u)) return 0;
+    chol_inverse<B>(u, x);
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) X[r * B + c] = x[r][c];
+    return 1;
+}
+
+extern "C" {
+
+static void fill_domain(Domain& g, int D, const double* mins, const double* maxs, const int32_t* periodic,
+                        const double* dx, double R) {
+    std::memset(&g, 0, sizeof(g));
+    g.D = D; g.R = R;
+    for (int d = 0; d < D; ++d) { g.mins[d] = mins[d]; g.maxs[d] = maxs[d]; g.periodic[d] = periodic[d]; g.dx[d] = dx[d]; }
+    computational_domain(g);
+}
+
+// Point cloud of a rectangular domain, Morton-ordered. Returns N (or -1 if cap is too small; *n_out
+// then holds the size needed). canvas: 2*D doubles (mins then maxs) of the computational domain.
+int64_t orc_cloud_generate(int D, const double* mins, const double* maxs, const int32_t* periodic, const double* dx,
+                           double R, const double* jitter, uint64_t seed, int64_t cap, double* pos, uint8_t* is_boundary,
+                           int32_t* tags, int64_t* n_out, double* canvas, int64_t* morton_ties) {
+    Domain g; fill_domain(g, D, mins, maxs, periodic, dx, R);
+    Cloud c; get_points(g, jitter, seed, c); morton_reorder(g, c);
+    *n_out = c.N;
+    if (canvas) for (int d = 0; d < D; ++d) { canvas[d] = g.canvas_min[d]; canvas[D + d] = g.canvas_max[d]; }
+    if (morton_ties) *morton_ties = c.morton_ties;
+    if (c.N > cap) return -1;
+    std::memcpy(pos, c.pos.data(), sizeof(double) * c.pos.size());
+    std::memcpy(is_boundary, c.is_boundary.data(), c.N);
+    std::memcpy(tags, c.tags.data(), sizeof(int32_t) * c.N);
+    return c.N;
+}
+
+void* orc_sim_new(int D, int M, int64_t N, const double* pos, const uint8_t* isb, const int32_t* tags,
+                  const double* mins, const double* maxs, const int32_t* periodic, const double* dx, double R, double alpha) {
+    Domain g; fill_domain(g, D, mins, maxs, periodic, dx, R);
+    if (D == 1 && M == 1) return new Sim<1, 1>(g, N, pos, isb, tags, alpha);
+    if (D == 2 && M == 1) return new Sim<2, 1>(g, N, pos, isb, tags, alpha);
+    if (D == 2 && M == 2) return new Sim<2, 2>(g, N, pos, isb, tags, alpha);
+    if (D == 2 && M == 4) return new Sim<2, 4>(g, N, pos, isb, tags, alpha);
+    if (D == 3 && M == 1) return new Sim<3, 1>(g, N, pos, isb, tags, alpha);
+    if (D == 3 && M == 5) return new Sim<3, 5>(g, N, pos, isb, tags, alpha);
+    return nullptr;
+}
+void orc_sim_free(void* s) { delete (SimBase*)s; }
+void orc_sim_set_threads(void* s, int n) { ((SimBase*)s)->set_threads(n); }
+
+// pde_kind: 1 linear advection (c: D*8 speeds, row per axis), 2 Euler (gamma)
+void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, int limiter, int mode, int strategy,
+                        int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
+                        double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds) {
+    SimBase* sb = (SimBase*)s;
+    PDE e; e.kind = pde_kind; e.gamma = gamma;
+    if (pde_kind == PDE_LINADV) {
+        for (int d = 0; d < sb->dim; ++d) {
+            double a = 0.0;
+            for (int m = 0; m < sb->ncomp; ++m) { e.c[d][m] = c[d * 8 + m]; a = std::max(a, std::fabs(c[d * 8 + m])); }
+            e.amax[d] = a;
+        }
+    }
+    Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
+    sc.s = stages;
+    for (int i = 0; i < stages; ++i) { sc.b_t[i] = b_t[i]; sc.c_t[i] = c_t[i]; for (int j = 0; j < stages; ++j) sc.a_t[i][j] = a_t[i * stages + j]; }
+    sc.substep_ratio = substep_ratio;
+    sc.n_bc = n_bc;
+    for (int b = 0; b < n_bc; ++b) { sc.bc_tag[b] = bc_tags[b]; sc.bc_kind[b] = bc_kinds[b]; }
+    sb->set_scheme(e, sc);
+}
+void orc_sim_set_state(void* s, const double* U) { ((SimBase*)s)->set_state(U); }
+void orc_sim_get_state(void* s, double* U) { ((SimBase*)s)->get_state(U); }
+double orc_sim_time_step(void* s) { return ((SimBase*)s)->time_step(); }
+void orc_sim_step(void* s, double t, double dt) { ((SimBase*)s)->step(t, dt); }
+void orc_sim_rhs(void* s, const double* Y, double* K, double* grads) { ((SimBase*)s)->rhs(Y, K, grads); }
+int64_t orc_sim_num_slots(void* s) { return ((SimBase*)s)->num_slots(); }
+void orc_sim_get_neighbors(void* s, int64_t* ranges, int32_t* idx, double* w, double* dist) { ((SimBase*)s)->get_neighbors(ranges, idx, w, dist); }
+void orc_sim_get_moments(void* s, double* ig, uint8_t* og, double* id, uint8_t* od, double* dw) { ((SimBase*)s)->get_moments(ig, og, id, od, dw); }
+void orc_sim_get_orders(void* s, int32_t* o) { ((SimBase*)s)->get_orders(o); }
+void orc_sim_get_flags(void* s, uint8_t* f) { ((SimBase*)s)->get_flags(f); }
+int64_t orc_sim_inadmissible(void* s) { return ((SimBase*)s)->inadmissible(); }
+int64_t orc_sim_mood_iterations(void* s) { return ((SimBase*)s)->mood_iterations(); }
+void orc_sim_get_bins(void* s, int32_t* bo, int64_t* cs, int32_t* cp, int64_t* dims) { ((SimBase*)s)->get_bins(bo, cs, cp, dims); }
+
+// scalar building blocks for known-answer tests
+double orc_weight(double alpha, double range, double d2) { return weight_fn(alpha, 1.0 / (range * range), d2); }
+double orc_limiter_phi(int limiter, double r) {
+    if (limiter == LIM_BJ) return jmin(1.0, r);
+    double r_pos = jmax(0.0, r);
+    double s = 1.0 / jmax(1.0, r_pos);
+    double small = (r_pos * r_pos + 2.0 * r_pos) / (r_pos * r_pos + r_pos + 2.0);
+    double large = (1.0 + 2.0 * s) / (1.0 + s + 2.0 * (s * s));
+    return (r_pos > 1.0) ? large : small;
+}
+double orc_jmax(double a, double b) { return jmax(a, b); }
+double orc_jmin(double a, double b) { return jmin(a, b); }
+double orc_jmod(double a, double b) { return jmod(a, b); }
+// inverse of a B x B SPD matrix via the conditional-Tikhonov Cholesky; returns ok
+int orc_spd_inverse(int B, const double* A, double* X) {
+    if (B == 2) return spd_inverse_b<2>(A, X);
+    if (B == 3) return spd_inverse_b<3>(A, X);
+    if (B == 4) return spd_inverse_b<4>(A, X);
+    return -1;
+}
+// Euler flux / eigenvalues / admissibility of one state (D = 2 or 3)
+void orc_euler(int D, double gamma, const double* U, double* F, double* lam, int32_t* adm) {
+    PDE e; e.kind = PDE_EULER; e.gamma = gamma;
+    if (D == 2) { double f[2][4], l[2]; Phys<2, 4>::flux(e, U, f); Phys<2, 4>::maxeig(e, U, l);
+        for (int d = 0; d < 2; ++d) { lam[d] = l[d]; for (int m = 0; m < 4; ++m) F[d * 4 + m] = f[d][m]; } *adm = Phys<2, 4>::admissible(e, U); }
+    else { double f[3][5], l[3]; Phys<3, 5>::flux(e, U, f); Phys<3, 5>::maxeig(e, U, l);
+        for (int d = 0; d < 3; ++d) { lam[d] = l[d]; for (int m = 0; m < 5; ++m) F[d * 5 + m] = f[d][m]; } *adm = Phys<3, 5>::admissible(e, U); }
+}
+void orc_rng_uniform(uint64_t seed, int64_t n, double* out) {
+    Rng r(seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = r.uniform();
+}
+int orc_num_threads_max(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it as the checker
+of the HIP path (and as the timed CPU baseline). Nothing in the product imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run oracle/build_oracle.py (or __graft_entry__.build())")
+        _lib = C.CDLL(LIB)
+        vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+        P = C.POINTER
+        sig = {
+            "orc_cloud_generate": (i64, [C.c_int, vp, vp, vp, vp, d, vp, C.c_uint64, i64, vp, vp, vp, P(i64), vp, P(i64)]),
+            "orc_sim_new": (vp, [C.c_int, C.c_int, i64, vp, vp, vp, vp, vp, vp, vp, d, d]),
+            "orc_sim_free": (None, [vp]),
+            "orc_sim_set_threads": (None, [vp, C.c_int]),
+            "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
+                                          d, C.c_int, vp, vp]),
+            "orc_sim_set_state": (None, [vp, vp]),
+            "orc_sim_get_state": (None, [vp, vp]),
+            "orc_sim_time_step": (d, [vp]),
+            "orc_sim_step": (None, [vp, d, d]),
+            "orc_sim_rhs": (None, [vp, vp, vp, vp]),
+            "orc_sim_num_slots": (i64, [vp]),
+            "orc_sim_get_neighbors": (None, [vp, vp, vp, vp, vp]),
+            "orc_sim_get_moments": (None, [vp, vp, vp, vp, vp, vp]),
+            "orc_sim_get_orders": (None, [vp, vp]),
+            "orc_sim_get_flags": (None, [vp, vp]),
+            "orc_sim_inadmissible": (i64, [vp]),
+            "orc_sim_mood_iterations": (i64, [vp]),
+            "orc_sim_get_bins": (None, [vp, vp, vp, vp, vp]),
+            "orc_weight": (d, [d, d, d]),
+            "orc_limiter_phi": (d, [C.c_int, d]),
+            "orc_jmax": (d, [d, d]),
+            "orc_jmin": (d, [d, d]),
+            "orc_jmod": (d, [d, d]),
+            "orc_spd_inverse": (C.c_int, [C.c_int, vp, vp]),
+            "orc_euler": (None, [C.c_int, d, vp, vp, vp, vp]),
+            "orc_rng_uniform": (None, [C.c_uint64, i64, vp]),
+            "orc_num_threads_max": (C.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(_lib, name)
+            f.restype, f.argtypes = res, args
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _arr(v, dt, n=3):
+    a = np.zeros(n, dt)
+    a[:len(v)] = v
+    return a
+
+
+def cloud_generate(dom):
+    D = dom["D"]
+    mins, maxs = _arr(dom["mins"], np.float64), _arr(dom["maxs"], np.float64)
+    per = _arr([int(bool(x)) for x in dom["periodic"]], np.int32)
+    dx, jit = _arr(dom["dx"], np.float64), _arr(dom["jitter"], np.float64)
+    n = C.c_int64(0)
+    ties = C.c_int64(0)
+    canvas = np.zeros(2 * D)
+    L = lib()
+    L.orc_cloud_generate(D, _p(mins), _p(maxs), _p(per), _p(dx), dom["R"], _p(jit), dom["seed"], 0, None, None, None,
+                         C.byref(n), _p(canvas), C.byref(ties))
+    N = n.value
+    pos = np.zeros((N, D))
+    isb = np.zeros(N, np.uint8)
+    tags = np.zeros(N, np.int32)
+    L.orc_cloud_generate(D, _p(mins), _p(maxs), _p(per), _p(dx), dom["R"], _p(jit), dom["seed"], N, _p(pos), _p(isb),
+                         _p(tags), C.byref(n), _p(canvas), C.byref(ties))
+    return {"pos": pos, "is_boundary": isb, "tags": tags, "canvas": canvas, "morton_ties": ties.value, "N": N}
+
+
+class OracleSim:
+    """Grid + MUSCL(2) + GeneralRKTimeStepper restated on the CPU."""
+
+    def __init__(self, dom, pos, isb, tags, scheme=None, threads=1):
+        D = dom["D"]
+        self.D = D
+        self.N = pos.shape[0]
+        self.M = scheme["M"] if scheme else 1
+        self._keep = [np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(isb, np.uint8),
+                      np.ascontiguousarray(tags, np.int32)]
+        mins, maxs = _arr(dom["mins"], np.float64), _arr(dom["maxs"], np.float64)
+        per = _arr([int(bool(x)) for x in dom["periodic"]], np.int32)
+        dx = _arr(dom["dx"], np.float64)
+        self.h = lib().orc_sim_new(D, self.M, self.N, _p(self._keep[0]), _p(self._keep[1]), _p(self._keep[2]), _p(mins),
+                                   _p(maxs), _p(per), _p(dx), dom["R"], dom["alpha"])
+        if not self.h:
+            raise ValueError(f"oracle has no (D={D}, M={self.M}) variant")
+        lib().orc_sim_set_threads(self.h, int(threads))
+        if scheme:
+            self.set_scheme(scheme)
+
+    def set_scheme(self, sch):
+        s = len(sch["b_t"])
+        a_t = np.zeros((s, s))
+        for i in range(s):
+            for j in range(s):
+                a_t[i, j] = sch["a_t"][i][j]
+        b_t, c_t = np.array(sch["b_t"], float), np.array(sch["c_t"], float)
+        c = np.zeros((3, 8))
+        for d, row in enumerate(sch.get("adv_speed", [])):
+            c[d, :len(row)] = row
+        bcs = sch.get("bcs", [])
+        tags = np.array([t for t, _ in bcs] or [0], np.int32)
+        kinds = np.array([k for _, k in bcs] or [0], np.int32)
+        self._keep += [a_t, b_t, c_t, c, tags, kinds]
+        lib().orc_sim_set_scheme(self.h, sch["pde"], sch.get("gamma", 1.4), _p(c), sch["limiter"],
+                                 sch.get("limiter_mode", 1), sch["strategy"], sch["criterion"], sch.get("delta", 0.0), s,
+                                 _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds))
+
+    def set_threads(self, n):
+        lib().orc_sim_set_threads(self.h, int(n))
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)
+        lib().orc_sim_set_state(self.h, _p(U))
+
+    def state(self):
+        U = np.zeros((self.N, self.M))
+        lib().orc_sim_get_state(self.h, _p(U))
+        return U
+
+    def time_step(self):
+        return lib().orc_sim_time_step(self.h)
+
+    def step(self, t, dt):
+        lib().orc_sim_step(self.h, float(t), float(dt))
+
+    def rhs(self, Y):
+        Y = np.ascontiguousarray(Y, np.float64).reshape(self.N, self.M)
+        K = np.zeros((self.N, self.M))
+        G = np.zeros((self.N, self.D, self.M))
+        lib().orc_sim_rhs(self.h, _p(Y), _p(K), _p(G))
+        return K, G
+
+    def neighbors(self):
+        S = lib().orc_sim_num_slots(self.h)
+        ranges = np.zeros(self.N + 1, np.int64)
+        idx = np.zeros(max(S, 1), np.int32)
+        w = np.zeros(max(S, 1))
+        dist = np.zeros((max(S, 1), self.D))
+        lib().orc_sim_get_neighbors(self.h, _p(ranges), _p(idx), _p(w), _p(dist))
+        return ranges, idx[:S], w[:S], dist[:S]
+
+    def moments(self):
+        D, N = self.D, self.N
+        S = lib().orc_sim_num_slots(self.h)
+        ig = np.zeros((N, D, D))
+        og = np.zeros(N, np.uint8)
+        idv = np.zeros((N, D + 1, D + 1))
+        od = np.zeros(N, np.uint8)
+        dw = np.zeros((max(S, 1), D))
+        lib().orc_sim_get_moments(self.h, _p(ig), _p(og), _p(idv), _p(od), _p(dw))
+        return ig, og, idv, od, dw[:S]
+
+    def bins(self):
+        dims = np.zeros(4, np.int64)
+        lib().orc_sim_get_bins(self.h, None, None, None, _p(dims))
+        nb = int(dims[3])
+        bo = np.zeros(self.N, np.int32)
+        cs = np.zeros(nb + 2, np.int64)
+        cp = np.zeros(self.N, np.int32)
+        lib().orc_sim_get_bins(self.h, _p(bo), _p(cs), _p(cp), _p(dims))
+        return bo, cs, cp, dims
+
+    def orders(self):
+        o = np.zeros(self.N, np.int32)
+        lib().orc_sim_get_orders(self.h, _p(o))
+        return o
+
+    def mood_flags(self):
+        f = np.zeros(self.N, np.uint8)
+        lib().orc_sim_get_flags(self.h, _p(f))
+        return f
+
+    def inadmissible(self):
+        return lib().orc_sim_inadmissible(self.h)
+
+    def mood_iterations(self):
+        return lib().orc_sim_mood_iterations(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_sim_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def weight(alpha, R, d2):
+    return lib().orc_weight(alpha, R, d2)
+
+
+def limiter_phi(kind, r):
+    return lib().orc_limiter_phi(kind, r)
+
+
+def spd_inverse(A):
+    A = np.ascontiguousarray(A, np.float64)
+    X = np.zeros_like(A)
+    ok = lib().orc_spd_inverse(A.shape[0], _p(A), _p(X))
+    return X, ok
+
+
+def euler(D, gamma, U):
+    U = np.ascontiguousarray(U, np.float64)
+    F = np.zeros((D, D + 2))
+    lam = np.zeros(D)
+    adm = C.c_int32()
+    lib().orc_euler(D, gamma, _p(U), _p(F), _p(lam), C.byref(adm))
+    return F, lam, bool(adm.value)
+
+
+def rng_uniform(seed, n):
+    out = np.zeros(n)
+    lib().orc_rng_uniform(seed, n, _p(out))
+    return out
