@@ -1,0 +1,152 @@
+"""
+    HyperCloudHIP
+
+Julia binding of libhcc (include/hcc.h): the MI355X time-step path behind HyperCloudCore.jl's operator API.
+
+A `HIPRKTimeStepper` is a `TimeStepper` with the functor signature of `GeneralRKTimeStepper`
+(src/TimeStepping/TimeSteppers.jl:531): `ts(eq, pg, t, dt)` advances `pg.rhos` in place. `hip_time_step(ts, pg, eq)`
+replaces `get_time_step(pg, eq, interp)` (src/Grid/ParticleGrids.jl:386). PDE types map to compiled kernel variants
+through `hip_descriptor`; a PDE without one raises (there is no fallback path).
+
+This file is not exercised in this repository (no Julia toolchain); the Python host package exercises the same
+C ABI in the tests.
+"""
+module HyperCloudHIP
+
+using HyperCloudCore
+using StaticArrays
+
+const libhcc = joinpath(@__DIR__, "..", "lib", "libhcc.so")
+
+const HCC_PDE_LINADV = Int32(1)
+const HCC_PDE_EULER  = Int32(2)
+const HCC_BC_OUTFLOW = Int32(1)
+const HCC_BC_DIRICHLET = Int32(2)
+
+struct DomainDesc
+    D::Int32
+    mins::NTuple{3, Float64}
+    maxs::NTuple{3, Float64}
+    periodic::NTuple{3, Int32}
+    dx::NTuple{3, Float64}
+    R::Float64
+    alpha::Float64
+    jitter::NTuple{3, Float64}
+    seed::UInt64
+end
+
+struct StepperDesc
+    M::Int32
+    pde::Int32
+    gamma::Float64
+    adv_speed::NTuple{3, NTuple{8, Float64}}
+    muscl_order::Int32
+    limiter::Int32
+    limiter_mode::Int32
+    mood_strategy::Int32
+    mood_criterion::Int32
+    mood_delta::Float64
+    stages::Int32
+    a_t::NTuple{5, NTuple{5, Float64}}
+    b_t::NTuple{5, Float64}
+    c_t::NTuple{5, Float64}
+    substep_ratio::Float64
+    n_bc::Int32
+    bc_tag::NTuple{16, Int32}
+    bc_kind::NTuple{16, Int32}
+end
+
+checkrc(rc, ctx) = rc == 0 || error("libhcc: ", unsafe_string(ccall((:hcc_last_error, libhcc), Cstring, (Ptr{Cvoid},), ctx)))
+
+"PDE descriptor of a physics type: `(kind, gamma, speeds)`; add a method for each PDE the kernels compile."
+hip_descriptor(eq) = error("no compiled HIP kernel for $(typeof(eq)): add a `hip_descriptor` method (there is no fallback)")
+
+pad3(v, x) = ntuple(i -> i <= length(v) ? v[i] : x, 3)
+
+mutable struct HIPRKTimeStepper <: TimeStepper
+    ctx::Ptr{Cvoid}
+    grid::Ptr{Cvoid}
+    st::Ptr{Cvoid}
+    N::Int
+    M::Int
+end
+
+"""
+    HIPRKTimeStepper(eq, pg, desc::DomainDesc, interp::MUSCL, mood::MOOD, tableau; device = 0, substep_ratio)
+
+Uploads the nodes of `pg` and builds the device grid (neighbour search on the GPU) and the stepper.
+`substep_ratio` is `ts.substep_ratio` of the equivalent `GeneralRKTimeStepper` (Julia's bisection result).
+"""
+function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::Integer = 0)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_ctx_create, libhcc), Cint, (Cint, Ref{Ptr{Cvoid}}), device, ctx), C_NULL)
+    N = pg.meta.N
+    pos = Vector{Float64}(undef, N * desc.D)
+    for i in 1:N, d in 1:desc.D
+        pos[(i - 1) * desc.D + d] = pg.core.positions[i][d]
+    end
+    isb = UInt8.(pg.core.is_boundary[1:N])
+    tags = Int32.(pg.core.tags[1:N])
+    grid = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve pos isb tags begin
+        checkrc(ccall((:hcc_grid_create, libhcc), Cint,
+                      (Ptr{Cvoid}, Ref{DomainDesc}, Int64, Ptr{Float64}, Ptr{UInt8}, Ptr{Int32}, Ref{Ptr{Cvoid}}),
+                      ctx[], desc, N, pos, isb, tags, grid), ctx[])
+    end
+    st = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_stepper_create, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{StepperDesc}, Ref{Ptr{Cvoid}}),
+                  ctx[], grid[], sdesc, st), ctx[])
+    ts = HIPRKTimeStepper(ctx[], grid[], st[], N, Int(sdesc.M))
+    upload!(ts, pg)
+    finalizer(ts) do t
+        ccall((:hcc_stepper_destroy, libhcc), Cint, (Ptr{Cvoid},), t.st)
+        ccall((:hcc_grid_destroy, libhcc), Cint, (Ptr{Cvoid},), t.grid)
+        ccall((:hcc_ctx_destroy, libhcc), Cint, (Ptr{Cvoid},), t.ctx)
+    end
+    return ts
+end
+
+# pg.rhos is a Vector{SVector{M,Float64}}: M contiguous doubles per node, the AoS layout of the ABI
+function upload!(ts::HIPRKTimeStepper, pg)
+    U = pg.rhos
+    GC.@preserve U checkrc(ccall((:hcc_state_upload, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ts.st, pointer(U)), ts.ctx)
+end
+function download!(ts::HIPRKTimeStepper, pg)
+    U = pg.rhos
+    GC.@preserve U checkrc(ccall((:hcc_state_download, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ts.st, pointer(U)), ts.ctx)
+end
+
+"get_time_step(pg, eq, interp) on the device (CFL number 1)."
+function hip_time_step(ts::HIPRKTimeStepper)
+    dt = Ref{Float64}(0.0)
+    checkrc(ccall((:hcc_time_step, libhcc), Cint, (Ptr{Cvoid}, Ref{Float64}), ts.st, dt), ts.ctx)
+    return dt[]
+end
+
+"The stepper functor: the state stays on the device between steps; call `download!` to read `pg.rhos`."
+function (ts::HIPRKTimeStepper)(eq, pg, time::Real, dt::Float64)
+    checkrc(ccall((:hcc_step, libhcc), Cint, (Ptr{Cvoid}, Float64, Float64), ts.st, time, dt), ts.ctx)
+    return nothing
+end
+
+function mood_flags(ts::HIPRKTimeStepper)
+    f = zeros(UInt8, ts.N)
+    checkrc(ccall((:hcc_get_mood_flags, libhcc), Cint, (Ptr{Cvoid}, Ptr{UInt8}), ts.st, f), ts.ctx)
+    return Bool.(f)
+end
+
+function particle_orders(ts::HIPRKTimeStepper)
+    o = zeros(Int32, ts.N)
+    checkrc(ccall((:hcc_get_orders, libhcc), Cint, (Ptr{Cvoid}, Ptr{Int32}), ts.st, o), ts.ctx)
+    return Int.(o)
+end
+
+function mood_inadmissible(ts::HIPRKTimeStepper)
+    n = Ref{Int64}(0)
+    checkrc(ccall((:hcc_get_inadmissible, libhcc), Cint, (Ptr{Cvoid}, Ref{Int64}), ts.st, n), ts.ctx)
+    return Int(n[])
+end
+
+export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc
+
+end # module
