@@ -83,7 +83,12 @@ struct StepArgs {
     uint8_t* dirty;
     uint8_t* trig;               // mood_flags of the step
     uint8_t* inadm;              // last check: candidate not admissible
-    unsigned long long* counters;// [0] orders changed in the last check, [1] inadmissible accepted, [2] dt bits
+    unsigned long long* counters;// [0] orders changed in the last check (= clist length), [1] inadmissible accepted,
+                                 // [2] dt bits, [3] dlist length
+    int32_t* clist;              // particles whose order changed in the last check
+    int32_t* dlist;              // dirty particles of the fixed point (changed plus one hop)
+    unsigned* dmark;             // dirty marks (cleared by the pass that consumes dlist)
+    const double* lam;           // per-particle wave speeds for the CFL estimate, [d * N + i]
 };
 
 }  // namespace hcc

@@ -720,13 +720,10 @@ __global__ __launch_bounds__(kBlock) void k_predict(int64_t N, int64_t nblk, con
 }
 
 // update_content! (MUSCL.jl:351-384): cached reconstruction fit (Interpolators.jl:585-602) and the
-// Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143). Interior particles, all (full) or changed.
+// Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143) for one interior particle.
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int64_t nblk) {
-    const int64_t i = node_index(nblk);
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
     const int64_t N = a.g.N;
-    if (i >= N) return;
-    if (a.g.isb[i] || !(full || a.changed[i])) return;
     const int cnt = a.g.cnt[i];
     int order = a.orders[i];
     if (order > 1 && cnt < D) { order = 1; a.orders[i] = 1; }   // starved neighbourhood (basis_count(D, 1) = D)
@@ -788,94 +785,202 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
     for (int q = 0; q < D; ++q) for (int m = 0; m < M; ++m) a.grads[(q * M + m) * N + i] = g[q][m];
 }
 
+// The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    if (a.g.isb[i] || !(full || a.changed[i])) return;
+    gather_node<D, M, KIND>(a, Y, i);
+}
+
+// The gather pass over a compacted list of particles whose order changed (the `changed` set of the MOOD fixed point).
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double* __restrict__ Y, const int32_t* __restrict__ list,
+                                                         int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = list[t];
+    if (a.g.isb[i]) return;
+    gather_node<D, M, KIND>(a, Y, i);
+}
+
 // MUSCL functor + _flux_loop (MUSCL.jl:390-508): per slot the EPD-masked midpoint reconstructions, two physical
 // fluxes, the Rusanov flux sorted by the sign of the distance (Utils.jl:95-103, FluxFunctions.jl:16-76), and
 // the flux difference accumulated straight into the moments of the cached divergence fit
 // (apply_weights_divergence, Interpolators.jl:610-626); K_E = -2 div (TimeSteppers.jl:354).
-template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, const double* __restrict__ Y, double* __restrict__ K, int full,
-                                                  int64_t nblk) {
+//
+// With CHECK the kernel also runs MOOD's candidate check on the node it just updated (_mood_flag!, MOOD.jl:289-295):
+// the forward Euler substep W = Y + h K is node-local (form_substep!, TimeSteppers.jl:501-518) and the DMP bounds of
+// MOODu1 (_dmp_fails, MOOD.jl:175-184) are the extrema of Y over the same neighbour loop. The set of nodes it checks
+// is exactly the set whose K it recomputes (all interior nodes, or the dirty ones), as in detect_and_drop!. The order
+// decrement (_mood_drop!) is deferred to k_expand_apply: neighbours read this node's order in this very kernel.
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, double* __restrict__ K,
+                                          int64_t i, double h) {
     using P = Pde<D, M, KIND>;
     constexpr int BD = D + 1;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    double fi[M];
+    for (int m = 0; m < M; ++m) fi[m] = Y[m * N + i];
+    double mn[M], mx[M];
+    for (int m = 0; m < M; ++m) { mn[m] = fi[m]; mx[m] = fi[m]; }
+    double Kout[M];
+    if (cnt == 0) {
+        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
+    } else {
+        const NodePos<D> pi = load_pos<D>(a.g, i);
+        const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
+        double gi[D][M], Fi[D][M];
+        for (int q = 0; q < D; ++q) for (int m = 0; m < M; ++m) gi[q][m] = a.grads[(q * M + m) * N + i];
+        P::flux(a.pde, fi, Fi);
+        const int oi = a.orders[i];
+        double b[BD][D * M];
+        for (int c = 0; c < BD; ++c) for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double dd[D];
+            const double d2 = slot_geom<D>(a.g, pi, j, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            const int oj = a.orders[j];
+            int o_i = oi, o_j = oj;
+            if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
+            double pii[D], pjj[D];
+            for (int d = 0; d < D; ++d) {
+                pii[d] = (o_i == 2) ? 0.5 * dd[d] : 0.0;
+                pjj[d] = (o_j == 2) ? -0.5 * dd[d] : 0.0;
+            }
+            double fij[M], fji[M];
+            for (int m = 0; m < M; ++m) {
+                double si = gi[0][m] * pii[0];
+                for (int q = 1; q < D; ++q) si = si + gi[q][m] * pii[q];
+                double sj = a.grads[(0 * M + m) * N + j] * pjj[0];
+                for (int q = 1; q < D; ++q) sj = sj + a.grads[(q * M + m) * N + j] * pjj[q];
+                const double uj = Y[m * N + j];
+                if (CHECK) { mn[m] = jmin(mn[m], uj); mx[m] = jmax(mx[m], uj); }
+                fij[m] = fi[m] + si;
+                fji[m] = uj + sj;
+            }
+            double Fij[D][M], Fji[D][M], lij[D], lji[D];
+            P::flux_eig(a.pde, fij, Fij, lij);
+            P::flux_eig(a.pde, fji, Fji, lji);
+            double pt[BD];
+            pt[0] = 1.0;
+            for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
+            for (int d = 0; d < D; ++d) {
+                // sort_flux orders the two sides by sign(d); F_L + F_R is the same sum either way (IEEE addition
+                // commutes), only the dissipation f_R - f_L needs the sorted states
+                const bool right = dd[d] > 0.0;
+                const double sd = right ? jmax(lij[d], lji[d]) : jmax(lji[d], lij[d]);
+                for (int m = 0; m < M; ++m) {
+                    const double fL = right ? fij[m] : fji[m], fR = right ? fji[m] : fij[m];
+                    const double diss = sd * (fR - fL);
+                    const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
+                    const double dF = (Fnum - Fi[d][m]) + 0.0;
+                    for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
+                }
+            }
+        }
+        if (a.okd[i]) {
+            for (int m = 0; m < M; ++m) {
+                double acc = 0.0;
+                for (int d = 0; d < D; ++d) {
+                    double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
+                    for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
+                    acc = acc + r;
+                }
+                Kout[m] = -(2.0 * (acc * invL)) + 0.0;
+            }
+        } else {
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+        }
+    }
+    for (int m = 0; m < M; ++m) K[m * N + i] = Kout[m];
+    if (!CHECK) return false;
+    double W[M];
+    for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
+    const bool adm = P::admissible(a.pde, W);
+    a.inadm[i] = !adm;
+    bool flag = !adm;
+    if (!flag) {
+        if (a.criterion == HCC_CRIT_ONLY) {
+            flag = true;
+        } else if (a.criterion == HCC_CRIT_U1) {
+            const double dl = a.delta, d3 = dl * dl * dl;
+            for (int m = 0; m < M; ++m)
+                if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) { flag = true; break; }
+        }
+    }
+    if (flag && a.orders[i] > 1) { a.trig[i] = 1; return true; }
+    return false;
+}
+
+// Appends the flagged particles of a wavefront to `clist` (one atomic per wavefront) and counts them.
+__device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64_t i) {
+    const unsigned long long ball = __ballot(ch);
+    if (!ball) return;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
+    base = __shfl(base, (int)leader);
+    if (ch) {
+        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        a.clist[base + __popcll(below)] = (int32_t)i;
+    }
+}
+
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, const double* __restrict__ Y, double* __restrict__ K, int full,
+                                                  double h, int64_t nblk) {
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
-    if (i >= N) return;
-    if (a.g.isb[i]) {
-        if (full) for (int m = 0; m < M; ++m) K[m * N + i] = 0.0;
-        return;
+    bool ch = false;
+    if (i < N) {
+        if (a.g.isb[i]) {
+            if (full) for (int m = 0; m < M; ++m) K[m * N + i] = 0.0;
+        } else if (full || a.dirty[i]) {
+            ch = flux_node<D, M, KIND, CHECK>(a, Y, K, i, h);
+        }
+        if (CHECK) a.changed[i] = ch;
     }
-    if (!(full || a.dirty[i])) return;
+    if (CHECK) append_changed(a, ch, i);
+}
+
+// The flux pass (with the MOOD check) over the compacted dirty list of the fixed point; clears each entry's mark.
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
+                                                       double h) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t n = (int64_t)a.counters[3];
+    bool ch = false;
+    int64_t i = 0;
+    if (t < n) {
+        i = a.dlist[t];
+        a.dmark[i] = 0;
+        if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, K, i, h);
+        a.changed[i] = ch;
+    }
+    append_changed(a, ch, i);
+}
+
+// _mood_drop! (MOOD.jl:299-309) and one hop of expand_mask! (MOOD.jl:323-351) for the changed list: the particle
+// drops one order, and it and its neighbours join the dirty list once (first marker appends). The neighbour
+// relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
+__global__ __launch_bounds__(kBlock) void k_mark(StepArgs a, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = a.clist[t];
+    a.orders[i] = (int8_t)(a.orders[i] - 1);
+    if (atomicExch(&a.dmark[i], 1u) == 0u) a.dlist[atomicAdd(&a.counters[3], 1ull)] = i;
     const int cnt = a.g.cnt[i];
-    if (cnt == 0) {
-        for (int m = 0; m < M; ++m) K[m * N + i] = -0.0 + 0.0;
-        return;
-    }
-    const NodePos<D> pi = load_pos<D>(a.g, i);
     const int64_t base = sell_base(a.g, i);
-    const double invL = a.g.invL;
-    double fi[M], gi[D][M], Fi[D][M];
-    for (int m = 0; m < M; ++m) fi[m] = Y[m * N + i];
-    for (int q = 0; q < D; ++q) for (int m = 0; m < M; ++m) gi[q][m] = a.grads[(q * M + m) * N + i];
-    P::flux(a.pde, fi, Fi);
-    const int oi = a.orders[i];
-    double b[BD][D * M];
-    for (int c = 0; c < BD; ++c) for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
     for (int k = 0; k < cnt; ++k) {
         const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
-        double dd[D];
-        const double d2 = slot_geom<D>(a.g, pi, j, dd);
-        const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
-        const int oj = a.orders[j];
-        int o_i = oi, o_j = oj;
-        if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
-        double pii[D], pjj[D];
-        for (int d = 0; d < D; ++d) {
-            pii[d] = (o_i == 2) ? 0.5 * dd[d] : 0.0;
-            pjj[d] = (o_j == 2) ? -0.5 * dd[d] : 0.0;
-        }
-        double fij[M], fji[M];
-        for (int m = 0; m < M; ++m) {
-            double si = gi[0][m] * pii[0];
-            for (int q = 1; q < D; ++q) si = si + gi[q][m] * pii[q];
-            double sj = a.grads[(0 * M + m) * N + j] * pjj[0];
-            for (int q = 1; q < D; ++q) sj = sj + a.grads[(q * M + m) * N + j] * pjj[q];
-            fij[m] = fi[m] + si;
-            fji[m] = Y[m * N + j] + sj;
-        }
-        double Fij[D][M], Fji[D][M], lij[D], lji[D];
-        P::flux_eig(a.pde, fij, Fij, lij);
-        P::flux_eig(a.pde, fji, Fji, lji);
-        double pt[BD];
-        pt[0] = 1.0;
-        for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
-        for (int d = 0; d < D; ++d) {
-            const bool right = dd[d] > 0.0;
-            const double sd = right ? jmax(lij[d], lji[d]) : jmax(lji[d], lij[d]);
-            for (int m = 0; m < M; ++m) {
-                const double fL = right ? fij[m] : fji[m], fR = right ? fji[m] : fij[m];
-                const double FL = right ? Fij[d][m] : Fji[d][m], FR = right ? Fji[d][m] : Fij[d][m];
-                const double diss = sd * (fR - fL);
-                const double Fnum = 0.5 * (FL + FR - diss);
-                const double dF = (Fnum - Fi[d][m]) + 0.0;
-                for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
-            }
-        }
+        if (atomicExch(&a.dmark[j], 1u) == 0u) a.dlist[atomicAdd(&a.counters[3], 1ull)] = j;
     }
-    double div[M];
-    if (a.okd[i]) {
-        for (int m = 0; m < M; ++m) {
-            double acc = 0.0;
-            for (int d = 0; d < D; ++d) {
-                double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
-                for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
-                acc = acc + r;
-            }
-            div[m] = acc * invL;
-        }
-    } else {
-        for (int m = 0; m < M; ++m) div[m] = 0.0;
-    }
-    for (int m = 0; m < M; ++m) K[m * N + i] = -(2.0 * div[m]) + 0.0;
 }
 
 // _mood_flag! + _mood_drop! (MOOD.jl:289-309) fused, on the forward Euler substep W = Y + h K
@@ -927,11 +1032,13 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
         atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
 }
 
-// expand_mask! by one hop (MOOD.jl:323-351): dirty = changed plus the neighbours of changed particles
-__global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, int64_t nblk) {
+// expand_mask! by one hop (MOOD.jl:323-351): dirty = changed plus the neighbours of changed particles. With
+// apply_drop the decrement of _mood_drop! (MOOD.jl:299-309) flagged by the fused check is applied here first.
+__global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, int apply_drop, int64_t nblk) {
     const int64_t i = node_index(nblk);
     if (i >= a.g.N) return;
     uint8_t m = a.changed[i];
+    if (apply_drop && m) a.orders[i] = (int8_t)(a.orders[i] - 1);
     if (!m) {
         const int cnt = a.g.cnt[i];
         const int64_t base = sell_base(a.g, i);
@@ -950,30 +1057,53 @@ __global__ __launch_bounds__(kBlock) void k_count_inadm(StepArgs a, int64_t nblk
         atomicAdd(&a.counters[1], (unsigned long long)__popcll(ball));
 }
 
-// _cfl_weights_body! (ParticleGrids.jl:439-450) and the minimum over particles (positive doubles order as
-// their bit patterns, so the minimum is an integer atomic)
+// max_eigenvalue of every particle once per CFL estimate (_get_Lambda, ParticleGrids.jl:294)
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __restrict__ U, const double* __restrict__ dw,
-                                                 int64_t S_sell, int64_t nblk) {
+__global__ __launch_bounds__(kBlock) void k_lambda(StepArgs a, const double* __restrict__ U, double* __restrict__ lam, int64_t nblk) {
     using P = Pde<D, M, KIND>;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    if (i >= N) return;
+    double f[M], l[D];
+    for (int m = 0; m < M; ++m) f[m] = U[m * N + i];
+    P::maxeig(a.pde, f, l);
+    for (int d = 0; d < D; ++d) lam[d * N + i] = l[d];
+}
+
+// _cfl_weights_body! (ParticleGrids.jl:439-450) and the minimum over particles (positive doubles order as their
+// bit patterns, so the minimum is an integer atomic). The per-slot derivative weights are recomputed from the
+// divergence inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223): the same
+// bits as the cached dweights, without streaming them.
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, int64_t nblk) {
+    constexpr int BD = D + 1;
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
     double dt = INFINITY;
     if (i < N && !a.g.isb[i] && a.g.cnt[i] > 0) {
-        double fi[M], li[D];
-        for (int m = 0; m < M; ++m) fi[m] = U[m * N + i];
-        P::maxeig(a.pde, fi, li);
+        double li[D], X[D][BD];
+        for (int d = 0; d < D; ++d) li[d] = a.lam[d * N + i];
+        for (int r = 0; r < D; ++r) for (int c = 0; c < BD; ++c) X[r][c] = a.invd[(r * BD + c) * N + i];
+        const NodePos<D> pi = load_pos<D>(a.g, i);
         const int cnt = a.g.cnt[i];
         const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
         double sum_c = 0.0;
         for (int k = 0; k < cnt; ++k) {
-            const int64_t s = base + (int64_t)k * kChunk;
-            const int32_t j = a.g.nbr[s];
-            double fj[M], lj[D];
-            for (int m = 0; m < M; ++m) fj[m] = U[m * N + j];
-            P::maxeig(a.pde, fj, lj);
-            double dv = fabs(dw[s]) * jmax(li[0], lj[0]);
-            for (int d = 1; d < D; ++d) dv = dv + fabs(dw[d * S_sell + s]) * jmax(li[d], lj[d]);
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double dd[D];
+            const double d2 = slot_geom<D>(a.g, pi, j, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            double v[BD];
+            v[0] = w * 1.0;
+            for (int d = 0; d < D; ++d) v[1 + d] = w * (dd[d] * invL);
+            double dv = 0.0;
+            for (int r = 0; r < D; ++r) {
+                double acc = X[r][0] * v[0];
+                for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
+                const double term = fabs(acc * invL) * jmax(li[r], a.lam[r * N + j]);
+                dv = (r == 0) ? term : dv + term;
+            }
             sum_c = sum_c + dv;
         }
         if (sum_c > kFloatMin) dt = (1.0 / (double)D) / sum_c;
@@ -990,10 +1120,12 @@ __global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* p, uint8_t v, int64
     int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t < n) p[t] = v;
 }
-__global__ void k_reset_counters(unsigned long long* c, int full) {
+// mode 0: the changed count; 1: everything of a new step / CFL estimate; 2: the dirty-list length
+__global__ void k_reset_counters(unsigned long long* c, int mode) {
     if (threadIdx.x == 0) {
+        if (mode == 2) { c[3] = 0; return; }
         c[0] = 0;
-        if (full) { c[1] = 0; c[2] = 0x7FF0000000000000ull; }
+        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[3] = 0; }
     }
 }
 
@@ -1020,6 +1152,9 @@ struct hcc_stepper {
     int8_t* orders = nullptr;
     uint8_t *changed = nullptr, *dirty = nullptr, *trig = nullptr, *inadm = nullptr;
     unsigned long long* counters = nullptr;
+    int32_t *clist = nullptr, *dlist = nullptr;
+    unsigned* dmark = nullptr;
+    double* lam = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_rhs_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
@@ -1041,6 +1176,7 @@ StepArgs hcc_stepper::args() const {
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.grads = grads; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
     a.counters = counters;
+    a.clist = clist; a.dlist = dlist; a.dmark = dmark; a.lam = lam;
     return a;
 }
 
@@ -1053,7 +1189,11 @@ struct Variant {
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full) {
         int64_t nb = nblocks(s->N);
-        k_flux<D, M, KIND><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, Y, K, full, nb);
+        k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, Y, K, full, 0.0, nb);
+    }
+    static void flux_check(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, double h) {
+        int64_t nb = nblocks(s->N);
+        k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, Y, K, full, h, nb);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* Kp, const double* cand, double h, int full) {
         int64_t nb = nblocks(s->N);
@@ -1061,7 +1201,14 @@ struct Variant {
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a) {
         int64_t nb = nblocks(s->N);
-        k_cfl<D, M, KIND><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->U, s->dw, s->grid->S_sell, nb);
+        k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->U, s->lam, nb);
+        k_cfl<D, M, KIND><<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, nb);
+    }
+    static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y, int64_t n) {
+        if (n > 0) k_gather_list<D, M, KIND><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s->ctx->stream>>>(a, Y, s->clist, n);
+    }
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int64_t bound, double h) {
+        if (bound > 0) k_flux_list<D, M, KIND><<<(unsigned)((bound + kBlock - 1) / kBlock), kBlock, 0, s->ctx->stream>>>(a, Y, K, h);
     }
     static void predict(const hcc_stepper* s, const double* const* Ks, const double* c, int nterm, unsigned use, double* target) {
         int64_t nb = nblocks(s->N);
@@ -1073,14 +1220,17 @@ struct Variant {
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const double*, double*, int);
+    void (*flux_check)(const hcc_stepper*, const StepArgs&, const double*, double*, int, double);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, const double*, double, int);
     void (*cfl)(const hcc_stepper*, const StepArgs&);
+    void (*gather_list)(const hcc_stepper*, const StepArgs&, const double*, int64_t);
+    void (*flux_list)(const hcc_stepper*, const StepArgs&, const double*, double*, int64_t, double);
     void (*predict)(const hcc_stepper*, const double* const*, const double*, int, unsigned, double*);
 };
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::flux, &V::flag, &V::cfl, &V::predict};
+    return VariantOps{&V::gather, &V::flux, &V::flux_check, &V::flag, &V::cfl, &V::gather_list, &V::flux_list, &V::predict};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     if (kind == HCC_PDE_LINADV) {
@@ -1104,6 +1254,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
     (void)hipFree(s->grads); (void)hipFree(s->orders); (void)hipFree(s->changed); (void)hipFree(s->dirty); (void)hipFree(s->trig); (void)hipFree(s->inadm);
     (void)hipFree(s->counters);
+    (void)hipFree(s->clist); (void)hipFree(s->dlist); (void)hipFree(s->dmark); (void)hipFree(s->lam);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
@@ -1152,6 +1303,11 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->trig, N));
     HCC_TRY_HIP(ctx, dalloc(&s->inadm, N));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
+    HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dmark, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), ctx->stream));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), ctx->stream));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), ctx->stream));
@@ -1271,31 +1427,55 @@ struct StepRunner {
         n = ctx->h_pinned[0];
         return e;
     }
+    // MOOD checks the Euler substep with a criterion that reads neighbour states: the check is fused into the flux
+    // pass that produces K (flux_node with CHECK).
+    bool fused() const { return substeps(); }
+
     // mood_checkpoint! (TimeSteppers.jl:454-477). The candidate Y_i is formed once the fixed point is reached:
     // on the substep path the check reads only Y_{i-1} and K_{i-1}, so re-forming it every iteration changes nothing.
+    // Fused path: the first (full) check already ran inside stage i-1's flux pass; each recompute of K_{i-1} on the
+    // dirty set re-checks exactly that set.
     hipError_t checkpoint(int i, double dt) {
-        const bool use_sub = substeps();
         if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt); return hipSuccess; }
         const double* Yp = s->Y[i - 2];
         double* Kp = s->K[i - 2];
         const double h = s->desc.substep_ratio * dt;
+        if (fused()) {
+            // the changed particles of the last check are listed in clist (counters[0] of them)
+            for (;;) {
+                unsigned long long n = 0;
+                hipError_t e = read_changed(n);
+                if (e != hipSuccess) return e;
+                if (n == 0) break;
+                ++iters;
+                hipStream_t str = s->ctx->stream;
+                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 2);               // dlist length = 0
+                k_mark<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, str>>>(a, (int64_t)n);
+                ops.gather_list(s, a, Yp, (int64_t)n);
+                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);               // clist restarts
+                const int64_t bound = std::min<int64_t>(s->N, (int64_t)n * (1 + s->grid->max_nb));
+                ops.flux_list(s, a, Yp, Kp, bound, h);
+            }
+            k_count_inadm<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, nb);
+            predict(i, dt);
+            return hipGetLastError();
+        }
         int full = 1;
         for (;;) {
-            if (!use_sub) predict(i, dt);
+            predict(i, dt);
             k_reset_counters<<<1, 64, 0, s->ctx->stream>>>(s->counters, 0);
-            ops.flag(s, a, Yp, Kp, use_sub ? nullptr : (i > s->desc.stages ? s->cand : s->Y[i - 1]), h, full);
+            ops.flag(s, a, Yp, Kp, i > s->desc.stages ? s->cand : s->Y[i - 1], h, full);
             unsigned long long n = 0;
             hipError_t e = read_changed(n);
             if (e != hipSuccess) return e;
             if (n == 0) break;
             ++iters;
-            k_expand<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, nb);
+            k_expand<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, 0, nb);
             ops.gather(s, a, Yp, 0);
             ops.flux(s, a, Yp, Kp, 0);
             full = 0;
         }
         k_count_inadm<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, nb);
-        if (use_sub) predict(i, dt);
         return hipGetLastError();
     }
     hipError_t run(double dt) {
@@ -1315,8 +1495,10 @@ struct StepRunner {
             (void)hipEventRecord(s->evp[4 * (i - 1) + 0], str);
             ops.gather(s, a, s->Y[i - 1], 1);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 1], str);
+            if (fused()) k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 2], str);
-            ops.flux(s, a, s->Y[i - 1], s->K[i - 1], 1);
+            if (fused()) ops.flux_check(s, a, s->Y[i - 1], s->K[i - 1], 1, s->desc.substep_ratio * dt);
+            else ops.flux(s, a, s->Y[i - 1], s->K[i - 1], 1);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1, dt);
