@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhcc.so")
-SOURCES = ["hcc_runtime.hip", "hcc_host.cpp"]
+SOURCES = ["hcc_runtime.hip", "hcc_step.hip", "hcc_host.cpp"]
 ARCH = os.environ.get("HCC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -20,7 +20,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose: bool = False, jobs: int = 2) -> str:
+def build(verbose: bool = False, jobs: int = 3) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs, procs = [], []
     for src in SOURCES:

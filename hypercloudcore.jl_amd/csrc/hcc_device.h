@@ -1,0 +1,198 @@
+// Device-side building blocks shared by the kernels of libhcc: launch geometry, the per-slot geometry of the
+// compute kernels, the PDE variants, error helpers.
+#pragma once
+#include "hcc_internal.h"
+#include <string>
+
+namespace hcc {
+
+// ============================================================================================================
+// errors
+// ============================================================================================================
+extern thread_local std::string g_create_err;
+
+#define HCC_TRY_HIP(ctx, expr)                                                                 \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+            return HCC_E_HIP;                                                                  \
+        }                                                                                      \
+    } while (0)
+
+inline int set_err(hcc_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg; else g_create_err = msg;
+    return code;
+}
+
+template <class T>
+inline hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+// ============================================================================================================
+// launch geometry: 256-thread workgroups, XCD-aware block order
+// ============================================================================================================
+constexpr int kBlock = 256;
+
+// The dispatcher hands workgroup b to XCD b % 8. Remap so that XCD x runs one contiguous range of logical
+// blocks: Morton-ordered neighbours then share that XCD's L2.
+__device__ __forceinline__ int64_t node_index(int64_t nblk) {
+    const int64_t b = blockIdx.x, per = (nblk + 7) / 8;
+    const int64_t lb = (b & 7) * per + (b >> 3);
+    return lb * kBlock + threadIdx.x;
+}
+inline int64_t nblocks(int64_t N) { return (N + kBlock - 1) / kBlock; }
+inline unsigned grid_for(int64_t nblk) { return (unsigned)(((nblk + 7) / 8) * 8); }
+
+// ============================================================================================================
+// per-slot geometry inside the compute kernels: distance and weight from the positions, exactly as the search
+// ============================================================================================================
+template <int D>
+struct NodePos {
+    double x[D];
+};
+template <int D>
+__device__ __forceinline__ NodePos<D> load_pos(const GeomArgs& g, int64_t i) {
+    NodePos<D> p;
+    p.x[0] = g.px[i];
+    if (D > 1) p.x[1] = g.py[i];
+    if (D > 2) p.x[2] = g.pz[i];
+    return p;
+}
+template <int D>
+__device__ __forceinline__ double slot_geom(const GeomArgs& g, const NodePos<D>& pi, int32_t j, double (&dd)[D]) {
+    NodePos<D> pj = load_pos<D>(g, j);
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+        double x = pj.x[d] - pi.x[d];
+        x = x - g.Lw[d] * rint(x * g.invLw[d]);
+        dd[d] = x;
+        s = (d == 0) ? x * x : s + x * x;
+    }
+    return s;
+}
+__device__ __forceinline__ int64_t sell_base(const GeomArgs& g, int64_t i) { return g.chunk_off[i >> 6] + (i & 63); }
+
+// ============================================================================================================
+// PDE kernels variants: KIND 1 = linear advection (diagonal), KIND 2 = Euler (M = D + 2)
+// ============================================================================================================
+template <int D, int M, int KIND>
+struct Pde {
+    __device__ static __forceinline__ void flux(const PdeArgs& e, const double (&U)[M], double (&F)[D][M]) {
+        if constexpr (KIND == HCC_PDE_EULER) {
+            EulerPrim<D> q = euler_prim<D>(e.gamma, U);
+            for (int d = 0; d < D; ++d) {
+                F[d][0] = U[1 + d];
+                for (int k = 0; k < D; ++k) F[d][1 + k] = (k == d) ? U[1 + d] * q.u[k] + q.p : U[1 + d] * q.u[k];
+                F[d][D + 1] = (U[D + 1] + q.p) * q.u[d];
+            }
+        } else {
+            for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m) F[d][m] = e.c[d][m] * U[m];
+        }
+    }
+    __device__ static __forceinline__ void maxeig(const PdeArgs& e, const double (&U)[M], double (&lam)[D]) {
+        if constexpr (KIND == HCC_PDE_EULER) {
+            EulerPrim<D> q = euler_prim<D>(e.gamma, U);
+            double cs = sqrt((e.gamma * q.p) * q.inv);
+            for (int d = 0; d < D; ++d) lam[d] = fabs(q.u[d]) + cs;
+        } else {
+            for (int d = 0; d < D; ++d) lam[d] = e.amax[d];
+        }
+    }
+    // flux and eigenvalues of one state sharing the primitive variables
+    __device__ static __forceinline__ void flux_eig(const PdeArgs& e, const double (&U)[M], double (&F)[D][M], double (&lam)[D]) {
+        if constexpr (KIND == HCC_PDE_EULER) {
+            EulerPrim<D> q = euler_prim<D>(e.gamma, U);
+            for (int d = 0; d < D; ++d) {
+                F[d][0] = U[1 + d];
+                for (int k = 0; k < D; ++k) F[d][1 + k] = (k == d) ? U[1 + d] * q.u[k] + q.p : U[1 + d] * q.u[k];
+                F[d][D + 1] = (U[D + 1] + q.p) * q.u[d];
+            }
+            double cs = sqrt((e.gamma * q.p) * q.inv);
+            for (int d = 0; d < D; ++d) lam[d] = fabs(q.u[d]) + cs;
+        } else {
+            flux(e, U, F);
+            maxeig(e, U, lam);
+        }
+    }
+    // is_admissible: finite, and for Euler rho > 0 and p > 0
+    __device__ static __forceinline__ bool admissible(const PdeArgs& e, const double (&U)[M]) {
+        for (int m = 0; m < M; ++m) if (!isfinite(U[m])) return false;
+        if constexpr (KIND == HCC_PDE_EULER) {
+            EulerPrim<D> q = euler_prim<D>(e.gamma, U);
+            return U[0] > 0.0 && q.p > 0.0;
+        }
+        return true;
+    }
+};
+
+// ============================================================================================================
+// geometry caches: prepare_geometry! (src/Interpolation/MUSCL.jl:172-223, Interpolators.jl:517-577)
+// ============================================================================================================
+struct MomentArgs {
+    GeomArgs g;
+    double* invg; uint8_t* okg;
+    double* invd; uint8_t* okd;
+    double* dw; int64_t S_sell;
+};
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_moments(MomentArgs a, int64_t nblk) {
+    constexpr int BG = D, BD = D + 1;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    if (i >= N) return;
+    const NodePos<D> pi = load_pos<D>(a.g, i);
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    const double invL = a.g.invL;
+    double AG[BG][BG], AD[BD][BD];
+    for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) AG[r][c] = 0.0;
+    for (int r = 0; r < BD; ++r) for (int c = 0; c < BD; ++c) AD[r][c] = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+        double dd[D];
+        double d2 = slot_geom<D>(a.g, pi, j, dd);
+        double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+        double p[BD];
+        p[0] = 1.0;
+        for (int d = 0; d < D; ++d) p[1 + d] = dd[d] * invL;
+        for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) AG[r][c] = AG[r][c] + w * (p[1 + r] * p[1 + c]);
+        for (int r = 0; r < BD; ++r) for (int c = 0; c < BD; ++c) AD[r][c] = AD[r][c] + w * (p[r] * p[c]);
+    }
+    {
+        double U[BG][BG], X[BG][BG];
+        bool ok = chol_moment<BG>(AG, U);
+        if (ok) chol_inverse<BG>(U, X);
+        for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) a.invg[(r * BG + c) * N + i] = ok ? X[r][c] : 0.0;
+        a.okg[i] = ok;
+    }
+    double U[BD][BD], X[BD][BD];
+    bool ok = chol_moment<BD>(AD, U);
+    if (ok) chol_inverse<BD>(U, X);
+    for (int r = 0; r < D; ++r) for (int c = 0; c < BD; ++c) a.invd[(r * BD + c) * N + i] = ok ? X[1 + r][c] : 0.0;
+    a.okd[i] = ok;
+    // dweights: (rows 2..D+1 of inv(N)) * (w p~) * invL (MUSCL.jl:211-223)
+    for (int k = 0; k < cnt; ++k) {
+        const int64_t s = base + (int64_t)k * kChunk;
+        if (!ok) { for (int d = 0; d < D; ++d) a.dw[d * a.S_sell + s] = 0.0; continue; }
+        int32_t j = a.g.nbr[s];
+        double dd[D];
+        double d2 = slot_geom<D>(a.g, pi, j, dd);
+        double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+        double v[BD];
+        v[0] = w * 1.0;
+        for (int d = 0; d < D; ++d) v[1 + d] = w * (dd[d] * invL);
+        for (int r = 0; r < D; ++r) {
+            double acc = X[1 + r][0] * v[0];
+            for (int c = 1; c < BD; ++c) acc = acc + X[1 + r][c] * v[c];
+            a.dw[r * a.S_sell + s] = acc * invL;
+        }
+    }
+}
+
+
+}  // namespace hcc

@@ -68,6 +68,17 @@ struct PdeArgs {
     double amax[3];
 };
 
+// Record layout of the step kernels. A state record holds a node's position followed by its state, padded to
+// 2, 4 or 8 doubles; a gradient record the D*M limited gradient coefficients (basis term major), padded likewise.
+// One neighbour then costs a few 16-byte loads from one or two cache lines.
+template <int D, int M>
+struct Layout {
+    static constexpr int NR = D + M;
+    static constexpr int PU = NR <= 2 ? 2 : NR <= 4 ? 4 : 8;
+    static constexpr int NG = D * M;
+    static constexpr int GS = NG <= 2 ? 2 : NG <= 4 ? 4 : NG <= 8 ? 8 : 16;
+};
+
 struct StepArgs {
     GeomArgs g;
     PdeArgs pde;
@@ -77,18 +88,19 @@ struct StepArgs {
     const double* invd;          // divergence inverse moments, derivative rows 1..D: [(r * (D+1) + c) * N + i]
     const uint8_t* okg;
     const uint8_t* okd;
-    double* grads;               // limited gradients: [(a * M + m) * N + i]
+    double* G;                   // gradient records
     int8_t* orders;
     uint8_t* changed;
     uint8_t* dirty;
     uint8_t* trig;               // mood_flags of the step
     uint8_t* inadm;              // last check: candidate not admissible
     unsigned long long* counters;// [0] orders changed in the last check (= clist length), [1] inadmissible accepted,
-                                 // [2] dt bits, [3] dlist length
+                                 // [2] dt bits, [3] dirty chunk-list length
     int32_t* clist;              // particles whose order changed in the last check
-    int32_t* dlist;              // dirty particles of the fixed point (changed plus one hop)
-    unsigned* dmark;             // dirty marks (cleared by the pass that consumes dlist)
-    const double* lam;           // per-particle wave speeds for the CFL estimate, [d * N + i]
+    int32_t* chlist;             // chunks (64 particles) holding dirty particles
+    uint8_t* dmark;              // dirty particles of the fixed point (changed plus one hop)
+    unsigned* cmark;             // dirty chunks
+    double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
 };
 
 }  // namespace hcc
@@ -117,3 +129,7 @@ struct hcc_grid {
     std::vector<uint8_t> h_isb;
     hcc::GeomArgs args() const;
 };
+
+namespace hcc {
+int download_lists(hcc_grid* g, std::vector<int64_t>& rstart, std::vector<int32_t>& idx, std::vector<int64_t>* sell_of_slot);
+}

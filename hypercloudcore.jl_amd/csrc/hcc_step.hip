@@ -1,0 +1,1173 @@
+// The explicit time step on the device: stage combine + outflow, gather (gradient + limiter), flux (reconstruction,
+// Rusanov, divergence) with the fused MOOD check, the MOOD fixed point on compacted work lists, the CFL estimate,
+// and the stepper object of the C ABI.
+//
+// Layout. Every stage state lives in node records (position then state, padded to 2/4/8 doubles; Layout<D, M>) and
+// the limited gradients in gradient records, so a neighbour is read with a few 16-byte loads from one or two cache
+// lines. K_E is stored node-major (M doubles per node). One thread per node walks its SELL-64 neighbour slots in
+// slot order; the loads of slot k+1 are issued before slot k is computed (index two slots ahead), so the gather
+// latency overlaps the arithmetic of the previous slot. Every per-node sum keeps the reference's operation order.
+#include "hcc_device.h"
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+using namespace hcc;
+
+// ============================================================================================================
+// record access
+// ============================================================================================================
+template <int NREAD, int STRIDE>
+__device__ __forceinline__ void load_rec(const double* __restrict__ base, int64_t i, double (&r)[NREAD]) {
+    const double2* p = reinterpret_cast<const double2*>(base + i * STRIDE);
+#pragma unroll
+    for (int q = 0; q < (NREAD + 1) / 2; ++q) {
+        const double2 v = p[q];
+        r[2 * q] = v.x;
+        if (2 * q + 1 < NREAD) r[2 * q + 1] = v.y;
+    }
+}
+template <int NW, int STRIDE>
+__device__ __forceinline__ void store_rec(double* __restrict__ base, int64_t i, const double (&r)[NW]) {
+    double2* p = reinterpret_cast<double2*>(base + i * STRIDE);
+#pragma unroll
+    for (int q = 0; q < STRIDE / 2; ++q) {
+        double2 v;
+        v.x = (2 * q < NW) ? r[2 * q] : 0.0;
+        v.y = (2 * q + 1 < NW) ? r[2 * q + 1] : 0.0;
+        p[q] = v;
+    }
+}
+
+// get_distance (ParticleGrids.jl:209-219) between two record positions and |d|^2, the operations of the search
+template <int D>
+__device__ __forceinline__ double rec_geom(const GeomArgs& g, const double* pi, const double* pj, double (&dd)[D]) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double x = pj[d] - pi[d];
+        x = x - g.Lw[d] * rint(x * g.invLw[d]);
+        dd[d] = x;
+        s = (d == 0) ? x * x : s + x * x;
+    }
+    return s;
+}
+
+__device__ __forceinline__ int32_t slot_index(const GeomArgs& g, int64_t base, int k, int cnt) {
+    const int kk = k < cnt ? k : cnt - 1;                    // clamped: a redundant load past the last slot
+    return g.nbr[base + (int64_t)kk * kChunk];
+}
+
+// ============================================================================================================
+// stage combine + outflow (_stage_predictor!, TimeSteppers.jl:400-421; OutflowBC, BoundaryConditions.jl:82-131)
+// ============================================================================================================
+// A boundary particle with an outflow donor takes the stage value of its interior root (the same combination,
+// bit-identical); other boundary particles keep pg.rhos (= U^n during the step). Records keep each node's position.
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_predict(int64_t N, int64_t nblk, const uint8_t* __restrict__ isb,
+                                                     const int32_t* __restrict__ root, const double* __restrict__ Un,
+                                                     const double* K0, const double* K1, const double* K2, const double* K3,
+                                                     const double* K4, double c0, double c1, double c2, double c3, double c4,
+                                                     int nterm, unsigned use, double* __restrict__ target) {
+    using L = Layout<D, M>;
+    const int64_t i = node_index(nblk);
+    if (i >= N) return;
+    double r[L::PU];
+    load_rec<L::PU, L::PU>(Un, i, r);
+    const int64_t src = isb[i] ? (int64_t)root[i] : i;
+    if (src >= 0) {
+        double s[L::NR];
+        load_rec<L::NR, L::PU>(Un, src, s);
+        const double* Ks[5] = {K0, K1, K2, K3, K4};
+        const double cs[5] = {c0, c1, c2, c3, c4};
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double y = s[D + m];
+            for (int j = 0; j < nterm; ++j)
+                if ((use >> j) & 1u) y = y + cs[j] * Ks[j][src * M + m];
+            r[D + m] = y;
+        }
+    }
+    store_rec<L::PU, L::PU>(target, i, r);
+}
+
+// ============================================================================================================
+// gather pass: update_content! (MUSCL.jl:351-384), cached reconstruction fit (Interpolators.jl:585-602),
+// Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143)
+// ============================================================================================================
+template <int D, int M, int KIND>
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+    using L = Layout<D, M>;
+    constexpr int NR = L::NR;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    int order = a.orders[i];
+    if (order > 1 && cnt < D) { order = 1; a.orders[i] = 1; }   // starved neighbourhood (basis_count(D, 1) = D)
+    double g[L::NG];
+#pragma unroll
+    for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
+    if (order == 2 && a.okg[i]) {
+        double ri[NR];
+        load_rec<NR, L::PU>(Y, i, ri);
+        const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
+        double umax[M], umin[M], b[D][M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; }
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) b[q][m] = 0.0;
+        int32_t j1 = slot_index(a.g, base, 1, cnt);
+        double rn[NR];
+        load_rec<NR, L::PU>(Y, slot_index(a.g, base, 0, cnt), rn);
+        for (int k = 0; k < cnt; ++k) {
+            double rj[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) rj[q] = rn[q];
+            const int32_t jn = j1;
+            j1 = slot_index(a.g, base, k + 2, cnt);
+            load_rec<NR, L::PU>(Y, jn, rn);
+            double dd[D];
+            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            double p[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) p[d] = dd[d] * invL;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double fj = rj[D + m];
+                const double df = fj - ri[D + m];
+#pragma unroll
+                for (int q = 0; q < D; ++q) b[q][m] = b[q][m] + w * (p[q] * df);
+                umax[m] = jmax(umax[m], fj);
+                umin[m] = jmin(umin[m], fj);
+            }
+        }
+        double inv[D][D];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) inv[r][c] = a.invg[(r * D + c) * N + i];
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double c = inv[q][0] * b[0][m];
+#pragma unroll
+                for (int r = 1; r < D; ++r) c = c + inv[q][r] * b[r][m];
+                g[q * M + m] = c * invL;
+            }
+        if (a.limiter != HCC_LIM_NONE) {
+            double phi[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) phi[m] = 1.0;
+            int32_t k1 = slot_index(a.g, base, 1, cnt);
+            double pn[D];
+            load_rec<D, L::PU>(Y, slot_index(a.g, base, 0, cnt), pn);
+            for (int k = 0; k < cnt; ++k) {
+                double pj[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) pj[d] = pn[d];
+                const int32_t jn = k1;
+                k1 = slot_index(a.g, base, k + 2, cnt);
+                load_rec<D, L::PU>(Y, jn, pn);
+                double dd[D];
+                rec_geom<D>(a.g, ri, pj, dd);
+                double pint[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) pint[d] = 0.5 * dd[d];
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    double rec = 0.0;
+#pragma unroll
+                    for (int q = 0; q < D; ++q) rec = rec + g[q * M + m] * pint[q];
+                    const double bound = rec > 0.0 ? umax[m] : umin[m];
+                    const double tol = 16 * kEps * jmax(fabs(ri[D + m]), fabs(bound));
+                    if (fabs(rec) > tol) {
+                        const double r = (bound - ri[D + m]) / rec;
+                        phi[m] = jmin(phi[m], a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r));
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < D; ++q)
+#pragma unroll
+                for (int m = 0; m < M; ++m) g[q * M + m] = g[q * M + m] * phi[m];
+        }
+    }
+    store_rec<L::NG, L::GS>(a.G, i, g);
+}
+
+// The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    if (a.g.isb[i] || !(full || a.changed[i])) return;
+    gather_node<D, M, KIND>(a, Y, i);
+}
+
+// The gather pass over the particles whose order changed in the last MOOD check (clist).
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double* __restrict__ Y, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = a.clist[t];
+    if (a.g.isb[i]) return;
+    gather_node<D, M, KIND>(a, Y, i);
+}
+
+// ============================================================================================================
+// flux pass: MUSCL functor + _flux_loop (MUSCL.jl:390-508), Rusanov (FluxFunctions.jl:16-76), sort_flux
+// (Utils.jl:95-103), cached divergence fit (apply_weights_divergence, Interpolators.jl:610-626), K_E = -2 div
+// (TimeSteppers.jl:354).
+//
+// With CHECK it also runs MOOD's candidate check on the node it just updated (_mood_flag!, MOOD.jl:289-295): the
+// forward Euler substep W = Y + h K is node-local (form_substep!, TimeSteppers.jl:501-518), and the DMP bounds of
+// MOODu1 (_dmp_fails, MOOD.jl:175-184) are the extrema of Y over the same neighbour loop. The nodes checked are
+// exactly those whose K is (re)computed (all interior nodes, or the dirty ones), as in detect_and_drop!. The
+// decrement (_mood_drop!) is applied by k_mark: neighbours read this node's order in this very pass.
+// ============================================================================================================
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, double* __restrict__ K,
+                                          int64_t i, double h) {
+    using P = Pde<D, M, KIND>;
+    using L = Layout<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, BD = D + 1;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    double ri[NR];
+    load_rec<NR, L::PU>(Y, i, ri);
+    double fi[M], mn[M], mx[M], Kout[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) { fi[m] = ri[D + m]; mn[m] = fi[m]; mx[m] = fi[m]; }
+    if (cnt == 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
+    } else {
+        const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
+        double gi[NG], Fi[D][M];
+        load_rec<NG, L::GS>(a.G, i, gi);
+        P::flux(a.pde, fi, Fi);
+        const int oi = a.orders[i];
+        double b[BD][D * M];
+#pragma unroll
+        for (int c = 0; c < BD; ++c)
+#pragma unroll
+            for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
+        int32_t j1 = slot_index(a.g, base, 1, cnt);
+        const int32_t j0 = slot_index(a.g, base, 0, cnt);
+        double rn[NR], gn[NG];
+        load_rec<NR, L::PU>(Y, j0, rn);
+        load_rec<NG, L::GS>(a.G, j0, gn);
+        int on = a.orders[j0];
+        for (int k = 0; k < cnt; ++k) {
+            double rj[NR], gj[NG];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) rj[q] = rn[q];
+#pragma unroll
+            for (int q = 0; q < NG; ++q) gj[q] = gn[q];
+            const int oj = on;
+            const int32_t jn = j1;
+            j1 = slot_index(a.g, base, k + 2, cnt);
+            load_rec<NR, L::PU>(Y, jn, rn);
+            load_rec<NG, L::GS>(a.G, jn, gn);
+            on = a.orders[jn];
+
+            double dd[D];
+            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            int o_i = oi, o_j = oj;
+            if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
+            double pii[D], pjj[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                pii[d] = (o_i == 2) ? 0.5 * dd[d] : 0.0;
+                pjj[d] = (o_j == 2) ? -0.5 * dd[d] : 0.0;
+            }
+            double fij[M], fji[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double si = gi[m] * pii[0];
+#pragma unroll
+                for (int q = 1; q < D; ++q) si = si + gi[q * M + m] * pii[q];
+                double sj = gj[m] * pjj[0];
+#pragma unroll
+                for (int q = 1; q < D; ++q) sj = sj + gj[q * M + m] * pjj[q];
+                const double uj = rj[D + m];
+                if (CHECK) { mn[m] = jmin(mn[m], uj); mx[m] = jmax(mx[m], uj); }
+                fij[m] = fi[m] + si;
+                fji[m] = uj + sj;
+            }
+            double Fij[D][M], Fji[D][M], lij[D], lji[D];
+            P::flux_eig(a.pde, fij, Fij, lij);
+            P::flux_eig(a.pde, fji, Fji, lji);
+            double pt[BD];
+            pt[0] = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                // sort_flux orders the two sides by sign(d); F_L + F_R is the same sum either way (IEEE addition
+                // commutes), only the dissipation f_R - f_L needs the sorted states
+                const bool right = dd[d] > 0.0;
+                const double sd = right ? jmax(lij[d], lji[d]) : jmax(lji[d], lij[d]);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const double fL = right ? fij[m] : fji[m], fR = right ? fji[m] : fij[m];
+                    const double diss = sd * (fR - fL);
+                    const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
+                    const double dF = (Fnum - Fi[d][m]) + 0.0;
+#pragma unroll
+                    for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
+                }
+            }
+        }
+        if (a.okd[i]) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
+#pragma unroll
+                    for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
+                    acc = acc + r;
+                }
+                Kout[m] = -(2.0 * (acc * invL)) + 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
+    if (!CHECK) return false;
+    double W[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
+    const bool adm = P::admissible(a.pde, W);
+    a.inadm[i] = !adm;
+    bool flag = !adm;
+    if (!flag) {
+        if (a.criterion == HCC_CRIT_ONLY) {
+            flag = true;
+        } else if (a.criterion == HCC_CRIT_U1) {
+            const double dl = a.delta, d3 = dl * dl * dl;
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
+        }
+    }
+    if (flag && a.orders[i] > 1) { a.trig[i] = 1; return true; }
+    return false;
+}
+
+// Appends the flagged particles of a wavefront to `clist` (one atomic per wavefront) and counts them.
+__device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64_t i) {
+    const unsigned long long ball = __ballot(ch);
+    if (!ball) return;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
+    base = __shfl(base, (int)leader);
+    if (ch) {
+        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        a.clist[base + __popcll(below)] = (int32_t)i;
+    }
+}
+
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, const double* __restrict__ Y, double* __restrict__ K, int full,
+                                                  double h, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    bool ch = false;
+    if (i < N) {
+        if (a.g.isb[i]) {
+            if (full)
+#pragma unroll
+                for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+        } else if (full || a.dirty[i]) {
+            ch = flux_node<D, M, KIND, CHECK>(a, Y, K, i, h);
+        }
+    }
+    if (CHECK) append_changed(a, ch, i);
+}
+
+// The flux pass (with the MOOD check) over the dirty chunks of the fixed point: one wavefront per chunk of 64
+// consecutive (Morton-ordered) particles, the marked ones compute; marks are cleared for the next iteration.
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_flux_chunks(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
+                                                         double h) {
+    const int64_t w = (int64_t)blockIdx.x * (kBlock / kChunk) + (threadIdx.x >> 6);
+    const int64_t nch = (int64_t)a.counters[3];
+    bool ch = false;
+    int64_t i = 0;
+    if (w < nch) {
+        const int64_t c = a.chlist[w];
+        i = c * kChunk + (threadIdx.x & 63);
+        if ((threadIdx.x & 63) == 0) a.cmark[c] = 0u;
+        if (i < a.g.N && a.dmark[i]) {
+            a.dmark[i] = 0;
+            if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, K, i, h);
+        }
+    }
+    append_changed(a, ch, i);
+}
+
+// _mood_drop! (MOOD.jl:299-309) and one hop of expand_mask! (MOOD.jl:323-351) for the changed list: the particle
+// drops one order, it and its neighbours are marked dirty, and each chunk holding a marked particle is listed once.
+// The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
+__global__ __launch_bounds__(kBlock) void k_mark(StepArgs a, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = a.clist[t];
+    a.orders[i] = (int8_t)(a.orders[i] - 1);
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    for (int k = -1; k < cnt; ++k) {
+        const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
+        a.dmark[j] = 1;
+        const int32_t c = j >> 6;
+        if (a.cmark[c] == 0u && atomicExch(&a.cmark[c], 1u) == 0u)
+            a.chlist[atomicAdd(&a.counters[3], 1ull)] = c;
+    }
+}
+
+// ============================================================================================================
+// the MOOD path that checks a stored candidate (no SSP substep): _mood_flag! + _mood_drop! (MOOD.jl:289-309) on
+// all interior particles or the dirty mask, then expand_mask! (MOOD.jl:323-351)
+// ============================================================================================================
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* __restrict__ Yp, const double* __restrict__ cand,
+                                                       int full, int64_t nblk) {
+    using P = Pde<D, M, KIND>;
+    using L = Layout<D, M>;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    bool ch = false;
+    if (i < N && !a.g.isb[i] && (full || a.dirty[i])) {
+        double c[L::NR], W[M];
+        load_rec<L::NR, L::PU>(cand, i, c);
+#pragma unroll
+        for (int m = 0; m < M; ++m) W[m] = c[D + m];
+        const bool adm = P::admissible(a.pde, W);
+        a.inadm[i] = !adm;
+        bool flag = !adm;
+        if (!flag) {
+            if (a.criterion == HCC_CRIT_ONLY) {
+                flag = true;
+            } else if (a.criterion == HCC_CRIT_U1) {
+                double r[L::NR], mn[M], mx[M];
+                load_rec<L::NR, L::PU>(Yp, i, r);
+#pragma unroll
+                for (int m = 0; m < M; ++m) { mn[m] = r[D + m]; mx[m] = r[D + m]; }
+                const int cnt = a.g.cnt[i];
+                const int64_t base = sell_base(a.g, i);
+                for (int k = 0; k < cnt; ++k) {
+                    load_rec<L::NR, L::PU>(Yp, a.g.nbr[base + (int64_t)k * kChunk], r);
+#pragma unroll
+                    for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], r[D + m]); mx[m] = jmax(mx[m], r[D + m]); }
+                }
+                const double dl = a.delta, d3 = dl * dl * dl;
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
+            }
+        }
+        if (flag) {
+            const int o = a.orders[i];
+            if (o > 1) { a.orders[i] = (int8_t)(o - 1); a.trig[i] = 1; ch = true; }
+        }
+    }
+    if (i < N) a.changed[i] = ch;
+    const unsigned long long ball = __ballot(ch);
+    if (ball && ((threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1)))
+        atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
+}
+
+__global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    uint8_t m = a.changed[i];
+    if (!m) {
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        for (int k = 0; k < cnt; ++k)
+            if (a.changed[a.g.nbr[base + (int64_t)k * kChunk]]) { m = 1; break; }
+    }
+    a.dirty[i] = m;
+}
+
+// _count_inadmissible (TimeSteppers.jl:481-490): interior candidates accepted while not admissible
+__global__ __launch_bounds__(kBlock) void k_count_inadm(StepArgs a, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    const bool bad = i < a.g.N && !a.g.isb[i] && a.inadm[i];
+    const unsigned long long ball = __ballot(bad);
+    if (ball && ((threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1)))
+        atomicAdd(&a.counters[1], (unsigned long long)__popcll(ball));
+}
+
+// ============================================================================================================
+// CFL estimate: get_time_step with cached weights (ParticleGrids.jl:386-450)
+// ============================================================================================================
+// max_eigenvalue of every particle once per estimate (_get_Lambda, ParticleGrids.jl:294)
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_lambda(StepArgs a, const double* __restrict__ U, int64_t nblk) {
+    using P = Pde<D, M, KIND>;
+    using L = Layout<D, M>;
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    double r[L::NR], f[M], l[D];
+    load_rec<L::NR, L::PU>(U, i, r);
+#pragma unroll
+    for (int m = 0; m < M; ++m) f[m] = r[D + m];
+    P::maxeig(a.pde, f, l);
+#pragma unroll
+    for (int d = 0; d < D; ++d) a.lam[i * D + d] = l[d];
+}
+
+// _cfl_weights_body! (ParticleGrids.jl:439-450) and the minimum over particles (positive doubles order as their bit
+// patterns, so the minimum is an integer atomic). The per-slot derivative weights are recomputed from the divergence
+// inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223): the same bits as the
+// cached dweights, without streaming them.
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __restrict__ U, int64_t nblk) {
+    using L = Layout<D, M>;
+    constexpr int BD = D + 1;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    double dt = INFINITY;
+    if (i < N && !a.g.isb[i] && a.g.cnt[i] > 0) {
+        double li[D], X[D][BD], pi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) li[d] = a.lam[i * D + d];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < BD; ++c) X[r][c] = a.invd[(r * BD + c) * N + i];
+        load_rec<D, L::PU>(U, i, pi);
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
+        double sum_c = 0.0;
+        int32_t j1 = slot_index(a.g, base, 1, cnt);
+        const int32_t j0 = slot_index(a.g, base, 0, cnt);
+        double pn[D], ln[D];
+        load_rec<D, L::PU>(U, j0, pn);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ln[d] = a.lam[(int64_t)j0 * D + d];
+        for (int k = 0; k < cnt; ++k) {
+            double pj[D], lj[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) { pj[d] = pn[d]; lj[d] = ln[d]; }
+            const int32_t jn = j1;
+            j1 = slot_index(a.g, base, k + 2, cnt);
+            load_rec<D, L::PU>(U, jn, pn);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ln[d] = a.lam[(int64_t)jn * D + d];
+            double dd[D];
+            const double d2 = rec_geom<D>(a.g, pi, pj, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            double v[BD];
+            v[0] = w * 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) v[1 + d] = w * (dd[d] * invL);
+            double dv = 0.0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                double acc = X[r][0] * v[0];
+#pragma unroll
+                for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
+                const double term = fabs(acc * invL) * jmax(li[r], lj[r]);
+                dv = (r == 0) ? term : dv + term;
+            }
+            sum_c = sum_c + dv;
+        }
+        if (sum_c > kFloatMin) dt = (1.0 / (double)D) / sum_c;
+    }
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0 && dt < INFINITY) atomicMin(&a.counters[2], (unsigned long long)__double_as_longlong(dt));
+}
+
+// ============================================================================================================
+// packing, fills, counters
+// ============================================================================================================
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_pack(int64_t N, const double* __restrict__ px, const double* __restrict__ py,
+                                                  const double* __restrict__ pz, const double* __restrict__ U_aos,
+                                                  double* __restrict__ rec) {
+    using L = Layout<D, M>;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double r[L::NR];
+    r[0] = px[i];
+    if (D > 1) r[1] = py[i];
+    if (D > 2) r[2] = pz[i];
+#pragma unroll
+    for (int m = 0; m < M; ++m) r[D + m] = U_aos[i * M + m];
+    store_rec<L::NR, L::PU>(rec, i, r);
+}
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_unpack(int64_t N, const double* __restrict__ rec, double* __restrict__ U_aos) {
+    using L = Layout<D, M>;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double r[L::NR];
+    load_rec<L::NR, L::PU>(rec, i, r);
+#pragma unroll
+    for (int m = 0; m < M; ++m) U_aos[i * M + m] = r[D + m];
+}
+__global__ __launch_bounds__(kBlock) void k_fill_f64(double* p, double v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+__global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+// mode 0: the changed count; 1: everything of a new step / CFL estimate; 2: the dirty chunk-list length
+__global__ void k_reset_counters(unsigned long long* c, int mode) {
+    if (threadIdx.x == 0) {
+        if (mode == 2) { c[3] = 0; return; }
+        c[0] = 0;
+        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[3] = 0; }
+    }
+}
+
+// ============================================================================================================
+// the stepper
+// ============================================================================================================
+struct hcc_stepper {
+    hcc_ctx* ctx = nullptr;
+    hcc_grid* grid = nullptr;
+    hcc_stepper_desc desc{};
+    int D = 0, M = 0, kind = 0, PU = 0, GS = 0;
+    int64_t N = 0;
+    // geometry caches
+    double* invg = nullptr; uint8_t* okg = nullptr;
+    double* invd = nullptr; uint8_t* okd = nullptr;
+    double* dw = nullptr;
+    int32_t* root = nullptr;
+    // states (records) and right-hand sides (node-major)
+    double* U = nullptr;       // pg.rhos (U^n during a step)
+    double* cand = nullptr;    // rho_cand
+    double* Y[kMaxStages] = {};
+    double* K[kMaxStages] = {};
+    double* G = nullptr;
+    double* scratch = nullptr; // N * M doubles for up/downloads
+    int8_t* orders = nullptr;
+    uint8_t *changed = nullptr, *dirty = nullptr, *trig = nullptr, *inadm = nullptr, *dmark = nullptr;
+    unsigned* cmark = nullptr;
+    int32_t *clist = nullptr, *chlist = nullptr;
+    double* lam = nullptr;
+    unsigned long long* counters = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
+    double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
+    int64_t inadmissible = 0, mood_iters = 0;
+    StepArgs args() const;
+};
+
+StepArgs hcc_stepper::args() const {
+    StepArgs a{};
+    a.g = grid->args();
+    a.pde.gamma = desc.gamma;
+    for (int d = 0; d < 3; ++d) {
+        double mx = 0.0;
+        for (int m = 0; m < 8; ++m) {
+            a.pde.c[d][m] = desc.adv_speed[d][m];
+            if (m < M) mx = std::max(mx, std::fabs(desc.adv_speed[d][m]));
+        }
+        a.pde.amax[d] = mx;
+    }
+    a.limiter = desc.limiter; a.mode = desc.limiter_mode; a.strategy = desc.mood_strategy;
+    a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
+    a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
+    a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
+    a.counters = counters; a.clist = clist; a.chlist = chlist; a.dmark = dmark; a.cmark = cmark; a.lam = lam;
+    return a;
+}
+
+// compiled variants: (D, M, kind)
+template <int D, int M, int KIND>
+struct Variant {
+    static hipStream_t st(const hcc_stepper* s) { return s->ctx->stream; }
+    static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full) {
+        const int64_t nb = nblocks(s->N);
+        k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
+    }
+    static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y, int64_t n) {
+        if (n > 0) k_gather_list<D, M, KIND><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(a, Y, n);
+    }
+    static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, bool check, double h) {
+        const int64_t nb = nblocks(s->N);
+        if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, h, nb);
+        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, 0.0, nb);
+    }
+    static void flux_chunks(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int64_t bound, double h) {
+        if (bound > 0) {
+            const int64_t per = kBlock / kChunk;
+            k_flux_chunks<D, M, KIND><<<(unsigned)((bound + per - 1) / per), kBlock, 0, st(s)>>>(a, Y, K, h);
+        }
+    }
+    static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
+        const int64_t nb = nblocks(s->N);
+        k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, full, nb);
+    }
+    static void cfl(const hcc_stepper* s, const StepArgs& a) {
+        const int64_t nb = nblocks(s->N);
+        k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        k_cfl<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+    }
+    static void predict(const hcc_stepper* s, const double* const* Ks, const double* c, int nterm, unsigned use, double* target) {
+        const int64_t nb = nblocks(s->N);
+        k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(s->N, nb, s->grid->d_isb, s->root, s->U, Ks[0], Ks[1], Ks[2],
+                                                             Ks[3], Ks[4], c[0], c[1], c[2], c[3], c[4], nterm, use, target);
+    }
+    static void pack(const hcc_stepper* s, const double* U_aos, double* rec) {
+        const hcc_grid* g = s->grid;
+        k_pack<D, M><<<(unsigned)((s->N + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(
+            s->N, g->d_pos[0], g->d_pos[1] ? g->d_pos[1] : g->d_pos[0], g->d_pos[2] ? g->d_pos[2] : g->d_pos[0], U_aos, rec);
+    }
+    static void unpack(const hcc_stepper* s, const double* rec, double* U_aos) {
+        k_unpack<D, M><<<(unsigned)((s->N + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(s->N, rec, U_aos);
+    }
+};
+
+struct VariantOps {
+    void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int);
+    void (*gather_list)(const hcc_stepper*, const StepArgs&, const double*, int64_t);
+    void (*flux)(const hcc_stepper*, const StepArgs&, const double*, double*, int, bool, double);
+    void (*flux_chunks)(const hcc_stepper*, const StepArgs&, const double*, double*, int64_t, double);
+    void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
+    void (*cfl)(const hcc_stepper*, const StepArgs&);
+    void (*predict)(const hcc_stepper*, const double* const*, const double*, int, unsigned, double*);
+    void (*pack)(const hcc_stepper*, const double*, double*);
+    void (*unpack)(const hcc_stepper*, const double*, double*);
+    int PU, GS;
+};
+template <int D, int M, int KIND>
+static VariantOps make_ops() {
+    using V = Variant<D, M, KIND>;
+    return VariantOps{&V::gather, &V::gather_list, &V::flux, &V::flux_chunks, &V::flag, &V::cfl, &V::predict,
+                      &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
+}
+static bool find_ops(int D, int M, int kind, VariantOps& ops) {
+    if (kind == HCC_PDE_LINADV) {
+        if (D == 1 && M == 1) { ops = make_ops<1, 1, HCC_PDE_LINADV>(); return true; }
+        if (D == 2 && M == 1) { ops = make_ops<2, 1, HCC_PDE_LINADV>(); return true; }
+        if (D == 2 && M == 2) { ops = make_ops<2, 2, HCC_PDE_LINADV>(); return true; }
+        if (D == 3 && M == 1) { ops = make_ops<3, 1, HCC_PDE_LINADV>(); return true; }
+    } else if (kind == HCC_PDE_EULER) {
+        if (D == 2 && M == 4) { ops = make_ops<2, 4, HCC_PDE_EULER>(); return true; }
+        if (D == 3 && M == 5) { ops = make_ops<3, 5, HCC_PDE_EULER>(); return true; }
+    }
+    return false;
+}
+
+extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
+    if (!s) return HCC_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->U,
+                    (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
+                    (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->cmark, (void*)s->clist, (void*)s->chlist,
+                    (void*)s->lam, (void*)s->counters})
+        (void)hipFree(p);
+    for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
+    delete s;
+    return HCC_OK;
+}
+
+extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_desc* d, hcc_stepper** out) {
+    if (!ctx || !g || !d || !out) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: NULL argument");
+    *out = nullptr;
+    const int D = g->geom.D;
+    VariantOps ops;
+    if (!find_ops(D, d->M, d->pde, ops))
+        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: no compiled kernel for this (D, M, PDE) -- there is no fallback path");
+    if (d->muscl_order != 2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL order 2 is the compiled scheme");
+    if (d->stages < 1 || d->stages > kMaxStages) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: 1..5 stages");
+    if (d->limiter < HCC_LIM_NONE || d->limiter > HCC_LIM_VK) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: unknown limiter");
+    if (d->mood_strategy < HCC_STRAT_NONE || d->mood_strategy > HCC_STRAT_EPD1) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD strategy");
+    if (d->mood_criterion < HCC_CRIT_NONE || d->mood_criterion > HCC_CRIT_ONLY) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD criterion");
+    if (d->n_bc < 0 || d->n_bc > 16) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: at most 16 boundary conditions");
+    for (int i = 0; i < d->stages; ++i)
+        for (int j = i; j < d->stages; ++j)
+            if (d->a_t[i][j] != 0.0) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: a_t must be strictly lower triangular");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<hcc_stepper, int (*)(hcc_stepper*)> s(new hcc_stepper(), hcc_stepper_destroy);
+    s->ctx = ctx; s->grid = g; s->desc = *d; s->D = D; s->M = d->M; s->kind = d->pde; s->N = g->N;
+    s->PU = ops.PU; s->GS = ops.GS;
+    const int64_t N = g->N, M = d->M;
+    hipStream_t st = ctx->stream;
+    HCC_TRY_HIP(ctx, dalloc(&s->invg, (int64_t)D * D * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->okg, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->invd, (int64_t)D * (D + 1) * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->okd, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)D * g->S_sell));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)D * g->S_sell, 1), st));
+    HCC_TRY_HIP(ctx, dalloc(&s->root, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->U, s->PU * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->cand, s->PU * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->scratch, M * N));
+    for (int k = 0; k < d->stages; ++k) {
+        HCC_TRY_HIP(ctx, dalloc(&s->Y[k], s->PU * N));
+        HCC_TRY_HIP(ctx, dalloc(&s->K[k], M * N));
+        HCC_TRY_HIP(ctx, hipMemsetAsync(s->K[k], 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
+    }
+    HCC_TRY_HIP(ctx, dalloc(&s->G, s->GS * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->orders, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->changed, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dirty, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->trig, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->inadm, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dmark, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->cmark, g->nchunks + 1));
+    HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->chlist, g->nchunks + 1));
+    HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
+    for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
+        HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->cmark, 0, sizeof(unsigned) * (g->nchunks + 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 8 * sizeof(unsigned long long), st));
+    HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
+    HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
+    for (auto& e : s->evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
+    if (N > 0) {
+        ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
+        MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
+        const int64_t nb = nblocks(N);
+        if (D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        else if (D == 2) k_moments<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        HCC_TRY_HIP(ctx, hipGetLastError());
+    }
+    {   // outflow donor roots (host, from the device lists and the wrapped positions)
+        std::vector<int64_t> rstart;
+        std::vector<int32_t> idx;
+        HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+        int rc = download_lists(g, rstart, idx, nullptr);
+        if (rc) return rc;
+        std::vector<double> pos((size_t)(N * D));
+        rc = hcc_grid_get_positions(g, pos.data());
+        if (rc) return rc;
+        std::vector<int32_t> root;
+        outflow_roots(g->geom, N, pos.data(), g->h_isb.data(), g->h_tags.data(), rstart, idx, d->n_bc, d->bc_tag,
+                      d->bc_kind, root);
+        for (auto& r : root) if (r == -2) r = 0;   // interior entries are never read
+        if (N) HCC_TRY_HIP(ctx, hipMemcpy(s->root, root.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    }
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+    *out = s.release();
+    return HCC_OK;
+}
+
+extern "C" int hcc_state_upload(hcc_stepper* s, const double* U) {
+    if (!s || (!U && s->N)) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (!s->N) return HCC_OK;
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, U, sizeof(double) * s->N * s->M, hipMemcpyHostToDevice, ctx->stream));
+    ops.pack(s, s->scratch, s->U);
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HCC_OK;
+}
+
+extern "C" int hcc_state_download(hcc_stepper* s, double* U) {
+    if (!s || (!U && s->N)) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (!s->N) return HCC_OK;
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    ops.unpack(s, s->U, s->scratch);
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(U, s->scratch, sizeof(double) * s->N * s->M, hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HCC_OK;
+}
+
+extern "C" int hcc_time_step(hcc_stepper* s, double* dt) {
+    if (!s || !dt) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
+    if (s->N) ops.cfl(s, s->args());
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long bits = ctx->h_pinned[2];
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *dt = v;
+    return HCC_OK;
+}
+
+// host driver of one step (TimeSteppers.jl:531-577)
+namespace {
+struct StepRunner {
+    hcc_stepper* s;
+    VariantOps ops;
+    StepArgs a;
+    int64_t nb = 0;
+    int iters = 0;
+    bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
+    // MOOD checks the Euler substep: the check is fused into the flux pass that produces K (flux_node with CHECK)
+    bool fused() const { return substeps(); }
+
+    // form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state)
+    void predict(int i, double dt) {
+        const int st = s->desc.stages;
+        const bool fin = i > st;
+        const int n_prev = fin ? st : i - 1;
+        const double* Ks[5] = {s->K[0], s->K[0], s->K[0], s->K[0], s->K[0]};
+        double c[5] = {0, 0, 0, 0, 0};
+        unsigned use = 0;
+        for (int j = 0; j < n_prev; ++j) {
+            Ks[j] = s->K[j];
+            const double at = fin ? s->desc.b_t[j] : s->desc.a_t[i - 1][j];
+            c[j] = dt * at;
+            if (at != 0.0) use |= 1u << j;
+        }
+        ops.predict(s, Ks, c, n_prev, use, fin ? s->cand : s->Y[i - 1]);
+    }
+    hipError_t read_changed(unsigned long long& n) {
+        hcc_ctx* ctx = s->ctx;
+        hipError_t e = hipMemcpyAsync(ctx->h_pinned, s->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(ctx->stream);
+        n = ctx->h_pinned[0];
+        return e;
+    }
+    // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
+    // flux pass; every recompute of K_{i-1} on the dirty set re-checks exactly that set. The candidate Y_i is
+    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}.
+    hipError_t checkpoint(int i, double dt) {
+        if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt); return hipSuccess; }
+        const double* Yp = s->Y[i - 2];
+        double* Kp = s->K[i - 2];
+        const double h = s->desc.substep_ratio * dt;
+        hipStream_t str = s->ctx->stream;
+        if (fused()) {
+            for (;;) {
+                unsigned long long n = 0;
+                hipError_t e = read_changed(n);
+                if (e != hipSuccess) return e;
+                if (n == 0) break;
+                ++iters;
+                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 2);
+                k_mark<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, str>>>(a, (int64_t)n);
+                ops.gather_list(s, a, Yp, (int64_t)n);
+                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
+                const int64_t bound = std::min<int64_t>(s->grid->nchunks, (int64_t)n * (1 + s->grid->max_nb));
+                ops.flux_chunks(s, a, Yp, Kp, bound, h);
+            }
+            k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
+            predict(i, dt);
+            return hipGetLastError();
+        }
+        int full = 1;
+        for (;;) {
+            predict(i, dt);
+            k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
+            ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], full);
+            unsigned long long n = 0;
+            hipError_t e = read_changed(n);
+            if (e != hipSuccess) return e;
+            if (n == 0) break;
+            ++iters;
+            k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
+            ops.gather(s, a, Yp, 0);
+            ops.flux(s, a, Yp, Kp, 0, false, 0.0);
+            full = 0;
+        }
+        k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
+        return hipGetLastError();
+    }
+    hipError_t run(double dt) {
+        hcc_ctx* ctx = s->ctx;
+        const int64_t N = s->N;
+        const int st = s->desc.stages;
+        hipStream_t str = ctx->stream;
+        nb = nblocks(N);
+        k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, str>>>(s->G, 0.0, (int64_t)s->GS * N);
+        k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>((uint8_t*)s->orders, 2, N);
+        k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>(s->trig, 0, N);
+        k_reset_counters<<<1, 64, 0, str>>>(s->counters, 1);
+        for (int i = 1; i <= st; ++i) {
+            if (i == 1) predict(1, dt);
+            else { hipError_t e = checkpoint(i, dt); if (e != hipSuccess) return e; }
+            (void)hipEventRecord(s->evp[4 * (i - 1) + 0], str);
+            ops.gather(s, a, s->Y[i - 1], 1);
+            (void)hipEventRecord(s->evp[4 * (i - 1) + 1], str);
+            if (fused()) k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
+            (void)hipEventRecord(s->evp[4 * (i - 1) + 2], str);
+            ops.flux(s, a, s->Y[i - 1], s->K[i - 1], 1, fused(), s->desc.substep_ratio * dt);
+            (void)hipEventRecord(s->evp[4 * (i - 1) + 3], str);
+        }
+        hipError_t e = checkpoint(st + 1, dt);
+        if (e != hipSuccess) return e;
+        std::swap(s->U, s->cand);        // pg.rhos <- rho_cand
+        return hipGetLastError();
+    }
+};
+}  // namespace
+
+extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
+    (void)t;
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (s->N == 0) return HCC_OK;
+    StepRunner r;
+    r.s = s;
+    find_ops(s->D, s->M, s->kind, r.ops);
+    r.a = s->args();
+    HCC_TRY_HIP(ctx, hipEventRecord(s->ev0, ctx->stream));
+    hipError_t e = r.run(dt);
+    if (e != hipSuccess) { ctx->err = std::string("hcc_step: ") + hipGetErrorString(e); return HCC_E_HIP; }
+    HCC_TRY_HIP(ctx, hipEventRecord(s->ev1, ctx->stream));
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->inadmissible = (int64_t)ctx->h_pinned[1];
+    s->mood_iters = r.iters;
+    float ms = 0;
+    HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms;
+    s->last_gather_ms = s->last_flux_ms = 0;
+    for (int i = 0; i < s->desc.stages; ++i) {
+        float gm = 0, fm = 0;
+        HCC_TRY_HIP(ctx, hipEventElapsedTime(&gm, s->evp[4 * i + 0], s->evp[4 * i + 1]));
+        HCC_TRY_HIP(ctx, hipEventElapsedTime(&fm, s->evp[4 * i + 2], s->evp[4 * i + 3]));
+        s->last_gather_ms += gm;
+        s->last_flux_ms += fm;
+    }
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_orders(hcc_stepper* s, int32_t* orders) {
+    if (!s || (!orders && s->N)) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int8_t> o((size_t)s->N);
+    if (s->N) HCC_TRY_HIP(ctx, hipMemcpy(o.data(), s->orders, s->N, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < s->N; ++i) orders[i] = o[i];
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_mood_flags(hcc_stepper* s, uint8_t* flags) {
+    if (!s || (!flags && s->N)) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (s->N) HCC_TRY_HIP(ctx, hipMemcpy(flags, s->trig, s->N, hipMemcpyDeviceToHost));
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_inadmissible(hcc_stepper* s, int64_t* count) {
+    if (!s || !count) return HCC_E_ARG;
+    *count = s->inadmissible;
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_mood_iterations(hcc_stepper* s, int64_t* count) {
+    if (!s || !count) return HCC_E_ARG;
+    *count = s->mood_iters;
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_moments(hcc_stepper* s, double* inv_grad, uint8_t* ok_grad, double* inv_div, uint8_t* ok_div,
+                               double* dweights) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t N = s->N;
+    const int D = s->D, BD = D + 1;
+    if (inv_grad && N) {
+        std::vector<double> v((size_t)(D * D * N));
+        HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->invg, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i) for (int q = 0; q < D * D; ++q) inv_grad[i * D * D + q] = v[(size_t)(q * N + i)];
+    }
+    if (inv_div && N) {   // rows 1..D of the (D+1)^2 inverse; row 0 is not kept on the device (NaN)
+        std::vector<double> v((size_t)(D * BD * N));
+        HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->invd, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i) {
+            for (int c = 0; c < BD; ++c) inv_div[i * BD * BD + c] = NAN;
+            for (int r = 0; r < D; ++r)
+                for (int c = 0; c < BD; ++c) inv_div[i * BD * BD + (1 + r) * BD + c] = v[(size_t)((r * BD + c) * N + i)];
+        }
+    }
+    if (ok_grad && N) HCC_TRY_HIP(ctx, hipMemcpy(ok_grad, s->okg, N, hipMemcpyDeviceToHost));
+    if (ok_div && N) HCC_TRY_HIP(ctx, hipMemcpy(ok_div, s->okd, N, hipMemcpyDeviceToHost));
+    if (dweights && N) {
+        std::vector<int64_t> rstart, sell;
+        std::vector<int32_t> idx;
+        int rc = download_lists(s->grid, rstart, idx, &sell);
+        if (rc) return rc;
+        const int64_t SS = s->grid->S_sell;
+        std::vector<double> v((size_t)(D * SS));
+        HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->dw, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < sell.size(); ++k)
+            for (int d = 0; d < D; ++d) dweights[k * D + d] = v[(size_t)(d * SS + sell[k])];
+    }
+    return HCC_OK;
+}
+
+extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradients) {
+    if (!s || (s->N > 0 && (!Y || !K))) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = s->N;
+    const int M = s->M, D = s->D;
+    if (!N) return HCC_OK;
+    VariantOps ops;
+    find_ops(D, M, s->kind, ops);
+    hipStream_t st = ctx->stream;
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, Y, sizeof(double) * N * M, hipMemcpyHostToDevice, st));
+    ops.pack(s, s->scratch, s->Y[0]);
+    const StepArgs a = s->args();
+    k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, st>>>(s->G, 0.0, (int64_t)s->GS * N);
+    k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, st>>>((uint8_t*)s->orders, 2, N);
+    ops.gather(s, a, s->Y[0], 1);
+    ops.flux(s, a, s->Y[0], s->K[0], 1, false, 0.0);
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(K, s->K[0], sizeof(double) * N * M, hipMemcpyDeviceToHost, st));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (gradients) {
+        std::vector<double> gs((size_t)(s->GS * N));
+        HCC_TRY_HIP(ctx, hipMemcpy(gs.data(), s->G, sizeof(double) * gs.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i)
+            for (int q = 0; q < D * M; ++q) gradients[i * D * M + q] = gs[(size_t)(i * s->GS + q)];
+    }
+    return HCC_OK;
+}
+
+extern "C" int hcc_last_step_timing(hcc_stepper* s, double* ms_total, double* ms_gather, double* ms_flux) {
+    if (!s) return HCC_E_ARG;
+    if (ms_total) *ms_total = s->last_ms;
+    if (ms_gather) *ms_gather = s->last_gather_ms;
+    if (ms_flux) *ms_flux = s->last_flux_ms;
+    return HCC_OK;
+}
