@@ -250,6 +250,40 @@ void bin_window_table(int D, const int64_t* dims, const int32_t* periodic, std::
     }
 }
 
+bool build_halos(int64_t N, int block, const std::vector<int32_t>& cnt, const std::vector<int64_t>& chunk_off,
+                 const std::vector<int32_t>& nbr, std::vector<int32_t>& halo_off, std::vector<int32_t>& halo_idx,
+                 std::vector<uint16_t>& nbr_local, int64_t& max_halo) {
+    const int64_t nb = (N + block - 1) / block;
+    halo_off.assign((size_t)nb + 1, 0);
+    halo_idx.clear();
+    nbr_local.assign(nbr.size(), 0);
+    max_halo = 0;
+    std::vector<int32_t> h;
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t lo = b * block, hi = std::min<int64_t>(N, lo + block);
+        h.clear();
+        for (int64_t i = lo; i < hi; ++i) {
+            h.push_back((int32_t)i);
+            const int64_t base = chunk_off[i / kChunk] + (i % kChunk);
+            for (int32_t k = 0; k < cnt[i]; ++k) h.push_back(nbr[(size_t)(base + (int64_t)k * kChunk)]);
+        }
+        std::sort(h.begin(), h.end());
+        h.erase(std::unique(h.begin(), h.end()), h.end());
+        if (h.size() > 65535) return false;
+        max_halo = std::max<int64_t>(max_halo, (int64_t)h.size());
+        for (int64_t i = lo; i < hi; ++i) {
+            const int64_t base = chunk_off[i / kChunk] + (i % kChunk);
+            for (int32_t k = 0; k < cnt[i]; ++k) {
+                const size_t s = (size_t)(base + (int64_t)k * kChunk);
+                nbr_local[s] = (uint16_t)(std::lower_bound(h.begin(), h.end(), nbr[s]) - h.begin());
+            }
+        }
+        halo_idx.insert(halo_idx.end(), h.begin(), h.end());
+        halo_off[(size_t)b + 1] = (int32_t)halo_idx.size();
+    }
+    return true;
+}
+
 // Outflow donor map. OutflowBC (src/Grid/BoundaryConditions.jl:82-131) copies, in up to five passes, the
 // state of the nearest donor resolved in an earlier pass (strict < on |d|^2, first slot wins ties); donors
 // are interior particles or same-tag particles resolved before. Which donor a particle takes depends on the

@@ -42,6 +42,11 @@ struct HostCloud {
 
 void generate_cloud(const DomainGeom& g, HostCloud& out);
 void bin_window_table(int D, const int64_t* dims, const int32_t* periodic, std::vector<int32_t>& table);
+// Per-block halos for LDS staging: sorted unique neighbours (and own nodes) of each block of `block` nodes, and
+// every SELL slot's neighbour as a local index into its block's halo. Returns false if a halo exceeds 65535.
+bool build_halos(int64_t N, int block, const std::vector<int32_t>& cnt, const std::vector<int64_t>& chunk_off,
+                 const std::vector<int32_t>& nbr, std::vector<int32_t>& halo_off, std::vector<int32_t>& halo_idx,
+                 std::vector<uint16_t>& nbr_local, int64_t& max_halo);
 void outflow_roots(const DomainGeom& g, int64_t N, const double* pos, const uint8_t* isb, const int32_t* tags,
                    const std::vector<int64_t>& rstart, const std::vector<int32_t>& nbr, int n_bc, const int32_t* bc_tag,
                    const int32_t* bc_kind, std::vector<int32_t>& root);
@@ -60,7 +65,14 @@ struct GeomArgs {
     const int32_t* nbr;        // SELL-64 neighbour indices: slot k of node i at chunk_off[i/64] + 64 k + i%64
     double Lw[3], invLw[3];
     double invL, alpha, inv_range_sq;
+    // LDS staging tables (2-D): the nodes of workgroup b (kStage consecutive nodes) read their neighbours from the
+    // block's halo, the sorted union of their neighbourhoods (and themselves): halo_idx[halo_off[b] .. halo_off[b+1])
+    const int32_t* halo_off;
+    const int32_t* halo_idx;
+    const uint16_t* nbr_local;  // SELL-64 like nbr: the neighbour's position in the block's halo
+    int hmax;                   // largest halo of any block
 };
+constexpr int kStage = 256;     // nodes per staged workgroup
 
 struct PdeArgs {
     double gamma;
@@ -101,6 +113,7 @@ struct StepArgs {
     uint8_t* dmark;              // dirty particles of the fixed point (changed plus one hop)
     unsigned* cmark;             // dirty chunks
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
+    double* bmin;                // per-workgroup minima of the CFL estimate
 };
 
 }  // namespace hcc
@@ -125,6 +138,10 @@ struct hcc_grid {
     int64_t nchunks = 0, S = 0, S_sell = 0, max_nb = 0;
     int64_t* d_chunk_off = nullptr;
     int32_t* d_nbr = nullptr;
+    int32_t* d_halo_off = nullptr;
+    int32_t* d_halo_idx = nullptr;
+    uint16_t* d_nbr_local = nullptr;
+    int64_t max_halo = 0;
     std::vector<int32_t> h_tags;
     std::vector<uint8_t> h_isb;
     hcc::GeomArgs args() const;

@@ -279,6 +279,7 @@ hcc::GeomArgs hcc_grid::args() const {
     a.isb = d_isb; a.cnt = d_cnt; a.chunk_off = d_chunk_off; a.nbr = d_nbr;
     for (int d = 0; d < 3; ++d) { a.Lw[d] = geom.Lw[d]; a.invLw[d] = geom.invLw[d]; }
     a.invL = geom.invL; a.alpha = geom.desc.alpha; a.inv_range_sq = geom.inv_range_sq;
+    a.halo_off = d_halo_off; a.halo_idx = d_halo_idx; a.nbr_local = d_nbr_local; a.hmax = (int)max_halo;
     return a;
 }
 
@@ -351,6 +352,7 @@ extern "C" int hcc_grid_destroy(hcc_grid* g) {
     for (int d = 0; d < 3; ++d) (void)hipFree(g->d_pos[d]);
     (void)hipFree(g->d_isb); (void)hipFree(g->d_tags); (void)hipFree(g->d_bin_of); (void)hipFree(g->d_cell_start); (void)hipFree(g->d_cell_cursor);
     (void)hipFree(g->d_cell_particles); (void)hipFree(g->d_bin_window); (void)hipFree(g->d_cnt); (void)hipFree(g->d_chunk_off); (void)hipFree(g->d_nbr);
+    (void)hipFree(g->d_halo_off); (void)hipFree(g->d_halo_idx); (void)hipFree(g->d_nbr_local);
     delete g;
     return HCC_OK;
 }
@@ -406,6 +408,24 @@ extern "C" int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_
     HCC_TRY_HIP(ctx, dalloc(&g->d_cnt, N));
     rc = D == 1 ? run_search<1>(g.get()) : D == 2 ? run_search<2>(g.get()) : run_search<3>(g.get());
     if (rc) return rc;
+    if (D == 2 && N > 0) {   // LDS staging tables of the 2-D passes
+        std::vector<int32_t> cnt((size_t)N), nbr((size_t)std::max<int64_t>(g->S_sell, 1)), hoff, hidx;
+        std::vector<int64_t> off((size_t)(g->nchunks + 1));
+        std::vector<uint16_t> nl;
+        HCC_TRY_HIP(ctx, hipMemcpy(cnt.data(), g->d_cnt, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        HCC_TRY_HIP(ctx, hipMemcpy(off.data(), g->d_chunk_off, sizeof(int64_t) * (g->nchunks + 1), hipMemcpyDeviceToHost));
+        if (g->S_sell) HCC_TRY_HIP(ctx, hipMemcpy(nbr.data(), g->d_nbr, sizeof(int32_t) * g->S_sell, hipMemcpyDeviceToHost));
+        if (build_halos(N, kStage, cnt, off, nbr, hoff, hidx, nl, g->max_halo)) {
+            HCC_TRY_HIP(ctx, dalloc(&g->d_halo_off, hoff.size()));
+            HCC_TRY_HIP(ctx, dalloc(&g->d_halo_idx, hidx.size()));
+            HCC_TRY_HIP(ctx, dalloc(&g->d_nbr_local, nl.size()));
+            HCC_TRY_HIP(ctx, hipMemcpy(g->d_halo_off, hoff.data(), sizeof(int32_t) * hoff.size(), hipMemcpyHostToDevice));
+            HCC_TRY_HIP(ctx, hipMemcpy(g->d_halo_idx, hidx.data(), sizeof(int32_t) * hidx.size(), hipMemcpyHostToDevice));
+            HCC_TRY_HIP(ctx, hipMemcpy(g->d_nbr_local, nl.data(), sizeof(uint16_t) * nl.size(), hipMemcpyHostToDevice));
+        } else {
+            g->max_halo = 0;   // a halo too large for 16-bit local indices: the passes read neighbours from HBM
+        }
+    }
     *out = g.release();
     return HCC_OK;
 }

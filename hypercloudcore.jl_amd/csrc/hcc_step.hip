@@ -48,16 +48,103 @@ __device__ __forceinline__ double rec_geom(const GeomArgs& g, const double* pi, 
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         double x = pj[d] - pi[d];
-        x = x - g.Lw[d] * rint(x * g.invLw[d]);
+        // on a non-periodic axis (Lw = invLw = 0) x - 0 * rint(x * 0) equals x + 0.0 for every finite x (a -0 becomes
+        // +0 either way); the wrap costs a multiply, a rint and a multiply-subtract more
+        if (g.Lw[d] != 0.0) x = x - g.Lw[d] * rint(x * g.invLw[d]);
+        else x = x + 0.0;
         dd[d] = x;
         s = (d == 0) ? x * x : s + x * x;
     }
     return s;
 }
 
-__device__ __forceinline__ int32_t slot_index(const GeomArgs& g, int64_t base, int k, int cnt) {
+// ============================================================================================================
+// neighbour sources: records gathered from HBM, or the block's halo staged in LDS (SoA by record component)
+// ============================================================================================================
+template <int D, int M>
+struct HbmSrc {
+    using L = Layout<D, M>;
+    static constexpr bool kPrefetch = true;     // HBM latency: load slot k+1 while slot k computes
+    const double* Y;
+    const double* G;
+    const int8_t* ord;
+    const int32_t* nbr;
+    __device__ __forceinline__ int32_t idx(int64_t s) const { return nbr[s]; }
+    template <int K>
+    __device__ __forceinline__ void head(int32_t j, double (&r)[K]) const { load_rec<K, L::PU>(Y, j, r); }
+    __device__ __forceinline__ void grad(int32_t j, double (&g)[L::NG]) const { load_rec<L::NG, L::GS>(G, j, g); }
+    __device__ __forceinline__ int order(int32_t j) const { return ord[j]; }
+};
+// A staged halo node in LDS: [position, state | pad to even | gradient record | pad to even] doubles, 16-byte
+// aligned so that a neighbour costs a few ds_read_b128 at one base address; the orders follow as bytes.
+constexpr int lds_grad_offset(int NR) { return (NR + 1) & ~1; }
+constexpr int lds_stride(int NR, int NG) { return (lds_grad_offset(NR) + NG + 1) & ~1; }
+
+template <int K>
+__device__ __forceinline__ void lds_read(const double* p, double (&r)[K]) {
+    const double2* q2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int q = 0; q < (K + 1) / 2; ++q) {
+        const double2 v = q2[q];
+        r[2 * q] = v.x;
+        if (2 * q + 1 < K) r[2 * q + 1] = v.y;
+    }
+}
+
+template <int D, int M, bool GRADS>
+struct LdsSrc {
+    using L = Layout<D, M>;
+    static constexpr bool kPrefetch = false;    // LDS latency is short: load at the top of each slot
+    static constexpr int GO = lds_grad_offset(L::NR);
+    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
+    const double* sh;          // halo node l at sh + l * ST
+    const int8_t* ord;         // orders of the halo nodes (GRADS)
+    const uint16_t* nl;
+    __device__ __forceinline__ int32_t idx(int64_t s) const { return nl[s]; }
+    template <int K>
+    __device__ __forceinline__ void head(int32_t l, double (&r)[K]) const { lds_read<K>(sh + l * ST, r); }
+    __device__ __forceinline__ void grad(int32_t l, double (&g)[L::NG]) const { lds_read<L::NG>(sh + l * ST + GO, g); }
+    __device__ __forceinline__ int order(int32_t l) const { return ord[l]; }
+};
+
+template <class Src>
+__device__ __forceinline__ int32_t src_slot(const Src& src, int64_t base, int k, int cnt) {
     const int kk = k < cnt ? k : cnt - 1;                    // clamped: a redundant load past the last slot
-    return g.nbr[base + (int64_t)kk * kChunk];
+    return src.idx(base + (int64_t)kk * kChunk);
+}
+
+// The logical workgroup of the XCD-aware order (see node_index).
+__device__ __forceinline__ int64_t logical_block(int64_t nblk) {
+    const int64_t b = blockIdx.x, per = (nblk + 7) / 8;
+    return (b & 7) * per + (b >> 3);
+}
+
+// Loads the halo of workgroup lb into LDS (records, with GRADS also gradient records and orders) and returns the
+// source the neighbour loops read. Coalesced: the halo is sorted by node index.
+template <int D, int M, bool GRADS>
+__device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, const double* __restrict__ Y, double* sh,
+                                                          int64_t lb) {
+    using L = Layout<D, M>;
+    using S = LdsSrc<D, M, GRADS>;
+    const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
+    int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
+    for (int t = threadIdx.x; t < nh; t += kBlock) {
+        const int32_t j = a.g.halo_idx[h0 + t];
+        double r[L::NR];
+        load_rec<L::NR, L::PU>(Y, j, r);
+        double* dst = sh + t * S::ST;
+#pragma unroll
+        for (int q = 0; q < L::NR; ++q) dst[q] = r[q];
+        if (GRADS) {
+            double g[L::NG];
+            load_rec<L::NG, L::GS>(a.G, j, g);
+#pragma unroll
+            for (int q = 0; q < L::NG; ++q) dst[S::GO + q] = g[q];
+            ord[t] = a.orders[j];
+        }
+    }
+    __syncthreads();
+    return S{sh, ord, a.g.nbr_local};
 }
 
 // ============================================================================================================
@@ -97,8 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_predict(int64_t N, int64_t nblk, con
 // gather pass: update_content! (MUSCL.jl:351-384), cached reconstruction fit (Interpolators.jl:585-602),
 // Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143)
 // ============================================================================================================
-template <int D, int M, int KIND>
-__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+template <int D, int M, int KIND, class Src>
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i) {
     using L = Layout<D, M>;
     constexpr int NR = L::NR;
     const int64_t N = a.g.N;
@@ -120,16 +207,23 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         for (int q = 0; q < D; ++q)
 #pragma unroll
             for (int m = 0; m < M; ++m) b[q][m] = 0.0;
-        int32_t j1 = slot_index(a.g, base, 1, cnt);
+        int32_t j1 = 0;
         double rn[NR];
-        load_rec<NR, L::PU>(Y, slot_index(a.g, base, 0, cnt), rn);
+        if constexpr (Src::kPrefetch) {
+            j1 = src_slot(src, base, 1, cnt);
+            src.template head<NR>(src_slot(src, base, 0, cnt), rn);
+        }
         for (int k = 0; k < cnt; ++k) {
             double rj[NR];
+            if constexpr (Src::kPrefetch) {
 #pragma unroll
-            for (int q = 0; q < NR; ++q) rj[q] = rn[q];
-            const int32_t jn = j1;
-            j1 = slot_index(a.g, base, k + 2, cnt);
-            load_rec<NR, L::PU>(Y, jn, rn);
+                for (int q = 0; q < NR; ++q) rj[q] = rn[q];
+                const int32_t jn = j1;
+                j1 = src_slot(src, base, k + 2, cnt);
+                src.template head<NR>(jn, rn);
+            } else {
+                src.template head<NR>(src.idx(base + (int64_t)k * kChunk), rj);
+            }
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
             const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
@@ -164,16 +258,23 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
             double phi[M];
 #pragma unroll
             for (int m = 0; m < M; ++m) phi[m] = 1.0;
-            int32_t k1 = slot_index(a.g, base, 1, cnt);
+            int32_t k1 = 0;
             double pn[D];
-            load_rec<D, L::PU>(Y, slot_index(a.g, base, 0, cnt), pn);
+            if constexpr (Src::kPrefetch) {
+                k1 = src_slot(src, base, 1, cnt);
+                src.template head<D>(src_slot(src, base, 0, cnt), pn);
+            }
             for (int k = 0; k < cnt; ++k) {
                 double pj[D];
+                if constexpr (Src::kPrefetch) {
 #pragma unroll
-                for (int d = 0; d < D; ++d) pj[d] = pn[d];
-                const int32_t jn = k1;
-                k1 = slot_index(a.g, base, k + 2, cnt);
-                load_rec<D, L::PU>(Y, jn, pn);
+                    for (int d = 0; d < D; ++d) pj[d] = pn[d];
+                    const int32_t jn = k1;
+                    k1 = src_slot(src, base, k + 2, cnt);
+                    src.template head<D>(jn, pn);
+                } else {
+                    src.template head<D>(src.idx(base + (int64_t)k * kChunk), pj);
+                }
                 double dd[D];
                 rec_geom<D>(a.g, ri, pj, dd);
                 double pint[D];
@@ -207,7 +308,19 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
     const int64_t i = node_index(nblk);
     if (i >= a.g.N) return;
     if (a.g.isb[i] || !(full || a.changed[i])) return;
-    gather_node<D, M, KIND>(a, Y, i);
+    gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+}
+
+// The full gather pass with the workgroup's halo staged in LDS (2-D).
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int64_t nblk) {
+    extern __shared__ double sh[];
+    const int64_t lb = logical_block(nblk);
+    if (lb >= nblk) return;
+    const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    if (i >= a.g.N || a.g.isb[i]) return;
+    gather_node<D, M, KIND>(a, Y, src, i);
 }
 
 // The gather pass over the particles whose order changed in the last MOOD check (clist).
@@ -217,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double
     if (t >= n) return;
     const int64_t i = a.clist[t];
     if (a.g.isb[i]) return;
-    gather_node<D, M, KIND>(a, Y, i);
+    gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
 }
 
 // ============================================================================================================
@@ -231,9 +344,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double
 // exactly those whose K is (re)computed (all interior nodes, or the dirty ones), as in detect_and_drop!. The
 // decrement (_mood_drop!) is applied by k_mark: neighbours read this node's order in this very pass.
 // ============================================================================================================
-template <int D, int M, int KIND, bool CHECK>
-__device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, double* __restrict__ K,
-                                          int64_t i, double h) {
+template <int D, int M, int KIND, bool CHECK, class Src>
+__device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, const Src& src,
+                                          double* __restrict__ K, int64_t i, double h) {
     using P = Pde<D, M, KIND>;
     using L = Layout<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = D + 1;
@@ -259,24 +372,36 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __res
         for (int c = 0; c < BD; ++c)
 #pragma unroll
             for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
-        int32_t j1 = slot_index(a.g, base, 1, cnt);
-        const int32_t j0 = slot_index(a.g, base, 0, cnt);
+        int32_t j1 = 0;
         double rn[NR], gn[NG];
-        load_rec<NR, L::PU>(Y, j0, rn);
-        load_rec<NG, L::GS>(a.G, j0, gn);
-        int on = a.orders[j0];
+        int on = 0;
+        if constexpr (Src::kPrefetch) {
+            j1 = src_slot(src, base, 1, cnt);
+            const int32_t j0 = src_slot(src, base, 0, cnt);
+            src.template head<NR>(j0, rn);
+            src.grad(j0, gn);
+            on = src.order(j0);
+        }
         for (int k = 0; k < cnt; ++k) {
             double rj[NR], gj[NG];
+            int oj;
+            if constexpr (Src::kPrefetch) {
 #pragma unroll
-            for (int q = 0; q < NR; ++q) rj[q] = rn[q];
+                for (int q = 0; q < NR; ++q) rj[q] = rn[q];
 #pragma unroll
-            for (int q = 0; q < NG; ++q) gj[q] = gn[q];
-            const int oj = on;
-            const int32_t jn = j1;
-            j1 = slot_index(a.g, base, k + 2, cnt);
-            load_rec<NR, L::PU>(Y, jn, rn);
-            load_rec<NG, L::GS>(a.G, jn, gn);
-            on = a.orders[jn];
+                for (int q = 0; q < NG; ++q) gj[q] = gn[q];
+                oj = on;
+                const int32_t jn = j1;
+                j1 = src_slot(src, base, k + 2, cnt);
+                src.template head<NR>(jn, rn);
+                src.grad(jn, gn);
+                on = src.order(jn);
+            } else {
+                const int32_t j = src.idx(base + (int64_t)k * kChunk);
+                src.template head<NR>(j, rj);
+                src.grad(j, gj);
+                oj = src.order(j);
+            }
 
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
@@ -312,16 +437,21 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __res
             for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                // sort_flux orders the two sides by sign(d); F_L + F_R is the same sum either way (IEEE addition
-                // commutes), only the dissipation f_R - f_L needs the sorted states
+                // sort_flux orders the two sides by sign(d). Three exact shortcuts: F_L + F_R is the same sum either
+                // way (IEEE addition commutes); max(λ_L, λ_R) is symmetric for the non-negative speeds |u| + c (a NaN
+                // stays a NaN); f_R - f_L = ±(f_ji - f_ij) up to the sign of a zero difference. The sign of a zero
+                // dissipation only reaches dF = F* - F_i when dF is a zero, and the reference's + nc_jump (+0.0)
+                // turns such a zero into +0; the fit accumulators start at +0 and can never hold -0 (x + (-0) = x,
+                // exact cancellation gives +0), so a zero dF of either sign leaves them bit for bit unchanged, and
+                // the + 0.0 is dropped as well.
+                const double sd = jmax(lij[d], lji[d]);
                 const bool right = dd[d] > 0.0;
-                const double sd = right ? jmax(lij[d], lji[d]) : jmax(lji[d], lij[d]);
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
-                    const double fL = right ? fij[m] : fji[m], fR = right ? fji[m] : fij[m];
-                    const double diss = sd * (fR - fL);
+                    const double x = fji[m] - fij[m];
+                    const double diss = sd * (right ? x : -x);
                     const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
-                    const double dF = (Fnum - Fi[d][m]) + 0.0;
+                    const double dF = Fnum - Fi[d][m];
 #pragma unroll
                     for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
                 }
@@ -395,7 +525,29 @@ __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, const double* __res
 #pragma unroll
                 for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else if (full || a.dirty[i]) {
-            ch = flux_node<D, M, KIND, CHECK>(a, Y, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+        }
+    }
+    if (CHECK) append_changed(a, ch, i);
+}
+
+// The full flux pass with the workgroup's halo (records, gradients, orders) staged in LDS (2-D).
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
+                                                         double h, int64_t nblk) {
+    extern __shared__ double sh[];
+    const int64_t lb = logical_block(nblk);
+    if (lb >= nblk) return;
+    const auto src = stage_halo<D, M, true>(a, Y, sh, lb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const int64_t N = a.g.N;
+    bool ch = false;
+    if (i < N) {
+        if (a.g.isb[i]) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+        } else {
+            ch = flux_node<D, M, KIND, CHECK>(a, Y, src, K, i, h);
         }
     }
     if (CHECK) append_changed(a, ch, i);
@@ -416,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_chunks(StepArgs a, const double
         if ((threadIdx.x & 63) == 0) a.cmark[c] = 0u;
         if (i < a.g.N && a.dmark[i]) {
             a.dmark[i] = 0;
-            if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, K, i, h);
+            if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
         }
     }
     append_changed(a, ch, i);
@@ -538,12 +690,30 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepArgs a, const double* __r
 // patterns, so the minimum is an integer atomic). The per-slot derivative weights are recomputed from the divergence
 // inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223): the same bits as the
 // cached dweights, without streaming them.
-template <int D, int M, int KIND>
+template <int D, int M, int KIND, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __restrict__ U, int64_t nblk) {
     using L = Layout<D, M>;
     constexpr int BD = D + 1;
-    const int64_t i = node_index(nblk);
+    extern __shared__ double sh[];
     const int64_t N = a.g.N;
+    const int hm = a.g.hmax;
+    int64_t i;
+    if (STAGED) {
+        const int64_t lb = logical_block(nblk);
+        if (lb >= nblk) return;
+        const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
+        for (int t = threadIdx.x; t < nh; t += kBlock) {
+            const int32_t j = a.g.halo_idx[h0 + t];
+            double r[D];
+            load_rec<D, L::PU>(U, j, r);
+#pragma unroll
+            for (int d = 0; d < D; ++d) { sh[d * hm + t] = r[d]; sh[(D + d) * hm + t] = a.lam[(int64_t)j * D + d]; }
+        }
+        __syncthreads();
+        i = lb * kBlock + threadIdx.x;
+    } else {
+        i = node_index(nblk);
+    }
     double dt = INFINITY;
     if (i < N && !a.g.isb[i] && a.g.cnt[i] > 0) {
         double li[D], X[D][BD], pi[D];
@@ -557,22 +727,38 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
         const int cnt = a.g.cnt[i];
         const int64_t base = sell_base(a.g, i);
         const double invL = a.g.invL;
-        double sum_c = 0.0;
-        int32_t j1 = slot_index(a.g, base, 1, cnt);
-        const int32_t j0 = slot_index(a.g, base, 0, cnt);
-        double pn[D], ln[D];
-        load_rec<D, L::PU>(U, j0, pn);
+        auto nidx = [&](int k) -> int32_t {
+            const int kk = k < cnt ? k : cnt - 1;
+            return STAGED ? (int32_t)a.g.nbr_local[base + (int64_t)kk * kChunk] : a.g.nbr[base + (int64_t)kk * kChunk];
+        };
+        auto fetch = [&](int32_t j, double (&p)[D], double (&l)[D]) {
+            if (STAGED) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) ln[d] = a.lam[(int64_t)j0 * D + d];
+                for (int d = 0; d < D; ++d) { p[d] = sh[d * hm + j]; l[d] = sh[(D + d) * hm + j]; }
+            } else {
+                load_rec<D, L::PU>(U, j, p);
+#pragma unroll
+                for (int d = 0; d < D; ++d) l[d] = a.lam[(int64_t)j * D + d];
+            }
+        };
+        double sum_c = 0.0;
+        int32_t j1 = 0;
+        double pn[D], ln[D];
+        if (!STAGED) {
+            j1 = nidx(1);
+            fetch(nidx(0), pn, ln);
+        }
         for (int k = 0; k < cnt; ++k) {
             double pj[D], lj[D];
+            if (!STAGED) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) { pj[d] = pn[d]; lj[d] = ln[d]; }
-            const int32_t jn = j1;
-            j1 = slot_index(a.g, base, k + 2, cnt);
-            load_rec<D, L::PU>(U, jn, pn);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ln[d] = a.lam[(int64_t)jn * D + d];
+                for (int d = 0; d < D; ++d) { pj[d] = pn[d]; lj[d] = ln[d]; }
+                const int32_t jn = j1;
+                j1 = nidx(k + 2);
+                fetch(jn, pn, ln);
+            } else {
+                fetch(nidx(k), pj, lj);
+            }
             double dd[D];
             const double d2 = rec_geom<D>(a.g, pi, pj, dd);
             const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
@@ -593,8 +779,33 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
         }
         if (sum_c > kFloatMin) dt = (1.0 / (double)D) / sum_c;
     }
+    // block minimum into bmin[lb]: one store per workgroup, no contended atomics on a single address
+    __shared__ double wmin[kBlock / 64];
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
-    if ((threadIdx.x & 63) == 0 && dt < INFINITY) atomicMin(&a.counters[2], (unsigned long long)__double_as_longlong(dt));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int64_t lb = logical_block(nblk);
+        double m = wmin[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) m = fmin(m, wmin[w]);
+        if (lb < nblk) a.bmin[lb] = m;
+    }
+}
+
+// dt = min over the workgroup minima (ParticleGrids.jl:386-415 reduces with minimum); +inf if no node contributes
+__global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ bmin, int64_t nblk, unsigned long long* counters) {
+    __shared__ double wmin[16];
+    double dt = INFINITY;
+    for (int64_t b = threadIdx.x; b < nblk; b += 1024) dt = fmin(dt, bmin[b]);
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < 16; ++w) m = fmin(m, wmin[w]);
+        counters[2] = (unsigned long long)__double_as_longlong(m);
+    }
 }
 
 // ============================================================================================================
@@ -668,7 +879,10 @@ struct hcc_stepper {
     unsigned* cmark = nullptr;
     int32_t *clist = nullptr, *chlist = nullptr;
     double* lam = nullptr;
+    double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
     unsigned long long* counters = nullptr;
+    // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
+    size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
@@ -693,6 +907,7 @@ StepArgs hcc_stepper::args() const {
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
     a.counters = counters; a.clist = clist; a.chlist = chlist; a.dmark = dmark; a.cmark = cmark; a.lam = lam;
+    a.bmin = bmin;
     return a;
 }
 
@@ -702,14 +917,18 @@ struct Variant {
     static hipStream_t st(const hcc_stepper* s) { return s->ctx->stream; }
     static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full) {
         const int64_t nb = nblocks(s->N);
-        k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
+        if (full && s->lds_gather) k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, nb);
+        else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
     }
     static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y, int64_t n) {
         if (n > 0) k_gather_list<D, M, KIND><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(a, Y, n);
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, bool check, double h) {
         const int64_t nb = nblocks(s->N);
-        if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, h, nb);
+        if (full && s->lds_flux) {
+            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, Y, K, h, nb);
+            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, Y, K, 0.0, nb);
+        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, h, nb);
         else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, 0.0, nb);
     }
     static void flux_chunks(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int64_t bound, double h) {
@@ -725,7 +944,9 @@ struct Variant {
     static void cfl(const hcc_stepper* s, const StepArgs& a) {
         const int64_t nb = nblocks(s->N);
         k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
-        k_cfl<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        if (s->lds_cfl) k_cfl<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_cfl, st(s)>>>(a, s->U, nb);
+        else k_cfl<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters);
     }
     static void predict(const hcc_stepper* s, const double* const* Ks, const double* c, int nterm, unsigned use, double* target) {
         const int64_t nb = nblocks(s->N);
@@ -780,7 +1001,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->cmark, (void*)s->clist, (void*)s->chlist,
-                    (void*)s->lam, (void*)s->counters})
+                    (void*)s->lam, (void*)s->bmin, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -838,6 +1059,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
     HCC_TRY_HIP(ctx, dalloc(&s->chlist, g->nchunks + 1));
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
@@ -846,6 +1068,16 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 8 * sizeof(unsigned long long), st));
+    if (g->max_halo > 0) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
+        int lds_max = 0;
+        HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        const size_t hm = (size_t)g->max_halo;
+        const int NR = D + (int)M, NG = D * (int)M;
+        auto fit = [&](size_t bytes) { return bytes <= (size_t)lds_max ? bytes : 0; };
+        s->lds_gather = fit((size_t)lds_stride(NR, 0) * hm * sizeof(double));
+        s->lds_flux = fit((size_t)lds_stride(NR, NG) * hm * sizeof(double) + hm);
+        s->lds_cfl = fit(2 * (size_t)D * hm * sizeof(double));
+    }
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
     for (auto& e : s->evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
