@@ -107,11 +107,11 @@ struct StepArgs {
     uint8_t* trig;               // mood_flags of the step
     uint8_t* inadm;              // last check: candidate not admissible
     unsigned long long* counters;// [0] orders changed in the last check (= clist length), [1] inadmissible accepted,
-                                 // [2] dt bits, [3] dirty chunk-list length
+                                 // [2] dt bits, [4] fixed-point iterations of the step, [5] dirty list length
     int32_t* clist;              // particles whose order changed in the last check
-    int32_t* chlist;             // chunks (64 particles) holding dirty particles
+    int32_t* dlist;              // dirty particles of the fixed point, densely
     uint8_t* dmark;              // dirty particles of the fixed point (changed plus one hop)
-    unsigned* cmark;             // dirty chunks
+    uint8_t* bmark;              // dirty workgroup blocks
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
     double* bmin;                // per-workgroup minima of the CFL estimate
 };

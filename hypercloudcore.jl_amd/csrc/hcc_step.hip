@@ -325,12 +325,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
 
 // The gather pass over the particles whose order changed in the last MOOD check (clist).
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double* __restrict__ Y, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    const int64_t i = a.clist[t];
-    if (a.g.isb[i]) return;
-    gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+// The gather pass over the changed list; its length counters[0] is read on the device (grid-stride loop).
+__global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double* __restrict__ Y) {
+    const int64_t n = (int64_t)a.counters[0];
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = a.clist[t];
+        if (!a.g.isb[i]) gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    }
 }
 
 // ============================================================================================================
@@ -553,43 +554,71 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, const double
     if (CHECK) append_changed(a, ch, i);
 }
 
-// The flux pass (with the MOOD check) over the dirty chunks of the fixed point: one wavefront per chunk of 64
-// consecutive (Morton-ordered) particles, the marked ones compute; marks are cleared for the next iteration.
+// The flux pass (with the MOOD check) over the dense dirty list of the fixed point (dlist, counters[5], read on the
+// device): one thread per dirty particle, so the recompute fills whole wavefronts; each wavefront's entries are the
+// dirty particles of one block in index order. Dirty marks are cleared for the next iteration.
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_flux_chunks(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
-                                                         double h) {
-    const int64_t w = (int64_t)blockIdx.x * (kBlock / kChunk) + (threadIdx.x >> 6);
-    const int64_t nch = (int64_t)a.counters[3];
-    bool ch = false;
-    int64_t i = 0;
-    if (w < nch) {
-        const int64_t c = a.chlist[w];
-        i = c * kChunk + (threadIdx.x & 63);
-        if ((threadIdx.x & 63) == 0) a.cmark[c] = 0u;
-        if (i < a.g.N && a.dmark[i]) {
+__global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
+                                                       double h) {
+    const int64_t n = (int64_t)a.counters[5];
+    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t t = t0 + threadIdx.x;
+        bool ch = false;
+        int64_t i = 0;
+        if (t < n) {
+            i = a.dlist[t];
             a.dmark[i] = 0;
             if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
         }
+        append_changed(a, ch, i);
     }
-    append_changed(a, ch, i);
 }
 
-// _mood_drop! (MOOD.jl:299-309) and one hop of expand_mask! (MOOD.jl:323-351) for the changed list: the particle
-// drops one order, it and its neighbours are marked dirty, and each chunk holding a marked particle is listed once.
-// The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
-__global__ __launch_bounds__(kBlock) void k_mark(StepArgs a, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    const int32_t i = a.clist[t];
-    a.orders[i] = (int8_t)(a.orders[i] - 1);
-    const int cnt = a.g.cnt[i];
-    const int64_t base = sell_base(a.g, i);
-    for (int k = -1; k < cnt; ++k) {
-        const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
-        a.dmark[j] = 1;
-        const int32_t c = j >> 6;
-        if (a.cmark[c] == 0u && atomicExch(&a.cmark[c], 1u) == 0u)
-            a.chlist[atomicAdd(&a.counters[3], 1ull)] = c;
+// _mood_drop! (MOOD.jl:299-309) and one hop of expand_mask! (MOOD.jl:323-351) for the changed list (length
+// counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty, and so is
+// each workgroup block holding one. Plain idempotent byte stores, no atomics. The neighbour relation is symmetric,
+// so marking the neighbours of changed particles gives the pull form's set. Also resets the dirty-list length and
+// counts the fixed-point iterations that changed an order.
+__global__ __launch_bounds__(kBlock) void k_mark(StepArgs a) {
+    const int64_t n = (int64_t)a.counters[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counters[5] = 0;
+        if (n > 0) a.counters[4] += 1;
+    }
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        const int32_t i = a.clist[t];
+        a.orders[i] = (int8_t)(a.orders[i] - 1);
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        for (int k = -1; k < cnt; ++k) {
+            const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
+            a.dmark[j] = 1;
+            a.bmark[j / kBlock] = 1;
+        }
+    }
+}
+
+// Lists the dirty particles of the marked blocks densely (dlist, counters[5]): a workgroup per block, which leaves at
+// once when its block is unmarked; one atomic per wavefront with dirty particles. Clears the block marks and resets
+// the changed count for the flux pass that follows.
+__global__ __launch_bounds__(kBlock) void k_list_dirty(StepArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;
+    const int64_t b = blockIdx.x;
+    if (!a.bmark[b]) return;
+    __syncthreads();                                   // every thread has read the mark before it is cleared
+    if (threadIdx.x == 0) a.bmark[b] = 0;
+    const int64_t i = b * kBlock + threadIdx.x;
+    const bool f = i < a.g.N && a.dmark[i];
+    const unsigned long long ball = __ballot(f);
+    if (!ball) return;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&a.counters[5], (unsigned long long)__popcll(ball));
+    base = __shfl(base, (int)leader);
+    if (f) {
+        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        a.dlist[base + __popcll(below)] = (int32_t)i;
     }
 }
 
@@ -844,12 +873,11 @@ __global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* p, uint8_t v, int64
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t < n) p[t] = v;
 }
-// mode 0: the changed count; 1: everything of a new step / CFL estimate; 2: the dirty chunk-list length
+// mode 0: the changed count; 1: everything of a new step / CFL estimate
 __global__ void k_reset_counters(unsigned long long* c, int mode) {
     if (threadIdx.x == 0) {
-        if (mode == 2) { c[3] = 0; return; }
         c[0] = 0;
-        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[3] = 0; }
+        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[4] = 0; c[5] = 0; }
     }
 }
 
@@ -876,14 +904,15 @@ struct hcc_stepper {
     double* scratch = nullptr; // N * M doubles for up/downloads
     int8_t* orders = nullptr;
     uint8_t *changed = nullptr, *dirty = nullptr, *trig = nullptr, *inadm = nullptr, *dmark = nullptr;
-    unsigned* cmark = nullptr;
-    int32_t *clist = nullptr, *chlist = nullptr;
+    uint8_t* bmark = nullptr;
+    int32_t *clist = nullptr, *dlist = nullptr;
     double* lam = nullptr;
     double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evm[2] = {};                // changed-count snapshots of the MOOD fixed point
     hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
@@ -906,7 +935,7 @@ StepArgs hcc_stepper::args() const {
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
-    a.counters = counters; a.clist = clist; a.chlist = chlist; a.dmark = dmark; a.cmark = cmark; a.lam = lam;
+    a.counters = counters; a.clist = clist; a.dlist = dlist; a.dmark = dmark; a.bmark = bmark; a.lam = lam;
     a.bmin = bmin;
     return a;
 }
@@ -920,8 +949,8 @@ struct Variant {
         if (full && s->lds_gather) k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
     }
-    static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y, int64_t n) {
-        if (n > 0) k_gather_list<D, M, KIND><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(a, Y, n);
+    static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y) {
+        k_gather_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 256), kBlock, 0, st(s)>>>(a, Y);
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, bool check, double h) {
         const int64_t nb = nblocks(s->N);
@@ -931,11 +960,8 @@ struct Variant {
         } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, h, nb);
         else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, 0.0, nb);
     }
-    static void flux_chunks(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int64_t bound, double h) {
-        if (bound > 0) {
-            const int64_t per = kBlock / kChunk;
-            k_flux_chunks<D, M, KIND><<<(unsigned)((bound + per - 1) / per), kBlock, 0, st(s)>>>(a, Y, K, h);
-        }
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, double h) {
+        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, K, h);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
         const int64_t nb = nblocks(s->N);
@@ -965,9 +991,9 @@ struct Variant {
 
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int);
-    void (*gather_list)(const hcc_stepper*, const StepArgs&, const double*, int64_t);
+    void (*gather_list)(const hcc_stepper*, const StepArgs&, const double*);
     void (*flux)(const hcc_stepper*, const StepArgs&, const double*, double*, int, bool, double);
-    void (*flux_chunks)(const hcc_stepper*, const StepArgs&, const double*, double*, int64_t, double);
+    void (*flux_list)(const hcc_stepper*, const StepArgs&, const double*, double*, double);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
     void (*cfl)(const hcc_stepper*, const StepArgs&);
     void (*predict)(const hcc_stepper*, const double* const*, const double*, int, unsigned, double*);
@@ -978,7 +1004,7 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::gather_list, &V::flux, &V::flux_chunks, &V::flag, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::gather_list, &V::flux, &V::flux_list, &V::flag, &V::cfl, &V::predict,
                       &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
@@ -1000,13 +1026,14 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
-                    (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->cmark, (void*)s->clist, (void*)s->chlist,
+                    (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->bmark, (void*)s->clist, (void*)s->dlist,
                     (void*)s->lam, (void*)s->bmin, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
+    for (auto& e : s->evm) if (e) (void)hipEventDestroy(e);
     delete s;
     return HCC_OK;
 }
@@ -1055,15 +1082,15 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->trig, N));
     HCC_TRY_HIP(ctx, dalloc(&s->inadm, N));
     HCC_TRY_HIP(ctx, dalloc(&s->dmark, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->cmark, g->nchunks + 1));
+    HCC_TRY_HIP(ctx, dalloc(&s->bmark, nblocks(N) + 1));
     HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->chlist, g->nchunks + 1));
+    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N));
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->cmark, 0, sizeof(unsigned) * (g->nchunks + 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->bmark, 0, nblocks(N) + 1, st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
@@ -1081,6 +1108,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
     for (auto& e : s->evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
+    for (auto& e : s->evm) HCC_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (N > 0) {
         ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
         MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
@@ -1190,9 +1218,27 @@ struct StepRunner {
         n = ctx->h_pinned[0];
         return e;
     }
+    // One iteration of the fused fixed point; every count it needs is read on the device, so iterations queue
+    // without a host round trip, and one enqueued after the fixed point is reached does nothing.
+    void mood_iteration(const double* Yp, double* Kp, double h) {
+        hipStream_t str = s->ctx->stream;
+        const unsigned g = (unsigned)std::min<int64_t>(nb, 512);
+        k_mark<<<g, kBlock, 0, str>>>(a);
+        ops.gather_list(s, a, Yp);
+        k_list_dirty<<<(unsigned)nb, kBlock, 0, str>>>(a);
+        ops.flux_list(s, a, Yp, Kp, h);
+    }
+    // snapshot of the changed count into pinned slot q (4 or 5) and its event
+    hipError_t snapshot(int q) {
+        hcc_ctx* ctx = s->ctx;
+        hipError_t e = hipMemcpyAsync(ctx->h_pinned + q, s->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(s->evm[q - 4], ctx->stream);
+        return e;
+    }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
     // flux pass; every recompute of K_{i-1} on the dirty set re-checks exactly that set. The candidate Y_i is
-    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}.
+    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}. The host keeps one
+    // iteration queued beyond the one whose count it waits for, so the device does not idle on the round trip.
     hipError_t checkpoint(int i, double dt) {
         if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt); return hipSuccess; }
         const double* Yp = s->Y[i - 2];
@@ -1200,18 +1246,16 @@ struct StepRunner {
         const double h = s->desc.substep_ratio * dt;
         hipStream_t str = s->ctx->stream;
         if (fused()) {
-            for (;;) {
-                unsigned long long n = 0;
-                hipError_t e = read_changed(n);
-                if (e != hipSuccess) return e;
-                if (n == 0) break;
-                ++iters;
-                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 2);
-                k_mark<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, str>>>(a, (int64_t)n);
-                ops.gather_list(s, a, Yp, (int64_t)n);
-                k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
-                const int64_t bound = std::min<int64_t>(s->grid->nchunks, (int64_t)n * (1 + s->grid->max_nb));
-                ops.flux_chunks(s, a, Yp, Kp, bound, h);
+            hipError_t e = snapshot(4);                  // count of the full pass's check
+            if (e != hipSuccess) return e;
+            mood_iteration(Yp, Kp, h);
+            if ((e = snapshot(5)) != hipSuccess) return e;
+            for (int k = 0;; ++k) {
+                const int q = 4 + (k & 1);
+                if ((e = hipEventSynchronize(s->evm[q - 4])) != hipSuccess) return e;
+                if (s->ctx->h_pinned[q] == 0) break;     // the queued iteration k + 1 finds nothing to do
+                mood_iteration(Yp, Kp, h);               // iteration k + 2
+                if ((e = snapshot(q)) != hipSuccess) return e;
             }
             k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
             predict(i, dt);
@@ -1241,7 +1285,8 @@ struct StepRunner {
         const int st = s->desc.stages;
         hipStream_t str = ctx->stream;
         nb = nblocks(N);
-        k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, str>>>(s->G, 0.0, (int64_t)s->GS * N);
+        // gradients: every full gather pass writes each interior particle's record; ghosts keep the zeros written at
+        // set-up (the reference's gather skips boundary particles)
         k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>((uint8_t*)s->orders, 2, N);
         k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>(s->trig, 0, N);
         k_reset_counters<<<1, 64, 0, str>>>(s->counters, 1);
@@ -1279,10 +1324,10 @@ extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     hipError_t e = r.run(dt);
     if (e != hipSuccess) { ctx->err = std::string("hcc_step: ") + hipGetErrorString(e); return HCC_E_HIP; }
     HCC_TRY_HIP(ctx, hipEventRecord(s->ev1, ctx->stream));
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->inadmissible = (int64_t)ctx->h_pinned[1];
-    s->mood_iters = r.iters;
+    s->mood_iters = r.iters + (int64_t)ctx->h_pinned[4];
     float ms = 0;
     HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
