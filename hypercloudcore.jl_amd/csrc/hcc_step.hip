@@ -152,21 +152,41 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
 // ============================================================================================================
 // A boundary particle with an outflow donor takes the stage value of its interior root (the same combination,
 // bit-identical); other boundary particles keep pg.rhos (= U^n during the step). Records keep each node's position.
+// Fused with the predictor: PRED_RESET (first stage of a step) resets the particle orders to the maximum order, the
+// step's MOOD flags and the step counters; PRED_COUNT (after a MOOD fixed point) counts the accepted inadmissible
+// candidates of the checkpoint (_count_inadmissible, TimeSteppers.jl:481-490), one atomic per wavefront that has any.
+constexpr unsigned PRED_RESET = 1u, PRED_COUNT = 2u;
 template <int D, int M>
-__global__ __launch_bounds__(kBlock) void k_predict(int64_t N, int64_t nblk, const uint8_t* __restrict__ isb,
-                                                     const int32_t* __restrict__ root, const double* __restrict__ Un,
-                                                     const double* K0, const double* K1, const double* K2, const double* K3,
-                                                     const double* K4, double c0, double c1, double c2, double c3, double c4,
-                                                     int nterm, unsigned use, double* __restrict__ target) {
+__global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, const int32_t* __restrict__ root,
+                                                     const double* __restrict__ Un, const double* K0, const double* K1,
+                                                     const double* K2, const double* K3, const double* K4, double c0,
+                                                     double c1, double c2, double c3, double c4, int nterm, unsigned use,
+                                                     double* __restrict__ target, unsigned flags) {
     using L = Layout<D, M>;
     const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    if ((flags & PRED_RESET) && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counters[0] = 0; a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0; a.counters[5] = 0;
+    }
+    if (flags & PRED_COUNT) {
+        const bool bad = i < N && !a.g.isb[i] && a.inadm[i];
+        const unsigned long long ball = __ballot(bad);
+        if (ball && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1))
+            atomicAdd(&a.counters[1], (unsigned long long)__popcll(ball));
+    }
     if (i >= N) return;
+    if (flags & PRED_RESET) { a.orders[i] = 2; a.trig[i] = 0; }
     double r[L::PU];
     load_rec<L::PU, L::PU>(Un, i, r);
-    const int64_t src = isb[i] ? (int64_t)root[i] : i;
+    const bool ghost = a.g.isb[i];
+    const int64_t src = ghost ? (int64_t)root[i] : i;
     if (src >= 0) {
         double s[L::NR];
-        load_rec<L::NR, L::PU>(Un, src, s);
+        if (ghost) load_rec<L::NR, L::PU>(Un, src, s);
+        else {
+#pragma unroll
+            for (int q = 0; q < L::NR; ++q) s[q] = r[q];
+        }
         const double* Ks[5] = {K0, K1, K2, K3, K4};
         const double cs[5] = {c0, c1, c2, c3, c4};
 #pragma unroll
@@ -305,6 +325,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int64_t nblk) {
+    if (full && blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
     const int64_t i = node_index(nblk);
     if (i >= a.g.N) return;
     if (a.g.isb[i] || !(full || a.changed[i])) return;
@@ -315,23 +336,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int64_t nblk) {
     extern __shared__ double sh[];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
     const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
     const int64_t i = lb * kBlock + threadIdx.x;
     if (i >= a.g.N || a.g.isb[i]) return;
     gather_node<D, M, KIND>(a, Y, src, i);
-}
-
-// The gather pass over the particles whose order changed in the last MOOD check (clist).
-template <int D, int M, int KIND>
-// The gather pass over the changed list; its length counters[0] is read on the device (grid-stride loop).
-__global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double* __restrict__ Y) {
-    const int64_t n = (int64_t)a.counters[0];
-    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = a.clist[t];
-        if (!a.g.isb[i]) gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
-    }
 }
 
 // ============================================================================================================
@@ -343,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_list(StepArgs a, const double
 // forward Euler substep W = Y + h K is node-local (form_substep!, TimeSteppers.jl:501-518), and the DMP bounds of
 // MOODu1 (_dmp_fails, MOOD.jl:175-184) are the extrema of Y over the same neighbour loop. The nodes checked are
 // exactly those whose K is (re)computed (all interior nodes, or the dirty ones), as in detect_and_drop!. The
-// decrement (_mood_drop!) is applied by k_mark: neighbours read this node's order in this very pass.
+// decrement (_mood_drop!) is applied by k_mark_gather: neighbours read this node's order in this very pass.
 // ============================================================================================================
 template <int D, int M, int KIND, bool CHECK, class Src>
 __device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, const Src& src,
@@ -574,16 +585,20 @@ __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, const double* 
     }
 }
 
-// _mood_drop! (MOOD.jl:299-309) and one hop of expand_mask! (MOOD.jl:323-351) for the changed list (length
-// counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty, and so is
-// each workgroup block holding one. Plain idempotent byte stores, no atomics. The neighbour relation is symmetric,
-// so marking the neighbours of changed particles gives the pull form's set. Also resets the dirty-list length and
-// counts the fixed-point iterations that changed an order.
-__global__ __launch_bounds__(kBlock) void k_mark(StepArgs a) {
+// _mood_drop! (MOOD.jl:299-309), one hop of expand_mask! (MOOD.jl:323-351) and the gather of the changed list
+// (length counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty, and
+// so is each workgroup block holding one (plain idempotent byte stores, no atomics); then its gradient is recomputed
+// at the new order (the gather reads only the particle's own order, which this thread just wrote, and stage states).
+// The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
+// Block 0 also resets the dirty-list length, counts the fixed-point iterations that changed an order and publishes
+// the count to the host (pub, pinned host memory) with a system-scope store.
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub) {
     const int64_t n = (int64_t)a.counters[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.counters[5] = 0;
         if (n > 0) a.counters[4] += 1;
+        __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
         const int32_t i = a.clist[t];
@@ -595,6 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_mark(StepArgs a) {
             a.dmark[j] = 1;
             a.bmark[j / kBlock] = 1;
         }
+        if (!a.g.isb[i]) gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
     }
 }
 
@@ -715,10 +731,10 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepArgs a, const double* __r
     for (int d = 0; d < D; ++d) a.lam[i * D + d] = l[d];
 }
 
-// _cfl_weights_body! (ParticleGrids.jl:439-450) and the minimum over particles (positive doubles order as their bit
-// patterns, so the minimum is an integer atomic). The per-slot derivative weights are recomputed from the divergence
-// inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223): the same bits as the
-// cached dweights, without streaming them.
+// _cfl_weights_body! (ParticleGrids.jl:439-450) and the minimum over particles (a workgroup minimum per block,
+// reduced by k_cfl_final). The per-slot derivative weights are recomputed from the divergence inverse and the slot
+// geometry with the operations of _weights_body! (MUSCL.jl:211-223): the same bits as the cached dweights, without
+// streaming them. STAGED: the block's halo positions and wave speeds (computed while staging) sit in LDS.
 template <int D, int M, int KIND, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __restrict__ U, int64_t nblk) {
     using L = Layout<D, M>;
@@ -731,12 +747,15 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
         const int64_t lb = logical_block(nblk);
         if (lb >= nblk) return;
         const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
-        for (int t = threadIdx.x; t < nh; t += kBlock) {
+        for (int t = threadIdx.x; t < nh; t += kBlock) {   // positions and wave speeds (_get_Lambda) of the halo
             const int32_t j = a.g.halo_idx[h0 + t];
-            double r[D];
-            load_rec<D, L::PU>(U, j, r);
+            double r[L::NR], f[M], l[D];
+            load_rec<L::NR, L::PU>(U, j, r);
 #pragma unroll
-            for (int d = 0; d < D; ++d) { sh[d * hm + t] = r[d]; sh[(D + d) * hm + t] = a.lam[(int64_t)j * D + d]; }
+            for (int m = 0; m < M; ++m) f[m] = r[D + m];
+            Pde<D, M, KIND>::maxeig(a.pde, f, l);
+#pragma unroll
+            for (int d = 0; d < D; ++d) { sh[d * hm + t] = r[d]; sh[(D + d) * hm + t] = l[d]; }
         }
         __syncthreads();
         i = lb * kBlock + threadIdx.x;
@@ -746,8 +765,16 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
     double dt = INFINITY;
     if (i < N && !a.g.isb[i] && a.g.cnt[i] > 0) {
         double li[D], X[D][BD], pi[D];
+        if (STAGED) {
+            double r[L::NR], f[M];
+            load_rec<L::NR, L::PU>(U, i, r);
 #pragma unroll
-        for (int d = 0; d < D; ++d) li[d] = a.lam[i * D + d];
+            for (int m = 0; m < M; ++m) f[m] = r[D + m];
+            Pde<D, M, KIND>::maxeig(a.pde, f, li);
+        } else {
+#pragma unroll
+            for (int d = 0; d < D; ++d) li[d] = a.lam[i * D + d];
+        }
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
@@ -912,7 +939,9 @@ struct hcc_stepper {
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t evm[2] = {};                // changed-count snapshots of the MOOD fixed point
+    hipEvent_t evm[2] = {};                // follow the kernels that publish the MOOD fixed point's counts
+    unsigned long long* h_pub = nullptr;   // two count slots in coherent pinned host memory ...
+    unsigned long long* d_pub = nullptr;   // ... and their device address
     hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
@@ -949,8 +978,8 @@ struct Variant {
         if (full && s->lds_gather) k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
     }
-    static void gather_list(const hcc_stepper* s, const StepArgs& a, const double* Y) {
-        k_gather_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 256), kBlock, 0, st(s)>>>(a, Y);
+    static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub) {
+        k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub);
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, bool check, double h) {
         const int64_t nb = nblocks(s->N);
@@ -969,15 +998,19 @@ struct Variant {
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a) {
         const int64_t nb = nblocks(s->N);
-        k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
-        if (s->lds_cfl) k_cfl<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_cfl, st(s)>>>(a, s->U, nb);
-        else k_cfl<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        if (s->lds_cfl) {
+            k_cfl<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_cfl, st(s)>>>(a, s->U, nb);
+        } else {
+            k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+            k_cfl<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        }
         k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters);
     }
-    static void predict(const hcc_stepper* s, const double* const* Ks, const double* c, int nterm, unsigned use, double* target) {
+    static void predict(const hcc_stepper* s, const StepArgs& a, const double* const* Ks, const double* c, int nterm,
+                        unsigned use, double* target, unsigned flags) {
         const int64_t nb = nblocks(s->N);
-        k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(s->N, nb, s->grid->d_isb, s->root, s->U, Ks[0], Ks[1], Ks[2],
-                                                             Ks[3], Ks[4], c[0], c[1], c[2], c[3], c[4], nterm, use, target);
+        k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, Ks[0], Ks[1], Ks[2], Ks[3], Ks[4], c[0],
+                                                             c[1], c[2], c[3], c[4], nterm, use, target, flags);
     }
     static void pack(const hcc_stepper* s, const double* U_aos, double* rec) {
         const hcc_grid* g = s->grid;
@@ -991,12 +1024,12 @@ struct Variant {
 
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int);
-    void (*gather_list)(const hcc_stepper*, const StepArgs&, const double*);
+    void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*);
     void (*flux)(const hcc_stepper*, const StepArgs&, const double*, double*, int, bool, double);
     void (*flux_list)(const hcc_stepper*, const StepArgs&, const double*, double*, double);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
     void (*cfl)(const hcc_stepper*, const StepArgs&);
-    void (*predict)(const hcc_stepper*, const double* const*, const double*, int, unsigned, double*);
+    void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*, unsigned);
     void (*pack)(const hcc_stepper*, const double*, double*);
     void (*unpack)(const hcc_stepper*, const double*, double*);
     int PU, GS;
@@ -1004,7 +1037,7 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::gather_list, &V::flux, &V::flux_list, &V::flag, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::cfl, &V::predict,
                       &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
@@ -1034,6 +1067,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
     for (auto& e : s->evm) if (e) (void)hipEventDestroy(e);
+    if (s->h_pub) (void)hipHostFree(s->h_pub);
     delete s;
     return HCC_OK;
 }
@@ -1109,6 +1143,9 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
     for (auto& e : s->evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
     for (auto& e : s->evm) HCC_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 2 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
+    s->h_pub[0] = s->h_pub[1] = 0;
+    HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_pub, s->h_pub, 0));
     if (N > 0) {
         ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
         MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
@@ -1195,7 +1232,7 @@ struct StepRunner {
     bool fused() const { return substeps(); }
 
     // form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state)
-    void predict(int i, double dt) {
+    void predict(int i, double dt, unsigned flags) {
         const int st = s->desc.stages;
         const bool fin = i > st;
         const int n_prev = fin ? st : i - 1;
@@ -1208,7 +1245,7 @@ struct StepRunner {
             c[j] = dt * at;
             if (at != 0.0) use |= 1u << j;
         }
-        ops.predict(s, Ks, c, n_prev, use, fin ? s->cand : s->Y[i - 1]);
+        ops.predict(s, a, Ks, c, n_prev, use, fin ? s->cand : s->Y[i - 1], flags);
     }
     hipError_t read_changed(unsigned long long& n) {
         hcc_ctx* ctx = s->ctx;
@@ -1218,52 +1255,43 @@ struct StepRunner {
         n = ctx->h_pinned[0];
         return e;
     }
-    // One iteration of the fused fixed point; every count it needs is read on the device, so iterations queue
-    // without a host round trip, and one enqueued after the fixed point is reached does nothing.
-    void mood_iteration(const double* Yp, double* Kp, double h) {
+    // One iteration of the fused fixed point. Every count it needs is read on the device, so iterations queue without
+    // a host round trip, and an iteration enqueued after the fixed point is reached does nothing. Its first kernel
+    // publishes the count it starts from into pinned slot q, and event evm[q] follows that kernel.
+    hipError_t mood_iteration(const double* Yp, double* Kp, double h, int q) {
         hipStream_t str = s->ctx->stream;
-        const unsigned g = (unsigned)std::min<int64_t>(nb, 512);
-        k_mark<<<g, kBlock, 0, str>>>(a);
-        ops.gather_list(s, a, Yp);
+        ops.mark_gather(s, a, Yp, s->d_pub + q);
+        hipError_t e = hipEventRecord(s->evm[q], str);
         k_list_dirty<<<(unsigned)nb, kBlock, 0, str>>>(a);
         ops.flux_list(s, a, Yp, Kp, h);
-    }
-    // snapshot of the changed count into pinned slot q (4 or 5) and its event
-    hipError_t snapshot(int q) {
-        hcc_ctx* ctx = s->ctx;
-        hipError_t e = hipMemcpyAsync(ctx->h_pinned + q, s->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(s->evm[q - 4], ctx->stream);
         return e;
     }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
     // flux pass; every recompute of K_{i-1} on the dirty set re-checks exactly that set. The candidate Y_i is
-    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}. The host keeps one
-    // iteration queued beyond the one whose count it waits for, so the device does not idle on the round trip.
+    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}. Iteration j publishes the
+    // count n_{j-1} its predecessor left; while the host waits for it, the rest of iteration j runs, and iteration
+    // j + 1 is enqueued only if n_{j-1} > 0 (if n_j turns out 0, iteration j + 1 does nothing).
     hipError_t checkpoint(int i, double dt) {
-        if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt); return hipSuccess; }
+        if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt, 0); return hipSuccess; }
         const double* Yp = s->Y[i - 2];
         double* Kp = s->K[i - 2];
         const double h = s->desc.substep_ratio * dt;
         hipStream_t str = s->ctx->stream;
         if (fused()) {
-            hipError_t e = snapshot(4);                  // count of the full pass's check
+            hipError_t e = mood_iteration(Yp, Kp, h, 0);
             if (e != hipSuccess) return e;
-            mood_iteration(Yp, Kp, h);
-            if ((e = snapshot(5)) != hipSuccess) return e;
             for (int k = 0;; ++k) {
-                const int q = 4 + (k & 1);
-                if ((e = hipEventSynchronize(s->evm[q - 4])) != hipSuccess) return e;
-                if (s->ctx->h_pinned[q] == 0) break;     // the queued iteration k + 1 finds nothing to do
-                mood_iteration(Yp, Kp, h);               // iteration k + 2
-                if ((e = snapshot(q)) != hipSuccess) return e;
+                const int q = k & 1;
+                if ((e = hipEventSynchronize(s->evm[q])) != hipSuccess) return e;
+                if (*(volatile unsigned long long*)&s->h_pub[q] == 0) break;             // n_k = 0: iteration k + 1 found nothing to do
+                if ((e = mood_iteration(Yp, Kp, h, q ^ 1)) != hipSuccess) return e;
             }
-            k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
-            predict(i, dt);
+            predict(i, dt, PRED_COUNT);
             return hipGetLastError();
         }
         int full = 1;
         for (;;) {
-            predict(i, dt);
+            predict(i, dt, 0);
             k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
             ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], full);
             unsigned long long n = 0;
@@ -1281,22 +1309,18 @@ struct StepRunner {
     }
     hipError_t run(double dt) {
         hcc_ctx* ctx = s->ctx;
-        const int64_t N = s->N;
         const int st = s->desc.stages;
         hipStream_t str = ctx->stream;
-        nb = nblocks(N);
-        // gradients: every full gather pass writes each interior particle's record; ghosts keep the zeros written at
-        // set-up (the reference's gather skips boundary particles)
-        k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>((uint8_t*)s->orders, 2, N);
-        k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, str>>>(s->trig, 0, N);
-        k_reset_counters<<<1, 64, 0, str>>>(s->counters, 1);
+        nb = nblocks(s->N);
+        // The first predictor resets orders, flags and counters (PRED_RESET). Gradients: every full gather pass writes
+        // each interior particle's record; ghosts keep the zeros written at set-up (the reference's gather skips
+        // boundary particles). The full gather also clears the changed count for the check of the flux pass.
         for (int i = 1; i <= st; ++i) {
-            if (i == 1) predict(1, dt);
+            if (i == 1) predict(1, dt, PRED_RESET);
             else { hipError_t e = checkpoint(i, dt); if (e != hipSuccess) return e; }
             (void)hipEventRecord(s->evp[4 * (i - 1) + 0], str);
             ops.gather(s, a, s->Y[i - 1], 1);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 1], str);
-            if (fused()) k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 2], str);
             ops.flux(s, a, s->Y[i - 1], s->K[i - 1], 1, fused(), s->desc.substep_ratio * dt);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 3], str);
