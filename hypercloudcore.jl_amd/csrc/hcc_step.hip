@@ -176,28 +176,40 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
     }
     if (i >= N) return;
     if (flags & PRED_RESET) { a.orders[i] = 2; a.trig[i] = 0; }
-    double r[L::PU];
-    load_rec<L::PU, L::PU>(Un, i, r);
+    // Only the 16-byte pairs of the record that hold state components are read and written: every state buffer
+    // keeps the node positions written at set-up (a pair shared by the last position and the first state component
+    // is rewritten with the same position).
+    constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1;
     const bool ghost = a.g.isb[i];
     const int64_t src = ghost ? (int64_t)root[i] : i;
-    if (src >= 0) {
-        double s[L::NR];
-        if (ghost) load_rec<L::NR, L::PU>(Un, src, s);
-        else {
+    double r[2 * NP];
+    const double2* own = reinterpret_cast<const double2*>(Un + i * L::PU) + P0;
 #pragma unroll
-            for (int q = 0; q < L::NR; ++q) s[q] = r[q];
+    for (int q = 0; q < NP; ++q) { const double2 v = own[q]; r[2 * q] = v.x; r[2 * q + 1] = v.y; }
+    if (src >= 0) {
+        double sv[2 * NP];
+        if (ghost) {
+            const double2* sp = reinterpret_cast<const double2*>(Un + src * L::PU) + P0;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) { const double2 v = sp[q]; sv[2 * q] = v.x; sv[2 * q + 1] = v.y; }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2 * NP; ++q) sv[q] = r[q];
         }
         const double* Ks[5] = {K0, K1, K2, K3, K4};
         const double cs[5] = {c0, c1, c2, c3, c4};
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            double y = s[D + m];
+            const int q = D + m - 2 * P0;
+            double y = sv[q];
             for (int j = 0; j < nterm; ++j)
                 if ((use >> j) & 1u) y = y + cs[j] * Ks[j][src * M + m];
-            r[D + m] = y;
+            r[q] = y;
         }
     }
-    store_rec<L::PU, L::PU>(target, i, r);
+    double2* dst = reinterpret_cast<double2*>(target + i * L::PU) + P0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
 // ============================================================================================================
@@ -275,9 +287,18 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 g[q * M + m] = c * invL;
             }
         if (a.limiter != HCC_LIM_NONE) {
-            double phi[M];
+            // per component, for either bound (local max when the reconstruction rises, min when it falls): the
+            // bound minus U_i and the tolerance 16 eps max(|U_i|, |bound|), the same operations hoisted out of the loop
+            double phi[M], dup[M], ddn[M], tup[M], tdn[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m) phi[m] = 1.0;
+            for (int m = 0; m < M; ++m) {
+                phi[m] = 1.0;
+                const double ui = ri[D + m];
+                dup[m] = umax[m] - ui;
+                ddn[m] = umin[m] - ui;
+                tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
+                tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
+            }
             int32_t k1 = 0;
             double pn[D];
             if constexpr (Src::kPrefetch) {
@@ -305,10 +326,10 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                     double rec = 0.0;
 #pragma unroll
                     for (int q = 0; q < D; ++q) rec = rec + g[q * M + m] * pint[q];
-                    const double bound = rec > 0.0 ? umax[m] : umin[m];
-                    const double tol = 16 * kEps * jmax(fabs(ri[D + m]), fabs(bound));
+                    const bool up = rec > 0.0;
+                    const double tol = up ? tup[m] : tdn[m];
                     if (fabs(rec) > tol) {
-                        const double r = (bound - ri[D + m]) / rec;
+                        const double r = (up ? dup[m] : ddn[m]) / rec;
                         phi[m] = jmin(phi[m], a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r));
                     }
                 }
@@ -945,6 +966,13 @@ struct hcc_stepper {
     hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
+    // hcc_step returns without waiting for the device; its statistics (counters copied to h_stats, events) are read
+    // when the next step starts or a getter asks, and summed into the totals
+    bool pending = false;
+    int64_t host_iters = 0;                // fixed-point iterations counted on the host (stored-candidate path)
+    unsigned long long* h_stats = nullptr; // pinned copy of the step counters
+    int64_t tot_steps = 0, tot_iters = 0;
+    double tot_ms = 0, tot_gather_ms = 0, tot_flux_ms = 0;
     StepArgs args() const;
 };
 
@@ -1068,6 +1096,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
     for (auto& e : s->evm) if (e) (void)hipEventDestroy(e);
     if (s->h_pub) (void)hipHostFree(s->h_pub);
+    if (s->h_stats) (void)hipHostFree(s->h_stats);
     delete s;
     return HCC_OK;
 }
@@ -1145,9 +1174,13 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     for (auto& e : s->evm) HCC_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 2 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
     s->h_pub[0] = s->h_pub[1] = 0;
+    HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_stats, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_pub, s->h_pub, 0));
     if (N > 0) {
         ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
+        ops.pack(s.get(), s->scratch, s->cand);   // every state buffer carries the positions (the predictor
+        for (int k = 0; k < d->stages; ++k)       // writes state components only)
+            ops.pack(s.get(), s->scratch, s->Y[k]);
         MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
         const int64_t nb = nblocks(N);
         if (D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
@@ -1333,25 +1366,14 @@ struct StepRunner {
 };
 }  // namespace
 
-extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
-    (void)t;
-    if (!s) return HCC_E_ARG;
+// Waits for the last step's work (if not yet read) and folds its statistics into the step fields and the totals.
+static int finish_pending(hcc_stepper* s) {
+    if (!s->pending) return HCC_OK;
     hcc_ctx* ctx = s->ctx;
-    if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
-    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
-    if (s->N == 0) return HCC_OK;
-    StepRunner r;
-    r.s = s;
-    find_ops(s->D, s->M, s->kind, r.ops);
-    r.a = s->args();
-    HCC_TRY_HIP(ctx, hipEventRecord(s->ev0, ctx->stream));
-    hipError_t e = r.run(dt);
-    if (e != hipSuccess) { ctx->err = std::string("hcc_step: ") + hipGetErrorString(e); return HCC_E_HIP; }
-    HCC_TRY_HIP(ctx, hipEventRecord(s->ev1, ctx->stream));
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    s->inadmissible = (int64_t)ctx->h_pinned[1];
-    s->mood_iters = r.iters + (int64_t)ctx->h_pinned[4];
+    HCC_TRY_HIP(ctx, hipEventSynchronize(s->ev1));
+    s->pending = false;
+    s->inadmissible = (int64_t)s->h_stats[1];
+    s->mood_iters = s->host_iters + (int64_t)s->h_stats[4];
     float ms = 0;
     HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
@@ -1363,6 +1385,34 @@ extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
         s->last_gather_ms += gm;
         s->last_flux_ms += fm;
     }
+    s->tot_steps += 1;
+    s->tot_iters += s->mood_iters;
+    s->tot_ms += s->last_ms;
+    s->tot_gather_ms += s->last_gather_ms;
+    s->tot_flux_ms += s->last_flux_ms;
+    return HCC_OK;
+}
+
+extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
+    (void)t;
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (s->N == 0) return HCC_OK;
+    int rc = finish_pending(s);
+    if (rc) return rc;
+    StepRunner r;
+    r.s = s;
+    find_ops(s->D, s->M, s->kind, r.ops);
+    r.a = s->args();
+    HCC_TRY_HIP(ctx, hipEventRecord(s->ev0, ctx->stream));
+    hipError_t e = r.run(dt);
+    if (e != hipSuccess) { ctx->err = std::string("hcc_step: ") + hipGetErrorString(e); return HCC_E_HIP; }
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(s->h_stats, s->counters, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipEventRecord(s->ev1, ctx->stream));
+    s->host_iters = r.iters;
+    s->pending = true;
     return HCC_OK;
 }
 
@@ -1388,12 +1438,14 @@ extern "C" int hcc_get_mood_flags(hcc_stepper* s, uint8_t* flags) {
 
 extern "C" int hcc_get_inadmissible(hcc_stepper* s, int64_t* count) {
     if (!s || !count) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
     *count = s->inadmissible;
     return HCC_OK;
 }
 
 extern "C" int hcc_get_mood_iterations(hcc_stepper* s, int64_t* count) {
     if (!s || !count) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
     *count = s->mood_iters;
     return HCC_OK;
 }
@@ -1467,8 +1519,22 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
 
 extern "C" int hcc_last_step_timing(hcc_stepper* s, double* ms_total, double* ms_gather, double* ms_flux) {
     if (!s) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
     if (ms_total) *ms_total = s->last_ms;
     if (ms_gather) *ms_gather = s->last_gather_ms;
     if (ms_flux) *ms_flux = s->last_flux_ms;
+    return HCC_OK;
+}
+
+extern "C" int hcc_step_totals(hcc_stepper* s, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
+                               double* ms_gather, double* ms_flux) {
+    if (!s) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
+    if (steps) *steps = s->tot_steps;
+    if (mood_iterations) *mood_iterations = s->tot_iters;
+    if (ms_total) *ms_total = s->tot_ms;
+    if (ms_gather) *ms_gather = s->tot_gather_ms;
+    if (ms_flux) *ms_flux = s->tot_flux_ms;
+    if (reset) { s->tot_steps = s->tot_iters = 0; s->tot_ms = s->tot_gather_ms = s->tot_flux_ms = 0; }
     return HCC_OK;
 }

@@ -25,7 +25,7 @@ EXPORTS = [
     "hcc_rng_uniform", "hcc_grid_create", "hcc_grid_destroy", "hcc_grid_num_slots", "hcc_grid_get_neighbors",
     "hcc_grid_get_bins", "hcc_grid_get_positions", "hcc_stepper_create", "hcc_stepper_destroy", "hcc_state_upload",
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
-    "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing",
+    "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
         "hcc_get_moments": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "hcc_rhs": (C.c_int, [vp, vp, vp, vp]),
         "hcc_last_step_timing": (C.c_int, [vp, P(d), P(d), P(d)]),
+        "hcc_step_totals": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(d), P(d), P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -316,6 +317,14 @@ class Stepper:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(lib().hcc_last_step_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.ctx.h, "hcc_last_step_timing")
         return a.value, b.value, c.value
+
+    def totals(self, reset=True):
+        """(steps, MOOD iterations, step ms, full gather-pass ms, full flux-pass ms) summed since the last reset"""
+        n, it = C.c_int64(), C.c_int64()
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(lib().hcc_step_totals(self.h, int(reset), C.byref(n), C.byref(it), C.byref(a), C.byref(b), C.byref(c)),
+              self.ctx.h, "hcc_step_totals")
+        return n.value, it.value, a.value, b.value, c.value
 
     def close(self):
         if getattr(self, "h", None):

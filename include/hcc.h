@@ -136,8 +136,14 @@ int hcc_get_moments(hcc_stepper* st, double* inv_grad, uint8_t* ok_grad, double*
  * K (N*M) and the limited gradients (N*D*M, node-major then basis term then component). */
 int hcc_rhs(hcc_stepper* st, const double* Y, double* K, double* gradients);
 /* Timing of the last hcc_step on the context stream (HIP events), milliseconds: the whole step, and the summed
- * durations of its full-domain gather (gradient + limiter) and flux passes, one of each per stage. */
+ * durations of its full-domain gather (gradient + limiter) and flux passes, one of each per stage.
+ * hcc_step returns once its work is enqueued; this call, the diagnostics above and hcc_step_totals wait for it. */
 int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, double* ms_flux);
+/* Sums over the steps since the last reset (reset != 0 clears them after reading): steps, MOOD fixed-point
+ * iterations, and the HIP-event milliseconds of hcc_last_step_timing. No counterpart in the reference: it lets a
+ * driver time a run of steps without a host round trip per step (benchmark/step.jl times its loop as a whole). */
+int hcc_step_totals(hcc_stepper* st, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
+                    double* ms_gather, double* ms_flux);
 
 #ifdef __cplusplus
 }
