@@ -71,6 +71,22 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_euler2d_1M.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the full flux pass from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (calibrated against a plain device copy); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes k_flux_staged ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +176,7 @@ def main():
                    "nodes": N, "interior_nodes": N_int, "mean_neighbours": round(nnb, 3), "stages": stages,
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_flux<2,4,Euler> (full pass)", "bytes_per_launch": fb,
                      "avg_launch_ms": flux_avg_ms},
         "detail": {"gather_avg_ms": gather_avg_ms, "gather_bytes": gb, "flux_avg_ms": flux_avg_ms,

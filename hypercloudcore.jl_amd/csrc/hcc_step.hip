@@ -196,16 +196,28 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
 #pragma unroll
             for (int q = 0; q < 2 * NP; ++q) sv[q] = r[q];
         }
-        const double* Ks[5] = {K0, K1, K2, K3, K4};
-        const double cs[5] = {c0, c1, c2, c3, c4};
+        // terms unrolled with compile-time indices: the K rows are plain global loads (a runtime-indexed pointer
+        // array would be demoted to flat loads, one per component and term, each waited for)
+        const double* Ks[kMaxStages] = {K0, K1, K2, K3, K4};
+        const double cs[kMaxStages] = {c0, c1, c2, c3, c4};
+        double y[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int q = D + m - 2 * P0;
-            double y = sv[q];
-            for (int j = 0; j < nterm; ++j)
-                if ((use >> j) & 1u) y = y + cs[j] * Ks[j][src * M + m];
-            r[q] = y;
+        for (int m = 0; m < M; ++m) y[m] = sv[D + m - 2 * P0];
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j) {
+            if (j < nterm && ((use >> j) & 1u)) {
+                double kv[M];
+                if constexpr (M % 2 == 0) load_rec<M, M>(Ks[j], src, kv);
+                else {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) kv[m] = Ks[j][src * M + m];
+                }
+#pragma unroll
+                for (int m = 0; m < M; ++m) y[m] = y[m] + cs[j] * kv[m];
+            }
         }
+#pragma unroll
+        for (int m = 0; m < M; ++m) r[D + m - 2 * P0] = y[m];
     }
     double2* dst = reinterpret_cast<double2*>(target + i * L::PU) + P0;
 #pragma unroll
