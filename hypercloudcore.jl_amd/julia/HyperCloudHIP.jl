@@ -147,6 +147,16 @@ function mood_inadmissible(ts::HIPRKTimeStepper)
     return Int(n[])
 end
 
-export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc
+# Steps, MOOD iterations and HIP-event milliseconds (step, gather passes, flux passes) since the last reset.
+function step_totals(ts::HIPRKTimeStepper; reset::Bool = true)
+    n, it = Ref{Int64}(0), Ref{Int64}(0)
+    a, b, c = Ref{Float64}(0), Ref{Float64}(0), Ref{Float64}(0)
+    checkrc(ccall((:hcc_step_totals, libhcc), Cint,
+                  (Ptr{Cvoid}, Cint, Ref{Int64}, Ref{Int64}, Ref{Float64}, Ref{Float64}, Ref{Float64}),
+                  ts.st, Cint(reset), n, it, a, b, c), ts.ctx)
+    return (steps = Int(n[]), mood_iterations = Int(it[]), ms_step = a[], ms_gather = b[], ms_flux = c[])
+end
+
+export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals
 
 end # module

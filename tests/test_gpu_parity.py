@@ -134,3 +134,60 @@ def test_euler2d_256_last_step(ctx):
     wl = W.euler_2d(n=256, seed=1)
     g, st, o = build_pair(ctx, wl)
     _run_steps(st, o, wl["cfl"], 3, check_each=False)
+
+
+def test_euler2d_full_size_step_bit_exact(ctx):
+    """The benchmark workload itself (h = 1/1000, 1 013 475 nodes, LDS-staged passes): three GPU steps evolve the
+    four-quadrant state until MOOD fires, then one more step from that state on both sides must agree bit for bit
+    (dt, state, orders, flags, inadmissible count). The oracle runs a single step, so the test stays in seconds."""
+    wl = W.euler_2d(h=1e-3, seed=0)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    t = 0.0
+    for _ in range(3):
+        dt = wl["cfl"] * st.time_step()
+        st.step(t, dt)
+        t += dt
+    U = st.download()
+    from oracle import oracle as O
+    o = O.OracleSim(wl["domain"], c["pos"], c["is_boundary"], c["tags"], wl["scheme"], threads=16)
+    o.set_state(U)
+    dt_g, dt_o = st.time_step(), o.time_step()
+    assert dt_g == dt_o
+    dt = wl["cfl"] * dt_g
+    st.step(t, dt)
+    o.step(t, dt)
+    f, f_o = st.mood_flags(), o.mood_flags()
+    assert f.sum() > 0, "MOOD did not fire at full size"
+    assert np.array_equal(f, f_o)
+    assert np.array_equal(st.orders(), o.orders())
+    assert st.inadmissible() == o.inadmissible()
+    U1, U1_o = st.download(), o.state()
+    assert rel_linf(U1, U1_o) <= STATE_RTOL
+    assert np.array_equal(U1, U1_o), first_diff(U1, U1_o)
+    rho, p = U1[:, 0], (1.4 - 1.0) * (U1[:, 3] - 0.5 * (U1[:, 1] ** 2 + U1[:, 2] ** 2) / U1[:, 0])
+    assert np.all(rho > 0) and np.all(p > 0)
+
+
+def test_step_totals_match_per_step_values(ctx):
+    """hcc_step returns once its work is queued; the per-step getters and the summed totals see the same steps."""
+    wl = W.euler_2d(n=64, seed=2)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    st.totals(reset=True)
+    t, iters, ms = 0.0, 0, 0.0
+    for _ in range(6):
+        dt = wl["cfl"] * st.time_step()
+        st.step(t, dt)
+        t += dt
+        iters += st.mood_iterations()
+        ms += st.timing()[0]
+    n, it, tot_ms, g_ms, f_ms = st.totals(reset=True)
+    assert n == 6 and it == iters
+    assert abs(tot_ms - ms) <= 1e-6 * max(ms, 1.0)
+    assert 0 < g_ms < tot_ms and 0 < f_ms < tot_ms
+    assert st.totals(reset=False)[0] == 0
