@@ -41,10 +41,13 @@ HCC_HD double mls_weight(double alpha, double inv_range_sq, double d2) {
 
 // Venkatakrishnan / Barth-Jespersen phi (src/Interpolation/Limiter.jl:35-45)
 HCC_HD double vk_phi(double r) {
-    double r_pos = jmax(0.0, r);
-    double s = 1.0 / jmax(1.0, r_pos);
-    if (r_pos > 1.0) return (1.0 + 2.0 * s) / (1.0 + s + 2.0 * (s * s));
-    return (r_pos * r_pos + 2.0 * r_pos) / (r_pos * r_pos + r_pos + 2.0);
+    // both branches' numerator and denominator are formed and one quotient taken: the same operations, no branch
+    const double r_pos = jmax(0.0, r);
+    const double s = 1.0 / jmax(1.0, r_pos);
+    const bool large = r_pos > 1.0;
+    const double num = large ? 1.0 + 2.0 * s : r_pos * r_pos + 2.0 * r_pos;
+    const double den = large ? 1.0 + s + 2.0 * (s * s) : r_pos * r_pos + r_pos + 2.0;
+    return num / den;
 }
 HCC_HD double bj_phi(double r) { return jmin(1.0, r); }
 

@@ -333,16 +333,26 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 double pint[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) pint[d] = 0.5 * dd[d];
+                // The components are evaluated without branches (a skipped slot's quotient is computed and discarded;
+                // a zero reconstruction divides to an unused inf or NaN), so that the four division chains of a slot
+                // sit in one basic block and interleave; a slot no lane limits is skipped as a whole.
+                double rec[M];
+                bool lim[M], any = false;
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
-                    double rec = 0.0;
+                    double v = 0.0;
 #pragma unroll
-                    for (int q = 0; q < D; ++q) rec = rec + g[q * M + m] * pint[q];
-                    const bool up = rec > 0.0;
-                    const double tol = up ? tup[m] : tdn[m];
-                    if (fabs(rec) > tol) {
-                        const double r = (up ? dup[m] : ddn[m]) / rec;
-                        phi[m] = jmin(phi[m], a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r));
+                    for (int q = 0; q < D; ++q) v = v + g[q * M + m] * pint[q];
+                    rec[m] = v;
+                    lim[m] = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
+                    any = any || lim[m];
+                }
+                if (any) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const double r = (rec[m] > 0.0 ? dup[m] : ddn[m]) / rec[m];
+                        const double pk = a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r);
+                        phi[m] = lim[m] ? jmin(phi[m], pk) : phi[m];
                     }
                 }
             }
