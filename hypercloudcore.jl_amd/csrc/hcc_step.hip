@@ -152,30 +152,16 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
 // ============================================================================================================
 // A boundary particle with an outflow donor takes the stage value of its interior root (the same combination,
 // bit-identical); other boundary particles keep pg.rhos (= U^n during the step). Records keep each node's position.
-// Fused with the predictor: PRED_RESET (first stage of a step) resets the particle orders to the maximum order, the
-// step's MOOD flags and the step counters; PRED_COUNT (after a MOOD fixed point) counts the accepted inadmissible
-// candidates of the checkpoint (_count_inadmissible, TimeSteppers.jl:481-490), one atomic per wavefront that has any.
-constexpr unsigned PRED_RESET = 1u, PRED_COUNT = 2u;
 template <int D, int M>
 __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, const int32_t* __restrict__ root,
                                                      const double* __restrict__ Un, const double* K0, const double* K1,
                                                      const double* K2, const double* K3, const double* K4, double c0,
                                                      double c1, double c2, double c3, double c4, int nterm, unsigned use,
-                                                     double* __restrict__ target, unsigned flags) {
+                                                     double* __restrict__ target) {
     using L = Layout<D, M>;
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
-    if ((flags & PRED_RESET) && blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counters[0] = 0; a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0; a.counters[5] = 0;
-    }
-    if (flags & PRED_COUNT) {
-        const bool bad = i < N && !a.g.isb[i] && a.inadm[i];
-        const unsigned long long ball = __ballot(bad);
-        if (ball && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1))
-            atomicAdd(&a.counters[1], (unsigned long long)__popcll(ball));
-    }
     if (i >= N) return;
-    if (flags & PRED_RESET) { a.orders[i] = 2; a.trig[i] = 0; }
     // Only the 16-byte pairs of the record that hold state components are read and written: every state buffer
     // keeps the node positions written at set-up (a pair shared by the last position and the first state component
     // is rewritten with the same position).
@@ -366,10 +352,21 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
 }
 
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
+// reset (first stage of a step): orders back to the maximum order and the step's MOOD flags and counters cleared
+// (TimeSteppers.jl:531-540), each thread for its own node before the gather reads that node's order.
+__device__ __forceinline__ void step_reset(const StepArgs& a, int64_t i) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0; a.counters[5] = 0;
+    }
+    if (i < a.g.N) { a.orders[i] = 2; a.trig[i] = 0; }
+}
+
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int64_t nblk) {
+__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
+                                                    int64_t nblk) {
     if (full && blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
     const int64_t i = node_index(nblk);
+    if (reset) step_reset(a, i);
     if (i >= a.g.N) return;
     if (a.g.isb[i] || !(full || a.changed[i])) return;
     gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
@@ -377,15 +374,84 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
 
 // The full gather pass with the workgroup's halo staged in LDS (2-D).
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int64_t nblk) {
+__global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
+                                                           int64_t nblk) {
     extern __shared__ double sh[];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
-    const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
     const int64_t i = lb * kBlock + threadIdx.x;
+    if (reset) step_reset(a, i);
+    const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
     if (i >= a.g.N || a.g.isb[i]) return;
     gather_node<D, M, KIND>(a, Y, src, i);
+}
+
+// The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
+// :400-421): Y_i(n) = U^n(n) + sum_j (dt a_ij) K_j(n) needs only node n's data, and its last term is the K this pass
+// has just computed for n. A recompute of K (MOOD) recomputes Y_i(n) with it. Outflow ghosts take their interior
+// root's value: the thread of root n writes the ghosts listed for it (gidx[goff[n] .. goff[n+1])); ghosts without a
+// root keep pg.rhos, which never changes, in every state buffer. count: the pass keeps the accepted-inadmissible
+// count of the checkpoint (_count_inadmissible, TimeSteppers.jl:481-490) up to date.
+struct PredArgs {
+    const double* Un;
+    const double* K[kMaxStages];
+    double c[kMaxStages];
+    int nterm;                 // terms; the last one is the K of this pass
+    unsigned use;              // terms with a non-zero tableau entry
+    double* target;
+    const int32_t* goff;
+    const int32_t* gidx;
+    int on, count;
+};
+
+template <int D, int M>
+__device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, const double (&Kn)[M]) {
+    using L = Layout<D, M>;
+    constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
+    double r[2 * NP];
+    const double2* own = reinterpret_cast<const double2*>(p.Un + n * L::PU) + P0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) { const double2 v = own[q]; r[2 * q] = v.x; r[2 * q + 1] = v.y; }
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) {
+        if (j < p.nterm && ((p.use >> j) & 1u)) {
+            double kv[M];
+            if (j == p.nterm - 1) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) kv[m] = Kn[m];
+            } else if constexpr (M % 2 == 0) {
+                load_rec<M, M>(p.K[j], n, kv);
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; ++m) kv[m] = p.K[j][n * M + m];
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) r[O + m] = r[O + m] + p.c[j] * kv[m];
+        }
+    }
+    double2* dst = reinterpret_cast<double2*>(p.target + n * L::PU) + P0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    for (int32_t e = p.goff[n], e1 = p.goff[n + 1]; e < e1; ++e) {
+        const int64_t g = p.gidx[e];
+        if constexpr (O == 0) {
+            double2* gd = reinterpret_cast<double2*>(p.target + g * L::PU) + P0;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) gd[q] = make_double2(r[2 * q], r[2 * q + 1]);
+        } else {                                   // the first pair holds the ghost's own last coordinate
+#pragma unroll
+            for (int m = 0; m < M; ++m) p.target[g * L::PU + D + m] = r[O + m];
+        }
+    }
+}
+
+// Adds the wavefront's change of the accepted-inadmissible count (delta per lane in -1..1) to counters[1]; one
+// atomic per wavefront that has a change (rare).
+__device__ __forceinline__ void add_inadmissible(const StepArgs& a, int delta) {
+    const unsigned long long up = __ballot(delta > 0), dn = __ballot(delta < 0);
+    if ((up | dn) && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1))
+        atomicAdd(&a.counters[1], (unsigned long long)((long long)__popcll(up) - (long long)__popcll(dn)));
 }
 
 // ============================================================================================================
@@ -400,8 +466,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
 // decrement (_mood_drop!) is applied by k_mark_gather: neighbours read this node's order in this very pass.
 // ============================================================================================================
 template <int D, int M, int KIND, bool CHECK, class Src>
-__device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __restrict__ Y, const Src& src,
-                                          double* __restrict__ K, int64_t i, double h) {
+__device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
+                                          const Src& src, double* __restrict__ K, int64_t i, double h) {
     using P = Pde<D, M, KIND>;
     using L = Layout<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = D + 1;
@@ -532,6 +598,7 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const double* __res
     }
 #pragma unroll
     for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
+    if (pred.on) fused_predict<D, M>(pred, i, Kout);
     if (!CHECK) return false;
     double W[M];
 #pragma unroll
@@ -569,27 +636,32 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
 }
 
 template <int D, int M, int KIND, bool CHECK>
-__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, const double* __restrict__ Y, double* __restrict__ K, int full,
-                                                  double h, int64_t nblk) {
+__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                  double* __restrict__ K, int full, double h, int64_t nblk) {
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
     bool ch = false;
+    int bad = 0;
     if (i < N) {
         if (a.g.isb[i]) {
             if (full)
 #pragma unroll
                 for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else if (full || a.dirty[i]) {
-            ch = flux_node<D, M, KIND, CHECK>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+            if (CHECK) bad = a.inadm[i];
         }
     }
-    if (CHECK) append_changed(a, ch, i);
+    if (CHECK) {
+        append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, bad);
+    }
 }
 
 // The full flux pass with the workgroup's halo (records, gradients, orders) staged in LDS (2-D).
 template <int D, int M, int KIND, bool CHECK>
-__global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
-                                                         double h, int64_t nblk) {
+__global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                         double* __restrict__ K, double h, int64_t nblk) {
     extern __shared__ double sh[];
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
@@ -597,34 +669,45 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, const double
     const int64_t i = lb * kBlock + threadIdx.x;
     const int64_t N = a.g.N;
     bool ch = false;
+    int bad = 0;
     if (i < N) {
         if (a.g.isb[i]) {
 #pragma unroll
             for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else {
-            ch = flux_node<D, M, KIND, CHECK>(a, Y, src, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, src, K, i, h);
+            if (CHECK) bad = a.inadm[i];
         }
     }
-    if (CHECK) append_changed(a, ch, i);
+    if (CHECK) {
+        append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, bad);
+    }
 }
 
 // The flux pass (with the MOOD check) over the dense dirty list of the fixed point (dlist, counters[5], read on the
 // device): one thread per dirty particle, so the recompute fills whole wavefronts; each wavefront's entries are the
 // dirty particles of one block in index order. Dirty marks are cleared for the next iteration.
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, const double* __restrict__ Y, double* __restrict__ K,
-                                                       double h) {
+__global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                       double* __restrict__ K, double h) {
     const int64_t n = (int64_t)a.counters[5];
     for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
         const int64_t t = t0 + threadIdx.x;
         bool ch = false;
         int64_t i = 0;
+        int delta = 0;
         if (t < n) {
             i = a.dlist[t];
             a.dmark[i] = 0;
-            if (!a.g.isb[i]) ch = flux_node<D, M, KIND, true>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+            if (!a.g.isb[i]) {
+                const int before = a.inadm[i];
+                ch = flux_node<D, M, KIND, true>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+                delta = (int)a.inadm[i] - before;
+            }
         }
         append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, delta);
     }
 }
 
@@ -965,6 +1048,8 @@ struct hcc_stepper {
     double* invd = nullptr; uint8_t* okd = nullptr;
     double* dw = nullptr;
     int32_t* root = nullptr;
+    int32_t *goff = nullptr, *gidx = nullptr;   // outflow ghosts of each interior root (CSR over nodes)
+    bool fresh = true;         // U was (re)loaded: the first stage state is formed by the predictor once
     // states (records) and right-hand sides (node-major)
     double* U = nullptr;       // pg.rhos (U^n during a step)
     double* cand = nullptr;    // rho_cand
@@ -1023,24 +1108,26 @@ StepArgs hcc_stepper::args() const {
 template <int D, int M, int KIND>
 struct Variant {
     static hipStream_t st(const hcc_stepper* s) { return s->ctx->stream; }
-    static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full) {
+    static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full, int reset) {
         const int64_t nb = nblocks(s->N);
-        if (full && s->lds_gather) k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, nb);
-        else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, nb);
+        if (full && s->lds_gather)
+            k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
+        else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub) {
         k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub);
     }
-    static void flux(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, int full, bool check, double h) {
+    static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
+                     bool check, double h) {
         const int64_t nb = nblocks(s->N);
         if (full && s->lds_flux) {
-            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, Y, K, h, nb);
-            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, Y, K, 0.0, nb);
-        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, h, nb);
-        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, K, full, 0.0, nb);
+            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, h, nb);
+            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, 0.0, nb);
+        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, h, nb);
+        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, 0.0, nb);
     }
-    static void flux_list(const hcc_stepper* s, const StepArgs& a, const double* Y, double* K, double h) {
-        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, K, h);
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, double h) {
+        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, p, Y, K, h);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
         const int64_t nb = nblocks(s->N);
@@ -1057,10 +1144,10 @@ struct Variant {
         k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters);
     }
     static void predict(const hcc_stepper* s, const StepArgs& a, const double* const* Ks, const double* c, int nterm,
-                        unsigned use, double* target, unsigned flags) {
+                        unsigned use, double* target) {
         const int64_t nb = nblocks(s->N);
         k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, Ks[0], Ks[1], Ks[2], Ks[3], Ks[4], c[0],
-                                                             c[1], c[2], c[3], c[4], nterm, use, target, flags);
+                                                             c[1], c[2], c[3], c[4], nterm, use, target);
     }
     static void pack(const hcc_stepper* s, const double* U_aos, double* rec) {
         const hcc_grid* g = s->grid;
@@ -1073,13 +1160,13 @@ struct Variant {
 };
 
 struct VariantOps {
-    void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int);
+    void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int, int);
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*);
-    void (*flux)(const hcc_stepper*, const StepArgs&, const double*, double*, int, bool, double);
-    void (*flux_list)(const hcc_stepper*, const StepArgs&, const double*, double*, double);
+    void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool, double);
+    void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, double);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
     void (*cfl)(const hcc_stepper*, const StepArgs&);
-    void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*, unsigned);
+    void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
     void (*pack)(const hcc_stepper*, const double*, double*);
     void (*unpack)(const hcc_stepper*, const double*, double*);
     int PU, GS;
@@ -1107,7 +1194,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     if (!s) return HCC_OK;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
-    for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->U,
+    for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->goff, (void*)s->gidx, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->bmark, (void*)s->clist, (void*)s->dlist,
                     (void*)s->lam, (void*)s->bmin, (void*)s->counters})
@@ -1224,6 +1311,21 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
                       d->bc_kind, root);
         for (auto& r : root) if (r == -2) r = 0;   // interior entries are never read
         if (N) HCC_TRY_HIP(ctx, hipMemcpy(s->root, root.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+        // the ghosts of each root, for the predictor fused into the flux passes
+        std::vector<int32_t> goff((size_t)N + 1, 0), gidx;
+        for (int64_t i = 0; i < N; ++i)
+            if (g->h_isb[i] && root[i] >= 0) ++goff[(size_t)root[i] + 1];
+        for (int64_t i = 0; i < N; ++i) goff[(size_t)i + 1] += goff[(size_t)i];
+        gidx.resize((size_t)std::max<int32_t>(goff[(size_t)N], 1));
+        {
+            std::vector<int32_t> cur(goff.begin(), goff.end() - 1);
+            for (int64_t i = 0; i < N; ++i)
+                if (g->h_isb[i] && root[i] >= 0) gidx[(size_t)cur[(size_t)root[i]]++] = (int32_t)i;
+        }
+        HCC_TRY_HIP(ctx, dalloc(&s->goff, N + 1));
+        HCC_TRY_HIP(ctx, dalloc(&s->gidx, (int64_t)gidx.size()));
+        HCC_TRY_HIP(ctx, hipMemcpy(s->goff, goff.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
+        HCC_TRY_HIP(ctx, hipMemcpy(s->gidx, gidx.data(), sizeof(int32_t) * gidx.size(), hipMemcpyHostToDevice));
     }
     HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
     *out = s.release();
@@ -1238,7 +1340,11 @@ extern "C" int hcc_state_upload(hcc_stepper* s, const double* U) {
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, U, sizeof(double) * s->N * s->M, hipMemcpyHostToDevice, ctx->stream));
+    // every state buffer: ghosts without an outflow root keep pg.rhos in all stages and are never written again
     ops.pack(s, s->scratch, s->U);
+    ops.pack(s, s->scratch, s->cand);
+    for (int k = 0; k < s->desc.stages; ++k) ops.pack(s, s->scratch, s->Y[k]);
+    s->fresh = true;
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HCC_OK;
 }
@@ -1285,22 +1391,46 @@ struct StepRunner {
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
     // MOOD checks the Euler substep: the check is fused into the flux pass that produces K (flux_node with CHECK)
     bool fused() const { return substeps(); }
+    // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself
+    bool fuse_pred() const { return fused() || s->desc.mood_criterion == HCC_CRIT_NONE; }
+    bool y1_in_U = false;
+    // state of stage k (1-based): U itself once the boundary conditions hold on it (every step after the first since
+    // the state was loaded), else the predictor's copy in Y[0]
+    const double* stage_state(int k) const { return k == 1 ? (y1_in_U ? s->U : s->Y[0]) : s->Y[k - 1]; }
 
-    // form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state)
-    void predict(int i, double dt, unsigned flags) {
+    // the terms of form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state)
+    void terms(int i, double dt, const double** Ks, double* c, int& n_prev, unsigned& use, double*& target) const {
         const int st = s->desc.stages;
         const bool fin = i > st;
-        const int n_prev = fin ? st : i - 1;
-        const double* Ks[5] = {s->K[0], s->K[0], s->K[0], s->K[0], s->K[0]};
-        double c[5] = {0, 0, 0, 0, 0};
-        unsigned use = 0;
+        n_prev = fin ? st : i - 1;
+        use = 0;
+        for (int j = 0; j < kMaxStages; ++j) { Ks[j] = s->K[0]; c[j] = 0.0; }
         for (int j = 0; j < n_prev; ++j) {
             Ks[j] = s->K[j];
             const double at = fin ? s->desc.b_t[j] : s->desc.a_t[i - 1][j];
             c[j] = dt * at;
             if (at != 0.0) use |= 1u << j;
         }
-        ops.predict(s, a, Ks, c, n_prev, use, fin ? s->cand : s->Y[i - 1], flags);
+        target = fin ? s->cand : s->Y[i - 1];
+    }
+    void predict(int i, double dt) {
+        const double* Ks[kMaxStages];
+        double c[kMaxStages];
+        int n_prev;
+        unsigned use;
+        double* target;
+        terms(i, dt, Ks, c, n_prev, use, target);
+        ops.predict(s, a, Ks, c, n_prev, use, target);
+    }
+    PredArgs pred_args(int i, double dt, bool count) const {
+        PredArgs p{};
+        terms(i, dt, p.K, p.c, p.nterm, p.use, p.target);
+        p.Un = s->U;
+        p.goff = s->goff;
+        p.gidx = s->gidx;
+        p.on = 1;
+        p.count = count ? 1 : 0;
+        return p;
     }
     hipError_t read_changed(unsigned long long& n) {
         hcc_ctx* ctx = s->ctx;
@@ -1313,40 +1443,42 @@ struct StepRunner {
     // One iteration of the fused fixed point. Every count it needs is read on the device, so iterations queue without
     // a host round trip, and an iteration enqueued after the fixed point is reached does nothing. Its first kernel
     // publishes the count it starts from into pinned slot q, and event evm[q] follows that kernel.
-    hipError_t mood_iteration(const double* Yp, double* Kp, double h, int q) {
+    hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, double h, int q) {
         hipStream_t str = s->ctx->stream;
         ops.mark_gather(s, a, Yp, s->d_pub + q);
         hipError_t e = hipEventRecord(s->evm[q], str);
         k_list_dirty<<<(unsigned)nb, kBlock, 0, str>>>(a);
-        ops.flux_list(s, a, Yp, Kp, h);
+        ops.flux_list(s, a, p, Yp, Kp, h);
         return e;
     }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
-    // flux pass; every recompute of K_{i-1} on the dirty set re-checks exactly that set. The candidate Y_i is
-    // formed once the fixed point is reached: the check reads only Y_{i-1} and K_{i-1}. Iteration j publishes the
-    // count n_{j-1} its predecessor left; while the host waits for it, the rest of iteration j runs, and iteration
-    // j + 1 is enqueued only if n_{j-1} > 0 (if n_j turns out 0, iteration j + 1 does nothing).
+    // flux pass, which also formed the candidate Y_i and counted the accepted inadmissible candidates; every
+    // recompute of K_{i-1} on the dirty set re-checks exactly that set and re-forms its Y_i (the check reads only
+    // Y_{i-1} and K_{i-1}). Iteration j publishes the count n_{j-1} its predecessor left; while the host waits for it,
+    // the rest of iteration j runs, and iteration j + 1 is enqueued only if n_{j-1} > 0 (if n_j turns out 0,
+    // iteration j + 1 does nothing).
     hipError_t checkpoint(int i, double dt) {
-        if (s->desc.mood_criterion == HCC_CRIT_NONE) { predict(i, dt, 0); return hipSuccess; }
-        const double* Yp = s->Y[i - 2];
+        if (s->desc.mood_criterion == HCC_CRIT_NONE) return hipSuccess;   // Y_i formed by the flux pass
+        const double* Yp = stage_state(i - 1);
         double* Kp = s->K[i - 2];
         const double h = s->desc.substep_ratio * dt;
         hipStream_t str = s->ctx->stream;
         if (fused()) {
-            hipError_t e = mood_iteration(Yp, Kp, h, 0);
+            const PredArgs p = pred_args(i, dt, true);
+            hipError_t e = mood_iteration(p, Yp, Kp, h, 0);
             if (e != hipSuccess) return e;
             for (int k = 0;; ++k) {
                 const int q = k & 1;
                 if ((e = hipEventSynchronize(s->evm[q])) != hipSuccess) return e;
-                if (*(volatile unsigned long long*)&s->h_pub[q] == 0) break;             // n_k = 0: iteration k + 1 found nothing to do
-                if ((e = mood_iteration(Yp, Kp, h, q ^ 1)) != hipSuccess) return e;
+                if (*(volatile unsigned long long*)&s->h_pub[q] == 0) break;   // n_k = 0: iteration k + 1 does nothing
+                if ((e = mood_iteration(p, Yp, Kp, h, q ^ 1)) != hipSuccess) return e;
             }
-            predict(i, dt, PRED_COUNT);
             return hipGetLastError();
         }
+        const PredArgs off{};
         int full = 1;
         for (;;) {
-            predict(i, dt, 0);
+            predict(i, dt);
             k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
             ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], full);
             unsigned long long n = 0;
@@ -1355,8 +1487,8 @@ struct StepRunner {
             if (n == 0) break;
             ++iters;
             k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
-            ops.gather(s, a, Yp, 0);
-            ops.flux(s, a, Yp, Kp, 0, false, 0.0);
+            ops.gather(s, a, Yp, 0, 0);
+            ops.flux(s, a, off, Yp, Kp, 0, false, 0.0);
             full = 0;
         }
         k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
@@ -1367,22 +1499,30 @@ struct StepRunner {
         const int st = s->desc.stages;
         hipStream_t str = ctx->stream;
         nb = nblocks(s->N);
-        // The first predictor resets orders, flags and counters (PRED_RESET). Gradients: every full gather pass writes
-        // each interior particle's record; ghosts keep the zeros written at set-up (the reference's gather skips
-        // boundary particles). The full gather also clears the changed count for the check of the flux pass.
+        y1_in_U = !s->fresh;
+        // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
+        // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
+        // particles). The full gather also clears the changed count for the check of the flux pass.
         for (int i = 1; i <= st; ++i) {
-            if (i == 1) predict(1, dt, PRED_RESET);
-            else { hipError_t e = checkpoint(i, dt); if (e != hipSuccess) return e; }
+            if (i == 1) {
+                if (!y1_in_U) predict(1, dt);   // Y_1 = pg.rhos with the boundary conditions applied
+            } else {
+                hipError_t e = checkpoint(i, dt);
+                if (e != hipSuccess) return e;
+            }
+            const double* Y = stage_state(i);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 0], str);
-            ops.gather(s, a, s->Y[i - 1], 1);
+            ops.gather(s, a, Y, 1, i == 1);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 1], str);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 2], str);
-            ops.flux(s, a, s->Y[i - 1], s->K[i - 1], 1, fused(), s->desc.substep_ratio * dt);
+            const PredArgs p = fuse_pred() ? pred_args(i + 1, dt, fused()) : PredArgs{};
+            ops.flux(s, a, p, Y, s->K[i - 1], 1, fused(), s->desc.substep_ratio * dt);
             (void)hipEventRecord(s->evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1, dt);
         if (e != hipSuccess) return e;
-        std::swap(s->U, s->cand);        // pg.rhos <- rho_cand
+        std::swap(s->U, s->cand);        // pg.rhos <- rho_cand (its ghosts hold their roots' values)
+        s->fresh = false;
         return hipGetLastError();
     }
 };
@@ -1525,8 +1665,8 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     const StepArgs a = s->args();
     k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, st>>>(s->G, 0.0, (int64_t)s->GS * N);
     k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, st>>>((uint8_t*)s->orders, 2, N);
-    ops.gather(s, a, s->Y[0], 1);
-    ops.flux(s, a, s->Y[0], s->K[0], 1, false, 0.0);
+    ops.gather(s, a, s->Y[0], 1, 0);
+    ops.flux(s, a, PredArgs{}, s->Y[0], s->K[0], 1, false, 0.0);
     HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipMemcpyAsync(K, s->K[0], sizeof(double) * N * M, hipMemcpyDeviceToHost, st));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
