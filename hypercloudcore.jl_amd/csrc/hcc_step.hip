@@ -94,7 +94,9 @@ __device__ __forceinline__ void lds_read(const double* p, double (&r)[K]) {
 template <int D, int M, bool GRADS>
 struct LdsSrc {
     using L = Layout<D, M>;
-    static constexpr bool kPrefetch = false;    // LDS latency is short: load at the top of each slot
+    // the flux loop loads slot k+1 (and the 16-bit index of slot k+2, which comes from HBM/L2) while slot k
+    // computes; the gather's loops, with more waves per SIMD, load at the top of each slot
+    static constexpr bool kPrefetch = GRADS;
     static constexpr int GO = lds_grad_offset(L::NR);
     static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
     const double* sh;          // halo node l at sh + l * ST
@@ -237,12 +239,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         for (int q = 0; q < D; ++q)
 #pragma unroll
             for (int m = 0; m < M; ++m) b[q][m] = 0.0;
-        int32_t j1 = 0;
+        int32_t j1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
         double rn[NR];
-        if constexpr (Src::kPrefetch) {
-            j1 = src_slot(src, base, 1, cnt);
-            src.template head<NR>(src_slot(src, base, 0, cnt), rn);
-        }
+        if constexpr (Src::kPrefetch) src.template head<NR>(src_slot(src, base, 0, cnt), rn);
         for (int k = 0; k < cnt; ++k) {
             double rj[NR];
             if constexpr (Src::kPrefetch) {
@@ -251,8 +250,10 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 const int32_t jn = j1;
                 j1 = src_slot(src, base, k + 2, cnt);
                 src.template head<NR>(jn, rn);
-            } else {
-                src.template head<NR>(src.idx(base + (int64_t)k * kChunk), rj);
+            } else {   // the index of slot k+1 (from HBM/L2) loads while slot k computes; records come from LDS
+                const int32_t j = j1;
+                j1 = src_slot(src, base, k + 1, cnt);
+                src.template head<NR>(j, rj);
             }
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
@@ -297,12 +298,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
                 tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
             }
-            int32_t k1 = 0;
+            int32_t k1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
             double pn[D];
-            if constexpr (Src::kPrefetch) {
-                k1 = src_slot(src, base, 1, cnt);
-                src.template head<D>(src_slot(src, base, 0, cnt), pn);
-            }
+            if constexpr (Src::kPrefetch) src.template head<D>(src_slot(src, base, 0, cnt), pn);
             for (int k = 0; k < cnt; ++k) {
                 double pj[D];
                 if constexpr (Src::kPrefetch) {
@@ -312,7 +310,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                     k1 = src_slot(src, base, k + 2, cnt);
                     src.template head<D>(jn, pn);
                 } else {
-                    src.template head<D>(src.idx(base + (int64_t)k * kChunk), pj);
+                    const int32_t j = k1;
+                    k1 = src_slot(src, base, k + 1, cnt);
+                    src.template head<D>(j, pj);
                 }
                 double dd[D];
                 rec_geom<D>(a.g, ri, pj, dd);
@@ -924,23 +924,16 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
             }
         };
         double sum_c = 0.0;
-        int32_t j1 = 0;
+        int32_t j1 = nidx(1);
         double pn[D], ln[D];
-        if (!STAGED) {
-            j1 = nidx(1);
-            fetch(nidx(0), pn, ln);
-        }
-        for (int k = 0; k < cnt; ++k) {
+        fetch(nidx(0), pn, ln);
+        for (int k = 0; k < cnt; ++k) {   // slot k+1 loaded (index k+2) while slot k computes
             double pj[D], lj[D];
-            if (!STAGED) {
 #pragma unroll
-                for (int d = 0; d < D; ++d) { pj[d] = pn[d]; lj[d] = ln[d]; }
-                const int32_t jn = j1;
-                j1 = nidx(k + 2);
-                fetch(jn, pn, ln);
-            } else {
-                fetch(nidx(k), pj, lj);
-            }
+            for (int d = 0; d < D; ++d) { pj[d] = pn[d]; lj[d] = ln[d]; }
+            const int32_t jn = j1;
+            j1 = nidx(k + 2);
+            fetch(jn, pn, ln);
             double dd[D];
             const double d2 = rec_geom<D>(a.g, pi, pj, dd);
             const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
