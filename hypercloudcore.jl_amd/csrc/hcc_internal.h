@@ -71,6 +71,7 @@ struct GeomArgs {
     const int32_t* halo_idx;
     const uint16_t* nbr_local;  // SELL-64 like nbr: the neighbour's position in the block's halo
     int hmax;                   // largest halo of any block
+    int bslots;                 // largest SELL slot range of any block (its chunks are consecutive)
 };
 constexpr int kStage = 256;     // nodes per staged workgroup
 
@@ -141,7 +142,7 @@ struct hcc_grid {
     int32_t* d_halo_off = nullptr;
     int32_t* d_halo_idx = nullptr;
     uint16_t* d_nbr_local = nullptr;
-    int64_t max_halo = 0;
+    int64_t max_halo = 0, max_block_slots = 0;
     std::vector<int32_t> h_tags;
     std::vector<uint8_t> h_isb;
     hcc::GeomArgs args() const;

@@ -280,6 +280,7 @@ hcc::GeomArgs hcc_grid::args() const {
     for (int d = 0; d < 3; ++d) { a.Lw[d] = geom.Lw[d]; a.invLw[d] = geom.invLw[d]; }
     a.invL = geom.invL; a.alpha = geom.desc.alpha; a.inv_range_sq = geom.inv_range_sq;
     a.halo_off = d_halo_off; a.halo_idx = d_halo_idx; a.nbr_local = d_nbr_local; a.hmax = (int)max_halo;
+    a.bslots = (int)max_block_slots;
     return a;
 }
 
@@ -422,6 +423,10 @@ extern "C" int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_
             HCC_TRY_HIP(ctx, hipMemcpy(g->d_halo_off, hoff.data(), sizeof(int32_t) * hoff.size(), hipMemcpyHostToDevice));
             HCC_TRY_HIP(ctx, hipMemcpy(g->d_halo_idx, hidx.data(), sizeof(int32_t) * hidx.size(), hipMemcpyHostToDevice));
             HCC_TRY_HIP(ctx, hipMemcpy(g->d_nbr_local, nl.data(), sizeof(uint16_t) * nl.size(), hipMemcpyHostToDevice));
+            const int64_t cpb = kStage / kChunk, nst = (N + kStage - 1) / kStage;   // chunks per staged block
+            for (int64_t b = 0; b < nst; ++b)
+                g->max_block_slots = std::max<int64_t>(g->max_block_slots,
+                                                       off[(size_t)std::min<int64_t>((b + 1) * cpb, g->nchunks)] - off[(size_t)(b * cpb)]);
         } else {
             g->max_halo = 0;   // a halo too large for 16-bit local indices: the passes read neighbours from HBM
         }

@@ -79,6 +79,10 @@ struct HbmSrc {
 // aligned so that a neighbour costs a few ds_read_b128 at one base address; the orders follow as bytes.
 constexpr int lds_grad_offset(int NR) { return (NR + 1) & ~1; }
 constexpr int lds_stride(int NR, int NG) { return (lds_grad_offset(NR) + NG + 1) & ~1; }
+// uint16 offset of the staged index table: after hm records of st doubles and, with orders, hm bytes; 16-byte aligned
+__host__ __device__ constexpr size_t lds_index_offset(int64_t hm, int st, bool orders) {
+    return ((size_t)hm * st * 8 + (orders ? (size_t)hm : 0) + 15) / 16 * 8;
+}
 
 template <int K>
 __device__ __forceinline__ void lds_read(const double* p, double (&r)[K]) {
@@ -94,15 +98,16 @@ __device__ __forceinline__ void lds_read(const double* p, double (&r)[K]) {
 template <int D, int M, bool GRADS>
 struct LdsSrc {
     using L = Layout<D, M>;
-    // the flux loop loads slot k+1 (and the 16-bit index of slot k+2, which comes from HBM/L2) while slot k
-    // computes; the gather's loops, with more waves per SIMD, load at the top of each slot
+    // the flux loop loads slot k+1 while slot k computes; the gather's loops, with more waves per SIMD, load the
+    // next index ahead and the records at the top of each slot
     static constexpr bool kPrefetch = GRADS;
     static constexpr int GO = lds_grad_offset(L::NR);
     static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
     const double* sh;          // halo node l at sh + l * ST
     const int8_t* ord;         // orders of the halo nodes (GRADS)
-    const uint16_t* nl;
-    __device__ __forceinline__ int32_t idx(int64_t s) const { return nl[s]; }
+    const uint16_t* nl;        // the block's local indices in LDS: SELL slot s at nl[s - s0]
+    int64_t s0;
+    __device__ __forceinline__ int32_t idx(int64_t s) const { return nl[s - s0]; }
     template <int K>
     __device__ __forceinline__ void head(int32_t l, double (&r)[K]) const { lds_read<K>(sh + l * ST, r); }
     __device__ __forceinline__ void grad(int32_t l, double (&g)[L::NG]) const { lds_read<L::NG>(sh + l * ST + GO, g); }
@@ -130,7 +135,15 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
     using S = LdsSrc<D, M, GRADS>;
     const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
     int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
-    for (int t = threadIdx.x; t < nh; t += kBlock) {
+    // the block's SELL slot range (its chunks are consecutive; each chunk holds 64 x width slots, so the range is
+    // 128-byte aligned and a multiple of 64 entries), copied 8 indices at a time
+    uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST, GRADS);
+    const int64_t cpb = kStage / kChunk, nch = (a.g.N + kChunk - 1) / kChunk;
+    const int64_t s0 = a.g.chunk_off[lb * cpb], s1 = a.g.chunk_off[(lb + 1) * cpb < nch ? (lb + 1) * cpb : nch];
+    const uint4* isrc = reinterpret_cast<const uint4*>(a.g.nbr_local + s0);
+    uint4* idst = reinterpret_cast<uint4*>(nl);
+    for (int64_t t = threadIdx.x; t < (s1 - s0) / 8; t += kStage) idst[t] = isrc[t];
+    for (int t = threadIdx.x; t < nh; t += kStage) {
         const int32_t j = a.g.halo_idx[h0 + t];
         double r[L::NR];
         load_rec<L::NR, L::PU>(Y, j, r);
@@ -146,7 +159,7 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
         }
     }
     __syncthreads();
-    return S{sh, ord, a.g.nbr_local};
+    return S{sh, ord, nl, s0};
 }
 
 // ============================================================================================================
@@ -1266,8 +1279,9 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         const size_t hm = (size_t)g->max_halo;
         const int NR = D + (int)M, NG = D * (int)M;
         auto fit = [&](size_t bytes) { return bytes <= (size_t)lds_max ? bytes : 0; };
-        s->lds_gather = fit((size_t)lds_stride(NR, 0) * hm * sizeof(double));
-        s->lds_flux = fit((size_t)lds_stride(NR, NG) * hm * sizeof(double) + hm);
+        const size_t idx_bytes = (size_t)g->max_block_slots * sizeof(uint16_t);
+        s->lds_gather = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0), false) + idx_bytes);
+        s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
         s->lds_cfl = fit(2 * (size_t)D * hm * sizeof(double));
     }
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
