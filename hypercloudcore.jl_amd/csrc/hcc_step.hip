@@ -79,6 +79,9 @@ struct HbmSrc {
 // aligned so that a neighbour costs a few ds_read_b128 at one base address; the orders follow as bytes.
 constexpr int lds_grad_offset(int NR) { return (NR + 1) & ~1; }
 constexpr int lds_stride(int NR, int NG) { return (lds_grad_offset(NR) + NG + 1) & ~1; }
+// staged record of the CFL estimate: position and wave speeds, padded to an even count of doubles
+template <int D>
+constexpr int kCflStride = (2 * D + 1) & ~1;
 // uint16 offset of the staged index table: after hm records of st doubles and, with orders, hm bytes; 16-byte aligned
 __host__ __device__ constexpr size_t lds_index_offset(int64_t hm, int st, bool orders) {
     return ((size_t)hm * st * 8 + (orders ? (size_t)hm : 0) + 15) / 16 * 8;
@@ -126,6 +129,20 @@ __device__ __forceinline__ int64_t logical_block(int64_t nblk) {
     return (b & 7) * per + (b >> 3);
 }
 
+static_assert(kStage == kBlock, "staged passes launch one workgroup per halo block");
+
+// Copies block lb's local-index table into LDS (nl) and returns its first SELL slot. The block's chunks are
+// consecutive and each holds 64 x width slots, so the range is 128-byte aligned and a multiple of 64 entries: it is
+// copied 8 indices at a time. The caller synchronises the workgroup before reading it.
+__device__ __forceinline__ int64_t stage_index_table(const GeomArgs& g, uint16_t* nl, int64_t lb) {
+    const int64_t cpb = kStage / kChunk, nch = (g.N + kChunk - 1) / kChunk;
+    const int64_t s0 = g.chunk_off[lb * cpb], s1 = g.chunk_off[(lb + 1) * cpb < nch ? (lb + 1) * cpb : nch];
+    const uint4* isrc = reinterpret_cast<const uint4*>(g.nbr_local + s0);
+    uint4* idst = reinterpret_cast<uint4*>(nl);
+    for (int64_t t = threadIdx.x; t < (s1 - s0) / 8; t += kStage) idst[t] = isrc[t];
+    return s0;
+}
+
 // Loads the halo of workgroup lb into LDS (records, with GRADS also gradient records and orders) and returns the
 // source the neighbour loops read. Coalesced: the halo is sorted by node index.
 template <int D, int M, bool GRADS>
@@ -135,14 +152,8 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
     using S = LdsSrc<D, M, GRADS>;
     const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
     int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
-    // the block's SELL slot range (its chunks are consecutive; each chunk holds 64 x width slots, so the range is
-    // 128-byte aligned and a multiple of 64 entries), copied 8 indices at a time
     uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST, GRADS);
-    const int64_t cpb = kStage / kChunk, nch = (a.g.N + kChunk - 1) / kChunk;
-    const int64_t s0 = a.g.chunk_off[lb * cpb], s1 = a.g.chunk_off[(lb + 1) * cpb < nch ? (lb + 1) * cpb : nch];
-    const uint4* isrc = reinterpret_cast<const uint4*>(a.g.nbr_local + s0);
-    uint4* idst = reinterpret_cast<uint4*>(nl);
-    for (int64_t t = threadIdx.x; t < (s1 - s0) / 8; t += kStage) idst[t] = isrc[t];
+    const int64_t s0 = stage_index_table(a.g, nl, lb);
     for (int t = threadIdx.x; t < nh; t += kStage) {
         const int32_t j = a.g.halo_idx[h0 + t];
         double r[L::NR];
@@ -880,13 +891,17 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
     constexpr int BD = D + 1;
     extern __shared__ double sh[];
     const int64_t N = a.g.N;
-    const int hm = a.g.hmax;
-    int64_t i;
+    constexpr int CST = kCflStride<D>;
+    int64_t i, s0 = 0;
+    const uint16_t* nl = nullptr;
     if (STAGED) {
         const int64_t lb = logical_block(nblk);
         if (lb >= nblk) return;
         const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
-        for (int t = threadIdx.x; t < nh; t += kBlock) {   // positions and wave speeds (_get_Lambda) of the halo
+        uint16_t* nlw = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, CST, false);
+        s0 = stage_index_table(a.g, nlw, lb);
+        nl = nlw;
+        for (int t = threadIdx.x; t < nh; t += kStage) {   // positions and wave speeds (_get_Lambda) of the halo
             const int32_t j = a.g.halo_idx[h0 + t];
             double r[L::NR], f[M], l[D];
             load_rec<L::NR, L::PU>(U, j, r);
@@ -894,10 +909,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
             for (int m = 0; m < M; ++m) f[m] = r[D + m];
             Pde<D, M, KIND>::maxeig(a.pde, f, l);
 #pragma unroll
-            for (int d = 0; d < D; ++d) { sh[d * hm + t] = r[d]; sh[(D + d) * hm + t] = l[d]; }
+            for (int d = 0; d < D; ++d) { sh[t * CST + d] = r[d]; sh[t * CST + D + d] = l[d]; }
         }
         __syncthreads();
-        i = lb * kBlock + threadIdx.x;
+        i = lb * kStage + threadIdx.x;
     } else {
         i = node_index(nblk);
     }
@@ -924,12 +939,14 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
         const double invL = a.g.invL;
         auto nidx = [&](int k) -> int32_t {
             const int kk = k < cnt ? k : cnt - 1;
-            return STAGED ? (int32_t)a.g.nbr_local[base + (int64_t)kk * kChunk] : a.g.nbr[base + (int64_t)kk * kChunk];
+            return STAGED ? (int32_t)nl[base + (int64_t)kk * kChunk - s0] : a.g.nbr[base + (int64_t)kk * kChunk];
         };
         auto fetch = [&](int32_t j, double (&p)[D], double (&l)[D]) {
             if (STAGED) {
+                double v[2 * D];
+                lds_read<2 * D>(sh + j * CST, v);
 #pragma unroll
-                for (int d = 0; d < D; ++d) { p[d] = sh[d * hm + j]; l[d] = sh[(D + d) * hm + j]; }
+                for (int d = 0; d < D; ++d) { p[d] = v[d]; l[d] = v[D + d]; }
             } else {
                 load_rec<D, L::PU>(U, j, p);
 #pragma unroll
@@ -1282,7 +1299,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         const size_t idx_bytes = (size_t)g->max_block_slots * sizeof(uint16_t);
         s->lds_gather = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0), false) + idx_bytes);
         s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
-        s->lds_cfl = fit(2 * (size_t)D * hm * sizeof(double));
+        s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
     }
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
     HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
