@@ -115,6 +115,8 @@ struct StepArgs {
     uint8_t* bmark;              // dirty workgroup blocks
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
     double* bmin;                // per-workgroup minima of the CFL estimate
+    double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima
+    int ext_on;                  // full gather passes store ext, the fused MOOD checks read it (MOODu1)
 };
 
 }  // namespace hcc

@@ -240,7 +240,10 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
 // gather pass: update_content! (MUSCL.jl:351-384), cached reconstruction fit (Interpolators.jl:585-602),
 // Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143)
 // ============================================================================================================
-template <int D, int M, int KIND, class Src>
+// EXT: also store the local extrema of Y over {i} and its neighbours (the limiter's bounds, which are MOODu1's DMP
+// bounds too, _dmp_fails MOOD.jl:175-184) for the checks of the flux pass of the same stage. The neighbour loop then
+// runs for every interior particle; one without a gradient fit discards the fit sums.
+template <int D, int M, int KIND, bool EXT, class Src>
 __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i) {
     using L = Layout<D, M>;
     constexpr int NR = L::NR;
@@ -251,12 +254,12 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
     double g[L::NG];
 #pragma unroll
     for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
-    if (order == 2 && a.okg[i]) {
-        double ri[NR];
+    const bool fit = order == 2 && a.okg[i];
+    const int64_t base = sell_base(a.g, i);
+    const double invL = a.g.invL;
+    double ri[NR], umax[M], umin[M], b[D][M];
+    if (fit || (EXT && a.ext_on)) {
         load_rec<NR, L::PU>(Y, i, ri);
-        const int64_t base = sell_base(a.g, i);
-        const double invL = a.g.invL;
-        double umax[M], umin[M], b[D][M];
 #pragma unroll
         for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; }
 #pragma unroll
@@ -295,6 +298,14 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 umin[m] = jmin(umin[m], fj);
             }
         }
+        if (EXT && a.ext_on) {
+            double e[2 * M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) { e[m] = umax[m]; e[M + m] = umin[m]; }
+            store_rec<2 * M, 2 * M>(a.ext, i, e);
+        }
+    }
+    if (fit) {
         double inv[D][D];
 #pragma unroll
         for (int r = 0; r < D; ++r)
@@ -393,7 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
     if (reset) step_reset(a, i);
     if (i >= a.g.N) return;
     if (a.g.isb[i] || !(full || a.changed[i])) return;
-    gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    if (full) gather_node<D, M, KIND, true>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    else gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
 }
 
 // The full gather pass with the workgroup's halo staged in LDS (2-D).
@@ -408,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (reset) step_reset(a, i);
     const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
     if (i >= a.g.N || a.g.isb[i]) return;
-    gather_node<D, M, KIND>(a, Y, src, i);
+    gather_node<D, M, KIND, true>(a, Y, src, i);
 }
 
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
@@ -499,9 +511,9 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
     const int cnt = a.g.cnt[i];
     double ri[NR];
     load_rec<NR, L::PU>(Y, i, ri);
-    double fi[M], mn[M], mx[M], Kout[M];
+    double fi[M], Kout[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) { fi[m] = ri[D + m]; mn[m] = fi[m]; mx[m] = fi[m]; }
+    for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
     if (cnt == 0) {
 #pragma unroll
         for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
@@ -569,7 +581,6 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
 #pragma unroll
                 for (int q = 1; q < D; ++q) sj = sj + gj[q * M + m] * pjj[q];
                 const double uj = rj[D + m];
-                if (CHECK) { mn[m] = jmin(mn[m], uj); mx[m] = jmax(mx[m], uj); }
                 fij[m] = fi[m] + si;
                 fji[m] = uj + sj;
             }
@@ -634,6 +645,12 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
         if (a.criterion == HCC_CRIT_ONLY) {
             flag = true;
         } else if (a.criterion == HCC_CRIT_U1) {
+            // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
+            // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
+            double e[2 * M], mx[M], mn[M];
+            load_rec<2 * M, 2 * M>(a.ext, i, e);
+#pragma unroll
+            for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
             const double dl = a.delta, d3 = dl * dl * dl;
 #pragma unroll
             for (int m = 0; m < M; ++m)
@@ -760,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double
             a.dmark[j] = 1;
             a.bmark[j / kBlock] = 1;
         }
-        if (!a.g.isb[i]) gather_node<D, M, KIND>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+        if (!a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
     }
 }
 
@@ -1086,6 +1103,7 @@ struct hcc_stepper {
     int32_t *clist = nullptr, *dlist = nullptr;
     double* lam = nullptr;
     double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
+    double* ext = nullptr;     // neighbourhood extrema of the stage state (fused MOODu1 checks)
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
@@ -1124,6 +1142,8 @@ StepArgs hcc_stepper::args() const {
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
     a.counters = counters; a.clist = clist; a.dlist = dlist; a.dmark = dmark; a.bmark = bmark; a.lam = lam;
     a.bmin = bmin;
+    a.ext = ext;
+    a.ext_on = (desc.substep_ratio > 0.0 && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
     return a;
 }
 
@@ -1220,7 +1240,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->goff, (void*)s->gidx, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->bmark, (void*)s->clist, (void*)s->dlist,
-                    (void*)s->lam, (void*)s->bmin, (void*)s->counters})
+                    (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -1282,6 +1302,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->dlist, N));
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
+    HCC_TRY_HIP(ctx, dalloc(&s->ext, 2 * M * N));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
