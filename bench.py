@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-dt", action="store_true",
+                    help="return dt to the host between the CFL estimate and the step (hcc_time_step + hcc_step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for profiles/)")
     args = ap.parse_args()
@@ -128,13 +130,18 @@ def main():
         t += dt
     hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
     st.totals(reset=True)
-    # the timed loop is the driver loop of the reference (get_time_step, then the stepper); the per-step HIP events
-    # are read after it (hcc_step_totals), so timing adds no host round trip per step
+    # the timed loop is the driver loop of the reference: each step is the CFL estimate (get_time_step) and the
+    # stepper. By default both run back to back on the device (hcc_step_cfl: dt = CFL * CFL step kept on the
+    # device); --host-dt returns dt to the host between them (hcc_time_step, hcc_step). The per-step HIP events are
+    # read after the loop (hcc_step_totals), so timing adds no host round trip per step.
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        dt = wl["cfl"] * st.time_step()
-        st.step(t, dt)
-        t += dt
+        if args.host_dt:
+            dt = wl["cfl"] * st.time_step()
+            st.step(t, dt)
+            t += dt
+        else:
+            st.step_cfl(wl["cfl"])
     hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
     wall = time.perf_counter() - t0
     n_steps, iters, tot_ms, tot_g_ms, tot_f_ms = st.totals(reset=True)

@@ -117,6 +117,8 @@ struct StepArgs {
     double* bmin;                // per-workgroup minima of the CFL estimate
     double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima
     int ext_on;                  // full gather passes store ext, the fused MOOD checks read it (MOODu1)
+    const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate)
+    double hratio;               // MOOD substep: h = hratio * dt
 };
 
 }  // namespace hcc

@@ -181,8 +181,8 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
 template <int D, int M>
 __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, const int32_t* __restrict__ root,
                                                      const double* __restrict__ Un, const double* K0, const double* K1,
-                                                     const double* K2, const double* K3, const double* K4, double c0,
-                                                     double c1, double c2, double c3, double c4, int nterm, unsigned use,
+                                                     const double* K2, const double* K3, const double* K4, double a0,
+                                                     double a1, double a2, double a3, double a4, int nterm, unsigned use,
                                                      double* __restrict__ target) {
     using L = Layout<D, M>;
     const int64_t i = node_index(nblk);
@@ -211,7 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
         // terms unrolled with compile-time indices: the K rows are plain global loads (a runtime-indexed pointer
         // array would be demoted to flat loads, one per component and term, each waited for)
         const double* Ks[kMaxStages] = {K0, K1, K2, K3, K4};
-        const double cs[kMaxStages] = {c0, c1, c2, c3, c4};
+        const double dt = *a.dtp;
+        const double cs[kMaxStages] = {dt * a0, dt * a1, dt * a2, dt * a3, dt * a4};
         double y[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) y[m] = sv[D + m - 2 * P0];
@@ -432,12 +433,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
 struct PredArgs {
     const double* Un;
     const double* K[kMaxStages];
-    double c[kMaxStages];
+    double at[kMaxStages];     // tableau entries a_ij (b_j for the final state); the coefficient is dt * a_ij
     int nterm;                 // terms; the last one is the K of this pass
     unsigned use;              // terms with a non-zero tableau entry
     double* target;
     const int32_t* goff;
     const int32_t* gidx;
+    const double* dtp;
     int on, count;
 };
 
@@ -449,9 +451,11 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     const double2* own = reinterpret_cast<const double2*>(p.Un + n * L::PU) + P0;
 #pragma unroll
     for (int q = 0; q < NP; ++q) { const double2 v = own[q]; r[2 * q] = v.x; r[2 * q + 1] = v.y; }
+    const double dt = *p.dtp;
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j) {
         if (j < p.nterm && ((p.use >> j) & 1u)) {
+            const double cj = dt * p.at[j];
             double kv[M];
             if (j == p.nterm - 1) {
 #pragma unroll
@@ -463,7 +467,7 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
                 for (int m = 0; m < M; ++m) kv[m] = p.K[j][n * M + m];
             }
 #pragma unroll
-            for (int m = 0; m < M; ++m) r[O + m] = r[O + m] + p.c[j] * kv[m];
+            for (int m = 0; m < M; ++m) r[O + m] = r[O + m] + cj * kv[m];
         }
     }
     double2* dst = reinterpret_cast<double2*>(p.target + n * L::PU) + P0;
@@ -678,7 +682,8 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
 
 template <int D, int M, int KIND, bool CHECK>
 __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
-                                                  double* __restrict__ K, int full, double h, int64_t nblk) {
+                                                  double* __restrict__ K, int full, int64_t nblk) {
+    const double h = CHECK ? a.hratio * *a.dtp : 0.0;   // MOOD's Euler substep (form_substep!, TimeSteppers.jl:501-518)
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
     bool ch = false;
@@ -702,8 +707,9 @@ __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, cons
 // The full flux pass with the workgroup's halo (records, gradients, orders) staged in LDS (2-D).
 template <int D, int M, int KIND, bool CHECK>
 __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
-                                                         double* __restrict__ K, double h, int64_t nblk) {
+                                                         double* __restrict__ K, int64_t nblk) {
     extern __shared__ double sh[];
+    const double h = CHECK ? a.hratio * *a.dtp : 0.0;
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
     const auto src = stage_halo<D, M, true>(a, Y, sh, lb);
@@ -731,7 +737,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
 // dirty particles of one block in index order. Dirty marks are cleared for the next iteration.
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
-                                                       double* __restrict__ K, double h) {
+                                                       double* __restrict__ K) {
+    const double h = a.hratio * *a.dtp;
     const int64_t n = (int64_t)a.counters[5];
     for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
         const int64_t t = t0 + threadIdx.x;
@@ -1016,7 +1023,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
 }
 
 // dt = min over the workgroup minima (ParticleGrids.jl:386-415 reduces with minimum); +inf if no node contributes
-__global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ bmin, int64_t nblk, unsigned long long* counters) {
+// With dtp: also the step's dt = min(cfl * dt_cfl, cap) (the driver loop's min(CFL * get_time_step, t_end - t),
+// test/time_integration.jl:107-109, with Julia's min), kept on the device for the step that follows.
+__global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ bmin, int64_t nblk, unsigned long long* counters,
+                                                    double* dtp, double cfl, double cap) {
     __shared__ double wmin[16];
     double dt = INFINITY;
     for (int64_t b = threadIdx.x; b < nblk; b += 1024) dt = fmin(dt, bmin[b]);
@@ -1027,7 +1037,12 @@ __global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ b
         double m = wmin[0];
         for (int w = 1; w < 16; ++w) m = fmin(m, wmin[w]);
         counters[2] = (unsigned long long)__double_as_longlong(m);
+        if (dtp) *dtp = jmin(cfl * m, cap);
     }
+}
+
+__global__ void k_set_dt(double* dtp, double dt) {
+    if (threadIdx.x == 0) *dtp = dt;
 }
 
 // ============================================================================================================
@@ -1104,21 +1119,29 @@ struct hcc_stepper {
     double* lam = nullptr;
     double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
     double* ext = nullptr;     // neighbourhood extrema of the stage state (fused MOODu1 checks)
+    double* dtp = nullptr;     // the step's dt on the device
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // statistics of queued steps, read when a getter asks or the ring is full (a new step does not wait)
+    struct StepStats {
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass
+        unsigned long long* h = nullptr;      // pinned copy of the step counters; [7] = dt
+        int64_t host_iters = 0;
+        bool dt_from_device = false;
+    };
+    static constexpr int kRing = 4;
+    StepStats ring[kRing];
+    int ring_head = 0, ring_count = 0;
     hipEvent_t evm[2] = {};                // follow the kernels that publish the MOOD fixed point's counts
     unsigned long long* h_pub = nullptr;   // two count slots in coherent pinned host memory ...
     unsigned long long* d_pub = nullptr;   // ... and their device address
-    hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass of a step
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
-    // hcc_step returns without waiting for the device; its statistics (counters copied to h_stats, events) are read
-    // when the next step starts or a getter asks, and summed into the totals
-    bool pending = false;
-    int64_t host_iters = 0;                // fixed-point iterations counted on the host (stored-candidate path)
-    unsigned long long* h_stats = nullptr; // pinned copy of the step counters
+    // hcc_step returns without waiting for the device; the statistics of its queued steps (ring slots) are read when
+    // a getter asks or the ring is full, and summed into the totals
+    double last_dt = 0;
     int64_t tot_steps = 0, tot_iters = 0;
     double tot_ms = 0, tot_gather_ms = 0, tot_flux_ms = 0;
     StepArgs args() const;
@@ -1144,6 +1167,8 @@ StepArgs hcc_stepper::args() const {
     a.bmin = bmin;
     a.ext = ext;
     a.ext_on = (desc.substep_ratio > 0.0 && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
+    a.dtp = dtp;
+    a.hratio = desc.substep_ratio;
     return a;
 }
 
@@ -1161,22 +1186,22 @@ struct Variant {
         k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub);
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
-                     bool check, double h) {
+                     bool check) {
         const int64_t nb = nblocks(s->N);
         if (full && s->lds_flux) {
-            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, h, nb);
-            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, 0.0, nb);
-        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, h, nb);
-        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, 0.0, nb);
+            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
+            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
+        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
     }
-    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, double h) {
-        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, p, Y, K, h);
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K) {
+        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, p, Y, K);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
         const int64_t nb = nblocks(s->N);
         k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, full, nb);
     }
-    static void cfl(const hcc_stepper* s, const StepArgs& a) {
+    static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
         if (s->lds_cfl) {
             k_cfl<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_cfl, st(s)>>>(a, s->U, nb);
@@ -1184,10 +1209,10 @@ struct Variant {
             k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
             k_cfl<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
         }
-        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters);
+        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters, dtp, cfl, cap);
     }
     static void predict(const hcc_stepper* s, const StepArgs& a, const double* const* Ks, const double* c, int nterm,
-                        unsigned use, double* target) {
+                        unsigned use, double* target) {   // c: tableau entries (scaled by the device dt in the kernel)
         const int64_t nb = nblocks(s->N);
         k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, Ks[0], Ks[1], Ks[2], Ks[3], Ks[4], c[0],
                                                              c[1], c[2], c[3], c[4], nterm, use, target);
@@ -1205,10 +1230,10 @@ struct Variant {
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int, int);
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*);
-    void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool, double);
-    void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, double);
+    void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
+    void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
-    void (*cfl)(const hcc_stepper*, const StepArgs&);
+    void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
     void (*pack)(const hcc_stepper*, const double*, double*);
     void (*unpack)(const hcc_stepper*, const double*, double*);
@@ -1240,15 +1265,17 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->goff, (void*)s->gidx, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->bmark, (void*)s->clist, (void*)s->dlist,
-                    (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->counters})
+                    (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->dtp, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
-    if (s->ev0) (void)hipEventDestroy(s->ev0);
-    if (s->ev1) (void)hipEventDestroy(s->ev1);
-    for (auto& e : s->evp) if (e) (void)hipEventDestroy(e);
+    for (auto& r : s->ring) {
+        if (r.ev0) (void)hipEventDestroy(r.ev0);
+        if (r.ev1) (void)hipEventDestroy(r.ev1);
+        for (auto& e : r.evp) if (e) (void)hipEventDestroy(e);
+        if (r.h) (void)hipHostFree(r.h);
+    }
     for (auto& e : s->evm) if (e) (void)hipEventDestroy(e);
     if (s->h_pub) (void)hipHostFree(s->h_pub);
-    if (s->h_stats) (void)hipHostFree(s->h_stats);
     delete s;
     return HCC_OK;
 }
@@ -1303,6 +1330,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
     HCC_TRY_HIP(ctx, dalloc(&s->ext, 2 * M * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dtp, 2));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
@@ -1322,13 +1350,15 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
         s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
     }
-    HCC_TRY_HIP(ctx, hipEventCreate(&s->ev0));
-    HCC_TRY_HIP(ctx, hipEventCreate(&s->ev1));
-    for (auto& e : s->evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
+    for (auto& r : s->ring) {
+        HCC_TRY_HIP(ctx, hipEventCreate(&r.ev0));
+        HCC_TRY_HIP(ctx, hipEventCreate(&r.ev1));
+        for (auto& e : r.evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
+        HCC_TRY_HIP(ctx, hipHostMalloc((void**)&r.h, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    }
     for (auto& e : s->evm) HCC_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 2 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
     s->h_pub[0] = s->h_pub[1] = 0;
-    HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_stats, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_pub, s->h_pub, 0));
     if (N > 0) {
         ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
@@ -1414,7 +1444,7 @@ extern "C" int hcc_time_step(hcc_stepper* s, double* dt) {
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
-    if (s->N) ops.cfl(s, s->args());
+    if (s->N) ops.cfl(s, s->args(), nullptr, 0.0, 0.0);
     HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, s->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1431,6 +1461,7 @@ struct StepRunner {
     hcc_stepper* s;
     VariantOps ops;
     StepArgs a;
+    hipEvent_t* evp = nullptr;   // the pass events of this step's statistics slot
     int64_t nb = 0;
     int iters = 0;
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
@@ -1443,36 +1474,37 @@ struct StepRunner {
     // the state was loaded), else the predictor's copy in Y[0]
     const double* stage_state(int k) const { return k == 1 ? (y1_in_U ? s->U : s->Y[0]) : s->Y[k - 1]; }
 
-    // the terms of form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state)
-    void terms(int i, double dt, const double** Ks, double* c, int& n_prev, unsigned& use, double*& target) const {
+    // the terms of form_candidate! (TimeSteppers.jl:365-397) for checkpoint i (1-based; i = s + 1 is the final state):
+    // the K buffers and tableau entries; the kernels multiply each entry by the device-resident dt (dt * a_ij)
+    void terms(int i, const double** Ks, double* at, int& n_prev, unsigned& use, double*& target) const {
         const int st = s->desc.stages;
         const bool fin = i > st;
         n_prev = fin ? st : i - 1;
         use = 0;
-        for (int j = 0; j < kMaxStages; ++j) { Ks[j] = s->K[0]; c[j] = 0.0; }
+        for (int j = 0; j < kMaxStages; ++j) { Ks[j] = s->K[0]; at[j] = 0.0; }
         for (int j = 0; j < n_prev; ++j) {
             Ks[j] = s->K[j];
-            const double at = fin ? s->desc.b_t[j] : s->desc.a_t[i - 1][j];
-            c[j] = dt * at;
-            if (at != 0.0) use |= 1u << j;
+            at[j] = fin ? s->desc.b_t[j] : s->desc.a_t[i - 1][j];
+            if (at[j] != 0.0) use |= 1u << j;
         }
         target = fin ? s->cand : s->Y[i - 1];
     }
-    void predict(int i, double dt) {
+    void predict(int i) {
         const double* Ks[kMaxStages];
-        double c[kMaxStages];
+        double at[kMaxStages];
         int n_prev;
         unsigned use;
         double* target;
-        terms(i, dt, Ks, c, n_prev, use, target);
-        ops.predict(s, a, Ks, c, n_prev, use, target);
+        terms(i, Ks, at, n_prev, use, target);
+        ops.predict(s, a, Ks, at, n_prev, use, target);
     }
-    PredArgs pred_args(int i, double dt, bool count) const {
+    PredArgs pred_args(int i, bool count) const {
         PredArgs p{};
-        terms(i, dt, p.K, p.c, p.nterm, p.use, p.target);
+        terms(i, p.K, p.at, p.nterm, p.use, p.target);
         p.Un = s->U;
         p.goff = s->goff;
         p.gidx = s->gidx;
+        p.dtp = s->dtp;
         p.on = 1;
         p.count = count ? 1 : 0;
         return p;
@@ -1488,12 +1520,12 @@ struct StepRunner {
     // One iteration of the fused fixed point. Every count it needs is read on the device, so iterations queue without
     // a host round trip, and an iteration enqueued after the fixed point is reached does nothing. Its first kernel
     // publishes the count it starts from into pinned slot q, and event evm[q] follows that kernel.
-    hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, double h, int q) {
+    hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, int q) {
         hipStream_t str = s->ctx->stream;
         ops.mark_gather(s, a, Yp, s->d_pub + q);
         hipError_t e = hipEventRecord(s->evm[q], str);
         k_list_dirty<<<(unsigned)nb, kBlock, 0, str>>>(a);
-        ops.flux_list(s, a, p, Yp, Kp, h);
+        ops.flux_list(s, a, p, Yp, Kp);
         return e;
     }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
@@ -1502,28 +1534,27 @@ struct StepRunner {
     // Y_{i-1} and K_{i-1}). Iteration j publishes the count n_{j-1} its predecessor left; while the host waits for it,
     // the rest of iteration j runs, and iteration j + 1 is enqueued only if n_{j-1} > 0 (if n_j turns out 0,
     // iteration j + 1 does nothing).
-    hipError_t checkpoint(int i, double dt) {
+    hipError_t checkpoint(int i) {
         if (s->desc.mood_criterion == HCC_CRIT_NONE) return hipSuccess;   // Y_i formed by the flux pass
         const double* Yp = stage_state(i - 1);
         double* Kp = s->K[i - 2];
-        const double h = s->desc.substep_ratio * dt;
         hipStream_t str = s->ctx->stream;
         if (fused()) {
-            const PredArgs p = pred_args(i, dt, true);
-            hipError_t e = mood_iteration(p, Yp, Kp, h, 0);
+            const PredArgs p = pred_args(i, true);
+            hipError_t e = mood_iteration(p, Yp, Kp, 0);
             if (e != hipSuccess) return e;
             for (int k = 0;; ++k) {
                 const int q = k & 1;
                 if ((e = hipEventSynchronize(s->evm[q])) != hipSuccess) return e;
                 if (*(volatile unsigned long long*)&s->h_pub[q] == 0) break;   // n_k = 0: iteration k + 1 does nothing
-                if ((e = mood_iteration(p, Yp, Kp, h, q ^ 1)) != hipSuccess) return e;
+                if ((e = mood_iteration(p, Yp, Kp, q ^ 1)) != hipSuccess) return e;
             }
             return hipGetLastError();
         }
         const PredArgs off{};
         int full = 1;
         for (;;) {
-            predict(i, dt);
+            predict(i);
             k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
             ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], full);
             unsigned long long n = 0;
@@ -1533,13 +1564,13 @@ struct StepRunner {
             ++iters;
             k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
             ops.gather(s, a, Yp, 0, 0);
-            ops.flux(s, a, off, Yp, Kp, 0, false, 0.0);
+            ops.flux(s, a, off, Yp, Kp, 0, false);
             full = 0;
         }
         k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
         return hipGetLastError();
     }
-    hipError_t run(double dt) {
+    hipError_t run() {
         hcc_ctx* ctx = s->ctx;
         const int st = s->desc.stages;
         hipStream_t str = ctx->stream;
@@ -1550,21 +1581,20 @@ struct StepRunner {
         // particles). The full gather also clears the changed count for the check of the flux pass.
         for (int i = 1; i <= st; ++i) {
             if (i == 1) {
-                if (!y1_in_U) predict(1, dt);   // Y_1 = pg.rhos with the boundary conditions applied
+                if (!y1_in_U) predict(1);   // Y_1 = pg.rhos with the boundary conditions applied
             } else {
-                hipError_t e = checkpoint(i, dt);
+                hipError_t e = checkpoint(i);
                 if (e != hipSuccess) return e;
             }
             const double* Y = stage_state(i);
-            (void)hipEventRecord(s->evp[4 * (i - 1) + 0], str);
+            (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
             ops.gather(s, a, Y, 1, i == 1);
-            (void)hipEventRecord(s->evp[4 * (i - 1) + 1], str);
-            (void)hipEventRecord(s->evp[4 * (i - 1) + 2], str);
-            const PredArgs p = fuse_pred() ? pred_args(i + 1, dt, fused()) : PredArgs{};
-            ops.flux(s, a, p, Y, s->K[i - 1], 1, fused(), s->desc.substep_ratio * dt);
-            (void)hipEventRecord(s->evp[4 * (i - 1) + 3], str);
+            (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
+            const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
+            ops.flux(s, a, p, Y, s->K[i - 1], 1, fused());
+            (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
-        hipError_t e = checkpoint(st + 1, dt);
+        hipError_t e = checkpoint(st + 1);
         if (e != hipSuccess) return e;
         std::swap(s->U, s->cand);        // pg.rhos <- rho_cand (its ghosts hold their roots' values)
         s->fresh = false;
@@ -1573,22 +1603,25 @@ struct StepRunner {
 };
 }  // namespace
 
-// Waits for the last step's work (if not yet read) and folds its statistics into the step fields and the totals.
-static int finish_pending(hcc_stepper* s) {
-    if (!s->pending) return HCC_OK;
+// Waits for the oldest queued step's work and folds its statistics into the step fields and the totals.
+static int fold_oldest(hcc_stepper* s) {
     hcc_ctx* ctx = s->ctx;
-    HCC_TRY_HIP(ctx, hipEventSynchronize(s->ev1));
-    s->pending = false;
-    s->inadmissible = (int64_t)s->h_stats[1];
-    s->mood_iters = s->host_iters + (int64_t)s->h_stats[4];
+    auto& r = s->ring[(s->ring_head - s->ring_count + hcc_stepper::kRing) % hcc_stepper::kRing];
+    HCC_TRY_HIP(ctx, hipEventSynchronize(r.ev1));
+    --s->ring_count;
+    std::memcpy(&s->last_dt, r.h + 7, sizeof(double));
+    if (r.dt_from_device && !(s->last_dt > 0 && std::isfinite(s->last_dt)))
+        return set_err(ctx, HCC_E_ARG, "hcc_step_cfl: the CFL step was not positive and finite; the state is invalid");
+    s->inadmissible = (int64_t)r.h[1];
+    s->mood_iters = r.host_iters + (int64_t)r.h[4];
     float ms = 0;
-    HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, r.ev0, r.ev1));
     s->last_ms = ms;
     s->last_gather_ms = s->last_flux_ms = 0;
     for (int i = 0; i < s->desc.stages; ++i) {
         float gm = 0, fm = 0;
-        HCC_TRY_HIP(ctx, hipEventElapsedTime(&gm, s->evp[4 * i + 0], s->evp[4 * i + 1]));
-        HCC_TRY_HIP(ctx, hipEventElapsedTime(&fm, s->evp[4 * i + 2], s->evp[4 * i + 3]));
+        HCC_TRY_HIP(ctx, hipEventElapsedTime(&gm, r.evp[4 * i + 0], r.evp[4 * i + 1]));
+        HCC_TRY_HIP(ctx, hipEventElapsedTime(&fm, r.evp[4 * i + 1], r.evp[4 * i + 3]));
         s->last_gather_ms += gm;
         s->last_flux_ms += fm;
     }
@@ -1600,6 +1633,37 @@ static int finish_pending(hcc_stepper* s) {
     return HCC_OK;
 }
 
+// Folds every queued step (the getters report the last one).
+static int finish_pending(hcc_stepper* s) {
+    while (s->ring_count > 0)
+        if (int rc = fold_oldest(s)) return rc;
+    return HCC_OK;
+}
+
+// Queues one step whose dt is already on the device (dtp), then the copies its statistics are read from.
+static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
+    hcc_ctx* ctx = s->ctx;
+    if (s->ring_count == hcc_stepper::kRing)
+        if (int rc = fold_oldest(s)) return rc;
+    auto& slot = s->ring[s->ring_head];
+    StepRunner r;
+    r.s = s;
+    find_ops(s->D, s->M, s->kind, r.ops);
+    r.a = s->args();
+    r.evp = slot.evp;
+    HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
+    hipError_t e = r.run();
+    if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return HCC_E_HIP; }
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h, s->counters, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h + 7, s->dtp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipEventRecord(slot.ev1, ctx->stream));
+    slot.host_iters = r.iters;
+    slot.dt_from_device = dt_from_device;
+    s->ring_head = (s->ring_head + 1) % hcc_stepper::kRing;
+    ++s->ring_count;
+    return HCC_OK;
+}
+
 extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     (void)t;
     if (!s) return HCC_E_ARG;
@@ -1607,19 +1671,27 @@ extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (s->N == 0) return HCC_OK;
-    int rc = finish_pending(s);
-    if (rc) return rc;
-    StepRunner r;
-    r.s = s;
-    find_ops(s->D, s->M, s->kind, r.ops);
-    r.a = s->args();
-    HCC_TRY_HIP(ctx, hipEventRecord(s->ev0, ctx->stream));
-    hipError_t e = r.run(dt);
-    if (e != hipSuccess) { ctx->err = std::string("hcc_step: ") + hipGetErrorString(e); return HCC_E_HIP; }
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(s->h_stats, s->counters, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HCC_TRY_HIP(ctx, hipEventRecord(s->ev1, ctx->stream));
-    s->host_iters = r.iters;
-    s->pending = true;
+    k_set_dt<<<1, 64, 0, ctx->stream>>>(s->dtp, dt);
+    return launch_step(s, "hcc_step", false);
+}
+
+extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    if (!(cfl > 0) || !std::isfinite(cfl) || !(dt_cap > 0))
+        return set_err(ctx, HCC_E_ARG, "hcc_step_cfl: cfl must be positive and finite, dt_cap positive");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (s->N == 0) return HCC_OK;
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    ops.cfl(s, s->args(), s->dtp, cfl, dt_cap);
+    return launch_step(s, "hcc_step_cfl", true);
+}
+
+extern "C" int hcc_last_dt(hcc_stepper* s, double* dt) {
+    if (!s || !dt) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
+    *dt = s->last_dt;
     return HCC_OK;
 }
 
@@ -1711,7 +1783,7 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, st>>>(s->G, 0.0, (int64_t)s->GS * N);
     k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, st>>>((uint8_t*)s->orders, 2, N);
     ops.gather(s, a, s->Y[0], 1, 0);
-    ops.flux(s, a, PredArgs{}, s->Y[0], s->K[0], 1, false, 0.0);
+    ops.flux(s, a, PredArgs{}, s->Y[0], s->K[0], 1, false);
     HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipMemcpyAsync(K, s->K[0], sizeof(double) * N * M, hipMemcpyDeviceToHost, st));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(st));

@@ -26,6 +26,7 @@ EXPORTS = [
     "hcc_grid_get_bins", "hcc_grid_get_positions", "hcc_stepper_create", "hcc_stepper_destroy", "hcc_state_upload",
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
+    "hcc_step_cfl", "hcc_last_dt",
 ]
 
 
@@ -85,6 +86,8 @@ def lib():
         "hcc_rhs": (C.c_int, [vp, vp, vp, vp]),
         "hcc_last_step_timing": (C.c_int, [vp, P(d), P(d), P(d)]),
         "hcc_step_totals": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(d), P(d), P(d)]),
+        "hcc_step_cfl": (C.c_int, [vp, C.c_double, C.c_double]),
+        "hcc_last_dt": (C.c_int, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -317,6 +320,15 @@ class Stepper:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(lib().hcc_last_step_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), self.ctx.h, "hcc_last_step_timing")
         return a.value, b.value, c.value
+
+    def step_cfl(self, cfl, dt_cap=float("inf")):
+        """One step with dt = min(cfl * CFL step, dt_cap) computed on the device; returns once queued"""
+        check(lib().hcc_step_cfl(self.h, float(cfl), float(dt_cap)), self.ctx.h, "hcc_step_cfl")
+
+    def last_dt(self):
+        v = C.c_double()
+        check(lib().hcc_last_dt(self.h, C.byref(v)), self.ctx.h, "hcc_last_dt")
+        return v.value
 
     def totals(self, reset=True):
         """(steps, MOOD iterations, step ms, full gather-pass ms, full flux-pass ms) summed since the last reset"""

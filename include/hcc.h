@@ -123,6 +123,13 @@ int hcc_state_download(hcc_stepper* st, double* U);
 int hcc_time_step(hcc_stepper* st, double* dt);
 /* Replaces ts(eq, pg, t, dt) (TimeSteppers.jl:531-577): one time step in place on the device state. */
 int hcc_step(hcc_stepper* st, double t, double dt);
+/* The driver loop's step with its time step taken on the device (test/time_integration.jl:107-111):
+ * dt = min(cfl * get_time_step, dt_cap) with Julia's min, then ts(eq, pg, t, dt), queued without a host round trip
+ * (pass dt_cap = t_end - t, or +inf). No counterpart in the reference: it removes the host wait of get_time_step from
+ * device-resident driver loops. hcc_last_dt returns the dt used (waiting for the step); a CFL step that is not
+ * positive and finite is reported by the next call that waits, and the state is then invalid. */
+int hcc_step_cfl(hcc_stepper* st, double cfl, double dt_cap);
+int hcc_last_dt(hcc_stepper* st, double* dt);
 /* Diagnostics of the last step: particle_orders (N), mood_flags (TimeSteppers.jl:258, N bytes),
  * mood_inadmissible (:272), and the number of MOOD recompute iterations. */
 int hcc_get_orders(hcc_stepper* st, int32_t* orders);

@@ -191,3 +191,26 @@ def test_step_totals_match_per_step_values(ctx):
     assert abs(tot_ms - ms) <= 1e-6 * max(ms, 1.0)
     assert 0 < g_ms < tot_ms and 0 < f_ms < tot_ms
     assert st.totals(reset=False)[0] == 0
+
+
+def test_step_cfl_matches_host_driver_loop(ctx):
+    """hcc_step_cfl (dt = min(CFL * CFL step, cap) on the device) gives the same dt and states, bit for bit, as the
+    driver loop with the host-side dt (time_step, then step); the cap binds in the last step."""
+    wl = W.euler_2d(n=64, seed=4)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    a, b = Stepper(ctx, g, wl["scheme"]), Stepper(ctx, g, wl["scheme"])
+    a.upload(wl["U0"])
+    b.upload(wl["U0"])
+    t = 0.0
+    for k in range(6):
+        cap = 1e-5 if k == 5 else float("inf")
+        dt = min(wl["cfl"] * a.time_step(), cap)
+        a.step(t, dt)
+        b.step_cfl(wl["cfl"], cap)
+        assert b.last_dt() == dt
+        t += dt
+        assert np.array_equal(a.mood_flags(), b.mood_flags())
+    Ua, Ub = a.download(), b.download()
+    assert np.array_equal(Ua, Ub), first_diff(Ua, Ub)
+    assert a.inadmissible() == b.inadmissible()
