@@ -25,17 +25,23 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "node-stage updates/s, 2D Euler MUSCL+MOOD @1M nodes; % of HBM roofline"
 
 
-def flux_pass_bytes(N, N_int, S_int, D, M):
-    """Compulsory HBM bytes of one full flux pass of this implementation (neighbour data assumed re-used from
-    cache): per interior node its SELL indices (4 B per slot), count, order, state, gradients, position, the
-    derivative rows of the divergence inverse and the output; boundary nodes write a zero K."""
-    per_node = 4 + 1 + 1 + 8 * M + 8 * D * M + 8 * D + 8 * D * (D + 1) + 8 * M + 1
-    return N_int * per_node + 4 * S_int + (N - N_int) * (8 * M + 1)
+def flux_pass_bytes(N, N_int, S_int, D, M, k_terms=1.0):
+    """Algorithmic HBM bytes of one full flux pass of this implementation (neighbour data read once): per interior
+    node its SELL indices (4 B per slot), count, order, state, gradients, position, the derivative rows of the
+    divergence inverse, the output K and the MOODu1 bounds stored by the gather pass (2M doubles); boundary nodes
+    write a zero K. The fused stage predictor adds per interior node the U^n state, k_terms earlier K rows (1 on
+    average over the three RK3 passes) and the next stage state."""
+    per_node = 4 + 1 + 1 + 8 * M + 8 * D * M + 8 * D + 8 * D * (D + 1) + 8 * M + 1 + 16 * M
+    predictor = 8 * M + k_terms * 8 * M + 8 * M
+    return N_int * (per_node + predictor) + 4 * S_int + (N - N_int) * (8 * M + 1)
 
 
 def gather_pass_bytes(N_int, S_int, D, M):
-    per_node = 4 + 1 + 1 + 8 * M + 8 * D + 8 * D * D + 8 * D * M + 1
+    """Algorithmic HBM bytes of one full gather pass: per interior node its indices, count, order, state, position,
+    the reconstruction inverse, the gradient record and the MOODu1 bounds it stores (2M doubles)."""
+    per_node = 4 + 1 + 1 + 8 * M + 8 * D + 8 * D * D + 8 * D * M + 1 + 16 * M
     return N_int * per_node + 4 * S_int
+
 
 
 def cpu_baseline(wl, steps, threads):
@@ -184,7 +190,7 @@ def main():
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_flux<2,4,Euler> (full pass)", "bytes_per_launch": fb,
+                     "kernel": "k_flux_staged<2,4,Euler> (full pass: flux, MOOD check, stage predictor)", "bytes_per_launch": fb,
                      "avg_launch_ms": flux_avg_ms},
         "detail": {"gather_avg_ms": gather_avg_ms, "gather_bytes": gb, "flux_avg_ms": flux_avg_ms,
                    "step_event_ms": tot_ms / n_steps, "mood_iterations": iters, "flags_last_step": flags,
