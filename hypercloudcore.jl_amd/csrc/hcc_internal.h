@@ -107,12 +107,13 @@ struct StepArgs {
     uint8_t* dirty;
     uint8_t* trig;               // mood_flags of the step
     uint8_t* inadm;              // last check: candidate not admissible
-    unsigned long long* counters;// [0] orders changed in the last check (= clist length), [1] inadmissible accepted,
-                                 // [2] dt bits, [4] fixed-point iterations of the step, [5] dirty list length
-    int32_t* clist;              // particles whose order changed in the last check
-    int32_t* dlist;              // dirty particles of the fixed point, densely
-    uint8_t* dmark;              // dirty particles of the fixed point (changed plus one hop)
-    uint8_t* bmark;              // dirty workgroup blocks
+    unsigned long long* counters;// [0] / [6]: lengths of changed lists 0 / 1 (clist / clist2), [1] inadmissible
+                                 // accepted, [2] dt bits, [4] fixed-point iterations of the step, [5] / [7]: work-list
+                                 // lengths of iterations with parity 0 / 1
+    int32_t* clist;              // particles whose order changed in a check (changed list 0)
+    int32_t* clist2;             // changed list 1
+    int32_t* dlist;              // neighbourhoods of the changed particles (with repeats), the recompute's work list
+    unsigned* dmark;             // dirty particles of the fixed point (changed plus one hop); cleared by the recompute
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
     double* bmin;                // per-workgroup minima of the CFL estimate
     double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima

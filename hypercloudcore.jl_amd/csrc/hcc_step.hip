@@ -392,7 +392,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
 // (TimeSteppers.jl:531-540), each thread for its own node before the gather reads that node's order.
 __device__ __forceinline__ void step_reset(const StepArgs& a, int64_t i) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0; a.counters[5] = 0;
+        a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0;
     }
     if (i < a.g.N) { a.orders[i] = 2; a.trig[i] = 0; }
 }
@@ -400,7 +400,7 @@ __device__ __forceinline__ void step_reset(const StepArgs& a, int64_t i) {
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
                                                     int64_t nblk) {
-    if (full && blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
+    if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t i = node_index(nblk);
     if (reset) step_reset(a, i);
     if (i >= a.g.N) return;
@@ -414,7 +414,7 @@ template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
                                                            int64_t nblk) {
     extern __shared__ double sh[];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;   // for the check of the flux pass that follows
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
     const int64_t i = lb * kBlock + threadIdx.x;
@@ -665,18 +665,32 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
     return false;
 }
 
-// Appends the flagged particles of a wavefront to `clist` (one atomic per wavefront) and counts them.
-__device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64_t i) {
+// The lists of the fused fixed point alternate between two copies, so that no kernel resets a count another
+// workgroup of the same kernel still reads or fills. Iteration k (k = 0, 1, ...) has parity p = k % 2:
+// k_mark_gather reads changed list p and fills work list p (and resets changed and work counts 1 - p), k_flux_list
+// reads work list p and fills changed list 1 - p. The full flux pass fills changed list 0; the full gather before
+// it resets changed and work counts 0. Changed list p: count changed_count(p), particles clist / clist2; work list:
+// one buffer (dlist), count work_count(p).
+__device__ __forceinline__ unsigned long long* changed_count(const StepArgs& a, int p) {
+    return p ? &a.counters[6] : &a.counters[0];
+}
+__device__ __forceinline__ int32_t* changed_list(const StepArgs& a, int p) { return p ? a.clist2 : a.clist; }
+__device__ __forceinline__ unsigned long long* work_count(const StepArgs& a, int p) {
+    return p ? &a.counters[7] : &a.counters[5];
+}
+
+// Appends the flagged particles of a wavefront to changed list p (one atomic per wavefront) and counts them.
+__device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64_t i, int p = 0) {
     const unsigned long long ball = __ballot(ch);
     if (!ball) return;
     const unsigned lane = threadIdx.x & 63;
     const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
     unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
+    if (lane == leader) base = atomicAdd(changed_count(a, p), (unsigned long long)__popcll(ball));
     base = __shfl(base, (int)leader);
     if (ch) {
         const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        a.clist[base + __popcll(below)] = (int32_t)i;
+        changed_list(a, p)[base + __popcll(below)] = (int32_t)i;
     }
 }
 
@@ -732,14 +746,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
     }
 }
 
-// The flux pass (with the MOOD check) over the dense dirty list of the fixed point (dlist, counters[5], read on the
-// device): one thread per dirty particle, so the recompute fills whole wavefronts; each wavefront's entries are the
-// dirty particles of one block in index order. Dirty marks are cleared for the next iteration.
+// The flux pass (with the MOOD check) of the fixed-point iteration with parity p over its work list (dlist, length
+// read on the device): the neighbourhoods of the changed particles, so a particle can appear several times; the
+// thread that clears its dirty mark recomputes it, the others leave. Changes go to changed list 1 - p.
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
-                                                       double* __restrict__ K) {
+                                                       double* __restrict__ K, int p) {
     const double h = a.hratio * *a.dtp;
-    const int64_t n = (int64_t)a.counters[5];
+    const int64_t n = (int64_t)*work_count(a, p);
     for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
         const int64_t t = t0 + threadIdx.x;
         bool ch = false;
@@ -747,68 +761,68 @@ __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred,
         int delta = 0;
         if (t < n) {
             i = a.dlist[t];
-            a.dmark[i] = 0;
-            if (!a.g.isb[i]) {
+            // a particle listed several times (it neighbours several changed particles) is taken by one thread
+            if (atomicExch(&a.dmark[i], 0u) == 1u && !a.g.isb[i]) {
                 const int before = a.inadm[i];
                 ch = flux_node<D, M, KIND, true>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
                 delta = (int)a.inadm[i] - before;
             }
         }
-        append_changed(a, ch, i);
+        append_changed(a, ch, i, 1 - p);
         if (pred.count) add_inadmissible(a, delta);
     }
 }
 
 // _mood_drop! (MOOD.jl:299-309), one hop of expand_mask! (MOOD.jl:323-351) and the gather of the changed list
-// (length counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty, and
-// so is each workgroup block holding one (plain idempotent byte stores, no atomics); then its gradient is recomputed
-// at the new order (the gather reads only the particle's own order, which this thread just wrote, and stage states).
+// (length counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty
+// (idempotent stores) and appended to the recompute's work list (one atomic per wavefront); then its gradient is
+// recomputed at the new order (the gather reads only the particle's own order, which this thread just wrote, and
+// stage states).
 // The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
 // Block 0 also resets the dirty-list length, counts the fixed-point iterations that changed an order and publishes
 // the count to the host (pub, pinned host memory) with a system-scope store.
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub) {
-    const int64_t n = (int64_t)a.counters[0];
+__global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
+                                                        int p) {
+    const int64_t n = (int64_t)*changed_count(a, p);
+    const int32_t* cl = changed_list(a, p);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counters[5] = 0;
+        *changed_count(a, 1 - p) = 0;
+        *work_count(a, 1 - p) = 0;
         if (n > 0) a.counters[4] += 1;
         __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
-        const int32_t i = a.clist[t];
-        a.orders[i] = (int8_t)(a.orders[i] - 1);
-        const int cnt = a.g.cnt[i];
-        const int64_t base = sell_base(a.g, i);
-        for (int k = -1; k < cnt; ++k) {
-            const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
-            a.dmark[j] = 1;
-            a.bmark[j / kBlock] = 1;
+    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t t = t0 + threadIdx.x;
+        int32_t i = 0;
+        int cnt = -1;   // -1: no particle on this lane; else its neighbour count (it lists cnt + 1 entries)
+        if (t < n) {
+            i = cl[t];
+            a.orders[i] = (int8_t)(a.orders[i] - 1);
+            cnt = a.g.cnt[i];
         }
-        if (!a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
-    }
-}
-
-// Lists the dirty particles of the marked blocks densely (dlist, counters[5]): a workgroup per block, which leaves at
-// once when its block is unmarked; one atomic per wavefront with dirty particles. Clears the block marks and resets
-// the changed count for the flux pass that follows.
-__global__ __launch_bounds__(kBlock) void k_list_dirty(StepArgs a) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = 0;
-    const int64_t b = blockIdx.x;
-    if (!a.bmark[b]) return;
-    __syncthreads();                                   // every thread has read the mark before it is cleared
-    if (threadIdx.x == 0) a.bmark[b] = 0;
-    const int64_t i = b * kBlock + threadIdx.x;
-    const bool f = i < a.g.N && a.dmark[i];
-    const unsigned long long ball = __ballot(f);
-    if (!ball) return;
-    const unsigned lane = threadIdx.x & 63;
-    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(&a.counters[5], (unsigned long long)__popcll(ball));
-    base = __shfl(base, (int)leader);
-    if (f) {
-        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        a.dlist[base + __popcll(below)] = (int32_t)i;
+        // the wavefront's entries are placed with one atomic: an inclusive scan of cnt + 1 over the lanes
+        const unsigned lane = threadIdx.x & 63;
+        int incl = cnt + 1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if ((int)lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        unsigned long long base_w = 0;
+        if (lane == 0 && total > 0) base_w = atomicAdd(work_count(a, p), (unsigned long long)total);
+        base_w = __shfl(base_w, 0);
+        if (cnt >= 0) {
+            int32_t* out = a.dlist + base_w + (incl - (cnt + 1));
+            const int64_t base = sell_base(a.g, i);
+            for (int k = -1; k < cnt; ++k) {
+                const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
+                a.dmark[j] = 1u;
+                out[k + 1] = j;
+            }
+            if (!a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+        }
     }
 }
 
@@ -1085,7 +1099,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* p, uint8_t v, int64
 __global__ void k_reset_counters(unsigned long long* c, int mode) {
     if (threadIdx.x == 0) {
         c[0] = 0;
-        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[4] = 0; c[5] = 0; }
+        if (mode == 1) { c[1] = 0; c[2] = 0x7FF0000000000000ull; c[4] = 0; }
     }
 }
 
@@ -1113,9 +1127,9 @@ struct hcc_stepper {
     double* G = nullptr;
     double* scratch = nullptr; // N * M doubles for up/downloads
     int8_t* orders = nullptr;
-    uint8_t *changed = nullptr, *dirty = nullptr, *trig = nullptr, *inadm = nullptr, *dmark = nullptr;
-    uint8_t* bmark = nullptr;
-    int32_t *clist = nullptr, *dlist = nullptr;
+    uint8_t *changed = nullptr, *dirty = nullptr, *trig = nullptr, *inadm = nullptr;
+    unsigned* dmark = nullptr;
+    int32_t *clist = nullptr, *clist2 = nullptr, *dlist = nullptr;
     double* lam = nullptr;
     double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
     double* ext = nullptr;     // neighbourhood extrema of the stage state (fused MOODu1 checks)
@@ -1163,7 +1177,7 @@ StepArgs hcc_stepper::args() const {
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
-    a.counters = counters; a.clist = clist; a.dlist = dlist; a.dmark = dmark; a.bmark = bmark; a.lam = lam;
+    a.counters = counters; a.clist = clist; a.clist2 = clist2; a.dlist = dlist; a.dmark = dmark; a.lam = lam;
     a.bmin = bmin;
     a.ext = ext;
     a.ext_on = (desc.substep_ratio > 0.0 && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
@@ -1182,8 +1196,8 @@ struct Variant {
             k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
-    static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub) {
-        k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub);
+    static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
+        k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub, p);
     }
     static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
                      bool check) {
@@ -1194,8 +1208,8 @@ struct Variant {
         } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
         else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
     }
-    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K) {
-        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, p, Y, K);
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
+        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 1024), kBlock, 0, st(s)>>>(a, pr, Y, K, p);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
         const int64_t nb = nblocks(s->N);
@@ -1229,9 +1243,9 @@ struct Variant {
 
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int, int);
-    void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*);
+    void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
-    void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*);
+    void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int);
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
     void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
@@ -1264,7 +1278,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->goff, (void*)s->gidx, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
-                    (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->bmark, (void*)s->clist, (void*)s->dlist,
+                    (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->clist, (void*)s->clist2, (void*)s->dlist,
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->dtp, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
@@ -1324,17 +1338,17 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->trig, N));
     HCC_TRY_HIP(ctx, dalloc(&s->inadm, N));
     HCC_TRY_HIP(ctx, dalloc(&s->dmark, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->bmark, nblocks(N) + 1));
     HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N + g->S));   // at most every particle's neighbourhood once
+    HCC_TRY_HIP(ctx, dalloc(&s->clist2, N));
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
     HCC_TRY_HIP(ctx, dalloc(&s->ext, 2 * M * N));
     HCC_TRY_HIP(ctx, dalloc(&s->dtp, 2));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
-    for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm, s->dmark})
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), st));
+    for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->bmark, 0, nblocks(N) + 1, st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
@@ -1522,10 +1536,9 @@ struct StepRunner {
     // publishes the count it starts from into pinned slot q, and event evm[q] follows that kernel.
     hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, int q) {
         hipStream_t str = s->ctx->stream;
-        ops.mark_gather(s, a, Yp, s->d_pub + q);
+        ops.mark_gather(s, a, Yp, s->d_pub + q, q);
         hipError_t e = hipEventRecord(s->evm[q], str);
-        k_list_dirty<<<(unsigned)nb, kBlock, 0, str>>>(a);
-        ops.flux_list(s, a, p, Yp, Kp);
+        ops.flux_list(s, a, p, Yp, Kp, q);
         return e;
     }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
