@@ -97,6 +97,8 @@ struct StepArgs {
     PdeArgs pde;
     int limiter, mode, strategy, criterion;
     double delta;
+    double plateau;              // MOODu2: a component whose local range is below plateau x range[m] is not checked
+    const double* range;         // MOODu2: global range of the previous stage per component (nullptr: plateau off)
     const double* invg;          // reconstruction inverse moments, D*D per node: [q * N + i]
     const double* invd;          // divergence inverse moments, derivative rows 1..D: [(r * (D+1) + c) * N + i]
     const uint8_t* okg;

@@ -827,30 +827,42 @@ __global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double
 }
 
 // ============================================================================================================
-// the MOOD path that checks a stored candidate (no SSP substep): _mood_flag! + _mood_drop! (MOOD.jl:289-309) on
-// all interior particles or the dirty mask, then expand_mask! (MOOD.jl:323-351)
+// the unfused MOOD path (a stored candidate, or the strategies and criteria the flux pass does not check):
+// _mood_flag! + _mood_drop! (MOOD.jl:289-309) on all interior particles or the dirty mask, the forcing pass of
+// StrictEPD0 (_mood_force!, MOOD.jl:311-320), expand_mask! (MOOD.jl:323-351)
 // ============================================================================================================
+// The candidate is the stored one (cand), or with Kp the Euler substep W = Y + h K formed on the fly (form_substep!,
+// TimeSteppers.jl:501-518). changed = triggered after the decrement (a flagged particle at order 1 is un-flagged).
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* __restrict__ Yp, const double* __restrict__ cand,
-                                                       int full, int64_t nblk) {
+                                                       const double* __restrict__ Kp, int full, int64_t nblk) {
     using P = Pde<D, M, KIND>;
     using L = Layout<D, M>;
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
     bool ch = false;
     if (i < N && !a.g.isb[i] && (full || a.dirty[i])) {
-        double c[L::NR], W[M];
-        load_rec<L::NR, L::PU>(cand, i, c);
+        double r[L::NR], W[M];
+        if (Kp) {
+            const double h = a.hratio * *a.dtp;
+            load_rec<L::NR, L::PU>(Yp, i, r);
 #pragma unroll
-        for (int m = 0; m < M; ++m) W[m] = c[D + m];
+            for (int m = 0; m < M; ++m) W[m] = r[D + m] + h * Kp[i * M + m];
+        } else {
+            load_rec<L::NR, L::PU>(cand, i, r);
+#pragma unroll
+            for (int m = 0; m < M; ++m) W[m] = r[D + m];
+        }
         const bool adm = P::admissible(a.pde, W);
         a.inadm[i] = !adm;
         bool flag = !adm;
         if (!flag) {
             if (a.criterion == HCC_CRIT_ONLY) {
                 flag = true;
-            } else if (a.criterion == HCC_CRIT_U1) {
-                double r[L::NR], mn[M], mx[M];
+            } else if (a.criterion == HCC_CRIT_U1 || a.criterion == HCC_CRIT_U2) {
+                // findLocalExtrema over {i} and its neighbours in Y_prev; _dmp_fails (MOOD.jl:175-184), and for MOODu2
+                // (_u2_fails, MOOD.jl:215-270, MUSCL order 2) the components below the plateau are exempt
+                double mn[M], mx[M];
                 load_rec<L::NR, L::PU>(Yp, i, r);
 #pragma unroll
                 for (int m = 0; m < M; ++m) { mn[m] = r[D + m]; mx[m] = r[D + m]; }
@@ -862,9 +874,13 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
                     for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], r[D + m]); mx[m] = jmax(mx[m], r[D + m]); }
                 }
                 const double dl = a.delta, d3 = dl * dl * dl;
+                const bool u2 = a.criterion == HCC_CRIT_U2 && a.range;
 #pragma unroll
-                for (int m = 0; m < M; ++m)
-                    if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
+                for (int m = 0; m < M; ++m) {
+                    const bool viol = fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl);
+                    const bool plateau = u2 && mx[m] - mn[m] < a.plateau * a.range[m];
+                    if (viol && !plateau) flag = true;
+                }
             }
         }
         if (flag) {
@@ -878,17 +894,106 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
         atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
 }
 
-__global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, int64_t nblk) {
+// forced_order of StrictEPD0 (Halo{1}, MOOD.jl:49-64) with _mood_force! (:311-320): every interior particle pulls the
+// minimum order of its triggered neighbours (changed) from the snapshot taken after the decrement, so nothing
+// cascades. dirty receives the new changed set (triggered or forced); the caller copies it back into changed.
+__global__ __launch_bounds__(kBlock) void k_force(StepArgs a, const int8_t* __restrict__ snap, int64_t nblk) {
     const int64_t i = node_index(nblk);
     if (i >= a.g.N) return;
-    uint8_t m = a.changed[i];
+    uint8_t ch = a.changed[i];
+    if (!a.g.isb[i]) {
+        const int own = snap[i];
+        int o = own;
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t p = a.g.nbr[base + (int64_t)k * kChunk];
+            if (a.changed[p]) o = o < snap[p] ? o : snap[p];
+        }
+        if (o < own) { a.orders[i] = (int8_t)o; ch = 1; }
+    }
+    a.dirty[i] = ch;
+}
+
+// One hop of expand_mask! (_expand_body!, MOOD.jl:323-351): dst = src or any neighbour in src.
+__global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                    int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    uint8_t m = src[i];
     if (!m) {
         const int cnt = a.g.cnt[i];
         const int64_t base = sell_base(a.g, i);
         for (int k = 0; k < cnt; ++k)
-            if (a.changed[a.g.nbr[base + (int64_t)k * kChunk]]) { m = 1; break; }
+            if (src[a.g.nbr[base + (int64_t)k * kChunk]]) { m = 1; break; }
     }
-    a.dirty[i] = m;
+    dst[i] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// _nb_min_order (MOOD.jl:66-74) of every particle: EPD2's interface order of the edge (i, j) is the minimum of the
+// two (_get_interface_orders, MOOD.jl:82-85), so the flux pass reads these as orders with the edge-minimum rule.
+__global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict__ out, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= a.g.N) return;
+    int o = a.orders[i];
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    for (int k = 0; k < cnt; ++k) {
+        const int oj = a.orders[a.g.nbr[base + (int64_t)k * kChunk]];
+        o = o < oj ? o : oj;
+    }
+    out[i] = (int8_t)o;
+}
+
+// _global_range (MOOD.jl:206-213) of the state components of the records Y[0 .. N): each workgroup reduces its nodes
+// with Julia's min / max (IEEE minimum / maximum: exact, so the order of the reduction does not matter) into
+// part[b * 2M ..]: M minima, M maxima; k_range_final reduces the parts and writes range = max - min.
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_range_part(const double* __restrict__ Y, int64_t N, double* __restrict__ part) {
+    using L = Layout<D, M>;
+    __shared__ double sl[kBlock / 64][2 * M];
+    double lo[M], hi[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) { lo[m] = INFINITY; hi[m] = -INFINITY; }
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        double r[L::NR];
+        load_rec<L::NR, L::PU>(Y, i, r);
+#pragma unroll
+        for (int m = 0; m < M; ++m) { lo[m] = r[D + m]; hi[m] = r[D + m]; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int m = 0; m < M; ++m) { lo[m] = jmin(lo[m], __shfl_xor(lo[m], o)); hi[m] = jmax(hi[m], __shfl_xor(hi[m], o)); }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) { sl[w][m] = lo[m]; sl[w][M + m] = hi[m]; }
+    __syncthreads();
+    if (threadIdx.x < 2 * M) {
+        const int c = threadIdx.x;
+        double v = sl[0][c];
+        for (int q = 1; q < kBlock / 64; ++q) v = c < M ? jmin(v, sl[q][c]) : jmax(v, sl[q][c]);
+        part[(int64_t)blockIdx.x * 2 * M + c] = v;
+    }
+}
+template <int M>
+__global__ __launch_bounds__(64) void k_range_final(const double* __restrict__ part, int64_t nparts, double* __restrict__ range) {
+    if (threadIdx.x != 0) return;
+    double lo[M], hi[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) { lo[m] = INFINITY; hi[m] = -INFINITY; }
+    for (int64_t b = 0; b < nparts; ++b)
+#pragma unroll
+        for (int m = 0; m < M; ++m) { lo[m] = jmin(lo[m], part[b * 2 * M + m]); hi[m] = jmax(hi[m], part[b * 2 * M + M + m]); }
+#pragma unroll
+    for (int m = 0; m < M; ++m) range[m] = hi[m] - lo[m];
 }
 
 // _count_inadmissible (TimeSteppers.jl:481-490): interior candidates accepted while not admissible
@@ -1133,6 +1238,11 @@ struct hcc_stepper {
     double* lam = nullptr;
     double* bmin = nullptr;    // per-workgroup minima of the CFL estimate
     double* ext = nullptr;     // neighbourhood extrema of the stage state (fused MOODu1 checks)
+    int8_t* snap = nullptr;    // StrictEPD0: orders after the decrement (the forcing reads them)
+    int8_t* nbmin = nullptr;   // EPD2: minimum order over each neighbourhood
+    uint8_t* mask2 = nullptr;  // EPD2: the first hop of the dirty mask's expansion
+    double* rpart = nullptr;   // MOODu2: per-workgroup extrema of the previous stage ...
+    double* range = nullptr;   // ... and its global range per component
     double* dtp = nullptr;     // the step's dt on the device
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
@@ -1161,6 +1271,16 @@ struct hcc_stepper {
     StepArgs args() const;
 };
 
+// MOOD checks the Euler substeps inside the flux pass that produces K (flux_node with CHECK) for the strategies whose
+// recompute set is one hop (_order_reach 1, no forcing) and the criteria that need no global reduction
+static bool mood_fused(const hcc_stepper_desc& d) {
+    const bool strat = d.mood_strategy == HCC_STRAT_NONE || d.mood_strategy == HCC_STRAT_EPD0 ||
+                       d.mood_strategy == HCC_STRAT_EPD1;
+    const bool crit = d.mood_criterion == HCC_CRIT_U1 || d.mood_criterion == HCC_CRIT_PAD ||
+                      d.mood_criterion == HCC_CRIT_ONLY;
+    return d.substep_ratio > 0.0 && strat && crit;
+}
+
 StepArgs hcc_stepper::args() const {
     StepArgs a{};
     a.g = grid->args();
@@ -1175,12 +1295,14 @@ StepArgs hcc_stepper::args() const {
     }
     a.limiter = desc.limiter; a.mode = desc.limiter_mode; a.strategy = desc.mood_strategy;
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
+    a.plateau = desc.mood_plateau;
+    a.range = (desc.mood_criterion == HCC_CRIT_U2 && desc.mood_plateau > 0.0) ? range : nullptr;
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
     a.counters = counters; a.clist = clist; a.clist2 = clist2; a.dlist = dlist; a.dmark = dmark; a.lam = lam;
     a.bmin = bmin;
     a.ext = ext;
-    a.ext_on = (desc.substep_ratio > 0.0 && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
+    a.ext_on = (mood_fused(desc) && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
     a.dtp = dtp;
     a.hratio = desc.substep_ratio;
     return a;
@@ -1211,9 +1333,15 @@ struct Variant {
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
         k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 1024), kBlock, 0, st(s)>>>(a, pr, Y, K, p);
     }
-    static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, int full) {
+    static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
+                     int full) {
         const int64_t nb = nblocks(s->N);
-        k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, full, nb);
+        k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, Kp, full, nb);
+    }
+    static void range(const hcc_stepper* s, const double* Y) {
+        const int64_t nb = (s->N + kBlock - 1) / kBlock;
+        k_range_part<D, M><<<(unsigned)nb, kBlock, 0, st(s)>>>(Y, s->N, s->rpart);
+        k_range_final<M><<<1, 64, 0, st(s)>>>(s->rpart, nb, s->range);
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
@@ -1246,7 +1374,8 @@ struct VariantOps {
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
     void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int);
-    void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, int);
+    void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, const double*, int);
+    void (*range)(const hcc_stepper*, const double*);
     void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
     void (*pack)(const hcc_stepper*, const double*, double*);
@@ -1256,7 +1385,7 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::range, &V::cfl, &V::predict,
                       &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
@@ -1279,7 +1408,8 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     for (void* p : {(void*)s->invg, (void*)s->okg, (void*)s->invd, (void*)s->okd, (void*)s->dw, (void*)s->root, (void*)s->goff, (void*)s->gidx, (void*)s->U,
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->clist, (void*)s->clist2, (void*)s->dlist,
-                    (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->dtp, (void*)s->counters})
+                    (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
+                    (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
     for (auto& r : s->ring) {
@@ -1304,8 +1434,9 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     if (d->muscl_order != 2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL order 2 is the compiled scheme");
     if (d->stages < 1 || d->stages > kMaxStages) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: 1..5 stages");
     if (d->limiter < HCC_LIM_NONE || d->limiter > HCC_LIM_VK) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: unknown limiter");
-    if (d->mood_strategy < HCC_STRAT_NONE || d->mood_strategy > HCC_STRAT_EPD1) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD strategy");
-    if (d->mood_criterion < HCC_CRIT_NONE || d->mood_criterion > HCC_CRIT_ONLY) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD criterion");
+    if (d->mood_strategy < HCC_STRAT_NONE || d->mood_strategy > HCC_STRAT_STRICT_EPD0) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD strategy");
+    if (d->mood_criterion < HCC_CRIT_NONE || d->mood_criterion > HCC_CRIT_U2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD criterion");
+    if (!(d->mood_plateau >= 0.0) || !std::isfinite(d->mood_plateau)) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: MOODu2 plateau must be finite and >= 0");
     if (d->n_bc < 0 || d->n_bc > 16) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: at most 16 boundary conditions");
     for (int i = 0; i < d->stages; ++i)
         for (int j = i; j < d->stages; ++j)
@@ -1344,6 +1475,11 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
     HCC_TRY_HIP(ctx, dalloc(&s->ext, 2 * M * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->snap, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->nbmin, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->mask2, N));
+    HCC_TRY_HIP(ctx, dalloc(&s->rpart, 2 * M * ((N + kBlock - 1) / kBlock)));
+    HCC_TRY_HIP(ctx, dalloc(&s->range, M));
     HCC_TRY_HIP(ctx, dalloc(&s->dtp, 2));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), st));
@@ -1479,8 +1615,19 @@ struct StepRunner {
     int64_t nb = 0;
     int iters = 0;
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
-    // MOOD checks the Euler substep: the check is fused into the flux pass that produces K (flux_node with CHECK)
-    bool fused() const { return substeps(); }
+    // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK)
+    bool fused() const { return mood_fused(s->desc); }
+    bool epd2() const { return s->desc.mood_strategy == HCC_STRAT_EPD2; }
+    // the arguments of a flux pass: with EPD2 the interface order of an edge is the minimum of the two particles'
+    // neighbourhood minima, i.e. the edge-minimum rule of EPD1 applied to nbmin (computed before the pass)
+    StepArgs flux_args() {
+        if (!epd2()) return a;
+        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, nb);
+        StepArgs f = a;
+        f.orders = s->nbmin;
+        f.strategy = HCC_STRAT_EPD1;
+        return f;
+    }
     // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself
     bool fuse_pred() const { return fused() || s->desc.mood_criterion == HCC_CRIT_NONE; }
     bool y1_in_U = false;
@@ -1564,22 +1711,40 @@ struct StepRunner {
             }
             return hipGetLastError();
         }
+        // Unfused: the check reads the stored candidate, or the Euler substep formed from Y_{i-1} and K_{i-1}; the
+        // candidate Y_i is formed after the fixed point (it reads only K_1 .. K_{i-1}).
         const PredArgs off{};
+        const bool sub = substeps();
+        const double* Kchk = sub ? Kp : nullptr;
+        if (a.range) ops.range(s, Yp);   // MOODu2's global range of Y_{i-1} (_global_range, MOOD.jl:206-213)
+        const int reach = epd2() ? 2 : 1;   // _order_reach (MUSCL_hooks.jl:10-11)
         int full = 1;
         for (;;) {
-            predict(i);
+            if (!sub) predict(i);
             k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
-            ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], full);
+            ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], Kchk, full);
             unsigned long long n = 0;
             hipError_t e = read_changed(n);
             if (e != hipSuccess) return e;
             if (n == 0) break;
             ++iters;
-            k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
+            if (s->desc.mood_strategy == HCC_STRAT_STRICT_EPD0) {   // forcing from the post-decrement snapshot
+                k_copy_u8<<<(unsigned)nb, kBlock, 0, str>>>(reinterpret_cast<const uint8_t*>(s->orders),
+                                                             reinterpret_cast<uint8_t*>(s->snap), s->N);
+                k_force<<<grid_for(nb), kBlock, 0, str>>>(a, s->snap, nb);
+                k_copy_u8<<<(unsigned)nb, kBlock, 0, str>>>(s->dirty, s->changed, s->N);
+            }
+            if (reach == 1) {
+                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->changed, s->dirty, nb);
+            } else {
+                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->changed, s->mask2, nb);
+                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->mask2, s->dirty, nb);
+            }
             ops.gather(s, a, Yp, 0, 0);
-            ops.flux(s, a, off, Yp, Kp, 0, false);
+            ops.flux(s, flux_args(), off, Yp, Kp, 0, false);
             full = 0;
         }
+        if (sub) predict(i);
         k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
         return hipGetLastError();
     }
@@ -1604,7 +1769,7 @@ struct StepRunner {
             ops.gather(s, a, Y, 1, i == 1);
             (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
-            ops.flux(s, a, p, Y, s->K[i - 1], 1, fused());
+            ops.flux(s, flux_args(), p, Y, s->K[i - 1], 1, fused());
             (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1);

@@ -5,6 +5,7 @@ call raises. There is no CPU fallback.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -14,8 +15,8 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libhcc.so")
 HCC_PDE_LINADV, HCC_PDE_EULER = 1, 2
 HCC_LIM_NONE, HCC_LIM_BJ, HCC_LIM_VK = 0, 1, 2
 HCC_MODE_SOFT, HCC_MODE_HARD = 0, 1
-HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1 = 0, 1, 2
-HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY = 0, 1, 2, 3
+HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1, HCC_STRAT_EPD2, HCC_STRAT_STRICT_EPD0 = 0, 1, 2, 3, 4
+HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY, HCC_CRIT_U2 = 0, 1, 2, 3, 4
 HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
 HCC_E_CAPACITY = -4
 
@@ -42,7 +43,7 @@ class StepperDesc(C.Structure):
                 ("mood_strategy", C.c_int32), ("mood_criterion", C.c_int32), ("mood_delta", C.c_double),
                 ("stages", C.c_int32), ("a_t", (C.c_double * 5) * 5), ("b_t", C.c_double * 5), ("c_t", C.c_double * 5),
                 ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
-                ("bc_kind", C.c_int32 * 16)]
+                ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double)]
 
 
 _lib = None
@@ -147,16 +148,21 @@ def rng_uniform(seed, n):
 
 
 class Context:
-    """One GPU context (device + stream)."""
+    """One GPU context (device + stream). Grids and steppers hold its handle: closing the context closes the ones
+    still alive first (steppers, then grids), so that none is destroyed later through a freed context."""
 
     def __init__(self, device=0):
         L = lib()
         h = C.c_void_p()
         check(L.hcc_ctx_create(int(device), C.byref(h)), None, "hcc_ctx_create")
         self.h = h
+        self._steppers = weakref.WeakSet()
+        self._grids = weakref.WeakSet()
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
+            for obj in list(self._steppers) + list(self._grids):
+                obj.close()
             lib().hcc_ctx_destroy(self.h)
             self.h = None
 
@@ -181,6 +187,7 @@ class Grid:
         check(L.hcc_grid_create(ctx.h, C.byref(dd), self.N, ptr(self._keep[0]), ptr(self._keep[1]), ptr(self._keep[2]),
                                 C.byref(h)), ctx.h, "hcc_grid_create")
         self.h = h
+        ctx._grids.add(self)
         S, mx = C.c_int64(), C.c_int64()
         check(L.hcc_grid_num_slots(self.h, C.byref(S), C.byref(mx)), ctx.h, "hcc_grid_num_slots")
         self.S, self.max_nb = S.value, mx.value
@@ -235,6 +242,7 @@ def stepper_desc(scheme):
     sd.limiter, sd.limiter_mode = scheme["limiter"], scheme.get("limiter_mode", HCC_MODE_HARD)
     sd.mood_strategy, sd.mood_criterion = scheme["strategy"], scheme["criterion"]
     sd.mood_delta = scheme.get("delta", 0.0)
+    sd.mood_plateau = scheme.get("plateau", 0.0)
     a_t, b_t, c_t = scheme["a_t"], scheme["b_t"], scheme["c_t"]
     s = len(b_t)
     sd.stages = s
@@ -260,6 +268,7 @@ class Stepper:
         h = C.c_void_p()
         check(lib().hcc_stepper_create(ctx.h, grid.h, C.byref(self.sd), C.byref(h)), ctx.h, "hcc_stepper_create")
         self.h = h
+        ctx._steppers.add(self)
 
     def upload(self, U):
         U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)

@@ -54,6 +54,7 @@ struct StepperDesc
     n_bc::Int32
     bc_tag::NTuple{16, Int32}
     bc_kind::NTuple{16, Int32}
+    mood_plateau::Float64
 end
 
 checkrc(rc, ctx) = rc == 0 || error("libhcc: ", unsafe_string(ccall((:hcc_last_error, libhcc), Cstring, (Ptr{Cvoid},), ctx)))

@@ -91,8 +91,13 @@ int hcc_grid_get_positions(hcc_grid* g, double* pos);
 enum { HCC_PDE_LINADV = 1, HCC_PDE_EULER = 2 };
 enum { HCC_LIM_NONE = 0, HCC_LIM_BJ = 1, HCC_LIM_VK = 2 };
 enum { HCC_MODE_SOFT = 0, HCC_MODE_HARD = 1 };
-enum { HCC_STRAT_NONE = 0, HCC_STRAT_EPD0 = 1, HCC_STRAT_EPD1 = 2 };
-enum { HCC_CRIT_NONE = 0, HCC_CRIT_U1 = 1, HCC_CRIT_PAD = 2, HCC_CRIT_ONLY = 3 };
+/* MOOD strategies (src/TimeStepping/MOOD.jl:7-26): interface orders (o_i, o_j) for EPD0 and StrictEPD0, the edge
+ * minimum for EPD1, the minimum over N(i) u N(j) for EPD2; StrictEPD0 also lowers the neighbours of a dropped
+ * particle to its order (one hop, no cascade). Criteria (MOOD.jl:140-205): MOODu1 (DMP), MOODPAD (admissibility
+ * only), OnlyMOOD (every check flags), MOODu2 (MOODu1 with the plateau test; at MUSCL order 2 the curvature
+ * override does not apply). */
+enum { HCC_STRAT_NONE = 0, HCC_STRAT_EPD0 = 1, HCC_STRAT_EPD1 = 2, HCC_STRAT_EPD2 = 3, HCC_STRAT_STRICT_EPD0 = 4 };
+enum { HCC_CRIT_NONE = 0, HCC_CRIT_U1 = 1, HCC_CRIT_PAD = 2, HCC_CRIT_ONLY = 3, HCC_CRIT_U2 = 4 };
 enum { HCC_BC_NONE = 0, HCC_BC_OUTFLOW = 1, HCC_BC_DIRICHLET = 2 };
 
 typedef struct {
@@ -103,12 +108,13 @@ typedef struct {
     int32_t muscl_order;       /* MUSCL max_order (2 is compiled) */
     int32_t limiter, limiter_mode;
     int32_t mood_strategy, mood_criterion;
-    double mood_delta;         /* MOODu1 d */
+    double mood_delta;         /* MOODu1 / MOODu2 d */
     int32_t stages;            /* explicit tableau, s <= 5 */
     double a_t[5][5], b_t[5], c_t[5];
     double substep_ratio;      /* Δt/C per Δt of the MOOD Euler substeps (TimeSteppers.jl:218); 0: check stage candidates */
     int32_t n_bc;
     int32_t bc_tag[16], bc_kind[16];
+    double mood_plateau;       /* MOODu2 plateau (fraction of the global range of the previous stage; 0: off) */
 } hcc_stepper_desc;
 
 /* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL

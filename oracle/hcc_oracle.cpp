@@ -16,6 +16,9 @@
 //     reference unit-test expectations).
 //   * weight function, VK limiter phi, Rusanov, RK3 tableau, DMP, outflow: pinned (closed forms and
 //     the reference's unit-test values).
+//   * MOOD strategies (EPD0/EPD1/EPD2/StrictEPD0 interface orders, order reach, forcing from the
+//     snapshot), mask expansion, MOODu2 plateau: pinned (the reference's unit-test expectations,
+//     step-level invariants: forcing reaches every neighbour, MOODu2 without plateau == MOODu1).
 //   * MLS Cholesky / inverse: pinned to tolerance only (exact linear reproduction, zero-sum
 //     weights); the bit-level rounding of StaticArrays' small-matrix cholesky/inv cannot be imported
 //     here -- "parity unpinned" at the bit level for that part.
@@ -441,14 +444,15 @@ struct Phys {
 // ---------------------------------------------------------------------------------------------
 enum { LIM_NONE = 0, LIM_BJ = 1, LIM_VK = 2 };
 enum { MODE_SOFT = 0, MODE_HARD = 1 };
-enum { STRAT_NONE = 0, STRAT_EPD0 = 1, STRAT_EPD1 = 2 };
-enum { CRIT_NONE = 0, CRIT_U1 = 1, CRIT_PAD = 2, CRIT_ONLY = 3 };
+enum { STRAT_NONE = 0, STRAT_EPD0 = 1, STRAT_EPD1 = 2, STRAT_EPD2 = 3, STRAT_SEPD0 = 4 };
+enum { CRIT_NONE = 0, CRIT_U1 = 1, CRIT_PAD = 2, CRIT_ONLY = 3, CRIT_U2 = 4 };
 enum { BC_NONE = 0, BC_OUTFLOW = 1, BC_DIRICHLET = 2 };
 
 struct Scheme {
     int limiter = LIM_VK, mode = MODE_HARD;
     int strategy = STRAT_EPD1, criterion = CRIT_U1;
     double delta = 0.0;
+    double plateau = 0.0;      // MOODu2 plateau fraction of the global range (0: off)
     int s = 3;
     double a_t[5][5] = {}, b_t[5] = {}, c_t[5] = {};
     double substep_ratio = 0.0;
@@ -480,6 +484,12 @@ struct SimBase {
     virtual void get_bins(int32_t* bin_of, int64_t* cell_start, int32_t* cell_particles, int64_t* dims) const = 0;
     virtual void set_threads(int n) = 0;
     virtual int64_t mood_iterations() const = 0;
+    // unit hooks of the MOOD building blocks (the reference's own unit tests drive the same functions)
+    virtual void interface_orders_of(int strategy, const int32_t* o, int64_t i, int64_t j, int32_t* out) = 0;
+    virtual int order_reach_of(int strategy) = 0;
+    virtual void force_of(const uint8_t* trig, const int32_t* snap, int32_t* o, uint8_t* ch) = 0;
+    virtual void expand_of(uint8_t* mask, int hops) = 0;
+    virtual int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) = 0;
 };
 
 template <int D, int M>
@@ -520,7 +530,7 @@ struct Sim : SimBase {
     std::vector<State> rhos, rho_n, rho_cand, rho_sub;
     std::vector<State> Y[5], K[5];
     std::vector<std::array<State, BG>> grads;
-    std::vector<int32_t> orders;
+    std::vector<int32_t> orders, snapshot_orders;
     std::vector<uint8_t> triggered, changed, dirty, scratch, triggered_step;
     std::vector<int64_t> status;
     std::vector<std::array<State, D>> dflux;   // per-slot flux differences (buffered divergence)
@@ -792,6 +802,22 @@ struct Sim : SimBase {
         grads[i] = raw;
     }
 
+    // _nb_min_order (MOOD.jl:66-74): the minimum order over the particle and its neighbours
+    int nb_min_order(int64_t i) const {
+        int o = orders[i];
+        for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) o = std::min(o, (int)orders[nbr[k]]);
+        return o;
+    }
+    // _get_interface_orders (MOOD.jl:76-87): EPD0 / StrictEPD0 / none (o_i, o_j); EPD1 the edge minimum; EPD2 the
+    // minimum over N(i) u N(j) u {i, j}
+    void interface_orders(int64_t i, int64_t j, int& o_i, int& o_j) const {
+        if (sc.strategy == STRAT_EPD1) { o_i = o_j = std::min((int)orders[i], (int)orders[j]); }
+        else if (sc.strategy == STRAT_EPD2) { o_i = o_j = std::min(nb_min_order(i), nb_min_order(j)); }
+        else { o_i = orders[i]; o_j = orders[j]; }
+    }
+    // _order_reach (MUSCL_hooks.jl:10-11): the hops over which a flux depends on neighbours' orders
+    int order_reach() const { return sc.strategy == STRAT_EPD2 ? 2 : 1; }
+
     // MUSCL functor (MUSCL.jl:390-424) with _flux_loop (:463-508) and apply_weights_divergence
     // (Interpolators.jl:610-626); returns K_E = -2 div + 0 (TimeSteppers.jl:354)
     void flux_node(int64_t i, const std::vector<State>& U, State& KE) {
@@ -800,16 +826,14 @@ struct Sim : SimBase {
         const State& fi = U[i];
         double Fi[D][M]; P::flux(pde, fi.data(), Fi);
         const auto& gi = grads[i];
-        int oi = orders[i];
         for (int64_t k = k0; k < k1; ++k) {
             int64_t j = nbr[k];
             const State& fj = U[j];
             const auto& gj = grads[j];
             double pi_[BG], pj_[BG];
             for (int d = 0; d < D; ++d) { pi_[d] = 0.5 * dist[k][d]; pj_[d] = -0.5 * dist[k][d]; }
-            int oj = orders[j];
             int o_i, o_j;
-            if (sc.strategy == STRAT_EPD1) { o_i = o_j = std::min(oi, oj); } else { o_i = oi; o_j = oj; }
+            interface_orders(i, j, o_i, o_j);
             if (o_i != 2) for (int d = 0; d < D; ++d) pi_[d] = 0.0;       // mask_basis
             if (o_j != 2) for (int d = 0; d < D; ++d) pj_[d] = 0.0;
             State fij, fji;
@@ -944,11 +968,42 @@ struct Sim : SimBase {
         return false;
     }
 
-    // detect_and_drop! (MOOD.jl:375-405) for EPD strategies (no forcing)
+    // _u2_fails (MOOD.jl:215-270) for MUSCL order 2: the DMP test of MOODu1 on the components that are not a
+    // plateau (max - min below plateau x the global range, when the plateau test is on); a violating component
+    // flags the particle, since the curvature override needs consistency order >= 3
+    bool u2_fails(int64_t p, const std::vector<State>& Up, const State& cand, const State* range) const {
+        State mn = Up[p], mx = Up[p];
+        for (int64_t k = rstart[p]; k < rstart[p + 1]; ++k) {
+            const State& fj = Up[nbr[k]];
+            for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], fj[m]); mx[m] = jmax(mx[m], fj[m]); }
+        }
+        double d = sc.delta, d3 = d * d * d;
+        for (int m = 0; m < M; ++m) {
+            const bool violates = std::fabs(mx[m] - mn[m]) >= d3 && (cand[m] < mn[m] - d || cand[m] > mx[m] + d);
+            const bool plateau = range && mx[m] - mn[m] < sc.plateau * (*range)[m];
+            if (violates && !plateau) return true;
+        }
+        return false;
+    }
+    // _global_range (MOOD.jl:206-213): max - min per component over all particles 1:N, with Julia's min / max
+    State global_range(const std::vector<State>& U) const {
+        State lo = U[0], hi = U[0];
+        for (int64_t p = 1; p < N; ++p)
+            for (int m = 0; m < M; ++m) { lo[m] = jmin(lo[m], U[p][m]); hi[m] = jmax(hi[m], U[p][m]); }
+        State r;
+        for (int m = 0; m < M; ++m) r[m] = hi[m] - lo[m];
+        return r;
+    }
+
+    // detect_and_drop! (MOOD.jl:375-405): check, decrement (a flagged particle at order 1 is un-flagged), then the
+    // forcing pass of StrictEPD0 from the post-decrement snapshot (_mood_force!, forced_order MOOD.jl:49-64)
     bool detect_and_drop(const std::vector<State>& Up, const std::vector<State>& cand, bool full) {
         if (sc.criterion == CRIT_NONE) return false;
         std::fill(triggered.begin(), triggered.end(), 0);
         std::fill(changed.begin(), changed.end(), 0);
+        State range{};
+        const bool use_range = sc.criterion == CRIT_U2 && sc.plateau > 0.0;
+        if (use_range && N > 0) range = global_range(Up);
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int64_t p = 0; p < N; ++p) {
             if (isb[p] || !(full || dirty[p])) continue;
@@ -956,6 +1011,7 @@ struct Sim : SimBase {
             if (!flag) {
                 if (sc.criterion == CRIT_ONLY) flag = true;
                 else if (sc.criterion == CRIT_U1) flag = dmp_fails(p, Up, cand[p]);
+                else if (sc.criterion == CRIT_U2) flag = u2_fails(p, Up, cand[p], use_range ? &range : nullptr);
             }
             if (flag) triggered[p] = 1;
         }
@@ -965,7 +1021,27 @@ struct Sim : SimBase {
             if (orders[p] > 1) { orders[p] -= 1; changed[p] = 1; triggered_step[p] = 1; any = true; }
             else triggered[p] = 0;
         }
-        return any;
+        if (!any) return false;
+        if (sc.strategy == STRAT_SEPD0) force_neighbours();
+        return true;
+    }
+
+    // force_neighbours! (MOOD.jl:356-362) with forced_order for Halo{1}: interior particles pull the minimum order of
+    // their triggered neighbours from the snapshot taken after the decrement (no cascade)
+    void force_neighbours() {
+        snapshot_orders = orders;
+        force_pull();
+    }
+    void force_pull() {
+        for (int64_t j = 0; j < N; ++j) {
+            if (isb[j]) continue;
+            int o = snapshot_orders[j];
+            for (int64_t k = rstart[j]; k < rstart[j + 1]; ++k) {
+                const int64_t p = nbr[k];
+                if (triggered[p]) o = std::min(o, (int)snapshot_orders[p]);
+            }
+            if (o < snapshot_orders[j]) { orders[j] = o; changed[j] = 1; }
+        }
     }
 
     // expand_mask! one hop (MOOD.jl:323-351)
@@ -1001,7 +1077,7 @@ struct Sim : SimBase {
             }
             ++n_mood_iter;
             dirty = changed;
-            expand_dirty();
+            for (int hop = 0; hop < order_reach(); ++hop) expand_dirty();
             compute_rhs(Kp, Yp, false);
             full = false;
         }
@@ -1023,6 +1099,50 @@ struct Sim : SimBase {
         }
         mood_checkpoint(sc.s + 1, dt);
         rhos = rho_cand;
+    }
+
+    void interface_orders_of(int strategy, const int32_t* o, int64_t i, int64_t j, int32_t* out) override {
+        const int keep = sc.strategy;
+        sc.strategy = strategy;
+        orders.assign(o, o + N);
+        int a = 0, b = 0;
+        interface_orders(i, j, a, b);
+        out[0] = a; out[1] = b;
+        sc.strategy = keep;
+    }
+    int order_reach_of(int strategy) override {
+        const int keep = sc.strategy;
+        sc.strategy = strategy;
+        const int r = order_reach();
+        sc.strategy = keep;
+        return r;
+    }
+    void force_of(const uint8_t* trig, const int32_t* snap, int32_t* o, uint8_t* ch) override {
+        triggered.assign(trig, trig + N);
+        snapshot_orders.assign(snap, snap + N);
+        orders.assign(o, o + N);
+        changed.assign(ch, ch + N);
+        force_pull();
+        std::copy(orders.begin(), orders.end(), o);
+        std::copy(changed.begin(), changed.end(), ch);
+    }
+    void expand_of(uint8_t* mask, int hops) override {
+        dirty.assign(mask, mask + N);
+        for (int h = 0; h < hops; ++h) expand_dirty();
+        std::copy(dirty.begin(), dirty.end(), mask);
+    }
+    int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) override {
+        std::vector<State> Uv(N);
+        for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Uv[i][m] = U[i * M + m];
+        State c;
+        for (int m = 0; m < M; ++m) c[m] = cand[m];
+        const double kd = sc.delta, kp = sc.plateau;
+        sc.delta = delta; sc.plateau = plateau;
+        State range{};
+        if (plateau > 0.0) range = global_range(Uv);
+        const int r = u2_fails(p, Uv, c, plateau > 0.0 ? &range : nullptr) ? 1 : 0;
+        sc.delta = kd; sc.plateau = kp;
+        return r;
     }
 
     // one full right-hand side at orders = 2 on the given state (a gather + flux pass)
@@ -1126,7 +1246,8 @@ void orc_sim_set_threads(void* s, int n) { ((SimBase*)s)->set_threads(n); }
 // pde_kind: 1 linear advection (c: D*8 speeds, row per axis), 2 Euler (gamma)
 void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, int limiter, int mode, int strategy,
                         int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
-                        double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds) {
+                        double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds,
+                        double plateau) {
     SimBase* sb = (SimBase*)s;
     PDE e; e.kind = pde_kind; e.gamma = gamma;
     if (pde_kind == PDE_LINADV) {
@@ -1137,6 +1258,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
         }
     }
     Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
+    sc.plateau = plateau;
     sc.s = stages;
     for (int i = 0; i < stages; ++i) { sc.b_t[i] = b_t[i]; sc.c_t[i] = c_t[i]; for (int j = 0; j < stages; ++j) sc.a_t[i][j] = a_t[i * stages + j]; }
     sc.substep_ratio = substep_ratio;
@@ -1156,6 +1278,17 @@ void orc_sim_get_orders(void* s, int32_t* o) { ((SimBase*)s)->get_orders(o); }
 void orc_sim_get_flags(void* s, uint8_t* f) { ((SimBase*)s)->get_flags(f); }
 int64_t orc_sim_inadmissible(void* s) { return ((SimBase*)s)->inadmissible(); }
 int64_t orc_sim_mood_iterations(void* s) { return ((SimBase*)s)->mood_iterations(); }
+void orc_sim_interface_orders(void* s, int strategy, const int32_t* o, int64_t i, int64_t j, int32_t* out) {
+    ((SimBase*)s)->interface_orders_of(strategy, o, i, j, out);
+}
+int orc_sim_order_reach(void* s, int strategy) { return ((SimBase*)s)->order_reach_of(strategy); }
+void orc_sim_force(void* s, const uint8_t* trig, const int32_t* snap, int32_t* o, uint8_t* ch) {
+    ((SimBase*)s)->force_of(trig, snap, o, ch);
+}
+void orc_sim_expand(void* s, uint8_t* mask, int hops) { ((SimBase*)s)->expand_of(mask, hops); }
+int orc_sim_u2_flags(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau) {
+    return ((SimBase*)s)->u2_flags_of(U, p, cand, delta, plateau);
+}
 void orc_sim_get_bins(void* s, int32_t* bo, int64_t* cs, int32_t* cp, int64_t* dims) { ((SimBase*)s)->get_bins(bo, cs, cp, dims); }
 
 // scalar building blocks for known-answer tests

@@ -27,7 +27,7 @@ def lib():
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
-                                          d, C.c_int, vp, vp]),
+                                          d, C.c_int, vp, vp, d]),
             "orc_sim_set_state": (None, [vp, vp]),
             "orc_sim_get_state": (None, [vp, vp]),
             "orc_sim_time_step": (d, [vp]),
@@ -41,6 +41,11 @@ def lib():
             "orc_sim_inadmissible": (i64, [vp]),
             "orc_sim_mood_iterations": (i64, [vp]),
             "orc_sim_get_bins": (None, [vp, vp, vp, vp, vp]),
+            "orc_sim_interface_orders": (None, [vp, C.c_int, vp, i64, i64, vp]),
+            "orc_sim_order_reach": (C.c_int, [vp, C.c_int]),
+            "orc_sim_force": (None, [vp, vp, vp, vp, vp]),
+            "orc_sim_expand": (None, [vp, vp, C.c_int]),
+            "orc_sim_u2_flags": (C.c_int, [vp, vp, i64, vp, d, d]),
             "orc_weight": (d, [d, d, d]),
             "orc_limiter_phi": (d, [C.c_int, d]),
             "orc_jmax": (d, [d, d]),
@@ -124,7 +129,8 @@ class OracleSim:
         self._keep += [a_t, b_t, c_t, c, tags, kinds]
         lib().orc_sim_set_scheme(self.h, sch["pde"], sch.get("gamma", 1.4), _p(c), sch["limiter"],
                                  sch.get("limiter_mode", 1), sch["strategy"], sch["criterion"], sch.get("delta", 0.0), s,
-                                 _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds))
+                                 _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds),
+                                 sch.get("plateau", 0.0))
 
     def set_threads(self, n):
         lib().orc_sim_set_threads(self.h, int(n))
@@ -193,6 +199,34 @@ class OracleSim:
 
     def inadmissible(self):
         return lib().orc_sim_inadmissible(self.h)
+
+    # MOOD building blocks on their own (the reference's unit tests call the same functions)
+    def interface_orders(self, strategy, orders, i, j):
+        o = np.ascontiguousarray(orders, np.int32)
+        out = np.zeros(2, np.int32)
+        lib().orc_sim_interface_orders(self.h, strategy, _p(o), i, j, _p(out))
+        return int(out[0]), int(out[1])
+
+    def order_reach(self, strategy):
+        return lib().orc_sim_order_reach(self.h, strategy)
+
+    def force(self, triggered, snapshot, orders, changed):
+        t = np.ascontiguousarray(triggered, np.uint8)
+        sn = np.ascontiguousarray(snapshot, np.int32)
+        o = np.array(orders, np.int32)
+        c = np.array(changed, np.uint8)
+        lib().orc_sim_force(self.h, _p(t), _p(sn), _p(o), _p(c))
+        return o, c.astype(bool)
+
+    def expand(self, mask, hops):
+        m = np.array(mask, np.uint8)
+        lib().orc_sim_expand(self.h, _p(m), hops)
+        return m.astype(bool)
+
+    def u2_flags(self, U, p, cand, delta, plateau=0.0):
+        U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)
+        c = np.ascontiguousarray(cand, np.float64)
+        return bool(lib().orc_sim_u2_flags(self.h, _p(U), p, _p(c), delta, plateau))
 
     def mood_iterations(self):
         return lib().orc_sim_mood_iterations(self.h)

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from hcc_hip import Grid, Stepper
+from hcc_hip import _lib as L
 from hcc_hip import workloads as W
 from helpers import build_pair, first_diff, rel_linf
 
@@ -121,6 +122,46 @@ def test_euler2d_quadrants_steps(ctx, n, seed, steps):
     g, st, o = build_pair(ctx, wl)
     log = _run_steps(st, o, wl["cfl"], steps)
     assert sum(f for _, f, _ in log) > 0, "MOOD never fired: the case does not exercise the fixed point"
+
+
+MOOD_VARIANTS = {
+    # strategy, criterion, plateau, Euler substeps checked (else the stored candidate)
+    "epd2_u1_sub": (L.HCC_STRAT_EPD2, L.HCC_CRIT_U1, 0.0, True),
+    "strict_epd0_u1_sub": (L.HCC_STRAT_STRICT_EPD0, L.HCC_CRIT_U1, 0.0, True),
+    "epd1_u2_plateau_sub": (L.HCC_STRAT_EPD1, L.HCC_CRIT_U2, 1e-2, True),
+    "epd0_u2_stored": (L.HCC_STRAT_EPD0, L.HCC_CRIT_U2, 0.0, False),
+    "strict_epd0_u1_stored": (L.HCC_STRAT_STRICT_EPD0, L.HCC_CRIT_U1, 0.0, False),
+    "epd2_u2_plateau_stored": (L.HCC_STRAT_EPD2, L.HCC_CRIT_U2, 1e-3, False),
+    "strict_epd0_u2_plateau_sub": (L.HCC_STRAT_STRICT_EPD0, L.HCC_CRIT_U2, 1e-3, True),
+}
+
+
+@pytest.mark.parametrize("variant", list(MOOD_VARIANTS))
+def test_mood_variants_euler2d(ctx, variant):
+    """MOOD strategies EPD2 / StrictEPD0 and criterion MOODu2 (MOOD.jl:7-64, 140-270) on the four-quadrant Euler
+    problem: the unfused fixed point (forcing, two-hop recompute, global range) bit-exact step for step."""
+    strat, crit, plateau, sub = MOOD_VARIANTS[variant]
+    wl = W.euler_2d(n=48, seed=3)
+    sch = wl["scheme"]
+    wl["scheme"] = W.scheme(4, L.HCC_PDE_EULER, delta=sch["delta"], bcs=sch["bcs"], strategy=strat, criterion=crit,
+                            plateau=plateau, mood_substeps=sub)
+    g, st, o = build_pair(ctx, wl)
+    log = _run_steps(st, o, wl["cfl"], 8)
+    assert sum(f for _, f, _ in log) > 0, "MOOD never fired"
+
+
+def test_mood_epd2_u2_advection(ctx):
+    """EPD2 + MOODu2 with the plateau test on the periodic advection cloud (linear PDE, one component): an unlimited
+    MUSCL reconstruction of a square pulse on a sine background, so that MOOD is what keeps the update bounded."""
+    wl = W.advection_2d(64, 0)
+    sch = wl["scheme"]
+    x = wl["cloud"]["pos"]
+    wl["U0"] = np.all((x > 0.3) & (x < 0.6), axis=1).astype(float)[:, None] + 0.05 * np.sin(2 * np.pi * x[:, :1])
+    wl["scheme"] = W.scheme(1, L.HCC_PDE_LINADV, adv_speed=sch["adv_speed"], delta=sch["delta"], limiter=L.HCC_LIM_NONE,
+                            strategy=L.HCC_STRAT_EPD2, criterion=L.HCC_CRIT_U2, plateau=1e-3)
+    g, st, o = build_pair(ctx, wl)
+    log = _run_steps(st, o, wl["cfl"], 10)
+    assert sum(f for _, f, _ in log) > 0, "MOOD never fired"
 
 
 def test_euler3d_pulse_steps(ctx):
