@@ -99,6 +99,9 @@ struct StepArgs {
     double delta;
     double plateau;              // MOODu2: a component whose local range is below plateau x range[m] is not checked
     const double* range;         // MOODu2: global range of the previous stage per component (nullptr: plateau off)
+    int streamed;                // divergence accumulated per slot with the derivative weights (flux variant 1)
+    int aligned;                 // AlignedRusanovFlux: sort by the weights' signs below the full order (variant 1)
+    int packed;                  // the orders array of a flux pass holds EPD2's minimum | true order << 4
     const double* invg;          // reconstruction inverse moments, D*D per node: [q * N + i]
     const double* invd;          // divergence inverse moments, derivative rows 1..D: [(r * (D+1) + c) * N + i]
     const uint8_t* okg;

@@ -504,8 +504,15 @@ __device__ __forceinline__ void add_inadmissible(const StepArgs& a, int delta) {
 // MOODu1 (_dmp_fails, MOOD.jl:175-184) are the extrema of Y over the same neighbour loop. The nodes checked are
 // exactly those whose K is (re)computed (all interior nodes, or the dirty ones), as in detect_and_drop!. The
 // decrement (_mood_drop!) is applied by k_mark_gather: neighbours read this node's order in this very pass.
+//
+// FV selects the flux loop: 0 the buffered divergence fit with the geometric sorting (RusanovFlux), 1 the other
+// MUSCL variants by runtime flags: the streamed divergence (acc += sum_d c_k,d dF_d with the slot's derivative
+// weights c_k, _consume with CachedWeights, MUSCL.jl:431-461) and AlignedRusanovFlux (sorted by sign(c_k) where
+// either particle is below the full order, _dissipation_direction, MUSCL.jl:338-339). c_k is recomputed from the
+// divergence inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223), the bits of
+// the cached dweights (zero where the node has no cached fit).
 // ============================================================================================================
-template <int D, int M, int KIND, bool CHECK, class Src>
+template <int D, int M, int KIND, bool CHECK, int FV, class Src>
 __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
                                           const Src& src, double* __restrict__ K, int64_t i, double h) {
     using P = Pde<D, M, KIND>;
@@ -527,7 +534,18 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
         double gi[NG], Fi[D][M];
         load_rec<NG, L::GS>(a.G, i, gi);
         P::flux(a.pde, fi, Fi);
-        const int oi = a.orders[i];
+        int oi = a.orders[i], oi_t = oi;   // interface order source, true order
+        if (FV == 1 && a.packed) { oi_t = oi >> 4; oi &= 15; }
+        double X[FV == 1 ? D : 1][BD], sacc[M];
+        if constexpr (FV == 1) {
+            const bool okd = a.okd[i];
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int c = 0; c < BD; ++c) X[r][c] = okd ? a.invd[(r * BD + c) * N + i] : 0.0;
+#pragma unroll
+            for (int m = 0; m < M; ++m) sacc[m] = 0.0;
+        }
         double b[BD][D * M];
 #pragma unroll
         for (int c = 0; c < BD; ++c)
@@ -567,6 +585,8 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
             const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            int oj_t = oj;
+            if (FV == 1 && a.packed) { oj_t = oj >> 4; oj &= 15; }
             int o_i = oi, o_j = oj;
             if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
             double pii[D], pjj[D];
@@ -595,6 +615,23 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             pt[0] = 1.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
+            double cw[D], ctr[M];
+            bool alignd = false;
+            if constexpr (FV == 1) {
+                alignd = a.aligned && (oi_t < 2 || oj_t < 2);
+                if (a.streamed || alignd) {
+                    double v[BD];
+#pragma unroll
+                    for (int c = 0; c < BD; ++c) v[c] = w * pt[c];
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        double acc = X[r][0] * v[0];
+#pragma unroll
+                        for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
+                        cw[r] = acc * invL;
+                    }
+                }
+            }
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 // sort_flux orders the two sides by sign(d). Three exact shortcuts: F_L + F_R is the same sum either
@@ -605,19 +642,30 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
                 // exact cancellation gives +0), so a zero dF of either sign leaves them bit for bit unchanged, and
                 // the + 0.0 is dropped as well.
                 const double sd = jmax(lij[d], lji[d]);
-                const bool right = dd[d] > 0.0;
+                const bool right = alignd ? cw[d] > 0.0 : dd[d] > 0.0;
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
                     const double x = fji[m] - fij[m];
                     const double diss = sd * (right ? x : -x);
                     const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
                     const double dF = Fnum - Fi[d][m];
+                    if (FV == 1 && a.streamed) {
+                        ctr[m] = d == 0 ? 0.0 + cw[0] * dF : ctr[m] + cw[d] * dF;
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
+                        for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
+                    }
                 }
             }
+            if (FV == 1 && a.streamed)
+#pragma unroll
+                for (int m = 0; m < M; ++m) sacc[m] = sacc[m] + ctr[m];
         }
-        if (a.okd[i]) {
+        if (FV == 1 && a.streamed) {
+            const bool okd = a.okd[i];
+#pragma unroll
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? sacc[m] : 0.0)) + 0.0;
+        } else if (a.okd[i]) {
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 double acc = 0.0;
@@ -694,7 +742,7 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
     }
 }
 
-template <int D, int M, int KIND, bool CHECK>
+template <int D, int M, int KIND, bool CHECK, int FV>
 __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int full, int64_t nblk) {
     const double h = CHECK ? a.hratio * *a.dtp : 0.0;   // MOOD's Euler substep (form_substep!, TimeSteppers.jl:501-518)
@@ -708,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, cons
 #pragma unroll
                 for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else if (full || a.dirty[i]) {
-            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK, FV>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
             if (CHECK) bad = a.inadm[i];
         }
     }
@@ -719,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, cons
 }
 
 // The full flux pass with the workgroup's halo (records, gradients, orders) staged in LDS (2-D).
-template <int D, int M, int KIND, bool CHECK>
+template <int D, int M, int KIND, bool CHECK, int FV>
 __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                          double* __restrict__ K, int64_t nblk) {
     extern __shared__ double sh[];
@@ -736,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
 #pragma unroll
             for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else {
-            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, src, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK, FV>(a, pred, Y, src, K, i, h);
             if (CHECK) bad = a.inadm[i];
         }
     }
@@ -749,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
 // The flux pass (with the MOOD check) of the fixed-point iteration with parity p over its work list (dlist, length
 // read on the device): the neighbourhoods of the changed particles, so a particle can appear several times; the
 // thread that clears its dirty mark recomputes it, the others leave. Changes go to changed list 1 - p.
-template <int D, int M, int KIND>
+template <int D, int M, int KIND, int FV>
 __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                        double* __restrict__ K, int p) {
     const double h = a.hratio * *a.dtp;
@@ -764,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred,
             // a particle listed several times (it neighbours several changed particles) is taken by one thread
             if (atomicExch(&a.dmark[i], 0u) == 1u && !a.g.isb[i]) {
                 const int before = a.inadm[i];
-                ch = flux_node<D, M, KIND, true>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
+                ch = flux_node<D, M, KIND, true, FV>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
                 delta = (int)a.inadm[i] - before;
             }
         }
@@ -937,7 +985,8 @@ __global__ __launch_bounds__(kBlock) void k_copy_u8(const uint8_t* __restrict__ 
 
 // _nb_min_order (MOOD.jl:66-74) of every particle: EPD2's interface order of the edge (i, j) is the minimum of the
 // two (_get_interface_orders, MOOD.jl:82-85), so the flux pass reads these as orders with the edge-minimum rule.
-__global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict__ out, int64_t nblk) {
+// pack: AlignedRusanovFlux also needs the true orders (whether a pair is below the full order): out = min | order << 4.
+__global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict__ out, int pack, int64_t nblk) {
     const int64_t i = node_index(nblk);
     if (i >= a.g.N) return;
     int o = a.orders[i];
@@ -947,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict
         const int oj = a.orders[a.g.nbr[base + (int64_t)k * kChunk]];
         o = o < oj ? o : oj;
     }
-    out[i] = (int8_t)o;
+    out[i] = (int8_t)(pack ? o | (a.orders[i] << 4) : o);
 }
 
 // _global_range (MOOD.jl:206-213) of the state components of the records Y[0 .. N): each workgroup reduces its nodes
@@ -1216,6 +1265,7 @@ struct hcc_stepper {
     hcc_grid* grid = nullptr;
     hcc_stepper_desc desc{};
     int D = 0, M = 0, kind = 0, PU = 0, GS = 0;
+    int fv = 0;                // flux loop variant: 0 buffered fit + RusanovFlux, 1 streamed and/or AlignedRusanovFlux
     int64_t N = 0;
     // geometry caches
     double* invg = nullptr; uint8_t* okg = nullptr;
@@ -1296,6 +1346,9 @@ StepArgs hcc_stepper::args() const {
     a.limiter = desc.limiter; a.mode = desc.limiter_mode; a.strategy = desc.mood_strategy;
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
     a.plateau = desc.mood_plateau;
+    a.streamed = desc.muscl_streamed ? 1 : 0;
+    a.aligned = desc.flux == HCC_FLUX_ALIGNED_RUSANOV ? 1 : 0;
+    a.packed = 0;
     a.range = (desc.mood_criterion == HCC_CRIT_U2 && desc.mood_plateau > 0.0) ? range : nullptr;
     a.invg = invg; a.invd = invd; a.okg = okg; a.okd = okd;
     a.G = G; a.orders = orders; a.changed = changed; a.dirty = dirty; a.trig = trig; a.inadm = inadm;
@@ -1321,17 +1374,25 @@ struct Variant {
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
         k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub, p);
     }
-    static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
-                     bool check) {
+    template <int FV>
+    static void flux_fv(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
+                        bool check) {
         const int64_t nb = nblocks(s->N);
         if (full && s->lds_flux) {
-            if (check) k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
-            else k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
-        } else if (check) k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
-        else k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+            if (check) k_flux_staged<D, M, KIND, true, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
+            else k_flux_staged<D, M, KIND, false, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
+        } else if (check) k_flux<D, M, KIND, true, FV><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+        else k_flux<D, M, KIND, false, FV><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+    }
+    static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
+                     bool check) {
+        if (s->fv) flux_fv<1>(s, a, p, Y, K, full, check);
+        else flux_fv<0>(s, a, p, Y, K, full, check);
     }
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
-        k_flux_list<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 1024), kBlock, 0, st(s)>>>(a, pr, Y, K, p);
+        const unsigned g = (unsigned)std::min<int64_t>(nblocks(s->N), 1024);
+        if (s->fv) k_flux_list<D, M, KIND, 1><<<g, kBlock, 0, st(s)>>>(a, pr, Y, K, p);
+        else k_flux_list<D, M, KIND, 0><<<g, kBlock, 0, st(s)>>>(a, pr, Y, K, p);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
                      int full) {
@@ -1437,6 +1498,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     if (d->mood_strategy < HCC_STRAT_NONE || d->mood_strategy > HCC_STRAT_STRICT_EPD0) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD strategy");
     if (d->mood_criterion < HCC_CRIT_NONE || d->mood_criterion > HCC_CRIT_U2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD criterion");
     if (!(d->mood_plateau >= 0.0) || !std::isfinite(d->mood_plateau)) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: MOODu2 plateau must be finite and >= 0");
+    if (d->flux != HCC_FLUX_RUSANOV && d->flux != HCC_FLUX_ALIGNED_RUSANOV) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: numerical flux");
     if (d->n_bc < 0 || d->n_bc > 16) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: at most 16 boundary conditions");
     for (int i = 0; i < d->stages; ++i)
         for (int j = i; j < d->stages; ++j)
@@ -1445,6 +1507,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     std::unique_ptr<hcc_stepper, int (*)(hcc_stepper*)> s(new hcc_stepper(), hcc_stepper_destroy);
     s->ctx = ctx; s->grid = g; s->desc = *d; s->D = D; s->M = d->M; s->kind = d->pde; s->N = g->N;
     s->PU = ops.PU; s->GS = ops.GS;
+    s->fv = (d->muscl_streamed || d->flux == HCC_FLUX_ALIGNED_RUSANOV) ? 1 : 0;
     const int64_t N = g->N, M = d->M;
     hipStream_t st = ctx->stream;
     HCC_TRY_HIP(ctx, dalloc(&s->invg, (int64_t)D * D * N));
@@ -1622,10 +1685,11 @@ struct StepRunner {
     // neighbourhood minima, i.e. the edge-minimum rule of EPD1 applied to nbmin (computed before the pass)
     StepArgs flux_args() {
         if (!epd2()) return a;
-        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, nb);
+        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, nb);
         StepArgs f = a;
         f.orders = s->nbmin;
         f.strategy = HCC_STRAT_EPD1;
+        f.packed = a.aligned;
         return f;
     }
     // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself

@@ -17,6 +17,7 @@ HCC_LIM_NONE, HCC_LIM_BJ, HCC_LIM_VK = 0, 1, 2
 HCC_MODE_SOFT, HCC_MODE_HARD = 0, 1
 HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1, HCC_STRAT_EPD2, HCC_STRAT_STRICT_EPD0 = 0, 1, 2, 3, 4
 HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY, HCC_CRIT_U2 = 0, 1, 2, 3, 4
+HCC_FLUX_RUSANOV, HCC_FLUX_ALIGNED_RUSANOV = 0, 1
 HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
 HCC_E_CAPACITY = -4
 
@@ -43,7 +44,8 @@ class StepperDesc(C.Structure):
                 ("mood_strategy", C.c_int32), ("mood_criterion", C.c_int32), ("mood_delta", C.c_double),
                 ("stages", C.c_int32), ("a_t", (C.c_double * 5) * 5), ("b_t", C.c_double * 5), ("c_t", C.c_double * 5),
                 ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
-                ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double)]
+                ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double), ("muscl_streamed", C.c_int32),
+                ("flux", C.c_int32)]
 
 
 _lib = None
@@ -243,6 +245,8 @@ def stepper_desc(scheme):
     sd.mood_strategy, sd.mood_criterion = scheme["strategy"], scheme["criterion"]
     sd.mood_delta = scheme.get("delta", 0.0)
     sd.mood_plateau = scheme.get("plateau", 0.0)
+    sd.muscl_streamed = int(bool(scheme.get("streamed", False)))
+    sd.flux = scheme.get("flux", HCC_FLUX_RUSANOV)
     a_t, b_t, c_t = scheme["a_t"], scheme["b_t"], scheme["c_t"]
     s = len(b_t)
     sd.stages = s

@@ -55,6 +55,8 @@ struct StepperDesc
     bc_tag::NTuple{16, Int32}
     bc_kind::NTuple{16, Int32}
     mood_plateau::Float64
+    muscl_streamed::Int32
+    flux::Int32
 end
 
 checkrc(rc, ctx) = rc == 0 || error("libhcc: ", unsafe_string(ccall((:hcc_last_error, libhcc), Cstring, (Ptr{Cvoid},), ctx)))

@@ -99,6 +99,8 @@ enum { HCC_MODE_SOFT = 0, HCC_MODE_HARD = 1 };
 enum { HCC_STRAT_NONE = 0, HCC_STRAT_EPD0 = 1, HCC_STRAT_EPD1 = 2, HCC_STRAT_EPD2 = 3, HCC_STRAT_STRICT_EPD0 = 4 };
 enum { HCC_CRIT_NONE = 0, HCC_CRIT_U1 = 1, HCC_CRIT_PAD = 2, HCC_CRIT_ONLY = 3, HCC_CRIT_U2 = 4 };
 enum { HCC_BC_NONE = 0, HCC_BC_OUTFLOW = 1, HCC_BC_DIRICHLET = 2 };
+/* numerical flux (src/Interpolation/FluxFunctions.jl:66-111): RusanovFlux, AlignedRusanovFlux */
+enum { HCC_FLUX_RUSANOV = 0, HCC_FLUX_ALIGNED_RUSANOV = 1 };
 
 typedef struct {
     int32_t M;                 /* state components */
@@ -115,6 +117,8 @@ typedef struct {
     int32_t n_bc;
     int32_t bc_tag[16], bc_kind[16];
     double mood_plateau;       /* MOODu2 plateau (fraction of the global range of the previous stage; 0: off) */
+    int32_t muscl_streamed;    /* MUSCL(...; streamed = true): divergence accumulated per slot (MUSCL.jl:56, 407-408) */
+    int32_t flux;              /* HCC_FLUX_* */
 } hcc_stepper_desc;
 
 /* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL

@@ -453,6 +453,8 @@ struct Scheme {
     int strategy = STRAT_EPD1, criterion = CRIT_U1;
     double delta = 0.0;
     double plateau = 0.0;      // MOODu2 plateau fraction of the global range (0: off)
+    int streamed = 0;          // MUSCL(...; streamed = true): divergence accumulated per slot with dweights
+    int aligned = 0;           // AlignedRusanovFlux: sort by sign(dweights) on pairs below the full order
     int s = 3;
     double a_t[5][5] = {}, b_t[5] = {}, c_t[5] = {};
     double substep_ratio = 0.0;
@@ -475,6 +477,7 @@ struct SimBase {
     virtual double time_step() = 0;
     virtual void step(double t, double dt) = 0;
     virtual void rhs(const double* Y, double* K, double* grads) = 0;
+    virtual void rhs_at(const double* Y, const int32_t* orders, double* K, double* grads) = 0;
     virtual int64_t num_slots() const = 0;
     virtual void get_neighbors(int64_t* ranges, int32_t* idx, double* w, double* dist) const = 0;
     virtual void get_moments(double* invg, uint8_t* okg, double* invd, uint8_t* okd, double* dweights) const = 0;
@@ -818,7 +821,11 @@ struct Sim : SimBase {
     // _order_reach (MUSCL_hooks.jl:10-11): the hops over which a flux depends on neighbours' orders
     int order_reach() const { return sc.strategy == STRAT_EPD2 ? 2 : 1; }
 
-    // MUSCL functor (MUSCL.jl:390-424) with _flux_loop (:463-508) and apply_weights_divergence
+    // MUSCL functor (MUSCL.jl:390-424) with _flux_loop (:463-508) and apply_weights_divergence. streamed: the
+    // divergence is accumulated in the loop, acc += sum_d dweights[k][d] * dF[d] (_consume with CachedWeights,
+    // :431-461), and K = -(2 acc); AlignedRusanovFlux: _dissipation_direction (:338-339) sorts by sign(dweights[k])
+    // where either particle is below the full order (MUSCL order 2: order 1). Without a cached fit (okd false) both
+    // take the dynamic fit, whose singular moment matrix gives a zero divergence as in the buffered path.
     // (Interpolators.jl:610-626); returns K_E = -2 div + 0 (TimeSteppers.jl:354)
     void flux_node(int64_t i, const std::vector<State>& U, State& KE) {
         int64_t k0 = rstart[i], k1 = rstart[i + 1];
@@ -846,8 +853,10 @@ struct Sim : SimBase {
             P::flux(pde, fij.data(), Fij); P::flux(pde, fji.data(), Fji);
             double lij[D], lji[D];
             P::maxeig(pde, fij.data(), lij); P::maxeig(pde, fji.data(), lji);
+            const bool below = orders[i] < 2 || orders[j] < 2;
             for (int d = 0; d < D; ++d) {
-                bool pos_ = dist[k][d] > 0.0;           // sort_flux (Utils.jl:95-103)
+                // sort_flux (Utils.jl:95-103) by _dissipation_direction: sign(w_k) > 0 is w_k > 0
+                bool pos_ = (sc.aligned && below) ? dweights[k][d] > 0.0 : dist[k][d] > 0.0;
                 const double* fL = pos_ ? fij.data() : fji.data();
                 const double* fR = pos_ ? fji.data() : fij.data();
                 const double* FL = pos_ ? Fij[d] : Fji[d];
@@ -862,7 +871,17 @@ struct Sim : SimBase {
             }
         }
         double div[M];
-        if (okd[i]) {
+        if (sc.streamed && okd[i]) {
+            State acc{};
+            for (int64_t k = k0; k < k1; ++k) {
+                for (int m = 0; m < M; ++m) {
+                    double c = 0.0;                      // _contract: zero(State) + w[d] * F[d] over d
+                    for (int d = 0; d < D; ++d) c = c + dweights[k][d] * dflux[k][d][m];
+                    acc[m] = acc[m] + c;
+                }
+            }
+            for (int m = 0; m < M; ++m) div[m] = acc[m];
+        } else if (okd[i]) {
             double b[BD][D * M] = {};
             for (int64_t k = k0; k < k1; ++k) {
                 double p[BD]; p[0] = 1.0;
@@ -1146,11 +1165,14 @@ struct Sim : SimBase {
     }
 
     // one full right-hand side at orders = 2 on the given state (a gather + flux pass)
-    void rhs(const double* Yin, double* Kout, double* gout) override {
+    void rhs(const double* Yin, double* Kout, double* gout) override { rhs_at(Yin, nullptr, Kout, gout); }
+    // the same with given particle orders (nullptr: the maximum order everywhere)
+    void rhs_at(const double* Yin, const int32_t* ord, double* Kout, double* gout) override {
         std::vector<State> Yv(N), Kv(N);
         for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Yv[i][m] = Yin[i * M + m];
         for (auto& gr : grads) gr = {};
-        std::fill(orders.begin(), orders.end(), 2);
+        if (ord) orders.assign(ord, ord + N);
+        else std::fill(orders.begin(), orders.end(), 2);
         compute_rhs(Kv, Yv, true);
         for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Kout[i * M + m] = Kv[i][m];
         if (gout) for (int64_t i = 0; i < N; ++i) for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) gout[(i * BG + a) * M + m] = grads[i][a][m];
@@ -1247,7 +1269,7 @@ void orc_sim_set_threads(void* s, int n) { ((SimBase*)s)->set_threads(n); }
 void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, int limiter, int mode, int strategy,
                         int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
                         double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds,
-                        double plateau) {
+                        double plateau, int streamed, int aligned) {
     SimBase* sb = (SimBase*)s;
     PDE e; e.kind = pde_kind; e.gamma = gamma;
     if (pde_kind == PDE_LINADV) {
@@ -1259,6 +1281,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
     }
     Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
     sc.plateau = plateau;
+    sc.streamed = streamed; sc.aligned = aligned;
     sc.s = stages;
     for (int i = 0; i < stages; ++i) { sc.b_t[i] = b_t[i]; sc.c_t[i] = c_t[i]; for (int j = 0; j < stages; ++j) sc.a_t[i][j] = a_t[i * stages + j]; }
     sc.substep_ratio = substep_ratio;
@@ -1271,6 +1294,9 @@ void orc_sim_get_state(void* s, double* U) { ((SimBase*)s)->get_state(U); }
 double orc_sim_time_step(void* s) { return ((SimBase*)s)->time_step(); }
 void orc_sim_step(void* s, double t, double dt) { ((SimBase*)s)->step(t, dt); }
 void orc_sim_rhs(void* s, const double* Y, double* K, double* grads) { ((SimBase*)s)->rhs(Y, K, grads); }
+void orc_sim_rhs_orders(void* s, const double* Y, const int32_t* orders, double* K) {
+    ((SimBase*)s)->rhs_at(Y, orders, K, nullptr);
+}
 int64_t orc_sim_num_slots(void* s) { return ((SimBase*)s)->num_slots(); }
 void orc_sim_get_neighbors(void* s, int64_t* ranges, int32_t* idx, double* w, double* dist) { ((SimBase*)s)->get_neighbors(ranges, idx, w, dist); }
 void orc_sim_get_moments(void* s, double* ig, uint8_t* og, double* id, uint8_t* od, double* dw) { ((SimBase*)s)->get_moments(ig, og, id, od, dw); }

@@ -27,12 +27,13 @@ def lib():
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
-                                          d, C.c_int, vp, vp, d]),
+                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int]),
             "orc_sim_set_state": (None, [vp, vp]),
             "orc_sim_get_state": (None, [vp, vp]),
             "orc_sim_time_step": (d, [vp]),
             "orc_sim_step": (None, [vp, d, d]),
             "orc_sim_rhs": (None, [vp, vp, vp, vp]),
+            "orc_sim_rhs_orders": (None, [vp, vp, vp, vp]),
             "orc_sim_num_slots": (i64, [vp]),
             "orc_sim_get_neighbors": (None, [vp, vp, vp, vp, vp]),
             "orc_sim_get_moments": (None, [vp, vp, vp, vp, vp, vp]),
@@ -60,6 +61,9 @@ def lib():
             f = getattr(_lib, name)
             f.restype, f.argtypes = res, args
     return _lib
+
+
+FLUX_RUSANOV, FLUX_ALIGNED_RUSANOV = 0, 1   # scheme["flux"]: RusanovFlux, AlignedRusanovFlux
 
 
 def _p(a):
@@ -130,7 +134,8 @@ class OracleSim:
         lib().orc_sim_set_scheme(self.h, sch["pde"], sch.get("gamma", 1.4), _p(c), sch["limiter"],
                                  sch.get("limiter_mode", 1), sch["strategy"], sch["criterion"], sch.get("delta", 0.0), s,
                                  _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds),
-                                 sch.get("plateau", 0.0))
+                                 sch.get("plateau", 0.0), int(bool(sch.get("streamed", False))),
+                                 int(sch.get("flux", FLUX_RUSANOV) == FLUX_ALIGNED_RUSANOV))
 
     def set_threads(self, n):
         lib().orc_sim_set_threads(self.h, int(n))
@@ -156,6 +161,14 @@ class OracleSim:
         G = np.zeros((self.N, self.D, self.M))
         lib().orc_sim_rhs(self.h, _p(Y), _p(K), _p(G))
         return K, G
+
+    def rhs_at(self, Y, orders):
+        """K_E of a full gather + flux pass with the given particle orders."""
+        Y = np.ascontiguousarray(Y, np.float64).reshape(self.N, self.M)
+        o = np.ascontiguousarray(orders, np.int32)
+        K = np.zeros((self.N, self.M))
+        lib().orc_sim_rhs_orders(self.h, _p(Y), _p(o), _p(K))
+        return K
 
     def neighbors(self):
         S = lib().orc_sim_num_slots(self.h)

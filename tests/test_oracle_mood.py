@@ -215,3 +215,64 @@ def test_epd2_drops_more_interfaces_than_epd1():
             assert e2[0] <= e1[0]
             near = {5, *idx[r[5]:r[6]]}
             assert (e2[0] == 1) == (i in near or j in near)
+
+
+# ---- MUSCL variants: streamed divergence, AlignedRusanovFlux (unit_schemes.jl:6-68, 397-427) --------------------
+def _periodic_cloud(n, R_factor, jitter, seed, M, speeds, **kw):
+    dom = W.domain(2, (0, 0), (1, 1), (True, True), 1 / n, R_factor, jitter, seed)
+    c = O.cloud_generate(dom)
+    sch = W.scheme(M, L.HCC_PDE_LINADV, adv_speed=speeds, limiter=L.HCC_LIM_NONE, strategy=L.HCC_STRAT_NONE,
+                   criterion=L.HCC_CRIT_NONE, **kw)
+    return O.OracleSim(dom, c["pos"], c["is_boundary"], c["tags"], sch, threads=4), c["pos"]
+
+
+def _first_order_coefficients(flux):
+    """K_i = sum_j a_ij (u_j - u_i) at order 1 everywhere (constant reconstructions): column j from a unit state at j."""
+    o, _ = _periodic_cloud(24, 2.5, 0.2, 9, 1, [[1.0], [0.7]], flux=flux)
+    r, idx, _, _ = o.neighbors()
+    ones = np.ones(o.N, np.int32)
+    a = []
+    for j in range(o.N):
+        U = np.zeros((o.N, 1))
+        U[j] = 1.0
+        K = o.rhs_at(U, ones)[:, 0]
+        a.append(K)
+    A = np.array(a).T                     # A[i, j] = a_ij
+    return np.concatenate([A[i, idx[r[i]:r[i + 1]]] for i in range(o.N)])
+
+
+def test_aligned_rusanov_first_order_coefficients_are_non_negative():
+    """unit_schemes.jl:40-68: sorted by the signs of the divergence weights every first-order coefficient a_ij is
+    non-negative (the update is local-extremum-diminishing); sorted by the distance some are negative on a jittered
+    cloud with R = 2.5 dx."""
+    a_al = _first_order_coefficients(O.FLUX_ALIGNED_RUSANOV)
+    a_geo = _first_order_coefficients(O.FLUX_RUSANOV)
+    assert len(a_al) > 1000 and a_al.min() >= -1e-12
+    assert np.count_nonzero(a_geo < -1e-12) > 0
+
+
+def test_streamed_divergence_matches_the_buffered_fit():
+    """unit_schemes.jl:397-427: the streamed flux pass gives the divergence of the buffered fit up to round-off, with
+    both fluxes, at full order and with a strip of particles dropped to order 1; at full order the aligned flux is the
+    plain one bit for bit (no pair is below the full order)."""
+    speeds = [[1.0, 0.5], [0.7, -0.3]]
+    res = {}
+    for flux in (O.FLUX_RUSANOV, O.FLUX_ALIGNED_RUSANOV):
+        for streamed in (False, True):
+            o, x = _periodic_cloud(24, 3.5, 0.2, 21, 2, speeds, flux=flux, streamed=streamed)
+            U = np.column_stack([np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.05),
+                                 np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1])])
+            for drop in (False, True):
+                orders = np.full(o.N, 2, np.int32)
+                if drop:
+                    orders[np.abs(x[:, 0] - 0.5) < 0.1] = 1
+                res[flux, streamed, drop] = o.rhs_at(U, orders)
+    for flux in (O.FLUX_RUSANOV, O.FLUX_ALIGNED_RUSANOV):
+        for drop in (False, True):
+            Kb, Ks = res[flux, False, drop], res[flux, True, drop]
+            scale = np.abs(Kb).max()
+            assert np.abs(Kb - Ks).max() < 1e-11 * scale
+            assert not np.array_equal(Kb, Ks)          # a different summation order: not the same bits
+    for streamed in (False, True):
+        assert np.array_equal(res[O.FLUX_RUSANOV, streamed, False], res[O.FLUX_ALIGNED_RUSANOV, streamed, False])
+        assert not np.array_equal(res[O.FLUX_RUSANOV, streamed, True], res[O.FLUX_ALIGNED_RUSANOV, streamed, True])
