@@ -117,8 +117,9 @@ struct StepArgs {
                                  // lengths of iterations with parity 0 / 1
     int32_t* clist;              // particles whose order changed in a check (changed list 0)
     int32_t* clist2;             // changed list 1
-    int32_t* dlist;              // neighbourhoods of the changed particles (with repeats), the recompute's work list
-    unsigned* dmark;             // dirty particles of the fixed point (changed plus one hop); cleared by the recompute
+    int32_t* dlist;              // the recompute's work list: the dirty particles, each once
+    unsigned* dmark;             // dirty particles of the fixed point (changed plus one hop); set by the mark kernel
+                                 // (the setter lists the node), cleared by the recompute
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
     double* bmin;                // per-workgroup minima of the CFL estimate
     double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima

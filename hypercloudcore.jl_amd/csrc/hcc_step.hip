@@ -512,45 +512,204 @@ __device__ __forceinline__ void add_inadmissible(const StepArgs& a, int delta) {
 // divergence inverse and the slot geometry with the operations of _weights_body! (MUSCL.jl:211-223), the bits of
 // the cached dweights (zero where the node has no cached fit).
 // ============================================================================================================
+// The node's own data for its flux loop: state record, flux, gradients, orders and (FV = 1) the derivative rows of
+// its divergence inverse (zero without a cached fit).
+template <int D, int M, int KIND, int FV>
+struct FluxNode {
+    using L = Layout<D, M>;
+    double ri[L::NR], fi[M], gi[L::NG], Fi[D][M];
+    double X[FV == 1 ? D : 1][D + 1];
+    int oi, oi_t;   // interface order source, true order
+};
+
+template <int D, int M, int KIND, int FV>
+__device__ __forceinline__ void load_flux_node(const StepArgs& a, const double* __restrict__ Y, int64_t i,
+                                               FluxNode<D, M, KIND, FV>& nd) {
+    using L = Layout<D, M>;
+    constexpr int BD = D + 1;
+    load_rec<L::NR, L::PU>(Y, i, nd.ri);
+#pragma unroll
+    for (int m = 0; m < M; ++m) nd.fi[m] = nd.ri[D + m];
+    load_rec<L::NG, L::GS>(a.G, i, nd.gi);
+    Pde<D, M, KIND>::flux(a.pde, nd.fi, nd.Fi);
+    nd.oi = a.orders[i];
+    nd.oi_t = nd.oi;
+    if (FV == 1 && a.packed) { nd.oi_t = nd.oi >> 4; nd.oi &= 15; }
+    if constexpr (FV == 1) {
+        const bool okd = a.okd[i];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < BD; ++c) nd.X[r][c] = okd ? a.invd[(r * BD + c) * a.g.N + i] : 0.0;
+    }
+}
+
+// One slot of the flux loop (MUSCL.jl:473-506): the interface states at the interface orders, Rusanov per axis and the
+// flux difference dF = F* - F_i, the fit weight w and basis pt = [1, d invL]; with the streamed divergence (FV = 1)
+// also its contraction ctr = sum_d c_k,d dF_d.
+template <int D, int M, int KIND, int FV>
+__device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M, KIND, FV>& nd,
+                                          const double (&rj)[Layout<D, M>::NR], const double (&gj)[Layout<D, M>::NG],
+                                          int oj, double (&dF)[D][M], double& w, double (&pt)[D + 1], double (&ctr)[M]) {
+    using P = Pde<D, M, KIND>;
+    constexpr int BD = D + 1;
+    const double invL = a.g.invL;
+    double dd[D];
+    const double d2 = rec_geom<D>(a.g, nd.ri, rj, dd);
+    w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+    int oj_t = oj;
+    if (FV == 1 && a.packed) { oj_t = oj >> 4; oj &= 15; }
+    int o_i = nd.oi, o_j = oj;
+    if (a.strategy == HCC_STRAT_EPD1) { o_i = nd.oi < oj ? nd.oi : oj; o_j = o_i; }
+    double pii[D], pjj[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        pii[d] = (o_i == 2) ? 0.5 * dd[d] : 0.0;
+        pjj[d] = (o_j == 2) ? -0.5 * dd[d] : 0.0;
+    }
+    double fij[M], fji[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double si = nd.gi[m] * pii[0];
+#pragma unroll
+        for (int q = 1; q < D; ++q) si = si + nd.gi[q * M + m] * pii[q];
+        double sj = gj[m] * pjj[0];
+#pragma unroll
+        for (int q = 1; q < D; ++q) sj = sj + gj[q * M + m] * pjj[q];
+        const double uj = rj[D + m];
+        fij[m] = nd.fi[m] + si;
+        fji[m] = uj + sj;
+    }
+    double Fij[D][M], Fji[D][M], lij[D], lji[D];
+    P::flux_eig(a.pde, fij, Fij, lij);
+    P::flux_eig(a.pde, fji, Fji, lji);
+    pt[0] = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
+    double cw[D];
+    bool alignd = false;
+    if constexpr (FV == 1) {
+        alignd = a.aligned && (nd.oi_t < 2 || oj_t < 2);
+        if (a.streamed || alignd) {
+            double v[BD];
+#pragma unroll
+            for (int c = 0; c < BD; ++c) v[c] = w * pt[c];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                double acc = nd.X[r][0] * v[0];
+#pragma unroll
+                for (int c = 1; c < BD; ++c) acc = acc + nd.X[r][c] * v[c];
+                cw[r] = acc * invL;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        // sort_flux orders the two sides by sign(d). Three exact shortcuts: F_L + F_R is the same sum either way
+        // (IEEE addition commutes); max(λ_L, λ_R) is symmetric for the non-negative speeds |u| + c (a NaN stays a
+        // NaN); f_R - f_L = ±(f_ji - f_ij) up to the sign of a zero difference. The sign of a zero dissipation only
+        // reaches dF = F* - F_i when dF is a zero, and the reference's + nc_jump (+0.0) turns such a zero into +0;
+        // the fit accumulators start at +0 and can never hold -0 (x + (-0) = x, exact cancellation gives +0), so a
+        // zero dF of either sign leaves them bit for bit unchanged, and the + 0.0 is dropped as well.
+        const double sd = jmax(lij[d], lji[d]);
+        const bool right = alignd ? cw[d] > 0.0 : dd[d] > 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const double x = fji[m] - fij[m];
+            const double diss = sd * (right ? x : -x);
+            const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
+            dF[d][m] = Fnum - nd.Fi[d][m];
+            if (FV == 1 && a.streamed) ctr[m] = d == 0 ? 0.0 + cw[0] * dF[d][m] : ctr[m] + cw[d] * dF[d][m];
+        }
+    }
+}
+
+// The divergence of the buffered fit (apply_weights_divergence, Interpolators.jl:610-626) from the accumulators of
+// the derivative rows, K_E = -2 div (+ zero sources); a node without a cached fit gets a zero divergence.
+template <int D, int M>
+__device__ __forceinline__ void fit_divergence(const StepArgs& a, int64_t i, const double (&b)[D + 1][D * M],
+                                               double (&Kout)[M]) {
+    constexpr int BD = D + 1;
+    const int64_t N = a.g.N;
+    if (a.okd[i]) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
+#pragma unroll
+                for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
+                acc = acc + r;
+            }
+            Kout[m] = -(2.0 * (acc * a.g.invL)) + 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+    }
+}
+
+// After K_E of node i: the store, the fused stage predictor and (CHECK) MOOD's candidate check with the decrement
+// deferred to k_mark_gather. Returns whether the node's order is to drop.
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pred, double* __restrict__ K, int64_t i,
+                                          const double (&fi)[M], const double (&Kout)[M], double h) {
+    using P = Pde<D, M, KIND>;
+#pragma unroll
+    for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
+    if (pred.on) fused_predict<D, M>(pred, i, Kout);
+    if (!CHECK) return false;
+    double W[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
+    const bool adm = P::admissible(a.pde, W);
+    a.inadm[i] = !adm;
+    bool flag = !adm;
+    if (!flag) {
+        if (a.criterion == HCC_CRIT_ONLY) {
+            flag = true;
+        } else if (a.criterion == HCC_CRIT_U1) {
+            // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
+            // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
+            double e[2 * M], mx[M], mn[M];
+            load_rec<2 * M, 2 * M>(a.ext, i, e);
+#pragma unroll
+            for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+            const double dl = a.delta, d3 = dl * dl * dl;
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
+        }
+    }
+    if (flag && a.orders[i] > 1) { a.trig[i] = 1; return true; }
+    return false;
+}
+
 template <int D, int M, int KIND, bool CHECK, int FV, class Src>
 __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
                                           const Src& src, double* __restrict__ K, int64_t i, double h) {
-    using P = Pde<D, M, KIND>;
     using L = Layout<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = D + 1;
-    const int64_t N = a.g.N;
     const int cnt = a.g.cnt[i];
-    double ri[NR];
-    load_rec<NR, L::PU>(Y, i, ri);
-    double fi[M], Kout[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
+    FluxNode<D, M, KIND, FV> nd;
+    double Kout[M];
     if (cnt == 0) {
+        load_rec<NR, L::PU>(Y, i, nd.ri);
+#pragma unroll
+        for (int m = 0; m < M; ++m) nd.fi[m] = nd.ri[D + m];
 #pragma unroll
         for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
     } else {
         const int64_t base = sell_base(a.g, i);
-        const double invL = a.g.invL;
-        double gi[NG], Fi[D][M];
-        load_rec<NG, L::GS>(a.G, i, gi);
-        P::flux(a.pde, fi, Fi);
-        int oi = a.orders[i], oi_t = oi;   // interface order source, true order
-        if (FV == 1 && a.packed) { oi_t = oi >> 4; oi &= 15; }
-        double X[FV == 1 ? D : 1][BD], sacc[M];
-        if constexpr (FV == 1) {
-            const bool okd = a.okd[i];
-#pragma unroll
-            for (int r = 0; r < D; ++r)
-#pragma unroll
-                for (int c = 0; c < BD; ++c) X[r][c] = okd ? a.invd[(r * BD + c) * N + i] : 0.0;
-#pragma unroll
-            for (int m = 0; m < M; ++m) sacc[m] = 0.0;
-        }
-        double b[BD][D * M];
+        load_flux_node<D, M, KIND, FV>(a, Y, i, nd);
+        double b[BD][D * M], sacc[M];
 #pragma unroll
         for (int c = 0; c < BD; ++c)
 #pragma unroll
             for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) sacc[m] = 0.0;
         int32_t j1 = 0;
         double rn[NR], gn[NG];
         int on = 0;
@@ -581,136 +740,29 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
                 src.grad(j, gj);
                 oj = src.order(j);
             }
-
-            double dd[D];
-            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
-            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
-            int oj_t = oj;
-            if (FV == 1 && a.packed) { oj_t = oj >> 4; oj &= 15; }
-            int o_i = oi, o_j = oj;
-            if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
-            double pii[D], pjj[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                pii[d] = (o_i == 2) ? 0.5 * dd[d] : 0.0;
-                pjj[d] = (o_j == 2) ? -0.5 * dd[d] : 0.0;
-            }
-            double fij[M], fji[M];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                double si = gi[m] * pii[0];
-#pragma unroll
-                for (int q = 1; q < D; ++q) si = si + gi[q * M + m] * pii[q];
-                double sj = gj[m] * pjj[0];
-#pragma unroll
-                for (int q = 1; q < D; ++q) sj = sj + gj[q * M + m] * pjj[q];
-                const double uj = rj[D + m];
-                fij[m] = fi[m] + si;
-                fji[m] = uj + sj;
-            }
-            double Fij[D][M], Fji[D][M], lij[D], lji[D];
-            P::flux_eig(a.pde, fij, Fij, lij);
-            P::flux_eig(a.pde, fji, Fji, lji);
-            double pt[BD];
-            pt[0] = 1.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
-            double cw[D], ctr[M];
-            bool alignd = false;
-            if constexpr (FV == 1) {
-                alignd = a.aligned && (oi_t < 2 || oj_t < 2);
-                if (a.streamed || alignd) {
-                    double v[BD];
-#pragma unroll
-                    for (int c = 0; c < BD; ++c) v[c] = w * pt[c];
-#pragma unroll
-                    for (int r = 0; r < D; ++r) {
-                        double acc = X[r][0] * v[0];
-#pragma unroll
-                        for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
-                        cw[r] = acc * invL;
-                    }
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                // sort_flux orders the two sides by sign(d). Three exact shortcuts: F_L + F_R is the same sum either
-                // way (IEEE addition commutes); max(λ_L, λ_R) is symmetric for the non-negative speeds |u| + c (a NaN
-                // stays a NaN); f_R - f_L = ±(f_ji - f_ij) up to the sign of a zero difference. The sign of a zero
-                // dissipation only reaches dF = F* - F_i when dF is a zero, and the reference's + nc_jump (+0.0)
-                // turns such a zero into +0; the fit accumulators start at +0 and can never hold -0 (x + (-0) = x,
-                // exact cancellation gives +0), so a zero dF of either sign leaves them bit for bit unchanged, and
-                // the + 0.0 is dropped as well.
-                const double sd = jmax(lij[d], lji[d]);
-                const bool right = alignd ? cw[d] > 0.0 : dd[d] > 0.0;
-#pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    const double x = fji[m] - fij[m];
-                    const double diss = sd * (right ? x : -x);
-                    const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
-                    const double dF = Fnum - Fi[d][m];
-                    if (FV == 1 && a.streamed) {
-                        ctr[m] = d == 0 ? 0.0 + cw[0] * dF : ctr[m] + cw[d] * dF;
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF);
-                    }
-                }
-            }
-            if (FV == 1 && a.streamed)
+            double dF[D][M], w, pt[BD], ctr[M];
+            slot_flux<D, M, KIND, FV>(a, nd, rj, gj, oj, dF, w, pt, ctr);
+            if (FV == 1 && a.streamed) {
 #pragma unroll
                 for (int m = 0; m < M; ++m) sacc[m] = sacc[m] + ctr[m];
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+#pragma unroll
+                    for (int m = 0; m < M; ++m)
+#pragma unroll
+                        for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF[d][m]);
+            }
         }
         if (FV == 1 && a.streamed) {
             const bool okd = a.okd[i];
 #pragma unroll
             for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? sacc[m] : 0.0)) + 0.0;
-        } else if (a.okd[i]) {
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                double acc = 0.0;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
-#pragma unroll
-                    for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
-                    acc = acc + r;
-                }
-                Kout[m] = -(2.0 * (acc * invL)) + 0.0;
-            }
         } else {
-#pragma unroll
-            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+            fit_divergence<D, M>(a, i, b, Kout);
         }
     }
-#pragma unroll
-    for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
-    if (pred.on) fused_predict<D, M>(pred, i, Kout);
-    if (!CHECK) return false;
-    double W[M];
-#pragma unroll
-    for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
-    const bool adm = P::admissible(a.pde, W);
-    a.inadm[i] = !adm;
-    bool flag = !adm;
-    if (!flag) {
-        if (a.criterion == HCC_CRIT_ONLY) {
-            flag = true;
-        } else if (a.criterion == HCC_CRIT_U1) {
-            // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
-            // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
-            double e[2 * M], mx[M], mn[M];
-            load_rec<2 * M, 2 * M>(a.ext, i, e);
-#pragma unroll
-            for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
-            const double dl = a.delta, d3 = dl * dl * dl;
-#pragma unroll
-            for (int m = 0; m < M; ++m)
-                if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
-        }
-    }
-    if (flag && a.orders[i] > 1) { a.trig[i] = 1; return true; }
-    return false;
+    return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, nd.fi, Kout, h);
 }
 
 // The lists of the fused fixed point alternate between two copies, so that no kernel resets a count another
@@ -794,44 +846,281 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
     }
 }
 
+// The recompute kernels of the fixed point run one node per group of kSplit lanes of a 64-lane workgroup: each lane
+// takes every kSplit-th slot of the node (loads and per-slot arithmetic in parallel) and writes what the accumulation
+// needs to LDS; then one lane per accumulator adds the slot terms in slot order. Every sum keeps the reference's order,
+// so the results are those of flux_node / gather_node bit for bit; the chain of one node shrinks from cnt slots to
+// cnt / kSplit slots of work plus cnt additions.
+constexpr int kSplit = 16;
+constexpr int kSplitGroups = 64 / kSplit;
+
 // The flux pass (with the MOOD check) of the fixed-point iteration with parity p over its work list (dlist, length
-// read on the device): the neighbourhoods of the changed particles, so a particle can appear several times; the
-// thread that clears its dirty mark recomputes it, the others leave. Changes go to changed list 1 - p.
+// read on the device): the dirty particles (the changed ones and their neighbours), each listed once. Changes go to
+// changed list 1 - p.
 template <int D, int M, int KIND, int FV>
-__global__ __launch_bounds__(kBlock) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
-                                                       double* __restrict__ K, int p) {
+__global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                  double* __restrict__ K, int p) {
+    using L = Layout<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, BD = D + 1, NP = D * M, G = kSplit, GPW = kSplitGroups;
+    constexpr int SR = NP + 1 + D;   // per slot: dF (axis major), w, d invL; streamed: the contraction (M values)
+    static_assert(NP <= G && M <= SR, "one accumulator owner per lane");
+    __shared__ double rows[GPW][G][SR];
+    __shared__ double part[GPW][NP];
     const double h = a.hratio * *a.dtp;
     const int64_t n = (int64_t)*work_count(a, p);
-    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
-        const int64_t t = t0 + threadIdx.x;
-        bool ch = false;
+    const int64_t N = a.g.N;
+    const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+    const bool streamed = FV == 1 && a.streamed;
+    for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
+        const int64_t t = t0 + grp;
         int64_t i = 0;
-        int delta = 0;
+        int take = 0;
         if (t < n) {
             i = a.dlist[t];
-            // a particle listed several times (it neighbours several changed particles) is taken by one thread
-            if (atomicExch(&a.dmark[i], 0u) == 1u && !a.g.isb[i]) {
-                const int before = a.inadm[i];
-                ch = flux_node<D, M, KIND, true, FV>(a, pred, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, K, i, h);
-                delta = (int)a.inadm[i] - before;
+            if (gl == 0) {
+                a.dmark[i] = 0u;   // each dirty node is listed once (k_mark_gather); its mark is cleared for the next
+                take = !a.g.isb[i];
             }
         }
+        take = __shfl(take, grp * G);
+        const int cnt = take ? a.g.cnt[i] : 0;
+        int mc = cnt;   // slot count of the workgroup's largest node: the chunk loop is uniform
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+        FluxNode<D, M, KIND, FV> nd;
+        if (take) load_flux_node<D, M, KIND, FV>(a, Y, i, nd);
+        const int64_t base = take ? sell_base(a.g, i) : 0;
+        double b[BD], sacc = 0.0;   // owner lane gl: accumulators of pair gl = d M + m (streamed: component gl)
+#pragma unroll
+        for (int c = 0; c < BD; ++c) b[c] = 0.0;
+        for (int k0 = 0; k0 < mc; k0 += G) {
+            const int k = k0 + gl;
+            if (take && k < cnt) {
+                const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+                double rj[NR], gj[NG], dF[D][M], w, pt[BD], ctr[M];
+                load_rec<NR, L::PU>(Y, j, rj);
+                load_rec<NG, L::GS>(a.G, j, gj);
+                slot_flux<D, M, KIND, FV>(a, nd, rj, gj, a.orders[j], dF, w, pt, ctr);
+                double* row = rows[grp][gl];
+                if (streamed) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) row[m] = ctr[m];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int m = 0; m < M; ++m) row[d * M + m] = dF[d][m];
+                    row[NP] = w;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) row[NP + 1 + d] = pt[1 + d];
+                }
+            }
+            __syncthreads();
+            if (take) {
+                const int ns = min(G, cnt - k0);
+                if (streamed) {
+                    if (gl < M)
+                        for (int q = 0; q < ns; ++q) sacc = sacc + rows[grp][q][gl];
+                } else if (gl < NP) {
+                    for (int q = 0; q < ns; ++q) {
+                        const double* row = rows[grp][q];
+                        const double dFv = row[gl], w = row[NP];
+                        double pt[BD];
+                        pt[0] = 1.0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) pt[1 + d] = row[NP + 1 + d];
+#pragma unroll
+                        for (int c = 0; c < BD; ++c) b[c] = b[c] + w * (pt[c] * dFv);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // the derivative row of each pair (apply_weights_divergence), then K_E and the check by the group's first lane
+        if (take && streamed && gl < M) part[grp][gl] = sacc;
+        if (take && !streamed && gl < NP && a.okd[i]) {
+            const int d = gl / M;
+            double r = a.invd[(d * BD + 0) * N + i] * b[0];
+#pragma unroll
+            for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c];
+            part[grp][gl] = r;
+        }
+        __syncthreads();
+        bool ch = false;
+        int delta = 0;
+        if (take && gl == 0) {
+            double Kout[M];
+            const bool okd = a.okd[i];
+            if (cnt == 0) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
+            } else if (streamed) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? part[grp][m] : 0.0)) + 0.0;
+            } else if (okd) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) acc = acc + part[grp][d * M + m];
+                    Kout[m] = -(2.0 * (acc * a.g.invL)) + 0.0;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+            }
+            const int before = a.inadm[i];
+            ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, h);
+            delta = (int)a.inadm[i] - before;
+        }
+        __syncthreads();
         append_changed(a, ch, i, 1 - p);
         if (pred.count) add_inadmissible(a, delta);
     }
 }
 
+// gather_node (EXT off) of particle i by its group of kSplit lanes (all lanes of the workgroup call it; act: the group
+// has a particle to gather). The fit's accumulators are summed in slot order by one owner lane each; the extrema and
+// the limiter's phi are minima / maxima (IEEE minimum / maximum: exact, so any order), reduced over the group.
+template <int D, int M, int KIND>
+__device__ __forceinline__ void gather_split(const StepArgs& a, const double* __restrict__ Y, int64_t i, bool act,
+                                             int cnt, int order, int gl, double (*rows)[1 + D + M], double* bsh) {
+    using L = Layout<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, NP = D * M, G = kSplit;
+    static_assert(NP <= G, "one accumulator owner per lane");
+    const int64_t N = a.g.N;
+    const double invL = a.g.invL;
+    if (act && order > 1 && cnt < D) {   // starved neighbourhood (basis_count(D, 1) = D)
+        order = 1;
+        if (gl == 0) a.orders[i] = 1;
+    }
+    const bool fit = act && order == 2 && a.okg[i];
+    double ri[NR], umax[M], umin[M], g[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) g[q] = 0.0;
+    if (fit) load_rec<NR, L::PU>(Y, i, ri);
+#pragma unroll
+    for (int m = 0; m < M; ++m) { umax[m] = fit ? ri[D + m] : 0.0; umin[m] = umax[m]; }
+    int mc = fit ? cnt : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+    const int64_t base = fit ? sell_base(a.g, i) : 0;
+    double b = 0.0;   // owner lane gl: b[q][m] of pair gl = q M + m
+    for (int k0 = 0; k0 < mc; k0 += G) {
+        const int k = k0 + gl;
+        if (fit && k < cnt) {
+            double rj[NR], dd[D];
+            load_rec<NR, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], rj);
+            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+            double* row = rows[gl];
+            row[0] = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) row[1 + d] = dd[d] * invL;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double fj = rj[D + m];
+                row[1 + D + m] = fj - ri[D + m];
+                umax[m] = jmax(umax[m], fj);
+                umin[m] = jmin(umin[m], fj);
+            }
+        }
+        __syncthreads();
+        if (fit && gl < NP) {
+            const int q = gl / M, m = gl % M;
+            const int ns = min(G, cnt - k0);
+            for (int s = 0; s < ns; ++s) b = b + rows[s][0] * (rows[s][1 + q] * rows[s][1 + D + m]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            umax[m] = jmax(umax[m], __shfl_xor(umax[m], o));
+            umin[m] = jmin(umin[m], __shfl_xor(umin[m], o));
+        }
+    if (fit && gl < NP) bsh[gl] = b;
+    __syncthreads();
+    if (fit) {
+        double inv[D][D];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) inv[r][c] = a.invg[(r * D + c) * N + i];
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double c = inv[q][0] * bsh[0 * M + m];
+#pragma unroll
+                for (int r = 1; r < D; ++r) c = c + inv[q][r] * bsh[r * M + m];
+                g[q * M + m] = c * invL;
+            }
+    }
+    if (fit && a.limiter != HCC_LIM_NONE) {
+        double phi[M], dup[M], ddn[M], tup[M], tdn[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            phi[m] = 1.0;
+            const double ui = ri[D + m];
+            dup[m] = umax[m] - ui;
+            ddn[m] = umin[m] - ui;
+            tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
+            tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
+        }
+        for (int k = gl; k < cnt; k += G) {
+            double pj[D], dd[D], pint[D];
+            load_rec<D, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], pj);
+            rec_geom<D>(a.g, ri, pj, dd);
+#pragma unroll
+            for (int d = 0; d < D; ++d) pint[d] = 0.5 * dd[d];
+            double rec[M];
+            bool lim[M], any = false;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < D; ++q) v = v + g[q * M + m] * pint[q];
+                rec[m] = v;
+                lim[m] = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
+                any = any || lim[m];
+            }
+            if (any) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const double r = (rec[m] > 0.0 ? dup[m] : ddn[m]) / rec[m];
+                    const double pk = a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r);
+                    phi[m] = lim[m] ? jmin(phi[m], pk) : phi[m];
+                }
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int m = 0; m < M; ++m) phi[m] = jmin(phi[m], __shfl_xor(phi[m], o));
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) g[q * M + m] = g[q * M + m] * phi[m];
+    }
+    if (act && gl == 0) store_rec<NG, L::GS>(a.G, i, g);
+    __syncthreads();
+}
+
 // _mood_drop! (MOOD.jl:299-309), one hop of expand_mask! (MOOD.jl:323-351) and the gather of the changed list
-// (length counters[0], read on the device): the particle drops one order, it and its neighbours are marked dirty
-// (idempotent stores) and appended to the recompute's work list (one atomic per wavefront); then its gradient is
-// recomputed at the new order (the gather reads only the particle's own order, which this thread just wrote, and
-// stage states).
+// (length counters[0], read on the device), one particle per group of kSplit lanes: the particle drops one order, it
+// and its neighbours are marked dirty and the newly marked ones appended to the recompute's work list; then its
+// gradient is recomputed at the new order (the gather reads only the particle's own order, which
+// the group's first lane just wrote and hands to the group, and stage states).
 // The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
 // Block 0 also resets the dirty-list length, counts the fixed-point iterations that changed an order and publishes
 // the count to the host (pub, pinned host memory) with a system-scope store.
 template <int D, int M, int KIND>
-__global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
-                                                        int p) {
+__global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
+                                                    int p) {
+    constexpr int G = kSplit, GPW = kSplitGroups;
+    __shared__ double rows[GPW][G][1 + D + M];
+    __shared__ double bsh[GPW][D * M];
     const int64_t n = (int64_t)*changed_count(a, p);
     const int32_t* cl = changed_list(a, p);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -840,37 +1129,44 @@ __global__ __launch_bounds__(kBlock) void k_mark_gather(StepArgs a, const double
         if (n > 0) a.counters[4] += 1;
         __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
-        const int64_t t = t0 + threadIdx.x;
+    const int lane = threadIdx.x, grp = lane / G, gl = lane % G;
+    for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
+        const int64_t t = t0 + grp;
         int32_t i = 0;
-        int cnt = -1;   // -1: no particle on this lane; else its neighbour count (it lists cnt + 1 entries)
+        int cnt = -1, order = 0;   // cnt -1: no particle for this group
         if (t < n) {
             i = cl[t];
-            a.orders[i] = (int8_t)(a.orders[i] - 1);
             cnt = a.g.cnt[i];
-        }
-        // the wavefront's entries are placed with one atomic: an inclusive scan of cnt + 1 over the lanes
-        const unsigned lane = threadIdx.x & 63;
-        int incl = cnt + 1;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o);
-            if ((int)lane >= o) incl += v;
-        }
-        const int total = __shfl(incl, 63);
-        unsigned long long base_w = 0;
-        if (lane == 0 && total > 0) base_w = atomicAdd(work_count(a, p), (unsigned long long)total);
-        base_w = __shfl(base_w, 0);
-        if (cnt >= 0) {
-            int32_t* out = a.dlist + base_w + (incl - (cnt + 1));
-            const int64_t base = sell_base(a.g, i);
-            for (int k = -1; k < cnt; ++k) {
-                const int32_t j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
-                a.dmark[j] = 1u;
-                out[k + 1] = j;
+            if (gl == 0) {
+                order = a.orders[i] - 1;
+                a.orders[i] = (int8_t)order;
             }
-            if (!a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
         }
+        order = __shfl(order, grp * G);
+        // the particle and its neighbours are marked dirty; the lane whose atomicExch sets a node's mark appends the
+        // node to the work list (once per iteration), one atomicAdd per wavefront step
+        int mc = cnt;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+        const int64_t base = cnt >= 0 ? sell_base(a.g, i) : 0;
+        for (int k0 = -1; k0 < mc; k0 += G) {
+            const int k = k0 + gl;
+            int32_t j = 0;
+            bool first = false;
+            if (k < cnt) {
+                j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
+                first = atomicExch(&a.dmark[j], 1u) == 0u;
+            }
+            const unsigned long long ball = __ballot(first);
+            if (ball) {
+                unsigned long long wb = 0;
+                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)__popcll(ball));
+                wb = __shfl(wb, 0);
+                const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+                if (first) a.dlist[wb + __popcll(below)] = j;
+            }
+        }
+        gather_split<D, M, KIND>(a, Y, i, cnt >= 0 && !a.g.isb[i], cnt, order, gl, rows[grp], bsh[grp]);
     }
 }
 
@@ -1371,8 +1667,12 @@ struct Variant {
             k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
+    // the fixed point's kernels: kSplitGroups nodes per 64-lane workgroup, grid-strided over the device-side count
+    static unsigned split_grid(const hcc_stepper* s) {
+        return (unsigned)std::min<int64_t>((s->N + kSplitGroups - 1) / kSplitGroups, 4096);
+    }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
-        k_mark_gather<D, M, KIND><<<(unsigned)std::min<int64_t>(nblocks(s->N), 512), kBlock, 0, st(s)>>>(a, Y, pub, p);
+        k_mark_gather<D, M, KIND><<<split_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
     }
     template <int FV>
     static void flux_fv(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
@@ -1390,9 +1690,8 @@ struct Variant {
         else flux_fv<0>(s, a, p, Y, K, full, check);
     }
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
-        const unsigned g = (unsigned)std::min<int64_t>(nblocks(s->N), 1024);
-        if (s->fv) k_flux_list<D, M, KIND, 1><<<g, kBlock, 0, st(s)>>>(a, pr, Y, K, p);
-        else k_flux_list<D, M, KIND, 0><<<g, kBlock, 0, st(s)>>>(a, pr, Y, K, p);
+        if (s->fv) k_flux_list<D, M, KIND, 1><<<split_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+        else k_flux_list<D, M, KIND, 0><<<split_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
                      int full) {
@@ -1533,7 +1832,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->inadm, N));
     HCC_TRY_HIP(ctx, dalloc(&s->dmark, N));
     HCC_TRY_HIP(ctx, dalloc(&s->clist, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N + g->S));   // at most every particle's neighbourhood once
+    HCC_TRY_HIP(ctx, dalloc(&s->dlist, N));   // each dirty particle once per fixed-point iteration
     HCC_TRY_HIP(ctx, dalloc(&s->clist2, N));
     HCC_TRY_HIP(ctx, dalloc(&s->lam, (int64_t)D * N));
     HCC_TRY_HIP(ctx, dalloc(&s->bmin, nblocks(N)));
