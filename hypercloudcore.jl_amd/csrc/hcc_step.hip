@@ -98,14 +98,14 @@ __device__ __forceinline__ void lds_read(const double* p, double (&r)[K]) {
     }
 }
 
-template <int D, int M, bool GRADS>
+template <int D, int M, bool GRADS, bool LAM = false>
 struct LdsSrc {
     using L = Layout<D, M>;
     // the flux loop loads slot k+1 while slot k computes; the gather's loops, with more waves per SIMD, load the
     // next index ahead and the records at the top of each slot
     static constexpr bool kPrefetch = GRADS;
-    static constexpr int GO = lds_grad_offset(L::NR);
-    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
+    static constexpr int GO = lds_grad_offset(L::NR);   // gradient record (GRADS) or wave speeds (LAM)
+    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, LAM ? D : 0);
     const double* sh;          // halo node l at sh + l * ST
     const int8_t* ord;         // orders of the halo nodes (GRADS)
     const uint16_t* nl;        // the block's local indices in LDS: SELL slot s at nl[s - s0]
@@ -115,6 +115,7 @@ struct LdsSrc {
     __device__ __forceinline__ void head(int32_t l, double (&r)[K]) const { lds_read<K>(sh + l * ST, r); }
     __device__ __forceinline__ void grad(int32_t l, double (&g)[L::NG]) const { lds_read<L::NG>(sh + l * ST + GO, g); }
     __device__ __forceinline__ int order(int32_t l) const { return ord[l]; }
+    __device__ __forceinline__ void lam(int32_t l, double (&v)[D]) const { lds_read<D>(sh + l * ST + GO, v); }
 };
 
 template <class Src>
@@ -143,13 +144,14 @@ __device__ __forceinline__ int64_t stage_index_table(const GeomArgs& g, uint16_t
     return s0;
 }
 
-// Loads the halo of workgroup lb into LDS (records, with GRADS also gradient records and orders) and returns the
-// source the neighbour loops read. Coalesced: the halo is sorted by node index.
-template <int D, int M, bool GRADS>
-__device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, const double* __restrict__ Y, double* sh,
-                                                          int64_t lb) {
+// Loads the halo of workgroup lb into LDS (records, with GRADS also gradient records and orders, with LAM the wave
+// speeds of the CFL estimate, _get_Lambda) and returns the source the neighbour loops read. Coalesced: the halo is
+// sorted by node index.
+template <int D, int M, bool GRADS, bool LAM = false, int KIND = 0>
+__device__ __forceinline__ LdsSrc<D, M, GRADS, LAM> stage_halo(const StepArgs& a, const double* __restrict__ Y,
+                                                               double* sh, int64_t lb) {
     using L = Layout<D, M>;
-    using S = LdsSrc<D, M, GRADS>;
+    using S = LdsSrc<D, M, GRADS, LAM>;
     const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
     int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
     uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST, GRADS);
@@ -167,6 +169,14 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS> stage_halo(const StepArgs& a, con
 #pragma unroll
             for (int q = 0; q < L::NG; ++q) dst[S::GO + q] = g[q];
             ord[t] = a.orders[j];
+        }
+        if constexpr (LAM) {
+            double f[M], l[D];
+#pragma unroll
+            for (int m = 0; m < M; ++m) f[m] = r[D + m];
+            Pde<D, M, KIND>::maxeig(a.pde, f, l);
+#pragma unroll
+            for (int d = 0; d < D; ++d) dst[S::GO + d] = l[d];
         }
     }
     __syncthreads();
@@ -244,8 +254,11 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
 // EXT: also store the local extrema of Y over {i} and its neighbours (the limiter's bounds, which are MOODu1's DMP
 // bounds too, _dmp_fails MOOD.jl:175-184) for the checks of the flux pass of the same stage. The neighbour loop then
 // runs for every interior particle; one without a gradient fit discards the fit sums.
-template <int D, int M, int KIND, bool EXT, class Src>
-__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i) {
+// CFL: also the node's CFL step of get_time_step (the sums of k_cfl over the same slots, with the wave speeds staged
+// in LDS), into *dtc (infinity where k_cfl skips the node).
+template <int D, int M, int KIND, bool EXT, bool CFL = false, class Src>
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i,
+                                            double* dtc = nullptr) {
     using L = Layout<D, M>;
     constexpr int NR = L::NR;
     const int64_t N = a.g.N;
@@ -259,8 +272,20 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
     const int64_t base = sell_base(a.g, i);
     const double invL = a.g.invL;
     double ri[NR], umax[M], umin[M], b[D][M];
-    if (fit || (EXT && a.ext_on)) {
+    constexpr int BD = D + 1;
+    double X[CFL ? D : 1][BD], li[D], sum_c = 0.0;
+    if (fit || (EXT && a.ext_on) || (CFL && cnt > 0)) {
         load_rec<NR, L::PU>(Y, i, ri);
+        if constexpr (CFL) {
+            double f[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) f[m] = ri[D + m];
+            Pde<D, M, KIND>::maxeig(a.pde, f, li);
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int c = 0; c < BD; ++c) X[r][c] = a.invd[(r * BD + c) * N + i];
+        }
 #pragma unroll
         for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; }
 #pragma unroll
@@ -270,8 +295,10 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         int32_t j1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
         double rn[NR];
         if constexpr (Src::kPrefetch) src.template head<NR>(src_slot(src, base, 0, cnt), rn);
+        static_assert(!(CFL && Src::kPrefetch), "the fused CFL estimate reads the staged wave speeds");
         for (int k = 0; k < cnt; ++k) {
             double rj[NR];
+            int32_t jcur = 0;
             if constexpr (Src::kPrefetch) {
 #pragma unroll
                 for (int q = 0; q < NR; ++q) rj[q] = rn[q];
@@ -279,9 +306,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 j1 = src_slot(src, base, k + 2, cnt);
                 src.template head<NR>(jn, rn);
             } else {   // the index of slot k+1 (from HBM/L2) loads while slot k computes; records come from LDS
-                const int32_t j = j1;
+                jcur = j1;
                 j1 = src_slot(src, base, k + 1, cnt);
-                src.template head<NR>(j, rj);
+                src.template head<NR>(jcur, rj);
             }
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
@@ -289,6 +316,23 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
             double p[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) p[d] = dd[d] * invL;
+            if constexpr (CFL) {   // _cfl_weights_body!: sum_d |c_k,d| max(lambda_d(U_i), lambda_d(U_j))
+                double lj[D], v[BD];
+                src.lam(jcur, lj);
+                v[0] = w * 1.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) v[1 + d] = w * p[d];
+                double dv = 0.0;
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    double acc = X[r][0] * v[0];
+#pragma unroll
+                    for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
+                    const double term = fabs(acc * invL) * jmax(li[r], lj[r]);
+                    dv = (r == 0) ? term : dv + term;
+                }
+                sum_c = sum_c + dv;
+            }
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 const double fj = rj[D + m];
@@ -304,6 +348,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
 #pragma unroll
             for (int m = 0; m < M; ++m) { e[m] = umax[m]; e[M + m] = umin[m]; }
             store_rec<2 * M, 2 * M>(a.ext, i, e);
+        }
+        if constexpr (CFL) {
+            if (sum_c > kFloatMin) *dtc = (1.0 / (double)D) / sum_c;
         }
     }
     if (fit) {
@@ -397,6 +444,21 @@ __device__ __forceinline__ void step_reset(const StepArgs& a, int64_t i) {
     if (i < a.g.N) { a.orders[i] = 2; a.trig[i] = 0; }
 }
 
+// The CFL steps of a workgroup's nodes: their minimum into bmin[lb] (one store per workgroup, no contended atomics on
+// a single address); every thread of the workgroup calls it.
+__device__ __forceinline__ void block_min(const StepArgs& a, double dt, int64_t lb) {
+    __shared__ double wmin[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) m = fmin(m, wmin[w]);
+        if (lb < (a.g.N + kBlock - 1) / kBlock) a.bmin[lb] = m;
+    }
+}
+
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
                                                     int64_t nblk) {
@@ -410,7 +472,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
 }
 
 // The full gather pass with the workgroup's halo staged in LDS (2-D).
-template <int D, int M, int KIND>
+// CFL: the CFL estimate of the state being gathered (stage 1 of a step whose state is U) fused in: per-workgroup
+// minima into bmin, reduced by k_cfl_final.
+template <int D, int M, int KIND, bool CFL>
 __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
                                                            int64_t nblk) {
     extern __shared__ double sh[];
@@ -419,9 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (lb >= nblk) return;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
-    const auto src = stage_halo<D, M, false>(a, Y, sh, lb);
-    if (i >= a.g.N || a.g.isb[i]) return;
-    gather_node<D, M, KIND, true>(a, Y, src, i);
+    const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
+    double dtc = INFINITY;
+    if (i < a.g.N && !a.g.isb[i]) gather_node<D, M, KIND, true, CFL>(a, Y, src, i, &dtc);
+    if (CFL) block_min(a, dtc, lb);
 }
 
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
@@ -610,13 +675,15 @@ __device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M
         // NaN); f_R - f_L = ±(f_ji - f_ij) up to the sign of a zero difference. The sign of a zero dissipation only
         // reaches dF = F* - F_i when dF is a zero, and the reference's + nc_jump (+0.0) turns such a zero into +0;
         // the fit accumulators start at +0 and can never hold -0 (x + (-0) = x, exact cancellation gives +0), so a
-        // zero dF of either sign leaves them bit for bit unchanged, and the + 0.0 is dropped as well.
+        // zero dF of either sign leaves them bit for bit unchanged, and the + 0.0 is dropped as well. The side's
+        // sign goes on the speed: (-s) x and s (-x) are the same IEEE product.
         const double sd = jmax(lij[d], lji[d]);
         const bool right = alignd ? cw[d] > 0.0 : dd[d] > 0.0;
+        const double sds = right ? sd : -sd;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             const double x = fji[m] - fij[m];
-            const double diss = sd * (right ? x : -x);
+            const double diss = sds * x;
             const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - diss);
             dF[d][m] = Fnum - nd.Fi[d][m];
             if (FV == 1 && a.streamed) ctr[m] = d == 0 ? 0.0 + cw[0] * dF[d][m] : ctr[m] + cw[d] * dF[d][m];
@@ -710,36 +777,7 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
 #pragma unroll
         for (int m = 0; m < M; ++m) sacc[m] = 0.0;
-        int32_t j1 = 0;
-        double rn[NR], gn[NG];
-        int on = 0;
-        if constexpr (Src::kPrefetch) {
-            j1 = src_slot(src, base, 1, cnt);
-            const int32_t j0 = src_slot(src, base, 0, cnt);
-            src.template head<NR>(j0, rn);
-            src.grad(j0, gn);
-            on = src.order(j0);
-        }
-        for (int k = 0; k < cnt; ++k) {
-            double rj[NR], gj[NG];
-            int oj;
-            if constexpr (Src::kPrefetch) {
-#pragma unroll
-                for (int q = 0; q < NR; ++q) rj[q] = rn[q];
-#pragma unroll
-                for (int q = 0; q < NG; ++q) gj[q] = gn[q];
-                oj = on;
-                const int32_t jn = j1;
-                j1 = src_slot(src, base, k + 2, cnt);
-                src.template head<NR>(jn, rn);
-                src.grad(jn, gn);
-                on = src.order(jn);
-            } else {
-                const int32_t j = src.idx(base + (int64_t)k * kChunk);
-                src.template head<NR>(j, rj);
-                src.grad(j, gj);
-                oj = src.order(j);
-            }
+        auto slot = [&](const double (&rj)[NR], const double (&gj)[NG], int oj) {
             double dF[D][M], w, pt[BD], ctr[M];
             slot_flux<D, M, KIND, FV>(a, nd, rj, gj, oj, dF, w, pt, ctr);
             if (FV == 1 && a.streamed) {
@@ -752,6 +790,37 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
                     for (int m = 0; m < M; ++m)
 #pragma unroll
                         for (int c = 0; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (pt[c] * dF[d][m]);
+            }
+        };
+        if constexpr (Src::kPrefetch) {
+            // slot k+1's data loads while slot k computes (index two slots ahead); unrolled by two so that the two
+            // buffers alternate instead of being copied
+            double ra[NR], ga[NG], rb[NR], gb[NG];
+            int oa, ob;
+            int32_t jA = src_slot(src, base, 0, cnt), jB = src_slot(src, base, 1, cnt);
+            src.template head<NR>(jA, ra);
+            src.grad(jA, ga);
+            oa = src.order(jA);
+            for (int k = 0; k < cnt; k += 2) {
+                jA = src_slot(src, base, k + 2, cnt);
+                src.template head<NR>(jB, rb);
+                src.grad(jB, gb);
+                ob = src.order(jB);
+                slot(ra, ga, oa);
+                if (k + 1 >= cnt) break;
+                jB = src_slot(src, base, k + 3, cnt);
+                src.template head<NR>(jA, ra);
+                src.grad(jA, ga);
+                oa = src.order(jA);
+                slot(rb, gb, ob);
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                const int32_t j = src.idx(base + (int64_t)k * kChunk);
+                double rj[NR], gj[NG];
+                src.template head<NR>(j, rj);
+                src.grad(j, gj);
+                slot(rj, gj, src.order(j));
             }
         }
         if (FV == 1 && a.streamed) {
@@ -1472,18 +1541,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __rest
         }
         if (sum_c > kFloatMin) dt = (1.0 / (double)D) / sum_c;
     }
-    // block minimum into bmin[lb]: one store per workgroup, no contended atomics on a single address
-    __shared__ double wmin[kBlock / 64];
-    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
-    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int64_t lb = logical_block(nblk);
-        double m = wmin[0];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; ++w) m = fmin(m, wmin[w]);
-        if (lb < nblk) a.bmin[lb] = m;
-    }
+    block_min(a, dt, logical_block(nblk));
 }
 
 // dt = min over the workgroup minima (ParticleGrids.jl:386-415 reduces with minimum); +inf if no node contributes
@@ -1592,7 +1650,10 @@ struct hcc_stepper {
     double* dtp = nullptr;     // the step's dt on the device
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
-    size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0;
+    size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
+    // hcc_step_cfl on a state whose stage 1 is U: the CFL estimate runs inside the stage-1 gather (cfl, cap)
+    bool cfl_fused = false;
+    double cfl_num = 0, cfl_cap = 0;
     // statistics of queued steps, read when a getter asks or the ring is full (a new step does not wait)
     struct StepStats {
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1664,12 +1725,18 @@ struct Variant {
     static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full, int reset) {
         const int64_t nb = nblocks(s->N);
         if (full && s->lds_gather)
-            k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
+            k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
     // the fixed point's kernels: kSplitGroups nodes per 64-lane workgroup, grid-strided over the device-side count
     static unsigned split_grid(const hcc_stepper* s) {
         return (unsigned)std::min<int64_t>((s->N + kSplitGroups - 1) / kSplitGroups, 4096);
+    }
+    // the full gather of stage 1 with the CFL estimate of its state fused in, then dt = min(cfl * dt_cfl, cap)
+    static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double cfl, double cap) {
+        const int64_t nb = nblocks(s->N);
+        k_gather_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
+        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters, s->dtp, cfl, cap);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
         k_mark_gather<D, M, KIND><<<split_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
@@ -1731,6 +1798,7 @@ struct Variant {
 
 struct VariantOps {
     void (*gather)(const hcc_stepper*, const StepArgs&, const double*, int, int);
+    void (*gather_cfl)(const hcc_stepper*, const StepArgs&, const double*, int, double, double);
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
     void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int);
@@ -1745,7 +1813,7 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::range, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::range, &V::cfl, &V::predict,
                       &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
@@ -1861,6 +1929,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         s->lds_gather = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0), false) + idx_bytes);
         s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
         s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
+        s->lds_gcfl = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, D), false) + idx_bytes);
     }
     for (auto& r : s->ring) {
         HCC_TRY_HIP(ctx, hipEventCreate(&r.ev0));
@@ -2129,7 +2198,8 @@ struct StepRunner {
             }
             const double* Y = stage_state(i);
             (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
-            ops.gather(s, a, Y, 1, i == 1);
+            if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
+            else ops.gather(s, a, Y, 1, i == 1);
             (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
             ops.flux(s, flux_args(), p, Y, s->K[i - 1], 1, fused());
@@ -2223,10 +2293,19 @@ extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
         return set_err(ctx, HCC_E_ARG, "hcc_step_cfl: cfl must be positive and finite, dt_cap positive");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (s->N == 0) return HCC_OK;
-    VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
-    ops.cfl(s, s->args(), s->dtp, cfl, dt_cap);
-    return launch_step(s, "hcc_step_cfl", true);
+    // stage 1 reads U itself once the boundary conditions hold on it (every step after the first since the state was
+    // loaded): the CFL estimate of U then runs inside the stage-1 gather, which stages the same neighbourhoods
+    s->cfl_fused = !s->fresh && s->lds_gcfl > 0;
+    if (!s->cfl_fused) {
+        VariantOps ops;
+        find_ops(s->D, s->M, s->kind, ops);
+        ops.cfl(s, s->args(), s->dtp, cfl, dt_cap);
+    }
+    s->cfl_num = cfl;
+    s->cfl_cap = dt_cap;
+    const int rc = launch_step(s, "hcc_step_cfl", true);
+    s->cfl_fused = false;
+    return rc;
 }
 
 extern "C" int hcc_last_dt(hcc_stepper* s, double* dt) {
