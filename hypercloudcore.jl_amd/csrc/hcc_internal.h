@@ -112,9 +112,10 @@ struct StepArgs {
     uint8_t* dirty;
     uint8_t* trig;               // mood_flags of the step
     uint8_t* inadm;              // last check: candidate not admissible
-    unsigned long long* counters;// [0] / [6]: lengths of changed lists 0 / 1 (clist / clist2), [1] inadmissible
-                                 // accepted, [2] dt bits, [4] fixed-point iterations of the step, [5] / [7]: work-list
-                                 // lengths of iterations with parity 0 / 1
+    unsigned long long* counters;// [0] / [6]: lengths of changed lists 0 / 1 (clist / clist2; entries -1 are no node),
+                                 // [1] inadmissible accepted, [2] dt bits, [4] fixed-point iterations of the step,
+                                 // [5] / [7]: work-list lengths of iterations with parity 0 / 1, [8] / [9]: changed
+                                 // list 0 / 1 holds a node
     int32_t* clist;              // particles whose order changed in a check (changed list 0)
     int32_t* clist2;             // changed list 1
     int32_t* dlist;              // the recompute's work list: the dirty particles, each once

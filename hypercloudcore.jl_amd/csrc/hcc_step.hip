@@ -462,7 +462,7 @@ __device__ __forceinline__ void block_min(const StepArgs& a, double dt, int64_t 
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
                                                     int64_t nblk) {
-    if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; }   // lists 0 of the flux pass's fixed point
+    if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t i = node_index(nblk);
     if (reset) step_reset(a, i);
     if (i >= a.g.N) return;
@@ -478,7 +478,7 @@ template <int D, int M, int KIND, bool CFL>
 __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
                                                            int64_t nblk) {
     extern __shared__ double sh[];
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; }   // lists 0 of the flux pass's fixed point
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
     const int64_t i = lb * kBlock + threadIdx.x;
@@ -837,15 +837,20 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
 // The lists of the fused fixed point alternate between two copies, so that no kernel resets a count another
 // workgroup of the same kernel still reads or fills. Iteration k (k = 0, 1, ...) has parity p = k % 2:
 // k_mark_gather reads changed list p and fills work list p (and resets changed and work counts 1 - p), k_flux_list
-// reads work list p and fills changed list 1 - p. The full flux pass fills changed list 0; the full gather before
-// it resets changed and work counts 0. Changed list p: count changed_count(p), particles clist / clist2; work list:
-// one buffer (dlist), count work_count(p).
+// reads work list p and fills changed list 1 - p. The full flux pass fills changed list 0 (compact, appended); the
+// full gather before it resets changed and work counts 0. k_flux_list writes work item t's entry at index t (the node,
+// or -1 if its order stays), so it needs no atomics. Changed list p: length changed_count(p), any node at all
+// changed_any(p), entries clist / clist2; work list: one buffer (dlist), length work_count(p).
 __device__ __forceinline__ unsigned long long* changed_count(const StepArgs& a, int p) {
     return p ? &a.counters[6] : &a.counters[0];
 }
 __device__ __forceinline__ int32_t* changed_list(const StepArgs& a, int p) { return p ? a.clist2 : a.clist; }
 __device__ __forceinline__ unsigned long long* work_count(const StepArgs& a, int p) {
     return p ? &a.counters[7] : &a.counters[5];
+}
+// whether changed list p holds a change (its entries can be -1: a recomputed node whose order stays)
+__device__ __forceinline__ unsigned long long* changed_any(const StepArgs& a, int p) {
+    return p ? &a.counters[9] : &a.counters[8];
 }
 
 // Appends the flagged particles of a wavefront to changed list p (one atomic per wavefront) and counts them.
@@ -855,7 +860,10 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
     const unsigned lane = threadIdx.x & 63;
     const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
     unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(changed_count(a, p), (unsigned long long)__popcll(ball));
+    if (lane == leader) {
+        base = atomicAdd(changed_count(a, p), (unsigned long long)__popcll(ball));
+        *changed_any(a, p) = 1;
+    }
     base = __shfl(base, (int)leader);
     if (ch) {
         const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
@@ -920,8 +928,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
 // needs to LDS; then one lane per accumulator adds the slot terms in slot order. Every sum keeps the reference's order,
 // so the results are those of flux_node / gather_node bit for bit; the chain of one node shrinks from cnt slots to
 // cnt / kSplit slots of work plus cnt additions.
-constexpr int kSplit = 16;
-constexpr int kSplitGroups = 64 / kSplit;
+// lanes per node: at least one per accumulator of the fit (D M)
+template <int D, int M>
+constexpr int kSplit = D * M <= 8 ? 8 : 16;
 
 // The flux pass (with the MOOD check) of the fixed-point iteration with parity p over its work list (dlist, length
 // read on the device): the dirty particles (the changed ones and their neighbours), each listed once. Changes go to
@@ -930,7 +939,7 @@ template <int D, int M, int KIND, int FV>
 __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int p) {
     using L = Layout<D, M>;
-    constexpr int NR = L::NR, NG = L::NG, BD = D + 1, NP = D * M, G = kSplit, GPW = kSplitGroups;
+    constexpr int NR = L::NR, NG = L::NG, BD = D + 1, NP = D * M, G = kSplit<D, M>, GPW = 64 / G;
     constexpr int SR = NP + 1 + D;   // per slot: dF (axis major), w, d invL; streamed: the contraction (M values)
     static_assert(NP <= G && M <= SR, "one accumulator owner per lane");
     __shared__ double rows[GPW][G][SR];
@@ -939,6 +948,7 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
     const int64_t n = (int64_t)*work_count(a, p);
     const int64_t N = a.g.N;
     const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const bool streamed = FV == 1 && a.streamed;
     for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
         const int64_t t = t0 + grp;
@@ -1043,7 +1053,11 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
             delta = (int)a.inadm[i] - before;
         }
         __syncthreads();
-        append_changed(a, ch, i, 1 - p);
+        // work item t's entry of changed list 1 - p: the node if its order is to drop, else -1 (no atomics)
+        if (t < n && gl == 0) {
+            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+            if (ch) *changed_any(a, 1 - p) = 1;
+        }
         if (pred.count) add_inadmissible(a, delta);
     }
 }
@@ -1055,7 +1069,7 @@ template <int D, int M, int KIND>
 __device__ __forceinline__ void gather_split(const StepArgs& a, const double* __restrict__ Y, int64_t i, bool act,
                                              int cnt, int order, int gl, double (*rows)[1 + D + M], double* bsh) {
     using L = Layout<D, M>;
-    constexpr int NR = L::NR, NG = L::NG, NP = D * M, G = kSplit;
+    constexpr int NR = L::NR, NG = L::NG, NP = D * M, G = kSplit<D, M>;
     static_assert(NP <= G, "one accumulator owner per lane");
     const int64_t N = a.g.N;
     const double invL = a.g.invL;
@@ -1187,14 +1201,18 @@ __device__ __forceinline__ void gather_split(const StepArgs& a, const double* __
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
                                                     int p) {
-    constexpr int G = kSplit, GPW = kSplitGroups;
+    constexpr int G = kSplit<D, M>, GPW = 64 / G;
+    constexpr int kListCap = 512;   // newly marked nodes collected before they are appended
     __shared__ double rows[GPW][G][1 + D + M];
     __shared__ double bsh[GPW][D * M];
-    const int64_t n = (int64_t)*changed_count(a, p);
+    __shared__ int32_t lbuf[kListCap];
+    // changed list p: n entries, a node or -1 (none if no entry is a node)
+    const int64_t n = *changed_any(a, p) ? (int64_t)*changed_count(a, p) : 0;
     const int32_t* cl = changed_list(a, p);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *changed_count(a, 1 - p) = 0;
         *work_count(a, 1 - p) = 0;
+        *changed_any(a, 1 - p) = 0;
         if (n > 0) a.counters[4] += 1;
         __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -1203,7 +1221,7 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
         const int64_t t = t0 + grp;
         int32_t i = 0;
         int cnt = -1, order = 0;   // cnt -1: no particle for this group
-        if (t < n) {
+        if (t < n && cl[t] >= 0) {
             i = cl[t];
             cnt = a.g.cnt[i];
             if (gl == 0) {
@@ -1212,13 +1230,26 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             }
         }
         order = __shfl(order, grp * G);
-        // the particle and its neighbours are marked dirty; the lane whose atomicExch sets a node's mark appends the
-        // node to the work list (once per iteration), one atomicAdd per wavefront step
+        // the particle and its neighbours are marked dirty; the lane whose atomicExch sets a node's mark lists the node
+        // (once per iteration): collected in LDS, then appended to the work list with one atomicAdd per pass
         int mc = cnt;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
         const int64_t base = cnt >= 0 ? sell_base(a.g, i) : 0;
+        int nl = 0;   // entries in lbuf (uniform)
+        auto flush = [&]() {
+            __syncthreads();
+            if (nl > 0) {
+                unsigned long long wb = 0;
+                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
+                wb = __shfl(wb, 0);
+                for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
+            }
+            __syncthreads();
+            nl = 0;
+        };
         for (int k0 = -1; k0 < mc; k0 += G) {
+            if (nl > kListCap - 64) flush();   // a step adds at most 64 entries
             const int k = k0 + gl;
             int32_t j = 0;
             bool first = false;
@@ -1227,14 +1258,11 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
                 first = atomicExch(&a.dmark[j], 1u) == 0u;
             }
             const unsigned long long ball = __ballot(first);
-            if (ball) {
-                unsigned long long wb = 0;
-                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)__popcll(ball));
-                wb = __shfl(wb, 0);
-                const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-                if (first) a.dlist[wb + __popcll(below)] = j;
-            }
+            const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+            if (first) lbuf[nl + __popcll(below)] = j;
+            nl += __popcll(ball);
         }
+        flush();
         gather_split<D, M, KIND>(a, Y, i, cnt >= 0 && !a.g.isb[i], cnt, order, gl, rows[grp], bsh[grp]);
     }
 }
@@ -1728,9 +1756,10 @@ struct Variant {
             k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
-    // the fixed point's kernels: kSplitGroups nodes per 64-lane workgroup, grid-strided over the device-side count
+    // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count
     static unsigned split_grid(const hcc_stepper* s) {
-        return (unsigned)std::min<int64_t>((s->N + kSplitGroups - 1) / kSplitGroups, 4096);
+        constexpr int GPW = 64 / kSplit<D, M>;
+        return (unsigned)std::min<int64_t>((s->N + GPW - 1) / GPW, 16384 / GPW);
     }
     // the full gather of stage 1 with the CFL estimate of its state fused in, then dt = min(cfl * dt_cfl, cap)
     static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double cfl, double cap) {
@@ -1911,14 +1940,14 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->rpart, 2 * M * ((N + kBlock - 1) / kBlock)));
     HCC_TRY_HIP(ctx, dalloc(&s->range, M));
     HCC_TRY_HIP(ctx, dalloc(&s->dtp, 2));
-    HCC_TRY_HIP(ctx, dalloc(&s->counters, 8));
+    HCC_TRY_HIP(ctx, dalloc(&s->counters, 10));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), st));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 8 * sizeof(unsigned long long), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 10 * sizeof(unsigned long long), st));
     if (g->max_halo > 0) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
         int lds_max = 0;
         HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
