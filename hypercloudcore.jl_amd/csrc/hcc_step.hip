@@ -104,10 +104,15 @@ struct LdsSrc {
     // the flux loop loads slot k+1 while slot k computes; the gather's loops, with more waves per SIMD, load the
     // next index ahead and the records at the top of each slot
     static constexpr bool kPrefetch = GRADS;
-    static constexpr int GO = lds_grad_offset(L::NR);   // gradient record (GRADS) or wave speeds (LAM)
-    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, LAM ? D : 0);
+    static constexpr int GO = lds_grad_offset(L::NR);   // gradient record (GRADS)
+    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
+    // LAM: the wave speeds of halo node l at lam_sh + l LS (LS: D padded to even, 16-byte entries), an array of their
+    // own after the records (a record stride of a power of two bytes makes the irregular neighbour reads conflict in
+    // the LDS banks)
+    static constexpr int LS = (D + 1) & ~1;
     const double* sh;          // halo node l at sh + l * ST
     const int8_t* ord;         // orders of the halo nodes (GRADS)
+    const double* lam_sh;      // wave speeds of the halo nodes (LAM)
     const uint16_t* nl;        // the block's local indices in LDS: SELL slot s at nl[s - s0]
     int64_t s0;
     __device__ __forceinline__ int32_t idx(int64_t s) const { return nl[s - s0]; }
@@ -115,7 +120,7 @@ struct LdsSrc {
     __device__ __forceinline__ void head(int32_t l, double (&r)[K]) const { lds_read<K>(sh + l * ST, r); }
     __device__ __forceinline__ void grad(int32_t l, double (&g)[L::NG]) const { lds_read<L::NG>(sh + l * ST + GO, g); }
     __device__ __forceinline__ int order(int32_t l) const { return ord[l]; }
-    __device__ __forceinline__ void lam(int32_t l, double (&v)[D]) const { lds_read<D>(sh + l * ST + GO, v); }
+    __device__ __forceinline__ void lam(int32_t l, double (&v)[D]) const { lds_read<D>(lam_sh + l * LS, v); }
 };
 
 template <class Src>
@@ -154,7 +159,8 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS, LAM> stage_halo(const StepArgs& a
     using S = LdsSrc<D, M, GRADS, LAM>;
     const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
     int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
-    uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST, GRADS);
+    double* lam_sh = sh + (size_t)a.g.hmax * S::ST;
+    uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST + (LAM ? S::LS : 0), GRADS);
     const int64_t s0 = stage_index_table(a.g, nl, lb);
     for (int t = threadIdx.x; t < nh; t += kStage) {
         const int32_t j = a.g.halo_idx[h0 + t];
@@ -176,11 +182,11 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS, LAM> stage_halo(const StepArgs& a
             for (int m = 0; m < M; ++m) f[m] = r[D + m];
             Pde<D, M, KIND>::maxeig(a.pde, f, l);
 #pragma unroll
-            for (int d = 0; d < D; ++d) dst[S::GO + d] = l[d];
+            for (int d = 0; d < D; ++d) lam_sh[t * S::LS + d] = l[d];
         }
     }
     __syncthreads();
-    return S{sh, ord, nl, s0};
+    return S{sh, ord, lam_sh, nl, s0};
 }
 
 // ============================================================================================================
@@ -1958,7 +1964,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         s->lds_gather = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0), false) + idx_bytes);
         s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
         s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
-        s->lds_gcfl = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, D), false) + idx_bytes);
+        s->lds_gcfl = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0) + ((D + 1) & ~1), false) + idx_bytes);
     }
     for (auto& r : s->ring) {
         HCC_TRY_HIP(ctx, hipEventCreate(&r.ev0));
