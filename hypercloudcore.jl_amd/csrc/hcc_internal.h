@@ -155,6 +155,7 @@ struct hcc_grid {
     int32_t* d_halo_idx = nullptr;
     uint16_t* d_nbr_local = nullptr;
     int64_t max_halo = 0, max_block_slots = 0;
+    double* d_vol = nullptr;           // particle volumes (ShepardVolumes), N
     std::vector<int32_t> h_tags;
     std::vector<uint8_t> h_isb;
     hcc::GeomArgs args() const;

@@ -346,6 +346,126 @@ static int run_search(hcc_grid* g) {
     return HCC_OK;
 }
 
+// ============================================================================================================
+// particle volumes: update_volumes!(pg, ShepardVolumes(sweeps)) (src/Grid/Volumes.jl:183-231), which the ParticleGrid
+// constructor runs with one sweep (src/Grid/ParticleGrids.jl:275). The slot weights are recomputed from the positions
+// with the search's operations (the same bits as the stored weights).
+// ============================================================================================================
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_shepard_sums(GeomArgs g, double* S, double* V, double w0, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= g.N) return;
+    const NodePos<D> pi = load_pos<D>(g, i);
+    const int cnt = g.cnt[i];
+    const int64_t base = sell_base(g, i);
+    double sv = w0;                                   // _shepard_sums!: w(0) + sum_k w_k
+    for (int k = 0; k < cnt; ++k) {
+        double dd[D];
+        const double d2 = slot_geom<D>(g, pi, g.nbr[base + (int64_t)k * kChunk], dd);
+        sv = sv + mls_weight(g.alpha, g.inv_range_sq, d2);
+    }
+    S[i] = sv;
+    V[i] = 1.0 / sv;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_shepard_sweep(GeomArgs g, const double* __restrict__ S,
+                                                          const double* __restrict__ V, double* __restrict__ Vt,
+                                                          double w0, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    if (i >= g.N) return;
+    const NodePos<D> pi = load_pos<D>(g, i);
+    const int cnt = g.cnt[i];
+    const int64_t base = sell_base(g, i);
+    double acc = w0 / S[i] * V[i];                    // _shepard_sweep!
+    for (int k = 0; k < cnt; ++k) {
+        const int32_t j = g.nbr[base + (int64_t)k * kChunk];
+        double dd[D];
+        const double d2 = slot_geom<D>(g, pi, j, dd);
+        acc = acc + mls_weight(g.alpha, g.inv_range_sq, d2) / S[j] * V[j];
+    }
+    Vt[i] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_clamp_volumes(double* V, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) V[i] = jmax(V[i], 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_volumes(double* V, int64_t n, double dv, const double* total) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double f = dv / *total;                     // V .*= domain_volume(pg) / sum(V)
+    if (i < n) V[i] = V[i] * f;
+}
+
+__global__ __launch_bounds__(256) void k_fold_sum(const double* v, int64_t n, double* out) {
+    constexpr int C = 2048;
+    __shared__ double buf[C];
+    double acc = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += C) {
+        for (int t = threadIdx.x; t < C; t += 256)
+            if (c0 + t < n) buf[t] = v[c0 + t];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = (int)(n - c0 < C ? n - c0 : C);
+            int t = 0;
+            if (c0 == 0) { acc = buf[0]; t = 1; }      // sum(V) starts from its first element
+            for (; t < m; ++t) acc = acc + buf[t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = acc;
+}
+
+// The volumes into g->d_vol: sums and 1 / S, `sweeps` sweeps, clamp at zero, and the renormalisation to the domain's
+// volume (prod(maxs - mins), Volumes.jl:86) with the sum of the volumes as a left fold in index order (Julia's sum of
+// an array reassociates; that one sum agrees with it to rounding only).
+static int compute_volumes(hcc_grid* g, int sweeps) {
+    hcc_ctx* ctx = g->ctx;
+    const int64_t N = g->N;
+    if (N == 0) return HCC_OK;
+    hipStream_t st = ctx->stream;
+    if (!g->d_vol) HCC_TRY_HIP(ctx, dalloc(&g->d_vol, N));
+    double *S = nullptr, *Vt = nullptr, *tot = nullptr;
+    HCC_TRY_HIP(ctx, dalloc(&S, N));
+    HCC_TRY_HIP(ctx, dalloc(&Vt, N));
+    HCC_TRY_HIP(ctx, dalloc(&tot, 1));
+    const GeomArgs a = g->args();
+    const double w0 = mls_weight(a.alpha, a.inv_range_sq, 0.0);
+    const int64_t nb = nblocks(N);
+    const int D = g->geom.D;
+    if (D == 1) k_shepard_sums<1><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
+    else if (D == 2) k_shepard_sums<2><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
+    else k_shepard_sums<3><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
+    for (int sw = 0; sw < sweeps; ++sw) {
+        if (D == 1) k_shepard_sweep<1><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
+        else if (D == 2) k_shepard_sweep<2><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
+        else k_shepard_sweep<3><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
+        std::swap(g->d_vol, Vt);
+    }
+    k_clamp_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N);
+    k_fold_sum<<<1, 256, 0, st>>>(g->d_vol, N, tot);
+    double dv = g->geom.L[0];
+    for (int d = 1; d < D; ++d) dv = dv * g->geom.L[d];
+    k_scale_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N, dv, tot);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(S); (void)hipFree(Vt); (void)hipFree(tot);
+    if (e != hipSuccess) { ctx->err = std::string("volumes: ") + hipGetErrorString(e); return HCC_E_HIP; }
+    return HCC_OK;
+}
+
+extern "C" int hcc_grid_volumes(hcc_grid* g, int32_t sweeps, double* volumes) {
+    if (!g) return HCC_E_ARG;
+    hcc_ctx* ctx = g->ctx;
+    if (sweeps >= 0) {
+        HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+        if (int rc = compute_volumes(g, sweeps)) return rc;
+    }
+    if (volumes && g->N) HCC_TRY_HIP(ctx, hipMemcpy(volumes, g->d_vol, sizeof(double) * g->N, hipMemcpyDeviceToHost));
+    return HCC_OK;
+}
+
 extern "C" int hcc_grid_destroy(hcc_grid* g) {
     if (!g) return HCC_OK;
     (void)hipSetDevice(g->ctx->device);
@@ -354,6 +474,7 @@ extern "C" int hcc_grid_destroy(hcc_grid* g) {
     (void)hipFree(g->d_isb); (void)hipFree(g->d_tags); (void)hipFree(g->d_bin_of); (void)hipFree(g->d_cell_start); (void)hipFree(g->d_cell_cursor);
     (void)hipFree(g->d_cell_particles); (void)hipFree(g->d_bin_window); (void)hipFree(g->d_cnt); (void)hipFree(g->d_chunk_off); (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_halo_off); (void)hipFree(g->d_halo_idx); (void)hipFree(g->d_nbr_local);
+    (void)hipFree(g->d_vol);
     delete g;
     return HCC_OK;
 }
@@ -431,6 +552,8 @@ extern "C" int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_
             g->max_halo = 0;   // a halo too large for 16-bit local indices: the passes read neighbours from HBM
         }
     }
+    rc = compute_volumes(g.get(), 1);   // the constructor's ShepardVolumes() (ParticleGrids.jl:275)
+    if (rc) return rc;
     *out = g.release();
     return HCC_OK;
 }

@@ -9,6 +9,7 @@
 // latency overlaps the arithmetic of the previous slot. Every per-node sum keeps the reference's operation order.
 #include "hcc_device.h"
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -1645,6 +1646,37 @@ __global__ void k_reset_counters(unsigned long long* c, int mode) {
     }
 }
 
+// total_mass / interior_mass (src/Grid/Volumes.jl:103-117): sum_i V_i U_i as mapreduce with init = zero(State) does
+// it, a left fold in index order; thread m folds component m, the workgroup stages the products in LDS.
+template <int D, int M>
+__global__ __launch_bounds__(256) void k_mass(const double* __restrict__ U, const double* __restrict__ V,
+                                              const uint8_t* __restrict__ isb, int64_t N, int interior_only,
+                                              double* __restrict__ out) {
+    using L = Layout<D, M>;
+    constexpr int C = 1024;
+    __shared__ double buf[M][C];
+    double acc = 0.0;
+    for (int64_t c0 = 0; c0 < N; c0 += C) {
+        for (int t = threadIdx.x; t < C; t += 256) {
+            const int64_t i = c0 + t;
+            if (i < N) {
+                double r[L::NR];
+                load_rec<L::NR, L::PU>(U, i, r);
+                const bool skip = interior_only && isb[i];
+#pragma unroll
+                for (int m = 0; m < M; ++m) buf[m][t] = skip ? 0.0 : V[i] * r[D + m];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < M) {
+            const int m = threadIdx.x, n = (int)(N - c0 < C ? N - c0 : C);
+            for (int t = 0; t < n; ++t) acc = acc + buf[m][t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < M) out[threadIdx.x] = acc;
+}
+
 // ============================================================================================================
 // the stepper
 // ============================================================================================================
@@ -1821,6 +1853,9 @@ struct Variant {
         k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, Ks[0], Ks[1], Ks[2], Ks[3], Ks[4], c[0],
                                                              c[1], c[2], c[3], c[4], nterm, use, target);
     }
+    static void mass(const hcc_stepper* s, int interior_only, double* out_dev) {
+        k_mass<D, M><<<1, 256, 0, st(s)>>>(s->U, s->grid->d_vol, s->grid->d_isb, s->N, interior_only, out_dev);
+    }
     static void pack(const hcc_stepper* s, const double* U_aos, double* rec) {
         const hcc_grid* g = s->grid;
         k_pack<D, M><<<(unsigned)((s->N + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(
@@ -1841,6 +1876,7 @@ struct VariantOps {
     void (*range)(const hcc_stepper*, const double*);
     void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
+    void (*mass)(const hcc_stepper*, int, double*);
     void (*pack)(const hcc_stepper*, const double*, double*);
     void (*unpack)(const hcc_stepper*, const double*, double*);
     int PU, GS;
@@ -1849,7 +1885,7 @@ template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
     return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::range, &V::cfl, &V::predict,
-                      &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
+                      &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     if (kind == HCC_PDE_LINADV) {
@@ -2470,5 +2506,104 @@ extern "C" int hcc_step_totals(hcc_stepper* s, int reset, int64_t* steps, int64_
     if (ms_gather) *ms_gather = s->tot_gather_ms;
     if (ms_flux) *ms_flux = s->tot_flux_ms;
     if (reset) { s->tot_steps = s->tot_iters = 0; s->tot_ms = s->tot_gather_ms = s->tot_flux_ms = 0; }
+    return HCC_OK;
+}
+
+extern "C" int hcc_total_mass(hcc_stepper* s, int32_t interior_only, double* out) {
+    if (!s || !out) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    const int M = s->M;
+    if (s->N == 0) {
+        for (int m = 0; m < M; ++m) out[m] = 0.0;
+        return HCC_OK;
+    }
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    ops.mass(s, interior_only ? 1 : 0, s->scratch);   // after the queued steps, on the context stream
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(out, s->scratch, sizeof(double) * M, hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HCC_OK;
+}
+
+// The elements of range(0, T, length = n + 1) (test/time_integration.jl:92): Julia's TwicePrecision range gives the
+// correctly rounded k T / n; with a 113-bit significand k T is exact and the quotient, rounded once more to 53 bits,
+// is correctly rounded (k T / n lies on a 53-bit rounding boundary only if it is one exactly).
+static double snapshot_time(int64_t k, int64_t n, double T) {
+    if (k == 0) return 0.0;
+    if (k == n) return T;
+    const __float128 q = ((__float128)k * (__float128)T) / (__float128)n;
+    return (double)q;
+}
+
+extern "C" int hcc_solve(hcc_stepper* s, double tmax, double dt, int32_t is_cfl, int32_t snapshots,
+                         int32_t remove_ghosts, double* t_hist, double* x_hist, double* u_hist, double* mass_hist,
+                         int64_t* steps, double* elapsed) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    if (snapshots < 1 || !std::isfinite(tmax) || !(dt > 0) || !std::isfinite(dt))
+        return set_err(ctx, HCC_E_ARG, "hcc_solve: snapshots >= 1, tmax finite, dt positive and finite");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = s->N;
+    const int D = s->D, M = s->M;
+    const std::vector<uint8_t>& isb = s->grid->h_isb;
+    std::vector<int64_t> keep;   // the particles a snapshot holds (remove_ghosts: interior only, in order)
+    for (int64_t i = 0; i < N; ++i)
+        if (!remove_ghosts || !isb[(size_t)i]) keep.push_back(i);
+    const int64_t ns = (int64_t)keep.size();
+    std::vector<double> pos((size_t)(N * D)), U((size_t)(N * M));
+    if (x_hist && N) {
+        if (int rc = hcc_grid_get_positions(s->grid, pos.data())) return rc;
+    }
+    for (int c = 0; t_hist && c <= snapshots; ++c) t_hist[c] = NAN;   // a snapshot the loop never reaches
+    auto save = [&](int c, double t) -> int {   // saveData! (test/time_integration.jl:13-40)
+        if (t_hist) t_hist[c] = t;
+        if (u_hist && N) {
+            if (int rc = hcc_state_download(s, U.data())) return rc;
+            for (int64_t q = 0; q < ns; ++q)
+                for (int m = 0; m < M; ++m) u_hist[((int64_t)c * ns + q) * M + m] = U[(size_t)(keep[(size_t)q] * M + m)];
+        }
+        if (x_hist)
+            for (int64_t q = 0; q < ns; ++q)
+                for (int d = 0; d < D; ++d) x_hist[((int64_t)c * ns + q) * D + d] = pos[(size_t)(keep[(size_t)q] * D + d)];
+        if (mass_hist) {
+            if (int rc = hcc_total_mass(s, 0, mass_hist + (int64_t)c * 2 * M)) return rc;
+            if (int rc = hcc_total_mass(s, 1, mass_hist + (int64_t)c * 2 * M + M)) return rc;
+        }
+        return HCC_OK;
+    };
+    std::vector<double> t_snap((size_t)snapshots + 1);
+    for (int k = 0; k <= snapshots; ++k) t_snap[(size_t)k] = snapshot_time(k, snapshots, tmax);
+    double t = 0.0;
+    int64_t k_step = 0;
+    int counter = 0;
+    if (int rc = save(counter, t)) return rc;
+    ++counter;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (t < tmax) {
+        double dtv = dt;
+        if (is_cfl) {
+            double c = 0.0;
+            if (int rc = hcc_time_step(s, &c)) return rc;
+            dtv = dt * c;
+        }
+        dtv = jmin(dtv, tmax - t);
+        if (dtv <= 1e-12) break;
+        if (int rc = hcc_step(s, t, dtv)) return rc;
+        t += dtv;
+        ++k_step;
+        while (counter < snapshots && t >= t_snap[(size_t)counter]) {   // 1-based: snap_counter <= snapshots
+            if (int rc = save(counter, t)) return rc;
+            ++counter;
+        }
+    }
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (counter <= snapshots)
+        if (int rc = save(snapshots, t)) return rc;
+    if (int rc = finish_pending(s)) return rc;
+    if (steps) *steps = k_step;
+    if (elapsed) *elapsed = secs;
     return HCC_OK;
 }

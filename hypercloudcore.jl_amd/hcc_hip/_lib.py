@@ -28,7 +28,7 @@ EXPORTS = [
     "hcc_grid_get_bins", "hcc_grid_get_positions", "hcc_stepper_create", "hcc_stepper_destroy", "hcc_state_upload",
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
-    "hcc_step_cfl", "hcc_last_dt",
+    "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve",
 ]
 
 
@@ -91,6 +91,9 @@ def lib():
         "hcc_step_totals": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(d), P(d), P(d)]),
         "hcc_step_cfl": (C.c_int, [vp, C.c_double, C.c_double]),
         "hcc_last_dt": (C.c_int, [vp, P(d)]),
+        "hcc_grid_volumes": (C.c_int, [vp, C.c_int32, vp]),
+        "hcc_total_mass": (C.c_int, [vp, C.c_int32, vp]),
+        "hcc_solve": (C.c_int, [vp, d, d, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, P(i64), P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -219,6 +222,12 @@ class Grid:
         check(lib().hcc_grid_get_positions(self.h, ptr(out)), self.ctx.h, "hcc_grid_get_positions")
         return out
 
+    def volumes(self, sweeps=-1):
+        """Particle volumes (ShepardVolumes); sweeps >= 0 recomputes them with that many sweeps first."""
+        out = np.zeros(self.N)
+        check(lib().hcc_grid_volumes(self.h, int(sweeps), ptr(out)), self.ctx.h, "hcc_grid_volumes")
+        return out
+
     def close(self):
         if getattr(self, "h", None):
             lib().hcc_grid_destroy(self.h)
@@ -342,6 +351,25 @@ class Stepper:
         v = C.c_double()
         check(lib().hcc_last_dt(self.h, C.byref(v)), self.ctx.h, "hcc_last_dt")
         return v.value
+
+    def mass(self, interior_only=False):
+        """total_mass (or interior_mass) of the device state: sum_i V_i U_i, M values."""
+        out = np.zeros(self.M)
+        check(lib().hcc_total_mass(self.h, int(bool(interior_only)), ptr(out)), self.ctx.h, "hcc_total_mass")
+        return out
+
+    def solve(self, tmax, dt, is_cfl=True, snapshots=10, remove_ghosts=False, masses=True):
+        """solve_equation on the device: (xs, us, ts, steps, elapsed, masses) with xs / us of shape
+        (snapshots + 1, Ns, D / M), ts (snapshots + 1,), masses (snapshots + 1, 2, M) = (total, interior) or None."""
+        ns = self.N if not remove_ghosts else int(np.count_nonzero(~self.grid._keep[1].astype(bool)))
+        ts = np.zeros(snapshots + 1)
+        xs = np.zeros((snapshots + 1, ns, self.D))
+        us = np.zeros((snapshots + 1, ns, self.M))
+        ms = np.zeros((snapshots + 1, 2, self.M)) if masses else None
+        steps, el = C.c_int64(), C.c_double()
+        check(lib().hcc_solve(self.h, float(tmax), float(dt), int(bool(is_cfl)), int(snapshots), int(bool(remove_ghosts)),
+                              ptr(ts), ptr(xs), ptr(us), ptr(ms), C.byref(steps), C.byref(el)), self.ctx.h, "hcc_solve")
+        return xs, us, ts, steps.value, el.value, ms
 
     def totals(self, reset=True):
         """(steps, MOOD iterations, step ms, full gather-pass ms, full flux-pass ms) summed since the last reset"""

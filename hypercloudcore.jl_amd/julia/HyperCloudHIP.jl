@@ -160,6 +160,46 @@ function step_totals(ts::HIPRKTimeStepper; reset::Bool = true)
     return (steps = Int(n[]), mood_iterations = Int(it[]), ms_step = a[], ms_gather = b[], ms_flux = c[])
 end
 
-export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals
+"`pg.core.volumes` of the device grid (`update_volumes!(pg, ShepardVolumes(sweeps = sweeps))` first if `sweeps >= 0`)."
+function hip_volumes(ts::HIPRKTimeStepper; sweeps::Integer = -1)
+    v = zeros(Float64, ts.N)
+    checkrc(ccall((:hcc_grid_volumes, libhcc), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), ts.grid, Int32(sweeps), v), ts.ctx)
+    return v
+end
+
+"`total_mass(pg)` (or `interior_mass(pg)`) of the device state: `Σ V_i U_i`."
+function hip_total_mass(ts::HIPRKTimeStepper; interior::Bool = false)
+    m = zeros(Float64, ts.M)
+    checkrc(ccall((:hcc_total_mass, libhcc), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), ts.st, Int32(interior), m), ts.ctx)
+    return m
+end
+
+"""
+    hip_solve_equation(ts, tmax, dt; is_cfl = false, snapshots = 10, remove_ghosts = false)
+
+`solve_equation` (test/time_integration.jl) with the loop on the device state: returns
+`(position_history, state_history, time_history, total_steps, elapsed_wall_time, mass_history)`, the histories as
+matrices per snapshot (`D × Ns`, `M × Ns`) and `mass_history[c] = (total_mass, interior_mass)`.
+"""
+function hip_solve_equation(ts::HIPRKTimeStepper, tmax::Real, dt::Real; is_cfl::Bool = false, snapshots::Integer = 10,
+                            remove_ghosts::Bool = false, D::Integer = 2, N_interior::Integer = ts.N)
+    ns = remove_ghosts ? N_interior : ts.N
+    th = zeros(Float64, snapshots + 1)
+    xh = zeros(Float64, D, ns, snapshots + 1)
+    uh = zeros(Float64, ts.M, ns, snapshots + 1)
+    mh = zeros(Float64, ts.M, 2, snapshots + 1)
+    steps, el = Ref{Int64}(0), Ref{Float64}(0)
+    checkrc(ccall((:hcc_solve, libhcc), Cint,
+                  (Ptr{Cvoid}, Float64, Float64, Int32, Int32, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                   Ptr{Float64}, Ref{Int64}, Ref{Float64}),
+                  ts.st, tmax, dt, Int32(is_cfl), Int32(snapshots), Int32(remove_ghosts), th, xh, uh, mh, steps, el), ts.ctx)
+    xs = [xh[:, :, c] for c in 1:snapshots + 1]
+    us = [uh[:, :, c] for c in 1:snapshots + 1]
+    ms = [(mh[:, 1, c], mh[:, 2, c]) for c in 1:snapshots + 1]
+    return xs, us, th, Int(steps[]), el[], ms
+end
+
+export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
+       hip_volumes, hip_total_mass, hip_solve_equation
 
 end # module

@@ -162,6 +162,24 @@ int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, d
 int hcc_step_totals(hcc_stepper* st, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
                     double* ms_gather, double* ms_flux);
 
+/* ---- particle volumes, conservation diagnostics, driver loop ------------------------------------------- */
+/* Replaces update_volumes!(pg, ShepardVolumes(sweeps)) (src/Grid/Volumes.jl:183-231), which the ParticleGrid
+ * constructor runs with one sweep (src/Grid/ParticleGrids.jl:275; hcc_grid_create does the same). sweeps >= 0
+ * recomputes the volumes; volumes (N doubles, may be NULL) receives the current ones. The sum of the renormalisation
+ * is a left fold in index order. */
+int hcc_grid_volumes(hcc_grid* g, int32_t sweeps, double* volumes);
+/* Replaces total_mass(pg) / interior_mass(pg) (src/Grid/Volumes.jl:103-117): sum_i V_i U_i over all particles, or
+ * the interior ones, a left fold in index order like the reference's mapreduce with init; out: M doubles. */
+int hcc_total_mass(hcc_stepper* st, int32_t interior_only, double* out);
+/* Replaces solve_equation(ts, eq, pg, tmax, dt; is_cfl, snapshots, remove_ghosts) (test/time_integration.jl:72-132):
+ * the driver loop on the device state (dt = dt * get_time_step when is_cfl, capped at tmax - t, stop at dt <= 1e-12),
+ * snapshots at the first step reaching each of the snapshots + 1 evenly spaced times and at the end. Outputs (any may
+ * be NULL): t_hist[snapshots + 1] (NaN for a snapshot the loop never reaches), x_hist and u_hist
+ * [(snapshots + 1) x Ns x D or M] with Ns = N or the interior count (remove_ghosts), mass_hist
+ * [(snapshots + 1) x 2M] (total_mass, interior_mass of each snapshot), steps, elapsed (seconds of the loop). */
+int hcc_solve(hcc_stepper* st, double tmax, double dt, int32_t is_cfl, int32_t snapshots, int32_t remove_ghosts,
+              double* t_hist, double* x_hist, double* u_hist, double* mass_hist, int64_t* steps, double* elapsed);
+
 #ifdef __cplusplus
 }
 #endif

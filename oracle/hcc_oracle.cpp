@@ -493,6 +493,8 @@ struct SimBase {
     virtual void force_of(const uint8_t* trig, const int32_t* snap, int32_t* o, uint8_t* ch) = 0;
     virtual void expand_of(uint8_t* mask, int hops) = 0;
     virtual int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) = 0;
+    virtual void volumes_of(int sweeps, double* out) = 0;
+    virtual void mass_of(int interior_only, double* out) const = 0;
 };
 
 template <int D, int M>
@@ -552,7 +554,55 @@ struct Sim : SimBase {
         invL = 1.0 / mindx;
         search();
         prepare_geometry();
+        update_volumes(1);   // the ParticleGrid constructor's ShepardVolumes() (ParticleGrids.jl:275)
     }
+
+    // update_volumes! with ShepardVolumes (Volumes.jl:183-214): S_i = w(0) + sum_k w_k, V_i = 1 / S_i; `sweeps`
+    // partition-of-unity sweeps V_i <- (w(0) / S_i) V_i + sum_k (w_k / S_j) V_j over all particles; then
+    // _renormalise_volumes! (:226-231): V = max(V, 0), V *= |Omega| / sum(V). sum(V) is a left fold in index order
+    // here and on the device; Julia's sum of an array reassociates (pairwise blocks, @simd), so that one sum is
+    // parity unpinned at its last bits.
+    std::vector<double> vol;
+    void update_volumes(int sweeps) {
+        vol.assign(N, 0.0);
+        if (N == 0) return;
+        const double w0 = weight_fn(alpha, inv_range_sq, 0.0);
+        std::vector<double> S(N), Vt(N);
+        for (int64_t i = 0; i < N; ++i) {
+            double sv = w0;
+            for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) sv = sv + wts[k];
+            S[i] = sv;
+            vol[i] = 1.0 / sv;
+        }
+        for (int sw = 0; sw < sweeps; ++sw) {
+            for (int64_t i = 0; i < N; ++i) {
+                double acc = w0 / S[i] * vol[i];
+                for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) acc = acc + wts[k] / S[nbr[k]] * vol[nbr[k]];
+                Vt[i] = acc;
+            }
+            vol = Vt;
+        }
+        for (int64_t i = 0; i < N; ++i) vol[i] = jmax(vol[i], 0.0);
+        double tot = vol[0];
+        for (int64_t i = 1; i < N; ++i) tot = tot + vol[i];
+        double dv = g.L[0];   // domain_volume: prod(maxs - mins) (Volumes.jl:86)
+        for (int d = 1; d < D; ++d) dv = dv * g.L[d];
+        const double f = dv / tot;
+        for (int64_t i = 0; i < N; ++i) vol[i] = vol[i] * f;
+    }
+    // total_mass / interior_mass (Volumes.jl:103-117): mapreduce with init = zero(State), a left fold in index order
+    void mass(int interior_only, double* out) const {
+        State acc{};
+        if ((int64_t)rhos.size() != N) { for (int m = 0; m < M; ++m) out[m] = 0.0; return; }
+        for (int64_t i = 0; i < N; ++i)
+            for (int m = 0; m < M; ++m) acc[m] = acc[m] + ((interior_only && isb[i]) ? 0.0 : vol[i] * rhos[i][m]);
+        for (int m = 0; m < M; ++m) out[m] = acc[m];
+    }
+    void volumes_of(int sweeps, double* out) override {
+        if (sweeps >= 0) update_volumes(sweeps);
+        if (out) std::copy(vol.begin(), vol.end(), out);
+    }
+    void mass_of(int interior_only, double* out) const override { mass(interior_only, out); }
 
     void set_threads(int n) override { threads = n < 1 ? 1 : n; }
 
@@ -1312,6 +1362,9 @@ void orc_sim_force(void* s, const uint8_t* trig, const int32_t* snap, int32_t* o
     ((SimBase*)s)->force_of(trig, snap, o, ch);
 }
 void orc_sim_expand(void* s, uint8_t* mask, int hops) { ((SimBase*)s)->expand_of(mask, hops); }
+// sweeps < 0: read the current volumes
+void orc_sim_volumes(void* s, int sweeps, double* out) { ((SimBase*)s)->volumes_of(sweeps, out); }
+void orc_sim_mass(void* s, int interior_only, double* out) { ((SimBase*)s)->mass_of(interior_only, out); }
 int orc_sim_u2_flags(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau) {
     return ((SimBase*)s)->u2_flags_of(U, p, cand, delta, plateau);
 }

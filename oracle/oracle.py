@@ -47,6 +47,8 @@ def lib():
             "orc_sim_force": (None, [vp, vp, vp, vp, vp]),
             "orc_sim_expand": (None, [vp, vp, C.c_int]),
             "orc_sim_u2_flags": (C.c_int, [vp, vp, i64, vp, d, d]),
+            "orc_sim_volumes": (None, [vp, C.c_int, vp]),
+            "orc_sim_mass": (None, [vp, C.c_int, vp]),
             "orc_weight": (d, [d, d, d]),
             "orc_limiter_phi": (d, [C.c_int, d]),
             "orc_jmax": (d, [d, d]),
@@ -241,6 +243,18 @@ class OracleSim:
         c = np.ascontiguousarray(cand, np.float64)
         return bool(lib().orc_sim_u2_flags(self.h, _p(U), p, _p(c), delta, plateau))
 
+    def volumes(self, sweeps=-1):
+        """The particle volumes (ShepardVolumes); sweeps >= 0 recomputes them with that many sweeps first."""
+        out = np.zeros(self.N)
+        lib().orc_sim_volumes(self.h, int(sweeps), _p(out))
+        return out
+
+    def mass(self, interior_only=False):
+        """total_mass / interior_mass: sum_i V_i U_i (left fold)."""
+        out = np.zeros(self.M)
+        lib().orc_sim_mass(self.h, int(bool(interior_only)), _p(out))
+        return out
+
     def mood_iterations(self):
         return lib().orc_sim_mood_iterations(self.h)
 
@@ -281,3 +295,46 @@ def rng_uniform(seed, n):
     out = np.zeros(n)
     lib().orc_rng_uniform(seed, n, _p(out))
     return out
+
+
+def snapshot_times(tmax, snapshots):
+    """range(0, tmax, length = snapshots + 1) (test/time_integration.jl:92): Julia's TwicePrecision range, whose
+    elements are the correctly rounded k tmax / snapshots."""
+    from fractions import Fraction
+    n = int(snapshots)
+    return [0.0 if k == 0 else (float(tmax) if k == n else float(Fraction(k) * Fraction(float(tmax)) / n))
+            for k in range(n + 1)]
+
+
+def solve_equation(o, tmax, dt, is_cfl=True, snapshots=10, remove_ghosts=False, isb=None, pos=None):
+    """solve_equation (test/time_integration.jl:72-132) on the oracle: the driver loop with dt = dt * get_time_step
+    (is_cfl) capped to tmax - t, a stop below 1e-12, and snapshots of the state at the first step reaching each of
+    the snapshots + 1 evenly spaced times (the last one after the loop). Returns (xs, us, ts, k_step, masses) with
+    masses[c] = (total_mass, interior_mass) of snapshot c."""
+    t_snap = snapshot_times(tmax, snapshots)
+    xs, us, ts, ms = [None] * (snapshots + 1), [None] * (snapshots + 1), [0.0] * (snapshots + 1), [None] * (snapshots + 1)
+    keep = np.ones(o.N, bool) if not remove_ghosts else ~np.asarray(isb, bool)
+
+    def save(c, t):
+        ts[c] = t
+        us[c] = o.state()[keep]
+        xs[c] = None if pos is None else np.asarray(pos)[keep]
+        ms[c] = (o.mass(False), o.mass(True))
+
+    t, k_step, counter = 0.0, 0, 0
+    save(counter, t)
+    counter += 1
+    while t < tmax:
+        dt_val = dt * o.time_step() if is_cfl else dt
+        dt_val = min(dt_val, tmax - t)
+        if dt_val <= 1e-12:
+            break
+        o.step(t, dt_val)
+        t += dt_val
+        k_step += 1
+        while counter < snapshots and t >= t_snap[counter]:   # 1-based: snap_counter <= snapshots
+            save(counter, t)
+            counter += 1
+    if counter <= snapshots:
+        save(snapshots, t)
+    return xs, us, ts, k_step, ms
