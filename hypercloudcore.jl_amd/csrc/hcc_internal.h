@@ -102,6 +102,8 @@ struct StepArgs {
     int streamed;                // divergence accumulated per slot with the derivative weights (flux variant 1)
     int aligned;                 // AlignedRusanovFlux: sort by the weights' signs below the full order (variant 1)
     int packed;                  // the orders array of a flux pass holds EPD2's minimum | true order << 4
+    int src_mode;                // HCC_SRC_*: the decay source S_i(U) = -src_rate[i] U, explicit or implicit
+    const double* src_rate;
     const double* invg;          // reconstruction inverse moments, D*D per node: [q * N + i]
     const double* invd;          // divergence inverse moments, derivative rows 1..D: [(r * (D+1) + c) * N + i]
     const uint8_t* okg;

@@ -254,6 +254,92 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
     for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
+// The stage of a tableau with an implicit part or an implicit source (form_candidate!, TimeSteppers.jl:365-397;
+// _stage_predictor!, :400-421; _stage_implicit_step! / _stage_implicit_rhs!, :427-444): Y = U^n + sum_j
+// (dt a~_ij) K_E[j] + (dt a_ij) K_I[j] (each term for a non-zero weight); for a stage with the implicit decay source
+// the diagonal solve U_out = U_in / (1 + (dt a_ii) rate) and K_I = (U_out - U_in) inv(dt a_ii) if |a_ii| > 1e-14,
+// else K_I = -rate Y; K_I = 0 on boundary particles and without an implicit source. Outflow ghosts take their root's
+// value (its combination and solve, the same bits); other boundary particles keep pg.rhos.
+struct ImexTerms {
+    const double* KE[kMaxStages];
+    const double* KI[kMaxStages];
+    double at[kMaxStages], ai[kMaxStages];   // explicit and implicit weights (a~_ij, a_ij, or b~_j, b_j)
+    int nterm;
+    double aii;                              // the stage's diagonal implicit weight
+    int stage;                               // 1: stage i (K_I[i] formed), 0: the final state
+    double* KIout;
+};
+
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_predict_imex(StepArgs a, int64_t nblk, const int32_t* __restrict__ root,
+                                                          const double* __restrict__ Un, ImexTerms t,
+                                                          double* __restrict__ target) {
+    using L = Layout<D, M>;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    if (i >= N) return;
+    constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1;
+    const bool ghost = a.g.isb[i];
+    const int64_t src = ghost ? (int64_t)root[i] : i;
+    double r[2 * NP], ki[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) ki[m] = 0.0;
+    const double2* own = reinterpret_cast<const double2*>(Un + i * L::PU) + P0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) { const double2 v = own[q]; r[2 * q] = v.x; r[2 * q + 1] = v.y; }
+    if (src >= 0) {
+        double y[M];
+        if (ghost) {
+            double sv[L::NR];
+            load_rec<L::NR, L::PU>(Un, src, sv);
+#pragma unroll
+            for (int m = 0; m < M; ++m) y[m] = sv[D + m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) y[m] = r[D + m - 2 * P0];
+        }
+        const double dt = *a.dtp;
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j) {
+            if (j < t.nterm) {
+                if (t.at[j] != 0.0) {
+                    const double c = dt * t.at[j];
+#pragma unroll
+                    for (int m = 0; m < M; ++m) y[m] = y[m] + c * t.KE[j][src * M + m];
+                }
+                if (t.ai[j] != 0.0) {
+                    const double c = dt * t.ai[j];
+#pragma unroll
+                    for (int m = 0; m < M; ++m) y[m] = y[m] + c * t.KI[j][src * M + m];
+                }
+            }
+        }
+        if (t.stage && a.src_mode == HCC_SRC_DECAY_IMPLICIT) {
+            const double lam = a.src_rate[src];
+            if (fabs(t.aii) > 1e-14) {
+                const double den = 1.0 + (dt * t.aii) * lam, inv = 1.0 / (dt * t.aii);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const double uo = y[m] / den;
+                    ki[m] = (uo - y[m]) * inv;
+                    y[m] = uo;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; ++m) ki[m] = (-lam) * y[m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) r[D + m - 2 * P0] = y[m];
+    }
+    double2* dst = reinterpret_cast<double2*>(target + i * L::PU) + P0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    if (t.stage)
+#pragma unroll
+        for (int m = 0; m < M; ++m) t.KIout[i * M + m] = ghost ? 0.0 : ki[m];
+}
+
 // ============================================================================================================
 // gather pass: update_content! (MUSCL.jl:351-384), cached reconstruction fit (Interpolators.jl:585-602),
 // Venkatakrishnan / Barth-Jespersen limiter (Limiter.jl:61-143)
@@ -698,11 +784,18 @@ __device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M
     }
 }
 
+// evaluate_sources of the explicit sources added to K_E (TimeSteppers.jl:354): zero(U) without one, the decay source
+// -rate_i U_i with one
+template <int M>
+__device__ __forceinline__ double explicit_source(const StepArgs& a, int64_t i, const double (&fi)[M], int m) {
+    return a.src_mode == HCC_SRC_DECAY_EXPLICIT ? (-a.src_rate[i]) * fi[m] : 0.0;
+}
+
 // The divergence of the buffered fit (apply_weights_divergence, Interpolators.jl:610-626) from the accumulators of
-// the derivative rows, K_E = -2 div (+ zero sources); a node without a cached fit gets a zero divergence.
+// the derivative rows, K_E = -2 div + sources; a node without a cached fit gets a zero divergence.
 template <int D, int M>
 __device__ __forceinline__ void fit_divergence(const StepArgs& a, int64_t i, const double (&b)[D + 1][D * M],
-                                               double (&Kout)[M]) {
+                                               const double (&fi)[M], double (&Kout)[M]) {
     constexpr int BD = D + 1;
     const int64_t N = a.g.N;
     if (a.okd[i]) {
@@ -716,11 +809,11 @@ __device__ __forceinline__ void fit_divergence(const StepArgs& a, int64_t i, con
                 for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
                 acc = acc + r;
             }
-            Kout[m] = -(2.0 * (acc * a.g.invL)) + 0.0;
+            Kout[m] = -(2.0 * (acc * a.g.invL)) + explicit_source<M>(a, i, fi, m);
         }
     } else {
 #pragma unroll
-        for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+        for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, fi, m);
     }
 }
 
@@ -773,7 +866,7 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
 #pragma unroll
         for (int m = 0; m < M; ++m) nd.fi[m] = nd.ri[D + m];
 #pragma unroll
-        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
+        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + explicit_source<M>(a, i, nd.fi, m);
     } else {
         const int64_t base = sell_base(a.g, i);
         load_flux_node<D, M, KIND, FV>(a, Y, i, nd);
@@ -833,9 +926,9 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
         if (FV == 1 && a.streamed) {
             const bool okd = a.okd[i];
 #pragma unroll
-            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? sacc[m] : 0.0)) + 0.0;
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? sacc[m] : 0.0)) + explicit_source<M>(a, i, nd.fi, m);
         } else {
-            fit_divergence<D, M>(a, i, b, Kout);
+            fit_divergence<D, M>(a, i, b, nd.fi, Kout);
         }
     }
     return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, nd.fi, Kout, h);
@@ -1039,21 +1132,21 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
             const bool okd = a.okd[i];
             if (cnt == 0) {
 #pragma unroll
-                for (int m = 0; m < M; ++m) Kout[m] = -0.0 + 0.0;
+                for (int m = 0; m < M; ++m) Kout[m] = -0.0 + explicit_source<M>(a, i, nd.fi, m);
             } else if (streamed) {
 #pragma unroll
-                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? part[grp][m] : 0.0)) + 0.0;
+                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? part[grp][m] : 0.0)) + explicit_source<M>(a, i, nd.fi, m);
             } else if (okd) {
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
                     double acc = 0.0;
 #pragma unroll
                     for (int d = 0; d < D; ++d) acc = acc + part[grp][d * M + m];
-                    Kout[m] = -(2.0 * (acc * a.g.invL)) + 0.0;
+                    Kout[m] = -(2.0 * (acc * a.g.invL)) + explicit_source<M>(a, i, nd.fi, m);
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + 0.0;
+                for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, nd.fi, m);
             }
             const int before = a.inadm[i];
             ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, h);
@@ -1283,7 +1376,8 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
 // TimeSteppers.jl:501-518). changed = triggered after the decrement (a flagged particle at order 1 is un-flagged).
 template <int D, int M, int KIND>
 __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* __restrict__ Yp, const double* __restrict__ cand,
-                                                       const double* __restrict__ Kp, int full, int64_t nblk) {
+                                                       const double* __restrict__ Kp, const double* __restrict__ KIp,
+                                                       int full, int64_t nblk) {
     using P = Pde<D, M, KIND>;
     using L = Layout<D, M>;
     const int64_t i = node_index(nblk);
@@ -1295,7 +1389,8 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
             const double h = a.hratio * *a.dtp;
             load_rec<L::NR, L::PU>(Yp, i, r);
 #pragma unroll
-            for (int m = 0; m < M; ++m) W[m] = r[D + m] + h * Kp[i * M + m];
+            for (int m = 0; m < M; ++m)   // with implicit sources the substep is Y + h (K + K_I)
+                W[m] = KIp ? r[D + m] + h * (Kp[i * M + m] + KIp[i * M + m]) : r[D + m] + h * Kp[i * M + m];
         } else {
             load_rec<L::NR, L::PU>(cand, i, r);
 #pragma unroll
@@ -1699,6 +1794,10 @@ struct hcc_stepper {
     double* cand = nullptr;    // rho_cand
     double* Y[kMaxStages] = {};
     double* K[kMaxStages] = {};
+    double* KI[kMaxStages] = {};   // K_I of the stages (a tableau with an implicit part or an implicit source)
+    double* rate = nullptr;        // decay rates of the source
+    int src_mode = HCC_SRC_NONE;
+    bool imex = false;             // stages formed by k_predict_imex
     double* G = nullptr;
     double* scratch = nullptr; // N * M doubles for up/downloads
     int8_t* orders = nullptr;
@@ -1770,6 +1869,8 @@ StepArgs hcc_stepper::args() const {
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
     a.plateau = desc.mood_plateau;
     a.streamed = desc.muscl_streamed ? 1 : 0;
+    a.src_mode = src_mode;
+    a.src_rate = rate;
     a.aligned = desc.flux == HCC_FLUX_ALIGNED_RUSANOV ? 1 : 0;
     a.packed = 0;
     a.range = (desc.mood_criterion == HCC_CRIT_U2 && desc.mood_plateau > 0.0) ? range : nullptr;
@@ -1828,9 +1929,13 @@ struct Variant {
         else k_flux_list<D, M, KIND, 0><<<split_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
-                     int full) {
+                     const double* KIp, int full) {
         const int64_t nb = nblocks(s->N);
-        k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, Kp, full, nb);
+        k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, Kp, KIp, full, nb);
+    }
+    static void predict_imex(const hcc_stepper* s, const StepArgs& a, const ImexTerms& t, double* target) {
+        const int64_t nb = nblocks(s->N);
+        k_predict_imex<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, t, target);
     }
     static void range(const hcc_stepper* s, const double* Y) {
         const int64_t nb = (s->N + kBlock - 1) / kBlock;
@@ -1872,7 +1977,8 @@ struct VariantOps {
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
     void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int);
-    void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, const double*, int);
+    void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, const double*, const double*, int);
+    void (*predict_imex)(const hcc_stepper*, const StepArgs&, const ImexTerms&, double*);
     void (*range)(const hcc_stepper*, const double*);
     void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
@@ -1884,7 +1990,7 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::range, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex, &V::range, &V::cfl, &V::predict,
                       &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
@@ -1910,7 +2016,8 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters})
         (void)hipFree(p);
-    for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); }
+    for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); }
+    (void)hipFree(s->rate);
     for (auto& r : s->ring) {
         if (r.ev0) (void)hipEventDestroy(r.ev0);
         if (r.ev1) (void)hipEventDestroy(r.ev1);
@@ -1937,6 +2044,16 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     if (d->mood_criterion < HCC_CRIT_NONE || d->mood_criterion > HCC_CRIT_U2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD criterion");
     if (!(d->mood_plateau >= 0.0) || !std::isfinite(d->mood_plateau)) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: MOODu2 plateau must be finite and >= 0");
     if (d->flux != HCC_FLUX_RUSANOV && d->flux != HCC_FLUX_ALIGNED_RUSANOV) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: numerical flux");
+    bool implicit_part = false;
+    for (int i = 0; i < d->stages; ++i) {
+        implicit_part = implicit_part || d->b_im[i] != 0.0;
+        for (int j = 0; j < d->stages; ++j) {
+            implicit_part = implicit_part || d->a_im[i][j] != 0.0;
+            if (j > i && d->a_im[i][j] != 0.0) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: a_im must be lower triangular");
+        }
+    }
+    if (implicit_part && d->substep_ratio != 0.0)
+        return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: a tableau with an implicit part has no SSP coefficient (substep_ratio 0)");
     if (d->n_bc < 0 || d->n_bc > 16) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: at most 16 boundary conditions");
     for (int i = 0; i < d->stages; ++i)
         for (int j = i; j < d->stages; ++j)
@@ -1946,6 +2063,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     s->ctx = ctx; s->grid = g; s->desc = *d; s->D = D; s->M = d->M; s->kind = d->pde; s->N = g->N;
     s->PU = ops.PU; s->GS = ops.GS;
     s->fv = (d->muscl_streamed || d->flux == HCC_FLUX_ALIGNED_RUSANOV) ? 1 : 0;
+    s->imex = implicit_part;
     const int64_t N = g->N, M = d->M;
     hipStream_t st = ctx->stream;
     HCC_TRY_HIP(ctx, dalloc(&s->invg, (int64_t)D * D * N));
@@ -1963,6 +2081,11 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->K[k], M * N));
         HCC_TRY_HIP(ctx, hipMemsetAsync(s->K[k], 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     }
+    if (s->imex)
+        for (int k = 0; k < d->stages; ++k) {
+            HCC_TRY_HIP(ctx, dalloc(&s->KI[k], M * N));
+            HCC_TRY_HIP(ctx, hipMemsetAsync(s->KI[k], 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
+        }
     HCC_TRY_HIP(ctx, dalloc(&s->G, s->GS * N));
     HCC_TRY_HIP(ctx, dalloc(&s->orders, N));
     HCC_TRY_HIP(ctx, dalloc(&s->changed, N));
@@ -2117,8 +2240,9 @@ struct StepRunner {
     int64_t nb = 0;
     int iters = 0;
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
-    // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK)
-    bool fused() const { return mood_fused(s->desc); }
+    // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK); not with an implicit
+    // part, whose stages and substeps involve K_I
+    bool fused() const { return mood_fused(s->desc) && !s->imex; }
     bool epd2() const { return s->desc.mood_strategy == HCC_STRAT_EPD2; }
     // the arguments of a flux pass: with EPD2 the interface order of an edge is the minimum of the two particles'
     // neighbourhood minima, i.e. the edge-minimum rule of EPD1 applied to nbmin (computed before the pass)
@@ -2132,7 +2256,7 @@ struct StepRunner {
         return f;
     }
     // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself
-    bool fuse_pred() const { return fused() || s->desc.mood_criterion == HCC_CRIT_NONE; }
+    bool fuse_pred() const { return !s->imex && (fused() || s->desc.mood_criterion == HCC_CRIT_NONE); }
     bool y1_in_U = false;
     // state of stage k (1-based): U itself once the boundary conditions hold on it (every step after the first since
     // the state was loaded), else the predictor's copy in Y[0]
@@ -2160,6 +2284,24 @@ struct StepRunner {
         unsigned use;
         double* target;
         terms(i, Ks, at, n_prev, use, target);
+        if (s->imex) {   // the IMEX stage: both parts of the tableau, the diagonal solve and K_I[i]
+            const int st = s->desc.stages;
+            const bool fin = i > st;
+            ImexTerms t{};
+            for (int j = 0; j < kMaxStages; ++j) { t.KE[j] = s->K[0]; t.KI[j] = s->KI[0]; }
+            for (int j = 0; j < n_prev; ++j) {
+                t.KE[j] = s->K[j];
+                t.KI[j] = s->KI[j];
+                t.at[j] = fin ? s->desc.b_t[j] : s->desc.a_t[i - 1][j];
+                t.ai[j] = fin ? s->desc.b_im[j] : s->desc.a_im[i - 1][j];
+            }
+            t.nterm = n_prev;
+            t.stage = fin ? 0 : 1;
+            t.aii = fin ? 0.0 : s->desc.a_im[i - 1][i - 1];
+            t.KIout = fin ? nullptr : s->KI[i - 1];
+            ops.predict_imex(s, a, t, target);
+            return;
+        }
         ops.predict(s, a, Ks, at, n_prev, use, target);
     }
     PredArgs pred_args(int i, bool count) const {
@@ -2198,7 +2340,10 @@ struct StepRunner {
     // the rest of iteration j runs, and iteration j + 1 is enqueued only if n_{j-1} > 0 (if n_j turns out 0,
     // iteration j + 1 does nothing).
     hipError_t checkpoint(int i) {
-        if (s->desc.mood_criterion == HCC_CRIT_NONE) return hipSuccess;   // Y_i formed by the flux pass
+        if (s->desc.mood_criterion == HCC_CRIT_NONE) {   // Y_i formed by the flux pass, or here (IMEX)
+            if (!fuse_pred()) predict(i);
+            return hipGetLastError();
+        }
         const double* Yp = stage_state(i - 1);
         double* Kp = s->K[i - 2];
         hipStream_t str = s->ctx->stream;
@@ -2225,7 +2370,8 @@ struct StepRunner {
         for (;;) {
             if (!sub) predict(i);
             k_reset_counters<<<1, 64, 0, str>>>(s->counters, 0);
-            ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], Kchk, full);
+            const double* KIchk = (sub && s->src_mode == HCC_SRC_DECAY_IMPLICIT) ? s->KI[i - 2] : nullptr;
+            ops.flag(s, a, Yp, i > s->desc.stages ? s->cand : s->Y[i - 1], Kchk, KIchk, full);
             unsigned long long n = 0;
             hipError_t e = read_changed(n);
             if (e != hipSuccess) return e;
@@ -2256,7 +2402,7 @@ struct StepRunner {
         const int st = s->desc.stages;
         hipStream_t str = ctx->stream;
         nb = nblocks(s->N);
-        y1_in_U = !s->fresh;
+        y1_in_U = !s->fresh && !s->imex;
         // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
         // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
         // particles). The full gather also clears the changed count for the check of the flux pass.
@@ -2366,7 +2512,7 @@ extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
     if (s->N == 0) return HCC_OK;
     // stage 1 reads U itself once the boundary conditions hold on it (every step after the first since the state was
     // loaded): the CFL estimate of U then runs inside the stage-1 gather, which stages the same neighbourhoods
-    s->cfl_fused = !s->fresh && s->lds_gcfl > 0;
+    s->cfl_fused = !s->fresh && !s->imex && s->lds_gcfl > 0;
     if (!s->cfl_fused) {
         VariantOps ops;
         find_ops(s->D, s->M, s->kind, ops);
@@ -2605,5 +2751,43 @@ extern "C" int hcc_solve(hcc_stepper* s, double tmax, double dt, int32_t is_cfl,
     if (int rc = finish_pending(s)) return rc;
     if (steps) *steps = k_step;
     if (elapsed) *elapsed = secs;
+    return HCC_OK;
+}
+
+extern "C" int hcc_set_source(hcc_stepper* s, int32_t mode, const double* rates) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    if (mode < HCC_SRC_NONE || mode > HCC_SRC_DECAY_IMPLICIT) return set_err(ctx, HCC_E_ARG, "hcc_set_source: unknown mode");
+    if (mode != HCC_SRC_NONE && !rates && s->N) return set_err(ctx, HCC_E_ARG, "hcc_set_source: rates required");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    const int64_t N = s->N, M = s->M;
+    hcc_stepper_desc& d = s->desc;
+    if (mode == HCC_SRC_DECAY_IMPLICIT) {
+        bool zero = true;   // _explicit_implicit_part: an explicit tableau applies the implicit source explicitly
+        for (int i = 0; i < d.stages; ++i) {
+            zero = zero && d.b_im[i] == 0.0;
+            for (int j = 0; j < d.stages; ++j) zero = zero && d.a_im[i][j] == 0.0;
+        }
+        if (zero)
+            for (int i = 0; i < d.stages; ++i) {
+                d.b_im[i] = d.b_t[i];
+                d.c_im[i] = d.c_t[i];
+                for (int j = 0; j < d.stages; ++j) d.a_im[i][j] = d.a_t[i][j];
+            }
+        if (!s->imex) {
+            for (int k = 0; k < d.stages; ++k) {
+                HCC_TRY_HIP(ctx, dalloc(&s->KI[k], M * N));
+                HCC_TRY_HIP(ctx, hipMemsetAsync(s->KI[k], 0, sizeof(double) * std::max<int64_t>(M * N, 1), ctx->stream));
+            }
+            s->imex = true;
+        }
+    }
+    if (mode != HCC_SRC_NONE && N) {
+        if (!s->rate) HCC_TRY_HIP(ctx, dalloc(&s->rate, N));
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(s->rate, rates, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    }
+    s->src_mode = mode;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HCC_OK;
 }

@@ -18,6 +18,7 @@ HCC_MODE_SOFT, HCC_MODE_HARD = 0, 1
 HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1, HCC_STRAT_EPD2, HCC_STRAT_STRICT_EPD0 = 0, 1, 2, 3, 4
 HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY, HCC_CRIT_U2 = 0, 1, 2, 3, 4
 HCC_FLUX_RUSANOV, HCC_FLUX_ALIGNED_RUSANOV = 0, 1
+HCC_SRC_NONE, HCC_SRC_DECAY_EXPLICIT, HCC_SRC_DECAY_IMPLICIT = 0, 1, 2
 HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
 HCC_E_CAPACITY = -4
 
@@ -28,7 +29,7 @@ EXPORTS = [
     "hcc_grid_get_bins", "hcc_grid_get_positions", "hcc_stepper_create", "hcc_stepper_destroy", "hcc_state_upload",
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
-    "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve",
+    "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve", "hcc_set_source",
 ]
 
 
@@ -45,7 +46,8 @@ class StepperDesc(C.Structure):
                 ("stages", C.c_int32), ("a_t", (C.c_double * 5) * 5), ("b_t", C.c_double * 5), ("c_t", C.c_double * 5),
                 ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
                 ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double), ("muscl_streamed", C.c_int32),
-                ("flux", C.c_int32)]
+                ("flux", C.c_int32), ("a_im", (C.c_double * 5) * 5), ("b_im", C.c_double * 5),
+                ("c_im", C.c_double * 5)]
 
 
 _lib = None
@@ -94,6 +96,7 @@ def lib():
         "hcc_grid_volumes": (C.c_int, [vp, C.c_int32, vp]),
         "hcc_total_mass": (C.c_int, [vp, C.c_int32, vp]),
         "hcc_solve": (C.c_int, [vp, d, d, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, P(i64), P(d)]),
+        "hcc_set_source": (C.c_int, [vp, C.c_int32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -263,6 +266,11 @@ def stepper_desc(scheme):
         sd.b_t[i], sd.c_t[i] = b_t[i], c_t[i]
         for j in range(s):
             sd.a_t[i][j] = a_t[i][j]
+    if scheme.get("a_im") is not None:   # implicit part of an IMEX tableau
+        for i in range(s):
+            sd.b_im[i], sd.c_im[i] = scheme["b_im"][i], scheme["c_im"][i]
+            for j in range(s):
+                sd.a_im[i][j] = scheme["a_im"][i][j]
     sd.substep_ratio = scheme.get("substep_ratio", 0.0)
     bcs = scheme.get("bcs", [])
     sd.n_bc = len(bcs)
@@ -351,6 +359,11 @@ class Stepper:
         v = C.c_double()
         check(lib().hcc_last_dt(self.h, C.byref(v)), self.ctx.h, "hcc_last_dt")
         return v.value
+
+    def set_source(self, mode, rates):
+        """The decay source S_i(U) = -rates_i U: mode HCC_SRC_DECAY_EXPLICIT / _IMPLICIT, or HCC_SRC_NONE."""
+        r = np.ascontiguousarray(rates, np.float64) if rates is not None else None
+        check(lib().hcc_set_source(self.h, int(mode), ptr(r)), self.ctx.h, "hcc_set_source")
 
     def mass(self, interior_only=False):
         """total_mass (or interior_mass) of the device state: sum_i V_i U_i, M values."""

@@ -28,6 +28,34 @@ TABLEAUS = {
 }
 
 
+def _imex_tableaus():
+    """IMEX tableaus (ButcherTableaus.jl:183-264): explicit part (a~, b~, c~) and implicit part (a, b, c), entries
+    computed with the reference's Float64 expressions."""
+    g2 = 1 - 1 / math.sqrt(2)                      # ARS222, PRSSP3: 1 - 1/sqrt(2)
+    d2 = 1 - 1 / (2 * g2)
+    g3 = (3.0 + math.sqrt(3.0)) / 6.0             # ARS233
+    return {
+        "IMEX_Euler": dict(a_t=[[0.0, 0.0], [1.0, 0.0]], b_t=[1.0, 0.0], c_t=[0.0, 1.0],
+                           a_im=[[0.0, 0.0], [0.0, 1.0]], b_im=[0.0, 1.0], c_im=[0.0, 1.0]),
+        "IMEX_ARS233": dict(a_t=[[0.0, 0.0, 0.0], [g3, 0.0, 0.0], [g3 - 1.0, 2.0 * (1 - g3), 0.0]],
+                            b_t=[0.0, 0.5, 0.5], c_t=[0.0, g3, 1.0 - g3],
+                            a_im=[[0.0, 0.0, 0.0], [0.0, g3, 0.0], [0.0, 1.0 - 2 * g3, g3]],
+                            b_im=[0.0, 0.5, 0.5], c_im=[0.0, g3, 1.0 - g3]),
+        "IMEX_ARS222": dict(a_t=[[0.0, 0.0, 0.0], [g2, 0.0, 0.0], [d2, 1 - d2, 0.0]], b_t=[d2, 1 - d2, 0.0],
+                            c_t=[0.0, g2, 1.0], a_im=[[0.0, 0.0, 0.0], [0.0, g2, 0.0], [0.0, 1 - g2, g2]],
+                            b_im=[0.0, 1 - g2, g2], c_im=[0.0, g2, 1.0]),
+        "IMEX_PRSSP3": dict(a_t=[[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.25, 0.25, 0.0]], b_t=[1 / 6, 1 / 6, 2 / 3],
+                            c_t=[0.0, 1.0, 0.5], a_im=[[g2, 0.0, 0.0], [1 - 2 * g2, g2, 0.0], [0.5 - g2, 0.0, g2]],
+                            b_im=[1 / 6, 1 / 6, 2 / 3], c_im=[g2, 1 - g2, 0.5]),
+        "IMEX_SSP2332": dict(a_t=[[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.5, 0.5, 0.0]], b_t=[1 / 3, 1 / 3, 1 / 3],
+                             c_t=[0.0, 0.5, 1.0], a_im=[[0.25, 0.0, 0.0], [0.0, 0.25, 0.0], [1 / 3, 1 / 3, 1 / 3]],
+                             b_im=[1 / 3, 1 / 3, 1 / 3], c_im=[0.25, 0.25, 1.0]),
+    }
+
+
+IMEX_TABLEAUS = _imex_tableaus()
+
+
 def ssp_coefficient(a_t, b_t):
     """SSP coefficient by bisection on the canonical Shu-Osher form (ButcherTableaus.jl:101-118).
 
@@ -63,6 +91,8 @@ def ssp_coefficient(a_t, b_t):
 
 
 def tableau(name):
+    if name in IMEX_TABLEAUS:   # an implicit part: SSP coefficient 0 (ButcherTableaus.jl:20-21), no substep check
+        return dict(IMEX_TABLEAUS[name], ssp=0.0, substep_ratio=0.0)
     a_t, b_t, c_t = TABLEAUS[name]
     C = ssp_coefficient(a_t, b_t)
     return {"a_t": a_t, "b_t": b_t, "c_t": c_t, "ssp": C, "substep_ratio": (1.0 / C) if C > 0 else 0.0}
@@ -74,7 +104,8 @@ def scheme(M, pde, *, gamma=1.4, adv_speed=None, limiter=L.HCC_LIM_VK, mode=L.HC
     tb = tableau(tableau_name)
     return {"M": M, "pde": pde, "gamma": gamma, "adv_speed": adv_speed or [], "muscl_order": 2, "limiter": limiter,
             "limiter_mode": mode, "strategy": strategy, "criterion": criterion, "delta": delta, "plateau": plateau,
-            "streamed": streamed, "flux": flux, "a_t": tb["a_t"],
+            "streamed": streamed, "flux": flux, "a_t": tb["a_t"], "a_im": tb.get("a_im"), "b_im": tb.get("b_im"),
+            "c_im": tb.get("c_im"),
             "b_t": tb["b_t"], "c_t": tb["c_t"], "substep_ratio": tb["substep_ratio"] if mood_substeps else 0.0,
             "bcs": list(bcs)}
 

@@ -57,6 +57,9 @@ struct StepperDesc
     mood_plateau::Float64
     muscl_streamed::Int32
     flux::Int32
+    a_im::NTuple{5, NTuple{5, Float64}}
+    b_im::NTuple{5, Float64}
+    c_im::NTuple{5, Float64}
 end
 
 checkrc(rc, ctx) = rc == 0 || error("libhcc: ", unsafe_string(ccall((:hcc_last_error, libhcc), Cstring, (Ptr{Cvoid},), ctx)))
@@ -199,6 +202,14 @@ function hip_solve_equation(ts::HIPRKTimeStepper, tmax::Real, dt::Real; is_cfl::
     return xs, us, th, Int(steps[]), el[], ms
 end
 
+"Decay source `S_i(U) = -rates[i] U` (mode 1 explicit, 2 implicit); rates evaluated on the host (any Julia closure)."
+function hip_set_source!(ts::HIPRKTimeStepper, mode::Integer, rates::AbstractVector{Float64})
+    r = collect(rates)
+    checkrc(ccall((:hcc_set_source, libhcc), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), ts.st, Int32(mode), r), ts.ctx)
+    return ts
+end
+
+export hip_set_source!
 export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
        hip_volumes, hip_total_mass, hip_solve_equation
 

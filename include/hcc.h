@@ -101,6 +101,9 @@ enum { HCC_CRIT_NONE = 0, HCC_CRIT_U1 = 1, HCC_CRIT_PAD = 2, HCC_CRIT_ONLY = 3, 
 enum { HCC_BC_NONE = 0, HCC_BC_OUTFLOW = 1, HCC_BC_DIRICHLET = 2 };
 /* numerical flux (src/Interpolation/FluxFunctions.jl:66-111): RusanovFlux, AlignedRusanovFlux */
 enum { HCC_FLUX_RUSANOV = 0, HCC_FLUX_ALIGNED_RUSANOV = 1 };
+/* source terms (src/Types.jl:148-172): the built-in decay S_i(U) = -rate_i U (the reference tests' DecaySource,
+ * test/unit_schemes.jl:279-284), added to K_E (explicit) or solved in the diagonally implicit stages (implicit) */
+enum { HCC_SRC_NONE = 0, HCC_SRC_DECAY_EXPLICIT = 1, HCC_SRC_DECAY_IMPLICIT = 2 };
 
 typedef struct {
     int32_t M;                 /* state components */
@@ -119,6 +122,9 @@ typedef struct {
     double mood_plateau;       /* MOODu2 plateau (fraction of the global range of the previous stage; 0: off) */
     int32_t muscl_streamed;    /* MUSCL(...; streamed = true): divergence accumulated per slot (MUSCL.jl:56, 407-408) */
     int32_t flux;              /* HCC_FLUX_* */
+    /* implicit part of an IMEX tableau (IMEXButcherTableau a, b, c, ButcherTableaus.jl:23-45): lower triangular
+     * a_im; all zero for an explicit tableau. substep_ratio must be 0 for a tableau with an implicit part. */
+    double a_im[5][5], b_im[5], c_im[5];
 } hcc_stepper_desc;
 
 /* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL
@@ -161,6 +167,13 @@ int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, d
  * driver time a run of steps without a host round trip per step (benchmark/step.jl times its loop as a whole). */
 int hcc_step_totals(hcc_stepper* st, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
                     double* ms_gather, double* ms_flux);
+
+/* The source terms of GeneralRKTimeStepper(pde, interp, mood, explicit_sources, implicit_sources, tableau)
+ * (TimeSteppers.jl:195-247): the built-in decay S_i(U) = -rates[i] U with per-particle rates (N doubles, read once),
+ * mode HCC_SRC_*. An implicit source with a purely explicit tableau is applied explicitly with the explicit weights
+ * (_explicit_implicit_part). No counterpart for arbitrary Julia source closures: their rates are evaluated by the
+ * host. */
+int hcc_set_source(hcc_stepper* st, int32_t mode, const double* rates);
 
 /* ---- particle volumes, conservation diagnostics, driver loop ------------------------------------------- */
 /* Replaces update_volumes!(pg, ShepardVolumes(sweeps)) (src/Grid/Volumes.jl:183-231), which the ParticleGrid

@@ -457,7 +457,9 @@ struct Scheme {
     int aligned = 0;           // AlignedRusanovFlux: sort by sign(dweights) on pairs below the full order
     int s = 3;
     double a_t[5][5] = {}, b_t[5] = {}, c_t[5] = {};
+    double a_im[5][5] = {}, b_im[5] = {}, c_im[5] = {};   // implicit part of an IMEX tableau (zero: explicit)
     double substep_ratio = 0.0;
+    int src_mode = 0;          // decay source S_i(U) = -rate_i U: 0 none, 1 explicit, 2 implicit
     int n_bc = 0;
     int bc_tag[16] = {}, bc_kind[16] = {};
 };
@@ -494,6 +496,8 @@ struct SimBase {
     virtual void expand_of(uint8_t* mask, int hops) = 0;
     virtual int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) = 0;
     virtual void volumes_of(int sweeps, double* out) = 0;
+    virtual void set_source(int mode, const double* rates) = 0;
+    virtual void get_stage(int which, int i, double* out) const = 0;
     virtual void mass_of(int interior_only, double* out) const = 0;
 };
 
@@ -533,7 +537,8 @@ struct Sim : SimBase {
     PDE pde;
     Scheme sc;
     std::vector<State> rhos, rho_n, rho_cand, rho_sub;
-    std::vector<State> Y[5], K[5];
+    std::vector<State> Y[5], K[5], KI[5];
+    std::vector<double> src_rate;   // per-particle decay rates of the source
     std::vector<std::array<State, BG>> grads;
     std::vector<int32_t> orders, snapshot_orders;
     std::vector<uint8_t> triggered, changed, dirty, scratch, triggered_step;
@@ -603,6 +608,22 @@ struct Sim : SimBase {
         if (out) std::copy(vol.begin(), vol.end(), out);
     }
     void mass_of(int interior_only, double* out) const override { mass(interior_only, out); }
+    // stage buffers of the last step: which 0 Y_i, 1 K_E[i], 2 K_I[i] (i 0-based)
+    void get_stage(int which, int i, double* out) const override {
+        const std::vector<State>& b = which == 0 ? Y[i] : which == 1 ? K[i] : KI[i];
+        for (int64_t p = 0; p < N; ++p) for (int m = 0; m < M; ++m) out[p * M + m] = b[p][m];
+    }
+    // the decay source (mode 1 explicit, 2 implicit) with per-particle rates; an implicit source with a purely
+    // explicit tableau takes the explicit weights as its implicit part (_explicit_implicit_part, TimeSteppers.jl:238-247)
+    void set_source(int mode, const double* rates) override {
+        sc.src_mode = mode;
+        src_rate.assign(N, 0.0);
+        if (rates) std::copy(rates, rates + N, src_rate.begin());
+        bool zero = true;
+        for (int i = 0; i < sc.s; ++i) { zero = zero && sc.b_im[i] == 0.0; for (int j = 0; j < sc.s; ++j) zero = zero && sc.a_im[i][j] == 0.0; }
+        if (mode == 2 && zero)
+            for (int i = 0; i < sc.s; ++i) { sc.b_im[i] = sc.b_t[i]; sc.c_im[i] = sc.c_t[i]; for (int j = 0; j < sc.s; ++j) sc.a_im[i][j] = sc.a_t[i][j]; }
+    }
 
     void set_threads(int n) override { threads = n < 1 ? 1 : n; }
 
@@ -761,7 +782,8 @@ struct Sim : SimBase {
     void set_scheme(const PDE& e, const Scheme& s) override {
         pde = e; sc = s;
         rhos.assign(N, State{}); rho_n.assign(N, State{}); rho_cand.assign(N, State{}); rho_sub.assign(N, State{});
-        for (int i = 0; i < 5; ++i) { Y[i].assign(N, State{}); K[i].assign(N, State{}); }
+        for (int i = 0; i < 5; ++i) { Y[i].assign(N, State{}); K[i].assign(N, State{}); KI[i].assign(N, State{}); }
+        src_rate.assign(N, 0.0);
         grads.assign(N, {});
         orders.assign(N, 2);
         triggered.assign(N, 0); changed.assign(N, 0); dirty.assign(N, 0); scratch.assign(N, 0); triggered_step.assign(N, 0);
@@ -879,7 +901,7 @@ struct Sim : SimBase {
     // (Interpolators.jl:610-626); returns K_E = -2 div + 0 (TimeSteppers.jl:354)
     void flux_node(int64_t i, const std::vector<State>& U, State& KE) {
         int64_t k0 = rstart[i], k1 = rstart[i + 1];
-        if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + 0.0; return; }
+        if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + source(i, U[i], m); return; }
         const State& fi = U[i];
         double Fi[D][M]; P::flux(pde, fi.data(), Fi);
         const auto& gi = grads[i];
@@ -952,8 +974,11 @@ struct Sim : SimBase {
         } else {
             for (int m = 0; m < M; ++m) div[m] = 0.0;   // singular stencil: compute_dynamic_divergence returns zero
         }
-        for (int m = 0; m < M; ++m) KE[m] = -(2.0 * div[m]) + 0.0;
+        for (int m = 0; m < M; ++m) KE[m] = -(2.0 * div[m]) + source(i, U[i], m);
     }
+    // evaluate_sources of the explicit sources (TimeSteppers.jl:354): zero(U) without one; the decay source
+    // evaluate_source = -rate_i * U (the reference tests' DecaySource, unit_schemes.jl:279-284)
+    double source(int64_t i, const State& u, int m) const { return sc.src_mode == 1 ? (-src_rate[i]) * u[m] : 0.0; }
 
     // compute_explicit_rhs! (TimeSteppers.jl:312-356)
     void compute_rhs(std::vector<State>& KE, const std::vector<State>& Yv, bool full) {
@@ -1000,7 +1025,11 @@ struct Sim : SimBase {
         }
     }
 
-    // form_candidate! (TimeSteppers.jl:365-421), explicit tableau, no sources
+    // form_candidate! (TimeSteppers.jl:365-397) with _stage_predictor! (:400-421): Y = U^n + sum_j (dt a~_ij) K_E[j]
+    // + (dt a_ij) K_I[j] (each term only for a non-zero weight); for a stage the implicit step (:427-444): with
+    // |a_ii| > 1e-14 the solve U_out = U_in / (1 + (dt a_ii) rate) of the decay source and K_I = (U_out - U_in) *
+    // inv(dt a_ii) (one source), else K_I = evaluate_sources(Y) (zero(U) without an implicit source); boundary
+    // particles K_I = 0; then the boundary conditions.
     std::vector<State>& form_candidate(int i, double dt) {
         bool fin = i > sc.s;
         std::vector<State>& target = fin ? rho_cand : Y[i - 1];
@@ -1011,12 +1040,35 @@ struct Sim : SimBase {
             State y = rho_n[p];
             for (int j = 1; j <= n_prev; ++j) {
                 double at = fin ? sc.b_t[j - 1] : sc.a_t[i - 1][j - 1];
+                double ai = fin ? sc.b_im[j - 1] : sc.a_im[i - 1][j - 1];
                 if (at != 0.0) {
                     double c = dt * at;
                     for (int m = 0; m < M; ++m) y[m] = y[m] + c * K[j - 1][p][m];
                 }
+                if (ai != 0.0) {
+                    double c = dt * ai;
+                    for (int m = 0; m < M; ++m) y[m] = y[m] + c * KI[j - 1][p][m];
+                }
             }
             target[p] = y;
+        }
+        if (!fin) {
+            const double aii = sc.a_im[i - 1][i - 1];
+            const bool solve = std::fabs(aii) > 1e-14;
+            const double g = dt * aii, inv = 1.0 / (dt * aii);
+            for (int64_t p = 0; p < N; ++p) {
+                State& ki = KI[i - 1][p];
+                if (isb[p]) { ki = State{}; continue; }
+                if (sc.src_mode != 2) { ki = State{}; continue; }
+                if (solve) {
+                    const State uin = target[p];
+                    const double den = 1.0 + g * src_rate[p];
+                    for (int m = 0; m < M; ++m) target[p][m] = uin[m] / den;
+                    for (int m = 0; m < M; ++m) ki[m] = (target[p][m] - uin[m]) * inv;
+                } else {
+                    for (int m = 0; m < M; ++m) ki[m] = (-src_rate[p]) * target[p][m];
+                }
+            }
         }
         apply_bcs(target);
         return target;
@@ -1133,9 +1185,11 @@ struct Sim : SimBase {
             const std::vector<State>* cand = &target;
             if (checks_substeps()) {
                 double h = sc.substep_ratio * dt;
-                for (int64_t p = 0; p < N; ++p) {
+                const std::vector<State>& KIp = KI[i - 2];
+                for (int64_t p = 0; p < N; ++p) {   // _substep_body!: with implicit sources Y + h (K + K_I)
                     if (isb[p]) { rho_sub[p] = Yp[p]; continue; }
-                    for (int m = 0; m < M; ++m) rho_sub[p][m] = Yp[p][m] + h * Kp[p][m];
+                    for (int m = 0; m < M; ++m)
+                        rho_sub[p][m] = sc.src_mode == 2 ? Yp[p][m] + h * (Kp[p][m] + KIp[p][m]) : Yp[p][m] + h * Kp[p][m];
                 }
                 cand = &rho_sub;
             }
@@ -1319,7 +1373,8 @@ void orc_sim_set_threads(void* s, int n) { ((SimBase*)s)->set_threads(n); }
 void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, int limiter, int mode, int strategy,
                         int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
                         double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds,
-                        double plateau, int streamed, int aligned) {
+                        double plateau, int streamed, int aligned, const double* a_im, const double* b_im,
+                        const double* c_im) {
     SimBase* sb = (SimBase*)s;
     PDE e; e.kind = pde_kind; e.gamma = gamma;
     if (pde_kind == PDE_LINADV) {
@@ -1332,6 +1387,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
     Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
     sc.plateau = plateau;
     sc.streamed = streamed; sc.aligned = aligned;
+    if (a_im) for (int i = 0; i < stages; ++i) { sc.b_im[i] = b_im[i]; sc.c_im[i] = c_im[i]; for (int j = 0; j < stages; ++j) sc.a_im[i][j] = a_im[i * stages + j]; }
     sc.s = stages;
     for (int i = 0; i < stages; ++i) { sc.b_t[i] = b_t[i]; sc.c_t[i] = c_t[i]; for (int j = 0; j < stages; ++j) sc.a_t[i][j] = a_t[i * stages + j]; }
     sc.substep_ratio = substep_ratio;
@@ -1365,6 +1421,8 @@ void orc_sim_expand(void* s, uint8_t* mask, int hops) { ((SimBase*)s)->expand_of
 // sweeps < 0: read the current volumes
 void orc_sim_volumes(void* s, int sweeps, double* out) { ((SimBase*)s)->volumes_of(sweeps, out); }
 void orc_sim_mass(void* s, int interior_only, double* out) { ((SimBase*)s)->mass_of(interior_only, out); }
+void orc_sim_set_source(void* s, int mode, const double* rates) { ((SimBase*)s)->set_source(mode, rates); }
+void orc_sim_get_stage(void* s, int which, int i, double* out) { ((SimBase*)s)->get_stage(which, i, out); }
 int orc_sim_u2_flags(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau) {
     return ((SimBase*)s)->u2_flags_of(U, p, cand, delta, plateau);
 }

@@ -27,7 +27,7 @@ def lib():
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
-                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int]),
+                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int, vp, vp, vp]),
             "orc_sim_set_state": (None, [vp, vp]),
             "orc_sim_get_state": (None, [vp, vp]),
             "orc_sim_time_step": (d, [vp]),
@@ -49,6 +49,8 @@ def lib():
             "orc_sim_u2_flags": (C.c_int, [vp, vp, i64, vp, d, d]),
             "orc_sim_volumes": (None, [vp, C.c_int, vp]),
             "orc_sim_mass": (None, [vp, C.c_int, vp]),
+            "orc_sim_set_source": (None, [vp, C.c_int, vp]),
+            "orc_sim_get_stage": (None, [vp, C.c_int, C.c_int, vp]),
             "orc_weight": (d, [d, d, d]),
             "orc_limiter_phi": (d, [C.c_int, d]),
             "orc_jmax": (d, [d, d]),
@@ -126,18 +128,22 @@ class OracleSim:
             for j in range(s):
                 a_t[i, j] = sch["a_t"][i][j]
         b_t, c_t = np.array(sch["b_t"], float), np.array(sch["c_t"], float)
+        a_im = np.array(sch.get("a_im") or np.zeros((s, s)), float).reshape(s, s).copy()
+        b_im = np.array(sch.get("b_im") or np.zeros(s), float)
+        c_im = np.array(sch.get("c_im") or np.zeros(s), float)
         c = np.zeros((3, 8))
         for d, row in enumerate(sch.get("adv_speed", [])):
             c[d, :len(row)] = row
         bcs = sch.get("bcs", [])
         tags = np.array([t for t, _ in bcs] or [0], np.int32)
         kinds = np.array([k for _, k in bcs] or [0], np.int32)
-        self._keep += [a_t, b_t, c_t, c, tags, kinds]
+        self._keep += [a_t, b_t, c_t, c, tags, kinds, a_im, b_im, c_im]
         lib().orc_sim_set_scheme(self.h, sch["pde"], sch.get("gamma", 1.4), _p(c), sch["limiter"],
                                  sch.get("limiter_mode", 1), sch["strategy"], sch["criterion"], sch.get("delta", 0.0), s,
                                  _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds),
                                  sch.get("plateau", 0.0), int(bool(sch.get("streamed", False))),
-                                 int(sch.get("flux", FLUX_RUSANOV) == FLUX_ALIGNED_RUSANOV))
+                                 int(sch.get("flux", FLUX_RUSANOV) == FLUX_ALIGNED_RUSANOV), _p(a_im), _p(b_im),
+                                 _p(c_im))
 
     def set_threads(self, n):
         lib().orc_sim_set_threads(self.h, int(n))
@@ -242,6 +248,18 @@ class OracleSim:
         U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)
         c = np.ascontiguousarray(cand, np.float64)
         return bool(lib().orc_sim_u2_flags(self.h, _p(U), p, _p(c), delta, plateau))
+
+    def set_source(self, mode, rates):
+        """The decay source S_i(U) = -rates_i U: mode 1 explicit, 2 implicit, 0 none."""
+        r = np.ascontiguousarray(rates, np.float64)
+        self._keep.append(r)
+        lib().orc_sim_set_source(self.h, int(mode), _p(r))
+
+    def stage(self, which, i):
+        """Stage buffer of the last step: which 'Y', 'K_E' or 'K_I', stage i (0-based)."""
+        out = np.zeros((self.N, self.M))
+        lib().orc_sim_get_stage(self.h, {"Y": 0, "K_E": 1, "K_I": 2}[which], int(i), _p(out))
+        return out
 
     def volumes(self, sweeps=-1):
         """The particle volumes (ShepardVolumes); sweeps >= 0 recomputes them with that many sweeps first."""
