@@ -19,6 +19,7 @@ HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1, HCC_STRAT_EPD2, HCC_STRAT_STRICT
 HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY, HCC_CRIT_U2 = 0, 1, 2, 3, 4
 HCC_FLUX_RUSANOV, HCC_FLUX_ALIGNED_RUSANOV = 0, 1
 HCC_SRC_NONE, HCC_SRC_DECAY_EXPLICIT, HCC_SRC_DECAY_IMPLICIT = 0, 1, 2
+HCC_MOVER_NONE, HCC_MOVER_LAGRANGIAN, HCC_MOVER_SCALED = 0, 1, 2
 HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
 HCC_E_CAPACITY = -4
 

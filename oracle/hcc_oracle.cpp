@@ -422,6 +422,33 @@ struct Phys {
             for (int d = 0; d < D; ++d) lam[d] = std::fabs(u[d]) + cs;
         }
     }
+    // fluid_velocity (API.jl:104-109): scalar advection its velocity (test/PDEs/linear_advection.jl:41-43), Euler
+    // the primitive velocity m / rho with the operations of the flux
+    static inline void fluid_velocity(const PDE& e, const double* U, double (&w)[D]) {
+        if (e.kind == PDE_LINADV || M != D + 2) {
+            for (int d = 0; d < D; ++d) w[d] = e.c[d][0];
+        } else {
+            double inv = 1.0 / U[0];
+            for (int d = 0; d < D; ++d) w[d] = U[1 + d] * inv;
+        }
+    }
+    // max_eigenvalue(eq, U, d, w_d) of the ALE frame (GridMovers.jl:80-90): |c_d - w_d| for scalar advection
+    // (linear_advection.jl:33), |u_d - w_d| + c for Euler
+    static inline void maxeig_shift(const PDE& e, const double* U, const double* w, double (&lam)[D]) {
+        if (e.kind == PDE_LINADV || M != D + 2) {
+            for (int d = 0; d < D; ++d) lam[d] = std::fabs(e.c[d][0] - w[d]);
+        } else if constexpr (M == D + 2) {
+            double inv = 1.0 / U[0];
+            double u[D];
+            for (int d = 0; d < D; ++d) u[d] = U[1 + d] * inv;
+            double ke = U[1] * u[0];
+            for (int d = 1; d < D; ++d) ke = ke + U[1 + d] * u[d];
+            ke = 0.5 * ke;
+            double p = (e.gamma - 1.0) * (U[D + 1] - ke);
+            double cs = std::sqrt((e.gamma * p) * inv);
+            for (int d = 0; d < D; ++d) lam[d] = std::fabs(u[d] - w[d]) + cs;
+        }
+    }
     static inline bool admissible(const PDE& e, const double* U) {
         for (int m = 0; m < M; ++m) if (!std::isfinite(U[m])) return false;
         if constexpr (M == D + 2) if (e.kind == PDE_EULER) {
@@ -460,6 +487,8 @@ struct Scheme {
     double a_im[5][5] = {}, b_im[5] = {}, c_im[5] = {};   // implicit part of an IMEX tableau (zero: explicit)
     double substep_ratio = 0.0;
     int src_mode = 0;          // decay source S_i(U) = -rate_i U: 0 none, 1 explicit, 2 implicit
+    int mover = 0;             // 0 NoGridMover, 1 LagrangianMover, 2 ScaledLagrangianMover(mover_alpha)
+    double mover_alpha = 1.0;
     int n_bc = 0;
     int bc_tag[16] = {}, bc_kind[16] = {};
 };
@@ -498,6 +527,8 @@ struct SimBase {
     virtual void volumes_of(int sweeps, double* out) = 0;
     virtual void set_source(int mode, const double* rates) = 0;
     virtual void get_stage(int which, int i, double* out) const = 0;
+    virtual void get_positions(double* out) const = 0;
+    virtual void set_positions(const double* x, int full_search) = 0;
     virtual void mass_of(int interior_only, double* out) const = 0;
 };
 
@@ -539,6 +570,8 @@ struct Sim : SimBase {
     std::vector<State> rhos, rho_n, rho_cand, rho_sub;
     std::vector<State> Y[5], K[5], KI[5];
     std::vector<double> src_rate;   // per-particle decay rates of the source
+    // moving grids: positions at the start of the step, of the stages and of the candidate; grid velocities K_X
+    std::vector<std::array<double, D>> x_n, x_cand, Xs[5], KX[5];
     std::vector<std::array<State, BG>> grads;
     std::vector<int32_t> orders, snapshot_orders;
     std::vector<uint8_t> triggered, changed, dirty, scratch, triggered_step;
@@ -608,6 +641,16 @@ struct Sim : SimBase {
         if (out) std::copy(vol.begin(), vol.end(), out);
     }
     void mass_of(int interior_only, double* out) const override { mass(interior_only, out); }
+    void get_positions(double* out) const override {
+        for (int64_t p = 0; p < N; ++p) for (int d = 0; d < D; ++d) out[p * D + d] = pos[p][d];
+    }
+    // set_positions!(pg, X, full_search) (GridMovers.jl:123-140): the frozen-topology refresh, or a full search with
+    // the periodic wrap followed by the geometry caches
+    void set_positions(const double* x, int full_search) override {
+        for (int64_t p = 0; p < N; ++p) for (int d = 0; d < D; ++d) pos[p][d] = x[p * D + d];
+        if (full_search) { search(); prepare_geometry(); dflux.assign(nbr.size(), {}); }
+        else refresh_geometry();
+    }
     // stage buffers of the last step: which 0 Y_i, 1 K_E[i], 2 K_I[i] (i 0-based)
     void get_stage(int which, int i, double* out) const override {
         const std::vector<State>& b = which == 0 ? Y[i] : which == 1 ? K[i] : KI[i];
@@ -782,7 +825,11 @@ struct Sim : SimBase {
     void set_scheme(const PDE& e, const Scheme& s) override {
         pde = e; sc = s;
         rhos.assign(N, State{}); rho_n.assign(N, State{}); rho_cand.assign(N, State{}); rho_sub.assign(N, State{});
-        for (int i = 0; i < 5; ++i) { Y[i].assign(N, State{}); K[i].assign(N, State{}); KI[i].assign(N, State{}); }
+        for (int i = 0; i < 5; ++i) {
+            Y[i].assign(N, State{}); K[i].assign(N, State{}); KI[i].assign(N, State{});
+            Xs[i].assign(N, {}); KX[i].assign(N, {});
+        }
+        x_n.assign(N, {}); x_cand.assign(N, {});
         src_rate.assign(N, 0.0);
         grads.assign(N, {});
         orders.assign(N, 2);
@@ -904,6 +951,16 @@ struct Sim : SimBase {
         if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + source(i, U[i], m); return; }
         const State& fi = U[i];
         double Fi[D][M]; P::flux(pde, fi.data(), Fi);
+        // a moving grid: the PDE in the frame of the particle's own grid velocity (ALEFrame, GridMovers.jl:65-90),
+        // flux F - w_i U and wave speeds |lambda - w_i|, for both interface states
+        double wv[D] = {};
+        const bool ale = sc.mover != 0;
+        if (ale) {
+            grid_velocity(fi, wv);
+            KXcur[i] = {};
+            for (int d = 0; d < D; ++d) KXcur[i][d] = wv[d];
+            for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m) Fi[d][m] = Fi[d][m] - wv[d] * fi[m];
+        }
         const auto& gi = grads[i];
         for (int64_t k = k0; k < k1; ++k) {
             int64_t j = nbr[k];
@@ -924,7 +981,13 @@ struct Sim : SimBase {
             double Fij[D][M], Fji[D][M];
             P::flux(pde, fij.data(), Fij); P::flux(pde, fji.data(), Fji);
             double lij[D], lji[D];
-            P::maxeig(pde, fij.data(), lij); P::maxeig(pde, fji.data(), lji);
+            if (ale) {
+                for (int d = 0; d < D; ++d)
+                    for (int m = 0; m < M; ++m) { Fij[d][m] = Fij[d][m] - wv[d] * fij[m]; Fji[d][m] = Fji[d][m] - wv[d] * fji[m]; }
+                P::maxeig_shift(pde, fij.data(), wv, lij); P::maxeig_shift(pde, fji.data(), wv, lji);
+            } else {
+                P::maxeig(pde, fij.data(), lij); P::maxeig(pde, fji.data(), lji);
+            }
             const bool below = orders[i] < 2 || orders[j] < 2;
             for (int d = 0; d < D; ++d) {
                 // sort_flux (Utils.jl:95-103) by _dissipation_direction: sign(w_k) > 0 is w_k > 0
@@ -976,12 +1039,19 @@ struct Sim : SimBase {
         }
         for (int m = 0; m < M; ++m) KE[m] = -(2.0 * div[m]) + source(i, U[i], m);
     }
+    // grid_velocity (GridMovers.jl:52-63): LagrangianMover fluid_velocity(eq, U), ScaledLagrangianMover alpha times it
+    void grid_velocity(const State& u, double (&w)[D]) const {
+        P::fluid_velocity(pde, u.data(), w);
+        if (sc.mover == 2) for (int d = 0; d < D; ++d) w[d] = sc.mover_alpha * w[d];
+    }
+    std::vector<std::array<double, D>> KXcur;   // K_X of the right-hand side being computed
     // evaluate_sources of the explicit sources (TimeSteppers.jl:354): zero(U) without one; the decay source
     // evaluate_source = -rate_i * U (the reference tests' DecaySource, unit_schemes.jl:279-284)
     double source(int64_t i, const State& u, int m) const { return sc.src_mode == 1 ? (-src_rate[i]) * u[m] : 0.0; }
 
     // compute_explicit_rhs! (TimeSteppers.jl:312-356)
-    void compute_rhs(std::vector<State>& KE, const std::vector<State>& Yv, bool full) {
+    void compute_rhs(std::vector<State>& KE, const std::vector<State>& Yv, bool full, int stage = -1) {
+        KXcur.swap(KX[stage < 0 ? 0 : stage]);   // _rhs_flux! writes K_X[p] = w (boundary particles: 0)
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int64_t p = 0; p < N; ++p) {
             if (isb[p] || !(full || changed[p])) continue;
@@ -989,10 +1059,24 @@ struct Sim : SimBase {
         }
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int64_t p = 0; p < N; ++p) {
-            if (isb[p]) { if (full) KE[p] = State{}; continue; }
+            if (isb[p]) { if (full) { KE[p] = State{}; if (sc.mover) KXcur[p] = {}; } continue; }
             if (!(full || dirty[p])) continue;
             flux_node(p, Yv, KE[p]);
         }
+        KXcur.swap(KX[stage < 0 ? 0 : stage]);
+    }
+    // refresh_geometry! (GridMovers.jl:98-121): distances and weights of the existing slots from the current
+    // positions, weight 0 beyond R; then the geometry caches (prepare_geometry!)
+    void refresh_geometry() {
+        for (int64_t i = 0; i < N; ++i)
+            for (int64_t k = rstart[i]; k < rstart[i + 1]; ++k) {
+                std::array<double, D> dd;
+                get_distance(i, nbr[k], dd);
+                const double d2 = sumsq(dd);
+                dist[k] = dd;
+                wts[k] = d2 <= R_sq ? weight_fn(alpha, inv_range_sq, d2) : 0.0;
+            }
+        prepare_geometry();
     }
 
     // OutflowBC (BoundaryConditions.jl:82-131)
@@ -1034,6 +1118,21 @@ struct Sim : SimBase {
         bool fin = i > sc.s;
         std::vector<State>& target = fin ? rho_cand : Y[i - 1];
         int n_prev = fin ? sc.s : i - 1;
+        if (sc.mover) {   // the stage positions X = x^n + sum_j (dt a~_ij) K_X[j] (boundary particles stay at x^n)
+            std::vector<std::array<double, D>>& xt = fin ? x_cand : Xs[i - 1];
+            for (int64_t p = 0; p < N; ++p) {
+                std::array<double, D> x = x_n[p];
+                if (!isb[p])
+                    for (int j = 1; j <= n_prev; ++j) {
+                        double at = fin ? sc.b_t[j - 1] : sc.a_t[i - 1][j - 1];
+                        if (at != 0.0) {
+                            double c = dt * at;
+                            for (int d = 0; d < D; ++d) x[d] = x[d] + c * KX[j - 1][p][d];
+                        }
+                    }
+                xt[p] = x;
+            }
+        }
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int64_t p = 0; p < N; ++p) {
             if (isb[p]) { target[p] = rhos[p]; continue; }
@@ -1201,7 +1300,7 @@ struct Sim : SimBase {
             ++n_mood_iter;
             dirty = changed;
             for (int hop = 0; hop < order_reach(); ++hop) expand_dirty();
-            compute_rhs(Kp, Yp, false);
+            compute_rhs(Kp, Yp, false, i - 2);
             full = false;
         }
     }
@@ -1211,6 +1310,7 @@ struct Sim : SimBase {
         (void)t;
         for (auto& gr : grads) gr = {};
         rho_n = rhos;
+        if (sc.mover) x_n = pos;
         std::fill(orders.begin(), orders.end(), 2);
         std::fill(triggered_step.begin(), triggered_step.end(), 0);
         n_inadmissible = 0;
@@ -1218,10 +1318,23 @@ struct Sim : SimBase {
         for (int i = 1; i <= sc.s; ++i) {
             if (i == 1) form_candidate(1, dt);
             else mood_checkpoint(i, dt);
-            compute_rhs(K[i - 1], Y[i - 1], true);
+            if (i > 1 && sc.mover) apply_stage_geometry(Xs[i - 1]);   // _apply_stage_geometry! (TimeSteppers.jl:520-524)
+            compute_rhs(K[i - 1], Y[i - 1], true, i - 1);
         }
         mood_checkpoint(sc.s + 1, dt);
         rhos = rho_cand;
+        if (sc.mover) {   // final positions, a full search with the periodic wrap, geometry caches, volumes
+            pos = x_cand;
+            search();
+            prepare_geometry();
+            update_volumes(1);
+            dflux.assign(nbr.size(), {});
+        }
+    }
+    // set_positions!(pg, X, false) (GridMovers.jl:123-133): the stage positions with the frozen topology
+    void apply_stage_geometry(const std::vector<std::array<double, D>>& X) {
+        pos = X;
+        refresh_geometry();
     }
 
     void interface_orders_of(int strategy, const int32_t* o, int64_t i, int64_t j, int32_t* out) override {
@@ -1374,7 +1487,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
                         int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
                         double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds,
                         double plateau, int streamed, int aligned, const double* a_im, const double* b_im,
-                        const double* c_im) {
+                        const double* c_im, int mover, double mover_alpha) {
     SimBase* sb = (SimBase*)s;
     PDE e; e.kind = pde_kind; e.gamma = gamma;
     if (pde_kind == PDE_LINADV) {
@@ -1387,6 +1500,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
     Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
     sc.plateau = plateau;
     sc.streamed = streamed; sc.aligned = aligned;
+    sc.mover = mover; sc.mover_alpha = mover_alpha;
     if (a_im) for (int i = 0; i < stages; ++i) { sc.b_im[i] = b_im[i]; sc.c_im[i] = c_im[i]; for (int j = 0; j < stages; ++j) sc.a_im[i][j] = a_im[i * stages + j]; }
     sc.s = stages;
     for (int i = 0; i < stages; ++i) { sc.b_t[i] = b_t[i]; sc.c_t[i] = c_t[i]; for (int j = 0; j < stages; ++j) sc.a_t[i][j] = a_t[i * stages + j]; }
@@ -1423,6 +1537,8 @@ void orc_sim_volumes(void* s, int sweeps, double* out) { ((SimBase*)s)->volumes_
 void orc_sim_mass(void* s, int interior_only, double* out) { ((SimBase*)s)->mass_of(interior_only, out); }
 void orc_sim_set_source(void* s, int mode, const double* rates) { ((SimBase*)s)->set_source(mode, rates); }
 void orc_sim_get_stage(void* s, int which, int i, double* out) { ((SimBase*)s)->get_stage(which, i, out); }
+void orc_sim_get_positions(void* s, double* out) { ((SimBase*)s)->get_positions(out); }
+void orc_sim_set_positions(void* s, const double* x, int full_search) { ((SimBase*)s)->set_positions(x, full_search); }
 int orc_sim_u2_flags(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau) {
     return ((SimBase*)s)->u2_flags_of(U, p, cand, delta, plateau);
 }

@@ -27,7 +27,7 @@ def lib():
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
-                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int, vp, vp, vp]),
+                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int, vp, vp, vp, C.c_int, d]),
             "orc_sim_set_state": (None, [vp, vp]),
             "orc_sim_get_state": (None, [vp, vp]),
             "orc_sim_time_step": (d, [vp]),
@@ -51,6 +51,8 @@ def lib():
             "orc_sim_mass": (None, [vp, C.c_int, vp]),
             "orc_sim_set_source": (None, [vp, C.c_int, vp]),
             "orc_sim_get_stage": (None, [vp, C.c_int, C.c_int, vp]),
+            "orc_sim_get_positions": (None, [vp, vp]),
+            "orc_sim_set_positions": (None, [vp, vp, C.c_int]),
             "orc_weight": (d, [d, d, d]),
             "orc_limiter_phi": (d, [C.c_int, d]),
             "orc_jmax": (d, [d, d]),
@@ -143,7 +145,7 @@ class OracleSim:
                                  _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds),
                                  sch.get("plateau", 0.0), int(bool(sch.get("streamed", False))),
                                  int(sch.get("flux", FLUX_RUSANOV) == FLUX_ALIGNED_RUSANOV), _p(a_im), _p(b_im),
-                                 _p(c_im))
+                                 _p(c_im), int(sch.get("mover", 0)), float(sch.get("mover_alpha", 1.0)))
 
     def set_threads(self, n):
         lib().orc_sim_set_threads(self.h, int(n))
@@ -260,6 +262,16 @@ class OracleSim:
         out = np.zeros((self.N, self.M))
         lib().orc_sim_get_stage(self.h, {"Y": 0, "K_E": 1, "K_I": 2}[which], int(i), _p(out))
         return out
+
+    def positions(self):
+        out = np.zeros((self.N, self.D))
+        lib().orc_sim_get_positions(self.h, _p(out))
+        return out
+
+    def set_positions(self, x, full_search=False):
+        """set_positions!(pg, X, full_search): frozen-topology refresh of distances and weights, or a full search."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.N, self.D)
+        lib().orc_sim_set_positions(self.h, _p(x), int(bool(full_search)))
 
     def volumes(self, sweeps=-1):
         """The particle volumes (ShepardVolumes); sweeps >= 0 recomputes them with that many sweeps first."""
