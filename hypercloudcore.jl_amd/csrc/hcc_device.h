@@ -75,6 +75,11 @@ __device__ __forceinline__ double slot_geom(const GeomArgs& g, const NodePos<D>&
     return s;
 }
 __device__ __forceinline__ int64_t sell_base(const GeomArgs& g, int64_t i) { return g.chunk_off[i >> 6] + (i & 63); }
+// the weight of a slot at squared distance d2: wf(d2), or 0 beyond R_cut (weights[k] = d2 <= R_sq ? wf(d2) : 0)
+__device__ __forceinline__ double slot_weight(const GeomArgs& g, double d2) {
+    const double w = mls_weight(g.alpha, g.inv_range_sq, d2);
+    return d2 <= g.R_cut ? w : 0.0;
+}
 
 // ============================================================================================================
 // PDE kernels variants: KIND 1 = linear advection (diagonal), KIND 2 = Euler (M = D + 2)
@@ -118,6 +123,34 @@ struct Pde {
             maxeig(e, U, lam);
         }
     }
+    // grid_velocity (GridMovers.jl:52-63): the fluid velocity (API.jl:104-109: the advection speed, or Euler's m / rho
+    // with the operations of the flux), scaled by alpha for ScaledLagrangianMover
+    __device__ static __forceinline__ void grid_velocity(const PdeArgs& e, const double (&U)[M], int mover, double alpha,
+                                                         double (&w)[D]) {
+        if constexpr (KIND == HCC_PDE_EULER) {
+            const double inv = 1.0 / U[0];
+            for (int d = 0; d < D; ++d) w[d] = U[1 + d] * inv;
+        } else {
+            for (int d = 0; d < D; ++d) w[d] = e.c[d][0];
+        }
+        if (mover == HCC_MOVER_SCALED_LAGRANGIAN)
+            for (int d = 0; d < D; ++d) w[d] = alpha * w[d];
+    }
+    // flux and wave speeds of one state in the frame moving with w (ALEFrame, GridMovers.jl:65-90): F - w U and
+    // |u_d - w_d| + c for Euler, |c_d - w_d| for scalar advection
+    __device__ static __forceinline__ void flux_eig_ale(const PdeArgs& e, const double (&U)[M], const double (&w)[D],
+                                                        double (&F)[D][M], double (&lam)[D]) {
+        flux(e, U, F);
+        for (int d = 0; d < D; ++d)
+            for (int m = 0; m < M; ++m) F[d][m] = F[d][m] - w[d] * U[m];
+        if constexpr (KIND == HCC_PDE_EULER) {
+            EulerPrim<D> q = euler_prim<D>(e.gamma, U);
+            double cs = sqrt((e.gamma * q.p) * q.inv);
+            for (int d = 0; d < D; ++d) lam[d] = fabs(q.u[d] - w[d]) + cs;
+        } else {
+            for (int d = 0; d < D; ++d) lam[d] = fabs(e.c[d][0] - w[d]);
+        }
+    }
     // is_admissible: finite, and for Euler rho > 0 and p > 0
     __device__ static __forceinline__ bool admissible(const PdeArgs& e, const double (&U)[M]) {
         for (int m = 0; m < M; ++m) if (!isfinite(U[m])) return false;
@@ -156,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomentArgs a, int64_t nblk) 
         int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
         double dd[D];
         double d2 = slot_geom<D>(a.g, pi, j, dd);
-        double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+        double w = slot_weight(a.g, d2);
         double p[BD];
         p[0] = 1.0;
         for (int d = 0; d < D; ++d) p[1 + d] = dd[d] * invL;
@@ -182,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomentArgs a, int64_t nblk) 
         int32_t j = a.g.nbr[s];
         double dd[D];
         double d2 = slot_geom<D>(a.g, pi, j, dd);
-        double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+        double w = slot_weight(a.g, d2);
         double v[BD];
         v[0] = w * 1.0;
         for (int d = 0; d < D; ++d) v[1 + d] = w * (dd[d] * invL);

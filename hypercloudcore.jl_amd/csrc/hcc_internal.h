@@ -65,6 +65,9 @@ struct GeomArgs {
     const int32_t* nbr;        // SELL-64 neighbour indices: slot k of node i at chunk_off[i/64] + 64 k + i%64
     double Lw[3], invLw[3];
     double invL, alpha, inv_range_sq;
+    double R_cut;               // a slot weighs 0 where |d|^2 > R_cut: R^2 on the stage geometry of a moving grid, whose
+                                // lists stay those of the step's search (_refresh_slots!, GridMovers.jl:111-120); +inf
+                                // else (every listed pair is within R)
     // LDS staging tables (2-D): the nodes of workgroup b (kStage consecutive nodes) read their neighbours from the
     // block's halo, the sorted union of their neighbourhoods (and themselves): halo_idx[halo_off[b] .. halo_off[b+1])
     const int32_t* halo_off;
@@ -129,6 +132,9 @@ struct StepArgs {
     int ext_on;                  // full gather passes store ext, the fused MOOD checks read it (MOODu1)
     const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate)
     double hratio;               // MOOD substep: h = hratio * dt
+    int ale;                     // HCC_MOVER_*: flux in the frame of the particle's grid velocity w (generic flux loop)
+    double mover_alpha;          // ScaledLagrangianMover: w = mover_alpha x fluid velocity
+    double* KX;                  // K_X of the right-hand side being computed: w per interior particle, D per node
 };
 
 }  // namespace hcc
@@ -164,5 +170,10 @@ struct hcc_grid {
 };
 
 namespace hcc {
+// pg.neighbor(pg) on the device after the positions moved: periodic wrap, cell lists and SELL lists rebuilt (the LDS
+// halo tables are dropped: the passes then read neighbours from HBM)
+int research(hcc_grid* g);
+// update_volumes!(pg, ShepardVolumes(sweeps)) into g->d_vol
+int grid_volumes(hcc_grid* g, int sweeps);
 int download_lists(hcc_grid* g, std::vector<int64_t>& rstart, std::vector<int32_t>& idx, std::vector<int64_t>* sell_of_slot);
 }

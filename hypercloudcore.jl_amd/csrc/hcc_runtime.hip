@@ -279,6 +279,7 @@ hcc::GeomArgs hcc_grid::args() const {
     a.isb = d_isb; a.cnt = d_cnt; a.chunk_off = d_chunk_off; a.nbr = d_nbr;
     for (int d = 0; d < 3; ++d) { a.Lw[d] = geom.Lw[d]; a.invLw[d] = geom.invLw[d]; }
     a.invL = geom.invL; a.alpha = geom.desc.alpha; a.inv_range_sq = geom.inv_range_sq;
+    a.R_cut = INFINITY;
     a.halo_off = d_halo_off; a.halo_idx = d_halo_idx; a.nbr_local = d_nbr_local; a.hmax = (int)max_halo;
     a.bslots = (int)max_block_slots;
     return a;
@@ -466,6 +467,21 @@ extern "C" int hcc_grid_volumes(hcc_grid* g, int32_t sweeps, double* volumes) {
     return HCC_OK;
 }
 
+int hcc::grid_volumes(hcc_grid* g, int sweeps) { return compute_volumes(g, sweeps); }
+
+int hcc::research(hcc_grid* g) {
+    hcc_ctx* ctx = g->ctx;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(g->d_chunk_off); g->d_chunk_off = nullptr;
+    (void)hipFree(g->d_nbr); g->d_nbr = nullptr;
+    (void)hipFree(g->d_halo_off); g->d_halo_off = nullptr;
+    (void)hipFree(g->d_halo_idx); g->d_halo_idx = nullptr;
+    (void)hipFree(g->d_nbr_local); g->d_nbr_local = nullptr;
+    g->max_halo = 0; g->max_block_slots = 0;
+    const int D = g->geom.D;
+    return D == 1 ? run_search<1>(g) : D == 2 ? run_search<2>(g) : run_search<3>(g);
+}
+
 extern "C" int hcc_grid_destroy(hcc_grid* g) {
     if (!g) return HCC_OK;
     (void)hipSetDevice(g->ctx->device);
@@ -572,6 +588,7 @@ int hcc::download_lists(hcc_grid* g, std::vector<int64_t>& rstart, std::vector<i
     const int64_t N = g->N;
     std::vector<int32_t> cnt((size_t)N), nbr((size_t)std::max<int64_t>(g->S_sell, 1));
     std::vector<int64_t> off((size_t)(g->nchunks + 1));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (N) {
         HCC_TRY_HIP(ctx, hipMemcpy(cnt.data(), g->d_cnt, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
         HCC_TRY_HIP(ctx, hipMemcpy(off.data(), g->d_chunk_off, sizeof(int64_t) * (g->nchunks + 1), hipMemcpyDeviceToHost));
@@ -643,6 +660,7 @@ extern "C" int hcc_grid_get_positions(hcc_grid* g, double* pos) {
     if (!g || (!pos && g->N)) return HCC_E_ARG;
     hcc_ctx* ctx = g->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));   // positions a queued moving step still writes
     const int D = g->geom.D;
     std::vector<double> col((size_t)g->N);
     for (int d = 0; d < D; ++d) {

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_imex(StepArgs a, int64_t nbl
 // runs for every interior particle; one without a gradient fit discards the fit sums.
 // CFL: also the node's CFL step of get_time_step (the sums of k_cfl over the same slots, with the wave speeds staged
 // in LDS), into *dtc (infinity where k_cfl skips the node).
-template <int D, int M, int KIND, bool EXT, bool CFL = false, class Src>
+template <int D, int M, int KIND, bool EXT, bool CFL = false, bool CUT = false, class Src>
 __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i,
                                             double* dtc = nullptr) {
     using L = Layout<D, M>;
@@ -405,7 +405,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
             }
             double dd[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
-            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+            const double w = CUT ? slot_weight(a.g, d2) : mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
             double p[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) p[d] = dd[d] * invL;
@@ -552,7 +552,8 @@ __device__ __forceinline__ void block_min(const StepArgs& a, double dt, int64_t 
     }
 }
 
-template <int D, int M, int KIND>
+// CUT: the stage geometry of a moving grid (slot weights 0 beyond R)
+template <int D, int M, int KIND, bool CUT>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
                                                     int64_t nblk) {
     if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
@@ -560,8 +561,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
     if (reset) step_reset(a, i);
     if (i >= a.g.N) return;
     if (a.g.isb[i] || !(full || a.changed[i])) return;
-    if (full) gather_node<D, M, KIND, true>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
-    else gather_node<D, M, KIND, false>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    if (full) gather_node<D, M, KIND, true, false, CUT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    else gather_node<D, M, KIND, false, false, CUT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
 }
 
 // The full gather pass with the workgroup's halo staged in LDS (2-D).
@@ -677,6 +678,7 @@ struct FluxNode {
     using L = Layout<D, M>;
     double ri[L::NR], fi[M], gi[L::NG], Fi[D][M];
     double X[FV == 1 ? D : 1][D + 1];
+    double w[FV == 1 ? D : 1];   // a moving grid: the particle's grid velocity (the frame of its flux)
     int oi, oi_t;   // interface order source, true order
 };
 
@@ -699,6 +701,13 @@ __device__ __forceinline__ void load_flux_node(const StepArgs& a, const double* 
         for (int r = 0; r < D; ++r)
 #pragma unroll
             for (int c = 0; c < BD; ++c) nd.X[r][c] = okd ? a.invd[(r * BD + c) * a.g.N + i] : 0.0;
+        if (a.ale) {   // F_i - w_i U_i
+            Pde<D, M, KIND>::grid_velocity(a.pde, nd.fi, a.ale, a.mover_alpha, nd.w);
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int m = 0; m < M; ++m) nd.Fi[d][m] = nd.Fi[d][m] - nd.w[d] * nd.fi[m];
+        }
     }
 }
 
@@ -714,7 +723,7 @@ __device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M
     const double invL = a.g.invL;
     double dd[D];
     const double d2 = rec_geom<D>(a.g, nd.ri, rj, dd);
-    w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+    w = FV == 1 ? slot_weight(a.g, d2) : mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
     int oj_t = oj;
     if (FV == 1 && a.packed) { oj_t = oj >> 4; oj &= 15; }
     int o_i = nd.oi, o_j = oj;
@@ -739,8 +748,18 @@ __device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M
         fji[m] = uj + sj;
     }
     double Fij[D][M], Fji[D][M], lij[D], lji[D];
-    P::flux_eig(a.pde, fij, Fij, lij);
-    P::flux_eig(a.pde, fji, Fji, lji);
+    bool ale = false;
+    if constexpr (FV == 1) {
+        if (a.ale) {   // both interface states in the frame of particle i (ALEFrame(eq, w_i))
+            ale = true;
+            P::flux_eig_ale(a.pde, fij, nd.w, Fij, lij);
+            P::flux_eig_ale(a.pde, fji, nd.w, Fji, lji);
+        }
+    }
+    if (!ale) {
+        P::flux_eig(a.pde, fij, Fij, lij);
+        P::flux_eig(a.pde, fji, Fji, lji);
+    }
     pt[0] = 1.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) pt[1 + d] = dd[d] * invL;
@@ -929,6 +948,14 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * (okd ? sacc[m] : 0.0)) + explicit_source<M>(a, i, nd.fi, m);
         } else {
             fit_divergence<D, M>(a, i, b, nd.fi, Kout);
+        }
+    }
+    if constexpr (FV == 1) {   // a moving grid: K_X[p] = w (_rhs_flux!, TimeSteppers.jl:346-349), also without neighbours
+        if (a.ale) {
+            double w[D];
+            Pde<D, M, KIND>::grid_velocity(a.pde, nd.fi, a.ale, a.mover_alpha, w);
+#pragma unroll
+            for (int d = 0; d < D; ++d) a.KX[i * D + d] = w[d];
         }
     }
     return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, nd.fi, Kout, h);
@@ -1698,6 +1725,56 @@ __global__ void k_set_dt(double* dtp, double dt) {
 }
 
 // ============================================================================================================
+// moving grids: stage positions, and the positions between the records and the grid
+// ============================================================================================================
+// The position part of _stage_predictor! (TimeSteppers.jl:405-425): X = x^n + sum_j (dt a~_ij) K_X[j], a term for
+// each non-zero entry a~_ij (b~_j for the final state), boundary particles exactly at x^n; written into the position
+// components of the target state's records (the stage geometry its passes read). x^n: the records of U^n.
+struct PosTerms {
+    const double* KX[kMaxStages];
+    double at[kMaxStages];
+    int nterm;
+    unsigned use;
+};
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_stage_pos(StepArgs a, int64_t nblk, const double* __restrict__ Un,
+                                                       PosTerms t, double* __restrict__ target) {
+    using L = Layout<D, M>;
+    const int64_t n = node_index(nblk);
+    if (n >= a.g.N) return;
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = Un[n * L::PU + d];
+    if (!a.g.isb[n]) {
+        const double dt = *a.dtp;
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j)
+            if (j < t.nterm && ((t.use >> j) & 1u)) {
+                const double c = dt * t.at[j];
+#pragma unroll
+                for (int d = 0; d < D; ++d) x[d] = x[d] + c * t.KX[j][n * D + d];
+            }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) target[n * L::PU + d] = x[d];
+}
+// to_grid: the records' positions into the grid's position columns (set_positions!, GridMovers.jl:129-133), else the
+// grid's (wrapped) positions into the records
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_rec_pos(int64_t N, double* __restrict__ rec, double* __restrict__ px,
+                                                     double* __restrict__ py, double* __restrict__ pz, int to_grid) {
+    using L = Layout<D, M>;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double* col[3] = {px, py, pz};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (to_grid) col[d][i] = rec[i * L::PU + d];
+        else rec[i * L::PU + d] = col[d][i];
+    }
+}
+
+// ============================================================================================================
 // packing, fills, counters
 // ============================================================================================================
 template <int D, int M>
@@ -1798,6 +1875,8 @@ struct hcc_stepper {
     double* rate = nullptr;        // decay rates of the source
     int src_mode = HCC_SRC_NONE;
     bool imex = false;             // stages formed by k_predict_imex
+    bool moving = false;           // a grid mover: stage positions, stage geometry, a search after each step
+    double* KX[kMaxStages] = {};   // K_X of the stages (moving): the grid velocity w per interior particle, D per node
     double* G = nullptr;
     double* scratch = nullptr; // N * M doubles for up/downloads
     int8_t* orders = nullptr;
@@ -1882,6 +1961,12 @@ StepArgs hcc_stepper::args() const {
     a.ext_on = (mood_fused(desc) && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
     a.dtp = dtp;
     a.hratio = desc.substep_ratio;
+    if (moving) {
+        a.ale = desc.mover;
+        a.mover_alpha = desc.mover_alpha;
+        a.KX = KX[0];
+        a.g.R_cut = grid->geom.R_sq;   // the stage geometry keeps the step's lists
+    }
     return a;
 }
 
@@ -1893,7 +1978,8 @@ struct Variant {
         const int64_t nb = nblocks(s->N);
         if (full && s->lds_gather)
             k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
-        else k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+        else if (s->moving) k_gather<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+        else k_gather<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count
     static unsigned split_grid(const hcc_stepper* s) {
@@ -1958,6 +2044,16 @@ struct Variant {
         k_predict<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->root, s->U, Ks[0], Ks[1], Ks[2], Ks[3], Ks[4], c[0],
                                                              c[1], c[2], c[3], c[4], nterm, use, target);
     }
+    static void stage_pos(const hcc_stepper* s, const StepArgs& a, const PosTerms& t, double* target) {
+        const int64_t nb = nblocks(s->N);
+        k_stage_pos<D, M><<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb, s->U, t, target);
+    }
+    static void rec_pos(const hcc_stepper* s, double* rec, int to_grid) {
+        const hcc_grid* g = s->grid;
+        k_rec_pos<D, M><<<(unsigned)((s->N + kBlock - 1) / kBlock), kBlock, 0, st(s)>>>(
+            s->N, rec, g->d_pos[0], g->d_pos[1] ? g->d_pos[1] : g->d_pos[0], g->d_pos[2] ? g->d_pos[2] : g->d_pos[0],
+            to_grid);
+    }
     static void mass(const hcc_stepper* s, int interior_only, double* out_dev) {
         k_mass<D, M><<<1, 256, 0, st(s)>>>(s->U, s->grid->d_vol, s->grid->d_isb, s->N, interior_only, out_dev);
     }
@@ -1982,6 +2078,8 @@ struct VariantOps {
     void (*range)(const hcc_stepper*, const double*);
     void (*cfl)(const hcc_stepper*, const StepArgs&, double*, double, double);
     void (*predict)(const hcc_stepper*, const StepArgs&, const double* const*, const double*, int, unsigned, double*);
+    void (*stage_pos)(const hcc_stepper*, const StepArgs&, const PosTerms&, double*);
+    void (*rec_pos)(const hcc_stepper*, double*, int);
     void (*mass)(const hcc_stepper*, int, double*);
     void (*pack)(const hcc_stepper*, const double*, double*);
     void (*unpack)(const hcc_stepper*, const double*, double*);
@@ -1991,7 +2089,7 @@ template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
     return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex, &V::range, &V::cfl, &V::predict,
-                      &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
+                      &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     if (kind == HCC_PDE_LINADV) {
@@ -2006,6 +2104,19 @@ static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     return false;
 }
 
+// prepare_geometry! (MUSCL.jl:172-223): the moment inverses and derivative weights of the grid's current positions;
+// R_cut: R^2 on a moving grid's stage geometry (drifted pairs weigh 0), +inf on a searched grid
+static void launch_moments(hcc_stepper* s, double R_cut) {
+    hcc_grid* g = s->grid;
+    MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
+    ma.g.R_cut = R_cut;
+    const int64_t nb = nblocks(s->N);
+    hipStream_t st = s->ctx->stream;
+    if (s->D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+    else if (s->D == 2) k_moments<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+    else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+}
+
 extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
     if (!s) return HCC_OK;
     (void)hipSetDevice(s->ctx->device);
@@ -2016,7 +2127,9 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters})
         (void)hipFree(p);
-    for (int k = 0; k < kMaxStages; ++k) { (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); }
+    for (int k = 0; k < kMaxStages; ++k) {
+        (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); (void)hipFree(s->KX[k]);
+    }
     (void)hipFree(s->rate);
     for (auto& r : s->ring) {
         if (r.ev0) (void)hipEventDestroy(r.ev0);
@@ -2058,11 +2171,22 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     for (int i = 0; i < d->stages; ++i)
         for (int j = i; j < d->stages; ++j)
             if (d->a_t[i][j] != 0.0) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: a_t must be strictly lower triangular");
+    if (d->mover < HCC_MOVER_NONE || d->mover > HCC_MOVER_SCALED_LAGRANGIAN)
+        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: grid mover");
+    if (d->mover != HCC_MOVER_NONE) {
+        if (!std::isfinite(d->mover_alpha)) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: mover_alpha must be finite");
+        if (d->pde == HCC_PDE_LINADV && d->M != 1)
+            return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: a moving grid with linear advection needs M = 1");
+        for (int b = 0; b < d->n_bc; ++b)
+            if (d->bc_kind[b] == HCC_BC_OUTFLOW)
+                return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: outflow boundary conditions on a moving grid");
+    }
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<hcc_stepper, int (*)(hcc_stepper*)> s(new hcc_stepper(), hcc_stepper_destroy);
     s->ctx = ctx; s->grid = g; s->desc = *d; s->D = D; s->M = d->M; s->kind = d->pde; s->N = g->N;
     s->PU = ops.PU; s->GS = ops.GS;
-    s->fv = (d->muscl_streamed || d->flux == HCC_FLUX_ALIGNED_RUSANOV) ? 1 : 0;
+    s->moving = d->mover != HCC_MOVER_NONE;
+    s->fv = (d->muscl_streamed || d->flux == HCC_FLUX_ALIGNED_RUSANOV || s->moving) ? 1 : 0;
     s->imex = implicit_part;
     const int64_t N = g->N, M = d->M;
     hipStream_t st = ctx->stream;
@@ -2081,6 +2205,11 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->K[k], M * N));
         HCC_TRY_HIP(ctx, hipMemsetAsync(s->K[k], 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     }
+    if (s->moving)
+        for (int k = 0; k < d->stages; ++k) {
+            HCC_TRY_HIP(ctx, dalloc(&s->KX[k], (int64_t)D * N));
+            HCC_TRY_HIP(ctx, hipMemsetAsync(s->KX[k], 0, sizeof(double) * std::max<int64_t>((int64_t)D * N, 1), st));
+        }
     if (s->imex)
         for (int k = 0; k < d->stages; ++k) {
             HCC_TRY_HIP(ctx, dalloc(&s->KI[k], M * N));
@@ -2113,7 +2242,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 10 * sizeof(unsigned long long), st));
-    if (g->max_halo > 0) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
+    if (g->max_halo > 0 && !s->moving) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
         int lds_max = 0;
         HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
         const size_t hm = (size_t)g->max_halo;
@@ -2140,11 +2269,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         ops.pack(s.get(), s->scratch, s->cand);   // every state buffer carries the positions (the predictor
         for (int k = 0; k < d->stages; ++k)       // writes state components only)
             ops.pack(s.get(), s->scratch, s->Y[k]);
-        MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
-        const int64_t nb = nblocks(N);
-        if (D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
-        else if (D == 2) k_moments<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
-        else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        launch_moments(s.get(), INFINITY);
         HCC_TRY_HIP(ctx, hipGetLastError());
     }
     {   // outflow donor roots (host, from the device lists and the wrapped positions)
@@ -2241,22 +2366,25 @@ struct StepRunner {
     int iters = 0;
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
     // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK); not with an implicit
-    // part, whose stages and substeps involve K_I
-    bool fused() const { return mood_fused(s->desc) && !s->imex; }
+    // part, whose stages and substeps involve K_I, nor on a moving grid, whose stage positions and geometry are formed
+    // between the passes
+    bool fused() const { return mood_fused(s->desc) && !s->imex && !s->moving; }
     bool epd2() const { return s->desc.mood_strategy == HCC_STRAT_EPD2; }
     // the arguments of a flux pass: with EPD2 the interface order of an edge is the minimum of the two particles'
     // neighbourhood minima, i.e. the edge-minimum rule of EPD1 applied to nbmin (computed before the pass)
-    StepArgs flux_args() {
-        if (!epd2()) return a;
-        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, nb);
+    // kst: the stage (0-based) whose right-hand side the pass computes, for its K_X on a moving grid
+    StepArgs flux_args(int kst) {
         StepArgs f = a;
+        if (s->moving) f.KX = s->KX[kst];
+        if (!epd2()) return f;
+        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, nb);
         f.orders = s->nbmin;
         f.strategy = HCC_STRAT_EPD1;
         f.packed = a.aligned;
         return f;
     }
     // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself
-    bool fuse_pred() const { return !s->imex && (fused() || s->desc.mood_criterion == HCC_CRIT_NONE); }
+    bool fuse_pred() const { return !s->imex && !s->moving && (fused() || s->desc.mood_criterion == HCC_CRIT_NONE); }
     bool y1_in_U = false;
     // state of stage k (1-based): U itself once the boundary conditions hold on it (every step after the first since
     // the state was loaded), else the predictor's copy in Y[0]
@@ -2300,9 +2428,22 @@ struct StepRunner {
             t.aii = fin ? 0.0 : s->desc.a_im[i - 1][i - 1];
             t.KIout = fin ? nullptr : s->KI[i - 1];
             ops.predict_imex(s, a, t, target);
-            return;
+        } else {
+            ops.predict(s, a, Ks, at, n_prev, use, target);
         }
-        ops.predict(s, a, Ks, at, n_prev, use, target);
+        if (s->moving) {   // X_i (x_cand for the final state) into the target's records
+            PosTerms t{};
+            for (int j = 0; j < kMaxStages; ++j) { t.KX[j] = s->KX[j < n_prev ? j : 0]; t.at[j] = at[j]; }
+            t.nterm = n_prev;
+            t.use = use;
+            ops.stage_pos(s, a, t, target);
+        }
+    }
+    // _apply_stage_geometry! (TimeSteppers.jl:520-524) with the frozen topology: the stage positions (held by the
+    // stage state's records) into the grid, and the geometry caches of the stage geometry
+    void apply_stage_geometry(const double* Y) {
+        ops.rec_pos(s, const_cast<double*>(Y), 1);
+        launch_moments(s, s->grid->geom.R_sq);
     }
     PredArgs pred_args(int i, bool count) const {
         PredArgs p{};
@@ -2390,7 +2531,7 @@ struct StepRunner {
                 k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->mask2, s->dirty, nb);
             }
             ops.gather(s, a, Yp, 0, 0);
-            ops.flux(s, flux_args(), off, Yp, Kp, 0, false);
+            ops.flux(s, flux_args(i - 2), off, Yp, Kp, 0, false);
             full = 0;
         }
         if (sub) predict(i);
@@ -2412,6 +2553,7 @@ struct StepRunner {
             } else {
                 hipError_t e = checkpoint(i);
                 if (e != hipSuccess) return e;
+                if (s->moving) apply_stage_geometry(stage_state(i));
             }
             const double* Y = stage_state(i);
             (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
@@ -2419,7 +2561,7 @@ struct StepRunner {
             else ops.gather(s, a, Y, 1, i == 1);
             (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
-            ops.flux(s, flux_args(), p, Y, s->K[i - 1], 1, fused());
+            ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
             (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1);
@@ -2468,6 +2610,26 @@ static int finish_pending(hcc_stepper* s) {
     return HCC_OK;
 }
 
+// The end of a moving grid's step (TimeSteppers.jl:571-575): the final positions x_cand (in the new state's records)
+// into the grid, a full search with the periodic wrap (pg.neighbor(pg)), the wrapped positions back into the state's
+// records, the geometry caches of the new lists and ShepardVolumes() (one sweep). Waits for the device (the search
+// sizes its lists on the host).
+static int moving_step_end(hcc_stepper* s, const VariantOps& ops) {
+    hcc_ctx* ctx = s->ctx;
+    hcc_grid* g = s->grid;
+    ops.rec_pos(s, s->U, 1);
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    if (int rc = research(g)) return rc;
+    (void)hipFree(s->dw);
+    s->dw = nullptr;
+    HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)s->D * g->S_sell));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)s->D * g->S_sell, 1), ctx->stream));
+    ops.rec_pos(s, s->U, 0);
+    launch_moments(s, INFINITY);
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    return grid_volumes(g, 1);
+}
+
 // Queues one step whose dt is already on the device (dtp), then the copies its statistics are read from.
 static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     hcc_ctx* ctx = s->ctx;
@@ -2482,6 +2644,8 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
     hipError_t e = r.run();
     if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return HCC_E_HIP; }
+    if (s->moving)
+        if (int rc = moving_step_end(s, r.ops)) return rc;
     HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h, s->counters, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h + 7, s->dtp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev1, ctx->stream));
@@ -2710,6 +2874,8 @@ extern "C" int hcc_solve(hcc_stepper* s, double tmax, double dt, int32_t is_cfl,
             for (int64_t q = 0; q < ns; ++q)
                 for (int m = 0; m < M; ++m) u_hist[((int64_t)c * ns + q) * M + m] = U[(size_t)(keep[(size_t)q] * M + m)];
         }
+        if (x_hist && s->moving && N)   // the positions move with the grid
+            if (int rc = hcc_grid_get_positions(s->grid, pos.data())) return rc;
         if (x_hist)
             for (int64_t q = 0; q < ns; ++q)
                 for (int d = 0; d < D; ++d) x_hist[((int64_t)c * ns + q) * D + d] = pos[(size_t)(keep[(size_t)q] * D + d)];

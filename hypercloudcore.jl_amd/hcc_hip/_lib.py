@@ -19,7 +19,7 @@ HCC_STRAT_NONE, HCC_STRAT_EPD0, HCC_STRAT_EPD1, HCC_STRAT_EPD2, HCC_STRAT_STRICT
 HCC_CRIT_NONE, HCC_CRIT_U1, HCC_CRIT_PAD, HCC_CRIT_ONLY, HCC_CRIT_U2 = 0, 1, 2, 3, 4
 HCC_FLUX_RUSANOV, HCC_FLUX_ALIGNED_RUSANOV = 0, 1
 HCC_SRC_NONE, HCC_SRC_DECAY_EXPLICIT, HCC_SRC_DECAY_IMPLICIT = 0, 1, 2
-HCC_MOVER_NONE, HCC_MOVER_LAGRANGIAN, HCC_MOVER_SCALED = 0, 1, 2
+HCC_MOVER_NONE, HCC_MOVER_LAGRANGIAN, HCC_MOVER_SCALED = 0, 1, 2   # HCC_MOVER_SCALED_LAGRANGIAN in hcc.h
 HCC_BC_NONE, HCC_BC_OUTFLOW, HCC_BC_DIRICHLET = 0, 1, 2
 HCC_E_CAPACITY = -4
 
@@ -48,7 +48,7 @@ class StepperDesc(C.Structure):
                 ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
                 ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double), ("muscl_streamed", C.c_int32),
                 ("flux", C.c_int32), ("a_im", (C.c_double * 5) * 5), ("b_im", C.c_double * 5),
-                ("c_im", C.c_double * 5)]
+                ("c_im", C.c_double * 5), ("mover", C.c_int32), ("mover_alpha", C.c_double)]
 
 
 _lib = None
@@ -197,18 +197,29 @@ class Grid:
                                 C.byref(h)), ctx.h, "hcc_grid_create")
         self.h = h
         ctx._grids.add(self)
+
+    def _slots(self):
         S, mx = C.c_int64(), C.c_int64()
-        check(L.hcc_grid_num_slots(self.h, C.byref(S), C.byref(mx)), ctx.h, "hcc_grid_num_slots")
-        self.S, self.max_nb = S.value, mx.value
+        check(lib().hcc_grid_num_slots(self.h, C.byref(S), C.byref(mx)), self.ctx.h, "hcc_grid_num_slots")
+        return S.value, mx.value
+
+    @property
+    def S(self):
+        """Neighbour slots of the current lists (a moving grid's search after each step changes them)."""
+        return self._slots()[0]
+
+    @property
+    def max_nb(self):
+        return self._slots()[1]
 
     def neighbors(self, weights=True, distances=True):
+        S = self.S
         ranges = np.zeros(self.N + 1, np.int64)
-        idx = np.zeros(max(self.S, 1), np.int32)
-        w = np.zeros(max(self.S, 1)) if weights else None
-        dist = np.zeros((max(self.S, 1), self.D)) if distances else None
+        idx = np.zeros(max(S, 1), np.int32)
+        w = np.zeros(max(S, 1)) if weights else None
+        dist = np.zeros((max(S, 1), self.D)) if distances else None
         check(lib().hcc_grid_get_neighbors(self.h, ptr(ranges), ptr(idx), ptr(w), ptr(dist)), self.ctx.h,
               "hcc_grid_get_neighbors")
-        S = self.S
         return ranges, idx[:S], (w[:S] if w is not None else None), (dist[:S] if dist is not None else None)
 
     def bins(self):
@@ -260,6 +271,8 @@ def stepper_desc(scheme):
     sd.mood_plateau = scheme.get("plateau", 0.0)
     sd.muscl_streamed = int(bool(scheme.get("streamed", False)))
     sd.flux = scheme.get("flux", HCC_FLUX_RUSANOV)
+    sd.mover = scheme.get("mover", HCC_MOVER_NONE)
+    sd.mover_alpha = scheme.get("mover_alpha", 1.0)
     a_t, b_t, c_t = scheme["a_t"], scheme["b_t"], scheme["c_t"]
     s = len(b_t)
     sd.stages = s

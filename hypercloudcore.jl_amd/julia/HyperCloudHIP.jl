@@ -60,7 +60,15 @@ struct StepperDesc
     a_im::NTuple{5, NTuple{5, Float64}}
     b_im::NTuple{5, Float64}
     c_im::NTuple{5, Float64}
+    mover::Int32             # 0 NoGridMover, 1 LagrangianMover, 2 ScaledLagrangianMover (hip_mover_code)
+    mover_alpha::Float64
 end
+
+"`(mover, mover_alpha)` of the descriptor for `pg.mover`; PrescribedMover (a closure) has no compiled kernel."
+hip_mover_code(m) = error("no compiled grid mover for $(typeof(m))")
+hip_mover_code(::HyperCloudCore.NoGridMover) = (Int32(0), 1.0)
+hip_mover_code(::HyperCloudCore.LagrangianMover) = (Int32(1), 1.0)
+hip_mover_code(m::HyperCloudCore.ScaledLagrangianMover) = (Int32(2), Float64(m.alpha))
 
 checkrc(rc, ctx) = rc == 0 || error("libhcc: ", unsafe_string(ccall((:hcc_last_error, libhcc), Cstring, (Ptr{Cvoid},), ctx)))
 
@@ -209,7 +217,17 @@ function hip_set_source!(ts::HIPRKTimeStepper, mode::Integer, rates::AbstractVec
     return ts
 end
 
-export hip_set_source!
+"Copies the device grid's positions (moved and wrapped by a moving stepper) into `pg.core.positions`."
+function download_positions!(ts::HIPRKTimeStepper, pg; D::Integer = length(pg.core.positions[1]))
+    x = zeros(Float64, D * ts.N)
+    checkrc(ccall((:hcc_grid_get_positions, libhcc), Cint, (Ptr{Cvoid}, Ptr{Float64}), ts.grid, x), ts.ctx)
+    for i in 1:ts.N
+        pg.core.positions[i] = typeof(pg.core.positions[i])(ntuple(d -> x[(i - 1) * D + d], D))
+    end
+    return pg
+end
+
+export hip_set_source!, hip_mover_code, download_positions!
 export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
        hip_volumes, hip_total_mass, hip_solve_equation
 

@@ -104,6 +104,13 @@ enum { HCC_FLUX_RUSANOV = 0, HCC_FLUX_ALIGNED_RUSANOV = 1 };
 /* source terms (src/Types.jl:148-172): the built-in decay S_i(U) = -rate_i U (the reference tests' DecaySource,
  * test/unit_schemes.jl:279-284), added to K_E (explicit) or solved in the diagonally implicit stages (implicit) */
 enum { HCC_SRC_NONE = 0, HCC_SRC_DECAY_EXPLICIT = 1, HCC_SRC_DECAY_IMPLICIT = 2 };
+/* grid movers (src/TimeStepping/GridMovers.jl:14-63): NoGridMover, LagrangianMover (w = fluid velocity),
+ * ScaledLagrangianMover(alpha) (w = alpha x fluid velocity). With a mover the particle positions are integrated as
+ * stage variables of the same tableau, the flux is taken in each particle's frame (F - w U, wave speeds |lambda - w|),
+ * each stage's geometry keeps the step's neighbour lists (weight 0 beyond R), and every step ends with a device
+ * neighbour search (periodic wrap) and ShepardVolumes. Not with outflow boundary conditions; linear advection with
+ * M = 1 only. The grid's lists and positions belong to the moving stepper: one moving stepper per grid. */
+enum { HCC_MOVER_NONE = 0, HCC_MOVER_LAGRANGIAN = 1, HCC_MOVER_SCALED_LAGRANGIAN = 2 };
 
 typedef struct {
     int32_t M;                 /* state components */
@@ -125,6 +132,8 @@ typedef struct {
     /* implicit part of an IMEX tableau (IMEXButcherTableau a, b, c, ButcherTableaus.jl:23-45): lower triangular
      * a_im; all zero for an explicit tableau. substep_ratio must be 0 for a tableau with an implicit part. */
     double a_im[5][5], b_im[5], c_im[5];
+    int32_t mover;             /* HCC_MOVER_* (pg.mover of the ParticleGrid) */
+    double mover_alpha;        /* ScaledLagrangianMover's alpha */
 } hcc_stepper_desc;
 
 /* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL

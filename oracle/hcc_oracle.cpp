@@ -948,19 +948,21 @@ struct Sim : SimBase {
     // (Interpolators.jl:610-626); returns K_E = -2 div + 0 (TimeSteppers.jl:354)
     void flux_node(int64_t i, const std::vector<State>& U, State& KE) {
         int64_t k0 = rstart[i], k1 = rstart[i + 1];
-        if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + source(i, U[i], m); return; }
         const State& fi = U[i];
-        double Fi[D][M]; P::flux(pde, fi.data(), Fi);
-        // a moving grid: the PDE in the frame of the particle's own grid velocity (ALEFrame, GridMovers.jl:65-90),
-        // flux F - w_i U and wave speeds |lambda - w_i|, for both interface states
+        // a moving grid: K_X[p] = w before the divergence (_rhs_flux!, TimeSteppers.jl:336-356), then the PDE in the
+        // frame of the particle's own grid velocity (ALEFrame, GridMovers.jl:65-90), flux F - w_i U and wave speeds
+        // |lambda - w_i|, for both interface states
         double wv[D] = {};
         const bool ale = sc.mover != 0;
         if (ale) {
             grid_velocity(fi, wv);
             KXcur[i] = {};
             for (int d = 0; d < D; ++d) KXcur[i][d] = wv[d];
-            for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m) Fi[d][m] = Fi[d][m] - wv[d] * fi[m];
         }
+        if (k0 == k1) { for (int m = 0; m < M; ++m) KE[m] = -0.0 + source(i, U[i], m); return; }
+        double Fi[D][M]; P::flux(pde, fi.data(), Fi);
+        if (ale)
+            for (int d = 0; d < D; ++d) for (int m = 0; m < M; ++m) Fi[d][m] = Fi[d][m] - wv[d] * fi[m];
         const auto& gi = grads[i];
         for (int64_t k = k0; k < k1; ++k) {
             int64_t j = nbr[k];

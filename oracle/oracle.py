@@ -348,7 +348,7 @@ def solve_equation(o, tmax, dt, is_cfl=True, snapshots=10, remove_ghosts=False, 
     def save(c, t):
         ts[c] = t
         us[c] = o.state()[keep]
-        xs[c] = None if pos is None else np.asarray(pos)[keep]
+        xs[c] = (o.positions() if pos is None else np.asarray(pos))[keep]   # a moving grid: the current positions
         ms[c] = (o.mass(False), o.mass(True))
 
     t, k_step, counter = 0.0, 0, 0
