@@ -75,6 +75,21 @@ __device__ __forceinline__ double slot_geom(const GeomArgs& g, const NodePos<D>&
     return s;
 }
 __device__ __forceinline__ int64_t sell_base(const GeomArgs& g, int64_t i) { return g.chunk_off[i >> 6] + (i & 63); }
+// Running extrema with Julia's max / min (NaN-propagating, -0 < +0) at three VALU operations per element instead of
+// the eight of jmax + jmin: the hardware v_max_f64 / v_min_f64 order signed zeros and, given one NaN operand, return
+// the other, so a NaN is tracked in a lane mask (x != x) and the extrema are replaced at the end by the canonical NaN
+// that jmax / jmin produce (0x7FF8000000000000). Arithmetic only: the operands are registers.
+__device__ __forceinline__ double hw_max(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double hw_min(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // the weight of a slot at squared distance d2: wf(d2), or 0 beyond R_cut (weights[k] = d2 <= R_sq ? wf(d2) : 0)
 __device__ __forceinline__ double slot_weight(const GeomArgs& g, double d2) {
     const double w = mls_weight(g.alpha, g.inv_range_sq, d2);

@@ -379,8 +379,9 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
 #pragma unroll
                 for (int c = 0; c < BD; ++c) X[r][c] = a.invd[(r * BD + c) * N + i];
         }
+        bool nanf[M];   // a NaN among the extrema's operands (hw_max / hw_min)
 #pragma unroll
-        for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; }
+        for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; nanf[m] = ri[D + m] != ri[D + m]; }
 #pragma unroll
         for (int q = 0; q < D; ++q)
 #pragma unroll
@@ -432,10 +433,14 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 const double df = fj - ri[D + m];
 #pragma unroll
                 for (int q = 0; q < D; ++q) b[q][m] = b[q][m] + w * (p[q] * df);
-                umax[m] = jmax(umax[m], fj);
-                umin[m] = jmin(umin[m], fj);
+                umax[m] = hw_max(umax[m], fj);
+                umin[m] = hw_min(umin[m], fj);
+                nanf[m] = nanf[m] | (fj != fj);
             }
         }
+#pragma unroll
+        for (int m = 0; m < M; ++m)   // jmax / jmin over the same operands: the canonical NaN once one was seen
+            if (nanf[m]) { umax[m] = __builtin_nan(""); umin[m] = __builtin_nan(""); }
         if (EXT && a.ext_on) {
             double e[2 * M];
 #pragma unroll
