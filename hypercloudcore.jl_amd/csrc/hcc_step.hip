@@ -140,13 +140,23 @@ static_assert(kStage == kBlock, "staged passes launch one workgroup per halo blo
 
 // Copies block lb's local-index table into LDS (nl) and returns its first SELL slot. The block's chunks are
 // consecutive and each holds 64 x width slots, so the range is 128-byte aligned and a multiple of 64 entries: it is
-// copied 8 indices at a time. The caller synchronises the workgroup before reading it.
+// copied 8 indices at a time, up to four loads per thread in flight before the first store (a table of neighbourhoods
+// up to 32 wide takes one round). The caller synchronises the workgroup before reading it.
 __device__ __forceinline__ int64_t stage_index_table(const GeomArgs& g, uint16_t* nl, int64_t lb) {
     const int64_t cpb = kStage / kChunk, nch = (g.N + kChunk - 1) / kChunk;
     const int64_t s0 = g.chunk_off[lb * cpb], s1 = g.chunk_off[(lb + 1) * cpb < nch ? (lb + 1) * cpb : nch];
     const uint4* isrc = reinterpret_cast<const uint4*>(g.nbr_local + s0);
     uint4* idst = reinterpret_cast<uint4*>(nl);
-    for (int64_t t = threadIdx.x; t < (s1 - s0) / 8; t += kStage) idst[t] = isrc[t];
+    const int64_t n16 = (s1 - s0) / 8;
+    for (int64_t t0 = threadIdx.x; t0 < n16; t0 += 4 * kStage) {
+        const int64_t t1 = t0 + kStage, t2 = t0 + 2 * kStage, t3 = t0 + 3 * kStage;
+        const uint4 v0 = isrc[t0], v1 = isrc[t1 < n16 ? t1 : t0], v2 = isrc[t2 < n16 ? t2 : t0],
+                    v3 = isrc[t3 < n16 ? t3 : t0];   // entries past the table reload t0
+        idst[t0] = v0;
+        if (t1 < n16) idst[t1] = v1;
+        if (t2 < n16) idst[t2] = v2;
+        if (t3 < n16) idst[t3] = v3;
+    }
     return s0;
 }
 
@@ -163,27 +173,43 @@ __device__ __forceinline__ LdsSrc<D, M, GRADS, LAM> stage_halo(const StepArgs& a
     double* lam_sh = sh + (size_t)a.g.hmax * S::ST;
     uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST + (LAM ? S::LS : 0), GRADS);
     const int64_t s0 = stage_index_table(a.g, nl, lb);
-    for (int t = threadIdx.x; t < nh; t += kStage) {
-        const int32_t j = a.g.halo_idx[h0 + t];
-        double r[L::NR];
-        load_rec<L::NR, L::PU>(Y, j, r);
-        double* dst = sh + t * S::ST;
+    // two halo nodes per thread per round, both nodes' loads in flight before the first store (a halo of up to
+    // 2 kStage nodes takes one round); a second node past the halo reloads the first
+    constexpr int U = 2;
+    for (int t0 = threadIdx.x; t0 < nh; t0 += U * kStage) {
+        int32_t j[U];
+        double r[U][L::NR], g[U][GRADS ? L::NG : 1];
+        int8_t o[U];
 #pragma unroll
-        for (int q = 0; q < L::NR; ++q) dst[q] = r[q];
-        if (GRADS) {
-            double g[L::NG];
-            load_rec<L::NG, L::GS>(a.G, j, g);
+        for (int u = 0; u < U; ++u) j[u] = a.g.halo_idx[h0 + (t0 + u * kStage < nh ? t0 + u * kStage : t0)];
 #pragma unroll
-            for (int q = 0; q < L::NG; ++q) dst[S::GO + q] = g[q];
-            ord[t] = a.orders[j];
+        for (int u = 0; u < U; ++u) {
+            load_rec<L::NR, L::PU>(Y, j[u], r[u]);
+            if constexpr (GRADS) {
+                load_rec<L::NG, L::GS>(a.G, j[u], g[u]);
+                o[u] = a.orders[j[u]];
+            }
         }
-        if constexpr (LAM) {
-            double f[M], l[D];
 #pragma unroll
-            for (int m = 0; m < M; ++m) f[m] = r[D + m];
-            Pde<D, M, KIND>::maxeig(a.pde, f, l);
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * kStage;
+            if (t >= nh) continue;
+            double* dst = sh + t * S::ST;
 #pragma unroll
-            for (int d = 0; d < D; ++d) lam_sh[t * S::LS + d] = l[d];
+            for (int q = 0; q < L::NR; ++q) dst[q] = r[u][q];
+            if constexpr (GRADS) {
+#pragma unroll
+                for (int q = 0; q < L::NG; ++q) dst[S::GO + q] = g[u][q];
+                ord[t] = o[u];
+            }
+            if constexpr (LAM) {
+                double f[M], l[D];
+#pragma unroll
+                for (int m = 0; m < M; ++m) f[m] = r[u][D + m];
+                Pde<D, M, KIND>::maxeig(a.pde, f, l);
+#pragma unroll
+                for (int d = 0; d < D; ++d) lam_sh[t * S::LS + d] = l[d];
+            }
         }
     }
     __syncthreads();
