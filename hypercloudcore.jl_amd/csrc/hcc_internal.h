@@ -130,7 +130,9 @@ struct StepArgs {
     double* bmin;                // per-workgroup minima of the CFL estimate
     double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima
     int ext_on;                  // full gather passes store ext, the fused MOOD checks read it (MOODu1)
-    const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate)
+    const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate); one per
+                                 // instance on an ensemble grid
+    const uint16_t* inst;        // ensemble grid: the instance of each kStage block (nullptr: one instance)
     double hratio;               // MOOD substep: h = hratio * dt
     int ale;                     // HCC_MOVER_*: flux in the frame of the particle's grid velocity w (generic flux loop)
     double mover_alpha;          // ScaledLagrangianMover: w = mover_alpha x fluid velocity
@@ -164,6 +166,12 @@ struct hcc_grid {
     uint16_t* d_nbr_local = nullptr;
     int64_t max_halo = 0, max_block_slots = 0;
     double* d_vol = nullptr;           // particle volumes (ShepardVolumes), N
+    // ensemble (hcc_grid_concat): instance k's nodes at [inst_off[k], inst_off[k] + inst_n[k]), padded to whole kStage
+    // blocks with inert boundary nodes; the instance of each block, and each instance's block range
+    int n_inst = 1;
+    std::vector<int64_t> inst_off, inst_n;
+    uint16_t* d_blk_inst = nullptr;
+    int64_t* d_inst_blk = nullptr;     // n_inst + 1 block offsets
     std::vector<int32_t> h_tags;
     std::vector<uint8_t> h_isb;
     hcc::GeomArgs args() const;

@@ -459,6 +459,8 @@ static int compute_volumes(hcc_grid* g, int sweeps) {
 extern "C" int hcc_grid_volumes(hcc_grid* g, int32_t sweeps, double* volumes) {
     if (!g) return HCC_E_ARG;
     hcc_ctx* ctx = g->ctx;
+    if (sweeps >= 0 && g->n_inst > 1)
+        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_grid_volumes: an ensemble keeps its instances' volumes (recompute them on the parts)");
     if (sweeps >= 0) {
         HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
         if (int rc = compute_volumes(g, sweeps)) return rc;
@@ -482,6 +484,141 @@ int hcc::research(hcc_grid* g) {
     return D == 1 ? run_search<1>(g) : D == 2 ? run_search<2>(g) : run_search<3>(g);
 }
 
+// ============================================================================================================
+// ensembles: one grid assembled from instance grids, each padded to whole kStage blocks
+// ============================================================================================================
+__global__ __launch_bounds__(kBlock) void k_copy_offset(int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                                                        int64_t n, int32_t add) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) dst[t] = src[t] + add;
+}
+
+extern "C" int hcc_grid_concat(hcc_ctx* ctx, int32_t n, hcc_grid* const* parts, hcc_grid** out, int64_t* offsets) {
+    if (!ctx || !out || !parts || n < 1 || n > 65535) return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: bad arguments");
+    *out = nullptr;
+    const hcc_grid* p0 = parts[0];
+    for (int k = 0; k < n; ++k) {
+        const hcc_grid* p = parts[k];
+        if (!p || p->ctx != ctx) return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: a part is NULL or of another context");
+        if (p->n_inst != 1) return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: a part is itself an ensemble");
+        if (p->N == 0) return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: an empty part");
+        bool same = p->geom.D == p0->geom.D && p->geom.R_sq == p0->geom.R_sq && p->geom.inv_range_sq == p0->geom.inv_range_sq &&
+                    p->geom.desc.alpha == p0->geom.desc.alpha && p->geom.invL == p0->geom.invL;
+        for (int d = 0; d < 3; ++d) same = same && p->geom.Lw[d] == p0->geom.Lw[d] && p->geom.invLw[d] == p0->geom.invLw[d];
+        if (!same)
+            return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: the parts differ in D, R, alpha, minimum spacing or wrap lengths");
+    }
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+    std::unique_ptr<hcc_grid, int (*)(hcc_grid*)> g(new hcc_grid(), hcc_grid_destroy);
+    g->ctx = ctx;
+    g->geom = p0->geom;
+    const int D = p0->geom.D;
+    bool halos = D == 2;
+    int64_t N = 0, S_sell = 0, nst = 0;
+    g->n_inst = n;
+    g->inst_off.resize((size_t)n + 1);
+    g->inst_n.resize((size_t)n);
+    std::vector<int64_t> blk_off((size_t)n + 1), s_off((size_t)n + 1), h_off((size_t)n + 1);
+    for (int k = 0; k < n; ++k) {
+        const hcc_grid* p = parts[k];
+        g->inst_off[k] = N; g->inst_n[k] = p->N; blk_off[k] = nst; s_off[k] = S_sell; h_off[k] = 0;
+        const int64_t nb = (p->N + kStage - 1) / kStage;
+        N += nb * kStage; nst += nb; S_sell += p->S_sell;
+        g->S += p->S; g->max_nb = std::max(g->max_nb, p->max_nb);
+        halos = halos && p->max_halo > 0 && p->N > 0;
+    }
+    g->inst_off[n] = N; blk_off[n] = nst; s_off[n] = S_sell;
+    if (N > (int64_t)INT32_MAX - 1) return set_err(ctx, HCC_E_ARG, "hcc_grid_concat: more than 2^31 - 2 nodes");
+    g->N = N; g->S_sell = S_sell; g->nchunks = N / kChunk;
+    for (int d = 0; d < D; ++d) {
+        HCC_TRY_HIP(ctx, dalloc(&g->d_pos[d], N));
+        HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_pos[d], 0, sizeof(double) * std::max<int64_t>(N, 1), st));
+    }
+    HCC_TRY_HIP(ctx, dalloc(&g->d_isb, N));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_isb, 1, std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_tags, N));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_tags, 0, sizeof(int32_t) * std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_cnt, N));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_cnt, 0, sizeof(int32_t) * std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_vol, N));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_vol, 0, sizeof(double) * std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_nbr, S_sell));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_chunk_off, g->nchunks + 1));
+    g->h_isb.assign((size_t)N, 1);
+    g->h_tags.assign((size_t)N, 0);
+    std::vector<int64_t> coff((size_t)g->nchunks + 1);
+    std::vector<int32_t> hoff;
+    if (halos) { HCC_TRY_HIP(ctx, dalloc(&g->d_nbr_local, S_sell)); hoff.assign((size_t)nst + 1, 0); }
+    int64_t H = 0;
+    std::vector<int64_t> h_tot((size_t)n + 1, 0);
+    for (int k = 0; k < n; ++k) {   // halo sizes first: the index array's length
+        const hcc_grid* p = parts[k];
+        if (!halos) break;
+        int32_t last = 0;
+        const int64_t nb = blk_off[k + 1] - blk_off[k];
+        HCC_TRY_HIP(ctx, hipMemcpy(&last, p->d_halo_off + nb, sizeof(int32_t), hipMemcpyDeviceToHost));
+        h_tot[k] = H;
+        H += last;
+    }
+    if (halos) HCC_TRY_HIP(ctx, dalloc(&g->d_halo_idx, std::max<int64_t>(H, 1)));
+    std::vector<uint16_t> blk_inst((size_t)std::max<int64_t>(nst, 1));
+    for (int k = 0; k < n; ++k) {
+        const hcc_grid* p = parts[k];
+        const int64_t o = g->inst_off[k], Nk = p->N;
+        for (int64_t b = blk_off[k]; b < blk_off[k + 1]; ++b) blk_inst[(size_t)b] = (uint16_t)k;
+        for (int d = 0; d < D; ++d)
+            HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_pos[d] + o, p->d_pos[d], sizeof(double) * Nk, hipMemcpyDeviceToDevice, st));
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_isb + o, p->d_isb, Nk, hipMemcpyDeviceToDevice, st));
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_tags + o, p->d_tags, sizeof(int32_t) * Nk, hipMemcpyDeviceToDevice, st));
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_cnt + o, p->d_cnt, sizeof(int32_t) * Nk, hipMemcpyDeviceToDevice, st));
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_vol + o, p->d_vol, sizeof(double) * Nk, hipMemcpyDeviceToDevice, st));
+        std::copy(p->h_isb.begin(), p->h_isb.end(), g->h_isb.begin() + o);
+        std::copy(p->h_tags.begin(), p->h_tags.end(), g->h_tags.begin() + o);
+        // SELL: the part's chunks follow at chunk o / 64 (a multiple of 4); its padding chunks hold no slots
+        std::vector<int64_t> pc((size_t)p->nchunks + 1);
+        HCC_TRY_HIP(ctx, hipMemcpy(pc.data(), p->d_chunk_off, sizeof(int64_t) * pc.size(), hipMemcpyDeviceToHost));
+        const int64_t c0 = o / kChunk, nck = (blk_off[k + 1] - blk_off[k]) * (kStage / kChunk);
+        for (int64_t c = 0; c <= nck; ++c) coff[(size_t)(c0 + c)] = s_off[k] + pc[(size_t)std::min<int64_t>(c, p->nchunks)];
+        if (p->S_sell) {
+            k_copy_offset<<<(unsigned)((p->S_sell + kBlock - 1) / kBlock), kBlock, 0, st>>>(g->d_nbr + s_off[k], p->d_nbr,
+                                                                                          p->S_sell, (int32_t)o);
+            if (halos)
+                HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_nbr_local + s_off[k], p->d_nbr_local, sizeof(uint16_t) * p->S_sell,
+                                                hipMemcpyDeviceToDevice, st));
+        }
+        if (halos) {   // block halos: node ids shifted by the instance offset, offsets by the halos before
+            const int64_t nb = blk_off[k + 1] - blk_off[k];
+            std::vector<int32_t> ph((size_t)nb + 1);
+            HCC_TRY_HIP(ctx, hipMemcpy(ph.data(), p->d_halo_off, sizeof(int32_t) * ph.size(), hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b <= nb; ++b) hoff[(size_t)(blk_off[k] + b)] = (int32_t)(h_tot[k] + ph[(size_t)b]);
+            if (ph[(size_t)nb])
+                k_copy_offset<<<(unsigned)((ph[(size_t)nb] + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                    g->d_halo_idx + h_tot[k], p->d_halo_idx, ph[(size_t)nb], (int32_t)o);
+            g->max_halo = std::max(g->max_halo, p->max_halo);
+            g->max_block_slots = std::max(g->max_block_slots, p->max_block_slots);
+        }
+        HCC_TRY_HIP(ctx, hipGetLastError());
+    }
+    HCC_TRY_HIP(ctx, hipMemcpy(g->d_chunk_off, coff.data(), sizeof(int64_t) * coff.size(), hipMemcpyHostToDevice));
+    if (halos) {
+        HCC_TRY_HIP(ctx, dalloc(&g->d_halo_off, hoff.size()));
+        HCC_TRY_HIP(ctx, hipMemcpy(g->d_halo_off, hoff.data(), sizeof(int32_t) * hoff.size(), hipMemcpyHostToDevice));
+    } else {
+        g->max_halo = 0;
+        g->max_block_slots = 0;
+    }
+    HCC_TRY_HIP(ctx, dalloc(&g->d_blk_inst, blk_inst.size()));
+    HCC_TRY_HIP(ctx, hipMemcpy(g->d_blk_inst, blk_inst.data(), sizeof(uint16_t) * blk_inst.size(), hipMemcpyHostToDevice));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_inst_blk, blk_off.size()));
+    HCC_TRY_HIP(ctx, hipMemcpy(g->d_inst_blk, blk_off.data(), sizeof(int64_t) * blk_off.size(), hipMemcpyHostToDevice));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (offsets) std::copy(g->inst_off.begin(), g->inst_off.end(), offsets);
+    *out = g.release();
+    return HCC_OK;
+}
+
 extern "C" int hcc_grid_destroy(hcc_grid* g) {
     if (!g) return HCC_OK;
     (void)hipSetDevice(g->ctx->device);
@@ -491,6 +628,7 @@ extern "C" int hcc_grid_destroy(hcc_grid* g) {
     (void)hipFree(g->d_cell_particles); (void)hipFree(g->d_bin_window); (void)hipFree(g->d_cnt); (void)hipFree(g->d_chunk_off); (void)hipFree(g->d_nbr);
     (void)hipFree(g->d_halo_off); (void)hipFree(g->d_halo_idx); (void)hipFree(g->d_nbr_local);
     (void)hipFree(g->d_vol);
+    (void)hipFree(g->d_blk_inst); (void)hipFree(g->d_inst_blk);
     delete g;
     return HCC_OK;
 }
@@ -644,6 +782,7 @@ extern "C" int hcc_grid_get_neighbors(hcc_grid* g, int64_t* ranges, int32_t* ind
 extern "C" int hcc_grid_get_bins(hcc_grid* g, int32_t* bin_of, int64_t* cell_start, int32_t* cell_particles, int64_t* dims) {
     if (!g) return HCC_E_ARG;
     hcc_ctx* ctx = g->ctx;
+    if (g->n_inst > 1) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_grid_get_bins: an ensemble has no cell list (query the parts)");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (bin_of && g->N) HCC_TRY_HIP(ctx, hipMemcpy(bin_of, g->d_bin_of, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     if (cell_particles && g->N) HCC_TRY_HIP(ctx, hipMemcpy(cell_particles, g->d_cell_particles, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));

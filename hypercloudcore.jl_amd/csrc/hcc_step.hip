@@ -138,6 +138,12 @@ __device__ __forceinline__ int64_t logical_block(int64_t nblk) {
 
 static_assert(kStage == kBlock, "staged passes launch one workgroup per halo block");
 
+// The step's dt of node i: one per instance on an ensemble grid, whose instances fill whole kStage blocks (inst: the
+// instance of each block), else dtp[0]
+__device__ __forceinline__ double node_dt(const double* dtp, const uint16_t* inst, int64_t i) {
+    return inst ? dtp[inst[i / kStage]] : *dtp;
+}
+
 // Copies block lb's local-index table into LDS (nl) and returns its first SELL slot. The block's chunks are
 // consecutive and each holds 64 x width slots, so the range is 128-byte aligned and a multiple of 64 entries: it is
 // copied 8 indices at a time, up to four loads per thread in flight before the first store (a table of neighbourhoods
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_predict(StepArgs a, int64_t nblk, co
         // terms unrolled with compile-time indices: the K rows are plain global loads (a runtime-indexed pointer
         // array would be demoted to flat loads, one per component and term, each waited for)
         const double* Ks[kMaxStages] = {K0, K1, K2, K3, K4};
-        const double dt = *a.dtp;
+        const double dt = node_dt(a.dtp, a.inst, i);
         const double cs[kMaxStages] = {dt * a0, dt * a1, dt * a2, dt * a3, dt * a4};
         double y[M];
 #pragma unroll
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_imex(StepArgs a, int64_t nbl
 #pragma unroll
             for (int m = 0; m < M; ++m) y[m] = r[D + m - 2 * P0];
         }
-        const double dt = *a.dtp;
+        const double dt = node_dt(a.dtp, a.inst, i);
 #pragma unroll
         for (int j = 0; j < kMaxStages; ++j) {
             if (j < t.nterm) {
@@ -630,6 +636,7 @@ struct PredArgs {
     const int32_t* goff;
     const int32_t* gidx;
     const double* dtp;
+    const uint16_t* inst;      // ensemble grid: the instance of each kStage block (node_dt)
     int on, count;
 };
 
@@ -641,7 +648,7 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     const double2* own = reinterpret_cast<const double2*>(p.Un + n * L::PU) + P0;
 #pragma unroll
     for (int q = 0; q < NP; ++q) { const double2 v = own[q]; r[2 * q] = v.x; r[2 * q + 1] = v.y; }
-    const double dt = *p.dtp;
+    const double dt = node_dt(p.dtp, p.inst, n);
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j) {
         if (j < p.nterm && ((p.use >> j) & 1u)) {
@@ -1032,12 +1039,13 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
 template <int D, int M, int KIND, bool CHECK, int FV>
 __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int full, int64_t nblk) {
-    const double h = CHECK ? a.hratio * *a.dtp : 0.0;   // MOOD's Euler substep (form_substep!, TimeSteppers.jl:501-518)
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
     bool ch = false;
     int bad = 0;
     if (i < N) {
+        // MOOD's Euler substep (form_substep!, TimeSteppers.jl:501-518)
+        const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, i) : 0.0;
         if (a.g.isb[i]) {
             if (full)
 #pragma unroll
@@ -1058,9 +1066,9 @@ template <int D, int M, int KIND, bool CHECK, int FV>
 __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                          double* __restrict__ K, int64_t nblk) {
     extern __shared__ double sh[];
-    const double h = CHECK ? a.hratio * *a.dtp : 0.0;
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
+    const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, lb * kBlock) : 0.0;
     const auto src = stage_halo<D, M, true>(a, Y, sh, lb);
     const int64_t i = lb * kBlock + threadIdx.x;
     const int64_t N = a.g.N;
@@ -1102,7 +1110,6 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
     static_assert(NP <= G && M <= SR, "one accumulator owner per lane");
     __shared__ double rows[GPW][G][SR];
     __shared__ double part[GPW][NP];
-    const double h = a.hratio * *a.dtp;
     const int64_t n = (int64_t)*work_count(a, p);
     const int64_t N = a.g.N;
     const int grp = threadIdx.x / G, gl = threadIdx.x % G;
@@ -1207,7 +1214,7 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
                 for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, nd.fi, m);
             }
             const int before = a.inadm[i];
-            ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, h);
+            ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, a.hratio * node_dt(a.dtp, a.inst, i));
             delta = (int)a.inadm[i] - before;
         }
         __syncthreads();
@@ -1444,7 +1451,7 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
     if (i < N && !a.g.isb[i] && (full || a.dirty[i])) {
         double r[L::NR], W[M];
         if (Kp) {
-            const double h = a.hratio * *a.dtp;
+            const double h = a.hratio * node_dt(a.dtp, a.inst, i);
             load_rec<L::NR, L::PU>(Yp, i, r);
 #pragma unroll
             for (int m = 0; m < M; ++m)   // with implicit sources the substep is Y + h (K + K_I)
@@ -1751,8 +1758,28 @@ __global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ b
     }
 }
 
-__global__ void k_set_dt(double* dtp, double dt) {
-    if (threadIdx.x == 0) *dtp = dt;
+__global__ void k_set_dt(double* dtp, double dt, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) dtp[k] = dt;
+}
+
+// The CFL steps of an ensemble's instances: instance k (workgroup k) reduces the minima of its blocks
+// [blk[k], blk[k + 1]); dtp[k] = min(cfl dt_k, cap), counters[2] the smallest dt_k (bits of a non-negative double
+// order as integers)
+__global__ __launch_bounds__(256) void k_cfl_final_multi(const double* __restrict__ bmin, const int64_t* __restrict__ blk,
+                                                         unsigned long long* counters, double* dtp, double cfl, double cap) {
+    __shared__ double wmin[4];
+    const int k = blockIdx.x;
+    double dt = INFINITY;
+    for (int64_t b = blk[k] + threadIdx.x; b < blk[k + 1]; b += 256) dt = fmin(dt, bmin[b]);
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < 4; ++w) m = fmin(m, wmin[w]);
+        atomicMin(&counters[2], (unsigned long long)__double_as_longlong(m));
+        if (dtp) dtp[k] = jmin(cfl * m, cap);
+    }
 }
 
 // ============================================================================================================
@@ -1777,7 +1804,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_pos(StepArgs a, int64_t nblk, 
 #pragma unroll
     for (int d = 0; d < D; ++d) x[d] = Un[n * L::PU + d];
     if (!a.g.isb[n]) {
-        const double dt = *a.dtp;
+        const double dt = node_dt(a.dtp, a.inst, n);
 #pragma unroll
         for (int j = 0; j < kMaxStages; ++j)
             if (j < t.nterm && ((t.use >> j) & 1u)) {
@@ -1991,6 +2018,7 @@ StepArgs hcc_stepper::args() const {
     a.ext = ext;
     a.ext_on = (mood_fused(desc) && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
     a.dtp = dtp;
+    a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.hratio = desc.substep_ratio;
     if (moving) {
         a.ale = desc.mover;
@@ -2021,7 +2049,7 @@ struct Variant {
     static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
         k_gather_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
-        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters, s->dtp, cfl, cap);
+        cfl_final(s, s->dtp, cfl, cap);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
         k_mark_gather<D, M, KIND><<<split_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
@@ -2067,7 +2095,14 @@ struct Variant {
             k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
             k_cfl<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
         }
-        k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nb, s->counters, dtp, cfl, cap);
+        cfl_final(s, dtp, cfl, cap);
+    }
+    // the minimum over the workgroup minima: one dt, or one per ensemble instance
+    static void cfl_final(const hcc_stepper* s, double* dtp, double cfl, double cap) {
+        const hcc_grid* g = s->grid;
+        if (g->n_inst > 1)
+            k_cfl_final_multi<<<(unsigned)g->n_inst, 256, 0, st(s)>>>(s->bmin, g->d_inst_blk, s->counters, dtp, cfl, cap);
+        else k_cfl_final<<<1, 1024, 0, st(s)>>>(s->bmin, nblocks(s->N), s->counters, dtp, cfl, cap);
     }
     static void predict(const hcc_stepper* s, const StepArgs& a, const double* const* Ks, const double* c, int nterm,
                         unsigned use, double* target) {   // c: tableau entries (scaled by the device dt in the kernel)
@@ -2205,6 +2240,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     if (d->mover < HCC_MOVER_NONE || d->mover > HCC_MOVER_SCALED_LAGRANGIAN)
         return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: grid mover");
     if (d->mover != HCC_MOVER_NONE) {
+        if (g->n_inst > 1) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: a grid mover on an ensemble grid");
         if (!std::isfinite(d->mover_alpha)) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: mover_alpha must be finite");
         if (d->pde == HCC_PDE_LINADV && d->M != 1)
             return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: a moving grid with linear advection needs M = 1");
@@ -2264,7 +2300,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->mask2, N));
     HCC_TRY_HIP(ctx, dalloc(&s->rpart, 2 * M * ((N + kBlock - 1) / kBlock)));
     HCC_TRY_HIP(ctx, dalloc(&s->range, M));
-    HCC_TRY_HIP(ctx, dalloc(&s->dtp, 2));
+    HCC_TRY_HIP(ctx, dalloc(&s->dtp, std::max(2, g->n_inst)));
     HCC_TRY_HIP(ctx, dalloc(&s->counters, 10));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), st));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm})
@@ -2386,6 +2422,22 @@ extern "C" int hcc_time_step(hcc_stepper* s, double* dt) {
     return HCC_OK;
 }
 
+extern "C" int hcc_time_steps(hcc_stepper* s, double* dts) {
+    if (!s || !dts) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    const int n = s->grid->n_inst;
+    if (n == 1) return hcc_time_step(s, dts);
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    VariantOps ops;
+    find_ops(s->D, s->M, s->kind, ops);
+    k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
+    ops.cfl(s, s->args(), s->dtp, 1.0, INFINITY);   // dtp[k] = min(1 dt_k, inf) = dt_k; the next step sets dtp again
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(dts, s->dtp, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HCC_OK;
+}
+
 // host driver of one step (TimeSteppers.jl:531-577)
 namespace {
 struct StepRunner {
@@ -2483,6 +2535,7 @@ struct StepRunner {
         p.goff = s->goff;
         p.gidx = s->gidx;
         p.dtp = s->dtp;
+        p.inst = a.inst;
         p.on = 1;
         p.count = count ? 1 : 0;
         return p;
@@ -2694,7 +2747,7 @@ extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (s->N == 0) return HCC_OK;
-    k_set_dt<<<1, 64, 0, ctx->stream>>>(s->dtp, dt);
+    k_set_dt<<<1, 64, 0, ctx->stream>>>(s->dtp, dt, s->grid->n_inst);
     return launch_step(s, "hcc_step", false);
 }
 

@@ -31,6 +31,7 @@ EXPORTS = [
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
     "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve", "hcc_set_source",
+    "hcc_grid_concat", "hcc_time_steps",
 ]
 
 
@@ -98,6 +99,8 @@ def lib():
         "hcc_total_mass": (C.c_int, [vp, C.c_int32, vp]),
         "hcc_solve": (C.c_int, [vp, d, d, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, P(i64), P(d)]),
         "hcc_set_source": (C.c_int, [vp, C.c_int32, vp]),
+        "hcc_grid_concat": (C.c_int, [vp, C.c_int32, vp, P(vp), vp]),
+        "hcc_time_steps": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -255,6 +258,38 @@ class Grid:
             pass
 
 
+class Ensemble(Grid):
+    """One device grid holding copies of several instance grids (hcc_grid_concat): instance k's nodes at
+    [offsets[k], offsets[k] + sizes[k]), each instance padded to whole 256-node blocks with inert boundary nodes. A
+    Stepper on it advances every instance in one step; pack / split convert between per-instance arrays and the
+    ensemble's."""
+
+    def __init__(self, ctx, grids):
+        self.ctx, self.dd = ctx, grids[0].dd
+        self.D = grids[0].D
+        n = len(grids)
+        handles = (C.c_void_p * n)(*[g.h for g in grids])
+        offs = np.zeros(n + 1, np.int64)
+        h = C.c_void_p()
+        check(lib().hcc_grid_concat(ctx.h, n, handles, C.byref(h), ptr(offs)), ctx.h, "hcc_grid_concat")
+        self.h = h
+        ctx._grids.add(self)
+        self.offsets = offs
+        self.sizes = np.array([g.N for g in grids], np.int64)
+        self.N = int(offs[-1])
+
+    def pack(self, parts, fill=0.0):
+        """Per-instance arrays (N_k x ...) into one ensemble array (padding rows = fill)."""
+        first = np.asarray(parts[0])
+        out = np.full((self.N,) + first.shape[1:], fill, first.dtype)
+        for k, a in enumerate(parts):
+            out[self.offsets[k]:self.offsets[k] + self.sizes[k]] = a
+        return out
+
+    def split(self, a):
+        return [a[self.offsets[k]:self.offsets[k] + self.sizes[k]] for k in range(len(self.sizes))]
+
+
 def stepper_desc(scheme):
     """hcc_stepper_desc from a plain scheme dict (see hcc_hip.workloads)."""
     sd = StepperDesc()
@@ -318,6 +353,13 @@ class Stepper:
         v = C.c_double()
         check(lib().hcc_time_step(self.h, C.byref(v)), self.ctx.h, "hcc_time_step")
         return v.value
+
+    def time_steps(self):
+        """The CFL step of every instance of an ensemble grid (one value for a plain grid)."""
+        n = len(getattr(self.grid, "sizes", [0]))
+        out = np.zeros(n)
+        check(lib().hcc_time_steps(self.h, ptr(out)), self.ctx.h, "hcc_time_steps")
+        return out
 
     def step(self, t, dt):
         check(lib().hcc_step(self.h, float(t), float(dt)), self.ctx.h, "hcc_step")

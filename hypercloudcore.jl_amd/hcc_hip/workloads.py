@@ -163,6 +163,35 @@ def euler_2d(h=1e-3, seed=0, gamma=1.4, n=None):
     return {"name": f"euler2d_h{h:g}", "domain": dom, "scheme": sch, "cloud": cloud, "U0": U0, "cfl": 0.4, "steps": 200}
 
 
+def euler_2d_box(n, h, seed=0, gamma=1.4):
+    """The four-quadrant Euler problem on the box [0, n h]^2 with outflow ghosts: the config of euler_2d with the
+    spacing h shared by every box and the split at the box centre. An ensemble member (boxes of one h and different n
+    share every kernel constant)."""
+    L_box = n * h
+    dom = domain(2, (0.0, 0.0), (L_box, L_box), (False, False), h, 2.5, 0.25, seed)
+    bcs = [(t, L.HCC_BC_OUTFLOW) for t in (1, 2, 3, 4)]
+    sch = scheme(4, L.HCC_PDE_EULER, gamma=gamma, delta=1e-4 * h * h, bcs=bcs)
+    cloud = L.cloud_generate(domain_desc(dom))
+    x = cloud["pos"]
+    rho_q, p_q, u_q, v_q = quadrant_states(seed, gamma)
+    q = (x[:, 0] >= 0.5 * L_box).astype(int) + 2 * (x[:, 1] >= 0.5 * L_box).astype(int)
+    U0 = euler_conserved(rho_q[q], np.column_stack([u_q[q], v_q[q]]), p_q[q], gamma)
+    return {"name": f"euler2d_box_n{n}", "domain": dom, "scheme": sch, "cloud": cloud, "U0": U0, "cfl": 0.4}
+
+
+def ensemble_sizes(count, n_min, n_max, seed=0):
+    """Lattice sides of `count` ensemble members with node counts log-uniform in [n_min, n_max] (seeded)."""
+    r = L.rng_uniform(seed + 0xE5, count)
+    nodes = np.exp(np.log(n_min) + (np.log(n_max) - np.log(n_min)) * r)
+    return np.maximum(np.rint(np.sqrt(nodes)).astype(int), 4)
+
+
+def euler_2d_ensemble(count, n_min, n_max, h=1e-3, seed=0):
+    """`count` four-quadrant Euler boxes of spacing h with node counts log-uniform in [n_min, n_max]; member k has
+    seed seed + 1 + k (its own quadrant states and jitter)."""
+    return [euler_2d_box(int(n), h, seed + 1 + k) for k, n in enumerate(ensemble_sizes(count, n_min, n_max, seed))]
+
+
 def euler_3d(n=160, seed=0, gamma=1.4):
     h = 1.0 / n
     dom = domain(3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (True, True, True), h, 2.2, 0.25, seed)

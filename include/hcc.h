@@ -202,6 +202,20 @@ int hcc_total_mass(hcc_stepper* st, int32_t interior_only, double* out);
 int hcc_solve(hcc_stepper* st, double tmax, double dt, int32_t is_cfl, int32_t snapshots, int32_t remove_ghosts,
               double* t_hist, double* x_hist, double* u_hist, double* mass_hist, int64_t* steps, double* elapsed);
 
+/* ---- ensembles ---------------------------------------------------------------------------------------- */
+/* No counterpart in the reference (one ParticleGrid per stepper); for callers that advance many point clouds of
+ * uneven size (ensemble members, parameter sweeps) in one step. Builds a grid holding copies of n instance grids
+ * (each from hcc_grid_create, with the same D, R, alpha, minimum spacing and wrap lengths; the parts stay usable):
+ * instance k's nodes occupy [offsets[k], offsets[k] + N_k), padded to whole 256-node blocks with inert boundary nodes
+ * (no neighbours, tag 0). Every node keeps its instance's neighbour lists in their order and its volume, so a stepper
+ * on the ensemble computes each instance exactly as a stepper on its own grid. hcc_step gives every instance the
+ * same dt; hcc_step_cfl takes each instance's own CFL step. offsets: n + 1 values (offsets[n] = nodes including the
+ * padding; may be NULL). 1 <= n <= 65535. Not with grid movers. */
+int hcc_grid_concat(hcc_ctx* ctx, int32_t n, hcc_grid* const* parts, hcc_grid** out, int64_t* offsets);
+/* The CFL step (CFL number 1) of every instance of the stepper's grid: n doubles (n = 1 for a grid from
+ * hcc_grid_create). */
+int hcc_time_steps(hcc_stepper* st, double* dts);
+
 #ifdef __cplusplus
 }
 #endif
