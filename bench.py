@@ -93,6 +93,82 @@ def measured_traffic():
     return None
 
 
+def ensemble_main(args):
+    """The batched form: `members` four-quadrant Euler boxes (one spacing h = 1e-3, node counts log-uniform in
+    [n_min, n_max], seeded) in one ensemble grid advanced by one stepper, each member with its own CFL step, against
+    the plain form (a stepper per member, stepped in turn). Both are timed over the same steps; a few members are
+    compared bit for bit afterwards."""
+    import hcc_hip
+    from hcc_hip import Context, Ensemble, Grid, Stepper
+    from hcc_hip import workloads as W
+
+    t_setup = time.perf_counter()
+    wls = W.euler_2d_ensemble(args.members, args.n_min, args.n_max, h=1e-3, seed=args.seed)
+    ctx = Context(0)
+    grids, singles = [], []
+    for wl in wls:
+        c = wl["cloud"]
+        g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+        st = Stepper(ctx, g, wl["scheme"])
+        st.upload(wl["U0"])
+        grids.append(g)
+        singles.append(st)
+    ens = Ensemble(ctx, grids)
+    st_e = Stepper(ctx, ens, wls[0]["scheme"])
+    st_e.upload(ens.pack([wl["U0"] for wl in wls]))
+    nodes = int(ens.sizes.sum())
+    t_setup = time.perf_counter() - t_setup
+    print(f"# setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
+          file=sys.stderr, flush=True)
+    cfl = wls[0]["cfl"]
+
+    def run(steppers, k):
+        for _ in range(k):
+            for st in steppers:
+                st.step_cfl(cfl)
+        hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+
+    run([st_e], args.warmup)
+    run(singles, args.warmup)
+    t0 = time.perf_counter()
+    run([st_e], args.steps)
+    wall_e = time.perf_counter() - t0
+    print(f"# batched: {args.steps} steps in {wall_e:.3f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    run(singles, args.steps)
+    wall_p = time.perf_counter() - t0
+    print(f"# plain: {args.steps} steps in {wall_p:.3f} s", file=sys.stderr, flush=True)
+    U_e = ens.split(st_e.download())
+    for m in sorted({0, len(wls) // 2, len(wls) - 1}):
+        assert np.array_equal(U_e[m], singles[m].download()), f"member {m} differs from its own stepper"
+    stages = 3
+    value = nodes * stages * args.steps / wall_e
+    plain = nodes * stages * args.steps / wall_p
+    out = {
+        "metric": "node-stage updates/s, batched ensemble of 2D Euler MUSCL+MOOD instances",
+        "value": value,
+        "unit": "node-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_e / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(wls)} four-quadrant 2-D Euler boxes, h=1e-3, node counts log-uniform in "
+                               f"[{args.n_min}, {args.n_max}] (seed {args.seed}), each its own seeded IC, jitter and CFL "
+                               f"step; MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4",
+                   "members": len(wls), "nodes": nodes, "ensemble_nodes_with_padding": int(ens.N),
+                   "smallest": int(ens.sizes.min()), "largest": int(ens.sizes.max())},
+        "plain": {"value": plain, "ms_per_step": 1e3 * wall_p / args.steps,
+                  "form": "one stepper per member, stepped in turn (hcc_step_cfl)"},
+        "speedup_vs_plain": value / plain,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +182,12 @@ def main():
                     help="return dt to the host between the CFL estimate and the step (hcc_time_step + hcc_step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for profiles/)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="the batched form: many Euler boxes of uneven size in one ensemble grid, against a stepper "
+                         "per box")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--n-min", type=int, default=20000)
+    ap.add_argument("--n-max", type=int, default=400000)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -114,6 +196,9 @@ def main():
         print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is designed, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.ensemble:
+        ensemble_main(args)
+        return
 
     import hcc_hip
     from hcc_hip import Context, Grid, Stepper

@@ -217,6 +217,70 @@ function hip_set_source!(ts::HIPRKTimeStepper, mode::Integer, rates::AbstractVec
     return ts
 end
 
+"""
+    HIPEnsembleStepper(pgs, descs, sdesc; device = 0)
+
+Advances several ParticleGrids (same D, R, alpha, spacing and wrap lengths) with one device stepper: the instance
+grids are built on the device, concatenated (hcc_grid_concat) and stepped together, each with its own CFL step in
+`hip_step_cfl!`. `ts.offsets[k] + 1 : ts.offsets[k] + pgs[k].meta.N` are instance k's rows of the ensemble state.
+"""
+mutable struct HIPEnsembleStepper
+    ctx::Ptr{Cvoid}
+    parts::Vector{Ptr{Cvoid}}
+    grid::Ptr{Cvoid}
+    st::Ptr{Cvoid}
+    offsets::Vector{Int64}
+    M::Int
+end
+
+function HIPEnsembleStepper(pgs::AbstractVector, descs::AbstractVector{DomainDesc}, sdesc::StepperDesc; device::Integer = 0)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_ctx_create, libhcc), Cint, (Cint, Ref{Ptr{Cvoid}}), device, ctx), C_NULL)
+    parts = Ptr{Cvoid}[]
+    for (pg, desc) in zip(pgs, descs)
+        N = pg.meta.N
+        pos = [pg.core.positions[i][d] for d in 1:desc.D, i in 1:N]
+        isb = UInt8.(pg.core.is_boundary[1:N])
+        tags = Int32.(pg.core.tags[1:N])
+        g = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve pos isb tags checkrc(ccall((:hcc_grid_create, libhcc), Cint,
+            (Ptr{Cvoid}, Ref{DomainDesc}, Int64, Ptr{Float64}, Ptr{UInt8}, Ptr{Int32}, Ref{Ptr{Cvoid}}),
+            ctx[], desc, N, pos, isb, tags, g), ctx[])
+        push!(parts, g[])
+    end
+    offsets = zeros(Int64, length(parts) + 1)
+    grid = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_grid_concat, libhcc), Cint, (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}, Ptr{Int64}),
+                  ctx[], Int32(length(parts)), parts, grid, offsets), ctx[])
+    st = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_stepper_create, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{StepperDesc}, Ref{Ptr{Cvoid}}),
+                  ctx[], grid[], sdesc, st), ctx[])
+    ts = HIPEnsembleStepper(ctx[], parts, grid[], st[], offsets, Int(sdesc.M))
+    U = zeros(Float64, ts.M * offsets[end])
+    for (k, pg) in enumerate(pgs), i in 1:pg.meta.N, m in 1:ts.M
+        U[(offsets[k] + i - 1) * ts.M + m] = pg.rhos[i][m]
+    end
+    checkrc(ccall((:hcc_state_upload, libhcc), Cint, (Ptr{Cvoid}, Ptr{Float64}), ts.st, U), ts.ctx)
+    finalizer(ts) do t
+        ccall((:hcc_stepper_destroy, libhcc), Cint, (Ptr{Cvoid},), t.st)
+        ccall((:hcc_grid_destroy, libhcc), Cint, (Ptr{Cvoid},), t.grid)
+        foreach(p -> ccall((:hcc_grid_destroy, libhcc), Cint, (Ptr{Cvoid},), p), t.parts)
+        ccall((:hcc_ctx_destroy, libhcc), Cint, (Ptr{Cvoid},), t.ctx)
+    end
+    return ts
+end
+
+"One step of every instance with dt_k = min(cfl * CFL step of instance k, dt_cap), computed on the device."
+hip_step_cfl!(ts::HIPEnsembleStepper, cfl::Real; dt_cap::Real = Inf) =
+    checkrc(ccall((:hcc_step_cfl, libhcc), Cint, (Ptr{Cvoid}, Float64, Float64), ts.st, cfl, dt_cap), ts.ctx)
+
+"The CFL step of every instance (CFL number 1)."
+function hip_time_steps(ts::HIPEnsembleStepper)
+    dts = zeros(Float64, length(ts.parts))
+    checkrc(ccall((:hcc_time_steps, libhcc), Cint, (Ptr{Cvoid}, Ptr{Float64}), ts.st, dts), ts.ctx)
+    return dts
+end
+
 "Copies the device grid's positions (moved and wrapped by a moving stepper) into `pg.core.positions`."
 function download_positions!(ts::HIPRKTimeStepper, pg; D::Integer = length(pg.core.positions[1]))
     x = zeros(Float64, D * ts.N)
@@ -227,7 +291,7 @@ function download_positions!(ts::HIPRKTimeStepper, pg; D::Integer = length(pg.co
     return pg
 end
 
-export hip_set_source!, hip_mover_code, download_positions!
+export hip_set_source!, hip_mover_code, download_positions!, HIPEnsembleStepper, hip_step_cfl!, hip_time_steps
 export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
        hip_volumes, hip_total_mass, hip_solve_equation
 
