@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhcc.so")
-SOURCES = ["hcc_runtime.hip", "hcc_step.hip", "hcc_host.cpp"]
+SOURCES = ["hcc_runtime.hip", "hcc_step.hip", "hcc_mlp.hip", "hcc_host.cpp"]
 ARCH = os.environ.get("HCC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,10 @@
+// Internal declarations of the troubled-node indicator network (the C ABI is in include/hcc.h).
+#pragma once
+#include "hcc_internal.h"
+
+namespace hcc {
+constexpr int kMlpIn = 16, kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden + kMlpHidden + 1;
+// Logits of n bf16 feature rows (16 per row) on the device, queued on the network's context stream.
+void mlp_forward_device(const hcc_mlp* m, const uint16_t* X, int64_t n, float* z);
+}  // namespace hcc

@@ -31,8 +31,10 @@ EXPORTS = [
     "hcc_state_download", "hcc_time_step", "hcc_step", "hcc_get_orders", "hcc_get_mood_flags", "hcc_get_inadmissible",
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
     "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve", "hcc_set_source",
-    "hcc_grid_concat", "hcc_time_steps",
+    "hcc_grid_concat", "hcc_time_steps", "hcc_mlp_create", "hcc_mlp_destroy", "hcc_mlp_set_params",
+    "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_train",
 ]
+MLP_PARAMS = 5313
 
 
 class DomainDesc(C.Structure):
@@ -101,6 +103,13 @@ def lib():
         "hcc_set_source": (C.c_int, [vp, C.c_int32, vp]),
         "hcc_grid_concat": (C.c_int, [vp, C.c_int32, vp, P(vp), vp]),
         "hcc_time_steps": (C.c_int, [vp, vp]),
+        "hcc_mlp_create": (C.c_int, [vp, P(vp)]),
+        "hcc_mlp_destroy": (C.c_int, [vp]),
+        "hcc_mlp_set_params": (C.c_int, [vp, vp]),
+        "hcc_mlp_get_params": (C.c_int, [vp, vp]),
+        "hcc_mlp_forward": (C.c_int, [vp, vp, i64, vp]),
+        "hcc_mlp_gradients": (C.c_int, [vp, vp, vp, i64, C.c_float, vp, P(d)]),
+        "hcc_mlp_train": (C.c_int, [vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float, C.c_float, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -249,6 +258,71 @@ class Grid:
     def close(self):
         if getattr(self, "h", None):
             lib().hcc_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bf16_rows(X):
+    """Feature rows as bfloat16 bit patterns (round to nearest even), 16 per row."""
+    f = np.ascontiguousarray(X, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    return np.ascontiguousarray((((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16))
+
+
+class MLP:
+    """The troubled-node indicator network on the device (hcc_mlp_*): 16 -> 64 -> 64 -> 1, bf16 operands on the
+    matrix cores, fp32 parameters and accumulation."""
+
+    def __init__(self, ctx, params=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().hcc_mlp_create(ctx.h, C.byref(h)), ctx.h, "hcc_mlp_create")
+        self.h = h
+        ctx._grids.add(self)
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_PARAMS
+        check(lib().hcc_mlp_set_params(self.h, ptr(p)), self.ctx.h, "hcc_mlp_set_params")
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        check(lib().hcc_mlp_get_params(self.h, ptr(p)), self.ctx.h, "hcc_mlp_get_params")
+        return p
+
+    def forward(self, X):
+        Xb = bf16_rows(X) if X.dtype != np.uint16 else np.ascontiguousarray(X)
+        z = np.zeros(Xb.shape[0], np.float32)
+        check(lib().hcc_mlp_forward(self.h, ptr(Xb), Xb.shape[0], ptr(z)), self.ctx.h, "hcc_mlp_forward")
+        return z
+
+    def gradients(self, X, y, pos_weight=1.0):
+        Xb = bf16_rows(X) if X.dtype != np.uint16 else np.ascontiguousarray(X)
+        yb = np.ascontiguousarray(y, np.uint8)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        check(lib().hcc_mlp_gradients(self.h, ptr(Xb), ptr(yb), Xb.shape[0], float(pos_weight), ptr(g), C.byref(loss)),
+              self.ctx.h, "hcc_mlp_gradients")
+        return loss.value, g
+
+    def train(self, X, y, epochs, batch, lr, pos_weight=1.0):
+        Xb = bf16_rows(X) if X.dtype != np.uint16 else np.ascontiguousarray(X)
+        yb = np.ascontiguousarray(y, np.uint8)
+        losses = np.zeros(epochs)
+        check(lib().hcc_mlp_train(self.h, ptr(Xb), ptr(yb), Xb.shape[0], int(epochs), int(batch), float(lr),
+                                  float(pos_weight), ptr(losses)), self.ctx.h, "hcc_mlp_train")
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hcc_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -216,6 +216,30 @@ int hcc_grid_concat(hcc_ctx* ctx, int32_t n, hcc_grid* const* parts, hcc_grid** 
  * hcc_grid_create). */
 int hcc_time_steps(hcc_stepper* st, double* dts);
 
+/* ---- learned troubled-node indicator --------------------------------------------------------------------- */
+/* No counterpart in the reference (its MOOD is a posteriori only). A small network predicts, before a stage's flux
+ * pass, which particles MOOD will drop to first order, so that they take it at once and fewer fixed-point sweeps
+ * follow; MOOD's a posteriori check still runs on every candidate and catches what the network misses. The network:
+ * 16 features -> 64 -> 64 -> 1 logit, GELU (tanh form), bf16 weights and activations, fp32 accumulation, forward and
+ * backward on the matrix cores. Parameters: HCC_MLP_PARAMS floats, W1[16][64], b1[64], W2[64][64], b2[64], w3[64],
+ * b3. Feature rows are 16 bf16 values (the upper 16 bits of the float), labels 0/1 bytes. */
+#define HCC_MLP_PARAMS 5313
+typedef struct hcc_mlp hcc_mlp;
+int hcc_mlp_create(hcc_ctx* ctx, hcc_mlp** out);
+int hcc_mlp_destroy(hcc_mlp* m);
+int hcc_mlp_set_params(hcc_mlp* m, const float* params);   /* also resets the optimiser state */
+int hcc_mlp_get_params(hcc_mlp* m, float* params);
+/* logits z[n] of n feature rows */
+int hcc_mlp_forward(hcc_mlp* m, const uint16_t* X, int64_t n, float* z);
+/* gradient (HCC_MLP_PARAMS floats) and value of mean_s w_s (softplus(z_s) - y_s z_s), w_s = pos_weight where
+ * y_s = 1, else 1 */
+int hcc_mlp_gradients(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight, float* grads,
+                      double* loss);
+/* epochs of Adam (beta 0.9 / 0.999, eps 1e-8) over mini-batches of `batch` rows in the given order; losses (may be
+ * NULL): the mean batch loss of each epoch */
+int hcc_mlp_train(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, int32_t epochs, int32_t batch, float lr,
+                  float pos_weight, double* losses);
+
 #ifdef __cplusplus
 }
 #endif
