@@ -133,6 +133,9 @@ struct StepArgs {
     const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate); one per
                                  // instance on an ensemble grid
     const uint16_t* inst;        // ensemble grid: the instance of each kStage block (nullptr: one instance)
+    uint16_t* feat;              // indicator features of the full gathers: 16 bf16 per node
+    int32_t* ind_list;           // the indicator's candidates of the last full gather: particles at the full order
+    unsigned long long* ind_n;   // whose neighbourhood is not constant (MOOD's DMP test can flag no other), and count
     double hratio;               // MOOD substep: h = hratio * dt
     int ale;                     // HCC_MOVER_*: flux in the frame of the particle's grid velocity w (generic flux loop)
     double mover_alpha;          // ScaledLagrangianMover: w = mover_alpha x fluid velocity

@@ -38,13 +38,15 @@ struct alignas(16) MlpW {
     float b1[H], b2[H], b3;
 };
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1): one exponential and one reciprocal (saturates correctly at both ends)
+__device__ __forceinline__ float tanh_e(float u) { return 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f); }
 __device__ __forceinline__ float gelu(float x) {
     const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return 0.5f * x * (1.0f + tanhf(u));
+    return 0.5f * x * (1.0f + tanh_e(u));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
     const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    const float t = tanhf(u);
+    const float t = tanh_e(u);
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
 }
 __device__ __forceinline__ float softplus(float z) { return z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z)); }
@@ -85,10 +87,12 @@ __device__ __forceinline__ void load_weights(MlpW& w, const MlpW* gw) {
     for (int t = threadIdx.x; t < (int)(sizeof(MlpW) / 16); t += blockDim.x) d[t] = s[t];
 }
 
-// Forward of the tile in lds.x: a1 / a2 hold the pre-activations (result layout), h2 the layer-2 activations as bf16
-// values, lds.z the logits. Every lane of the wave takes part.
-__device__ __forceinline__ void forward_tile(const MlpW& w, TileLds& t, int l, float (&a1)[2][16], float (&a2)[2][16],
-                                             float (&h2)[2][16]) {
+// Forward of the tile in x ([32][16]): a1 / a2 hold the pre-activations (result layout), h2 the layer-2 activations as
+// bf16 values, z the logits; the layer-1 activations go to h1 ([32][64]) and, for training, h1t ([64][32]). Every
+// lane of the wave takes part.
+template <class Tile>
+__device__ __forceinline__ void forward_tile(const MlpW& w, Tile& t, __bf16* h1t, int l, float (&a1)[2][16],
+                                             float (&a2)[2][16], float (&h2)[2][16]) {
     const bf16x8 ax = lds8(&t.x[l % 32][8 * (l / 32)]);
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
@@ -97,7 +101,7 @@ __device__ __forceinline__ void forward_tile(const MlpW& w, TileLds& t, int l, f
         float h[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) { a1[nt][q] = c[q] + w.b1[nt * 32 + l % 32]; h[q] = gelu(a1[nt][q]); }
-        store_tile(h, l, nt * 32, &t.h1[0][0], H, &t.h1t[0][0]);
+        store_tile(h, l, nt * 32, &t.h1[0][0], H, h1t);
     }
     __syncthreads();
 #pragma unroll
@@ -127,31 +131,50 @@ __device__ __forceinline__ void forward_tile(const MlpW& w, TileLds& t, int l, f
     __syncthreads();
 }
 
-// Loads tile `tile` of X (n samples) into lds.x / lds.xt; samples past n are zero.
-__device__ __forceinline__ void load_x(TileLds& t, const __bf16* X, int64_t n, int64_t tile, int l) {
+// Loads tile `tile` of X (n samples; sample s is row idx[s] when idx is given) into t.x and, if xt, transposed;
+// samples past n are zero.
+template <class Tile>
+__device__ __forceinline__ void load_x(Tile& t, __bf16 (*xt)[T], const __bf16* X, int64_t n, int64_t tile, int l,
+                                       const int32_t* idx = nullptr) {
     const int r = l % 32, h = l / 32;
     const int64_t s = tile * T + r;
     bf16x8 v = {};
-    if (s < n) v = *reinterpret_cast<const bf16x8*>(X + s * F + 8 * h);
+    if (s < n) v = *reinterpret_cast<const bf16x8*>(X + (idx ? (int64_t)idx[s] : s) * F + 8 * h);
     *reinterpret_cast<bf16x8*>(&t.x[r][8 * h]) = v;
+    if (xt)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) t.xt[8 * h + e][r] = v[e];
+        for (int e = 0; e < 8; ++e) xt[8 * h + e][r] = v[e];
     __syncthreads();
 }
 
-// Logits of n samples: one wave per workgroup, tiles grid-strided.
-__global__ __launch_bounds__(64) void k_mlp_forward(const MlpW* __restrict__ gw, const __bf16* __restrict__ X, int64_t n,
-                                                    float* __restrict__ z) {
+// the LDS of an inference tile: inputs and layer-1 activations by sample
+struct alignas(16) FwdLds {
+    __bf16 x[T][F];
+    __bf16 h1[T][H];
+    float z[T];
+};
+
+// Logits of n samples: four waves per workgroup share the weights, each wave a tile of its own; the workgroup steps
+// over groups of four tiles, so every wave reaches each barrier (a tile past the end computes on zeros, unstored).
+// With idx, sample s is feature row idx[s] and n is read from *n_dev.
+constexpr int kFwdWaves = 4;
+__global__ __launch_bounds__(64 * kFwdWaves) void k_mlp_forward(const MlpW* __restrict__ gw, const __bf16* __restrict__ X,
+                                                                int64_t n_host, const int32_t* __restrict__ idx,
+                                                                const unsigned long long* __restrict__ n_dev,
+                                                                float* __restrict__ z) {
     __shared__ MlpW w;
-    __shared__ TileLds t;
+    __shared__ FwdLds tiles[kFwdWaves];
     load_weights(w, gw);
     __syncthreads();
-    const int l = threadIdx.x;
+    const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
+    FwdLds& t = tiles[wv];
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int64_t ntile = (n + T - 1) / T;
-    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        load_x(t, X, n, tile, l);
+    for (int64_t base = (int64_t)blockIdx.x * kFwdWaves; base < ntile; base += (int64_t)gridDim.x * kFwdWaves) {
+        const int64_t tile = base + wv;
+        load_x(t, nullptr, X, n, tile, l, idx);
         float a1[2][16], a2[2][16], h2[2][16];
-        forward_tile(w, t, l, a1, a2, h2);
+        forward_tile(w, t, nullptr, l, a1, a2, h2);
         if (l < T && tile * T + l < n) z[tile * T + l] = t.z[l];
         __syncthreads();
     }
@@ -176,9 +199,9 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const MlpW* __restrict__ gw, co
     float gb1[2] = {}, gb2[2] = {}, gw3[2] = {}, gb3 = 0.0f, loss = 0.0f;
     const int64_t ntile = (n + T - 1) / T;
     for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        load_x(t, X, n, tile, l);
+        load_x(t, t.xt, X, n, tile, l);
         float a1[2][16], a2[2][16], h2[2][16];
-        forward_tile(w, t, l, a1, a2, h2);
+        forward_tile(w, t, &t.h1t[0][0], l, a1, a2, h2);
         // dL/dz per row of this lane's result registers; rows past n weigh 0
         float dz[16];
 #pragma unroll
@@ -322,13 +345,21 @@ struct hcc_mlp {
 };
 
 namespace hcc {
-const void* mlp_weights(const hcc_mlp* m) { return m->w; }
+hcc_ctx* mlp_context(const hcc_mlp* m) { return m->ctx; }
 // Logits of n feature rows already on the device (the stepper's indicator pass), queued on the context stream.
 void mlp_forward_device(const hcc_mlp* m, const uint16_t* X, int64_t n, float* z) {
     if (n <= 0) return;
-    const int64_t ntile = (n + T - 1) / T;
-    const unsigned nb = (unsigned)std::min<int64_t>(ntile, 8192);
-    k_mlp_forward<<<nb, 64, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), n, z);
+    const int64_t ngroup = ((n + T - 1) / T + kFwdWaves - 1) / kFwdWaves;
+    const unsigned nb = (unsigned)std::min<int64_t>(ngroup, 4096);
+    k_mlp_forward<<<nb, 64 * kFwdWaves, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), n, nullptr,
+                                                             nullptr, z);
+}
+void mlp_forward_list(const hcc_mlp* m, const uint16_t* X, const int32_t* idx, const unsigned long long* n_dev,
+                      int64_t n_max, float* z) {
+    if (n_max <= 0) return;
+    const int64_t ngroup = ((n_max + T - 1) / T + kFwdWaves - 1) / kFwdWaves;
+    const unsigned nb = (unsigned)std::min<int64_t>(ngroup, 1024);
+    k_mlp_forward<<<nb, 64 * kFwdWaves, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), 0, idx, n_dev, z);
 }
 }  // namespace hcc
 

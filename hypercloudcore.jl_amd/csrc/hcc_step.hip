@@ -8,6 +8,7 @@
 // slot order; the loads of slot k+1 are issued before slot k is computed (index two slots ahead), so the gather
 // latency overlaps the arithmetic of the previous slot. Every per-node sum keeps the reference's operation order.
 #include "hcc_device.h"
+#include "hcc_mlp.h"
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -381,9 +382,45 @@ __global__ __launch_bounds__(kBlock) void k_predict_imex(StepArgs a, int64_t nbl
 // runs for every interior particle; one without a gradient fit discards the fit sums.
 // CFL: also the node's CFL step of get_time_step (the sums of k_cfl over the same slots, with the wave speeds staged
 // in LDS), into *dtc (infinity where k_cfl skips the node).
-template <int D, int M, int KIND, bool EXT, bool CFL = false, bool CUT = false, class Src>
+// The 16 indicator features of a 2-D Euler node from its gather (build-defined): per component m with the scale s_m
+// (|rho|, |E|, |rho| (|u| + c) for the momenta) the neighbourhood range (max - min) / s_m, the skew
+// (max + min - 2 U_m) / s_m and the limited gradient norm h |grad U_m| / s_m; the Mach number |u| / c, p / E, the
+// smallest limiter factor and the density's. Stored as bf16.
+template <int D, int M, int KIND>
+constexpr bool kFeat = KIND == HCC_PDE_EULER && D == 2 && M == 4;
+
+template <int D, int M>
+__device__ __forceinline__ void store_features(const StepArgs& a, int64_t i, const double* u, const double* umax,
+                                               const double* umin, const double* g, const double* phi) {
+    static_assert(M == 4 && D == 2, "2-D Euler features");
+    const double rho = u[0], E = u[3];
+    const double ux = u[1] / rho, uy = u[2] / rho;
+    const double p = (a.pde.gamma - 1.0) * (E - 0.5 * (u[1] * ux + u[2] * uy));
+    const double c = p > 0.0 && rho > 0.0 ? sqrt(a.pde.gamma * p / rho) : 0.0;
+    const double speed = sqrt(ux * ux + uy * uy);
+    const double tiny = 1e-300;
+    const double sc[4] = {fmax(fabs(rho), tiny), fmax(fabs(rho) * (speed + c), tiny), fmax(fabs(rho) * (speed + c), tiny),
+                          fmax(fabs(E), tiny)};
+    const double h = 1.0 / a.g.invL;
+    float f[16];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f[m] = (float)((umax[m] - umin[m]) / sc[m]);
+        f[4 + m] = (float)((umax[m] + umin[m] - 2.0 * u[m]) / sc[m]);
+        f[8 + m] = (float)(h * sqrt(g[m] * g[m] + g[M + m] * g[M + m]) / sc[m]);
+    }
+    f[12] = (float)(c > 0.0 ? speed / c : 10.0);
+    f[13] = (float)(p / fmax(fabs(E), tiny));
+    f[14] = (float)fmin(fmin(phi[0], phi[1]), fmin(phi[2], phi[3]));
+    f[15] = (float)phi[0];
+    __bf16* dst = reinterpret_cast<__bf16*>(a.feat) + i * 16;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dst[q] = (__bf16)(isfinite(f[q]) ? fminf(fmaxf(f[q], -1e4f), 1e4f) : 1e4f);
+}
+
+template <int D, int M, int KIND, bool EXT, bool CFL = false, bool CUT = false, bool FEAT = false, class Src>
 __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i,
-                                            double* dtc = nullptr) {
+                                            double* dtc = nullptr, bool* cand = nullptr) {
     using L = Layout<D, M>;
     constexpr int NR = L::NR;
     const int64_t N = a.g.N;
@@ -399,7 +436,10 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
     double ri[NR], umax[M], umin[M], b[D][M];
     constexpr int BD = D + 1;
     double X[CFL ? D : 1][BD], li[D], sum_c = 0.0;
-    if (fit || (EXT && a.ext_on) || (CFL && cnt > 0)) {
+    double phi_f[M];   // the limiter factors (features)
+#pragma unroll
+    for (int m = 0; m < M; ++m) phi_f[m] = 1.0;
+    if (fit || (EXT && a.ext_on) || (CFL && cnt > 0) || FEAT) {
         load_rec<NR, L::PU>(Y, i, ri);
         if constexpr (CFL) {
             double f[M];
@@ -418,9 +458,11 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         for (int q = 0; q < D; ++q)
 #pragma unroll
             for (int m = 0; m < M; ++m) b[q][m] = 0.0;
-        int32_t j1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
+        // a node without neighbours reads no slot (src_slot would index slot -1)
+        int32_t j1 = cnt > 0 ? src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt) : 0;
         double rn[NR];
-        if constexpr (Src::kPrefetch) src.template head<NR>(src_slot(src, base, 0, cnt), rn);
+        if constexpr (Src::kPrefetch)
+            if (cnt > 0) src.template head<NR>(src_slot(src, base, 0, cnt), rn);
         static_assert(!(CFL && Src::kPrefetch), "the fused CFL estimate reads the staged wave speeds");
         for (int k = 0; k < cnt; ++k) {
             double rj[NR];
@@ -501,7 +543,8 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         if (a.limiter != HCC_LIM_NONE) {
             // per component, for either bound (local max when the reconstruction rises, min when it falls): the
             // bound minus U_i and the tolerance 16 eps max(|U_i|, |bound|), the same operations hoisted out of the loop
-            double phi[M], dup[M], ddn[M], tup[M], tdn[M];
+            double* phi = phi_f;
+            double dup[M], ddn[M], tup[M], tdn[M];
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 phi[m] = 1.0;
@@ -562,6 +605,31 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         }
     }
     store_rec<L::NG, L::GS>(a.G, i, g);
+    if constexpr (FEAT && kFeat<D, M, KIND>) {   // ri and the extrema were formed above (FEAT enters that block)
+        double u[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) u[m] = ri[D + m];
+        store_features<D, M>(a, i, u, umax, umin, g, phi_f);
+        bool varies = false;
+#pragma unroll
+        for (int m = 0; m < M; ++m) varies = varies || umax[m] > umin[m];
+        if (cand) *cand = order == 2 && varies;
+    }
+}
+
+// Appends the wave's indicator candidates to a.cand (one atomic per wave); every lane of the wave calls it.
+__device__ __forceinline__ void append_cand(const StepArgs& a, bool c, int64_t i) {
+    const unsigned long long ball = __ballot(c);
+    if (!ball || !a.ind_list) return;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.ind_n, (unsigned long long)__popcll(ball));
+    base = __shfl(base, (int)leader);
+    if (c) {
+        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        a.ind_list[base + __popcll(below)] = (int32_t)i;
+    }
 }
 
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
@@ -589,23 +657,26 @@ __device__ __forceinline__ void block_min(const StepArgs& a, double dt, int64_t 
     }
 }
 
-// CUT: the stage geometry of a moving grid (slot weights 0 beyond R)
-template <int D, int M, int KIND, bool CUT>
+// CUT: the stage geometry of a moving grid (slot weights 0 beyond R); FEAT: the full pass stores the indicator
+// features
+template <int D, int M, int KIND, bool CUT, bool FEAT = false>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
                                                     int64_t nblk) {
     if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t i = node_index(nblk);
     if (reset) step_reset(a, i);
-    if (i >= a.g.N) return;
-    if (a.g.isb[i] || !(full || a.changed[i])) return;
-    if (full) gather_node<D, M, KIND, true, false, CUT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
-    else gather_node<D, M, KIND, false, false, CUT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    bool cand = false;
+    if (i < a.g.N && !a.g.isb[i] && (full || a.changed[i])) {
+        if (full) gather_node<D, M, KIND, true, false, CUT, FEAT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i, nullptr, &cand);
+        else gather_node<D, M, KIND, false, false, CUT>(a, Y, HbmSrc<D, M>{Y, a.G, a.orders, a.g.nbr}, i);
+    }
+    if (FEAT && full) append_cand(a, cand, i);
 }
 
 // The full gather pass with the workgroup's halo staged in LDS (2-D).
 // CFL: the CFL estimate of the state being gathered (stage 1 of a step whose state is U) fused in: per-workgroup
 // minima into bmin, reduced by k_cfl_final.
-template <int D, int M, int KIND, bool CFL>
+template <int D, int M, int KIND, bool CFL, bool FEAT = false>
 __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
                                                            int64_t nblk) {
     extern __shared__ double sh[];
@@ -616,7 +687,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (reset) step_reset(a, i);
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
     double dtc = INFINITY;
-    if (i < a.g.N && !a.g.isb[i]) gather_node<D, M, KIND, true, CFL>(a, Y, src, i, &dtc);
+    bool cand = false;
+    if (i < a.g.N && !a.g.isb[i]) gather_node<D, M, KIND, true, CFL, false, FEAT>(a, Y, src, i, &dtc, &cand);
+    if (FEAT) append_cand(a, cand, i);
     if (CFL) block_min(a, dtc, lb);
 }
 
@@ -1758,6 +1831,24 @@ __global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ b
     }
 }
 
+// The indicator's decision before a stage's flux pass: the candidates (interior particles at the full order with a
+// non-constant neighbourhood) whose logit exceeds the threshold take first order now (MOOD's check of the stage
+// candidate still follows). z[t] is the logit of candidate t; grid-strided over the device-side count.
+__global__ __launch_bounds__(kBlock) void k_apply_indicator(int8_t* __restrict__ orders, const int32_t* __restrict__ list,
+                                                             const unsigned long long* __restrict__ list_n,
+                                                             const float* __restrict__ z, float thr,
+                                                             unsigned long long* count) {
+    const int64_t n = (int64_t)*list_n;
+    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t t = t0 + threadIdx.x;
+        const bool hit = t < n && z[t] > thr;
+        if (hit) orders[list[t]] = 1;
+        const unsigned long long b = __ballot(hit);
+        if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)b) - 1))
+            atomicAdd(count, (unsigned long long)__popcll(b));
+    }
+}
+
 __global__ void k_set_dt(double* dtp, double dt, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) dtp[k] = dt;
 }
@@ -1934,6 +2025,18 @@ struct hcc_stepper {
     int src_mode = HCC_SRC_NONE;
     bool imex = false;             // stages formed by k_predict_imex
     bool moving = false;           // a grid mover: stage positions, stage geometry, a search after each step
+    // learned troubled-node indicator (hcc_set_indicator) and the collection of its training data
+    bool feat_on = false;          // the full gathers store the indicator features (2-D Euler)
+    const hcc_mlp* ind = nullptr;  // the network, nullptr: off
+    float ind_thr = 0.0f;          // logit threshold
+    uint16_t* feat = nullptr;      // N x 16 bf16 features of the last full gather
+    float* ind_z = nullptr;        // logits
+    unsigned long long* ind_cnt = nullptr;   // particles the indicator lowered in the last step
+    int32_t* ind_list = nullptr;   // the indicator's candidates of the last full gather (N) and their count
+    unsigned long long* ind_n = nullptr;
+    bool collect = false;
+    uint16_t* cfeat = nullptr;     // stages x N x 16: each stage's features (last step)
+    int8_t* cord = nullptr;        // 2 x stages x N: orders before each stage's flux pass, after its checkpoint
     double* KX[kMaxStages] = {};   // K_X of the stages (moving): the grid velocity w per interior particle, D per node
     double* G = nullptr;
     double* scratch = nullptr; // N * M doubles for up/downloads
@@ -2019,6 +2122,9 @@ StepArgs hcc_stepper::args() const {
     a.ext_on = (mood_fused(desc) && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
     a.dtp = dtp;
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
+    a.feat = feat;
+    a.ind_list = ind_list;
+    a.ind_n = ind_n;
     a.hratio = desc.substep_ratio;
     if (moving) {
         a.ale = desc.mover;
@@ -2035,6 +2141,14 @@ struct Variant {
     static hipStream_t st(const hcc_stepper* s) { return s->ctx->stream; }
     static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full, int reset) {
         const int64_t nb = nblocks(s->N);
+        if constexpr (kFeat<D, M, KIND>) {
+            if (full && s->feat_on) {   // the full pass with the indicator features
+                if (s->lds_gather)
+                    k_gather_staged<D, M, KIND, false, true><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
+                else k_gather<D, M, KIND, false, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+                return;
+            }
+        }
         if (full && s->lds_gather)
             k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else if (s->moving) k_gather<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
@@ -2048,7 +2162,14 @@ struct Variant {
     // the full gather of stage 1 with the CFL estimate of its state fused in, then dt = min(cfl * dt_cfl, cap)
     static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
-        k_gather_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
+        bool done = false;
+        if constexpr (kFeat<D, M, KIND>) {
+            if (s->feat_on) {   // with the indicator features
+                k_gather_staged<D, M, KIND, true, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
+                done = true;
+            }
+        }
+        if (!done) k_gather_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
         cfl_final(s, s->dtp, cfl, cap);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
@@ -2191,7 +2312,8 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->cand, (void*)s->G, (void*)s->scratch, (void*)s->orders, (void*)s->changed, (void*)s->dirty,
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->clist, (void*)s->clist2, (void*)s->dlist,
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
-                    (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters})
+                    (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
+                    (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
         (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); (void)hipFree(s->KX[k]);
@@ -2522,6 +2644,27 @@ struct StepRunner {
             ops.stage_pos(s, a, t, target);
         }
     }
+    // the orders into slot k of the collection (k < stages: before stage k+1's flux pass; else after the checkpoint of
+    // stage k - stages + 1)
+    void snap_orders(int k) {
+        (void)hipMemcpyAsync(s->cord + (int64_t)k * s->N, s->orders, s->N, hipMemcpyDeviceToDevice, s->ctx->stream);
+    }
+    // after stage i's full gather (features stored): the collection's copies, and the indicator's decision
+    hipError_t after_gather(int i) {
+        hipStream_t str = s->ctx->stream;
+        if (s->collect) {
+            hipError_t e = hipMemcpyAsync(s->cfeat + (int64_t)(i - 1) * s->N * 16, s->feat, sizeof(uint16_t) * 16 * s->N,
+                                          hipMemcpyDeviceToDevice, str);
+            if (e != hipSuccess) return e;
+            snap_orders(i - 1);
+        }
+        if (s->ind) {   // the network over the candidates only (their count stays on the device)
+            mlp_forward_list(s->ind, s->feat, s->ind_list, s->ind_n, s->N, s->ind_z);
+            k_apply_indicator<<<(unsigned)std::min<int64_t>(nblocks(s->N), 2048), kBlock, 0, str>>>(
+                s->orders, s->ind_list, s->ind_n, s->ind_z, s->ind_thr, s->ind_cnt);
+        }
+        return hipGetLastError();
+    }
     // _apply_stage_geometry! (TimeSteppers.jl:520-524) with the frozen topology: the stage positions (held by the
     // stage state's records) into the grid, and the geometry caches of the stage geometry
     void apply_stage_geometry(const double* Y) {
@@ -2631,18 +2774,31 @@ struct StepRunner {
         // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
         // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
         // particles). The full gather also clears the changed count for the check of the flux pass.
+        if (s->ind) {
+            hipError_t e = hipMemsetAsync(s->ind_cnt, 0, sizeof(unsigned long long), str);
+            if (e != hipSuccess) return e;
+        }
         for (int i = 1; i <= st; ++i) {
             if (i == 1) {
                 if (!y1_in_U) predict(1);   // Y_1 = pg.rhos with the boundary conditions applied
             } else {
                 hipError_t e = checkpoint(i);
                 if (e != hipSuccess) return e;
+                if (s->collect) snap_orders(st + i - 2);   // after stage i-1's checkpoint
                 if (s->moving) apply_stage_geometry(stage_state(i));
             }
             const double* Y = stage_state(i);
             (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
+            if (s->feat_on) {
+                hipError_t e = hipMemsetAsync(s->ind_n, 0, sizeof(unsigned long long), str);
+                if (e != hipSuccess) return e;
+            }
             if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else ops.gather(s, a, Y, 1, i == 1);
+            if (s->feat_on) {
+                hipError_t e = after_gather(i);
+                if (e != hipSuccess) return e;
+            }
             (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
             ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
@@ -2650,6 +2806,7 @@ struct StepRunner {
         }
         hipError_t e = checkpoint(st + 1);
         if (e != hipSuccess) return e;
+        if (s->collect) snap_orders(2 * st - 1);
         std::swap(s->U, s->cand);        // pg.rhos <- rho_cand (its ghosts hold their roots' values)
         s->fresh = false;
         return hipGetLastError();
@@ -3039,5 +3196,89 @@ extern "C" int hcc_set_source(hcc_stepper* s, int32_t mode, const double* rates)
     }
     s->src_mode = mode;
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HCC_OK;
+}
+
+// ============================================================================================================
+// learned troubled-node indicator: features, decision before the flux pass, training data
+// ============================================================================================================
+static int feature_buffers(hcc_stepper* s) {
+    hcc_ctx* ctx = s->ctx;
+    if (s->D != 2 || s->M != 4 || s->kind != HCC_PDE_EULER)
+        return set_err(ctx, HCC_E_UNSUPPORTED, "the indicator features are defined for 2-D Euler");
+    if (!s->feat) {
+        HCC_TRY_HIP(ctx, dalloc(&s->feat, 16 * s->N));
+        HCC_TRY_HIP(ctx, hipMemset(s->feat, 0, sizeof(uint16_t) * std::max<int64_t>(16 * s->N, 1)));
+        HCC_TRY_HIP(ctx, dalloc(&s->ind_list, s->N));
+        HCC_TRY_HIP(ctx, dalloc(&s->ind_n, 1));
+        HCC_TRY_HIP(ctx, hipMemset(s->ind_n, 0, sizeof(unsigned long long)));
+    }
+    return HCC_OK;
+}
+
+extern "C" int hcc_set_indicator(hcc_stepper* s, const hcc_mlp* m, float threshold) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    if (m) {
+        if (mlp_context(m) != ctx)
+            return set_err(ctx, HCC_E_ARG, "hcc_set_indicator: the network belongs to another context (another stream)");
+        if (int rc = feature_buffers(s)) return rc;
+        if (!s->ind_z) HCC_TRY_HIP(ctx, dalloc(&s->ind_z, s->N));
+        if (!s->ind_cnt) {
+            HCC_TRY_HIP(ctx, dalloc(&s->ind_cnt, 1));
+            HCC_TRY_HIP(ctx, hipMemset(s->ind_cnt, 0, sizeof(unsigned long long)));
+        }
+    }
+    s->ind = m;
+    s->ind_thr = threshold;
+    s->feat_on = s->ind || s->collect;
+    return HCC_OK;
+}
+
+extern "C" int hcc_indicator_collect(hcc_stepper* s, int32_t on) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    if (on) {
+        if (int rc = feature_buffers(s)) return rc;
+        const int st = s->desc.stages;
+        if (!s->cfeat) {
+            HCC_TRY_HIP(ctx, dalloc(&s->cfeat, (int64_t)st * 16 * s->N));
+            HCC_TRY_HIP(ctx, dalloc(&s->cord, (int64_t)2 * st * s->N));
+            HCC_TRY_HIP(ctx, hipMemset(s->cord, 0, std::max<int64_t>((int64_t)2 * st * s->N, 1)));
+        }
+    }
+    s->collect = on != 0;
+    s->feat_on = s->ind || s->collect;
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_indicator_samples(hcc_stepper* s, int32_t stage, uint16_t* feats, int8_t* pre, int8_t* post) {
+    if (!s || !s->cfeat || stage < 0 || stage >= s->desc.stages)
+        return set_err(s ? s->ctx : nullptr, HCC_E_ARG, "hcc_get_indicator_samples: no collection or bad stage");
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t N = s->N;
+    const int st = s->desc.stages;
+    if (feats) HCC_TRY_HIP(ctx, hipMemcpy(feats, s->cfeat + (int64_t)stage * 16 * N, sizeof(uint16_t) * 16 * N, hipMemcpyDeviceToHost));
+    if (pre) HCC_TRY_HIP(ctx, hipMemcpy(pre, s->cord + (int64_t)stage * N, N, hipMemcpyDeviceToHost));
+    if (post) HCC_TRY_HIP(ctx, hipMemcpy(post, s->cord + (int64_t)(st + stage) * N, N, hipMemcpyDeviceToHost));
+    return HCC_OK;
+}
+
+extern "C" int hcc_get_indicator_count(hcc_stepper* s, int64_t* count) {
+    if (!s || !count) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    *count = 0;
+    if (!s->ind_cnt) return HCC_OK;
+    if (int rc = finish_pending(s)) return rc;
+    unsigned long long c = 0;
+    HCC_TRY_HIP(ctx, hipMemcpy(&c, s->ind_cnt, sizeof(c), hipMemcpyDeviceToHost));
+    *count = (int64_t)c;
     return HCC_OK;
 }

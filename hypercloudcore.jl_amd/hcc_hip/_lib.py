@@ -3,6 +3,7 @@
 The library is the only compute path: if it is missing, or no GPU is present when a device call is made, the
 call raises. There is no CPU fallback.
 """
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -32,7 +33,8 @@ EXPORTS = [
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
     "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve", "hcc_set_source",
     "hcc_grid_concat", "hcc_time_steps", "hcc_mlp_create", "hcc_mlp_destroy", "hcc_mlp_set_params",
-    "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_train",
+    "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_train", "hcc_set_indicator",
+    "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
 ]
 MLP_PARAMS = 5313
 
@@ -110,6 +112,10 @@ def lib():
         "hcc_mlp_forward": (C.c_int, [vp, vp, i64, vp]),
         "hcc_mlp_gradients": (C.c_int, [vp, vp, vp, i64, C.c_float, vp, P(d)]),
         "hcc_mlp_train": (C.c_int, [vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float, C.c_float, vp]),
+        "hcc_set_indicator": (C.c_int, [vp, vp, C.c_float]),
+        "hcc_indicator_collect": (C.c_int, [vp, C.c_int32]),
+        "hcc_get_indicator_samples": (C.c_int, [vp, C.c_int32, vp, vp, vp]),
+        "hcc_get_indicator_count": (C.c_int, [vp, P(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -168,9 +174,27 @@ def rng_uniform(seed, n):
     return out
 
 
+_live_contexts = weakref.WeakSet()
+
+
+def _close_live_contexts():
+    """At interpreter exit, before modules are torn down: release every context still open (its steppers, grids and
+    networks first), while the library is still reachable. Objects left to module teardown could otherwise not call
+    into the library and would leave streams, events and pinned memory to the HIP runtime's own exit."""
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_live_contexts)
+
+
 class Context:
     """One GPU context (device + stream). Grids and steppers hold its handle: closing the context closes the ones
-    still alive first (steppers, then grids), so that none is destroyed later through a freed context."""
+    still alive first (steppers, then grids), so that none is destroyed later through a freed context. Contexts still
+    open at interpreter exit are closed by an exit handler."""
 
     def __init__(self, device=0):
         L = lib()
@@ -179,6 +203,7 @@ class Context:
         self.h = h
         self._steppers = weakref.WeakSet()
         self._grids = weakref.WeakSet()
+        _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -434,6 +459,28 @@ class Stepper:
         out = np.zeros(n)
         check(lib().hcc_time_steps(self.h, ptr(out)), self.ctx.h, "hcc_time_steps")
         return out
+
+    def set_indicator(self, mlp, threshold=0.0):
+        """The learned indicator before each flux pass (mlp None: off)."""
+        self._ind = mlp
+        check(lib().hcc_set_indicator(self.h, mlp.h if mlp is not None else None, float(threshold)), self.ctx.h,
+              "hcc_set_indicator")
+
+    def collect(self, on=True):
+        check(lib().hcc_indicator_collect(self.h, int(bool(on))), self.ctx.h, "hcc_indicator_collect")
+
+    def samples(self, stage):
+        """The last step's stage: (features N x 16 as bf16 bits, orders before the flux pass, after the checkpoint)."""
+        f = np.zeros((self.N, 16), np.uint16)
+        pre, post = np.zeros(self.N, np.int8), np.zeros(self.N, np.int8)
+        check(lib().hcc_get_indicator_samples(self.h, int(stage), ptr(f), ptr(pre), ptr(post)), self.ctx.h,
+              "hcc_get_indicator_samples")
+        return f, pre, post
+
+    def indicator_count(self):
+        c = C.c_int64()
+        check(lib().hcc_get_indicator_count(self.h, C.byref(c)), self.ctx.h, "hcc_get_indicator_count")
+        return c.value
 
     def step(self, t, dt):
         check(lib().hcc_step(self.h, float(t), float(dt)), self.ctx.h, "hcc_step")

@@ -240,6 +240,19 @@ int hcc_mlp_gradients(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n
 int hcc_mlp_train(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, int32_t epochs, int32_t batch, float lr,
                   float pos_weight, double* losses);
 
+/* The indicator in the step (2-D Euler): after each stage's full gather, which then also stores 16 features per
+ * particle, particles still at the full order whose logit exceeds `threshold` take first order before the flux pass;
+ * MOOD's a posteriori check of every candidate still runs. m = NULL switches it off (the step is the plain one). The
+ * network stays owned by the caller and must outlive its use. */
+int hcc_set_indicator(hcc_stepper* st, const hcc_mlp* m, float threshold);
+/* Training data: with on, each step keeps every stage's features and the orders before that stage's flux pass and
+ * after its checkpoint (the label of a particle at the full order: MOOD dropped it). */
+int hcc_indicator_collect(hcc_stepper* st, int32_t on);
+/* The last step's stage (0-based): feats N x 16 bf16, pre and post orders N each (any may be NULL). */
+int hcc_get_indicator_samples(hcc_stepper* st, int32_t stage, uint16_t* feats, int8_t* pre, int8_t* post);
+/* Particles the indicator lowered in the last step. */
+int hcc_get_indicator_count(hcc_stepper* st, int64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,7 @@ K_GELU = 0.7978845608028654
 
 
 def gelu(x):
+    """GELU, tanh form (the device evaluates tanh as 1 - 2 / (exp(2u) + 1), the same function)."""
     return 0.5 * x * (1.0 + np.tanh(K_GELU * (x + 0.044715 * x ** 3)))
 
 
