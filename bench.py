@@ -115,7 +115,16 @@ def ensemble_main(args):
         singles.append(st)
     ens = Ensemble(ctx, grids)
     st_e = Stepper(ctx, ens, wls[0]["scheme"])
-    st_e.upload(ens.pack([wl["U0"] for wl in wls]))
+    U0 = ens.pack([wl["U0"] for wl in wls])
+    st_e.upload(U0)
+    st_i, net = None, None
+    if args.indicator:   # the batched form with the learned indicator (the shipped network)
+        from hcc_hip import MLP
+        from hcc_hip import indicator as IND
+        net = MLP(ctx, IND.default_params())
+        st_i = Stepper(ctx, ens, wls[0]["scheme"])
+        st_i.upload(U0)
+        st_i.set_indicator(net, IND.DEFAULT_THRESHOLD)
     nodes = int(ens.sizes.sum())
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
@@ -134,11 +143,21 @@ def ensemble_main(args):
     run([st_e], args.steps)
     wall_e = time.perf_counter() - t0
     print(f"# batched: {args.steps} steps in {wall_e:.3f} s", file=sys.stderr, flush=True)
+    wall_i = None
+    if st_i is not None:
+        run([st_i], args.warmup)
+        st_i.totals(reset=True)
+        st_e.totals(reset=True)
+        t0 = time.perf_counter()
+        run([st_i], args.steps)
+        wall_i = time.perf_counter() - t0
+        print(f"# batched with the indicator: {args.steps} steps in {wall_i:.3f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     run(singles, args.steps)
     wall_p = time.perf_counter() - t0
     print(f"# plain: {args.steps} steps in {wall_p:.3f} s", file=sys.stderr, flush=True)
-    U_e = ens.split(st_e.download())
+    U_full = st_e.download()
+    U_e = ens.split(U_full)
     for m in sorted({0, len(wls) // 2, len(wls) - 1}):
         assert np.array_equal(U_e[m], singles[m].download()), f"member {m} differs from its own stepper"
     stages = 3
@@ -166,6 +185,15 @@ def ensemble_main(args):
                   "form": "one stepper per member, stepped in turn (hcc_step_cfl)"},
         "speedup_vs_plain": value / plain,
     }
+    if wall_i is not None:
+        U_i = st_i.download()
+        dev = float(np.abs(U_i - U_full).sum() / np.abs(U_full).sum())
+        v_i = nodes * stages * args.steps / wall_i
+        out["learned_indicator"] = {"value": v_i, "ms_per_step": 1e3 * wall_i / args.steps,
+                                    "speedup_vs_plain": v_i / plain,
+                                    "relative_l1_deviation_from_plain_mood": dev,
+                                    "note": "batched form with the shipped indicator network; same steps as the "
+                                            "batched plain form"}
     print(json.dumps(out), flush=True)
 
 
@@ -185,6 +213,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="the batched form: many Euler boxes of uneven size in one ensemble grid, against a stepper "
                          "per box")
+    ap.add_argument("--indicator", action="store_true",
+                    help="also run with the learned troubled-node indicator (the shipped network)")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--n-min", type=int, default=20000)
     ap.add_argument("--n-max", type=int, default=400000)
@@ -211,6 +241,12 @@ def main():
     g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
     st = Stepper(ctx, g, wl["scheme"])
     st.upload(wl["U0"])
+    net = None
+    if args.indicator:
+        from hcc_hip import MLP
+        from hcc_hip import indicator as IND
+        net = MLP(ctx, IND.default_params())
+        st.set_indicator(net, IND.DEFAULT_THRESHOLD)
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f} s: N={g.N} nodes, S={g.S} slots, max_nb={g.max_nb}", file=sys.stderr, flush=True)
 
@@ -272,7 +308,7 @@ def main():
         "config": {"workload": f"2-D Euler four-quadrant, jittered h=1/{args.n} unit square with outflow ghost band, "
                                f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4",
                    "nodes": N, "interior_nodes": N_int, "mean_neighbours": round(nnb, 3), "stages": stages,
-                   "seed": args.seed},
+                   "seed": args.seed, "learned_indicator": bool(args.indicator)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_flux_staged<2,4,Euler> (full pass: flux, MOOD check, stage predictor)", "bytes_per_launch": fb,
