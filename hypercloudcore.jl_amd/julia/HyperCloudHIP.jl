@@ -281,6 +281,25 @@ function hip_time_steps(ts::HIPEnsembleStepper)
     return dts
 end
 
+"""
+    hip_set_indicator!(ts::HIPRKTimeStepper, params::Vector{Float32}; threshold = 0f0)
+
+Switches on the learned troubled-node indicator (2-D Euler): a network with the given parameters (HCC_MLP_PARAMS
+floats, e.g. the shipped `hcc_hip/indicator_params.npy`) lowers predicted troubled particles to first order before each
+flux pass; MOOD's check still runs. Returns the network handle (kept alive by `ts`).
+"""
+function hip_set_indicator!(ts::HIPRKTimeStepper, params::Vector{Float32}; threshold::Real = 0f0)
+    length(params) == 5313 || error("the indicator network has 5313 parameters")
+    m = Ref{Ptr{Cvoid}}(C_NULL)
+    checkrc(ccall((:hcc_mlp_create, libhcc), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ts.ctx, m), ts.ctx)
+    checkrc(ccall((:hcc_mlp_set_params, libhcc), Cint, (Ptr{Cvoid}, Ptr{Float32}), m[], params), ts.ctx)
+    checkrc(ccall((:hcc_set_indicator, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float32), ts.st, m[], Float32(threshold)), ts.ctx)
+    finalizer(ts) do t   # after the stepper's own finalizer has run
+        ccall((:hcc_mlp_destroy, libhcc), Cint, (Ptr{Cvoid},), m[])
+    end
+    return m[]
+end
+
 "Copies the device grid's positions (moved and wrapped by a moving stepper) into `pg.core.positions`."
 function download_positions!(ts::HIPRKTimeStepper, pg; D::Integer = length(pg.core.positions[1]))
     x = zeros(Float64, D * ts.N)
@@ -291,7 +310,8 @@ function download_positions!(ts::HIPRKTimeStepper, pg; D::Integer = length(pg.co
     return pg
 end
 
-export hip_set_source!, hip_mover_code, download_positions!, HIPEnsembleStepper, hip_step_cfl!, hip_time_steps
+export hip_set_source!, hip_mover_code, download_positions!, HIPEnsembleStepper, hip_step_cfl!, hip_time_steps,
+       hip_set_indicator!
 export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
        hip_volumes, hip_total_mass, hip_solve_equation
 
