@@ -83,6 +83,7 @@ mutable struct HIPRKTimeStepper <: TimeStepper
     st::Ptr{Cvoid}
     N::Int
     M::Int
+    nets::Vector{Ptr{Cvoid}}   # indicator networks owned by the stepper (destroyed before its context)
 end
 
 """
@@ -110,10 +111,11 @@ function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::
     st = Ref{Ptr{Cvoid}}(C_NULL)
     checkrc(ccall((:hcc_stepper_create, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{StepperDesc}, Ref{Ptr{Cvoid}}),
                   ctx[], grid[], sdesc, st), ctx[])
-    ts = HIPRKTimeStepper(ctx[], grid[], st[], N, Int(sdesc.M))
+    ts = HIPRKTimeStepper(ctx[], grid[], st[], N, Int(sdesc.M), Ptr{Cvoid}[])
     upload!(ts, pg)
     finalizer(ts) do t
         ccall((:hcc_stepper_destroy, libhcc), Cint, (Ptr{Cvoid},), t.st)
+        foreach(m -> ccall((:hcc_mlp_destroy, libhcc), Cint, (Ptr{Cvoid},), m), t.nets)
         ccall((:hcc_grid_destroy, libhcc), Cint, (Ptr{Cvoid},), t.grid)
         ccall((:hcc_ctx_destroy, libhcc), Cint, (Ptr{Cvoid},), t.ctx)
     end
@@ -293,10 +295,8 @@ function hip_set_indicator!(ts::HIPRKTimeStepper, params::Vector{Float32}; thres
     m = Ref{Ptr{Cvoid}}(C_NULL)
     checkrc(ccall((:hcc_mlp_create, libhcc), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ts.ctx, m), ts.ctx)
     checkrc(ccall((:hcc_mlp_set_params, libhcc), Cint, (Ptr{Cvoid}, Ptr{Float32}), m[], params), ts.ctx)
+    push!(ts.nets, m[])   # destroyed by the stepper's finalizer, after the stepper and before the context
     checkrc(ccall((:hcc_set_indicator, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float32), ts.st, m[], Float32(threshold)), ts.ctx)
-    finalizer(ts) do t   # after the stepper's own finalizer has run
-        ccall((:hcc_mlp_destroy, libhcc), Cint, (Ptr{Cvoid},), m[])
-    end
     return m[]
 end
 
