@@ -399,38 +399,70 @@ __global__ __launch_bounds__(kBlock) void k_scale_volumes(double* V, int64_t n, 
     if (i < n) V[i] = V[i] * f;
 }
 
-__global__ __launch_bounds__(256) void k_fold_sum(const double* v, int64_t n, double* out) {
-    constexpr int C = 2048;
-    __shared__ double buf[C];
-    double acc = 0.0;
-    for (int64_t c0 = 0; c0 < n; c0 += C) {
-        for (int t = threadIdx.x; t < C; t += 256)
-            if (c0 + t < n) buf[t] = v[c0 + t];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int m = (int)(n - c0 < C ? n - c0 : C);
-            int t = 0;
-            if (c0 == 0) { acc = buf[0]; t = 1; }      // sum(V) starts from its first element
-            for (; t < m; ++t) acc = acc + buf[t];
-        }
-        __syncthreads();
+// The structure of Julia's sum of an array (mapreduce over IndexLinear, Base reduce.jl): fewer than 16 elements fold
+// left from a1 + a2; otherwise ranges [lo, hi] are halved at lo + (hi - lo) / 2 until hi - lo < 1024, each leaf folds
+// left from its first two elements, and the halves are added. The leaves (host-listed, left to right) are folded one
+// per thread; one thread then replays the recursion over the leaf sums. (Julia's leaf loops are @simd, which LLVM
+// may reassociate; these are sequential.)
+static void pairwise_leaves(int64_t lo, int64_t hi, std::vector<int64_t>& lv) {
+    if (hi - lo < 1024) { lv.push_back(lo); lv.push_back(hi); return; }
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    pairwise_leaves(lo, mid, lv);
+    pairwise_leaves(mid + 1, hi, lv);
+}
+__global__ __launch_bounds__(kBlock) void k_leaf_sums(const double* __restrict__ v, const int64_t* __restrict__ lv,
+                                                      int64_t nleaf, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nleaf) return;
+    const int64_t lo = lv[2 * t], hi = lv[2 * t + 1];
+    double s = v[lo];
+    if (hi > lo) {
+        s = v[lo] + v[lo + 1];
+        for (int64_t i = lo + 2; i <= hi; ++i) s = s + v[i];
     }
-    if (threadIdx.x == 0) *out = acc;
+    out[t] = s;
+}
+__global__ void k_pairwise_combine(const double* __restrict__ leaf, int64_t n, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (n < 16) { *out = leaf[0]; return; }   // one leaf: the whole left fold
+    struct Frame { int64_t lo, hi; int state; double x; };
+    Frame stk[64];   // depth <= log2(n / 1024) + 1
+    int sp = 0;
+    int64_t k = 0;
+    double ret = 0.0;
+    stk[sp++] = Frame{0, n - 1, 0, 0.0};
+    while (sp > 0) {
+        Frame& f = stk[sp - 1];
+        if (f.hi - f.lo < 1024) { ret = leaf[k++]; --sp; continue; }
+        const int64_t mid = f.lo + ((f.hi - f.lo) >> 1);
+        if (f.state == 0) { f.state = 1; stk[sp++] = Frame{f.lo, mid, 0, 0.0}; }
+        else if (f.state == 1) { f.x = ret; f.state = 2; stk[sp++] = Frame{mid + 1, f.hi, 0, 0.0}; }
+        else { ret = f.x + ret; --sp; }
+    }
+    *out = ret;
 }
 
 // The volumes into g->d_vol: sums and 1 / S, `sweeps` sweeps, clamp at zero, and the renormalisation to the domain's
-// volume (prod(maxs - mins), Volumes.jl:86) with the sum of the volumes as a left fold in index order (Julia's sum of
-// an array reassociates; that one sum agrees with it to rounding only).
+// volume (prod(maxs - mins), Volumes.jl:86) with the pairwise sum of the volumes (its leaves sequential, Julia's
+// @simd: that one sum agrees with Julia's to rounding only).
 static int compute_volumes(hcc_grid* g, int sweeps) {
     hcc_ctx* ctx = g->ctx;
     const int64_t N = g->N;
     if (N == 0) return HCC_OK;
     hipStream_t st = ctx->stream;
     if (!g->d_vol) HCC_TRY_HIP(ctx, dalloc(&g->d_vol, N));
-    double *S = nullptr, *Vt = nullptr, *tot = nullptr;
+    double *S = nullptr, *Vt = nullptr, *tot = nullptr, *lsum = nullptr;
+    int64_t* dlv = nullptr;
+    std::vector<int64_t> lv;   // the pairwise sum's leaves
+    if (N < 16) { lv.push_back(0); lv.push_back(N - 1); }
+    else pairwise_leaves(0, N - 1, lv);
+    const int64_t nleaf = (int64_t)lv.size() / 2;
     HCC_TRY_HIP(ctx, dalloc(&S, N));
     HCC_TRY_HIP(ctx, dalloc(&Vt, N));
     HCC_TRY_HIP(ctx, dalloc(&tot, 1));
+    HCC_TRY_HIP(ctx, dalloc(&lsum, nleaf));
+    HCC_TRY_HIP(ctx, dalloc(&dlv, lv.size()));
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(dlv, lv.data(), sizeof(int64_t) * lv.size(), hipMemcpyHostToDevice, st));
     const GeomArgs a = g->args();
     const double w0 = mls_weight(a.alpha, a.inv_range_sq, 0.0);
     const int64_t nb = nblocks(N);
@@ -445,13 +477,14 @@ static int compute_volumes(hcc_grid* g, int sweeps) {
         std::swap(g->d_vol, Vt);
     }
     k_clamp_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N);
-    k_fold_sum<<<1, 256, 0, st>>>(g->d_vol, N, tot);
+    k_leaf_sums<<<(unsigned)((nleaf + kBlock - 1) / kBlock), kBlock, 0, st>>>(g->d_vol, dlv, nleaf, lsum);
+    k_pairwise_combine<<<1, 64, 0, st>>>(lsum, N, tot);
     double dv = g->geom.L[0];
     for (int d = 1; d < D; ++d) dv = dv * g->geom.L[d];
     k_scale_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N, dv, tot);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(S); (void)hipFree(Vt); (void)hipFree(tot);
+    (void)hipFree(S); (void)hipFree(Vt); (void)hipFree(tot); (void)hipFree(lsum); (void)hipFree(dlv);
     if (e != hipSuccess) { ctx->err = std::string("volumes: ") + hipGetErrorString(e); return HCC_E_HIP; }
     return HCC_OK;
 }

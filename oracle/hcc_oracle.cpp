@@ -466,6 +466,33 @@ struct Phys {
     }
 };
 
+// Julia's sum of an array (mapreduce over IndexLinear, Base reduce.jl): fewer than 16 elements fold left from
+// a1 + a2; otherwise ranges [lo, hi] are halved at lo + (hi - lo) / 2 until hi - lo < 1024, a leaf folds left from its
+// first two elements, and the halves are added. (Julia's leaf loops are @simd, which LLVM may reassociate; these are
+// sequential.)
+static double pairwise_range(const double* a, int64_t lo, int64_t hi) {
+    if (lo == hi) return a[lo];
+    if (hi - lo < 1024) {
+        double v = a[lo] + a[lo + 1];
+        for (int64_t i = lo + 2; i <= hi; ++i) v = v + a[i];
+        return v;
+    }
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    const double x = pairwise_range(a, lo, mid);
+    const double y = pairwise_range(a, mid + 1, hi);
+    return x + y;
+}
+static double pairwise_sum(const double* a, int64_t n) {
+    if (n == 0) return 0.0;
+    if (n == 1) return a[0];
+    if (n < 16) {
+        double v = a[0] + a[1];
+        for (int64_t i = 2; i < n; ++i) v = v + a[i];
+        return v;
+    }
+    return pairwise_range(a, 0, n - 1);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Scheme / stepper configuration
 // ---------------------------------------------------------------------------------------------
@@ -597,8 +624,8 @@ struct Sim : SimBase {
 
     // update_volumes! with ShepardVolumes (Volumes.jl:183-214): S_i = w(0) + sum_k w_k, V_i = 1 / S_i; `sweeps`
     // partition-of-unity sweeps V_i <- (w(0) / S_i) V_i + sum_k (w_k / S_j) V_j over all particles; then
-    // _renormalise_volumes! (:226-231): V = max(V, 0), V *= |Omega| / sum(V). sum(V) is a left fold in index order
-    // here and on the device; Julia's sum of an array reassociates (pairwise blocks, @simd), so that one sum is
+    // _renormalise_volumes! (:226-231): V = max(V, 0), V *= |Omega| / sum(V). sum(V) follows the structure of Julia's
+    // sum of an array (pairwise_sum); its leaf loops are @simd in Julia, which reassociates them, so that one sum is
     // parity unpinned at its last bits.
     std::vector<double> vol;
     void update_volumes(int sweeps) {
@@ -621,8 +648,7 @@ struct Sim : SimBase {
             vol = Vt;
         }
         for (int64_t i = 0; i < N; ++i) vol[i] = jmax(vol[i], 0.0);
-        double tot = vol[0];
-        for (int64_t i = 1; i < N; ++i) tot = tot + vol[i];
+        const double tot = pairwise_sum(vol.data(), N);
         double dv = g.L[0];   // domain_volume: prod(maxs - mins) (Volumes.jl:86)
         for (int d = 1; d < D; ++d) dv = dv * g.L[d];
         const double f = dv / tot;
