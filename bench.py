@@ -197,6 +197,88 @@ def ensemble_main(args):
     print(json.dumps(out), flush=True)
 
 
+VARIANTS = ("moving", "imex", "epd2_u2", "strict_epd0", "streamed", "aligned", "solve")
+
+
+def variant_workload(W, L, name, n, seed):
+    """The 1 M-node four-quadrant problem with one of the scheme variants built beside the headline path."""
+    import numpy as np
+    h = 1.0 / n
+    wl = W.euler_2d(h=h, seed=seed)
+    bcs = wl["scheme"]["bcs"]
+    kw = dict(delta=1e-4 * h * h, bcs=bcs)
+    if name == "moving":   # Lagrangian particles on the periodic unit square (movers take no outflow conditions)
+        dom = W.domain(2, (0.0, 0.0), (1.0, 1.0), (True, True), h, 2.5, 0.25, seed)
+        cloud = L.cloud_generate(W.domain_desc(dom))
+        x = cloud["pos"]
+        rho_q, p_q, u_q, v_q = W.quadrant_states(seed, 1.4)
+        q = (x[:, 0] >= 0.5).astype(int) + 2 * (x[:, 1] >= 0.5).astype(int)
+        wl = {"domain": dom, "cloud": cloud, "cfl": 0.4,
+              "U0": W.euler_conserved(rho_q[q], np.column_stack([u_q[q], v_q[q]]), p_q[q], 1.4),
+              "scheme": W.scheme(4, L.HCC_PDE_EULER, delta=1e-4 * h * h, mover=L.HCC_MOVER_LAGRANGIAN)}
+        return wl, "LagrangianMover on the periodic unit square (same four quadrants)"
+    if name == "imex":
+        kw.update(tableau_name="IMEX_ARS233")
+        desc = "IMEX ARS(2,3,3) with the implicit decay source, rate 2 (1 + 0.5 sin 2 pi x)"
+    elif name == "epd2_u2":
+        kw.update(strategy=L.HCC_STRAT_EPD2, criterion=L.HCC_CRIT_U2, plateau=0.05)
+        desc = "MOOD(EPD2, MOODu2 with plateau 0.05)"
+    elif name == "strict_epd0":
+        kw.update(strategy=L.HCC_STRAT_STRICT_EPD0)
+        desc = "MOOD(StrictEPD0, MOODu1)"
+    elif name == "streamed":
+        kw.update(streamed=True)
+        desc = "MUSCL streamed divergence"
+    elif name == "aligned":
+        kw.update(flux=L.HCC_FLUX_ALIGNED_RUSANOV)
+        desc = "AlignedRusanovFlux"
+    else:
+        desc = "the headline scheme through hcc_solve (the device driver loop, snapshots at the ends only)"
+    wl["scheme"] = W.scheme(4, L.HCC_PDE_EULER, **kw)
+    return wl, desc
+
+
+def variant_main(args):
+    """One variant of the step on the 1 M-node problem: node-stage updates/s over the timed CFL steps."""
+    import hcc_hip
+    from hcc_hip import Context, Grid, Stepper
+    from hcc_hip import _lib as L
+    from hcc_hip import workloads as W
+    wl, desc = variant_workload(W, L, args.variant, args.n, args.seed)
+    c = wl["cloud"]
+    ctx = Context(0)
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    if args.variant == "imex":
+        st.set_source(L.HCC_SRC_DECAY_IMPLICIT, 2.0 * (1 + 0.5 * np.sin(2 * np.pi * c["pos"][:, 0])))
+    for _ in range(args.warmup):
+        st.step_cfl(wl["cfl"])
+    hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+    st.totals(reset=True)
+    if args.variant == "solve":   # the driver loop to t = 0.005 (about 20 CFL steps), timed by hcc_solve itself
+        _, _, _, n_run, wall, _ = st.solve(0.005, wl["cfl"], True, 1)
+        args.steps = int(n_run)
+    else:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st.step_cfl(wl["cfl"])
+        hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+        wall = time.perf_counter() - t0
+    n_steps, iters, tot_ms, _, _ = st.totals(reset=True)
+    U = st.download()
+    assert np.all(np.isfinite(U)), "non-finite state"
+    stages = len(wl["scheme"]["b_t"])
+    out = {"metric": f"node-stage updates/s, 2D Euler MUSCL+MOOD @1M nodes, variant {args.variant}",
+           "value": g.N * stages * args.steps / wall, "unit": "node-stage updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"2-D Euler four-quadrant, h=1/{args.n}, " + desc, "nodes": g.N, "stages": stages,
+                      "variant": args.variant},
+           "detail": {"step_event_ms": tot_ms / max(n_steps, 1), "mood_iterations": iters}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +297,8 @@ def main():
                          "per box")
     ap.add_argument("--indicator", action="store_true",
                     help="also run with the learned troubled-node indicator (the shipped network)")
+    ap.add_argument("--variant", choices=VARIANTS, default=None,
+                    help="a scheme variant built beside the headline path, on the same 1 M-node problem")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--n-min", type=int, default=20000)
     ap.add_argument("--n-max", type=int, default=400000)
@@ -228,6 +312,9 @@ def main():
         sys.exit(2)
     if args.ensemble:
         ensemble_main(args)
+        return
+    if args.variant:
+        variant_main(args)
         return
 
     import hcc_hip
