@@ -177,6 +177,18 @@ struct hcc_grid {
     int64_t* d_inst_blk = nullptr;     // n_inst + 1 block offsets
     std::vector<int32_t> h_tags;
     std::vector<uint8_t> h_isb;
+    // scratch kept across searches and volume updates (a moving grid repeats both every step): the prefix sums'
+    // temporary storage, the chunk widths, the list statistics, the neighbour array's capacity, and the volumes'
+    // Shepard sums, sweep buffer and pairwise-sum plan
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    int64_t* d_width = nullptr;
+    unsigned long long* d_stats = nullptr;   // sum and max of the counts, S_sell
+    int64_t nbr_cap = 0;
+    double *d_vS = nullptr, *d_vT = nullptr, *d_vnode = nullptr;
+    int64_t* d_vplan = nullptr;
+    int64_t vplan_n = -1, vleaf = 0;
+    std::vector<int32_t> vlevel;             // the combine's level offsets into the node triples
     hcc::GeomArgs args() const;
 };
 

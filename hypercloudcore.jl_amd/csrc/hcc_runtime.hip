@@ -214,14 +214,33 @@ __global__ __launch_bounds__(kBlock) void k_count(SearchArgs a, int64_t nblk) {
     a.cnt[i] = c;
 }
 
-// widest node of each chunk of 64 (one wave per chunk)
-__global__ __launch_bounds__(kBlock) void k_chunk_width(const int32_t* cnt, int64_t N, int64_t nchunks, int64_t* width) {
-    int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    int64_t c = t / kChunk;
-    int32_t v = (t < N) ? cnt[t] : 0;
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0 && c < nchunks) width[c] = (int64_t)v * kChunk;
+// widest node of each chunk of 64 (one wave per chunk, grid-stride); the sum and the maximum of the counts into
+// stats[0..1], one atomic pair per workgroup
+constexpr int kWidthBlocks = 512;
+__global__ __launch_bounds__(kBlock) void k_chunk_width(const int32_t* cnt, int64_t N, int64_t nchunks, int64_t* width,
+                                                        unsigned long long* stats) {
+    __shared__ unsigned long long red[2][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long sum = 0, mx = 0;
+    for (int64_t c = (int64_t)blockIdx.x * (kBlock / 64) + wv; c < nchunks; c += (int64_t)gridDim.x * (kBlock / 64)) {
+        const int64_t t = c * kChunk + lane;
+        int32_t v = (t < N) ? cnt[t] : 0;
+        sum += (unsigned long long)v;
+        for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+        if (lane == 0) width[c] = (int64_t)v * kChunk;
+        mx = max(mx, (unsigned long long)v);
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) { red[0][wv] = sum; red[1][wv] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) { sum += red[0][w]; mx = max(mx, red[1][w]); }
+        if (sum) atomicAdd(&stats[0], sum);
+        if (mx) atomicMax(&stats[1], mx);
+    }
 }
+
+__global__ void k_set_i32(int32_t* p, int32_t v) { if (threadIdx.x == 0) *p = v; }
 
 // _write_neighbours! (NeighborLogic.jl:309-329), into the SELL-64 layout
 template <int D>
@@ -258,18 +277,24 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(SearchArgs a, int64_t 
     }
 }
 
+// queued on the context's stream; the temporary storage is the grid's, grown when a scan needs more
 template <class T>
-static hipError_t exclusive_scan(hcc_ctx* ctx, const T* in, T* out, int64_t n) {
+static hipError_t exclusive_scan(hcc_grid* g, const T* in, T* out, int64_t n) {
+    hipStream_t st = g->ctx->stream;
     size_t bytes = 0;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st);
     if (e != hipSuccess) return e;
-    void* tmp = nullptr;
-    e = hipMalloc(&tmp, bytes ? bytes : 1);
-    if (e != hipSuccess) return e;
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, ctx->stream);
-    hipError_t e2 = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(tmp);
-    return e != hipSuccess ? e : e2;
+    if (bytes > g->scan_tmp_bytes) {
+        if (g->d_scan_tmp) {
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+            (void)hipFree(g->d_scan_tmp);
+            g->d_scan_tmp = nullptr;
+            g->scan_tmp_bytes = 0;
+        }
+        if ((e = hipMalloc(&g->d_scan_tmp, bytes)) != hipSuccess) return e;
+        g->scan_tmp_bytes = bytes;
+    }
+    return hipcub::DeviceScan::ExclusiveSum(g->d_scan_tmp, bytes, in, out, (int)n, st);
 }
 
 hcc::GeomArgs hcc_grid::args() const {
@@ -314,9 +339,8 @@ static int run_search(hcc_grid* g) {
         HCC_TRY_HIP(ctx, hipGetLastError());
     }
     // segment starts: exclusive prefix sum of the counts; the dummy bin nbins is empty
-    HCC_TRY_HIP(ctx, exclusive_scan<int32_t>(ctx, g->d_cell_cursor, g->d_cell_start, g->nbins + 1));
-    int32_t nn = (int32_t)N;
-    HCC_TRY_HIP(ctx, hipMemcpy(g->d_cell_start + g->nbins + 1, &nn, sizeof(int32_t), hipMemcpyHostToDevice));
+    HCC_TRY_HIP(ctx, exclusive_scan<int32_t>(g, g->d_cell_cursor, g->d_cell_start, g->nbins + 1));
+    k_set_i32<<<1, 64, 0, st>>>(g->d_cell_start + g->nbins + 1, (int32_t)N);
     HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_cell_cursor, 0, sizeof(int32_t) * (g->nbins + 1), st));
     if (N > 0) {
         k_bin_scatter<<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
@@ -324,26 +348,34 @@ static int run_search(hcc_grid* g) {
         k_count<D><<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
         HCC_TRY_HIP(ctx, hipGetLastError());
     }
-    // chunk widths -> SELL offsets
+    // chunk widths -> SELL offsets (N, and with it the chunk count, is fixed for the grid's life)
     g->nchunks = (N + kChunk - 1) / kChunk;
-    int64_t* d_width = nullptr;
-    HCC_TRY_HIP(ctx, dalloc(&d_width, g->nchunks + 1));
-    HCC_TRY_HIP(ctx, hipMemsetAsync(d_width, 0, sizeof(int64_t) * (g->nchunks + 1), st));
-    HCC_TRY_HIP(ctx, dalloc(&g->d_chunk_off, g->nchunks + 1));
-    if (N > 0) k_chunk_width<<<(unsigned)((g->nchunks * kChunk + kBlock - 1) / kBlock), kBlock, 0, st>>>(g->d_cnt, N, g->nchunks, d_width);
-    HCC_TRY_HIP(ctx, exclusive_scan<int64_t>(ctx, d_width, g->d_chunk_off, g->nchunks + 1));
-    (void)hipFree(d_width);
-    HCC_TRY_HIP(ctx, hipMemcpy(&g->S_sell, g->d_chunk_off + g->nchunks, sizeof(int64_t), hipMemcpyDeviceToHost));
-    HCC_TRY_HIP(ctx, dalloc(&g->d_nbr, g->S_sell));
+    if (!g->d_width) HCC_TRY_HIP(ctx, dalloc(&g->d_width, g->nchunks + 1));
+    if (!g->d_chunk_off) HCC_TRY_HIP(ctx, dalloc(&g->d_chunk_off, g->nchunks + 1));
+    if (!g->d_stats) HCC_TRY_HIP(ctx, dalloc(&g->d_stats, 3));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_width, 0, sizeof(int64_t) * (g->nchunks + 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_stats, 0, sizeof(unsigned long long) * 3, st));
+    if (N > 0)
+        k_chunk_width<<<(unsigned)std::min<int64_t>(kWidthBlocks, (g->nchunks + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, st>>>(
+            g->d_cnt, N, g->nchunks, g->d_width, g->d_stats);
+    HCC_TRY_HIP(ctx, exclusive_scan<int64_t>(g, g->d_width, g->d_chunk_off, g->nchunks + 1));
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(g->d_stats + 2, g->d_chunk_off + g->nchunks, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    unsigned long long stats[3];
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(stats, g->d_stats, sizeof(stats), hipMemcpyDeviceToHost, st));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
+    g->S = (int64_t)stats[0]; g->max_nb = (int64_t)stats[1]; g->S_sell = (int64_t)stats[2];
+    if (g->S_sell > g->nbr_cap || !g->d_nbr) {   // a research grows the capacity by a quarter at once
+        if (g->d_nbr) (void)hipFree(g->d_nbr);
+        g->d_nbr = nullptr;
+        const int64_t cap = g->nbr_cap ? g->S_sell + g->S_sell / 4 : g->S_sell;
+        g->nbr_cap = 0;
+        HCC_TRY_HIP(ctx, dalloc(&g->d_nbr, cap));
+        g->nbr_cap = cap;
+    }
     HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_nbr, 0, sizeof(int32_t) * std::max<int64_t>(g->S_sell, 1), st));
     a = search_args(g);
     if (N > 0) k_write<D><<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
     HCC_TRY_HIP(ctx, hipGetLastError());
-    std::vector<int32_t> cnt((size_t)N);
-    if (N > 0) HCC_TRY_HIP(ctx, hipMemcpyAsync(cnt.data(), g->d_cnt, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st));
-    HCC_TRY_HIP(ctx, hipStreamSynchronize(st));
-    g->S = 0; g->max_nb = 0;
-    for (int64_t i = 0; i < N; ++i) { g->S += cnt[i]; g->max_nb = std::max<int64_t>(g->max_nb, cnt[i]); }
     return HCC_OK;
 }
 
@@ -401,91 +433,131 @@ __global__ __launch_bounds__(kBlock) void k_scale_volumes(double* V, int64_t n, 
 
 // The structure of Julia's sum of an array (mapreduce over IndexLinear, Base reduce.jl): fewer than 16 elements fold
 // left from a1 + a2; otherwise ranges [lo, hi] are halved at lo + (hi - lo) / 2 until hi - lo < 1024, each leaf folds
-// left from its first two elements, and the halves are added. The leaves (host-listed, left to right) are folded one
-// per thread; one thread then replays the recursion over the leaf sums. (Julia's leaf loops are @simd, which LLVM
-// may reassociate; these are sequential.)
-static void pairwise_leaves(int64_t lo, int64_t hi, std::vector<int64_t>& lv) {
-    if (hi - lo < 1024) { lv.push_back(lo); lv.push_back(hi); return; }
+// left from its first two elements, and the halves are added. The plan (built once per node count): the leaves left to
+// right, then the internal nodes as (node, left, right) triples grouped by height, so that one workgroup adds a whole
+// height at a time. Each leaf is folded by one workgroup from LDS. (Julia's leaf loops are @simd, which LLVM may
+// reassociate; these are sequential.)
+namespace {
+struct PwLevels {
+    int32_t n = 0;
+    int32_t off[40] = {};     // height h's triples at [off[h], off[h + 1]); heights < 40 for any int64 node count
+};
+struct PwInner { int64_t left, right; int height; };
+// node ids: leaf k -> k; internal node k -> -(k + 1) until the leaf count is known
+int64_t pw_build(int64_t lo, int64_t hi, std::vector<int64_t>& leaves, std::vector<PwInner>& inner, int& h) {
+    if (hi - lo < 1024) {
+        leaves.push_back(lo); leaves.push_back(hi);
+        h = 0;
+        return (int64_t)leaves.size() / 2 - 1;
+    }
     const int64_t mid = lo + ((hi - lo) >> 1);
-    pairwise_leaves(lo, mid, lv);
-    pairwise_leaves(mid + 1, hi, lv);
+    int hl = 0, hr = 0;
+    const int64_t l = pw_build(lo, mid, leaves, inner, hl);
+    const int64_t r = pw_build(mid + 1, hi, leaves, inner, hr);
+    h = std::max(hl, hr) + 1;
+    inner.push_back(PwInner{l, r, h});
+    return -(int64_t)inner.size();
 }
-__global__ __launch_bounds__(kBlock) void k_leaf_sums(const double* __restrict__ v, const int64_t* __restrict__ lv,
-                                                      int64_t nleaf, double* __restrict__ out) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= nleaf) return;
-    const int64_t lo = lv[2 * t], hi = lv[2 * t + 1];
-    double s = v[lo];
-    if (hi > lo) {
-        s = v[lo] + v[lo + 1];
-        for (int64_t i = lo + 2; i <= hi; ++i) s = s + v[i];
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_leaf_sums(const double* __restrict__ v, const int64_t* __restrict__ leaves,
+                                                  double* __restrict__ node) {
+    __shared__ double x[1024];                        // a leaf holds at most 1024 elements
+    const int64_t lo = leaves[2 * (int64_t)blockIdx.x], hi = leaves[2 * (int64_t)blockIdx.x + 1];
+    const int n = (int)(hi - lo + 1);
+    for (int k = threadIdx.x; k < n; k += 64) x[k] = v[lo + k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s = x[0];
+    if (n > 1) {
+        s = x[0] + x[1];
+        for (int k = 2; k < n; ++k) s = s + x[k];
     }
-    out[t] = s;
+    node[blockIdx.x] = s;
 }
-__global__ void k_pairwise_combine(const double* __restrict__ leaf, int64_t n, double* __restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    if (n < 16) { *out = leaf[0]; return; }   // one leaf: the whole left fold
-    struct Frame { int64_t lo, hi; int state; double x; };
-    Frame stk[64];   // depth <= log2(n / 1024) + 1
-    int sp = 0;
-    int64_t k = 0;
-    double ret = 0.0;
-    stk[sp++] = Frame{0, n - 1, 0, 0.0};
-    while (sp > 0) {
-        Frame& f = stk[sp - 1];
-        if (f.hi - f.lo < 1024) { ret = leaf[k++]; --sp; continue; }
-        const int64_t mid = f.lo + ((f.hi - f.lo) >> 1);
-        if (f.state == 0) { f.state = 1; stk[sp++] = Frame{f.lo, mid, 0, 0.0}; }
-        else if (f.state == 1) { f.x = ret; f.state = 2; stk[sp++] = Frame{mid + 1, f.hi, 0, 0.0}; }
-        else { ret = f.x + ret; --sp; }
+__global__ __launch_bounds__(1024) void k_pairwise_tree(double* node, const int64_t* __restrict__ tri, PwLevels lv) {
+    for (int h = 0; h < lv.n; ++h) {
+        for (int64_t t = lv.off[h] + (int64_t)threadIdx.x; t < lv.off[h + 1]; t += 1024)
+            node[tri[3 * t]] = node[tri[3 * t + 1]] + node[tri[3 * t + 2]];
+        __syncthreads();
     }
-    *out = ret;
+}
+
+// The plan for g->N nodes into g->d_vplan (leaf bounds, then the triples) and g->vlevel, once per grid.
+static int volume_plan(hcc_grid* g) {
+    if (g->vplan_n == g->N) return HCC_OK;
+    hcc_ctx* ctx = g->ctx;
+    std::vector<int64_t> leaves;
+    std::vector<PwInner> inner;
+    int h = 0;
+    if (g->N < 16) { leaves.push_back(0); leaves.push_back(g->N - 1); }
+    else pw_build(0, g->N - 1, leaves, inner, h);
+    const int64_t nleaf = (int64_t)leaves.size() / 2, nin = (int64_t)inner.size();
+    if (nleaf + nin > INT32_MAX) return set_err(ctx, HCC_E_ARG, "volumes: too many nodes");
+    auto id = [&](int64_t c) { return c >= 0 ? c : nleaf + (-c - 1); };
+    const int H = nin ? inner.back().height : 0;   // the root is built last
+    std::vector<int64_t> cnt((size_t)H + 2, 0), plan((size_t)(2 * nleaf + 3 * nin));
+    for (const PwInner& q : inner) ++cnt[(size_t)q.height];
+    g->vlevel.assign((size_t)H + 1, 0);
+    for (int k = 1; k <= H; ++k) g->vlevel[(size_t)k] = g->vlevel[(size_t)k - 1] + (int32_t)cnt[(size_t)k];
+    std::copy(leaves.begin(), leaves.end(), plan.begin());
+    std::vector<int64_t> fill(g->vlevel.begin(), g->vlevel.end());
+    for (int64_t k = 0; k < nin; ++k) {
+        const PwInner& q = inner[(size_t)k];
+        const int64_t t = fill[(size_t)q.height - 1]++;
+        int64_t* o = plan.data() + 2 * nleaf + 3 * t;
+        o[0] = nleaf + k; o[1] = id(q.left); o[2] = id(q.right);
+    }
+    (void)hipFree(g->d_vplan); g->d_vplan = nullptr;
+    (void)hipFree(g->d_vnode); g->d_vnode = nullptr;
+    g->vplan_n = -1;
+    HCC_TRY_HIP(ctx, dalloc(&g->d_vplan, plan.size()));
+    HCC_TRY_HIP(ctx, dalloc(&g->d_vnode, nleaf + nin));
+    HCC_TRY_HIP(ctx, hipMemcpy(g->d_vplan, plan.data(), sizeof(int64_t) * plan.size(), hipMemcpyHostToDevice));
+    g->vleaf = nleaf;
+    g->vplan_n = g->N;
+    return HCC_OK;
 }
 
 // The volumes into g->d_vol: sums and 1 / S, `sweeps` sweeps, clamp at zero, and the renormalisation to the domain's
 // volume (prod(maxs - mins), Volumes.jl:86) with the pairwise sum of the volumes (its leaves sequential, Julia's
-// @simd: that one sum agrees with Julia's to rounding only).
+// @simd: that one sum agrees with Julia's to rounding only). Queued on the context's stream.
 static int compute_volumes(hcc_grid* g, int sweeps) {
     hcc_ctx* ctx = g->ctx;
     const int64_t N = g->N;
     if (N == 0) return HCC_OK;
     hipStream_t st = ctx->stream;
     if (!g->d_vol) HCC_TRY_HIP(ctx, dalloc(&g->d_vol, N));
-    double *S = nullptr, *Vt = nullptr, *tot = nullptr, *lsum = nullptr;
-    int64_t* dlv = nullptr;
-    std::vector<int64_t> lv;   // the pairwise sum's leaves
-    if (N < 16) { lv.push_back(0); lv.push_back(N - 1); }
-    else pairwise_leaves(0, N - 1, lv);
-    const int64_t nleaf = (int64_t)lv.size() / 2;
-    HCC_TRY_HIP(ctx, dalloc(&S, N));
-    HCC_TRY_HIP(ctx, dalloc(&Vt, N));
-    HCC_TRY_HIP(ctx, dalloc(&tot, 1));
-    HCC_TRY_HIP(ctx, dalloc(&lsum, nleaf));
-    HCC_TRY_HIP(ctx, dalloc(&dlv, lv.size()));
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(dlv, lv.data(), sizeof(int64_t) * lv.size(), hipMemcpyHostToDevice, st));
+    if (!g->d_vS) HCC_TRY_HIP(ctx, dalloc(&g->d_vS, N));
+    if (!g->d_vT) HCC_TRY_HIP(ctx, dalloc(&g->d_vT, N));
+    if (int rc = volume_plan(g)) return rc;
     const GeomArgs a = g->args();
     const double w0 = mls_weight(a.alpha, a.inv_range_sq, 0.0);
     const int64_t nb = nblocks(N);
     const int D = g->geom.D;
+    double* S = g->d_vS;
     if (D == 1) k_shepard_sums<1><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
     else if (D == 2) k_shepard_sums<2><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
     else k_shepard_sums<3><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, w0, nb);
     for (int sw = 0; sw < sweeps; ++sw) {
-        if (D == 1) k_shepard_sweep<1><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
-        else if (D == 2) k_shepard_sweep<2><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
-        else k_shepard_sweep<3><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, Vt, w0, nb);
-        std::swap(g->d_vol, Vt);
+        if (D == 1) k_shepard_sweep<1><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, g->d_vT, w0, nb);
+        else if (D == 2) k_shepard_sweep<2><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, g->d_vT, w0, nb);
+        else k_shepard_sweep<3><<<grid_for(nb), kBlock, 0, st>>>(a, S, g->d_vol, g->d_vT, w0, nb);
+        std::swap(g->d_vol, g->d_vT);
     }
     k_clamp_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N);
-    k_leaf_sums<<<(unsigned)((nleaf + kBlock - 1) / kBlock), kBlock, 0, st>>>(g->d_vol, dlv, nleaf, lsum);
-    k_pairwise_combine<<<1, 64, 0, st>>>(lsum, N, tot);
+    k_leaf_sums<<<(unsigned)g->vleaf, 64, 0, st>>>(g->d_vol, g->d_vplan, g->d_vnode);
+    const int64_t nin = g->vleaf - 1;
+    if (nin > 0) {
+        PwLevels lv;
+        lv.n = (int32_t)g->vlevel.size() - 1;
+        for (size_t k = 0; k < g->vlevel.size(); ++k) lv.off[k] = g->vlevel[k];
+        k_pairwise_tree<<<1, 1024, 0, st>>>(g->d_vnode, g->d_vplan + 2 * g->vleaf, lv);
+    }
     double dv = g->geom.L[0];
     for (int d = 1; d < D; ++d) dv = dv * g->geom.L[d];
-    k_scale_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N, dv, tot);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(S); (void)hipFree(Vt); (void)hipFree(tot); (void)hipFree(lsum); (void)hipFree(dlv);
-    if (e != hipSuccess) { ctx->err = std::string("volumes: ") + hipGetErrorString(e); return HCC_E_HIP; }
+    k_scale_volumes<<<(unsigned)nb, kBlock, 0, st>>>(g->d_vol, N, dv, g->d_vnode + (g->vleaf + nin - 1));
+    HCC_TRY_HIP(ctx, hipGetLastError());
     return HCC_OK;
 }
 
@@ -498,6 +570,7 @@ extern "C" int hcc_grid_volumes(hcc_grid* g, int32_t sweeps, double* volumes) {
         HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
         if (int rc = compute_volumes(g, sweeps)) return rc;
     }
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (volumes && g->N) HCC_TRY_HIP(ctx, hipMemcpy(volumes, g->d_vol, sizeof(double) * g->N, hipMemcpyDeviceToHost));
     return HCC_OK;
 }
@@ -507,11 +580,9 @@ int hcc::grid_volumes(hcc_grid* g, int sweeps) { return compute_volumes(g, sweep
 int hcc::research(hcc_grid* g) {
     hcc_ctx* ctx = g->ctx;
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(g->d_chunk_off); g->d_chunk_off = nullptr;
-    (void)hipFree(g->d_nbr); g->d_nbr = nullptr;
-    (void)hipFree(g->d_halo_off); g->d_halo_off = nullptr;
-    (void)hipFree(g->d_halo_idx); g->d_halo_idx = nullptr;
-    (void)hipFree(g->d_nbr_local); g->d_nbr_local = nullptr;
+    if (g->d_halo_off) { (void)hipFree(g->d_halo_off); g->d_halo_off = nullptr; }
+    if (g->d_halo_idx) { (void)hipFree(g->d_halo_idx); g->d_halo_idx = nullptr; }
+    if (g->d_nbr_local) { (void)hipFree(g->d_nbr_local); g->d_nbr_local = nullptr; }
     g->max_halo = 0; g->max_block_slots = 0;
     const int D = g->geom.D;
     return D == 1 ? run_search<1>(g) : D == 2 ? run_search<2>(g) : run_search<3>(g);
@@ -578,6 +649,7 @@ extern "C" int hcc_grid_concat(hcc_ctx* ctx, int32_t n, hcc_grid* const* parts, 
     HCC_TRY_HIP(ctx, dalloc(&g->d_vol, N));
     HCC_TRY_HIP(ctx, hipMemsetAsync(g->d_vol, 0, sizeof(double) * std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, dalloc(&g->d_nbr, S_sell));
+    g->nbr_cap = S_sell;
     HCC_TRY_HIP(ctx, dalloc(&g->d_chunk_off, g->nchunks + 1));
     g->h_isb.assign((size_t)N, 1);
     g->h_tags.assign((size_t)N, 0);
@@ -662,6 +734,8 @@ extern "C" int hcc_grid_destroy(hcc_grid* g) {
     (void)hipFree(g->d_halo_off); (void)hipFree(g->d_halo_idx); (void)hipFree(g->d_nbr_local);
     (void)hipFree(g->d_vol);
     (void)hipFree(g->d_blk_inst); (void)hipFree(g->d_inst_blk);
+    (void)hipFree(g->d_scan_tmp); (void)hipFree(g->d_width); (void)hipFree(g->d_stats);
+    (void)hipFree(g->d_vS); (void)hipFree(g->d_vT); (void)hipFree(g->d_vnode); (void)hipFree(g->d_vplan);
     delete g;
     return HCC_OK;
 }
@@ -717,6 +791,7 @@ extern "C" int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_
     HCC_TRY_HIP(ctx, dalloc(&g->d_cnt, N));
     rc = D == 1 ? run_search<1>(g.get()) : D == 2 ? run_search<2>(g.get()) : run_search<3>(g.get());
     if (rc) return rc;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (D == 2 && N > 0) {   // LDS staging tables of the 2-D passes
         std::vector<int32_t> cnt((size_t)N), nbr((size_t)std::max<int64_t>(g->S_sell, 1)), hoff, hidx;
         std::vector<int64_t> off((size_t)(g->nchunks + 1));
@@ -741,6 +816,7 @@ extern "C" int hcc_grid_create(hcc_ctx* ctx, const hcc_domain_desc* desc, int64_
     }
     rc = compute_volumes(g.get(), 1);   // the constructor's ShepardVolumes() (ParticleGrids.jl:275)
     if (rc) return rc;
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = g.release();
     return HCC_OK;
 }
@@ -817,6 +893,7 @@ extern "C" int hcc_grid_get_bins(hcc_grid* g, int32_t* bin_of, int64_t* cell_sta
     hcc_ctx* ctx = g->ctx;
     if (g->n_inst > 1) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_grid_get_bins: an ensemble has no cell list (query the parts)");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));   // a moving step's search may still be queued
     if (bin_of && g->N) HCC_TRY_HIP(ctx, hipMemcpy(bin_of, g->d_bin_of, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     if (cell_particles && g->N) HCC_TRY_HIP(ctx, hipMemcpy(cell_particles, g->d_cell_particles, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     if (cell_start) {

@@ -1665,17 +1665,30 @@ __global__ __launch_bounds__(kBlock) void k_range_part(const double* __restrict_
         part[(int64_t)blockIdx.x * 2 * M + c] = v;
     }
 }
+// one workgroup of 1024 threads: strided over the parts, then across the workgroup (any order gives the same bits)
 template <int M>
-__global__ __launch_bounds__(64) void k_range_final(const double* __restrict__ part, int64_t nparts, double* __restrict__ range) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(1024) void k_range_final(const double* __restrict__ part, int64_t nparts, double* __restrict__ range) {
+    __shared__ double sl[16][2 * M];
     double lo[M], hi[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) { lo[m] = INFINITY; hi[m] = -INFINITY; }
-    for (int64_t b = 0; b < nparts; ++b)
+    for (int64_t b = threadIdx.x; b < nparts; b += 1024)
 #pragma unroll
         for (int m = 0; m < M; ++m) { lo[m] = jmin(lo[m], part[b * 2 * M + m]); hi[m] = jmax(hi[m], part[b * 2 * M + M + m]); }
 #pragma unroll
-    for (int m = 0; m < M; ++m) range[m] = hi[m] - lo[m];
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int m = 0; m < M; ++m) { lo[m] = jmin(lo[m], __shfl_xor(lo[m], o)); hi[m] = jmax(hi[m], __shfl_xor(hi[m], o)); }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) { sl[threadIdx.x >> 6][m] = lo[m]; sl[threadIdx.x >> 6][M + m] = hi[m]; }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        const int m = threadIdx.x;
+        double a = sl[0][m], b = sl[0][M + m];
+        for (int q = 1; q < 16; ++q) { a = jmin(a, sl[q][m]); b = jmax(b, sl[q][M + m]); }
+        range[m] = b - a;
+    }
 }
 
 // _count_inadmissible (TimeSteppers.jl:481-490): interior candidates accepted while not admissible
@@ -2012,6 +2025,7 @@ struct hcc_stepper {
     double* invg = nullptr; uint8_t* okg = nullptr;
     double* invd = nullptr; uint8_t* okd = nullptr;
     double* dw = nullptr;
+    int64_t dw_cap = 0;           // slots dw holds (a moving grid's lists change size every step)
     int32_t* root = nullptr;
     int32_t *goff = nullptr, *gidx = nullptr;   // outflow ghosts of each interior root (CSR over nodes)
     bool fresh = true;         // U was (re)loaded: the first stage state is formed by the predictor once
@@ -2206,7 +2220,7 @@ struct Variant {
     static void range(const hcc_stepper* s, const double* Y) {
         const int64_t nb = (s->N + kBlock - 1) / kBlock;
         k_range_part<D, M><<<(unsigned)nb, kBlock, 0, st(s)>>>(Y, s->N, s->rpart);
-        k_range_final<M><<<1, 64, 0, st(s)>>>(s->rpart, nb, s->range);
+        k_range_final<M><<<1, 1024, 0, st(s)>>>(s->rpart, nb, s->range);
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
@@ -2384,6 +2398,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->invd, (int64_t)D * (D + 1) * N));
     HCC_TRY_HIP(ctx, dalloc(&s->okd, N));
     HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)D * g->S_sell));
+    s->dw_cap = g->S_sell;
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)D * g->S_sell, 1), st));
     HCC_TRY_HIP(ctx, dalloc(&s->root, N));
     HCC_TRY_HIP(ctx, dalloc(&s->U, s->PU * N));
@@ -2861,9 +2876,13 @@ static int moving_step_end(hcc_stepper* s, const VariantOps& ops) {
     ops.rec_pos(s, s->U, 1);
     HCC_TRY_HIP(ctx, hipGetLastError());
     if (int rc = research(g)) return rc;
-    (void)hipFree(s->dw);
-    s->dw = nullptr;
-    HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)s->D * g->S_sell));
+    if (g->S_sell > s->dw_cap) {                   // grown with the neighbour array's capacity
+        (void)hipFree(s->dw);
+        s->dw = nullptr;
+        s->dw_cap = 0;
+        HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)s->D * g->nbr_cap));
+        s->dw_cap = g->nbr_cap;
+    }
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)s->D * g->S_sell, 1), ctx->stream));
     ops.rec_pos(s, s->U, 0);
     launch_moments(s, INFINITY);
