@@ -2,8 +2,8 @@
 
 Bar: neighbour lists, bins, orders and MOOD flags bit-exact; states, weights, caches and dt identical to the last
 bit as well (the kernels keep the reference's operation order, -ffp-contract=off, correctly rounded FP64 division
-and sqrt). The north-star tolerance for states, 1e-12 relative L-infinity, is asserted alongside as the bound
-that must never be exceeded.
+and sqrt). The stated floating-point tolerance for states, 1e-12 relative L-infinity, is asserted alongside as the
+bound that must never be exceeded.
 """
 import numpy as np
 import pytest
@@ -15,7 +15,7 @@ from helpers import build_pair, first_diff, rel_linf
 
 pytestmark = pytest.mark.gpu
 
-STATE_RTOL = 1e-12   # relative L-infinity bound on states
+STATE_RTOL = 1e-12   # relative L-infinity bound on states (the stated tolerance)
 
 
 def _cases():
