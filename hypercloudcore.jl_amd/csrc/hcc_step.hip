@@ -1691,6 +1691,20 @@ __global__ __launch_bounds__(1024) void k_range_final(const double* __restrict__
     }
 }
 
+// The step's statistics into a ring slot in coherent pinned host memory: counters[0..5] and the dt bits at [7] (one
+// small kernel instead of two copies through the DMA engine)
+__global__ __launch_bounds__(64) void k_stats_out(const unsigned long long* __restrict__ counters,
+                                                  const double* __restrict__ dtp, unsigned long long* out) {
+    const int t = threadIdx.x;
+    if (t < 6) __hip_atomic_store(out + t, counters[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (t == 7) {
+        unsigned long long bits;
+        const double dt = *dtp;
+        __builtin_memcpy(&bits, &dt, sizeof(bits));
+        __hip_atomic_store(out + 7, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // _count_inadmissible (TimeSteppers.jl:481-490): interior candidates accepted while not admissible
 __global__ __launch_bounds__(kBlock) void k_count_inadm(StepArgs a, int64_t nblk) {
     const int64_t i = node_index(nblk);
@@ -2077,14 +2091,14 @@ struct hcc_stepper {
     struct StepStats {
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         hipEvent_t evp[4 * kMaxStages] = {};   // start/end of each full gather and flux pass
-        unsigned long long* h = nullptr;      // pinned copy of the step counters; [7] = dt
+        unsigned long long* h = nullptr;      // coherent pinned copy of the step counters; [7] = dt ...
+        unsigned long long* hd = nullptr;     // ... and its device address (k_stats_out writes it)
         int64_t host_iters = 0;
         bool dt_from_device = false;
     };
     static constexpr int kRing = 4;
     StepStats ring[kRing];
     int ring_head = 0, ring_count = 0;
-    hipEvent_t evm[2] = {};                // follow the kernels that publish the MOOD fixed point's counts
     unsigned long long* h_pub = nullptr;   // two count slots in coherent pinned host memory ...
     unsigned long long* d_pub = nullptr;   // ... and their device address
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
@@ -2339,7 +2353,6 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
         for (auto& e : r.evp) if (e) (void)hipEventDestroy(e);
         if (r.h) (void)hipHostFree(r.h);
     }
-    for (auto& e : s->evm) if (e) (void)hipEventDestroy(e);
     if (s->h_pub) (void)hipHostFree(s->h_pub);
     delete s;
     return HCC_OK;
@@ -2462,9 +2475,9 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, hipEventCreate(&r.ev0));
         HCC_TRY_HIP(ctx, hipEventCreate(&r.ev1));
         for (auto& e : r.evp) HCC_TRY_HIP(ctx, hipEventCreate(&e));
-        HCC_TRY_HIP(ctx, hipHostMalloc((void**)&r.h, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+        HCC_TRY_HIP(ctx, hipHostMalloc((void**)&r.h, 8 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
+        HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&r.hd, r.h, 0));
     }
-    for (auto& e : s->evm) HCC_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 2 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
     s->h_pub[0] = s->h_pub[1] = 0;
     HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_pub, s->h_pub, 0));
@@ -2708,13 +2721,30 @@ struct StepRunner {
     }
     // One iteration of the fused fixed point. Every count it needs is read on the device, so iterations queue without
     // a host round trip, and an iteration enqueued after the fixed point is reached does nothing. Its first kernel
-    // publishes the count it starts from into pinned slot q, and event evm[q] follows that kernel.
+    // publishes the count it starts from into pinned slot q, which holds kPending until then (a count is never
+    // kPending): the host polls the slot instead of waiting for an event, which would also hold the stream between the
+    // two kernels.
+    static constexpr unsigned long long kPending = ~0ull;
     hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, int q) {
-        hipStream_t str = s->ctx->stream;
+        *(volatile unsigned long long*)&s->h_pub[q] = kPending;   // its last writer has published (waited on)
         ops.mark_gather(s, a, Yp, s->d_pub + q, q);
-        hipError_t e = hipEventRecord(s->evm[q], str);
         ops.flux_list(s, a, p, Yp, Kp, q);
-        return e;
+        return hipGetLastError();
+    }
+    // The count published in slot q. Polls for up to a second; then waits for the stream, so that a failed launch
+    // surfaces as its error instead of a hang.
+    hipError_t wait_published(int q, unsigned long long& n) {
+        volatile unsigned long long* v = &s->h_pub[q];
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned spin = 0; (n = *v) == kPending; ++spin) {
+            if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+                hipError_t e = hipStreamSynchronize(s->ctx->stream);
+                if (e != hipSuccess) return e;
+                if ((n = *v) == kPending) return hipErrorUnknown;   // the stream drained without the count
+                return hipSuccess;
+            }
+        }
+        return hipSuccess;
     }
     // mood_checkpoint! (TimeSteppers.jl:454-477). Fused path: the first (full) check already ran inside stage i-1's
     // flux pass, which also formed the candidate Y_i and counted the accepted inadmissible candidates; every
@@ -2736,8 +2766,9 @@ struct StepRunner {
             if (e != hipSuccess) return e;
             for (int k = 0;; ++k) {
                 const int q = k & 1;
-                if ((e = hipEventSynchronize(s->evm[q])) != hipSuccess) return e;
-                if (*(volatile unsigned long long*)&s->h_pub[q] == 0) break;   // n_k = 0: iteration k + 1 does nothing
+                unsigned long long n = 0;
+                if ((e = wait_published(q, n)) != hipSuccess) return e;
+                if (n == 0) break;   // n_k = 0: iteration k + 1 does nothing
                 if ((e = mood_iteration(p, Yp, Kp, q ^ 1)) != hipSuccess) return e;
             }
             return hipGetLastError();
@@ -2906,8 +2937,8 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return HCC_E_HIP; }
     if (s->moving)
         if (int rc = moving_step_end(s, r.ops)) return rc;
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h, s->counters, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HCC_TRY_HIP(ctx, hipMemcpyAsync(slot.h + 7, s->dtp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    k_stats_out<<<1, 64, 0, ctx->stream>>>(s->counters, s->dtp, slot.hd);
+    HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev1, ctx->stream));
     slot.host_iters = r.iters;
     slot.dt_from_device = dt_from_device;
