@@ -1691,6 +1691,11 @@ __global__ __launch_bounds__(1024) void k_range_final(const double* __restrict__
     }
 }
 
+// counters[0] into the host's pinned slot pub
+__global__ __launch_bounds__(64) void k_publish(const unsigned long long* __restrict__ counters, unsigned long long* pub) {
+    if (threadIdx.x == 0) __hip_atomic_store(pub, counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The step's statistics into a ring slot in coherent pinned host memory: counters[0..5] and the dt bits at [7] (one
 // small kernel instead of two copies through the DMA engine)
 __global__ __launch_bounds__(64) void k_stats_out(const unsigned long long* __restrict__ counters,
@@ -2711,13 +2716,13 @@ struct StepRunner {
         p.count = count ? 1 : 0;
         return p;
     }
+    // the unfused check's count of changed particles (counters[0]), published into pinned slot 0 and polled
     hipError_t read_changed(unsigned long long& n) {
-        hcc_ctx* ctx = s->ctx;
-        hipError_t e = hipMemcpyAsync(ctx->h_pinned, s->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
+        *(volatile unsigned long long*)&s->h_pub[0] = kPending;
+        k_publish<<<1, 64, 0, s->ctx->stream>>>(s->counters, s->d_pub);
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(ctx->stream);
-        n = ctx->h_pinned[0];
-        return e;
+        return wait_published(0, n);
     }
     // One iteration of the fused fixed point. Every count it needs is read on the device, so iterations queue without
     // a host round trip, and an iteration enqueued after the fixed point is reached does nothing. Its first kernel
