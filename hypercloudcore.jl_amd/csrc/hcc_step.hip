@@ -1596,19 +1596,24 @@ __global__ __launch_bounds__(kBlock) void k_force(StepArgs a, const int8_t* __re
     a.dirty[i] = ch;
 }
 
-// One hop of expand_mask! (_expand_body!, MOOD.jl:323-351): dst = src or any neighbour in src.
+// One hop of expand_mask! (_expand_body!, MOOD.jl:323-351): dst = src or any neighbour in src, in push form on a
+// cleared dst: each particle of src marks itself and its neighbours (the neighbour relation is symmetric, so this is
+// the pull form's set; concurrent marks write the same 1). Only the few particles of src read their lists.
 __global__ __launch_bounds__(kBlock) void k_expand(StepArgs a, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                                                     int64_t nblk) {
     const int64_t i = node_index(nblk);
-    if (i >= a.g.N) return;
-    uint8_t m = src[i];
-    if (!m) {
-        const int cnt = a.g.cnt[i];
-        const int64_t base = sell_base(a.g, i);
-        for (int k = 0; k < cnt; ++k)
-            if (src[a.g.nbr[base + (int64_t)k * kChunk]]) { m = 1; break; }
-    }
-    dst[i] = m;
+    if (i >= a.g.N || !src[i]) return;
+    dst[i] = 1;
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    for (int k = 0; k < cnt; ++k) dst[a.g.nbr[base + (int64_t)k * kChunk]] = 1;
+}
+static hipError_t launch_expand(const StepArgs& a, const uint8_t* src, uint8_t* dst, int64_t N, int64_t nb,
+                                hipStream_t str) {
+    hipError_t e = hipMemsetAsync(dst, 0, (size_t)std::max<int64_t>(N, 1), str);
+    if (e != hipSuccess) return e;
+    k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, src, dst, nb);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n) {
@@ -1619,9 +1624,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_u8(const uint8_t* __restrict__ 
 // _nb_min_order (MOOD.jl:66-74) of every particle: EPD2's interface order of the edge (i, j) is the minimum of the
 // two (_get_interface_orders, MOOD.jl:82-85), so the flux pass reads these as orders with the edge-minimum rule.
 // pack: AlignedRusanovFlux also needs the true orders (whether a pair is below the full order): out = min | order << 4.
-__global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict__ out, int pack, int64_t nblk) {
+// only: the particles whose minimum can have changed since the last pass (the rest keep theirs), or null for all.
+__global__ __launch_bounds__(kBlock) void k_nbmin(StepArgs a, int8_t* __restrict__ out, int pack,
+                                                   const uint8_t* __restrict__ only, int64_t nblk) {
     const int64_t i = node_index(nblk);
-    if (i >= a.g.N) return;
+    if (i >= a.g.N || (only && !only[i])) return;
     int o = a.orders[i];
     const int cnt = a.g.cnt[i];
     const int64_t base = sell_base(a.g, i);
@@ -2610,12 +2617,14 @@ struct StepRunner {
     bool epd2() const { return s->desc.mood_strategy == HCC_STRAT_EPD2; }
     // the arguments of a flux pass: with EPD2 the interface order of an edge is the minimum of the two particles'
     // neighbourhood minima, i.e. the edge-minimum rule of EPD1 applied to nbmin (computed before the pass)
-    // kst: the stage (0-based) whose right-hand side the pass computes, for its K_X on a moving grid
-    StepArgs flux_args(int kst) {
+    // kst: the stage (0-based) whose right-hand side the pass computes, for its K_X on a moving grid. only: a
+    // recompute of the fixed point refreshes nbmin where it can have changed since the last pass of the same K, the
+    // dropped particles and their neighbours (the first hop of the expansion, mask2); null: every particle.
+    StepArgs flux_args(int kst, const uint8_t* only = nullptr) {
         StepArgs f = a;
         if (s->moving) f.KX = s->KX[kst];
         if (!epd2()) return f;
-        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, nb);
+        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, only, nb);
         f.orders = s->nbmin;
         f.strategy = HCC_STRAT_EPD1;
         f.packed = a.aligned;
@@ -2803,13 +2812,13 @@ struct StepRunner {
                 k_copy_u8<<<(unsigned)nb, kBlock, 0, str>>>(s->dirty, s->changed, s->N);
             }
             if (reach == 1) {
-                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->changed, s->dirty, nb);
+                if ((e = launch_expand(a, s->changed, s->dirty, s->N, nb, str)) != hipSuccess) return e;
             } else {
-                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->changed, s->mask2, nb);
-                k_expand<<<grid_for(nb), kBlock, 0, str>>>(a, s->mask2, s->dirty, nb);
+                if ((e = launch_expand(a, s->changed, s->mask2, s->N, nb, str)) != hipSuccess) return e;
+                if ((e = launch_expand(a, s->mask2, s->dirty, s->N, nb, str)) != hipSuccess) return e;
             }
             ops.gather(s, a, Yp, 0, 0);
-            ops.flux(s, flux_args(i - 2), off, Yp, Kp, 0, false);
+            ops.flux(s, flux_args(i - 2, reach == 2 ? s->mask2 : nullptr), off, Yp, Kp, 0, false);
             full = 0;
         }
         if (sub) predict(i);
