@@ -223,6 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomentArgs a, int64_t nblk) 
     if (ok) chol_inverse<BD>(U, X);
     for (int r = 0; r < D; ++r) for (int c = 0; c < BD; ++c) a.invd[(r * BD + c) * N + i] = ok ? X[1 + r][c] : 0.0;
     a.okd[i] = ok;
+    if (!a.dw) return;
     // dweights: (rows 2..D+1 of inv(N)) * (w p~) * invL (MUSCL.jl:211-223)
     for (int k = 0; k < cnt; ++k) {
         const int64_t s = base + (int64_t)k * kChunk;

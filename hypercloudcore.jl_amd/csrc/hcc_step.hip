@@ -2050,8 +2050,8 @@ struct hcc_stepper {
     // geometry caches
     double* invg = nullptr; uint8_t* okg = nullptr;
     double* invd = nullptr; uint8_t* okd = nullptr;
-    double* dw = nullptr;
-    int64_t dw_cap = 0;           // slots dw holds (a moving grid's lists change size every step)
+    double* dw = nullptr;         // derivative weights per slot, formed only when the geometry is downloaded
+    int64_t dw_cap = 0;           // slots dw holds
     int32_t* root = nullptr;
     int32_t *goff = nullptr, *gidx = nullptr;   // outflow ghosts of each interior root (CSR over nodes)
     bool fresh = true;         // U was (re)loaded: the first stage state is formed by the predictor once
@@ -2333,9 +2333,11 @@ static bool find_ops(int D, int M, int kind, VariantOps& ops) {
 
 // prepare_geometry! (MUSCL.jl:172-223): the moment inverses and derivative weights of the grid's current positions;
 // R_cut: R^2 on a moving grid's stage geometry (drifted pairs weigh 0), +inf on a searched grid
-static void launch_moments(hcc_stepper* s, double R_cut) {
+// dw: the derivative weights' buffer (D S_sell), or null: no pass reads them (the streamed and aligned flux loops
+// recompute them in registers), so they are formed only for the geometry download.
+static void launch_moments(hcc_stepper* s, double R_cut, double* dw = nullptr) {
     hcc_grid* g = s->grid;
-    MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, s->dw, g->S_sell};
+    MomentArgs ma{g->args(), s->invg, s->okg, s->invd, s->okd, dw, g->S_sell};
     ma.g.R_cut = R_cut;
     const int64_t nb = nblocks(s->N);
     hipStream_t st = s->ctx->stream;
@@ -2422,9 +2424,6 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, dalloc(&s->okg, N));
     HCC_TRY_HIP(ctx, dalloc(&s->invd, (int64_t)D * (D + 1) * N));
     HCC_TRY_HIP(ctx, dalloc(&s->okd, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)D * g->S_sell));
-    s->dw_cap = g->S_sell;
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)D * g->S_sell, 1), st));
     HCC_TRY_HIP(ctx, dalloc(&s->root, N));
     HCC_TRY_HIP(ctx, dalloc(&s->U, s->PU * N));
     HCC_TRY_HIP(ctx, dalloc(&s->cand, s->PU * N));
@@ -2921,14 +2920,6 @@ static int moving_step_end(hcc_stepper* s, const VariantOps& ops) {
     ops.rec_pos(s, s->U, 1);
     HCC_TRY_HIP(ctx, hipGetLastError());
     if (int rc = research(g)) return rc;
-    if (g->S_sell > s->dw_cap) {                   // grown with the neighbour array's capacity
-        (void)hipFree(s->dw);
-        s->dw = nullptr;
-        s->dw_cap = 0;
-        HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)s->D * g->nbr_cap));
-        s->dw_cap = g->nbr_cap;
-    }
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->dw, 0, sizeof(double) * std::max<int64_t>((int64_t)s->D * g->S_sell, 1), ctx->stream));
     ops.rec_pos(s, s->U, 0);
     launch_moments(s, INFINITY);
     HCC_TRY_HIP(ctx, hipGetLastError());
@@ -3065,6 +3056,18 @@ extern "C" int hcc_get_moments(hcc_stepper* s, double* inv_grad, uint8_t* ok_gra
         int rc = download_lists(s->grid, rstart, idx, &sell);
         if (rc) return rc;
         const int64_t SS = s->grid->S_sell;
+        // the derivative weights of the current geometry (the step's passes keep no copy): the moments of the
+        // grid's positions and lists, as the last geometry refresh formed them, with the weights written too
+        if (SS > s->dw_cap) {
+            (void)hipFree(s->dw);
+            s->dw = nullptr;
+            s->dw_cap = 0;
+            HCC_TRY_HIP(ctx, dalloc(&s->dw, (int64_t)D * SS));
+            s->dw_cap = SS;
+        }
+        launch_moments(s, INFINITY, s->dw);
+        HCC_TRY_HIP(ctx, hipGetLastError());
+        HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         std::vector<double> v((size_t)(D * SS));
         HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->dw, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < sell.size(); ++k)
