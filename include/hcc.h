@@ -161,7 +161,8 @@ int hcc_get_orders(hcc_stepper* st, int32_t* orders);
 int hcc_get_mood_flags(hcc_stepper* st, uint8_t* flags);
 int hcc_get_inadmissible(hcc_stepper* st, int64_t* count);
 int hcc_get_mood_iterations(hcc_stepper* st, int64_t* count);
-/* Geometry caches (row-major B*B per node; ok bytes; dweights S*D in row-major slot order). */
+/* Geometry caches (row-major B*B per node; ok bytes; dweights S*D in row-major slot order, formed by this call
+ * from the current geometry: the step's passes keep no copy). */
 int hcc_get_moments(hcc_stepper* st, double* inv_grad, uint8_t* ok_grad, double* inv_div, uint8_t* ok_div,
                     double* dweights);
 /* One full right-hand side (gather + flux pass, orders reset to the maximum) on state Y (N*M AoS):
