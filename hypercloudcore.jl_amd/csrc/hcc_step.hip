@@ -1162,6 +1162,22 @@ __global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pre
     }
 }
 
+// The unfused fixed point's recompute list: the dirty interior particles into dlist, their count into work count 0
+// (cleared before), one atomic per wavefront; the order of the list does not matter (each entry writes its own K).
+__global__ __launch_bounds__(kBlock) void k_compact_dirty(StepArgs a, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    const bool take = i < a.g.N && a.dirty[i] && !a.g.isb[i];
+    const unsigned long long ball = __ballot(take);
+    if (!ball) return;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned leader = (unsigned)(__ffsll((unsigned long long)ball) - 1);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(work_count(a, 0), (unsigned long long)__popcll(ball));
+    base = __shfl(base, (int)leader);
+    if (take) a.dlist[base + __popcll(ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = (int32_t)i;
+}
+__global__ void k_zero_u64(unsigned long long* p) { if (threadIdx.x == 0) *p = 0; }
+
 // The recompute kernels of the fixed point run one node per group of kSplit lanes of a 64-lane workgroup: each lane
 // takes every kSplit-th slot of the node (loads and per-slot arithmetic in parallel) and writes what the accumulation
 // needs to LDS; then one lane per accumulator adds the slot terms in slot order. Every sum keeps the reference's order,
@@ -1173,8 +1189,9 @@ constexpr int kSplit = D * M <= 8 ? 8 : 16;
 
 // The flux pass (with the MOOD check) of the fixed-point iteration with parity p over its work list (dlist, length
 // read on the device): the dirty particles (the changed ones and their neighbours), each listed once. Changes go to
-// changed list 1 - p.
-template <int D, int M, int KIND, int FV>
+// changed list 1 - p. FP = false: a recompute of the unfused fixed point over the list k_compact_dirty made (work
+// count 0), K only, as k_flux with full = 0 (a recomputed particle's K_X, its own grid velocity, does not change).
+template <int D, int M, int KIND, int FV, bool FP = true>
 __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int p) {
     using L = Layout<D, M>;
@@ -1186,7 +1203,7 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
     const int64_t n = (int64_t)*work_count(a, p);
     const int64_t N = a.g.N;
     const int grp = threadIdx.x / G, gl = threadIdx.x % G;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
+    if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const bool streamed = FV == 1 && a.streamed;
     for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
         const int64_t t = t0 + grp;
@@ -1195,7 +1212,7 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
         if (t < n) {
             i = a.dlist[t];
             if (gl == 0) {
-                a.dmark[i] = 0u;   // each dirty node is listed once (k_mark_gather); its mark is cleared for the next
+                if (FP) a.dmark[i] = 0u;   // each dirty node is listed once (k_mark_gather); cleared for the next
                 take = !a.g.isb[i];
             }
         }
@@ -1286,11 +1303,16 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
 #pragma unroll
                 for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, nd.fi, m);
             }
-            const int before = a.inadm[i];
-            ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, a.hratio * node_dt(a.dtp, a.inst, i));
-            delta = (int)a.inadm[i] - before;
+            if constexpr (FP) {
+                const int before = a.inadm[i];
+                ch = flux_tail<D, M, KIND, true>(a, pred, K, i, nd.fi, Kout, a.hratio * node_dt(a.dtp, a.inst, i));
+                delta = (int)a.inadm[i] - before;
+            } else {
+                flux_tail<D, M, KIND, false>(a, pred, K, i, nd.fi, Kout, 0.0);
+            }
         }
         __syncthreads();
+        if constexpr (!FP) continue;
         // work item t's entry of changed list 1 - p: the node if its order is to drop, else -1 (no atomics)
         if (t < n && gl == 0) {
             changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
@@ -2216,10 +2238,16 @@ struct Variant {
         k_mark_gather<D, M, KIND><<<split_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
     }
     template <int FV>
+    // full: 1 every particle, 0 the dirty ones (a pass over all that skips the clean), -1 the dirty ones of an
+    // unfused recompute listed first (kSplit lanes a particle)
     static void flux_fv(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
                         bool check) {
         const int64_t nb = nblocks(s->N);
-        if (full && s->lds_flux) {
+        if (full < 0) {
+            k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
+            k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
+            k_flux_list<D, M, KIND, FV, false><<<split_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
+        } else if (full && s->lds_flux) {
             if (check) k_flux_staged<D, M, KIND, true, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
             else k_flux_staged<D, M, KIND, false, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
         } else if (check) k_flux<D, M, KIND, true, FV><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
@@ -2817,7 +2845,11 @@ struct StepRunner {
                 if ((e = launch_expand(a, s->mask2, s->dirty, s->N, nb, str)) != hipSuccess) return e;
             }
             ops.gather(s, a, Yp, 0, 0);
-            ops.flux(s, flux_args(i - 2, reach == 2 ? s->mask2 : nullptr), off, Yp, Kp, 0, false);
+            // a small dirty set (the changed particles and their reach-hop neighbourhoods, estimated at 16 a hop) is
+            // listed and recomputed with kSplit lanes a particle; a large one (an implicit stage can drop nearly every
+            // particle) by the pass over all particles, which skips the clean ones
+            const int list = (int64_t)n * 16 * reach <= s->N / 4 ? -1 : 0;
+            ops.flux(s, flux_args(i - 2, reach == 2 ? s->mask2 : nullptr), off, Yp, Kp, list, false);
             full = 0;
         }
         if (sub) predict(i);
