@@ -159,6 +159,7 @@ struct hcc_grid {
     int32_t* d_cell_cursor = nullptr;  // nbins + 1
     int32_t* d_cell_particles = nullptr;
     int32_t* d_bin_window = nullptr;   // nbins * 3^D
+    double* d_spos[3] = {nullptr, nullptr, nullptr};   // positions in cell order (the search's candidate loops)
     // neighbours (SELL-64)
     int32_t* d_cnt = nullptr;
     int64_t nchunks = 0, S = 0, S_sell = 0, max_nb = 0;

@@ -121,6 +121,7 @@ struct SearchArgs {
     int32_t* cell_start;
     int32_t* cursor;
     int32_t* cell_particles;
+    double* sp[3];                      // positions in cell order: sp[d][s] = p[d][cell_particles[s]]
     const int32_t* window;
     int32_t* cnt;
     const int64_t* chunk_off;
@@ -181,6 +182,29 @@ __global__ __launch_bounds__(kBlock) void k_bin_sort(SearchArgs a, int64_t nbins
     }
 }
 
+// The bins' positions in cell order, so that the candidate loops read each bin's positions contiguously
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sort_pos(SearchArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= a.N) return;
+    const int32_t j = a.cell_particles[s];
+    for (int d = 0; d < D; ++d) a.sp[d][s] = a.p[d][j];
+}
+
+// get_distance (ParticleGrids.jl:209-219) + sum(abs2, .) from particle i to the candidate at cell-order slot cs
+// (the same operands as p[j] - p[i] with j = cell_particles[cs])
+template <int D>
+__device__ __forceinline__ double search_dist_cs(const SearchArgs& a, const double (&pi)[D], int64_t cs, double (&dd)[D]) {
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+        double x = a.sp[d][cs] - pi[d];
+        x = x - a.Lw[d] * rint(x * a.invLw[d]);
+        dd[d] = x;
+        s = (d == 0) ? x * x : s + x * x;
+    }
+    return s;
+}
+
 // get_distance (ParticleGrids.jl:209-219) + sum(abs2, .)
 template <int D>
 __device__ __forceinline__ double search_dist(const SearchArgs& a, int64_t i, int64_t j, double (&dd)[D]) {
@@ -201,14 +225,15 @@ __global__ __launch_bounds__(kBlock) void k_count(SearchArgs a, int64_t nblk) {
     int64_t i = node_index(nblk);
     if (i >= a.N) return;
     const int32_t* win = a.window + (int64_t)a.bin_of[i] * NW;
+    double pi[D];
+    for (int d = 0; d < D; ++d) pi[d] = a.p[d][i];
     int32_t c = 0;
     for (int w = 0; w < NW; ++w) {
         int32_t b = win[w];
         for (int32_t s = a.cell_start[b]; s < a.cell_start[b + 1]; ++s) {
-            int32_t j = a.cell_particles[s];
-            if (j == i) continue;
+            if (a.cell_particles[s] == i) continue;
             double dd[D];
-            if (search_dist<D>(a, i, j, dd) <= a.R_sq) ++c;
+            if (search_dist_cs<D>(a, pi, s, dd) <= a.R_sq) ++c;
         }
     }
     a.cnt[i] = c;
@@ -250,6 +275,8 @@ __global__ __launch_bounds__(kBlock) void k_write(SearchArgs a, int64_t nblk) {
     if (i >= a.N) return;
     const int32_t* win = a.window + (int64_t)a.bin_of[i] * NW;
     int64_t base = a.chunk_off[i / kChunk] + (i % kChunk);
+    double pi[D];
+    for (int d = 0; d < D; ++d) pi[d] = a.p[d][i];
     int32_t k = 0;
     for (int w = 0; w < NW; ++w) {
         int32_t b = win[w];
@@ -257,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_write(SearchArgs a, int64_t nblk) {
             int32_t j = a.cell_particles[s];
             if (j == i) continue;
             double dd[D];
-            if (search_dist<D>(a, i, j, dd) <= a.R_sq) { a.nbr[base + (int64_t)k * kChunk] = j; ++k; }
+            if (search_dist_cs<D>(a, pi, s, dd) <= a.R_sq) { a.nbr[base + (int64_t)k * kChunk] = j; ++k; }
         }
     }
 }
@@ -322,6 +349,7 @@ static SearchArgs search_args(const hcc_grid* g) {
     a.R_sq = g->geom.R_sq; a.alpha = g->geom.desc.alpha; a.inv_range_sq = g->geom.inv_range_sq;
     a.bin_of = g->d_bin_of; a.cell_start = g->d_cell_start; a.cursor = g->d_cell_cursor;
     a.cell_particles = g->d_cell_particles; a.window = g->d_bin_window;
+    for (int d = 0; d < 3; ++d) a.sp[d] = g->d_spos[d];
     a.cnt = g->d_cnt; a.chunk_off = g->d_chunk_off; a.nbr = g->d_nbr;
     return a;
 }
@@ -331,6 +359,8 @@ static int run_search(hcc_grid* g) {
     hcc_ctx* ctx = g->ctx;
     hipStream_t st = ctx->stream;
     const int64_t N = g->N, nblk = nblocks(N);
+    for (int d = 0; d < D; ++d)
+        if (!g->d_spos[d]) HCC_TRY_HIP(ctx, dalloc(&g->d_spos[d], N));
     SearchArgs a = search_args(g);
     if (N > 0) {
         k_wrap<D><<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
@@ -345,6 +375,7 @@ static int run_search(hcc_grid* g) {
     if (N > 0) {
         k_bin_scatter<<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
         k_bin_sort<<<(unsigned)((g->nbins + kBlock - 1) / kBlock), kBlock, 0, st>>>(a, g->nbins);
+        k_sort_pos<D><<<(unsigned)nblk, kBlock, 0, st>>>(a);
         k_count<D><<<grid_for(nblk), kBlock, 0, st>>>(a, nblk);
         HCC_TRY_HIP(ctx, hipGetLastError());
     }
@@ -735,6 +766,7 @@ extern "C" int hcc_grid_destroy(hcc_grid* g) {
     (void)hipFree(g->d_vol);
     (void)hipFree(g->d_blk_inst); (void)hipFree(g->d_inst_blk);
     (void)hipFree(g->d_scan_tmp); (void)hipFree(g->d_width); (void)hipFree(g->d_stats);
+    for (int d = 0; d < 3; ++d) (void)hipFree(g->d_spos[d]);
     (void)hipFree(g->d_vS); (void)hipFree(g->d_vT); (void)hipFree(g->d_vnode); (void)hipFree(g->d_vplan);
     delete g;
     return HCC_OK;
