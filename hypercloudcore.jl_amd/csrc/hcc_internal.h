@@ -129,7 +129,7 @@ struct StepArgs {
     double* lam;                 // per-particle wave speeds for the CFL estimate, D per node
     double* bmin;                // per-workgroup minima of the CFL estimate
     double* ext;                 // local extrema of the stage state over each neighbourhood: M maxima, M minima
-    int ext_on;                  // full gather passes store ext, the fused MOOD checks read it (MOODu1)
+    int ext_on;                  // full gather passes store ext, the MOOD checks read it (MOODu1, MOODu2)
     const double* dtp;           // the step's dt, device-resident (set by the host or by the CFL estimate); one per
                                  // instance on an ensemble grid
     const uint16_t* inst;        // ensemble grid: the instance of each kStage block (nullptr: one instance)

@@ -1566,15 +1566,22 @@ __global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* 
                 // findLocalExtrema over {i} and its neighbours in Y_prev; _dmp_fails (MOOD.jl:175-184), and for MOODu2
                 // (_u2_fails, MOOD.jl:215-270, MUSCL order 2) the components below the plateau are exempt
                 double mn[M], mx[M];
-                load_rec<L::NR, L::PU>(Yp, i, r);
+                if (a.ext_on) {   // the extrema the stage's full gather stored (the same operands, the same bits)
+                    double e[2 * M];
+                    load_rec<2 * M, 2 * M>(a.ext, i, e);
 #pragma unroll
-                for (int m = 0; m < M; ++m) { mn[m] = r[D + m]; mx[m] = r[D + m]; }
-                const int cnt = a.g.cnt[i];
-                const int64_t base = sell_base(a.g, i);
-                for (int k = 0; k < cnt; ++k) {
-                    load_rec<L::NR, L::PU>(Yp, a.g.nbr[base + (int64_t)k * kChunk], r);
+                    for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+                } else {
+                    load_rec<L::NR, L::PU>(Yp, i, r);
 #pragma unroll
-                    for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], r[D + m]); mx[m] = jmax(mx[m], r[D + m]); }
+                    for (int m = 0; m < M; ++m) { mn[m] = r[D + m]; mx[m] = r[D + m]; }
+                    const int cnt = a.g.cnt[i];
+                    const int64_t base = sell_base(a.g, i);
+                    for (int k = 0; k < cnt; ++k) {
+                        load_rec<L::NR, L::PU>(Yp, a.g.nbr[base + (int64_t)k * kChunk], r);
+#pragma unroll
+                        for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], r[D + m]); mx[m] = jmax(mx[m], r[D + m]); }
+                    }
                 }
                 const double dl = a.delta, d3 = dl * dl * dl;
                 const bool u2 = a.criterion == HCC_CRIT_U2 && a.range;
@@ -2181,7 +2188,9 @@ StepArgs hcc_stepper::args() const {
     a.counters = counters; a.clist = clist; a.clist2 = clist2; a.dlist = dlist; a.dmark = dmark; a.lam = lam;
     a.bmin = bmin;
     a.ext = ext;
-    a.ext_on = (mood_fused(desc) && desc.mood_criterion == HCC_CRIT_U1) ? 1 : 0;
+    // the full gathers store the neighbourhood extrema of their state for the DMP checks of MOODu1 and MOODu2 (the
+    // fused check in the flux pass, or k_mood_flag): the checkpoint's state is the one the stage's gather read
+    a.ext_on = (desc.mood_criterion == HCC_CRIT_U1 || desc.mood_criterion == HCC_CRIT_U2) ? 1 : 0;
     a.dtp = dtp;
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.feat = feat;
