@@ -1110,7 +1110,7 @@ __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64
 }
 
 template <int D, int M, int KIND, bool CHECK, int FV>
-__global__ __launch_bounds__(kBlock, 2) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+__global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int full, int64_t nblk) {
     const int64_t i = node_index(nblk);
     const int64_t N = a.g.N;
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_flux(StepArgs a, PredArgs pred, c
 
 // The full flux pass with the workgroup's halo (records, gradients, orders) staged in LDS (2-D).
 template <int D, int M, int KIND, bool CHECK, int FV>
-__global__ __launch_bounds__(kBlock, 2) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+__global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                          double* __restrict__ K, int64_t nblk) {
     extern __shared__ double sh[];
     const int64_t lb = logical_block(nblk);
