@@ -255,6 +255,47 @@ def test_euler2d_full_size_step_bit_exact(ctx):
     assert np.all(rho > 0) and np.all(p > 0)
 
 
+FULL_STEPS = 200       # CFL steps of the 1 013 475-node run compared with the oracle
+FULL_CHECK_EVERY = 10  # states, orders, flags and the inadmissible count compared every this many steps
+
+
+def test_euler2d_full_size_cfl_run_bit_exact(ctx):
+    """The benchmark run itself: the 1 013 475-node four-quadrant problem (h = 1/1000, seed 0) advanced by
+    hcc_step_cfl -- the device-side CFL estimate fused into the stage-1 gather, the path bench.py times -- for
+    FULL_STEPS steps from t = 0 beside the oracle's driver loop (dt = CFL x get_time_step, then the step;
+    test/time_integration.jl:107-111). dt is compared every step; states, orders, MOOD flags and the accepted
+    inadmissible count every FULL_CHECK_EVERY steps and at the end, bit for bit (the 1e-12 relative L-infinity bound
+    asserted too). The reference's own device bar: test/gpu_tests.jl:38-43."""
+    import os
+    wl = W.euler_2d(h=1e-3, seed=0)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    from oracle import oracle as O
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    o = O.OracleSim(wl["domain"], c["pos"], c["is_boundary"], c["tags"], wl["scheme"], threads=threads)
+    o.set_state(wl["U0"])
+    cfl, t, fired = wl["cfl"], 0.0, 0
+    for k in range(1, FULL_STEPS + 1):
+        st.step_cfl(cfl, float("inf"))
+        dt = cfl * o.time_step()
+        assert st.last_dt() == dt, f"dt differs at step {k}: {st.last_dt()!r} vs {dt!r}"
+        o.step(t, dt)
+        t += dt
+        if k % FULL_CHECK_EVERY == 0 or k == FULL_STEPS:
+            f, f_o = st.mood_flags(), o.mood_flags()
+            assert np.array_equal(f, f_o), f"MOOD flags differ at step {k}: {first_diff(f, f_o)}"
+            assert np.array_equal(st.orders(), o.orders()), f"orders differ at step {k}"
+            assert st.inadmissible() == o.inadmissible(), f"inadmissible count differs at step {k}"
+            U, U_o = st.download(), o.state()
+            assert rel_linf(U, U_o) <= STATE_RTOL, f"step {k}: rel Linf {rel_linf(U, U_o):.3e}"
+            assert np.array_equal(U, U_o), f"states differ at step {k}: {first_diff(U, U_o)}"
+            fired += int(f.sum())
+            print(f"step {k}: t = {t:.6e}, dt = {dt:.6e}, MOOD flags {int(f.sum())}, bit-exact", flush=True)
+    assert fired > 0, "MOOD never fired on the checked steps"
+
+
 def test_step_totals_match_per_step_values(ctx):
     """hcc_step returns once its work is queued; the per-step getters and the summed totals see the same steps."""
     wl = W.euler_2d(n=64, seed=2)
