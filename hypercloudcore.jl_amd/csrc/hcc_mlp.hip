@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 using namespace hcc;
@@ -365,6 +366,31 @@ void mlp_forward_list(const hcc_mlp* m, const uint16_t* X, const int32_t* idx, c
 
 static constexpr int kGradBlocks = 512;
 
+// Scratch of hcc_mlp_gradients_noalloc: the per-workgroup gradient partials (kGradBlocks rows of P + 1 floats,
+// 10.4 MiB), a file-scope device array that every GPU the library runs on holds until the process ends. Host code
+// never passes `scratch` itself to a kernel (on the host that name is a stand-in, not a device address): it passes
+// the address hipGetSymbolAddress returned for the current device, resolved once per device by hcc_mlp_create.
+constexpr int64_t kScratchRows = kGradBlocks;
+static __device__ __align__(256) float scratch[kScratchRows * (kMlpParams + 1)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the current device); resolved on the first call per device.
+static hipError_t scratch_address(int device, float** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 extern "C" int hcc_mlp_destroy(hcc_mlp* m) {
     if (!m) return HCC_OK;
     (void)hipSetDevice(m->ctx->device);
@@ -388,6 +414,8 @@ extern "C" int hcc_mlp_create(hcc_ctx* ctx, hcc_mlp** out) {
     HCC_TRY_HIP(ctx, dalloc(&m->part, (int64_t)kGradBlocks * (kMlpParams + 1)));
     for (float* p : {m->prm, m->m, m->v}) HCC_TRY_HIP(ctx, hipMemset(p, 0, sizeof(float) * kMlpParams));
     HCC_TRY_HIP(ctx, hipMemset(m->w, 0, sizeof(MlpW)));
+    float* sp = nullptr;   // the no-allocation gradient's scratch on this device (resolved once per device)
+    HCC_TRY_HIP(ctx, scratch_address(ctx->device, &sp));
     *out = m.release();
     return HCC_OK;
 }
@@ -449,13 +477,20 @@ extern "C" int hcc_mlp_forward(hcc_mlp* m, const uint16_t* X, int64_t n, float* 
     return HCC_OK;
 }
 
-// gradient of the mean weighted loss over the batch into m->g (+ loss), on the device
-static void grad_device(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight) {
-    const int64_t ntile = (n + T - 1) / T;
-    const int nb = (int)std::min<int64_t>(ntile, kGradBlocks);
+// workgroups of the gradient kernel for a batch of n samples (one tile of T samples each, at most kGradBlocks)
+static int grad_blocks(int64_t n) { return (int)std::min<int64_t>((n + T - 1) / T, kGradBlocks); }
+
+// gradient of the mean weighted loss over the batch into g (+ loss), on the device, with the workgroup partials in
+// part (grad_blocks(n) rows of P + 1 floats)
+static void grad_launch(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight, float* part,
+                        float* g) {
+    const int nb = grad_blocks(n);
     hipStream_t st = m->ctx->stream;
-    k_mlp_grad<<<nb, 64, 0, st>>>(m->w, reinterpret_cast<const __bf16*>(X), y, n, pos_weight, m->part);
-    k_mlp_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, st>>>(m->part, nb, m->g);
+    k_mlp_grad<<<nb, 64, 0, st>>>(m->w, reinterpret_cast<const __bf16*>(X), y, n, pos_weight, part);
+    k_mlp_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, st>>>(part, nb, g);
+}
+static void grad_device(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight) {
+    grad_launch(m, X, y, n, pos_weight, m->part, m->g);
 }
 
 extern "C" int hcc_mlp_gradients(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight,
@@ -472,6 +507,24 @@ extern "C" int hcc_mlp_gradients(hcc_mlp* m, const uint16_t* X, const uint8_t* y
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (grads) std::memcpy(grads, g.data(), sizeof(float) * kMlpParams);
     if (loss) *loss = g[kMlpParams];
+    return HCC_OK;
+}
+
+extern "C" int hcc_mlp_gradients_noalloc(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight,
+                                         float* grads) {
+    if (!m || n <= 0 || !X || !y || !grads)
+        return set_err(m ? m->ctx : nullptr, HCC_E_ARG, "hcc_mlp_gradients_noalloc: bad arguments");
+    hcc_ctx* ctx = m->ctx;
+    // the partials take one row per workgroup; every batch size fits (at most kScratchRows workgroups), the check
+    // guards the scratch against a change of the launch shape
+    if (grad_blocks(n) > kScratchRows)
+        return set_err(ctx, HCC_E_CAPACITY, "hcc_mlp_gradients_noalloc: batch needs more partial rows than the "
+                                            "static scratch holds");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    float* sp = nullptr;
+    HCC_TRY_HIP(ctx, scratch_address(ctx->device, &sp));   // resolved by hcc_mlp_create: a table read
+    grad_launch(m, X, y, n, pos_weight, sp, grads);
+    HCC_TRY_HIP(ctx, hipGetLastError());
     return HCC_OK;
 }
 

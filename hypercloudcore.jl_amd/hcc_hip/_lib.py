@@ -33,8 +33,8 @@ EXPORTS = [
     "hcc_get_mood_iterations", "hcc_get_moments", "hcc_rhs", "hcc_last_step_timing", "hcc_step_totals",
     "hcc_step_cfl", "hcc_last_dt", "hcc_grid_volumes", "hcc_total_mass", "hcc_solve", "hcc_set_source",
     "hcc_grid_concat", "hcc_time_steps", "hcc_mlp_create", "hcc_mlp_destroy", "hcc_mlp_set_params",
-    "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_train", "hcc_set_indicator",
-    "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
+    "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_gradients_noalloc", "hcc_mlp_train",
+    "hcc_set_indicator", "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
 ]
 MLP_PARAMS = 5313
 
@@ -111,6 +111,7 @@ def lib():
         "hcc_mlp_get_params": (C.c_int, [vp, vp]),
         "hcc_mlp_forward": (C.c_int, [vp, vp, i64, vp]),
         "hcc_mlp_gradients": (C.c_int, [vp, vp, vp, i64, C.c_float, vp, P(d)]),
+        "hcc_mlp_gradients_noalloc": (C.c_int, [vp, vp, vp, i64, C.c_float, vp]),
         "hcc_mlp_train": (C.c_int, [vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float, C.c_float, vp]),
         "hcc_set_indicator": (C.c_int, [vp, vp, C.c_float]),
         "hcc_indicator_collect": (C.c_int, [vp, C.c_int32]),

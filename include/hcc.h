@@ -236,6 +236,17 @@ int hcc_mlp_forward(hcc_mlp* m, const uint16_t* X, int64_t n, float* z);
  * y_s = 1, else 1 */
 int hcc_mlp_gradients(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight, float* grads,
                       double* loss);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight training loops, harnesses that forbid hipMalloc): hcc_mlp_gradients on device-resident data, allocating
+ * nothing. X (n x 16 bf16 rows) and y (n label bytes) are device pointers; grads (device, HCC_MLP_PARAMS + 1 floats)
+ * receives the gradient, then the loss. The work is queued on the context stream and the call returns without waiting.
+ * The workgroup partials go to a static device buffer of HCC_MLP_NOALLOC_SCRATCH_ROWS x (HCC_MLP_PARAMS + 1) floats
+ * (10.4 MiB, one per GPU, held until the process ends), resolved by hcc_mlp_create; every batch size fits, and a
+ * launch that would need more rows is refused with HCC_E_CAPACITY. Calls that overlap on one GPU (different contexts
+ * or threads) race on that buffer. Same bits as hcc_mlp_gradients (the same kernels and reduction order). */
+#define HCC_MLP_NOALLOC_SCRATCH_ROWS 512
+int hcc_mlp_gradients_noalloc(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, float pos_weight,
+                              float* grads);
 /* epochs of Adam (beta 0.9 / 0.999, eps 1e-8) over mini-batches of `batch` rows in the given order; losses (may be
  * NULL): the mean batch loss of each epoch */
 int hcc_mlp_train(hcc_mlp* m, const uint16_t* X, const uint8_t* y, int64_t n, int32_t epochs, int32_t batch, float lr,
