@@ -478,35 +478,48 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 j1 = src_slot(src, base, k + 1, cnt);
                 src.template head<NR>(jcur, rj);
             }
-            double dd[D];
-            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
-            const double w = CUT ? slot_weight(a.g, d2) : mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
-            double p[D];
+            // Flat slot: a neighbour whose state equals U_i (==, so +0 and -0 count as equal) has df = +-0 in every
+            // component, w (p df) = +-0 for the finite positive weight and finite basis, and b + (+-0) = b (the
+            // accumulators start at +0 and never hold -0). Its weight and fit update are skipped: the same bits.
+            bool flat = true;
 #pragma unroll
-            for (int d = 0; d < D; ++d) p[d] = dd[d] * invL;
-            if constexpr (CFL) {   // _cfl_weights_body!: sum_d |c_k,d| max(lambda_d(U_i), lambda_d(U_j))
-                double lj[D], v[BD];
-                src.lam(jcur, lj);
-                v[0] = w * 1.0;
+            for (int m = 0; m < M; ++m) flat = flat && (rj[D + m] == ri[D + m]);
+            if (CFL || !flat) {
+                double dd[D];
+                const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+                const double w = CUT ? slot_weight(a.g, d2) : mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+                double p[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) v[1 + d] = w * p[d];
-                double dv = 0.0;
+                for (int d = 0; d < D; ++d) p[d] = dd[d] * invL;
+                if constexpr (CFL) {   // _cfl_weights_body!: sum_d |c_k,d| max(lambda_d(U_i), lambda_d(U_j))
+                    double lj[D], v[BD];
+                    src.lam(jcur, lj);
+                    v[0] = w * 1.0;
 #pragma unroll
-                for (int r = 0; r < D; ++r) {
-                    double acc = X[r][0] * v[0];
+                    for (int d = 0; d < D; ++d) v[1 + d] = w * p[d];
+                    double dv = 0.0;
 #pragma unroll
-                    for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
-                    const double term = fabs(acc * invL) * jmax(li[r], lj[r]);
-                    dv = (r == 0) ? term : dv + term;
+                    for (int r = 0; r < D; ++r) {
+                        double acc = X[r][0] * v[0];
+#pragma unroll
+                        for (int c = 1; c < BD; ++c) acc = acc + X[r][c] * v[c];
+                        const double term = fabs(acc * invL) * jmax(li[r], lj[r]);
+                        dv = (r == 0) ? term : dv + term;
+                    }
+                    sum_c = sum_c + dv;
                 }
-                sum_c = sum_c + dv;
+                if (!flat) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const double df = rj[D + m] - ri[D + m];
+#pragma unroll
+                        for (int q = 0; q < D; ++q) b[q][m] = b[q][m] + w * (p[q] * df);
+                    }
+                }
             }
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 const double fj = rj[D + m];
-                const double df = fj - ri[D + m];
-#pragma unroll
-                for (int q = 0; q < D; ++q) b[q][m] = b[q][m] + w * (p[q] * df);
                 umax[m] = hw_max(umax[m], fj);
                 umin[m] = hw_min(umin[m], fj);
                 nanf[m] = nanf[m] | (fj != fj);
@@ -540,7 +553,12 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 for (int r = 1; r < D; ++r) c = c + inv[q][r] * b[r][m];
                 g[q * M + m] = c * invL;
             }
-        if (a.limiter != HCC_LIM_NONE) {
+        // A zero gradient (every coefficient +-0) reconstructs +-0 at every interface, which no tolerance
+        // (16 eps max(...) >= 0) passes: phi stays 1 and g phi = g, so the limiter's slot loop is skipped.
+        bool gzero = true;
+#pragma unroll
+        for (int q = 0; q < L::NG; ++q) gzero = gzero && (g[q] == 0.0);
+        if (a.limiter != HCC_LIM_NONE && !gzero) {
             // per component, for either bound (local max when the reconstruction rises, min when it falls): the
             // bound minus U_i and the tolerance 16 eps max(|U_i|, |bound|), the same operations hoisted out of the loop
             double* phi = phi_f;
@@ -791,6 +809,7 @@ struct FluxNode {
     double X[FV == 1 ? D : 1][D + 1];
     double w[FV == 1 ? D : 1];   // a moving grid: the particle's grid velocity (the frame of its flux)
     int oi, oi_t;   // interface order source, true order
+    bool flat_ok;   // FV = 0: the node's wave speeds are finite and 2 F_i does not overflow (flat_slot)
 };
 
 template <int D, int M, int KIND, int FV>
@@ -799,10 +818,26 @@ __device__ __forceinline__ void load_flux_node(const StepArgs& a, const double* 
     using L = Layout<D, M>;
     constexpr int BD = D + 1;
     load_rec<L::NR, L::PU>(Y, i, nd.ri);
+    load_rec<L::NG, L::GS>(a.G, i, nd.gi);
 #pragma unroll
     for (int m = 0; m < M; ++m) nd.fi[m] = nd.ri[D + m];
-    load_rec<L::NG, L::GS>(a.G, i, nd.gi);
-    Pde<D, M, KIND>::flux(a.pde, nd.fi, nd.Fi);
+    nd.flat_ok = false;
+    if constexpr (FV == 0) {
+        double lam[D];
+        Pde<D, M, KIND>::flux_eig(a.pde, nd.fi, nd.Fi, lam);   // the flux of flux(), and the wave speeds
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            ok = ok && isfinite(lam[d]);
+#pragma unroll
+            for (int m = 0; m < M; ++m) ok = ok && isfinite(nd.Fi[d][m] + nd.Fi[d][m]);
+        }
+#pragma unroll
+        for (int q = 0; q < L::NG; ++q) ok = ok && (nd.gi[q] == 0.0);
+        nd.flat_ok = ok;
+    } else {
+        Pde<D, M, KIND>::flux(a.pde, nd.fi, nd.Fi);
+    }
     nd.oi = a.orders[i];
     nd.oi_t = nd.oi;
     if (FV == 1 && a.packed) { nd.oi_t = nd.oi >> 4; nd.oi &= 15; }
@@ -914,6 +949,25 @@ __device__ __forceinline__ void slot_flux(const StepArgs& a, const FluxNode<D, M
     }
 }
 
+// A flat slot of the flux loop (FV = 0) contributes exact zeros to the fit, so flux_node skips it. Flat: both
+// gradients are zero (every coefficient +-0), the neighbour's state equals U_i (==), and the node's wave speeds are
+// finite with 2 F_i finite (nd.flat_ok). Then both interface states equal U_i up to the signs of zeros (U + sum of
+// +-0 terms), so F(f_ij), F(f_ji) and F_i agree in every non-zero value (the Euler flux and speeds only multiply and
+// add these values; a zero's sign changes no non-zero result, and |u| + c is the same non-negative value), the
+// speeds agree, f_R - f_L = +-0 and s (+-0) = +-0. Each component of the Rusanov flux is 0.5 ((F + F) - (+-0)) = F
+// exactly (2 F is exact, and finite by flat_ok), or +-0 where F is a zero, so dF = F* - F_i = +-0, w (pt dF) = +-0
+// and b + (+-0) = b: the accumulators start at +0 and never hold -0.
+template <int D, int M, int KIND, int FV>
+__device__ __forceinline__ bool flat_slot(const FluxNode<D, M, KIND, FV>& nd, const double (&rj)[Layout<D, M>::NR],
+                                          const double (&gj)[Layout<D, M>::NG]) {
+    bool flat = nd.flat_ok;
+#pragma unroll
+    for (int m = 0; m < M; ++m) flat = flat && (rj[D + m] == nd.fi[m]);
+#pragma unroll
+    for (int q = 0; q < Layout<D, M>::NG; ++q) flat = flat && (gj[q] == 0.0);
+    return flat;
+}
+
 // evaluate_sources of the explicit sources added to K_E (TimeSteppers.jl:354): zero(U) without one, the decay source
 // -rate_i U_i with one
 template <int M>
@@ -1008,6 +1062,8 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
 #pragma unroll
         for (int m = 0; m < M; ++m) sacc[m] = 0.0;
         auto slot = [&](const double (&rj)[NR], const double (&gj)[NG], int oj) {
+            if constexpr (FV == 0)
+                if (flat_slot<D, M, KIND, FV>(nd, rj, gj)) return;
             double dF[D][M], w, pt[BD], ctr[M];
             slot_flux<D, M, KIND, FV>(a, nd, rj, gj, oj, dF, w, pt, ctr);
             if (FV == 1 && a.streamed) {
