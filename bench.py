@@ -77,34 +77,19 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_euler2d_1M.md")
-
-
-def measured_traffic():
-    """HBM bytes per launch of the full flux pass from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (calibrated against a plain device copy); None when the summary is absent."""
-    try:
-        with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                if line.startswith("traffic_bytes k_flux_staged ="):
-                    return float(line.split("=")[1])
-    except OSError:
-        pass
-    return None
-
-
-def ensemble_main(args):
+def ensemble_run(args, ctx=None):
     """The batched form: `members` four-quadrant Euler boxes (one spacing h = 1e-3, node counts log-uniform in
-    [n_min, n_max], seeded) in one ensemble grid advanced by one stepper, each member with its own CFL step, against
-    the plain form (a stepper per member, stepped in turn). Both are timed over the same steps; a few members are
-    compared bit for bit afterwards."""
+    [n_min, n_max], seeded) in one ensemble grid advanced by one stepper, each member with its own CFL step, with and
+    without the learned indicator, against the plain form (a stepper per member, stepped in turn). All forms are timed
+    over the same number of steps from the same initial states; members are compared bit for bit afterwards."""
     import hcc_hip
     from hcc_hip import Context, Ensemble, Grid, Stepper
     from hcc_hip import workloads as W
 
     t_setup = time.perf_counter()
     wls = W.euler_2d_ensemble(args.members, args.n_min, args.n_max, h=1e-3, seed=args.seed)
-    ctx = Context(0)
+    own_ctx = ctx is None
+    ctx = ctx or Context(0)
     grids, singles = [], []
     for wl in wls:
         c = wl["cloud"]
@@ -127,9 +112,10 @@ def ensemble_main(args):
         st_i.set_indicator(net, IND.DEFAULT_THRESHOLD)
     nodes = int(ens.sizes.sum())
     t_setup = time.perf_counter() - t_setup
-    print(f"# setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
+    print(f"# ensemble setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
           file=sys.stderr, flush=True)
     cfl = wls[0]["cfl"]
+    steps, warmup = args.ensemble_steps, args.ensemble_warmup
 
     def run(steppers, k):
         for _ in range(k):
@@ -137,64 +123,68 @@ def ensemble_main(args):
                 st.step_cfl(cfl)
         hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
 
-    run([st_e], args.warmup)
-    run(singles, args.warmup)
-    t0 = time.perf_counter()
-    run([st_e], args.steps)
-    wall_e = time.perf_counter() - t0
-    print(f"# batched: {args.steps} steps in {wall_e:.3f} s", file=sys.stderr, flush=True)
-    wall_i = None
-    if st_i is not None:
-        run([st_i], args.warmup)
-        st_i.totals(reset=True)
-        st_e.totals(reset=True)
+    def timed(steppers):
+        run(steppers, warmup)
         t0 = time.perf_counter()
-        run([st_i], args.steps)
-        wall_i = time.perf_counter() - t0
-        print(f"# batched with the indicator: {args.steps} steps in {wall_i:.3f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    run(singles, args.steps)
-    wall_p = time.perf_counter() - t0
-    print(f"# plain: {args.steps} steps in {wall_p:.3f} s", file=sys.stderr, flush=True)
+        run(steppers, steps)
+        return time.perf_counter() - t0
+
+    wall_e = timed([st_e])
+    print(f"# ensemble batched: {steps} steps in {wall_e:.3f} s", file=sys.stderr, flush=True)
+    wall_i = timed([st_i]) if st_i is not None else None
+    if wall_i is not None:
+        print(f"# ensemble batched with the indicator: {steps} steps in {wall_i:.3f} s", file=sys.stderr, flush=True)
+    wall_p = timed(singles)
+    print(f"# ensemble plain: {steps} steps in {wall_p:.3f} s", file=sys.stderr, flush=True)
     U_full = st_e.download()
     U_e = ens.split(U_full)
     for m in sorted({0, len(wls) // 2, len(wls) - 1}):
         assert np.array_equal(U_e[m], singles[m].download()), f"member {m} differs from its own stepper"
     stages = 3
-    value = nodes * stages * args.steps / wall_e
-    plain = nodes * stages * args.steps / wall_p
+    value = nodes * stages * steps / wall_e
+    plain = nodes * stages * steps / wall_p
     out = {
-        "metric": "node-stage updates/s, batched ensemble of 2D Euler MUSCL+MOOD instances",
-        "value": value,
-        "unit": "node-stage updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall_e / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{len(wls)} four-quadrant 2-D Euler boxes, h=1e-3, node counts log-uniform in "
-                               f"[{args.n_min}, {args.n_max}] (seed {args.seed}), each its own seeded IC, jitter and CFL "
-                               f"step; MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4",
-                   "members": len(wls), "nodes": nodes, "ensemble_nodes_with_padding": int(ens.N),
-                   "smallest": int(ens.sizes.min()), "largest": int(ens.sizes.max())},
-        "plain": {"value": plain, "ms_per_step": 1e3 * wall_p / args.steps,
+        "workload": f"{len(wls)} four-quadrant 2-D Euler boxes, h=1e-3, node counts log-uniform in "
+                    f"[{args.n_min}, {args.n_max}] (seed {args.seed}), each its own seeded IC, jitter and CFL step; "
+                    f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4; {steps} timed steps after {warmup} "
+                    f"warm-up steps, every form from the same initial states",
+        "members": len(wls), "nodes": nodes, "ensemble_nodes_with_padding": int(ens.N),
+        "smallest": int(ens.sizes.min()), "largest": int(ens.sizes.max()), "setup_s": round(t_setup, 1),
+        "batched": {"value": value, "ms_per_step": 1e3 * wall_e / steps,
+                    "form": "one ensemble grid, one stepper (hcc_grid_concat + hcc_step_cfl)"},
+        "plain": {"value": plain, "ms_per_step": 1e3 * wall_p / steps,
                   "form": "one stepper per member, stepped in turn (hcc_step_cfl)"},
-        "speedup_vs_plain": value / plain,
+        "batched_vs_plain": value / plain,
     }
     if wall_i is not None:
         U_i = st_i.download()
         dev = float(np.abs(U_i - U_full).sum() / np.abs(U_full).sum())
-        v_i = nodes * stages * args.steps / wall_i
-        out["learned_indicator"] = {"value": v_i, "ms_per_step": 1e3 * wall_i / args.steps,
-                                    "speedup_vs_plain": v_i / plain,
-                                    "relative_l1_deviation_from_plain_mood": dev,
-                                    "note": "batched form with the shipped indicator network; same steps as the "
-                                            "batched plain form"}
-    print(json.dumps(out), flush=True)
+        v_i = nodes * stages * steps / wall_i
+        out["batched_indicator"] = {"value": v_i, "ms_per_step": 1e3 * wall_i / steps, "vs_plain": v_i / plain,
+                                    "vs_batched": v_i / value, "relative_l1_deviation_from_plain_mood": dev,
+                                    "form": "the batched form with the shipped indicator network"}
+    for st in singles + [st_e] + ([st_i] if st_i is not None else []):
+        st.close()
+    if net is not None:
+        net.close()
+    for g in grids:
+        g.close()
+    ens.close()
+    if own_ctx:
+        ctx.close()
+    return out
+
+
+def ensemble_main(args):
+    out = ensemble_run(args)
+    b = out["batched_indicator"] if "batched_indicator" in out else out["batched"]
+    line = {"metric": "node-stage updates/s, batched ensemble of 2D Euler MUSCL+MOOD instances", "value": b["value"],
+            "unit": "node-stage updates/s", "n_gpus": 1, "steps": args.ensemble_steps,
+            "warmup": args.ensemble_warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": out["workload"], "members": out["members"], "nodes": out["nodes"]},
+            "ensemble": out}
+    print(json.dumps(line), flush=True)
 
 
 VARIANTS = ("moving", "imex", "epd2_u2", "strict_epd0", "streamed", "aligned", "solve")
@@ -279,6 +269,42 @@ def variant_main(args):
     print(json.dumps(out), flush=True)
 
 
+def host_cpu():
+    """The host's CPU model and core counts (the CPU baseline's context)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def profile_value(name, key):
+    """A figure recorded in a committed rocprofv3 summary under profiles/ (a line `key = value`), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + " ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def timed_steps(st, k, cfl, sync):
+    """k driver-loop steps (CFL estimate + step, dt kept on the device), wall seconds."""
+    t0 = time.perf_counter()
+    for _ in range(k):
+        st.step_cfl(cfl)
+    sync()
+    return time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,22 +312,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1000, help="nodes per axis (h = 1/n)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--run-steps", type=int, default=200, help="steps of the configured run (config: 200)")
+    ap.add_argument("--window", type=int, default=100,
+                    help="the timed steps start after this many steps of the run (set-up steps + warm-up steps)")
     ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched-ensemble block")
     ap.add_argument("--host-dt", action="store_true",
                     help="return dt to the host between the CFL estimate and the step (hcc_time_step + hcc_step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for profiles/)")
     ap.add_argument("--ensemble", action="store_true",
-                    help="the batched form: many Euler boxes of uneven size in one ensemble grid, against a stepper "
-                         "per box")
+                    help="only the batched form: many Euler boxes of uneven size in one ensemble grid, against a "
+                         "stepper per box")
     ap.add_argument("--indicator", action="store_true",
-                    help="also run with the learned troubled-node indicator (the shipped network)")
+                    help="the headline run with the learned troubled-node indicator (the shipped network)")
+    ap.add_argument("--no-ensemble-indicator", dest="ens_indicator", action="store_false",
+                    help="the ensemble block without its learned-indicator form")
     ap.add_argument("--variant", choices=VARIANTS, default=None,
                     help="a scheme variant built beside the headline path, on the same 1 M-node problem")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--n-min", type=int, default=20000)
     ap.add_argument("--n-max", type=int, default=400000)
+    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-warmup", type=int, default=2)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -311,6 +345,7 @@ def main():
               "(the multi-GPU decomposition is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.ensemble:
+        args.indicator = args.indicator or args.ens_indicator
         ensemble_main(args)
         return
     if args.variant:
@@ -325,24 +360,31 @@ def main():
     wl = W.euler_2d(h=1.0 / args.n, seed=args.seed)
     c = wl["cloud"]
     ctx = Context(0)
+    sync = lambda: hcc_hip.lib().hcc_ctx_synchronize(ctx.h)   # noqa: E731
     g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
     st = Stepper(ctx, g, wl["scheme"])
-    st.upload(wl["U0"])
     net = None
     if args.indicator:
         from hcc_hip import MLP
         from hcc_hip import indicator as IND
         net = MLP(ctx, IND.default_params())
         st.set_indicator(net, IND.DEFAULT_THRESHOLD)
+    cfl = wl["cfl"]
+    # The timed window sits inside the configured run: its input is the state after `first` steps from t = 0,
+    # formed on the device during set-up (untimed), then W warm-up steps, then the K timed steps.
+    first = max(0, args.window - args.warmup)
+    st.upload(wl["U0"])
+    for _ in range(first):
+        st.step_cfl(cfl)
+    sync()
     t_setup = time.perf_counter() - t_setup
-    print(f"# setup {t_setup:.1f} s: N={g.N} nodes, S={g.S} slots, max_nb={g.max_nb}", file=sys.stderr, flush=True)
+    print(f"# setup {t_setup:.1f} s: N={g.N} nodes, S={g.S} slots, max_nb={g.max_nb}; {first} set-up steps",
+          file=sys.stderr, flush=True)
 
     t = 0.0
     for _ in range(args.warmup):
-        dt = wl["cfl"] * st.time_step()
-        st.step(t, dt)
-        t += dt
-    hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+        st.step_cfl(cfl)
+    sync()
     st.totals(reset=True)
     # the timed loop is the driver loop of the reference: each step is the CFL estimate (get_time_step) and the
     # stepper. By default both run back to back on the device (hcc_step_cfl: dt = CFL * CFL step kept on the
@@ -351,12 +393,12 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         if args.host_dt:
-            dt = wl["cfl"] * st.time_step()
+            dt = cfl * st.time_step()
             st.step(t, dt)
             t += dt
         else:
-            st.step_cfl(wl["cfl"])
-    hcc_hip.lib().hcc_ctx_synchronize(ctx.h)
+            st.step_cfl(cfl)
+    sync()
     wall = time.perf_counter() - t0
     n_steps, iters, tot_ms, tot_g_ms, tot_f_ms = st.totals(reset=True)
     assert n_steps == args.steps, (n_steps, args.steps)
@@ -379,6 +421,29 @@ def main():
     gb = gather_pass_bytes(N_int, S_int, D, M)
     achieved = fb / (flux_avg_ms * 1e-3) / 1e9
     model_b = W.bytes_per_node_stage(D, M, nnb)
+
+    # the whole configured run from t = 0 (run_steps CFL steps), timed as one region
+    st.upload(wl["U0"])
+    sync()
+    wall_run = timed_steps(st, args.run_steps, cfl, sync)
+    # the dense state: the four quadrants times a smooth factor, so that no neighbourhood is constant and every
+    # slot of every pass does its full arithmetic (the same cloud and scheme)
+    x = c["pos"]
+    Ud = wl["U0"].copy()
+    fac = 1.0 + 0.01 * np.sin(2 * np.pi * 3 * x[:, 0]) * np.sin(2 * np.pi * 5 * x[:, 1])
+    Ud[:, 0] *= fac
+    Ud[:, 3] *= fac
+    st.upload(Ud)
+    for _ in range(2):
+        st.step_cfl(cfl)
+    sync()
+    st.totals(reset=True)
+    dense_k = 10
+    wall_dense = timed_steps(st, dense_k, cfl, sync)
+    nd_, it_d, tot_d, g_d, f_d = st.totals(reset=True)
+
+    traffic = profile_value("hbm_traffic_euler2d_1M.md", "traffic_bytes k_flux_staged")
+    valu = profile_value("sq_counters_euler2d_1M.md", "valu_issue_share k_flux_staged")
     out = {
         "metric": METRIC,
         "value": value,
@@ -393,21 +458,50 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"2-D Euler four-quadrant, jittered h=1/{args.n} unit square with outflow ghost band, "
-                               f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4",
+                               f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4, {args.run_steps}-step run",
                    "nodes": N, "interior_nodes": N_int, "mean_neighbours": round(nnb, 3), "stages": stages,
-                   "seed": args.seed, "learned_indicator": bool(args.indicator)},
+                   "seed": args.seed, "learned_indicator": bool(args.indicator),
+                   "timed_steps": f"steps {first + args.warmup + 1}..{first + args.warmup + args.steps} of the "
+                                  f"{args.run_steps}-step run from t = 0 (the first {first} formed on the GPU in "
+                                  f"set-up, then {args.warmup} warm-up steps)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_flux_staged<2,4,Euler> (full pass: flux, MOOD check, stage predictor)", "bytes_per_launch": fb,
-                     "avg_launch_ms": flux_avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/hbm_traffic_euler2d_1M.md (rocprofv3 FETCH_SIZE / WRITE_SIZE of this "
+                                       "kernel, calibrated on a device copy)",
+                     "fp64_valu_issue_frac": valu,
+                     "fp64_valu_issue_source": "profiles/sq_counters_euler2d_1M.md (SQ_INSTS_VALU over 256 CUs x "
+                                               "2.4 GHz x kernel time)",
+                     "kernel": "k_flux_staged<2,4,Euler> (full pass: flux, MOOD check, stage predictor)",
+                     "bytes_per_launch": fb, "avg_launch_ms": flux_avg_ms},
         "detail": {"gather_avg_ms": gather_avg_ms, "gather_bytes": gb, "flux_avg_ms": flux_avg_ms,
                    "step_event_ms": tot_ms / n_steps, "mood_iterations": iters, "flags_last_step": flags,
                    "model_bytes_per_node_stage": model_b,
-                   "model_roofline_frac": value * model_b / (HBM_PEAK_GBS * 1e9)},
+                   "model_roofline_frac": value * model_b / (HBM_PEAK_GBS * 1e9),
+                   "full_run": {"steps": args.run_steps, "ms_per_step": 1e3 * wall_run / args.run_steps,
+                                "value": N * stages * args.run_steps / wall_run,
+                                "form": "every step of the run from t = 0, one timed region"},
+                   "dense_state": {"steps": dense_k, "ms_per_step": 1e3 * wall_dense / dense_k,
+                                   "value": N * stages * dense_k / wall_dense,
+                                   "gather_avg_ms": g_d / (stages * nd_), "flux_avg_ms": f_d / (stages * nd_),
+                                   "mood_iterations": it_d,
+                                   "form": "the four quadrants times 1 + 0.01 sin(6 pi x) sin(10 pi y): no constant "
+                                           "neighbourhood, every slot computed in full"}},
     }
+    st.close()
+    if net is not None:
+        net.close()
+    g.close()
+    if not args.no_ensemble:
+        ea = argparse.Namespace(**vars(args))
+        ea.indicator = args.ens_indicator
+        out["ensemble"] = ensemble_run(ea, ctx)
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        # the oracle on the host cores of this process: all of them available to it, at most the 16 that a one-GPU
+        # share of the box holds
+        host = host_cpu()
+        threads = max(1, min(host["affinity"], 16))
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps, threads)
+        out["cpu_baseline"].update(host)
     print(json.dumps(out), flush=True)
 
 

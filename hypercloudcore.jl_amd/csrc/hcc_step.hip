@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <thread>
 #include <vector>
 
 using namespace hcc;
@@ -2272,6 +2273,8 @@ struct Variant {
             if (full && s->feat_on) {   // the full pass with the indicator features
                 if (s->lds_gather)
                     k_gather_staged<D, M, KIND, false, true><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
+                else if (s->moving)   // the stage geometry of a moving grid: weight 0 beyond R, as the moments
+                    k_gather<D, M, KIND, true, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
                 else k_gather<D, M, KIND, false, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
                 return;
             }
@@ -2840,11 +2843,15 @@ struct StepRunner {
         return hipGetLastError();
     }
     // The count published in slot q. Polls for up to a second; then waits for the stream, so that a failed launch
-    // surfaces as its error instead of a hang.
+    // surfaces as its error instead of a hang. The first polls spin (the usual wait is a few to a few tens of
+    // microseconds, one fixed-point iteration); after kSpinPolls the thread yields between polls, so that steppers
+    // driven from several threads, or other work on the host, share the cores instead of each burning one.
+    static constexpr unsigned kSpinPolls = 4096;
     hipError_t wait_published(int q, unsigned long long& n) {
         volatile unsigned long long* v = &s->h_pub[q];
         const auto t0 = std::chrono::steady_clock::now();
         for (unsigned spin = 0; (n = *v) == kPending; ++spin) {
+            if (spin >= kSpinPolls) std::this_thread::yield();
             if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
                 hipError_t e = hipStreamSynchronize(s->ctx->stream);
                 if (e != hipSuccess) return e;

@@ -171,3 +171,22 @@ def test_moving_grid_rejects_unsupported_setups(ctx):
     sch2 = W.scheme(2, L.HCC_PDE_LINADV, adv_speed=[[1.0, 0.5], [1.0, 0.5]], mover=L.HCC_MOVER_LAGRANGIAN)
     with pytest.raises(L.HccError, match="M = 1"):
         Stepper(ctx, g2, sch2)
+
+
+def test_indicator_on_a_moving_grid_leaves_the_step_unchanged(ctx):
+    """The indicator's feature gather on a moving grid runs on the stage geometry (weight 0 beyond R, as the moments):
+    Lagrangian Euler with a network that never fires and with training-data collection on is bit-exact with the
+    oracle's plain moving step, five CFL steps."""
+    from hcc_hip import MLP
+    from oracle import mlp as NN
+    wl = _euler_moving("epd1_u1")
+    g, st, o = build_pair(ctx, wl)
+    p = np.zeros(NN.P, np.float32)
+    p[-1] = -100.0                      # logit -100 everywhere: never above the threshold
+    net = MLP(ctx, p)
+    st.set_indicator(net, 0.0)
+    st.collect(True)
+    _run(g, st, o, 5, cfl=wl["cfl"])
+    assert st.indicator_count() == 0
+    assert np.sum(st.mood_flags()) > 0
+    net.close()
