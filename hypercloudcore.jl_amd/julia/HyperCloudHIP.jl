@@ -3,13 +3,26 @@
 
 Julia binding of libhcc (include/hcc.h): the MI355X time-step path behind HyperCloudCore.jl's operator API.
 
-A `HIPRKTimeStepper` is a `TimeStepper` with the functor signature of `GeneralRKTimeStepper`
-(src/TimeStepping/TimeSteppers.jl:531): `ts(eq, pg, t, dt)` advances `pg.rhos` in place. `hip_time_step(ts, pg, eq)`
-replaces `get_time_step(pg, eq, interp)` (src/Grid/ParticleGrids.jl:386). PDE types map to compiled kernel variants
-through `hip_descriptor`; a PDE without one raises (there is no fallback path).
+A `HIPRKTimeStepper` is a `TimeStepper` with the functor signature and contract of `GeneralRKTimeStepper`
+(src/TimeStepping/TimeSteppers.jl:531; src/Types.jl:174-183, "updates the grid in-place"): `ts(eq, pg, t, dt)` advances
+the state and, by default, writes it back to `pg.rhos` (and, on a moving grid, the moved positions to
+`pg.core.positions`) before returning. `resident = true` keeps the state on the device between calls instead (no
+PCIe copy per step; read it with `download!`), which `hip_solve_equation` and device-resident drivers use.
 
-This file is not exercised in this repository (no Julia toolchain); the Python host package exercises the same
-C ABI in the tests.
+The stepper carries a `divergence_interpolator` field, a `HIPDivergence`, for which `get_time_step(pg, eq, d)` is
+defined: the reference's driver loop (test/time_integration.jl:96-111) takes `timestepper.divergence_interpolator`
+and calls `get_time_step(pg, eq, div_interp)`, then `timestepper(eq, pg, t, dt)`, then saves `pg.rhos`, and so runs
+unchanged on this stepper.
+
+Construction from the reference's objects: `HIPRKTimeStepper(eq, pg, interp::MUSCL, mood::MOOD, tableau)` reads the
+MUSCL order, limiter and mode, numerical flux and streamed flag, the MOOD strategy and criterion, the tableau (both
+parts) and the boundary-condition map and mover of `pg`; `substep_ratio` defaults to the reference's own
+`1 / ssp_coefficient(tableau)` (TimeSteppers.jl:218). PDE types map to compiled kernel variants through
+`hip_descriptor`; a PDE, limiter, flux, strategy, criterion or boundary condition without a compiled variant raises
+(there is no fallback path).
+
+This file is not exercised in this repository (no Julia toolchain); the Python host package exercises the same C ABI
+in the tests, and tests/test_julia_binding.py checks this file's descriptor layouts against include/hcc.h.
 """
 module HyperCloudHIP
 
@@ -77,22 +90,39 @@ hip_descriptor(eq) = error("no compiled HIP kernel for $(typeof(eq)): add a `hip
 
 pad3(v, x) = ntuple(i -> i <= length(v) ? v[i] : x, 3)
 
-mutable struct HIPRKTimeStepper <: TimeStepper
+"""
+    HIPDivergence
+
+The `divergence_interpolator` of a `HIPRKTimeStepper`: `get_time_step(pg, eq, d::HIPDivergence)` is the CFL step
+(CFL number 1) of the device state, computed on the device with the cached weights (hcc_time_step).
+"""
+mutable struct HIPDivergence <: HyperCloudCore.DivergenceInterpolator
+    st::Ptr{Cvoid}
+    ctx::Ptr{Cvoid}
+    owner::Any   # the HIPRKTimeStepper (set after construction)
+end
+
+mutable struct HIPRKTimeStepper <: HyperCloudCore.TimeStepper
     ctx::Ptr{Cvoid}
     grid::Ptr{Cvoid}
     st::Ptr{Cvoid}
     N::Int
     M::Int
     nets::Vector{Ptr{Cvoid}}   # indicator networks owned by the stepper (destroyed before its context)
+    resident::Bool             # false: every call exchanges the state with pg.rhos (the reference's contract)
+    moving::Bool               # a grid mover: the functor also writes the moved positions to pg.core.positions
+    fresh::Bool                # the device state was uploaded from pg.rhos since the last step (by get_time_step)
+    divergence_interpolator::HIPDivergence
 end
 
 """
-    HIPRKTimeStepper(eq, pg, desc::DomainDesc, interp::MUSCL, mood::MOOD, tableau; device = 0, substep_ratio)
+    HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device = 0, resident = false)
 
-Uploads the nodes of `pg` and builds the device grid (neighbour search on the GPU) and the stepper.
-`substep_ratio` is `ts.substep_ratio` of the equivalent `GeneralRKTimeStepper` (Julia's bisection result).
+Uploads the nodes of `pg` and builds the device grid (neighbour search on the GPU) and the stepper from explicit
+descriptors. `sdesc.substep_ratio` is `ts.substep_ratio` of the equivalent `GeneralRKTimeStepper` (Julia's bisection
+result). The constructor from the reference's objects below fills both descriptors.
 """
-function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::Integer = 0)
+function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::Integer = 0, resident::Bool = false)
     ctx = Ref{Ptr{Cvoid}}(C_NULL)
     checkrc(ccall((:hcc_ctx_create, libhcc), Cint, (Cint, Ref{Ptr{Cvoid}}), device, ctx), C_NULL)
     N = pg.meta.N
@@ -111,7 +141,9 @@ function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::
     st = Ref{Ptr{Cvoid}}(C_NULL)
     checkrc(ccall((:hcc_stepper_create, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{StepperDesc}, Ref{Ptr{Cvoid}}),
                   ctx[], grid[], sdesc, st), ctx[])
-    ts = HIPRKTimeStepper(ctx[], grid[], st[], N, Int(sdesc.M), Ptr{Cvoid}[])
+    div = HIPDivergence(st[], ctx[], nothing)
+    ts = HIPRKTimeStepper(ctx[], grid[], st[], N, Int(sdesc.M), Ptr{Cvoid}[], resident, sdesc.mover != 0, true, div)
+    div.owner = ts
     upload!(ts, pg)
     finalizer(ts) do t
         ccall((:hcc_stepper_destroy, libhcc), Cint, (Ptr{Cvoid},), t.st)
@@ -121,6 +153,107 @@ function HIPRKTimeStepper(eq, pg, desc::DomainDesc, sdesc::StepperDesc; device::
     end
     return ts
 end
+
+# ---- descriptors from the reference's objects ------------------------------------------------------------------
+const HC = HyperCloudCore
+
+hip_limiter(l) = error("no compiled limiter for $(typeof(l)) (compiled: NoLimiter, BarthJespersen, Venkatakrishnan)")
+hip_limiter(::HC.NoLimiter) = (Int32(0), Int32(0))
+hip_limiter(::HC.BarthJespersenLimiter{Mode}) where {Mode} = (Int32(1), Int32(Mode === :hard))
+hip_limiter(::HC.VenkatakrishnanLimiter{Mode}) where {Mode} = (Int32(2), Int32(Mode === :hard))
+
+hip_flux(f) = error("no compiled numerical flux for $(typeof(f)) (compiled: RusanovFlux, AlignedRusanovFlux)")
+hip_flux(::HC.RusanovFlux) = Int32(0)
+hip_flux(::HC.AlignedRusanovFlux) = Int32(1)
+
+hip_strategy(s) = error("no compiled MOOD strategy for $(typeof(s))")
+hip_strategy(::HC.NoStrategy) = Int32(0)
+hip_strategy(::HC.EPD0) = Int32(1)
+hip_strategy(::HC.EPD1) = Int32(2)
+hip_strategy(::HC.EPD2) = Int32(3)
+hip_strategy(::HC.StrictEPD0) = Int32(4)
+
+"`(criterion, delta, plateau)` of a MOOD criterion (MOODu2's `ratio` is unused at MUSCL order 2)."
+hip_criterion(c) = error("no compiled MOOD criterion for $(typeof(c))")
+hip_criterion(::HC.NoMOOD) = (Int32(0), 0.0, 0.0)
+hip_criterion(c::HC.MOODu1) = (Int32(1), Float64(c.d), 0.0)
+hip_criterion(::HC.MOODPAD) = (Int32(2), 0.0, 0.0)
+hip_criterion(::HC.OnlyMOOD) = (Int32(3), 0.0, 0.0)
+hip_criterion(c::HC.MOODu2) = (Int32(4), Float64(c.d), Float64(c.plateau))
+
+hip_bc(bc) = error("no compiled boundary condition for $(typeof(bc)) (compiled: OutflowBC)")
+hip_bc(::HC.OutflowBC) = HCC_BC_OUTFLOW
+
+_muscl_order(::HC.MUSCL{D, M, T, B, MO}) where {D, M, T, B, MO} = Int32(MO)
+_streamed(interp::HC.MUSCL) = Int32(interp.streamed isa Val{true})
+_mat5(a) = ntuple(i -> ntuple(j -> (i <= size(a, 1) && j <= size(a, 2)) ? Float64(a[i, j]) : 0.0, 5), 5)
+_vec5(v) = ntuple(i -> i <= length(v) ? Float64(v[i]) : 0.0, 5)
+_pad(v, n, x) = ntuple(i -> i <= length(v) ? v[i] : x, n)
+
+"""
+    hip_domain_desc(pg) -> DomainDesc
+
+The grid descriptor of a `ParticleGrid`: the geometry's box and periodicity, the nominal spacing, the weight
+function's range and alpha (the ghost canvas and the bins follow from these as in ComputationalDomain). The jitter
+is zero and the seed unused: the nodes are `pg.core.positions`, not regenerated.
+"""
+function hip_domain_desc(pg::HC.ParticleGrid{D}) where {D}
+    geo, wf = pg.geometry, pg.neighbor.weight_func
+    DomainDesc(Int32(D), pad3(Float64.(geo.mins), 0.0), pad3(Float64.(geo.maxs), 0.0),
+               pad3(Int32.(geo.is_periodic), Int32(0)), pad3(Float64.(pg.meta.dx), 0.0), Float64(pg.meta.R),
+               Float64(wf.alpha), (0.0, 0.0, 0.0), UInt64(0))
+end
+
+"""
+    hip_stepper_desc(eq, pg, interp::MUSCL, mood::MOOD, tableau; substep_ratio) -> StepperDesc
+
+The stepper descriptor of `GeneralRKTimeStepper(eq, interp, mood, tableau)` on `pg` (TimeSteppers.jl:195-229):
+`hip_descriptor(eq)` gives the PDE kernel `(kind, gamma, speeds)` (speeds: `speeds[d][m]`, the advection velocity of
+component m along axis d); the tableau's explicit part `(a_t, b_t, c_t)` and implicit part `(a, b, c)`; the
+boundary-condition map of `pg.geometry`; the mover of `pg`.
+"""
+function hip_stepper_desc(eq::HC.HyperbolicPDE{D, M}, pg, interp::HC.MUSCL, mood::HC.MOOD, tableau;
+                          substep_ratio::Real) where {D, M}
+    kind, gamma, speeds = hip_descriptor(eq)
+    lim, mode = hip_limiter(interp.limiter)
+    crit, delta, plateau = hip_criterion(mood.criterion)
+    s = length(tableau.b_t)
+    s <= 5 || error("tableaus of up to 5 stages are compiled")
+    bcm = pg.geometry.bc_map
+    length(bcm) <= 16 || error("up to 16 boundary-condition tags are compiled")
+    tags = sort(collect(keys(bcm)))
+    mover, malpha = hip_mover_code(pg.mover)
+    adv = ntuple(d -> ntuple(m -> (d <= length(speeds) && m <= length(speeds[d])) ? Float64(speeds[d][m]) : 0.0, 8), 3)
+    StepperDesc(Int32(M), Int32(kind), Float64(gamma), adv, _muscl_order(interp), lim, mode, hip_strategy(mood.strategy),
+                crit, delta, Int32(s), _mat5(tableau.a_t), _vec5(tableau.b_t), _vec5(tableau.c_t), Float64(substep_ratio),
+                Int32(length(tags)), _pad(Int32.(tags), 16, Int32(0)), _pad([hip_bc(bcm[t]) for t in tags], 16, Int32(0)),
+                plateau, _streamed(interp), hip_flux(interp.flux), _mat5(tableau.a), _vec5(tableau.b), _vec5(tableau.c),
+                mover, malpha)
+end
+
+"""
+    HIPRKTimeStepper(eq, pg, interp::MUSCL, mood::MOOD, tableau; substep_ratio, mood_substeps = true, device = 0,
+                     resident = false)
+
+The drop-in for `GeneralRKTimeStepper(eq, interp, mood, tableau; mood_substeps)` on `pg`: the same arguments, the
+descriptors filled from them (`hip_domain_desc`, `hip_stepper_desc`). `substep_ratio` defaults to the reference's own
+value, `1 / ssp_coefficient(tableau)` for an SSP tableau with `mood_substeps` (TimeSteppers.jl:218), else 0.
+"""
+function HIPRKTimeStepper(eq::HC.HyperbolicPDE, pg::HC.ParticleGrid, interp::HC.MUSCL, mood::HC.MOOD, tableau;
+                          substep_ratio::Union{Nothing, Real} = nothing, mood_substeps::Bool = true,
+                          device::Integer = 0, resident::Bool = false)
+    if substep_ratio === nothing
+        C = HC.ssp_coefficient(tableau)
+        substep_ratio = (mood_substeps && C > 0) ? 1 / C : 0.0
+    end
+    sdesc = hip_stepper_desc(eq, pg, interp, mood, tableau; substep_ratio = substep_ratio)
+    return HIPRKTimeStepper(eq, pg, hip_domain_desc(pg), sdesc; device = device, resident = resident)
+end
+
+"The drop-in built from a constructed `GeneralRKTimeStepper` (its interpolator, MOOD, tableau and substep ratio)."
+HIPRKTimeStepper(ts::HC.GeneralRKTimeStepper, eq::HC.HyperbolicPDE, pg::HC.ParticleGrid; kwargs...) =
+    HIPRKTimeStepper(eq, pg, ts.divergence_interpolator, ts.mood, ts.tableau; substep_ratio = ts.substep_ratio,
+                     kwargs...)
 
 # pg.rhos is a Vector{SVector{M,Float64}}: M contiguous doubles per node, the AoS layout of the ABI
 function upload!(ts::HIPRKTimeStepper, pg)
@@ -132,16 +265,48 @@ function download!(ts::HIPRKTimeStepper, pg)
     GC.@preserve U checkrc(ccall((:hcc_state_download, libhcc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ts.st, pointer(U)), ts.ctx)
 end
 
-"get_time_step(pg, eq, interp) on the device (CFL number 1)."
+"get_time_step(pg, eq, interp) on the device state (CFL number 1)."
 function hip_time_step(ts::HIPRKTimeStepper)
     dt = Ref{Float64}(0.0)
     checkrc(ccall((:hcc_time_step, libhcc), Cint, (Ptr{Cvoid}, Ref{Float64}), ts.st, dt), ts.ctx)
     return dt[]
 end
 
-"The stepper functor: the state stays on the device between steps; call `download!` to read `pg.rhos`."
+"""
+    get_time_step(pg, eq, d::HIPDivergence; cached_weights = true)
+
+The reference's `get_time_step(pg, eq, main_grad)` (src/Grid/ParticleGrids.jl:386) for the HIP stepper: unless the
+stepper is resident, `pg.rhos` is uploaded first (the CFL step is that of the grid's current state), then the CFL
+step is computed on the device.
+"""
+function HyperCloudCore.get_time_step(pg::HyperCloudCore.ParticleGrid, eq::HyperCloudCore.HyperbolicPDE, d::HIPDivergence;
+                                      cached_weights::Bool = true)
+    ts = d.owner::HIPRKTimeStepper
+    if !ts.resident
+        upload!(ts, pg)
+        ts.fresh = true
+    end
+    return hip_time_step(ts)
+end
+
+"""
+    (ts::HIPRKTimeStepper)(eq, pg, t, dt::Float64)
+
+One time step (`GeneralRKTimeStepper`'s functor, TimeSteppers.jl:531-577). Default mode: the state of `pg.rhos` is
+uploaded (unless `get_time_step` just did), stepped on the device and written back to `pg.rhos`; with a grid mover
+the moved and wrapped positions go to `pg.core.positions` too. Resident mode (`resident = true`): the device state
+is stepped in place and `pg` is left alone. Like the reference, a `dt` of another element type is a `MethodError`.
+"""
 function (ts::HIPRKTimeStepper)(eq, pg, time::Real, dt::Float64)
+    if !ts.resident && !ts.fresh
+        upload!(ts, pg)
+    end
+    ts.fresh = false
     checkrc(ccall((:hcc_step, libhcc), Cint, (Ptr{Cvoid}, Float64, Float64), ts.st, time, dt), ts.ctx)
+    if !ts.resident
+        download!(ts, pg)
+        ts.moving && download_positions!(ts, pg)
+    end
     return nothing
 end
 
@@ -190,7 +355,8 @@ end
 """
     hip_solve_equation(ts, tmax, dt; is_cfl = false, snapshots = 10, remove_ghosts = false)
 
-`solve_equation` (test/time_integration.jl) with the loop on the device state: returns
+`solve_equation` (test/time_integration.jl) with the loop on the stepper's device state (after a non-resident step
+that is `pg.rhos`; call `upload!(ts, pg)` after editing `pg.rhos`), snapshots downloaded by the library: returns
 `(position_history, state_history, time_history, total_steps, elapsed_wall_time, mass_history)`, the histories as
 matrices per snapshot (`D × Ns`, `M × Ns`) and `mass_history[c] = (total_mass, interior_mass)`.
 """
@@ -312,7 +478,7 @@ end
 
 export hip_set_source!, hip_mover_code, download_positions!, HIPEnsembleStepper, hip_step_cfl!, hip_time_steps,
        hip_set_indicator!
-export HIPRKTimeStepper, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc, step_totals,
-       hip_volumes, hip_total_mass, hip_solve_equation
+export HIPRKTimeStepper, HIPDivergence, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc,
+       step_totals, hip_volumes, hip_total_mass, hip_solve_equation, hip_domain_desc, hip_stepper_desc
 
 end # module
