@@ -134,6 +134,16 @@ def ensemble_run(args, ctx=None):
     wall_i = timed([st_i]) if st_i is not None else None
     if wall_i is not None:
         print(f"# ensemble batched with the indicator: {steps} steps in {wall_i:.3f} s", file=sys.stderr, flush=True)
+    wall_g = None
+    if getattr(args, "graph_form", False):   # the batched form as hipGraph-captured steps, from the same state
+        st_g = Stepper(ctx, ens, wls[0]["scheme"])
+        st_g.upload(U0)
+        st_g.set_graph(True)
+        wall_g = timed([st_g])
+        assert np.array_equal(st_g.download(), st_e.download()), "captured batched steps differ"
+        g_stats = st_g.graph_stats()
+        st_g.close()
+        print(f"# ensemble batched, captured: {steps} steps in {wall_g:.3f} s", file=sys.stderr, flush=True)
     wall_p = timed(singles)
     print(f"# ensemble plain: {steps} steps in {wall_p:.3f} s", file=sys.stderr, flush=True)
     U_full = st_e.download()
@@ -156,6 +166,10 @@ def ensemble_run(args, ctx=None):
                   "form": "one stepper per member, stepped in turn (hcc_step_cfl)"},
         "batched_vs_plain": value / plain,
     }
+    if wall_g is not None:
+        out["batched_graph"] = {"value": nodes * stages * steps / wall_g, "ms_per_step": 1e3 * wall_g / steps,
+                                "launches": g_stats[0], "captures": g_stats[1], "reruns": g_stats[2],
+                                "form": "the batched form, each step one hipGraph launch (bit-identical states)"}
     if wall_i is not None:
         U_i = st_i.download()
         dev = float(np.abs(U_i - U_full).sum() / np.abs(U_full).sum())
@@ -173,6 +187,42 @@ def ensemble_run(args, ctx=None):
     if own_ctx:
         ctx.close()
     return out
+
+
+def graph_compare(args):
+    """hcc_step_cfl uncaptured against hipGraph-captured steps (hcc_stepper_set_graph) on the four-quadrant problem at
+    each h = 1/n: the same 5 warm-up and `steps` timed CFL steps from t = 0, and a bit-for-bit comparison of the final
+    states. One JSON line per size."""
+    import hcc_hip
+    from hcc_hip import Context, Grid, Stepper
+    from hcc_hip import workloads as W
+    ctx = Context(0)
+    sync = lambda: hcc_hip.lib().hcc_ctx_synchronize(ctx.h)   # noqa: E731
+    for n in args.graph_compare:
+        wl = W.euler_2d(n=n, seed=args.seed)
+        c = wl["cloud"]
+        g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+        res = {}
+        for mode in ("uncaptured", "graph"):
+            st = Stepper(ctx, g, wl["scheme"])
+            st.upload(wl["U0"])
+            st.set_graph(mode == "graph")
+            for _ in range(5):
+                st.step_cfl(wl["cfl"])
+            sync()
+            wall = timed_steps(st, args.steps, wl["cfl"], sync)
+            res[mode] = {"ms_per_step": 1e3 * wall / args.steps, "value": g.N * 3 * args.steps / wall,
+                         "U": st.download()}
+            if mode == "graph":
+                la, ca, re, bo = st.graph_stats()
+                res[mode].update({"launches": la, "captures": ca, "reruns": re, "iterations_captured": bo.tolist()})
+            st.close()
+        same = bool(np.array_equal(res["uncaptured"].pop("U"), res["graph"].pop("U")))
+        print(json.dumps({"nodes_per_axis": n, "nodes": g.N, "steps": args.steps, "bit_identical": same,
+                          "graph_vs_uncaptured": res["uncaptured"]["ms_per_step"] / res["graph"]["ms_per_step"],
+                          **res}), flush=True)
+        g.close()
+    ctx.close()
 
 
 def ensemble_main(args):
@@ -336,6 +386,10 @@ def main():
     ap.add_argument("--n-max", type=int, default=400000)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-warmup", type=int, default=2)
+    ap.add_argument("--graph-form", action="store_true",
+                    help="the ensemble block also times the batched form as hipGraph-captured steps")
+    ap.add_argument("--graph-compare", type=int, nargs="+", default=None, metavar="N",
+                    help="only compare uncaptured and hipGraph-captured steps at h = 1/N for each N")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -344,6 +398,9 @@ def main():
         print(f"bench.py: this build runs the time step on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is designed, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.graph_compare:
+        graph_compare(args)
+        return
     if args.ensemble:
         args.indicator = args.indicator or args.ens_indicator
         ensemble_main(args)

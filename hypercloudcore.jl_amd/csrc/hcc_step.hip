@@ -1234,6 +1234,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_dirty(StepArgs a, int64_t nb
     if (take) a.dlist[base + __popcll(ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = (int32_t)i;
 }
 __global__ void k_zero_u64(unsigned long long* p) { if (threadIdx.x == 0) *p = 0; }
+// A captured fixed point runs a fixed number of iterations; the iteration after them would start from changed list q.
+// If that list holds a node the fixed point has not converged: flag it (the host then re-runs the step uncaptured).
+__global__ void k_fixed_point_overflow(const unsigned long long* __restrict__ counters, int q,
+                                       unsigned long long* __restrict__ flag) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long any = q ? counters[9] : counters[8], n = q ? counters[6] : counters[0];
+    if (any && n) *flag = 1ull;
+}
 
 // The recompute kernels of the fixed point run one node per group of kSplit lanes of a 64-lane workgroup: each lane
 // takes every kSplit-th slot of the node (loads and per-slot arithmetic in parallel) and writes what the accumulation
@@ -2193,12 +2201,24 @@ struct hcc_stepper {
         unsigned long long* hd = nullptr;     // ... and its device address (k_stats_out writes it)
         int64_t host_iters = 0;
         bool dt_from_device = false;
+        bool graph = false;                   // a captured step: no pass events (gather and flux times read 0)
     };
     static constexpr int kRing = 4;
     StepStats ring[kRing];
     int ring_head = 0, ring_count = 0;
-    unsigned long long* h_pub = nullptr;   // two count slots in coherent pinned host memory ...
-    unsigned long long* d_pub = nullptr;   // ... and their device address
+    unsigned long long* h_pub = nullptr;   // count slots 0, 1 and the captured step's overflow flag (slot 2) in
+    unsigned long long* d_pub = nullptr;   // coherent pinned host memory, and their device address
+    // hipGraph-captured steps (hcc_stepper_set_graph): one executable graph per state buffer that holds U^n (the
+    // buffers of U and the candidate swap every step), captured for one (cfl, cap) and one iteration bound per
+    // checkpoint; bounds learned from uncaptured steps (the most non-empty fixed-point iterations seen, plus one)
+    bool graph_on = false;
+    bool graph_pending = false;    // the last queued step was a graph launch whose overflow flag is unread
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const double* gU[2] = {nullptr, nullptr};
+    double gcfl = 0, gcap = 0;
+    int gbound[kMaxStages + 2] = {};   // iterations captured per checkpoint (index: checkpoint 2 .. stages + 1)
+    int gseen[kMaxStages + 2] = {};    // the most non-empty iterations an uncaptured checkpoint needed
+    int64_t g_launches = 0, g_captures = 0, g_reruns = 0;
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
     // hcc_step returns without waiting for the device; the statistics of its queued steps (ring slots) are read when
@@ -2466,6 +2486,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
         if (r.h) (void)hipHostFree(r.h);
     }
     if (s->h_pub) (void)hipHostFree(s->h_pub);
+    for (auto& g : s->gexec) if (g) (void)hipGraphExecDestroy(g);
     delete s;
     return HCC_OK;
 }
@@ -2587,8 +2608,8 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, hipHostMalloc((void**)&r.h, 8 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
         HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&r.hd, r.h, 0));
     }
-    HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 2 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
-    s->h_pub[0] = s->h_pub[1] = 0;
+    HCC_TRY_HIP(ctx, hipHostMalloc((void**)&s->h_pub, 4 * sizeof(unsigned long long), hipHostMallocCoherent | hipHostMallocMapped));
+    s->h_pub[0] = s->h_pub[1] = s->h_pub[2] = s->h_pub[3] = 0;
     HCC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_pub, s->h_pub, 0));
     if (N > 0) {
         ops.pack(s.get(), s->scratch, s->U);      // positions into the state records, zero state
@@ -2633,11 +2654,14 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     return HCC_OK;
 }
 
+static int graph_settle(hcc_stepper* s);
+
 extern "C" int hcc_state_upload(hcc_stepper* s, const double* U) {
     if (!s || (!U && s->N)) return HCC_E_ARG;
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (!s->N) return HCC_OK;
+    if (int rc = graph_settle(s)) return rc;   // a pending captured step completes (or is re-run) on the old state
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, U, sizeof(double) * s->N * s->M, hipMemcpyHostToDevice, ctx->stream));
@@ -2655,6 +2679,7 @@ extern "C" int hcc_state_download(hcc_stepper* s, double* U) {
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (!s->N) return HCC_OK;
+    if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     ops.unpack(s, s->U, s->scratch);
@@ -2667,6 +2692,7 @@ extern "C" int hcc_time_step(hcc_stepper* s, double* dt) {
     if (!s || !dt) return HCC_E_ARG;
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
@@ -2687,6 +2713,7 @@ extern "C" int hcc_time_steps(hcc_stepper* s, double* dts) {
     const int n = s->grid->n_inst;
     if (n == 1) return hcc_time_step(s, dts);
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
     find_ops(s->D, s->M, s->kind, ops);
     k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
@@ -2706,6 +2733,10 @@ struct StepRunner {
     hipEvent_t* evp = nullptr;   // the pass events of this step's statistics slot
     int64_t nb = 0;
     int iters = 0;
+    // capture: the fused fixed point of checkpoint i runs gbound[i] iterations queued without host decisions, then
+    // k_fixed_point_overflow; no pass events. Uncaptured: nonempty[i] counts the iterations that had work.
+    bool graph = false;
+    int nonempty[kMaxStages + 2] = {};
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
     // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK); not with an implicit
     // part, whose stages and substeps involve K_I, nor on a moving grid, whose stage positions and geometry are formed
@@ -2877,6 +2908,14 @@ struct StepRunner {
         hipStream_t str = s->ctx->stream;
         if (fused()) {
             const PredArgs p = pred_args(i, true);
+            if (graph) {   // a fixed number of iterations (an iteration after convergence does nothing), then the check
+                for (int k = 0; k < s->gbound[i]; ++k) {
+                    hipError_t e = mood_iteration(p, Yp, Kp, k & 1);
+                    if (e != hipSuccess) return e;
+                }
+                k_fixed_point_overflow<<<1, 64, 0, str>>>(s->counters, s->gbound[i] & 1, s->d_pub + 2);
+                return hipGetLastError();
+            }
             hipError_t e = mood_iteration(p, Yp, Kp, 0);
             if (e != hipSuccess) return e;
             for (int k = 0;; ++k) {
@@ -2884,6 +2923,7 @@ struct StepRunner {
                 unsigned long long n = 0;
                 if ((e = wait_published(q, n)) != hipSuccess) return e;
                 if (n == 0) break;   // n_k = 0: iteration k + 1 does nothing
+                ++nonempty[i];
                 if ((e = mood_iteration(p, Yp, Kp, q ^ 1)) != hipSuccess) return e;
             }
             return hipGetLastError();
@@ -2939,10 +2979,7 @@ struct StepRunner {
         // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
         // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
         // particles). The full gather also clears the changed count for the check of the flux pass.
-        if (s->ind) {
-            hipError_t e = hipMemsetAsync(s->ind_cnt, 0, sizeof(unsigned long long), str);
-            if (e != hipSuccess) return e;
-        }
+        if (s->ind) k_zero_u64<<<1, 64, 0, str>>>(s->ind_cnt);   // a kernel, not a memset: capturable
         for (int i = 1; i <= st; ++i) {
             if (i == 1) {
                 if (!y1_in_U) predict(1);   // Y_1 = pg.rhos with the boundary conditions applied
@@ -2953,27 +2990,26 @@ struct StepRunner {
                 if (s->moving) apply_stage_geometry(stage_state(i));
             }
             const double* Y = stage_state(i);
-            (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
-            if (s->feat_on) {
-                hipError_t e = hipMemsetAsync(s->ind_n, 0, sizeof(unsigned long long), str);
-                if (e != hipSuccess) return e;
-            }
+            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
+            if (s->feat_on) k_zero_u64<<<1, 64, 0, str>>>(s->ind_n);
             if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else ops.gather(s, a, Y, 1, i == 1);
             if (s->feat_on) {
                 hipError_t e = after_gather(i);
                 if (e != hipSuccess) return e;
             }
-            (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
+            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
             ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
-            (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
+            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1);
         if (e != hipSuccess) return e;
         if (s->collect) snap_orders(2 * st - 1);
-        std::swap(s->U, s->cand);        // pg.rhos <- rho_cand (its ghosts hold their roots' values)
-        s->fresh = false;
+        if (!graph) {   // a captured step swaps when its graph is launched
+            std::swap(s->U, s->cand);        // pg.rhos <- rho_cand (its ghosts hold their roots' values)
+            s->fresh = false;
+        }
         return hipGetLastError();
     }
 };
@@ -2994,7 +3030,7 @@ static int fold_oldest(hcc_stepper* s) {
     HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, r.ev0, r.ev1));
     s->last_ms = ms;
     s->last_gather_ms = s->last_flux_ms = 0;
-    for (int i = 0; i < s->desc.stages; ++i) {
+    for (int i = 0; i < s->desc.stages && !r.graph; ++i) {
         float gm = 0, fm = 0;
         HCC_TRY_HIP(ctx, hipEventElapsedTime(&gm, r.evp[4 * i + 0], r.evp[4 * i + 1]));
         HCC_TRY_HIP(ctx, hipEventElapsedTime(&fm, r.evp[4 * i + 1], r.evp[4 * i + 3]));
@@ -3011,6 +3047,7 @@ static int fold_oldest(hcc_stepper* s) {
 
 // Folds every queued step (the getters report the last one).
 static int finish_pending(hcc_stepper* s) {
+    if (int rc = graph_settle(s)) return rc;
     while (s->ring_count > 0)
         if (int rc = fold_oldest(s)) return rc;
     return HCC_OK;
@@ -3051,10 +3088,132 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     k_stats_out<<<1, 64, 0, ctx->stream>>>(s->counters, s->dtp, slot.hd);
     HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev1, ctx->stream));
+    for (int i = 0; i < kMaxStages + 2; ++i) s->gseen[i] = std::max(s->gseen[i], r.nonempty[i]);
     slot.host_iters = r.iters;
     slot.dt_from_device = dt_from_device;
+    slot.graph = false;
     s->ring_head = (s->ring_head + 1) % hcc_stepper::kRing;
     ++s->ring_count;
+    return HCC_OK;
+}
+
+// ---- hipGraph-captured steps -------------------------------------------------------------------------------------
+static void drop_graphs(hcc_stepper* s) {
+    for (int k = 0; k < 2; ++k) {
+        if (s->gexec[k]) (void)hipGraphExecDestroy(s->gexec[k]);
+        s->gexec[k] = nullptr;
+        s->gU[k] = nullptr;
+    }
+}
+
+// A step can be captured when nothing in it waits for the host: the fused fixed point (or no MOOD), no IMEX stage, no
+// moving grid (its search sizes lists on the host), no training-data copies, the CFL estimate fused into stage 1.
+static bool graph_eligible(const hcc_stepper* s) {
+    const bool mood_ok = s->desc.mood_criterion == HCC_CRIT_NONE || mood_fused(s->desc);
+    return mood_ok && !s->imex && !s->moving && !s->collect && !s->fresh && s->lds_gcfl > 0;
+}
+
+// The last step, if it was a graph launch: waits for it and reads its overflow flag. A fixed point that had not
+// converged within its captured iterations left later stages computed from an unconverged state; the step is then
+// undone and re-run uncaptured (U^n is intact: a step writes only the candidate buffer, which the host swaps in), its
+// statistics slot is dropped, and the bounds grow to what the re-run needed.
+static int graph_settle(hcc_stepper* s) {
+    if (!s->graph_pending) return HCC_OK;
+    hcc_ctx* ctx = s->ctx;
+    s->graph_pending = false;
+    auto& slot = s->ring[(s->ring_head - 1 + hcc_stepper::kRing) % hcc_stepper::kRing];
+    HCC_TRY_HIP(ctx, hipEventSynchronize(slot.ev1));
+    volatile unsigned long long* flag = &s->h_pub[2];
+    if (*flag == 0) return HCC_OK;
+    *flag = 0;
+    std::swap(s->U, s->cand);
+    s->ring_head = (s->ring_head - 1 + hcc_stepper::kRing) % hcc_stepper::kRing;
+    --s->ring_count;
+    drop_graphs(s);
+    ++s->g_reruns;
+    s->cfl_fused = true;   // the same step: stage 1 reads U, the CFL estimate fused into its gather
+    const int rc = launch_step(s, "hcc_step_cfl", true);
+    s->cfl_fused = false;
+    return rc;
+}
+
+// One step of hcc_step_cfl as a graph launch: captured (once per U buffer, cfl, cap and bounds), then launched between
+// the statistics events. The bounds are the iterations uncaptured steps needed plus one.
+static int graph_step(hcc_stepper* s) {
+    hcc_ctx* ctx = s->ctx;
+    const int st = s->desc.stages;
+    bool stale = s->gcfl != s->cfl_num || s->gcap != s->cfl_cap;
+    for (int i = 2; i <= st + 1; ++i) stale = stale || s->gbound[i] < s->gseen[i] + 1;
+    if (stale) {
+        drop_graphs(s);
+        for (int i = 2; i <= st + 1; ++i) s->gbound[i] = s->gseen[i] + 1;
+        s->gcfl = s->cfl_num;
+        s->gcap = s->cfl_cap;
+    }
+    int k = s->gU[0] == s->U ? 0 : s->gU[1] == s->U ? 1 : -1;
+    if (k < 0) {
+        k = !s->gexec[0] ? 0 : !s->gexec[1] ? 1 : -1;
+        if (k < 0) { drop_graphs(s); k = 0; }
+        StepRunner r;
+        r.s = s;
+        find_ops(s->D, s->M, s->kind, r.ops);
+        r.a = s->args();
+        r.graph = true;
+        HCC_TRY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        const hipError_t e = r.run();
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_err(ctx, HCC_E_HIP, std::string("hcc_step_cfl: capture failed: ") +
+                                                hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&s->gexec[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HCC_TRY_HIP(ctx, e3);
+        s->gU[k] = s->U;
+        ++s->g_captures;
+    }
+    if (s->ring_count == hcc_stepper::kRing)
+        if (int rc = fold_oldest(s)) return rc;
+    auto& slot = s->ring[s->ring_head];
+    s->h_pub[2] = 0;   // the last graph launch has been settled: no kernel writes the flag now
+    HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
+    HCC_TRY_HIP(ctx, hipGraphLaunch(s->gexec[k], ctx->stream));
+    k_stats_out<<<1, 64, 0, ctx->stream>>>(s->counters, s->dtp, slot.hd);
+    HCC_TRY_HIP(ctx, hipGetLastError());
+    HCC_TRY_HIP(ctx, hipEventRecord(slot.ev1, ctx->stream));
+    slot.host_iters = 0;
+    slot.dt_from_device = true;
+    slot.graph = true;
+    s->ring_head = (s->ring_head + 1) % hcc_stepper::kRing;
+    ++s->ring_count;
+    std::swap(s->U, s->cand);   // pg.rhos <- rho_cand
+    s->fresh = false;
+    s->graph_pending = true;
+    ++s->g_launches;
+    return HCC_OK;
+}
+
+extern "C" int hcc_stepper_set_graph(hcc_stepper* s, int32_t on) {
+    if (!s) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    if (!on) drop_graphs(s);
+    s->graph_on = on != 0;
+    return HCC_OK;
+}
+
+extern "C" int hcc_stepper_graph_stats(hcc_stepper* s, int64_t* launches, int64_t* captures, int64_t* reruns,
+                                       int32_t* bounds) {
+    if (!s) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
+    if (launches) *launches = s->g_launches;
+    if (captures) *captures = s->g_captures;
+    if (reruns) *reruns = s->g_reruns;
+    if (bounds)
+        for (int i = 0; i < s->desc.stages; ++i) bounds[i] = s->gbound[i + 2];
     return HCC_OK;
 }
 
@@ -3065,6 +3224,7 @@ extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, HCC_E_ARG, "hcc_step: dt must be positive and finite");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (s->N == 0) return HCC_OK;
+    if (int rc = graph_settle(s)) return rc;
     k_set_dt<<<1, 64, 0, ctx->stream>>>(s->dtp, dt, s->grid->n_inst);
     return launch_step(s, "hcc_step", false);
 }
@@ -3076,9 +3236,17 @@ extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
         return set_err(ctx, HCC_E_ARG, "hcc_step_cfl: cfl must be positive and finite, dt_cap positive");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (s->N == 0) return HCC_OK;
+    if (int rc = graph_settle(s)) return rc;
     // stage 1 reads U itself once the boundary conditions hold on it (every step after the first since the state was
     // loaded): the CFL estimate of U then runs inside the stage-1 gather, which stages the same neighbourhoods
     s->cfl_fused = !s->fresh && !s->imex && s->lds_gcfl > 0;
+    if (s->graph_on && graph_eligible(s)) {
+        s->cfl_num = cfl;
+        s->cfl_cap = dt_cap;
+        const int rc = graph_step(s);
+        s->cfl_fused = false;
+        return rc;
+    }
     if (!s->cfl_fused) {
         VariantOps ops;
         find_ops(s->D, s->M, s->kind, ops);
@@ -3102,6 +3270,7 @@ extern "C" int hcc_get_orders(hcc_stepper* s, int32_t* orders) {
     if (!s || (!orders && s->N)) return HCC_E_ARG;
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<int8_t> o((size_t)s->N);
     if (s->N) HCC_TRY_HIP(ctx, hipMemcpy(o.data(), s->orders, s->N, hipMemcpyDeviceToHost));
@@ -3186,6 +3355,7 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     if (!s || (s->N > 0 && (!Y || !K))) return HCC_E_ARG;
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     const int64_t N = s->N;
     const int M = s->M, D = s->D;
     if (!N) return HCC_OK;
@@ -3237,6 +3407,7 @@ extern "C" int hcc_total_mass(hcc_stepper* s, int32_t interior_only, double* out
     if (!s || !out) return HCC_E_ARG;
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     const int M = s->M;
     if (s->N == 0) {
         for (int m = 0; m < M; ++m) out[m] = 0.0;
@@ -3269,6 +3440,7 @@ extern "C" int hcc_solve(hcc_stepper* s, double tmax, double dt, int32_t is_cfl,
     if (snapshots < 1 || !std::isfinite(tmax) || !(dt > 0) || !std::isfinite(dt))
         return set_err(ctx, HCC_E_ARG, "hcc_solve: snapshots >= 1, tmax finite, dt positive and finite");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = graph_settle(s)) return rc;
     const int64_t N = s->N;
     const int D = s->D, M = s->M;
     const std::vector<uint8_t>& isb = s->grid->h_isb;
@@ -3341,6 +3513,7 @@ extern "C" int hcc_set_source(hcc_stepper* s, int32_t mode, const double* rates)
     if (mode != HCC_SRC_NONE && !rates && s->N) return set_err(ctx, HCC_E_ARG, "hcc_set_source: rates required");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = finish_pending(s)) return rc;
+    drop_graphs(s);   // captured steps hold the old configuration
     const int64_t N = s->N, M = s->M;
     hcc_stepper_desc& d = s->desc;
     if (mode == HCC_SRC_DECAY_IMPLICIT) {
@@ -3394,6 +3567,7 @@ extern "C" int hcc_set_indicator(hcc_stepper* s, const hcc_mlp* m, float thresho
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = finish_pending(s)) return rc;
+    drop_graphs(s);   // captured steps hold the old configuration
     if (m) {
         if (mlp_context(m) != ctx)
             return set_err(ctx, HCC_E_ARG, "hcc_set_indicator: the network belongs to another context (another stream)");
@@ -3415,6 +3589,7 @@ extern "C" int hcc_indicator_collect(hcc_stepper* s, int32_t on) {
     hcc_ctx* ctx = s->ctx;
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = finish_pending(s)) return rc;
+    drop_graphs(s);   // captured steps hold the old configuration
     if (on) {
         if (int rc = feature_buffers(s)) return rc;
         const int st = s->desc.stages;

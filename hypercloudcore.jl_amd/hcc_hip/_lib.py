@@ -35,6 +35,7 @@ EXPORTS = [
     "hcc_grid_concat", "hcc_time_steps", "hcc_mlp_create", "hcc_mlp_destroy", "hcc_mlp_set_params",
     "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_gradients_noalloc", "hcc_mlp_train",
     "hcc_set_indicator", "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
+    "hcc_stepper_set_graph", "hcc_stepper_graph_stats",
 ]
 MLP_PARAMS = 5313
 
@@ -113,6 +114,8 @@ def lib():
         "hcc_mlp_gradients": (C.c_int, [vp, vp, vp, i64, C.c_float, vp, P(d)]),
         "hcc_mlp_gradients_noalloc": (C.c_int, [vp, vp, vp, i64, C.c_float, vp]),
         "hcc_mlp_train": (C.c_int, [vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float, C.c_float, vp]),
+        "hcc_stepper_set_graph": (C.c_int, [vp, C.c_int32]),
+        "hcc_stepper_graph_stats": (C.c_int, [vp, P(i64), P(i64), P(i64), vp]),
         "hcc_set_indicator": (C.c_int, [vp, vp, C.c_float]),
         "hcc_indicator_collect": (C.c_int, [vp, C.c_int32]),
         "hcc_get_indicator_samples": (C.c_int, [vp, C.c_int32, vp, vp, vp]),
@@ -460,6 +463,18 @@ class Stepper:
         out = np.zeros(n)
         check(lib().hcc_time_steps(self.h, ptr(out)), self.ctx.h, "hcc_time_steps")
         return out
+
+    def set_graph(self, on=True):
+        """hcc_step_cfl as one hipGraph launch per step (hcc_stepper_set_graph)."""
+        check(lib().hcc_stepper_set_graph(self.h, 1 if on else 0), self.ctx.h, "hcc_stepper_set_graph")
+
+    def graph_stats(self):
+        """(launches, captures, re-runs, iterations captured per checkpoint)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        bounds = np.zeros(5, np.int32)
+        check(lib().hcc_stepper_graph_stats(self.h, C.byref(a), C.byref(b), C.byref(c), ptr(bounds)), self.ctx.h,
+              "hcc_stepper_graph_stats")
+        return a.value, b.value, c.value, bounds[:len(self.scheme["b_t"])].copy()
 
     def set_indicator(self, mlp, threshold=0.0):
         """The learned indicator before each flux pass (mlp None: off)."""

@@ -178,6 +178,22 @@ int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, d
 int hcc_step_totals(hcc_stepper* st, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
                     double* ms_gather, double* ms_flux);
 
+/* No counterpart in the reference (its step is a host loop over kernel launches). With on != 0, hcc_step_cfl runs
+ * each step as one hipGraph launch: the whole step -- the CFL estimate fused into the stage-1 gather, the gather and
+ * flux passes, the stage predictors and each MOOD fixed point as a fixed number of device-gated iterations (an
+ * iteration after convergence does nothing) -- captured once per state buffer (two, swapped every step), per (cfl,
+ * dt_cap) and per iteration bounds, then replayed with no host decision inside the step. The bounds are the
+ * fixed-point iterations the uncaptured steps needed plus one; a captured fixed point that has not converged within its
+ * bound sets a device flag, read before the next step is queued or any result is read: that step is then undone and
+ * re-run uncaptured (same results), and the bounds grow. Steps that wait for the host by design stay uncaptured: the
+ * first step after an upload, hcc_step (host dt), the unfused MOOD strategies and criteria, IMEX stages, moving grids
+ * and training-data collection. Results are those of the uncaptured step, bit for bit. hcc_last_step_timing reports
+ * 0 for the gather and flux times of a captured step (no events inside the graph). */
+int hcc_stepper_set_graph(hcc_stepper* st, int32_t on);
+/* Graph launches, captures and re-runs since creation, and the iterations captured per checkpoint (stages values,
+ * checkpoints 2 .. stages + 1; may be NULL). */
+int hcc_stepper_graph_stats(hcc_stepper* st, int64_t* launches, int64_t* captures, int64_t* reruns, int32_t* bounds);
+
 /* The source terms of GeneralRKTimeStepper(pde, interp, mood, explicit_sources, implicit_sources, tableau)
  * (TimeSteppers.jl:195-247): the built-in decay S_i(U) = -rates[i] U with per-particle rates (N doubles, read once),
  * mode HCC_SRC_*. An implicit source with a purely explicit tableau is applied explicitly with the explicit weights
