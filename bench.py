@@ -102,6 +102,12 @@ def ensemble_run(args, ctx=None):
     st_e = Stepper(ctx, ens, wls[0]["scheme"])
     U0 = ens.pack([wl["U0"] for wl in wls])
     st_e.upload(U0)
+    tuning = None
+    if not getattr(args, "no_autotune", False):   # the batched stepper's launch choices (its state restored)
+        st_e.step_cfl(wls[0]["cfl"])
+        knobs, ms_best, ms_def, timed = st_e.autotune(steps=2, cfl=wls[0]["cfl"])
+        tuning = dict(knobs, ms_per_step_best=round(ms_best, 3), ms_per_step_default=round(ms_def, 3), timed=timed)
+        st_e.upload(U0)
     st_i, net = None, None
     if args.indicator:   # the batched form with the learned indicator (the shipped network)
         from hcc_hip import MLP
@@ -110,6 +116,8 @@ def ensemble_run(args, ctx=None):
         st_i = Stepper(ctx, ens, wls[0]["scheme"])
         st_i.upload(U0)
         st_i.set_indicator(net, IND.DEFAULT_THRESHOLD)
+        if tuning:
+            st_i.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"])
     nodes = int(ens.sizes.sum())
     t_setup = time.perf_counter() - t_setup
     print(f"# ensemble setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
@@ -138,6 +146,8 @@ def ensemble_run(args, ctx=None):
     if getattr(args, "graph_form", False):   # the batched form as hipGraph-captured steps, from the same state
         st_g = Stepper(ctx, ens, wls[0]["scheme"])
         st_g.upload(U0)
+        if tuning:
+            st_g.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"])
         st_g.set_graph(True)
         wall_g = timed([st_g])
         assert np.array_equal(st_g.download(), st_e.download()), "captured batched steps differ"
@@ -160,6 +170,7 @@ def ensemble_run(args, ctx=None):
                     f"warm-up steps, every form from the same initial states",
         "members": len(wls), "nodes": nodes, "ensemble_nodes_with_padding": int(ens.N),
         "smallest": int(ens.sizes.min()), "largest": int(ens.sizes.max()), "setup_s": round(t_setup, 1),
+        "tuning": tuning,
         "batched": {"value": value, "ms_per_step": 1e3 * wall_e / steps,
                     "form": "one ensemble grid, one stepper (hcc_grid_concat + hcc_step_cfl)"},
         "plain": {"value": plain, "ms_per_step": 1e3 * wall_p / steps,
@@ -368,6 +379,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched-ensemble block")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
     ap.add_argument("--host-dt", action="store_true",
                     help="return dt to the host between the CFL estimate and the step (hcc_time_step + hcc_step)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -434,6 +446,10 @@ def main():
     for _ in range(first):
         st.step_cfl(cfl)
     sync()
+    tuning = None
+    if not args.no_autotune:   # launch choices timed on this state (restored afterwards), set-up time
+        knobs, ms_best, ms_def, timed = st.autotune(steps=3, cfl=cfl)
+        tuning = dict(knobs, ms_per_step_best=round(ms_best, 4), ms_per_step_default=round(ms_def, 4), timed=timed)
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f} s: N={g.N} nodes, S={g.S} slots, max_nb={g.max_nb}; {first} set-up steps",
           file=sys.stderr, flush=True)
@@ -517,7 +533,7 @@ def main():
         "config": {"workload": f"2-D Euler four-quadrant, jittered h=1/{args.n} unit square with outflow ghost band, "
                                f"MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.4, {args.run_steps}-step run",
                    "nodes": N, "interior_nodes": N_int, "mean_neighbours": round(nnb, 3), "stages": stages,
-                   "seed": args.seed, "learned_indicator": bool(args.indicator),
+                   "seed": args.seed, "learned_indicator": bool(args.indicator), "tuning": tuning,
                    "timed_steps": f"steps {first + args.warmup + 1}..{first + args.warmup + args.steps} of the "
                                   f"{args.run_steps}-step run from t = 0 (the first {first} formed on the GPU in "
                                   f"set-up, then {args.warmup} warm-up steps)"},

@@ -15,6 +15,10 @@
 #include <memory>
 #include <new>
 #include <thread>
+#include <type_traits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 using namespace hcc;
@@ -1256,11 +1260,11 @@ constexpr int kSplit = D * M <= 8 ? 8 : 16;
 // read on the device): the dirty particles (the changed ones and their neighbours), each listed once. Changes go to
 // changed list 1 - p. FP = false: a recompute of the unfused fixed point over the list k_compact_dirty made (work
 // count 0), K only, as k_flux with full = 0 (a recomputed particle's K_X, its own grid velocity, does not change).
-template <int D, int M, int KIND, int FV, bool FP = true>
+template <int D, int M, int KIND, int FV, bool FP = true, int G = kSplit<D, M>>
 __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                   double* __restrict__ K, int p) {
     using L = Layout<D, M>;
-    constexpr int NR = L::NR, NG = L::NG, BD = D + 1, NP = D * M, G = kSplit<D, M>, GPW = 64 / G;
+    constexpr int NR = L::NR, NG = L::NG, BD = D + 1, NP = D * M, GPW = 64 / G;
     constexpr int SR = NP + 1 + D;   // per slot: dF (axis major), w, d invL; streamed: the contraction (M values)
     static_assert(NP <= G && M <= SR, "one accumulator owner per lane");
     __shared__ double rows[GPW][G][SR];
@@ -1390,11 +1394,11 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
 // gather_node (EXT off) of particle i by its group of kSplit lanes (all lanes of the workgroup call it; act: the group
 // has a particle to gather). The fit's accumulators are summed in slot order by one owner lane each; the extrema and
 // the limiter's phi are minima / maxima (IEEE minimum / maximum: exact, so any order), reduced over the group.
-template <int D, int M, int KIND>
+template <int D, int M, int KIND, int G = kSplit<D, M>>
 __device__ __forceinline__ void gather_split(const StepArgs& a, const double* __restrict__ Y, int64_t i, bool act,
                                              int cnt, int order, int gl, double (*rows)[1 + D + M], double* bsh) {
     using L = Layout<D, M>;
-    constexpr int NR = L::NR, NG = L::NG, NP = D * M, G = kSplit<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, NP = D * M;
     static_assert(NP <= G, "one accumulator owner per lane");
     const int64_t N = a.g.N;
     const double invL = a.g.invL;
@@ -1523,10 +1527,10 @@ __device__ __forceinline__ void gather_split(const StepArgs& a, const double* __
 // The neighbour relation is symmetric, so marking the neighbours of changed particles gives the pull form's set.
 // Block 0 also resets the dirty-list length, counts the fixed-point iterations that changed an order and publishes
 // the count to the host (pub, pinned host memory) with a system-scope store.
-template <int D, int M, int KIND>
+template <int D, int M, int KIND, int G = kSplit<D, M>>
 __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
                                                     int p) {
-    constexpr int G = kSplit<D, M>, GPW = 64 / G;
+    constexpr int GPW = 64 / G;
     constexpr int kListCap = 512;   // newly marked nodes collected before they are appended
     __shared__ double rows[GPW][G][1 + D + M];
     __shared__ double bsh[GPW][D * M];
@@ -1588,7 +1592,7 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             nl += __popcll(ball);
         }
         flush();
-        gather_split<D, M, KIND>(a, Y, i, cnt >= 0 && !a.g.isb[i], cnt, order, gl, rows[grp], bsh[grp]);
+        gather_split<D, M, KIND, G>(a, Y, i, cnt >= 0 && !a.g.isb[i], cnt, order, gl, rows[grp], bsh[grp]);
     }
 }
 
@@ -2190,6 +2194,13 @@ struct hcc_stepper {
     unsigned long long* counters = nullptr;
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
+    // launch choices (hcc_stepper_set_tuning / hcc_stepper_autotune; results are the same bits under every choice):
+    // LDS-staged full passes or neighbour reads through the caches, the lanes per particle of the fixed point's
+    // kernels (kSplit or twice it), the particles one sweep of their grid covers, and the CFL estimate fused into the
+    // stage-1 gather or run as its own pass
+    struct Tuning {
+        int staged = 1, split = 0, sweep = 65536, fuse_cfl = 1;
+    } tune;
     // hcc_step_cfl on a state whose stage 1 is U: the CFL estimate runs inside the stage-1 gather (cfl, cap)
     bool cfl_fused = false;
     double cfl_num = 0, cfl_cap = 0;
@@ -2291,7 +2302,7 @@ struct Variant {
         const int64_t nb = nblocks(s->N);
         if constexpr (kFeat<D, M, KIND>) {
             if (full && s->feat_on) {   // the full pass with the indicator features
-                if (s->lds_gather)
+                if (s->lds_gather && s->tune.staged)
                     k_gather_staged<D, M, KIND, false, true><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
                 else if (s->moving)   // the stage geometry of a moving grid: weight 0 beyond R, as the moments
                     k_gather<D, M, KIND, true, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
@@ -2299,7 +2310,7 @@ struct Variant {
                 return;
             }
         }
-        if (full && s->lds_gather)
+        if (full && s->lds_gather && s->tune.staged)
             k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else if (s->moving) k_gather<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
         else k_gather<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
@@ -2307,9 +2318,17 @@ struct Variant {
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count;
     // up to 65536 / GPW workgroups (32 waves a CU for the first, largest lists; a cap of 16384 / GPW measured about
     // 0.7 % slower on the 1 M-node step, the idle workgroups of a short list costing less than the lost parallelism)
+    // (tuning: the lanes per particle, kSplit or twice it, and the particles one grid sweep covers)
+    template <int G>
     static unsigned split_grid(const hcc_stepper* s) {
-        constexpr int GPW = 64 / kSplit<D, M>;
-        return (unsigned)std::min<int64_t>((s->N + GPW - 1) / GPW, 65536 / GPW);
+        constexpr int GPW = 64 / G;
+        return (unsigned)std::min<int64_t>((s->N + GPW - 1) / GPW, s->tune.sweep / GPW);
+    }
+    template <class F>
+    static void with_split(const hcc_stepper* s, F&& f) {
+        constexpr int G0 = kSplit<D, M>;
+        if (s->tune.split == 2 * G0) f(std::integral_constant<int, 2 * G0>{});
+        else f(std::integral_constant<int, G0>{});
     }
     // the full gather of stage 1 with the CFL estimate of its state fused in, then dt = min(cfl * dt_cfl, cap)
     static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double cfl, double cap) {
@@ -2325,7 +2344,10 @@ struct Variant {
         cfl_final(s, s->dtp, cfl, cap);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
-        k_mark_gather<D, M, KIND><<<split_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
+        with_split(s, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            k_mark_gather<D, M, KIND, G><<<split_grid<G>(s), 64, 0, st(s)>>>(a, Y, pub, p);
+        });
     }
     template <int FV>
     // full: 1 every particle, 0 the dirty ones (a pass over all that skips the clean), -1 the dirty ones of an
@@ -2336,8 +2358,11 @@ struct Variant {
         if (full < 0) {
             k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
             k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
-            k_flux_list<D, M, KIND, FV, false><<<split_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
-        } else if (full && s->lds_flux) {
+            with_split(s, [&](auto g) {
+                constexpr int G = decltype(g)::value;
+                k_flux_list<D, M, KIND, FV, false, G><<<split_grid<G>(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
+            });
+        } else if (full && s->lds_flux && s->tune.staged) {
             if (check) k_flux_staged<D, M, KIND, true, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
             else k_flux_staged<D, M, KIND, false, FV><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
         } else if (check) k_flux<D, M, KIND, true, FV><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
@@ -2349,8 +2374,11 @@ struct Variant {
         else flux_fv<0>(s, a, p, Y, K, full, check);
     }
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
-        if (s->fv) k_flux_list<D, M, KIND, 1><<<split_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
-        else k_flux_list<D, M, KIND, 0><<<split_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+        with_split(s, [&](auto g) {
+            constexpr int G = decltype(g)::value;
+            if (s->fv) k_flux_list<D, M, KIND, 1, true, G><<<split_grid<G>(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+            else k_flux_list<D, M, KIND, 0, true, G><<<split_grid<G>(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+        });
     }
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
                      const double* KIp, int full) {
@@ -2368,7 +2396,7 @@ struct Variant {
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
-        if (s->lds_cfl) {
+        if (s->lds_cfl && s->tune.staged) {
             k_cfl<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_cfl, st(s)>>>(a, s->U, nb);
         } else {
             k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
@@ -3110,7 +3138,7 @@ static void drop_graphs(hcc_stepper* s) {
 // moving grid (its search sizes lists on the host), no training-data copies, the CFL estimate fused into stage 1.
 static bool graph_eligible(const hcc_stepper* s) {
     const bool mood_ok = s->desc.mood_criterion == HCC_CRIT_NONE || mood_fused(s->desc);
-    return mood_ok && !s->imex && !s->moving && !s->collect && !s->fresh && s->lds_gcfl > 0;
+    return mood_ok && !s->imex && !s->moving && !s->collect && !s->fresh && s->cfl_fused;
 }
 
 // The last step, if it was a graph launch: waits for it and reads its overflow flag. A fixed point that had not
@@ -3217,6 +3245,162 @@ extern "C" int hcc_stepper_graph_stats(hcc_stepper* s, int64_t* launches, int64_
     return HCC_OK;
 }
 
+// ---- tuning -------------------------------------------------------------------------------------------------
+// knobs: [0] staged full passes (1) or neighbour reads through the caches (0); [1] lanes per particle of the fixed
+// point's kernels (kSplit or twice it; 0: kSplit); [2] particles one sweep of their grid covers (16384 or 65536);
+// [3] the CFL estimate fused into the stage-1 gather (1) or its own pass (0)
+static int split_base(const hcc_stepper* s) { return s->D * s->M <= 8 ? 8 : 16; }
+
+extern "C" int hcc_stepper_set_tuning(hcc_stepper* s, const int32_t* knobs) {
+    if (!s || !knobs) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    const int g0 = split_base(s);
+    const int split = knobs[1] == 0 ? g0 : knobs[1];
+    if ((knobs[0] != 0 && knobs[0] != 1) || (split != g0 && split != 2 * g0) ||
+        (knobs[2] != 16384 && knobs[2] != 65536) || (knobs[3] != 0 && knobs[3] != 1))
+        return set_err(ctx, HCC_E_ARG, "hcc_stepper_set_tuning: knobs out of range");
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    drop_graphs(s);   // captured steps hold the old launch shapes
+    s->tune.staged = knobs[0];
+    s->tune.split = split;
+    s->tune.sweep = knobs[2];
+    s->tune.fuse_cfl = knobs[3];
+    return HCC_OK;
+}
+
+extern "C" int hcc_stepper_get_tuning(hcc_stepper* s, int32_t* knobs) {
+    if (!s || !knobs) return HCC_E_ARG;
+    knobs[0] = s->tune.staged;
+    knobs[1] = s->tune.split ? s->tune.split : split_base(s);
+    knobs[2] = s->tune.sweep;
+    knobs[3] = s->tune.fuse_cfl;
+    return HCC_OK;
+}
+
+// Choices already tuned in this process, by workload shape: (D, M, PDE, flux variant, LDS tables present, log2 of
+// the node count, mean neighbour count rounded, ensemble or not).
+struct TuneKey {
+    int D, M, kind, fv, lds, log2n, nnb, ens;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(D, M, kind, fv, lds, log2n, nnb, ens) < std::tie(o.D, o.M, o.kind, o.fv, o.lds, o.log2n, o.nnb, o.ens);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, hcc_stepper::Tuning> g_tuned;
+
+static TuneKey tune_key(const hcc_stepper* s) {
+    int l = 0;
+    while ((int64_t(1) << (l + 1)) <= s->N) ++l;
+    const int64_t ni = std::max<int64_t>(1, s->N);
+    return TuneKey{s->D, s->M, s->kind, s->fv, s->lds_gather > 0, l, (int)std::lround((double)s->grid->S / (double)ni),
+                   s->grid->n_inst > 1};
+}
+
+extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap);
+
+// Times `steps` CFL steps (after one untimed step; wall clock) of each launch choice from the current state, restored before
+// every trial and at the end, and keeps the fastest; a choice must beat the default by more than 2 % (the run-to-run
+// spread of the step) to replace it. A shape tuned before in this process takes the stored choice without timing
+// (*timed = 0). The statistics (hcc_step_totals, the last step's diagnostics) are those from before the call.
+extern "C" int hcc_stepper_autotune(hcc_stepper* s, int32_t steps, double cfl, int32_t* knobs, double* ms_best,
+                                    double* ms_default, int32_t* timed) {
+    if (!s || steps < 1 || !(cfl > 0)) return HCC_E_ARG;
+    hcc_ctx* ctx = s->ctx;
+    HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = finish_pending(s)) return rc;
+    const TuneKey key = tune_key(s);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            s->tune = it->second;
+            drop_graphs(s);
+            if (timed) *timed = 0;
+            if (ms_best) *ms_best = 0;
+            if (ms_default) *ms_default = 0;
+            return knobs ? hcc_stepper_get_tuning(s, knobs) : HCC_OK;
+        }
+    }
+    if (timed) *timed = 1;
+    // the state and the statistics to restore
+    const int64_t nrec = s->N * (int64_t)s->PU;
+    double* save = nullptr;
+    HCC_TRY_HIP(ctx, dalloc(&save, nrec));
+    std::unique_ptr<double, hipError_t (*)(void*)> keep(save, hipFree);
+    HCC_TRY_HIP(ctx, hipMemcpyAsync(save, s->U, sizeof(double) * nrec, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool fresh0 = s->fresh, graph0 = s->graph_on;
+    const int64_t ts0 = s->tot_steps, ti0 = s->tot_iters;
+    const double tm0 = s->tot_ms, tg0 = s->tot_gather_ms, tf0 = s->tot_flux_ms;
+    const double lm0 = s->last_ms, lg0 = s->last_gather_ms, lf0 = s->last_flux_ms, ldt0 = s->last_dt;
+    const int64_t in0 = s->inadmissible, mi0 = s->mood_iters;
+    s->graph_on = false;
+    auto restore = [&]() -> int {
+        if (int rc = finish_pending(s)) return rc;
+        HCC_TRY_HIP(ctx, hipMemcpyAsync(s->U, save, sizeof(double) * nrec, hipMemcpyDeviceToDevice, ctx->stream));
+        HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        s->fresh = fresh0;
+        return HCC_OK;
+    };
+    const int g0 = split_base(s);
+    std::vector<hcc_stepper::Tuning> cands;
+    const hcc_stepper::Tuning def{};
+    for (int staged : {1, 0}) {
+        if (staged && !s->lds_gather) continue;
+        for (int split : {g0, 2 * g0})
+            for (int sweep : {65536, 16384})
+                for (int fuse : {1, 0}) {
+                    if (!staged && fuse) continue;   // the fused CFL estimate needs the staged gather
+                    hcc_stepper::Tuning t;
+                    t.staged = staged; t.split = split; t.sweep = sweep; t.fuse_cfl = fuse;
+                    cands.push_back(t);
+                }
+    }
+    double best = INFINITY, tdef = INFINITY;
+    hcc_stepper::Tuning bt = s->tune;
+    for (const auto& t : cands) {
+        s->tune = t;
+        double ms = INFINITY;
+        for (int rep = 0; rep < 2; ++rep) {   // the faster of two runs
+            if (int rc = restore()) return rc;
+            if (int rc = hcc_step_cfl(s, cfl, INFINITY)) return rc;   // untimed: first launches, stage 1 of a fresh state
+            if (int rc = finish_pending(s)) return rc;
+            // wall time of the queued steps: every pass of the step counts, also a CFL pass queued ahead of the step's
+            // first event, and so do the host's turn-arounds inside the fixed points
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int k = 0; k < steps; ++k)
+                if (int rc = hcc_step_cfl(s, cfl, INFINITY)) return rc;
+            if (int rc = finish_pending(s)) return rc;
+            const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            ms = std::min(ms, wall / steps);
+        }
+        const bool is_def = t.staged == (s->lds_gather ? 1 : 0) && (t.split == g0) && t.sweep == def.sweep &&
+                            t.fuse_cfl == (s->lds_gather ? 1 : 0);
+        if (is_def) tdef = ms;
+        if (ms < best) { best = ms; bt = t; }
+    }
+    // the default unless a choice is clearly faster
+    hcc_stepper::Tuning chosen;
+    chosen.staged = s->lds_gather ? 1 : 0;
+    chosen.split = g0;
+    chosen.fuse_cfl = chosen.staged;
+    if (best < 0.98 * tdef) chosen = bt;
+    if (int rc = restore()) return rc;
+    s->tune = chosen;
+    s->graph_on = graph0;
+    drop_graphs(s);
+    s->tot_steps = ts0; s->tot_iters = ti0; s->tot_ms = tm0; s->tot_gather_ms = tg0; s->tot_flux_ms = tf0;
+    s->last_ms = lm0; s->last_gather_ms = lg0; s->last_flux_ms = lf0; s->last_dt = ldt0;
+    s->inadmissible = in0; s->mood_iters = mi0;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[key] = chosen;
+    }
+    if (ms_best) *ms_best = best;
+    if (ms_default) *ms_default = tdef;
+    return knobs ? hcc_stepper_get_tuning(s, knobs) : HCC_OK;
+}
+
 extern "C" int hcc_step(hcc_stepper* s, double t, double dt) {
     (void)t;
     if (!s) return HCC_E_ARG;
@@ -3239,7 +3423,7 @@ extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
     if (int rc = graph_settle(s)) return rc;
     // stage 1 reads U itself once the boundary conditions hold on it (every step after the first since the state was
     // loaded): the CFL estimate of U then runs inside the stage-1 gather, which stages the same neighbourhoods
-    s->cfl_fused = !s->fresh && !s->imex && s->lds_gcfl > 0;
+    s->cfl_fused = !s->fresh && !s->imex && s->lds_gcfl > 0 && s->tune.staged && s->tune.fuse_cfl;
     if (s->graph_on && graph_eligible(s)) {
         s->cfl_num = cfl;
         s->cfl_cap = dt_cap;

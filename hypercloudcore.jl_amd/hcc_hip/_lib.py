@@ -35,7 +35,8 @@ EXPORTS = [
     "hcc_grid_concat", "hcc_time_steps", "hcc_mlp_create", "hcc_mlp_destroy", "hcc_mlp_set_params",
     "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_gradients_noalloc", "hcc_mlp_train",
     "hcc_set_indicator", "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
-    "hcc_stepper_set_graph", "hcc_stepper_graph_stats",
+    "hcc_stepper_set_graph", "hcc_stepper_graph_stats", "hcc_stepper_set_tuning", "hcc_stepper_get_tuning",
+    "hcc_stepper_autotune",
 ]
 MLP_PARAMS = 5313
 
@@ -115,6 +116,9 @@ def lib():
         "hcc_mlp_gradients_noalloc": (C.c_int, [vp, vp, vp, i64, C.c_float, vp]),
         "hcc_mlp_train": (C.c_int, [vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float, C.c_float, vp]),
         "hcc_stepper_set_graph": (C.c_int, [vp, C.c_int32]),
+        "hcc_stepper_set_tuning": (C.c_int, [vp, vp]),
+        "hcc_stepper_get_tuning": (C.c_int, [vp, vp]),
+        "hcc_stepper_autotune": (C.c_int, [vp, C.c_int32, d, vp, P(d), P(d), P(i32)]),
         "hcc_stepper_graph_stats": (C.c_int, [vp, P(i64), P(i64), P(i64), vp]),
         "hcc_set_indicator": (C.c_int, [vp, vp, C.c_float]),
         "hcc_indicator_collect": (C.c_int, [vp, C.c_int32]),
@@ -475,6 +479,25 @@ class Stepper:
         check(lib().hcc_stepper_graph_stats(self.h, C.byref(a), C.byref(b), C.byref(c), ptr(bounds)), self.ctx.h,
               "hcc_stepper_graph_stats")
         return a.value, b.value, c.value, bounds[:len(self.scheme["b_t"])].copy()
+
+    def set_tuning(self, staged=1, split=0, sweep=65536, fuse_cfl=1):
+        """Launch choices (hcc_stepper_set_tuning); every choice gives the same results."""
+        k = np.array([staged, split, sweep, fuse_cfl], np.int32)
+        check(lib().hcc_stepper_set_tuning(self.h, ptr(k)), self.ctx.h, "hcc_stepper_set_tuning")
+
+    def tuning(self):
+        k = np.zeros(4, np.int32)
+        check(lib().hcc_stepper_get_tuning(self.h, ptr(k)), self.ctx.h, "hcc_stepper_get_tuning")
+        return {"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3])}
+
+    def autotune(self, steps=3, cfl=0.4):
+        """hcc_stepper_autotune: (chosen knobs, best ms/step, default ms/step, timed)."""
+        k = np.zeros(4, np.int32)
+        best, dflt, timed = C.c_double(), C.c_double(), C.c_int32()
+        check(lib().hcc_stepper_autotune(self.h, int(steps), float(cfl), ptr(k), C.byref(best), C.byref(dflt),
+                                         C.byref(timed)), self.ctx.h, "hcc_stepper_autotune")
+        return ({"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3])}, best.value,
+                dflt.value, bool(timed.value))
 
     def set_indicator(self, mlp, threshold=0.0):
         """The learned indicator before each flux pass (mlp None: off)."""
