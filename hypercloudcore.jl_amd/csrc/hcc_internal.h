@@ -140,6 +140,10 @@ struct StepArgs {
     int ale;                     // HCC_MOVER_*: flux in the frame of the particle's grid velocity w (generic flux loop)
     double mover_alpha;          // ScaledLagrangianMover: w = mover_alpha x fluid velocity
     double* KX;                  // K_X of the right-hand side being computed: w per interior particle, D per node
+    uint8_t* cnb;                // full gathers: 1 if every neighbour's state equals the particle's (==) and its wave
+                                 // speeds and 2 F are finite (nullptr: not recorded)
+    uint8_t* bflat;              // per kStage block: every halo particle has cnb (or is a boundary particle), so every
+                                 // slot of the block's flux pass is flat (k_block_flat; nullptr: not used)
 };
 
 }  // namespace hcc

@@ -444,6 +444,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
     double phi_f[M];   // the limiter factors (features)
 #pragma unroll
     for (int m = 0; m < M; ++m) phi_f[m] = 1.0;
+    bool cnb_all = false;   // every neighbour's state equals U_i (known only where the neighbour loop ran)
     if (fit || (EXT && a.ext_on) || (CFL && cnt > 0) || FEAT) {
         load_rec<NR, L::PU>(Y, i, ri);
         if constexpr (CFL) {
@@ -457,6 +458,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 for (int c = 0; c < BD; ++c) X[r][c] = a.invd[(r * BD + c) * N + i];
         }
         bool nanf[M];   // a NaN among the extrema's operands (hw_max / hw_min)
+        cnb_all = true;
 #pragma unroll
         for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; nanf[m] = ri[D + m] != ri[D + m]; }
 #pragma unroll
@@ -489,6 +491,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
             bool flat = true;
 #pragma unroll
             for (int m = 0; m < M; ++m) flat = flat && (rj[D + m] == ri[D + m]);
+            cnb_all = cnb_all && flat;
             if (CFL || !flat) {
                 double dd[D];
                 const double d2 = rec_geom<D>(a.g, ri, rj, dd);
@@ -628,6 +631,22 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         }
     }
     store_rec<L::NG, L::GS>(a.G, i, g);
+    if (a.cnb) {   // the flux pass's flat blocks (k_block_flat): a constant neighbourhood and finite speeds and 2 F
+        bool ok = cnb_all;
+        if (ok) {
+            double f[M], F[D][M], lam[D];
+#pragma unroll
+            for (int m = 0; m < M; ++m) f[m] = ri[D + m];
+            Pde<D, M, KIND>::flux_eig(a.pde, f, F, lam);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                ok = ok && isfinite(lam[d]);
+#pragma unroll
+                for (int m = 0; m < M; ++m) ok = ok && isfinite(F[d][m] + F[d][m]);
+            }
+        }
+        a.cnb[i] = ok ? 1 : 0;
+    }
     if constexpr (FEAT && kFeat<D, M, KIND>) {   // ri and the extrema were formed above (FEAT enters that block)
         double u[M];
 #pragma unroll
@@ -714,6 +733,25 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (i < a.g.N && !a.g.isb[i]) gather_node<D, M, KIND, true, CFL, false, FEAT>(a, Y, src, i, &dtc, &cand);
     if (FEAT) append_cand(a, cand, i);
     if (CFL) block_min(a, dtc, lb);
+}
+
+// Whether block lb of the next flux pass is flat: every particle of its halo (its own particles and their
+// neighbours) is a boundary particle or has cnb from the full gather just run. Then for every particle i of the block
+// and every neighbour j: U_j == U_i (i's neighbourhood is constant), g_i = g_j = 0 (a constant neighbourhood fits an
+// exactly zero gradient; boundary particles hold zero gradients), and i's speeds and 2 F_i are finite: every slot is
+// flat (flat_slot), the fit accumulators stay +0, and the pass skips the block's halo and its slot loops.
+__global__ __launch_bounds__(kBlock) void k_block_flat(GeomArgs g, const uint8_t* __restrict__ cnb,
+                                                       uint8_t* __restrict__ bflat, int64_t nblk) {
+    const int64_t lb = blockIdx.x;
+    if (lb >= nblk) return;
+    const int32_t h0 = g.halo_off[lb], nh = g.halo_off[lb + 1] - h0;
+    int ok = 1;
+    for (int t = threadIdx.x; t < nh; t += kBlock) {
+        const int32_t j = g.halo_idx[h0 + t];
+        ok &= (g.isb[j] | cnb[j]) ? 1 : 0;
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) bflat[lb] = (uint8_t)ok;
 }
 
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
@@ -1133,6 +1171,31 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
     return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, nd.fi, Kout, h);
 }
 
+// flux_node for a particle of a flat block (k_block_flat): every slot is flat, so the fit accumulators stay +0 and
+// K_E is the divergence of that zero fit (or -0 + sources without neighbours), exactly as flux_node's loop leaves it;
+// the store, the predictor and the MOOD check follow as there.
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_node_flat(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
+                                               double* __restrict__ K, int64_t i, double h) {
+    using L = Layout<D, M>;
+    double ri[L::NR], fi[M], Kout[M];
+    load_rec<L::NR, L::PU>(Y, i, ri);
+#pragma unroll
+    for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
+    if (a.g.cnt[i] == 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + explicit_source<M>(a, i, fi, m);
+    } else {
+        double b[D + 1][D * M];
+#pragma unroll
+        for (int c = 0; c < D + 1; ++c)
+#pragma unroll
+            for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
+        fit_divergence<D, M>(a, i, b, fi, Kout);
+    }
+    return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, fi, Kout, h);
+}
+
 // The lists of the fused fixed point alternate between two copies, so that no kernel resets a count another
 // workgroup of the same kernel still reads or fills. Iteration k (k = 0, 1, ...) has parity p = k % 2:
 // k_mark_gather reads changed list p and fills work list p (and resets changed and work counts 1 - p), k_flux_list
@@ -1203,7 +1266,11 @@ __global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux_staged(StepArgs 
     const int64_t lb = logical_block(nblk);
     if (lb >= nblk) return;
     const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, lb * kBlock) : 0.0;
-    const auto src = stage_halo<D, M, true>(a, Y, sh, lb);
+    // a flat block (FV = 0) needs neither its halo nor its slot loops (uniform per workgroup: no barrier is skipped
+    // by some threads only)
+    const bool bflat = FV == 0 && a.bflat && a.bflat[lb];
+    LdsSrc<D, M, true> src{};
+    if (!bflat) src = stage_halo<D, M, true>(a, Y, sh, lb);
     const int64_t i = lb * kBlock + threadIdx.x;
     const int64_t N = a.g.N;
     bool ch = false;
@@ -1213,7 +1280,8 @@ __global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux_staged(StepArgs 
 #pragma unroll
             for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else {
-            ch = flux_node<D, M, KIND, CHECK, FV>(a, pred, Y, src, K, i, h);
+            if (bflat) ch = flux_node_flat<D, M, KIND, CHECK>(a, pred, Y, K, i, h);
+            else ch = flux_node<D, M, KIND, CHECK, FV>(a, pred, Y, src, K, i, h);
             if (CHECK) bad = a.inadm[i];
         }
     }
@@ -2192,6 +2260,9 @@ struct hcc_stepper {
     double* range = nullptr;   // ... and its global range per component
     double* dtp = nullptr;     // the step's dt on the device
     unsigned long long* counters = nullptr;
+    uint8_t* cnb = nullptr;    // flat neighbourhoods of the last full gather (N) and the flux pass's flat blocks
+    uint8_t* bflat = nullptr;
+    bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
     // launch choices (hcc_stepper_set_tuning / hcc_stepper_autotune; results are the same bits under every choice):
@@ -2280,6 +2351,7 @@ StepArgs hcc_stepper::args() const {
     // fused check in the flux pass, or k_mood_flag): the checkpoint's state is the one the stage's gather read
     a.ext_on = (desc.mood_criterion == HCC_CRIT_U1 || desc.mood_criterion == HCC_CRIT_U2) ? 1 : 0;
     a.dtp = dtp;
+    if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; }
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.feat = feat;
     a.ind_list = ind_list;
@@ -2307,6 +2379,7 @@ struct Variant {
                 else if (s->moving)   // the stage geometry of a moving grid: weight 0 beyond R, as the moments
                     k_gather<D, M, KIND, true, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
                 else k_gather<D, M, KIND, false, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+                block_flat(s, a, full);
                 return;
             }
         }
@@ -2314,6 +2387,13 @@ struct Variant {
             k_gather_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
         else if (s->moving) k_gather<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
         else k_gather<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+        block_flat(s, a, full);
+    }
+    // after a full gather that recorded cnb: the flat blocks of the flux pass that follows
+    static void block_flat(const hcc_stepper* s, const StepArgs& a, int full) {
+        if (!full || !a.bflat) return;
+        const int64_t nb = nblocks(s->N);
+        k_block_flat<<<(unsigned)nb, kBlock, 0, st(s)>>>(a.g, a.cnb, a.bflat, nb);
     }
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count;
     // up to 65536 / GPW workgroups (32 waves a CU for the first, largest lists; a cap of 16384 / GPW measured about
@@ -2341,6 +2421,7 @@ struct Variant {
             }
         }
         if (!done) k_gather_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_gcfl, st(s)>>>(a, Y, reset, nb);
+        block_flat(s, a, 1);
         cfl_final(s, s->dtp, cfl, cap);
     }
     static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
@@ -2501,7 +2582,8 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->trig, (void*)s->inadm, (void*)s->dmark, (void*)s->clist, (void*)s->clist2, (void*)s->dlist,
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
-                    (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n})
+                    (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n,
+                    (void*)s->cnb, (void*)s->bflat})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
         (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); (void)hipFree(s->KX[k]);
@@ -2628,6 +2710,13 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         s->lds_flux = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, NG), true) + idx_bytes);
         s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
         s->lds_gcfl = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0) + ((D + 1) & ~1), false) + idx_bytes);
+    }
+    if (s->lds_flux && s->fv == 0 && !s->moving && N > 0) {   // flat blocks of the staged flux pass (k_block_flat)
+        HCC_TRY_HIP(ctx, dalloc(&s->cnb, N));
+        HCC_TRY_HIP(ctx, dalloc(&s->bflat, nblocks(N)));
+        HCC_TRY_HIP(ctx, hipMemset(s->cnb, 0, N));
+        HCC_TRY_HIP(ctx, hipMemset(s->bflat, 0, nblocks(N)));
+        s->flat_blocks = true;
     }
     for (auto& r : s->ring) {
         HCC_TRY_HIP(ctx, hipEventCreate(&r.ev0));
