@@ -144,6 +144,9 @@ struct StepArgs {
                                  // speeds and 2 F are finite (nullptr: not recorded)
     uint8_t* bflat;              // per kStage block: every halo particle has cnb (or is a boundary particle), so every
                                  // slot of the block's flux pass is flat (k_block_flat; nullptr: not used)
+    uint8_t* kz;                 // bit s: the particle's K of stage s (0-based) is exactly +-0 in every component (the
+                                 // fused predictor's passes write it, ghosts copy their root's; nullptr: not tracked)
+    const uint8_t* gflat;        // per kStage block: the stage's full gather is flat (k_block_gflat; nullptr: not used)
 };
 
 }  // namespace hcc

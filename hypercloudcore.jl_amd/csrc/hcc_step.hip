@@ -674,6 +674,46 @@ __device__ __forceinline__ void append_cand(const StepArgs& a, bool c, int64_t i
     }
 }
 
+// gather_node for a particle of a flat gather block (k_block_gflat): every neighbour's state equals U_i, so the fit
+// accumulators stay +0 and the gradient is the inverse times zero (g = 0 without a fit), which no slot limits; the
+// extrema are U_i's state (value-equal to what the neighbour loop folds; only the signs of zeros could differ, which the
+// DMP comparisons cannot see). The starved-order rule and the stores are those of gather_node.
+template <int D, int M, bool EXT>
+__device__ __forceinline__ void gather_node_flat(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+    using L = Layout<D, M>;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    int order = a.orders[i];
+    if (order > 1 && cnt < D) { order = 1; a.orders[i] = 1; }   // starved neighbourhood (basis_count(D, 1) = D)
+    double g[L::NG];
+#pragma unroll
+    for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
+    if (order == 2 && a.okg[i]) {
+        const double invL = a.g.invL;
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            double inv[D];
+#pragma unroll
+            for (int r = 0; r < D; ++r) inv[r] = a.invg[(q * D + r) * N + i];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double c = inv[0] * 0.0;
+#pragma unroll
+                for (int r = 1; r < D; ++r) c = c + inv[r] * 0.0;
+                g[q * M + m] = c * invL;
+            }
+        }
+    }
+    store_rec<L::NG, L::GS>(a.G, i, g);
+    if (EXT && a.ext_on) {
+        double ri[L::NR], e[2 * M];
+        load_rec<L::NR, L::PU>(Y, i, ri);
+#pragma unroll
+        for (int m = 0; m < M; ++m) { e[m] = ri[D + m]; e[M + m] = ri[D + m]; }
+        store_rec<2 * M, 2 * M>(a.ext, i, e);
+    }
+}
+
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
 // reset (first stage of a step): orders back to the maximum order and the step's MOOD flags and counters cleared
 // (TimeSteppers.jl:531-540), each thread for its own node before the gather reads that node's order.
@@ -727,6 +767,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (lb >= nblk) return;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
+    if constexpr (!CFL && !FEAT) {
+        if (a.gflat && a.gflat[lb]) {   // a flat gather block (uniform per workgroup): no halo, no slot loops
+            if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M, true>(a, Y, i);
+            return;
+        }
+    }
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
     double dtc = INFINITY;
     bool cand = false;
@@ -754,6 +800,26 @@ __global__ __launch_bounds__(kBlock) void k_block_flat(GeomArgs g, const uint8_t
     if (threadIdx.x == 0) bflat[lb] = (uint8_t)ok;
 }
 
+// Whether the full gather of stage i (1-based, i >= 2) of block lb is flat: every particle h of its halo has K_1 ..
+// K_{i-1} exactly +-0 (mask: bits 0 .. i-2 of kz) and, unless it is a boundary particle, a constant neighbourhood in
+// Y_{i-1} (cnb of the stage i-1 gather). Then Y_i(h) = U^n(h) + sum (+-0) = Y_{i-1}(h) in value for every halo particle,
+// so every particle of the block sees a constant neighbourhood in Y_i: the fit accumulators stay +0, the gradient is
+// the inverse times zero, no slot is limited, and the extrema are U_i's value.
+__global__ __launch_bounds__(kBlock) void k_block_gflat(GeomArgs g, const uint8_t* __restrict__ cnb,
+                                                        const uint8_t* __restrict__ kz, unsigned mask,
+                                                        uint8_t* __restrict__ gflat, int64_t nblk) {
+    const int64_t lb = blockIdx.x;
+    if (lb >= nblk) return;
+    const int32_t h0 = g.halo_off[lb], nh = g.halo_off[lb + 1] - h0;
+    int ok = 1;
+    for (int t = threadIdx.x; t < nh; t += kBlock) {
+        const int32_t j = g.halo_idx[h0 + t];
+        ok &= ((kz[j] & mask) == mask && (g.isb[j] | cnb[j])) ? 1 : 0;
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) gflat[lb] = (uint8_t)ok;
+}
+
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
 // :400-421): Y_i(n) = U^n(n) + sum_j (dt a_ij) K_j(n) needs only node n's data, and its last term is the K this pass
 // has just computed for n. A recompute of K (MOOD) recomputes Y_i(n) with it. Outflow ghosts take their interior
@@ -775,7 +841,8 @@ struct PredArgs {
 };
 
 template <int D, int M>
-__device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, const double (&Kn)[M]) {
+__device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, const double (&Kn)[M],
+                                              uint8_t* kz = nullptr, uint8_t kzv = 0) {
     using L = Layout<D, M>;
     constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
     double r[2 * NP];
@@ -806,6 +873,7 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
     for (int32_t e = p.goff[n], e1 = p.goff[n + 1]; e < e1; ++e) {
         const int64_t g = p.gidx[e];
+        if (kz) kz[g] = kzv;   // a ghost's stage states are its root's
         if constexpr (O == 0) {
             double2* gd = reinterpret_cast<double2*>(p.target + g * L::PU) + P0;
 #pragma unroll
@@ -1052,7 +1120,18 @@ __device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pre
     using P = Pde<D, M, KIND>;
 #pragma unroll
     for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
-    if (pred.on) fused_predict<D, M>(pred, i, Kout);
+    if (pred.on) {
+        uint8_t kzv = 0;
+        if (a.kz) {   // bit s of kz: K_s(i) is +-0 (s: this pass's K, the last term of the predictor)
+            bool z = true;
+#pragma unroll
+            for (int m = 0; m < M; ++m) z = z && Kout[m] == 0.0;
+            const int st = pred.nterm - 1;
+            kzv = st == 0 ? (uint8_t)z : (uint8_t)((a.kz[i] & ~(1u << st)) | ((unsigned)z << st));
+            a.kz[i] = kzv;
+        }
+        fused_predict<D, M>(pred, i, Kout, a.kz, kzv);
+    }
     if (!CHECK) return false;
     double W[M];
 #pragma unroll
@@ -2262,6 +2341,8 @@ struct hcc_stepper {
     unsigned long long* counters = nullptr;
     uint8_t* cnb = nullptr;    // flat neighbourhoods of the last full gather (N) and the flux pass's flat blocks
     uint8_t* bflat = nullptr;
+    uint8_t* kz = nullptr;     // per particle: which stages' K are exactly zero (N, boundary particles 0xFF) ...
+    uint8_t* gflat = nullptr;  // ... and the flat blocks of the gathers of stages 2 .. s
     bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
@@ -2319,6 +2400,16 @@ static bool mood_fused(const hcc_stepper_desc& d) {
     const bool crit = d.mood_criterion == HCC_CRIT_U1 || d.mood_criterion == HCC_CRIT_PAD ||
                       d.mood_criterion == HCC_CRIT_ONLY;
     return d.substep_ratio > 0.0 && strat && crit;
+}
+
+// The step's kernel arguments. The flat gather blocks of stages 2 .. s need the zero-K bits, which the fused
+// predictor's passes keep: tracked when those passes form every stage (fused MOOD or none, no IMEX, no moving grid),
+// with the staged passes and without the indicator's features (the flat gather does not form them).
+static StepArgs step_args(const hcc_stepper* s) {
+    StepArgs a = s->args();
+    const bool fused_pred = !s->imex && !s->moving && (s->desc.mood_criterion == HCC_CRIT_NONE || mood_fused(s->desc));
+    if (s->flat_blocks && s->tune.staged && s->lds_gather && !s->feat_on && fused_pred) a.kz = s->kz;
+    return a;
 }
 
 StepArgs hcc_stepper::args() const {
@@ -2583,7 +2674,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
                     (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n,
-                    (void*)s->cnb, (void*)s->bflat})
+                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
         (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); (void)hipFree(s->KX[k]);
@@ -2716,6 +2807,10 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->bflat, nblocks(N)));
         HCC_TRY_HIP(ctx, hipMemset(s->cnb, 0, N));
         HCC_TRY_HIP(ctx, hipMemset(s->bflat, 0, nblocks(N)));
+        HCC_TRY_HIP(ctx, dalloc(&s->kz, N));
+        HCC_TRY_HIP(ctx, dalloc(&s->gflat, nblocks(N)));
+        HCC_TRY_HIP(ctx, hipMemset(s->kz, 0xFF, N));   // boundary particles without a root never change
+        HCC_TRY_HIP(ctx, hipMemset(s->gflat, 0, nblocks(N)));
         s->flat_blocks = true;
     }
     for (auto& r : s->ring) {
@@ -3110,7 +3205,12 @@ struct StepRunner {
             if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
             if (s->feat_on) k_zero_u64<<<1, 64, 0, str>>>(s->ind_n);
             if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
-            else ops.gather(s, a, Y, 1, i == 1);
+            else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
+                k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
+                StepArgs ag = a;
+                ag.gflat = s->gflat;
+                ops.gather(s, ag, Y, 1, 0);
+            } else ops.gather(s, a, Y, 1, i == 1);
             if (s->feat_on) {
                 hipError_t e = after_gather(i);
                 if (e != hipSuccess) return e;
@@ -3195,7 +3295,7 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     StepRunner r;
     r.s = s;
     find_ops(s->D, s->M, s->kind, r.ops);
-    r.a = s->args();
+    r.a = step_args(s);
     r.evp = slot.evp;
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
     hipError_t e = r.run();
@@ -3274,7 +3374,7 @@ static int graph_step(hcc_stepper* s) {
         StepRunner r;
         r.s = s;
         find_ops(s->D, s->M, s->kind, r.ops);
-        r.a = s->args();
+        r.a = step_args(s);
         r.graph = true;
         HCC_TRY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
         const hipError_t e = r.run();
