@@ -37,12 +37,20 @@ inline hipError_t dalloc(T** p, size_t n) {
 // ============================================================================================================
 constexpr int kBlock = 256;
 
-// The dispatcher hands workgroup b to XCD b % 8. Remap so that XCD x runs one contiguous range of logical
-// blocks: Morton-ordered neighbours then share that XCD's L2.
+// The dispatcher hands workgroup b to XCD b % 8, as the k-th (k = b / 8) of that XCD. Remap so that each XCD runs
+// runs of kXcdChunk consecutive logical blocks (Morton-ordered neighbours then share that XCD's L2), the runs dealt
+// to the XCDs in turn: work that is concentrated in one part of the cloud (a shock, the non-constant region of a
+// Riemann problem) spreads over all eight XCDs instead of falling to the one that owns that eighth of the range.
+// A bijection of [0, 8 per) onto itself: whole chunks first, then each XCD's remaining k contiguously.
+constexpr int64_t kXcdChunk = 16;
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblk) {
+    const int64_t per = (nblk + 7) / 8, x = b & 7, k = b >> 3;
+    const int64_t full = per / kXcdChunk * kXcdChunk;   // blocks per XCD in whole chunks
+    if (k < full) return ((k / kXcdChunk) * 8 + x) * kXcdChunk + k % kXcdChunk;
+    return full * 8 + x * (per - full) + (k - full);
+}
 __device__ __forceinline__ int64_t node_index(int64_t nblk) {
-    const int64_t b = blockIdx.x, per = (nblk + 7) / 8;
-    const int64_t lb = (b & 7) * per + (b >> 3);
-    return lb * kBlock + threadIdx.x;
+    return xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
 }
 inline int64_t nblocks(int64_t N) { return (N + kBlock - 1) / kBlock; }
 inline unsigned grid_for(int64_t nblk) { return (unsigned)(((nblk + 7) / 8) * 8); }

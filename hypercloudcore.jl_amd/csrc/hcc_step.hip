@@ -137,10 +137,7 @@ __device__ __forceinline__ int32_t src_slot(const Src& src, int64_t base, int k,
 }
 
 // The logical workgroup of the XCD-aware order (see node_index).
-__device__ __forceinline__ int64_t logical_block(int64_t nblk) {
-    const int64_t b = blockIdx.x, per = (nblk + 7) / 8;
-    return (b & 7) * per + (b >> 3);
-}
+__device__ __forceinline__ int64_t logical_block(int64_t nblk) { return xcd_block(blockIdx.x, nblk); }
 
 static_assert(kStage == kBlock, "staged passes launch one workgroup per halo block");
 
