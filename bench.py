@@ -380,6 +380,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="leave out the batched-ensemble block")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="only the set-up, warm-up and timed steps (no full run, dense state, ensemble or CPU "
+                         "baseline): the short run the rocprofv3 summaries under profiles/ profile")
     ap.add_argument("--host-dt", action="store_true",
                     help="return dt to the host between the CFL estimate and the step (hcc_time_step + hcc_step)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -495,10 +498,13 @@ def main():
     achieved = fb / (flux_avg_ms * 1e-3) / 1e9
     model_b = W.bytes_per_node_stage(D, M, nnb)
 
+    if args.timed_only:
+        args.no_ensemble = args.no_cpu_baseline = True
+        args.run_steps = 0
     # the whole configured run from t = 0 (run_steps CFL steps), timed as one region
     st.upload(wl["U0"])
     sync()
-    wall_run = timed_steps(st, args.run_steps, cfl, sync)
+    wall_run = timed_steps(st, args.run_steps, cfl, sync) if args.run_steps else float("nan")
     # the dense state: the four quadrants times a smooth factor, so that no neighbourhood is constant and every
     # slot of every pass does its full arithmetic (the same cloud and scheme)
     x = c["pos"]
@@ -507,13 +513,14 @@ def main():
     Ud[:, 0] *= fac
     Ud[:, 3] *= fac
     st.upload(Ud)
-    for _ in range(2):
+    dense_k = 0 if args.timed_only else 10
+    for _ in range(2 if dense_k else 0):
         st.step_cfl(cfl)
     sync()
     st.totals(reset=True)
-    dense_k = 10
-    wall_dense = timed_steps(st, dense_k, cfl, sync)
+    wall_dense = timed_steps(st, dense_k, cfl, sync) if dense_k else float("nan")
     nd_, it_d, tot_d, g_d, f_d = st.totals(reset=True)
+    nd_ = max(nd_, 1)
 
     traffic = profile_value("hbm_traffic_euler2d_1M.md", "traffic_bytes k_flux_staged")
     valu = profile_value("sq_counters_euler2d_1M.md", "valu_issue_share k_flux_staged")
@@ -549,17 +556,19 @@ def main():
         "detail": {"gather_avg_ms": gather_avg_ms, "gather_bytes": gb, "flux_avg_ms": flux_avg_ms,
                    "step_event_ms": tot_ms / n_steps, "mood_iterations": iters, "flags_last_step": flags,
                    "model_bytes_per_node_stage": model_b,
-                   "model_roofline_frac": value * model_b / (HBM_PEAK_GBS * 1e9),
-                   "full_run": {"steps": args.run_steps, "ms_per_step": 1e3 * wall_run / args.run_steps,
-                                "value": N * stages * args.run_steps / wall_run,
-                                "form": "every step of the run from t = 0, one timed region"},
-                   "dense_state": {"steps": dense_k, "ms_per_step": 1e3 * wall_dense / dense_k,
-                                   "value": N * stages * dense_k / wall_dense,
-                                   "gather_avg_ms": g_d / (stages * nd_), "flux_avg_ms": f_d / (stages * nd_),
-                                   "mood_iterations": it_d,
-                                   "form": "the four quadrants times 1 + 0.01 sin(6 pi x) sin(10 pi y): no constant "
-                                           "neighbourhood, every slot computed in full"}},
+                   "model_roofline_frac": value * model_b / (HBM_PEAK_GBS * 1e9)},
     }
+    if not args.timed_only:
+        out["detail"].update({
+            "full_run": {"steps": args.run_steps, "ms_per_step": 1e3 * wall_run / args.run_steps,
+                         "value": N * stages * args.run_steps / wall_run,
+                         "form": "every step of the run from t = 0, one timed region"},
+            "dense_state": {"steps": dense_k, "ms_per_step": 1e3 * wall_dense / dense_k,
+                            "value": N * stages * dense_k / wall_dense,
+                            "gather_avg_ms": g_d / (stages * nd_), "flux_avg_ms": f_d / (stages * nd_),
+                            "mood_iterations": it_d,
+                            "form": "the four quadrants times 1 + 0.01 sin(6 pi x) sin(10 pi y): no constant "
+                                    "neighbourhood, every slot computed in full"}})
     st.close()
     if net is not None:
         net.close()
