@@ -25,6 +25,8 @@ TABLEAUS = {
     "RK2_SSP": ([[0.0, 0.0], [1.0, 0.0]], [1 / 2, 1 / 2], [0.0, 1.0]),
     "RK2_Ralston": ([[0.0, 0.0], [2 / 3, 0.0]], [1 / 4, 3 / 4], [0.0, 2 / 3]),
     "RK3_SSP": ([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.25, 0.25, 0.0]], [1 / 6, 1 / 6, 2 / 3], [0.0, 1.0, 0.5]),
+    "RK4_Classical": ([[0.0, 0.0, 0.0, 0.0], [0.5, 0.0, 0.0, 0.0], [0.0, 0.5, 0.0, 0.0], [0.0, 0.0, 1.0, 0.0]],
+                      [1 / 6, 1 / 3, 1 / 3, 1 / 6], [0.0, 0.5, 0.5, 1.0]),
 }
 
 
@@ -100,9 +102,9 @@ def tableau(name):
 
 def scheme(M, pde, *, gamma=1.4, adv_speed=None, limiter=L.HCC_LIM_VK, mode=L.HCC_MODE_HARD, strategy=L.HCC_STRAT_EPD1,
            criterion=L.HCC_CRIT_U1, delta=0.0, plateau=0.0, tableau_name="RK3_SSP", bcs=(), mood_substeps=True,
-           streamed=False, flux=L.HCC_FLUX_RUSANOV, mover=L.HCC_MOVER_NONE, mover_alpha=1.0):
+           streamed=False, flux=L.HCC_FLUX_RUSANOV, mover=L.HCC_MOVER_NONE, mover_alpha=1.0, order=2):
     tb = tableau(tableau_name)
-    return {"M": M, "pde": pde, "gamma": gamma, "adv_speed": adv_speed or [], "muscl_order": 2, "limiter": limiter,
+    return {"M": M, "pde": pde, "gamma": gamma, "adv_speed": adv_speed or [], "muscl_order": order, "limiter": limiter,
             "limiter_mode": mode, "strategy": strategy, "criterion": criterion, "delta": delta, "plateau": plateau,
             "streamed": streamed, "flux": flux, "a_t": tb["a_t"], "a_im": tb.get("a_im"), "b_im": tb.get("b_im"),
             "c_im": tb.get("c_im"), "mover": mover, "mover_alpha": mover_alpha,

@@ -23,7 +23,7 @@ def lib():
         P = C.POINTER
         sig = {
             "orc_cloud_generate": (i64, [C.c_int, vp, vp, vp, vp, d, vp, C.c_uint64, i64, vp, vp, vp, P(i64), vp, P(i64)]),
-            "orc_sim_new": (vp, [C.c_int, C.c_int, i64, vp, vp, vp, vp, vp, vp, vp, d, d]),
+            "orc_sim_new": (vp, [C.c_int, C.c_int, i64, vp, vp, vp, vp, vp, vp, vp, d, d, C.c_int]),
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
@@ -102,23 +102,40 @@ def cloud_generate(dom):
     return {"pos": pos, "is_boundary": isb, "tags": tags, "canvas": canvas, "morton_ties": ties.value, "N": N}
 
 
+def basis_count(D, q):
+    """Terms of degree 1..q in D variables, binomial(D + q, q) - 1 (Interpolators.jl:170-177)."""
+    if q <= 0:
+        return 0
+    n = 1
+    for k in range(1, q + 1):
+        n = (n * (D + k)) // k
+    return n - 1
+
+
 class OracleSim:
-    """Grid + MUSCL(2) + GeneralRKTimeStepper restated on the CPU."""
+    """Grid + MUSCL (order 2 or 3) + GeneralRKTimeStepper restated on the CPU."""
 
     def __init__(self, dom, pos, isb, tags, scheme=None, threads=1):
         D = dom["D"]
         self.D = D
         self.N = pos.shape[0]
         self.M = scheme["M"] if scheme else 1
+        self.order = int(scheme.get("muscl_order", 2)) if scheme else 2
+        if self.order == 3 and scheme and (scheme.get("streamed") or scheme.get("flux", FLUX_RUSANOV) != FLUX_RUSANOV
+                                           or scheme.get("mover", 0)):
+            raise ValueError("the MUSCL3 restatement covers the buffered RusanovFlux on static grids")
+        # basis sizes: reconstruction of degree order - 1, divergence (with intercept) of degree 1 or 3
+        self.BG = basis_count(D, self.order - 1)
+        self.BD = basis_count(D, 1 if self.order <= 2 else 3) + 1
         self._keep = [np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(isb, np.uint8),
                       np.ascontiguousarray(tags, np.int32)]
         mins, maxs = _arr(dom["mins"], np.float64), _arr(dom["maxs"], np.float64)
         per = _arr([int(bool(x)) for x in dom["periodic"]], np.int32)
         dx = _arr(dom["dx"], np.float64)
         self.h = lib().orc_sim_new(D, self.M, self.N, _p(self._keep[0]), _p(self._keep[1]), _p(self._keep[2]), _p(mins),
-                                   _p(maxs), _p(per), _p(dx), dom["R"], dom["alpha"])
+                                   _p(maxs), _p(per), _p(dx), dom["R"], dom["alpha"], self.order)
         if not self.h:
-            raise ValueError(f"oracle has no (D={D}, M={self.M}) variant")
+            raise ValueError(f"oracle has no (D={D}, M={self.M}, order {self.order}) variant")
         lib().orc_sim_set_threads(self.h, int(threads))
         if scheme:
             self.set_scheme(scheme)
@@ -168,7 +185,7 @@ class OracleSim:
     def rhs(self, Y):
         Y = np.ascontiguousarray(Y, np.float64).reshape(self.N, self.M)
         K = np.zeros((self.N, self.M))
-        G = np.zeros((self.N, self.D, self.M))
+        G = np.zeros((self.N, self.BG, self.M))
         lib().orc_sim_rhs(self.h, _p(Y), _p(K), _p(G))
         return K, G
 
@@ -192,9 +209,9 @@ class OracleSim:
     def moments(self):
         D, N = self.D, self.N
         S = lib().orc_sim_num_slots(self.h)
-        ig = np.zeros((N, D, D))
+        ig = np.zeros((N, self.BG, self.BG))
         og = np.zeros(N, np.uint8)
-        idv = np.zeros((N, D + 1, D + 1))
+        idv = np.zeros((N, self.BD, self.BD))
         od = np.zeros(N, np.uint8)
         dw = np.zeros((max(S, 1), D))
         lib().orc_sim_get_moments(self.h, _p(ig), _p(og), _p(idv), _p(od), _p(dw))
