@@ -1,0 +1,739 @@
+// MUSCL order 3 on the device (MUSCL(T, D, M, 3, limiter, RusanovFlux()), src/Interpolation/MUSCL.jl): the degree-2
+// reconstruction with the graded Taylor basis, the divergence fit of degree 3 with intercept where the stencil carries
+// it (the neighbour-count safeguard of _divergence_index lowers it to 2 or 1, zero-padded in the cache), the direct
+// Cholesky solves of the particles below the full order, :soft / :hard limiting of the degree-2 terms, and three MOOD
+// levels. Included by hcc_step.hip after the order-2 passes, whose record access, predictor, MOOD check and list
+// bookkeeping it shares.
+//
+// Layout: gradient records hold the BG = basis_count(D, 2) coefficients per component, term major ([b * M + m],
+// padded to an even stride). The reconstruction inverse is BG x BG per node, the divergence inverse holds its D
+// derivative rows (rows 2 .. D + 1 of the padded BD x BD inverse, BD = basis_count(D, 3) + 1), both SoA
+// ([q * N + i]). Neighbours are read through the caches (no LDS staging).
+#pragma once
+
+namespace m3 {
+
+// basis_count(D, q) = binomial(D + q, q) - 1 (Interpolators.jl:170-177)
+__host__ __device__ constexpr int bcount(int D, int q) {
+    int n = 1;
+    for (int k = 1; k <= q; ++k) n = (n * (D + k)) / k;
+    return q <= 0 ? 0 : n - 1;
+}
+
+// Exponent of axis `ax` in term b of the graded basis of degree <= 3 (_generate_exponents, Interpolators.jl:138-155):
+// each degree's tuples sorted by (-max(t), reverse(t)); 2-D: x, y | x^2, y^2, xy | x^3, y^3, x^2 y, x y^2.
+__host__ __device__ constexpr int texp(int D, int b, int ax) {
+    if (D == 1) return ax == 0 ? b + 1 : 0;
+    if (D == 2) {
+        const int t[9][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}, {1, 1}, {3, 0}, {0, 3}, {2, 1}, {1, 2}};
+        return ax < 2 ? t[b][ax] : 0;
+    }
+    const int t[19][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0},
+                          {1, 0, 1}, {0, 1, 1}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0},
+                          {2, 0, 1}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {1, 1, 1}};
+    return t[b][ax];
+}
+__host__ __device__ constexpr int tfact(int e) { return e <= 1 ? 1 : e == 2 ? 2 : 6; }
+// prod(t!) of term b, and its degree
+__host__ __device__ constexpr int tden(int D, int b) {
+    int d = 1;
+    for (int ax = 0; ax < D; ++ax) d *= tfact(texp(D, b, ax));
+    return d;
+}
+__host__ __device__ constexpr int tdeg(int D, int b) {
+    int s = 0;
+    for (int ax = 0; ax < D; ++ax) s += texp(D, b, ax);
+    return s;
+}
+
+// build_basis (Interpolators.jl:194-218) of the first NB terms: 1/prod(t!) (when not 1) times the powers axis by axis,
+// d^2 = d*d and d^3 = (d*d)*d, multiplied left to right
+template <int D, int NB>
+__device__ __forceinline__ void taylor(const double (&d)[D], double (&p)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int den = tden(D, b);
+        bool have = den != 1;
+        double t = have ? 1.0 / (double)den : 0.0;
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+            const int e = texp(D, b, ax);
+            if (e == 0) continue;
+            const double pw = e == 1 ? d[ax] : e == 2 ? d[ax] * d[ax] : (d[ax] * d[ax]) * d[ax];
+            t = have ? t * pw : pw;
+            have = true;
+        }
+        p[b] = t;
+    }
+}
+// the scale of term b's coefficient: invL^degree (build_scale_factors, Interpolators.jl:236-247)
+__device__ __forceinline__ double tscale(int deg, double invL) {
+    return deg == 1 ? invL : deg == 2 ? invL * invL : (invL * invL) * invL;
+}
+
+template <int D, int M>
+struct L3 {
+    static constexpr int NR = D + M, PU = Layout<D, M>::PU;
+    static constexpr int BG = bcount(D, 2);          // reconstruction terms
+    static constexpr int NG = BG * M;
+    static constexpr int GS = (NG + 1) & ~1;         // gradient record stride (16-byte loads)
+    static constexpr int BD = bcount(D, 3) + 1;      // divergence fit with intercept at full degree
+    static constexpr int B1 = D + 1;                 // the degree-1 fit with intercept
+};
+
+// _divergence_index(Val(0), Val(D), p_order, num_nb) (MUSCL.jl:11-32): the table entry (degree + 1)
+__device__ __forceinline__ int div_index(int D, int p, int nb) {
+    int idx = p <= 2 ? 2 : (((p & 1) && (D < 3 || p <= 3)) ? p + 1 : p);
+    const int base = p > 2 ? p : 2;
+    if (idx > base && 2 * nb < 3 * (bcount(D, idx - 1) + 1)) idx = base;
+    while (idx > 2 && nb < bcount(D, idx - 1) + 1) --idx;
+    return idx;
+}
+
+// C \ R for NR columns: U \ (U' \ R), forward then back substitution (StaticArrays' triangular solves)
+template <int B, int NRH>
+__device__ __forceinline__ void chol_solve(const double (&U)[B][B], const double (&R)[B][NRH], double (&X)[B][NRH]) {
+#pragma unroll
+    for (int j = 0; j < NRH; ++j) {
+        double y[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            double s = R[i][j];
+#pragma unroll
+            for (int k = 0; k < i; ++k) s = s - U[k][i] * y[k];
+            y[i] = s / U[i][i];
+        }
+#pragma unroll
+        for (int i = B - 1; i >= 0; --i) {
+            double s = y[i];
+#pragma unroll
+            for (int k = i + 1; k < B; ++k) s = s - U[i][k] * X[k][j];
+            X[i][j] = s / U[i][i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// geometry caches (prepare_geometry!, MUSCL.jl:172-223): the degree-2 reconstruction inverse, the divergence inverse
+// of the degree each stencil carries at full order (_FullOrderDivDegree) padded to BD x BD, its derivative rows, and
+// (dw set) the per-slot derivative weights rows(inv) (w p~) invL with the full-degree intercept basis
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int B>
+__device__ __forceinline__ bool div_inverse(const MomentArgs& a, const NodePos<D>& pi, int cnt, int64_t base, int q,
+                                            double (&X)[bcount(D, 3) + 1][bcount(D, 3) + 1]) {
+    constexpr int BD = bcount(D, 3) + 1;
+    const double invL = a.g.invL;
+    double A[B][B];
+    for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) A[r][c] = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+        double dd[D], ds[D];
+        const double d2 = slot_geom<D>(a.g, pi, j, dd);
+        const double w = slot_weight(a.g, d2);
+        for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+        double p[B];
+        p[0] = 1.0;
+        double t[B - 1];
+        taylor<D, B - 1>(ds, t);
+        for (int c = 1; c < B; ++c) p[c] = t[c - 1];
+        for (int r = 0; r < B; ++r) for (int c = 0; c < B; ++c) A[r][c] = A[r][c] + w * (p[r] * p[c]);
+    }
+    (void)q;
+    double U[B][B], Xi[B][B];
+    if (!chol_moment<B>(A, U)) return false;
+    chol_inverse<B>(U, Xi);
+    for (int r = 0; r < BD; ++r) for (int c = 0; c < BD; ++c) X[r][c] = (r < B && c < B) ? Xi[r][c] : 0.0;
+    return true;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_moments3(MomentArgs a, int64_t nblk) {
+    constexpr int BG = bcount(D, 2), BD = bcount(D, 3) + 1;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    if (i >= N) return;
+    const NodePos<D> pi = load_pos<D>(a.g, i);
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    const double invL = a.g.invL;
+    {
+        double AG[BG][BG];
+        for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) AG[r][c] = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double dd[D], ds[D], p[BG];
+            const double d2 = slot_geom<D>(a.g, pi, j, dd);
+            const double w = slot_weight(a.g, d2);
+            for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+            taylor<D, BG>(ds, p);
+            for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) AG[r][c] = AG[r][c] + w * (p[r] * p[c]);
+        }
+        double U[BG][BG], X[BG][BG];
+        const bool ok = chol_moment<BG>(AG, U);
+        if (ok) chol_inverse<BG>(U, X);
+        for (int r = 0; r < BG; ++r) for (int c = 0; c < BG; ++c) a.invg[(r * BG + c) * N + i] = ok ? X[r][c] : 0.0;
+        a.okg[i] = ok;
+    }
+    const int q = div_index(D, 3, cnt) - 1;
+    double X[BD][BD];
+    bool ok;
+    if (q == 3) ok = div_inverse<D, BD>(a, pi, cnt, base, q, X);
+    else if (q == 2) ok = div_inverse<D, bcount(D, 2) + 1>(a, pi, cnt, base, q, X);
+    else ok = div_inverse<D, D + 1>(a, pi, cnt, base, q, X);
+    for (int r = 0; r < D; ++r) for (int c = 0; c < BD; ++c) a.invd[(r * BD + c) * N + i] = ok ? X[1 + r][c] : 0.0;
+    a.okd[i] = ok;
+    if (!a.dw) return;
+    for (int k = 0; k < cnt; ++k) {
+        const int64_t s = base + (int64_t)k * kChunk;
+        if (!ok) { for (int d = 0; d < D; ++d) a.dw[d * a.S_sell + s] = 0.0; continue; }
+        const int32_t j = a.g.nbr[s];
+        double dd[D], ds[D], t[BD - 1], v[BD];
+        const double d2 = slot_geom<D>(a.g, pi, j, dd);
+        const double w = slot_weight(a.g, d2);
+        for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+        taylor<D, BD - 1>(ds, t);
+        v[0] = w * 1.0;
+        for (int c = 1; c < BD; ++c) v[c] = w * t[c - 1];
+        for (int r = 0; r < D; ++r) {
+            double acc = X[1 + r][0] * v[0];
+            for (int c = 1; c < BD; ++c) acc = acc + X[1 + r][c] * v[c];
+            a.dw[r * a.S_sell + s] = acc * invL;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// gather: update_content! (MUSCL.jl:351-384) with the starved-order rule, the cached degree-2 fit at order 3
+// (apply_weights, Interpolators.jl:585-602), the direct degree-1 solve at order 2 (_mls_solve, :372-400), and
+// _limit_slopes (Limiter.jl:120-143) with the degree-2 interface basis; EXT stores the neighbourhood extrema for
+// MOODu1's checks
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M, int KIND, bool EXT>
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+    using L = L3<D, M>;
+    constexpr int NR = L::NR, BG = L::BG;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    int order = a.orders[i];
+    const int o0 = order;
+    while (order > 1 && cnt < bcount(D, order - 1)) --order;   // starved neighbourhood
+    if (order < o0) a.orders[i] = (int8_t)order;
+    double g[L::NG];
+#pragma unroll
+    for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
+    const bool cached = order == 3 && a.okg[i];
+    const bool direct = order == 2;
+    const bool need = cached || direct || (EXT && a.ext_on) || (order > 1 && a.limiter != HCC_LIM_NONE);
+    if (!need) { store_rec<L::NG, L::GS>(a.G, i, g); return; }
+    const int64_t base = sell_base(a.g, i);
+    const double invL = a.g.invL;
+    double ri[NR], umax[M], umin[M];
+    load_rec<NR, L::PU>(Y, i, ri);
+    bool nanf[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) { umax[m] = ri[D + m]; umin[m] = ri[D + m]; nanf[m] = ri[D + m] != ri[D + m]; }
+    double b[BG][M], A[D][D];
+#pragma unroll
+    for (int q = 0; q < BG; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) b[q][m] = 0.0;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) A[r][c] = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        double rj[NR];
+        load_rec<NR, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], rj);
+        if (cached || direct) {
+            double dd[D], ds[D];
+            const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+            if (cached) {
+                double p[BG];
+                taylor<D, BG>(ds, p);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const double df = rj[D + m] - ri[D + m];
+#pragma unroll
+                    for (int q = 0; q < BG; ++q) b[q][m] = b[q][m] + w * (p[q] * df);
+                }
+            } else {   // the degree-1 basis is the distance itself
+#pragma unroll
+                for (int r = 0; r < D; ++r)
+#pragma unroll
+                    for (int c = 0; c < D; ++c) A[r][c] = A[r][c] + w * (ds[r] * ds[c]);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const double df = rj[D + m] - ri[D + m];
+#pragma unroll
+                    for (int q = 0; q < D; ++q) b[q][m] = b[q][m] + w * (ds[q] * df);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const double fj = rj[D + m];
+            umax[m] = hw_max(umax[m], fj);
+            umin[m] = hw_min(umin[m], fj);
+            nanf[m] = nanf[m] | (fj != fj);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+        if (nanf[m]) { umax[m] = __builtin_nan(""); umin[m] = __builtin_nan(""); }
+    if (EXT && a.ext_on) {
+        double e[2 * M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) { e[m] = umax[m]; e[M + m] = umin[m]; }
+        store_rec<2 * M, 2 * M>(a.ext, i, e);
+    }
+    if (cached) {
+#pragma unroll
+        for (int q = 0; q < BG; ++q) {
+            double inv[BG];
+#pragma unroll
+            for (int r = 0; r < BG; ++r) inv[r] = a.invg[(q * BG + r) * N + i];
+            const double sc = tscale(tdeg(D, q), invL);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double c = inv[0] * b[0][m];
+#pragma unroll
+                for (int r = 1; r < BG; ++r) c = c + inv[r] * b[r][m];
+                g[q * M + m] = c * sc;
+            }
+        }
+    } else if (direct) {
+        double U[D][D], rhs[D][M], X[D][M];
+        if (chol_moment<D>(A, U)) {
+#pragma unroll
+            for (int q = 0; q < D; ++q)
+#pragma unroll
+                for (int m = 0; m < M; ++m) rhs[q][m] = b[q][m];
+            chol_solve<D, M>(U, rhs, X);
+#pragma unroll
+            for (int q = 0; q < D; ++q)
+#pragma unroll
+                for (int m = 0; m < M; ++m) g[q * M + m] = X[q][m] * invL;
+        }
+    }
+    // order 1: zero; order 3 without a factorisation: the direct solve fails the same way, zero
+    bool gzero = true;
+#pragma unroll
+    for (int q = 0; q < L::NG; ++q) gzero = gzero && (g[q] == 0.0);
+    if (order > 1 && a.limiter != HCC_LIM_NONE && !gzero) {
+        double phi[M], dup[M], ddn[M], tup[M], tdn[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            phi[m] = 1.0;
+            const double ui = ri[D + m];
+            dup[m] = umax[m] - ui;
+            ddn[m] = umin[m] - ui;
+            tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
+            tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
+        }
+        for (int k = 0; k < cnt; ++k) {
+            double pj[D], dd[D], h[D], pint[BG];
+            load_rec<D, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], pj);
+            rec_geom<D>(a.g, ri, pj, dd);
+#pragma unroll
+            for (int d = 0; d < D; ++d) h[d] = 0.5 * dd[d];
+            taylor<D, BG>(h, pint);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < BG; ++q) v = v + g[q * M + m] * pint[q];
+                const bool lim = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
+                if (lim) {
+                    const double r = (v > 0.0 ? dup[m] : ddn[m]) / v;
+                    const double pk = a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r);
+                    phi[m] = jmin(phi[m], pk);
+                }
+            }
+        }
+        const bool hard = a.mode == HCC_MODE_HARD;
+#pragma unroll
+        for (int q = 0; q < BG; ++q)
+            if (hard || q < D)
+#pragma unroll
+                for (int m = 0; m < M; ++m) g[q * M + m] = g[q * M + m] * phi[m];
+    }
+    store_rec<L::NG, L::GS>(a.G, i, g);
+}
+
+// the step's order reset (TimeSteppers.jl:544-545): the maximum order 3
+__device__ __forceinline__ void step_reset3(const StepArgs& a, int64_t i) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counters[1] = 0; a.counters[2] = 0x7FF0000000000000ull; a.counters[4] = 0;
+    }
+    if (i < a.g.N) { a.orders[i] = 3; a.trig[i] = 0; }
+}
+
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __restrict__ Y, int full, int reset,
+                                                    int64_t nblk) {
+    if (full && blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }
+    const int64_t i = node_index(nblk);
+    if (reset) step_reset3(a, i);
+    if (i < a.g.N && !a.g.isb[i] && (full || a.changed[i])) {
+        if (full) gather_node<D, M, KIND, true>(a, Y, i);
+        else gather_node<D, M, KIND, false>(a, Y, i);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// flux: the MUSCL functor (MUSCL.jl:390-424) and _flux_loop (:463-508) with the degree-2 interface basis masked to
+// each side's interface order (mask_basis, Interpolators.jl:265-284), Rusanov, and the divergence: the cached fit
+// (apply_weights_divergence, Interpolators.jl:610-626; the particle's order is 3, or its fit is the cache's degree)
+// or the direct degree-1 fit with intercept (compute_dynamic_divergence -> _mls_divergence, :308-346); K_E = -2 div
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M, int KIND>
+__device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + M], const double (&fi)[M],
+                                      const double (&gi)[L3<D, M>::NG], const double (&Fi)[D][M], int oi,
+                                      const double (&rj)[D + M], const double (&gj)[L3<D, M>::NG], int oj,
+                                      double (&dd)[D], double& w, double (&dF)[D][M]) {
+    using P = Pde<D, M, KIND>;
+    constexpr int BG = L3<D, M>::BG;
+    const double d2 = rec_geom<D>(a.g, ri, rj, dd);
+    w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+    int o_i = oi, o_j = oj;
+    if (a.strategy == HCC_STRAT_EPD1) { o_i = oi < oj ? oi : oj; o_j = o_i; }
+    double hi[D], hj[D], pii[BG], pjj[BG];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { hi[d] = 0.5 * dd[d]; hj[d] = -0.5 * dd[d]; }
+    taylor<D, BG>(hi, pii);
+    taylor<D, BG>(hj, pjj);
+    const int ki = bcount(D, o_i - 1), kj = bcount(D, o_j - 1);
+#pragma unroll
+    for (int q = 0; q < BG; ++q) {
+        if (q >= ki) pii[q] = 0.0;
+        if (q >= kj) pjj[q] = 0.0;
+    }
+    double fij[M], fji[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double si = gi[m] * pii[0], sj = gj[m] * pjj[0];
+#pragma unroll
+        for (int q = 1; q < BG; ++q) { si = si + gi[q * M + m] * pii[q]; sj = sj + gj[q * M + m] * pjj[q]; }
+        fij[m] = fi[m] + si;
+        fji[m] = rj[D + m] + sj;
+    }
+    double Fij[D][M], Fji[D][M], lij[D], lji[D];
+    P::flux_eig(a.pde, fij, Fij, lij);
+    P::flux_eig(a.pde, fji, Fji, lji);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        // the exact shortcuts of the order-2 slot (commuted F_L + F_R, symmetric max of the speeds, the side's sign
+        // on the speed, the dropped + 0.0 of the jump)
+        const double sd = jmax(lij[d], lji[d]);
+        const double sds = dd[d] > 0.0 ? sd : -sd;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const double x = fji[m] - fij[m];
+            const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - sds * x);
+            dF[d][m] = Fnum - Fi[d][m];
+        }
+    }
+}
+
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
+                                          double* __restrict__ K, int64_t i, double h) {
+    using L = L3<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, BD = L::BD, B1 = L::B1, DM = D * M;
+    const int64_t N = a.g.N;
+    const int cnt = a.g.cnt[i];
+    double ri[NR], fi[M], Kout[M];
+    load_rec<NR, L::PU>(Y, i, ri);
+#pragma unroll
+    for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
+    if (cnt == 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) Kout[m] = -0.0 + explicit_source<M>(a, i, fi, m);
+        return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, fi, Kout, h);
+    }
+    double gi[NG], Fi[D][M];
+    load_rec<NG, L::GS>(a.G, i, gi);
+    Pde<D, M, KIND>::flux(a.pde, fi, Fi);
+    // interface order source and the particle's own order (EPD2: the flux pass reads min | order << 4)
+    int oi = a.orders[i], oi_t = oi;
+    if (a.packed) { oi_t = oi >> 4; oi &= 15; }
+    const int64_t base = sell_base(a.g, i);
+    const double invL = a.g.invL;
+    const int full_idx = div_index(D, 3, cnt);
+    const bool okd = a.okd[i];
+    const bool cached = okd && div_index(D, oi_t, cnt) == full_idx;
+    if (cached || oi_t == 3) {
+        // the cached fit (order 3 without a factorisation: the direct solve of the same matrix fails, zero)
+        double b[BD][DM];
+#pragma unroll
+        for (int c = 0; c < BD; ++c)
+#pragma unroll
+            for (int q = 0; q < DM; ++q) b[c][q] = 0.0;
+        if (okd) {
+            for (int k = 0; k < cnt; ++k) {
+                const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+                double rj[NR], gj[NG], dd[D], w, dF[D][M];
+                load_rec<NR, L::PU>(Y, j, rj);
+                load_rec<NG, L::GS>(a.G, j, gj);
+                slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (a.orders[j] & 15) : a.orders[j], dd, w, dF);
+                double ds[D], t[BD - 1];
+#pragma unroll
+                for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+                taylor<D, BD - 1>(ds, t);
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        b[0][d * M + m] = b[0][d * M + m] + w * (1.0 * dF[d][m]);
+#pragma unroll
+                        for (int c = 1; c < BD; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (t[c - 1] * dF[d][m]);
+                    }
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    double r = a.invd[(d * BD + 0) * N + i] * b[0][d * M + m];
+#pragma unroll
+                    for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c][d * M + m];
+                    acc = acc + r;
+                }
+                Kout[m] = -(2.0 * (acc * invL)) + explicit_source<M>(a, i, fi, m);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, fi, m);
+        }
+    } else {
+        // below the full order: the degree-1 fit with intercept, assembled and factorised here
+        double A[B1][B1], b[B1][DM];
+#pragma unroll
+        for (int r = 0; r < B1; ++r)
+#pragma unroll
+            for (int c = 0; c < B1; ++c) A[r][c] = 0.0;
+#pragma unroll
+        for (int c = 0; c < B1; ++c)
+#pragma unroll
+            for (int q = 0; q < DM; ++q) b[c][q] = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double rj[NR], gj[NG], dd[D], w, dF[D][M];
+            load_rec<NR, L::PU>(Y, j, rj);
+            load_rec<NG, L::GS>(a.G, j, gj);
+            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (a.orders[j] & 15) : a.orders[j], dd, w, dF);
+            double p[B1];
+            p[0] = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) p[1 + d] = dd[d] * invL;
+#pragma unroll
+            for (int r = 0; r < B1; ++r)
+#pragma unroll
+                for (int c = 0; c < B1; ++c) A[r][c] = A[r][c] + w * (p[r] * p[c]);
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+#pragma unroll
+                    for (int c = 0; c < B1; ++c) b[c][d * M + m] = b[c][d * M + m] + w * (p[c] * dF[d][m]);
+        }
+        double U[B1][B1];
+        if (chol_moment<B1>(A, U)) {
+            double E[B1][D], X[B1][D];
+#pragma unroll
+            for (int r = 0; r < B1; ++r)
+#pragma unroll
+                for (int d = 0; d < D; ++d) E[r][d] = r == 1 + d ? 1.0 : 0.0;
+            chol_solve<B1, D>(U, E, X);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    double r = X[0][d] * b[0][d * M + m];
+#pragma unroll
+                    for (int c = 1; c < B1; ++c) r = r + X[c][d] * b[c][d * M + m];
+                    acc = acc + r;
+                }
+                Kout[m] = -(2.0 * (acc * invL)) + explicit_source<M>(a, i, fi, m);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, fi, m);
+        }
+    }
+    return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, fi, Kout, h);
+}
+
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                  double* __restrict__ K, int full, int64_t nblk) {
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    bool ch = false;
+    int bad = 0;
+    if (i < N) {
+        const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, i) : 0.0;
+        if (a.g.isb[i]) {
+            if (full)
+#pragma unroll
+                for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+        } else if (full || a.dirty[i]) {
+            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, K, i, h);
+            if (CHECK) bad = a.inadm[i];
+        }
+    }
+    if (CHECK) {
+        append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, bad);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// the fused fixed point (one lane a particle): k_mark_gather drops the listed orders, marks the particles and their
+// neighbours dirty (listing each once), and recomputes the listed particles' gradients; k_flux_list recomputes K, the
+// candidate and the check over the work list. The bookkeeping is that of the order-2 kernels.
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __restrict__ Y, unsigned long long* pub,
+                                                    int p) {
+    constexpr int kListCap = 512;
+    __shared__ int32_t lbuf[kListCap];
+    const int64_t n = *changed_any(a, p) ? (int64_t)*changed_count(a, p) : 0;
+    const int32_t* cl = changed_list(a, p);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *changed_count(a, 1 - p) = 0;
+        *work_count(a, 1 - p) = 0;
+        *changed_any(a, 1 - p) = 0;
+        if (n > 0) a.counters[4] += 1;
+        __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    const int lane = threadIdx.x;
+    for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < n; t0 += (int64_t)gridDim.x * 64) {
+        const int64_t t = t0 + lane;
+        int32_t i = 0;
+        int cnt = -1;
+        if (t < n && cl[t] >= 0) {
+            i = cl[t];
+            cnt = a.g.cnt[i];
+            a.orders[i] = (int8_t)(a.orders[i] - 1);
+        }
+        int mc = cnt;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+        const int64_t base = cnt >= 0 ? sell_base(a.g, i) : 0;
+        int nl = 0;
+        auto flush = [&]() {
+            __syncthreads();
+            if (nl > 0) {
+                unsigned long long wb = 0;
+                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
+                wb = __shfl(wb, 0);
+                for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
+            }
+            __syncthreads();
+            nl = 0;
+        };
+        for (int k = -1; k < mc; ++k) {
+            if (nl > kListCap - 64) flush();
+            int32_t j = 0;
+            bool first = false;
+            if (k < cnt) {
+                j = k < 0 ? i : a.g.nbr[base + (int64_t)k * kChunk];
+                first = atomicExch(&a.dmark[j], 1u) == 0u;
+            }
+            const unsigned long long ball = __ballot(first);
+            const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+            if (first) lbuf[nl + __popcll(below)] = j;
+            nl += __popcll(ball);
+        }
+        flush();
+        if (cnt >= 0 && !a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, i);
+    }
+}
+
+template <int D, int M, int KIND, bool FP = true>
+__global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                  double* __restrict__ K, int p) {
+    const int64_t n = (int64_t)*work_count(a, p);
+    if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
+    for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < n; t0 += (int64_t)gridDim.x * 64) {
+        const int64_t t = t0 + threadIdx.x;
+        bool ch = false;
+        int delta = 0;
+        int64_t i = -1;
+        if (t < n) {
+            i = a.dlist[t];
+            if (FP) a.dmark[i] = 0u;
+            if (!a.g.isb[i]) {
+                if constexpr (FP) {
+                    const int before = a.inadm[i];
+                    ch = flux_node<D, M, KIND, true>(a, pred, Y, K, i, a.hratio * node_dt(a.dtp, a.inst, i));
+                    delta = (int)a.inadm[i] - before;
+                } else {
+                    flux_node<D, M, KIND, false>(a, pred, Y, K, i, 0.0);
+                }
+            }
+        }
+        if constexpr (!FP) continue;
+        if (t < n) {
+            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+            if (ch) *changed_any(a, 1 - p) = 1;
+        }
+        if (pred.count) add_inadmissible(a, delta);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// CFL estimate with the cached weights (_cfl_weights_body!, ParticleGrids.jl:439-450): the per-slot derivative
+// weights of the fit in use recomputed from its derivative rows with the operations of _weights_body!
+// (MUSCL.jl:211-223, the full-degree intercept basis); a.lam holds the wave speeds (k_lambda)
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_cfl(StepArgs a, const double* __restrict__ U, int64_t nblk) {
+    using L = L3<D, M>;
+    constexpr int BD = L::BD;
+    const int64_t N = a.g.N;
+    const int64_t i = node_index(nblk);
+    double dt = INFINITY;
+    if (i < N && !a.g.isb[i] && a.g.cnt[i] > 0) {
+        double li[D], pi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) li[d] = a.lam[i * D + d];
+        load_rec<D, L::PU>(U, i, pi);
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        const double invL = a.g.invL;
+        double sum_c = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double pj[D], dd[D], ds[D], t[BD - 1], v[BD];
+            load_rec<D, L::PU>(U, j, pj);
+            const double d2 = rec_geom<D>(a.g, pi, pj, dd);
+            const double w = mls_weight(a.g.alpha, a.g.inv_range_sq, d2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+            taylor<D, BD - 1>(ds, t);
+            v[0] = w * 1.0;
+#pragma unroll
+            for (int c = 1; c < BD; ++c) v[c] = w * t[c - 1];
+            double dv = 0.0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                double acc = a.invd[(r * BD + 0) * N + i] * v[0];
+#pragma unroll
+                for (int c = 1; c < BD; ++c) acc = acc + a.invd[(r * BD + c) * N + i] * v[c];
+                const double term = fabs(acc * invL) * jmax(li[r], a.lam[(int64_t)j * D + r]);
+                dv = (r == 0) ? term : dv + term;
+            }
+            sum_c = sum_c + dv;
+        }
+        if (sum_c > kFloatMin) dt = (1.0 / (double)D) / sum_c;
+    }
+    block_min(a, dt, logical_block(nblk));
+}
+
+}  // namespace m3

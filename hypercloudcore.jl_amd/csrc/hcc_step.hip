@@ -2282,11 +2282,14 @@ __global__ __launch_bounds__(256) void k_mass(const double* __restrict__ U, cons
 // ============================================================================================================
 // the stepper
 // ============================================================================================================
+#include "hcc_muscl3.h"
+
 struct hcc_stepper {
     hcc_ctx* ctx = nullptr;
     hcc_grid* grid = nullptr;
     hcc_stepper_desc desc{};
     int D = 0, M = 0, kind = 0, PU = 0, GS = 0;
+    int order = 2;             // MUSCL max_order: 2, or 3 (the kernels of hcc_muscl3.h)
     int fv = 0;                // flux loop variant: 0 buffered fit + RusanovFlux, 1 streamed and/or AlignedRusanovFlux
     int64_t N = 0;
     // geometry caches
@@ -2633,6 +2636,69 @@ static VariantOps make_ops() {
     return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex, &V::range, &V::cfl, &V::predict,
                       &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
+// MUSCL order 3: the passes of hcc_muscl3.h (neighbours through the caches, one lane a particle in the fixed point);
+// the predictors, the unfused check, the ranges and the record packing are the order-2 variant's
+template <int D, int M, int KIND>
+struct Variant3 : Variant<D, M, KIND> {
+    using V2 = Variant<D, M, KIND>;
+    static hipStream_t st(const hcc_stepper* s) { return s->ctx->stream; }
+    static unsigned list_grid(const hcc_stepper* s) {
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>((s->N + 63) / 64, s->tune.sweep / 64));
+    }
+    static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full, int reset) {
+        const int64_t nb = nblocks(s->N);
+        m3::k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+    }
+    static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
+        const int64_t nb = nblocks(s->N);
+        k_lambda<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        m3::k_cfl<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, s->U, nb);
+        V2::cfl_final(s, dtp, cfl, cap);
+    }
+    // never fused for order 3 (cfl_fused needs the staged gather); kept for the table: the estimate, then the gather
+    static void gather_cfl(const hcc_stepper* s, const StepArgs& a, const double* Y, int reset, double c, double cap) {
+        cfl(s, a, s->dtp, c, cap);
+        gather(s, a, Y, 1, reset);
+    }
+    static void mark_gather(const hcc_stepper* s, const StepArgs& a, const double* Y, unsigned long long* pub, int p) {
+        m3::k_mark_gather<D, M, KIND><<<list_grid(s), 64, 0, st(s)>>>(a, Y, pub, p);
+    }
+    // full: 1 every particle, 0 the dirty ones, -1 the dirty ones listed first (k_compact_dirty)
+    static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
+                     bool check) {
+        const int64_t nb = nblocks(s->N);
+        if (full < 0) {
+            k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
+            k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
+            m3::k_flux_list<D, M, KIND, false><<<list_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
+        } else if (check) {
+            m3::k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+        } else {
+            m3::k_flux<D, M, KIND, false><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
+        }
+    }
+    static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
+        m3::k_flux_list<D, M, KIND, true><<<list_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+    }
+};
+template <int D, int M, int KIND>
+static VariantOps make_ops3() {
+    using V = Variant3<D, M, KIND>;
+    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex,
+                      &V::range, &V::cfl, &V::predict, &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack,
+                      Layout<D, M>::PU, m3::L3<D, M>::GS};
+}
+// MUSCL3 variants: advection (1, 1), (2, 1), (2, 2) and 2-D Euler
+static bool find_ops3(int D, int M, int kind, VariantOps& ops) {
+    if (kind == HCC_PDE_LINADV) {
+        if (D == 1 && M == 1) { ops = make_ops3<1, 1, HCC_PDE_LINADV>(); return true; }
+        if (D == 2 && M == 1) { ops = make_ops3<2, 1, HCC_PDE_LINADV>(); return true; }
+        if (D == 2 && M == 2) { ops = make_ops3<2, 2, HCC_PDE_LINADV>(); return true; }
+    } else if (kind == HCC_PDE_EULER) {
+        if (D == 2 && M == 4) { ops = make_ops3<2, 4, HCC_PDE_EULER>(); return true; }
+    }
+    return false;
+}
 static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     if (kind == HCC_PDE_LINADV) {
         if (D == 1 && M == 1) { ops = make_ops<1, 1, HCC_PDE_LINADV>(); return true; }
@@ -2646,6 +2712,10 @@ static bool find_ops(int D, int M, int kind, VariantOps& ops) {
     return false;
 }
 
+static bool find_ops(const hcc_stepper* s, VariantOps& ops) {
+    return s->order == 3 ? find_ops3(s->D, s->M, s->kind, ops) : find_ops(s->D, s->M, s->kind, ops);
+}
+
 // prepare_geometry! (MUSCL.jl:172-223): the moment inverses and derivative weights of the grid's current positions;
 // R_cut: R^2 on a moving grid's stage geometry (drifted pairs weigh 0), +inf on a searched grid
 // dw: the derivative weights' buffer (D S_sell), or null: no pass reads them (the streamed and aligned flux loops
@@ -2656,6 +2726,11 @@ static void launch_moments(hcc_stepper* s, double R_cut, double* dw = nullptr) {
     ma.g.R_cut = R_cut;
     const int64_t nb = nblocks(s->N);
     hipStream_t st = s->ctx->stream;
+    if (s->order == 3) {   // the degree-2 reconstruction and the safeguarded degree-3 divergence
+        if (s->D == 1) m3::k_moments3<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        else m3::k_moments3<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        return;
+    }
     if (s->D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
     else if (s->D == 2) k_moments<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
     else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
@@ -2694,9 +2769,17 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     *out = nullptr;
     const int D = g->geom.D;
     VariantOps ops;
-    if (!find_ops(D, d->M, d->pde, ops))
-        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: no compiled kernel for this (D, M, PDE) -- there is no fallback path");
-    if (d->muscl_order != 2) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL order 2 is the compiled scheme");
+    if (d->muscl_order != 2 && d->muscl_order != 3)
+        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL orders 2 and 3 are compiled");
+    if (!(d->muscl_order == 3 ? find_ops3(D, d->M, d->pde, ops) : find_ops(D, d->M, d->pde, ops)))
+        return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: no compiled kernel for this (D, M, PDE, MUSCL order) -- there is no fallback path");
+    if (d->muscl_order == 3) {
+        if (d->muscl_streamed || d->flux != HCC_FLUX_RUSANOV)
+            return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL3 runs the buffered divergence with RusanovFlux");
+        if (d->mover != HCC_MOVER_NONE) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL3 on a moving grid");
+        if (d->mood_criterion == HCC_CRIT_U2)
+            return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOODu2's curvature override with MUSCL3");
+    }
     if (d->stages < 1 || d->stages > kMaxStages) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: 1..5 stages");
     if (d->limiter < HCC_LIM_NONE || d->limiter > HCC_LIM_VK) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: unknown limiter");
     if (d->mood_strategy < HCC_STRAT_NONE || d->mood_strategy > HCC_STRAT_STRICT_EPD0) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOOD strategy");
@@ -2732,14 +2815,16 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     std::unique_ptr<hcc_stepper, int (*)(hcc_stepper*)> s(new hcc_stepper(), hcc_stepper_destroy);
     s->ctx = ctx; s->grid = g; s->desc = *d; s->D = D; s->M = d->M; s->kind = d->pde; s->N = g->N;
     s->PU = ops.PU; s->GS = ops.GS;
+    s->order = d->muscl_order;
     s->moving = d->mover != HCC_MOVER_NONE;
     s->fv = (d->muscl_streamed || d->flux == HCC_FLUX_ALIGNED_RUSANOV || s->moving) ? 1 : 0;
     s->imex = implicit_part;
     const int64_t N = g->N, M = d->M;
     hipStream_t st = ctx->stream;
-    HCC_TRY_HIP(ctx, dalloc(&s->invg, (int64_t)D * D * N));
+    const int BG = s->order == 3 ? m3::bcount(D, 2) : D, BD = s->order == 3 ? m3::bcount(D, 3) + 1 : D + 1;
+    HCC_TRY_HIP(ctx, dalloc(&s->invg, (int64_t)BG * BG * N));
     HCC_TRY_HIP(ctx, dalloc(&s->okg, N));
-    HCC_TRY_HIP(ctx, dalloc(&s->invd, (int64_t)D * (D + 1) * N));
+    HCC_TRY_HIP(ctx, dalloc(&s->invd, (int64_t)D * BD * N));
     HCC_TRY_HIP(ctx, dalloc(&s->okd, N));
     HCC_TRY_HIP(ctx, dalloc(&s->root, N));
     HCC_TRY_HIP(ctx, dalloc(&s->U, s->PU * N));
@@ -2783,11 +2868,11 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->dmark, 0, sizeof(unsigned) * std::max<int64_t>(N, 1), st));
     for (uint8_t* p : {s->changed, s->dirty, s->trig, s->inadm})
         HCC_TRY_HIP(ctx, hipMemsetAsync(p, 0, std::max<int64_t>(N, 1), st));
-    HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, 2, std::max<int64_t>(N, 1), st));
+    HCC_TRY_HIP(ctx, hipMemsetAsync(s->orders, s->order, std::max<int64_t>(N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 10 * sizeof(unsigned long long), st));
-    if (g->max_halo > 0 && !s->moving) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
+    if (g->max_halo > 0 && !s->moving && s->order == 2) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
         int lds_max = 0;
         HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
         const size_t hm = (size_t)g->max_halo;
@@ -2872,7 +2957,7 @@ extern "C" int hcc_state_upload(hcc_stepper* s, const double* U) {
     if (!s->N) return HCC_OK;
     if (int rc = graph_settle(s)) return rc;   // a pending captured step completes (or is re-run) on the old state
     VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
+    find_ops(s, ops);
     HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, U, sizeof(double) * s->N * s->M, hipMemcpyHostToDevice, ctx->stream));
     // every state buffer: ghosts without an outflow root keep pg.rhos in all stages and are never written again
     ops.pack(s, s->scratch, s->U);
@@ -2890,7 +2975,7 @@ extern "C" int hcc_state_download(hcc_stepper* s, double* U) {
     if (!s->N) return HCC_OK;
     if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
+    find_ops(s, ops);
     ops.unpack(s, s->U, s->scratch);
     HCC_TRY_HIP(ctx, hipMemcpyAsync(U, s->scratch, sizeof(double) * s->N * s->M, hipMemcpyDeviceToHost, ctx->stream));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2903,7 +2988,7 @@ extern "C" int hcc_time_step(hcc_stepper* s, double* dt) {
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
+    find_ops(s, ops);
     k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
     if (s->N) ops.cfl(s, s->args(), nullptr, 0.0, 0.0);
     HCC_TRY_HIP(ctx, hipGetLastError());
@@ -2924,7 +3009,7 @@ extern "C" int hcc_time_steps(hcc_stepper* s, double* dts) {
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = graph_settle(s)) return rc;
     VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
+    find_ops(s, ops);
     k_reset_counters<<<1, 64, 0, ctx->stream>>>(s->counters, 1);
     ops.cfl(s, s->args(), s->dtp, 1.0, INFINITY);   // dtp[k] = min(1 dt_k, inf) = dt_k; the next step sets dtp again
     HCC_TRY_HIP(ctx, hipGetLastError());
@@ -2961,10 +3046,13 @@ struct StepRunner {
         StepArgs f = a;
         if (s->moving) f.KX = s->KX[kst];
         if (!epd2()) return f;
-        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, a.aligned, only, nb);
+        // the aligned flux and MUSCL3 (its divergence degree follows the particle's own order) also need the true
+        // orders: packed as min | order << 4
+        const int pack = a.aligned || s->order == 3;
+        k_nbmin<<<grid_for(nb), kBlock, 0, s->ctx->stream>>>(a, s->nbmin, pack, only, nb);
         f.orders = s->nbmin;
         f.strategy = HCC_STRAT_EPD1;
-        f.packed = a.aligned;
+        f.packed = pack;
         return f;
     }
     // the predictor runs inside the flux passes unless MOOD checks the stored candidate itself
@@ -3291,7 +3379,7 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     auto& slot = s->ring[s->ring_head];
     StepRunner r;
     r.s = s;
-    find_ops(s->D, s->M, s->kind, r.ops);
+    find_ops(s, r.ops);
     r.a = step_args(s);
     r.evp = slot.evp;
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
@@ -3370,7 +3458,7 @@ static int graph_step(hcc_stepper* s) {
         if (k < 0) { drop_graphs(s); k = 0; }
         StepRunner r;
         r.s = s;
-        find_ops(s->D, s->M, s->kind, r.ops);
+        find_ops(s, r.ops);
         r.a = step_args(s);
         r.graph = true;
         HCC_TRY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
@@ -3619,7 +3707,7 @@ extern "C" int hcc_step_cfl(hcc_stepper* s, double cfl, double dt_cap) {
     }
     if (!s->cfl_fused) {
         VariantOps ops;
-        find_ops(s->D, s->M, s->kind, ops);
+        find_ops(s, ops);
         ops.cfl(s, s->args(), s->dtp, cfl, dt_cap);
     }
     s->cfl_num = cfl;
@@ -3678,17 +3766,18 @@ extern "C" int hcc_get_moments(hcc_stepper* s, double* inv_grad, uint8_t* ok_gra
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int64_t N = s->N;
-    const int D = s->D, BD = D + 1;
+    const int D = s->D;
+    const int BG = s->order == 3 ? m3::bcount(D, 2) : D, BD = s->order == 3 ? m3::bcount(D, 3) + 1 : D + 1;
     if (inv_grad && N) {
-        std::vector<double> v((size_t)(D * D * N));
+        std::vector<double> v((size_t)(BG * BG * N));
         HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->invg, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < N; ++i) for (int q = 0; q < D * D; ++q) inv_grad[i * D * D + q] = v[(size_t)(q * N + i)];
+        for (int64_t i = 0; i < N; ++i) for (int q = 0; q < BG * BG; ++q) inv_grad[i * BG * BG + q] = v[(size_t)(q * N + i)];
     }
-    if (inv_div && N) {   // rows 1..D of the (D+1)^2 inverse; row 0 is not kept on the device (NaN)
+    if (inv_div && N) {   // rows 1..D of the BD x BD inverse; row 0 and rows past D are not kept on the device (NaN)
         std::vector<double> v((size_t)(D * BD * N));
         HCC_TRY_HIP(ctx, hipMemcpy(v.data(), s->invd, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < N; ++i) {
-            for (int c = 0; c < BD; ++c) inv_div[i * BD * BD + c] = NAN;
+            for (int c = 0; c < BD * BD; ++c) inv_div[i * BD * BD + c] = NAN;
             for (int r = 0; r < D; ++r)
                 for (int c = 0; c < BD; ++c) inv_div[i * BD * BD + (1 + r) * BD + c] = v[(size_t)((r * BD + c) * N + i)];
         }
@@ -3730,13 +3819,13 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     const int M = s->M, D = s->D;
     if (!N) return HCC_OK;
     VariantOps ops;
-    find_ops(D, M, s->kind, ops);
+    find_ops(s, ops);
     hipStream_t st = ctx->stream;
     HCC_TRY_HIP(ctx, hipMemcpyAsync(s->scratch, Y, sizeof(double) * N * M, hipMemcpyHostToDevice, st));
     ops.pack(s, s->scratch, s->Y[0]);
     const StepArgs a = s->args();
     k_fill_f64<<<(unsigned)((s->GS * N + kBlock - 1) / kBlock), kBlock, 0, st>>>(s->G, 0.0, (int64_t)s->GS * N);
-    k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, st>>>((uint8_t*)s->orders, 2, N);
+    k_fill_u8<<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, st>>>((uint8_t*)s->orders, (uint8_t)s->order, N);
     ops.gather(s, a, s->Y[0], 1, 0);
     ops.flux(s, a, PredArgs{}, s->Y[0], s->K[0], 1, false);
     HCC_TRY_HIP(ctx, hipGetLastError());
@@ -3745,8 +3834,9 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     if (gradients) {
         std::vector<double> gs((size_t)(s->GS * N));
         HCC_TRY_HIP(ctx, hipMemcpy(gs.data(), s->G, sizeof(double) * gs.size(), hipMemcpyDeviceToHost));
+        const int NG = (s->order == 3 ? m3::bcount(D, 2) : D) * M;   // basis terms x components, term major
         for (int64_t i = 0; i < N; ++i)
-            for (int q = 0; q < D * M; ++q) gradients[i * D * M + q] = gs[(size_t)(i * s->GS + q)];
+            for (int q = 0; q < NG; ++q) gradients[i * NG + q] = gs[(size_t)(i * s->GS + q)];
     }
     return HCC_OK;
 }
@@ -3784,7 +3874,7 @@ extern "C" int hcc_total_mass(hcc_stepper* s, int32_t interior_only, double* out
         return HCC_OK;
     }
     VariantOps ops;
-    find_ops(s->D, s->M, s->kind, ops);
+    find_ops(s, ops);
     ops.mass(s, interior_only ? 1 : 0, s->scratch);   // after the queued steps, on the context stream
     HCC_TRY_HIP(ctx, hipGetLastError());
     HCC_TRY_HIP(ctx, hipMemcpyAsync(out, s->scratch, sizeof(double) * M, hipMemcpyDeviceToHost, ctx->stream));
@@ -3920,8 +4010,8 @@ extern "C" int hcc_set_source(hcc_stepper* s, int32_t mode, const double* rates)
 // ============================================================================================================
 static int feature_buffers(hcc_stepper* s) {
     hcc_ctx* ctx = s->ctx;
-    if (s->D != 2 || s->M != 4 || s->kind != HCC_PDE_EULER)
-        return set_err(ctx, HCC_E_UNSUPPORTED, "the indicator features are defined for 2-D Euler");
+    if (s->D != 2 || s->M != 4 || s->kind != HCC_PDE_EULER || s->order != 2)
+        return set_err(ctx, HCC_E_UNSUPPORTED, "the indicator features are defined for 2-D Euler with MUSCL2");
     if (!s->feat) {
         HCC_TRY_HIP(ctx, dalloc(&s->feat, 16 * s->N));
         HCC_TRY_HIP(ctx, hipMemset(s->feat, 0, sizeof(uint16_t) * std::max<int64_t>(16 * s->N, 1)));

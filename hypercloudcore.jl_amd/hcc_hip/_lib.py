@@ -435,12 +435,26 @@ def stepper_desc(scheme):
     return sd
 
 
+def basis_count(D, q):
+    """Terms of degree 1..q in D variables, binomial(D + q, q) - 1 (Interpolators.jl:170-177)."""
+    if q <= 0:
+        return 0
+    n = 1
+    for k in range(1, q + 1):
+        n = (n * (D + k)) // k
+    return n - 1
+
+
 class Stepper:
-    """Device time stepper (GeneralRKTimeStepper with MUSCL-2 + MOOD) bound to one grid."""
+    """Device time stepper (GeneralRKTimeStepper with MUSCL order 2 or 3 + MOOD) bound to one grid."""
 
     def __init__(self, ctx, grid, scheme):
         self.ctx, self.grid, self.scheme = ctx, grid, scheme
         self.N, self.M, self.D = grid.N, scheme["M"], grid.D
+        self.order = int(scheme.get("muscl_order", 2))
+        # reconstruction terms (degree order - 1) and divergence terms with intercept (degree 1, or 3 for MUSCL3)
+        self.BG = basis_count(self.D, self.order - 1)
+        self.BD = basis_count(self.D, 1 if self.order <= 2 else 3) + 1
         self.sd = stepper_desc(scheme)
         h = C.c_void_p()
         check(lib().hcc_stepper_create(ctx.h, grid.h, C.byref(self.sd), C.byref(h)), ctx.h, "hcc_stepper_create")
@@ -546,9 +560,9 @@ class Stepper:
 
     def moments(self):
         D, N, S = self.D, self.N, self.grid.S
-        ig = np.zeros((N, D, D))
+        ig = np.zeros((N, self.BG, self.BG))
         og = np.zeros(N, np.uint8)
-        idv = np.zeros((N, D + 1, D + 1))
+        idv = np.zeros((N, self.BD, self.BD))
         od = np.zeros(N, np.uint8)
         dw = np.zeros((max(S, 1), D))
         check(lib().hcc_get_moments(self.h, ptr(ig), ptr(og), ptr(idv), ptr(od), ptr(dw)), self.ctx.h, "hcc_get_moments")
@@ -557,7 +571,7 @@ class Stepper:
     def rhs(self, Y):
         Y = np.ascontiguousarray(Y, np.float64).reshape(self.N, self.M)
         K = np.zeros((self.N, self.M))
-        G = np.zeros((self.N, self.D, self.M))
+        G = np.zeros((self.N, self.BG, self.M))
         check(lib().hcc_rhs(self.h, ptr(Y), ptr(K), ptr(G)), self.ctx.h, "hcc_rhs")
         return K, G
 

@@ -206,6 +206,28 @@ def euler_3d(n=160, seed=0, gamma=1.4):
     return {"name": f"euler3d_n{n}", "domain": dom, "scheme": sch, "cloud": cloud, "U0": U0, "cfl": 0.3, "steps": 100}
 
 
+def step_benchmark(n=1000, M=1, stab="both", D=2):
+    """The reference's own step benchmark (benchmark/step.jl:15-80): linear advection of a diagonal system of M
+    equations (speed (1 - 0.5 (d - 1)) (1 + 0.2 (m - 1)) along axis d for equation m) with MUSCL3 + RusanovFlux,
+    SSP-RK3, on the uniform periodic n^D lattice of the unit box, ExponentialWeightFunction(1, 2.5 dx); the state
+    exp(-|x - 0.5|^2 / 0.02) (1 + 0.1 (m - 1)); stab "both": VenkatakrishnanLimiter(:hard) + MOOD(EPD1, MOODu1(1e-6)),
+    "limiter", "mood" or "none"; dt = 0.3 get_time_step at t = 0, fixed."""
+    h = 1.0 / n
+    dom = domain(D, (0.0,) * D, (1.0,) * D, (True,) * D, h, 2.5, 0.0, 0)
+    speeds = [[(1 - 0.5 * d) * (1 + 0.2 * m) for m in range(M)] for d in range(D)]
+    lim = L.HCC_LIM_VK if stab in ("limiter", "both") else L.HCC_LIM_NONE
+    mood = stab in ("mood", "both")
+    sch = scheme(M, L.HCC_PDE_LINADV, adv_speed=speeds, limiter=lim, mode=L.HCC_MODE_HARD,
+                 strategy=L.HCC_STRAT_EPD1 if mood else L.HCC_STRAT_NONE,
+                 criterion=L.HCC_CRIT_U1 if mood else L.HCC_CRIT_NONE, delta=1e-6, order=3)
+    cloud = L.cloud_generate(domain_desc(dom))
+    x = cloud["pos"]
+    g = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
+    U0 = np.column_stack([g * (1 + 0.1 * m) for m in range(M)])
+    return {"name": f"step_benchmark_{D}d_n{n}_m{M}_{stab}", "domain": dom, "scheme": sch, "cloud": cloud, "U0": U0,
+            "dt_factor": 0.3, "steps": 30, "warmup": 3}
+
+
 def mean_neighbours(grid):
     return grid.S / max(grid.N, 1)
 

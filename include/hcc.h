@@ -117,7 +117,8 @@ typedef struct {
     int32_t pde;               /* HCC_PDE_* */
     double gamma;              /* Euler: ratio of specific heats */
     double adv_speed[3][8];    /* linear advection: speed of component m along axis d (diagonal system) */
-    int32_t muscl_order;       /* MUSCL max_order (2 is compiled) */
+    int32_t muscl_order;       /* MUSCL max_order: 2, or 3 (advection (1,1), (2,1), (2,2), 2-D Euler; buffered
+                                  RusanovFlux on static grids; MOODu2's curvature override not compiled) */
     int32_t limiter, limiter_mode;
     int32_t mood_strategy, mood_criterion;
     double mood_delta;         /* MOODu1 / MOODu2 d */
@@ -162,11 +163,14 @@ int hcc_get_mood_flags(hcc_stepper* st, uint8_t* flags);
 int hcc_get_inadmissible(hcc_stepper* st, int64_t* count);
 int hcc_get_mood_iterations(hcc_stepper* st, int64_t* count);
 /* Geometry caches (row-major B*B per node; ok bytes; dweights S*D in row-major slot order, formed by this call
- * from the current geometry: the step's passes keep no copy). */
+ * from the current geometry: the step's passes keep no copy). B: the reconstruction's basis_count(D, order - 1)
+ * terms; the divergence's basis_count(D, 1 or 3) + 1 (MUSCL order 2 or 3), of which only the D derivative rows are
+ * kept (the other rows read NaN; a stencil that carries a lower degree has its inverse zero-padded). */
 int hcc_get_moments(hcc_stepper* st, double* inv_grad, uint8_t* ok_grad, double* inv_div, uint8_t* ok_div,
                     double* dweights);
 /* One full right-hand side (gather + flux pass, orders reset to the maximum) on state Y (N*M AoS):
- * K (N*M) and the limited gradients (N*D*M, node-major then basis term then component). */
+ * K (N*M) and the limited gradients (N*B*M with B = basis_count(D, order - 1): node-major, then basis term in the
+ * order of _generate_exponents, then component). */
 int hcc_rhs(hcc_stepper* st, const double* Y, double* K, double* gradients);
 /* Timing of the last hcc_step on the context stream (HIP events), milliseconds: the whole step, and the summed
  * durations of its full-domain gather (gradient + limiter) and flux passes, one of each per stage.
