@@ -200,6 +200,65 @@ def ensemble_run(args, ctx=None):
     return out
 
 
+# The reference's published step benchmark (benchmark/step.jl; docs/src/gpu_roadmap.md:202-216, Float64, one RK3
+# step of linear advection with MUSCL3 on a uniform periodic cloud): milliseconds per step on an RX 9070 XT (gfx1201)
+# and on an i5-13600K limited to 8 cores.
+REFERENCE_STEP = {   # (n per axis, equations, stab): (reference GPU ms, reference CPU ms)
+    (1000, 1, "both"): (29.0, 387.0),
+    (500, 1, "both"): (8.7, 98.0),
+    (500, 1, "none"): (5.3, 71.0),
+    (500, 1, "limiter"): (7.1, 87.0),
+    (500, 1, "mood"): (7.3, 82.0),
+}
+
+
+def reference_step_run(ctx, steps=30, warmup=3, only=None):
+    """benchmark/step.jl on this engine: for each case the grid, the stepper, dt = 0.3 get_time_step at t = 0 and
+    then warm-up and timed steps ts(eq, pg, k dt, dt) with that fixed dt (hcc_step), wall clock over the timed steps;
+    the checksum is the reference's sum of rho_1 over the particles."""
+    import hcc_hip
+    from hcc_hip import Grid, Stepper
+    from hcc_hip import workloads as W
+    sync = lambda: hcc_hip.lib().hcc_ctx_synchronize(ctx.h)   # noqa: E731
+    cases = []
+    for (n, M, stab), (ref_gpu, ref_cpu) in REFERENCE_STEP.items():
+        if only is not None and (n, M, stab) != only:
+            continue
+        wl = W.step_benchmark(n=n, M=M, stab=stab)
+        c = wl["cloud"]
+        g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+        st = Stepper(ctx, g, wl["scheme"])
+        st.upload(wl["U0"])
+        dt = wl["dt_factor"] * st.time_step()
+        for k in range(warmup):
+            st.step((k + 1) * dt, dt)
+        sync()
+        st.totals(reset=True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            st.step((warmup + k + 1) * dt, dt)
+        sync()
+        wall = time.perf_counter() - t0
+        n_steps, iters, tot_ms, tot_g, tot_f = st.totals(reset=True)
+        U = st.download()
+        ms = 1e3 * wall / steps
+        cases.append({"case": f"2-D, {stab}, {M} equation{'s' if M > 1 else ''}", "particles": g.N, "stab": stab,
+                      "ms_per_step": round(ms, 4), "reference_gpu_ms": ref_gpu, "reference_cpu_ms": ref_cpu,
+                      "speedup_vs_reference_gpu": round(ref_gpu / ms, 2),
+                      "node_stage_updates_per_s": g.N * 3 * steps / wall,
+                      "gather_avg_ms": round(tot_g / (3 * n_steps), 4), "flux_avg_ms": round(tot_f / (3 * n_steps), 4),
+                      "mood_iterations": iters, "checksum": float(np.sum(U[:, 0]))})
+        st.close()
+        g.close()
+    return {"workload": "the reference's own benchmark (benchmark/step.jl): linear advection, MUSCL3 + RusanovFlux, "
+                        "SSP-RK3, uniform periodic n^2 lattice of the unit square, R = 2.5 dx, Gaussian state; stab "
+                        "both = VenkatakrishnanLimiter(:hard) + MOOD(EPD1, MOODu1(1e-6)); dt = 0.3 get_time_step, "
+                        f"fixed; {warmup} warm-up and {steps} timed steps",
+            "reference_hardware": "RX 9070 XT (gfx1201, ROCm 7.2) Float64 GPU, i5-13600K limited to 8 cores "
+                                  "(docs/src/gpu_roadmap.md, 2026-10-15)",
+            "cases": cases}
+
+
 def graph_compare(args):
     """hcc_step_cfl uncaptured against hipGraph-captured steps (hcc_stepper_set_graph) on the four-quadrant problem at
     each h = 1/n: the same 5 warm-up and `steps` timed CFL steps from t = 0, and a bit-for-bit comparison of the final
@@ -403,6 +462,11 @@ def main():
     ap.add_argument("--ensemble-warmup", type=int, default=2)
     ap.add_argument("--graph-form", action="store_true",
                     help="the ensemble block also times the batched form as hipGraph-captured steps")
+    ap.add_argument("--reference-step", action="store_true",
+                    help="only the reference's own step benchmark (benchmark/step.jl, MUSCL3 advection)")
+    ap.add_argument("--no-reference-step", action="store_true", help="leave out the reference step-benchmark block")
+    ap.add_argument("--reference-case", default=None, metavar="N,M,STAB",
+                    help="with --reference-step: only this case (e.g. 1000,1,both)")
     ap.add_argument("--graph-compare", type=int, nargs="+", default=None, metavar="N",
                     help="only compare uncaptured and hipGraph-captured steps at h = 1/N for each N")
     args = ap.parse_args()
@@ -415,6 +479,14 @@ def main():
         sys.exit(2)
     if args.graph_compare:
         graph_compare(args)
+        return
+    if args.reference_step:
+        from hcc_hip import Context
+        only = None
+        if args.reference_case:
+            n_, m_, stab_ = args.reference_case.split(",")
+            only = (int(n_), int(m_), stab_)
+        print(json.dumps({"reference_step_benchmark": reference_step_run(Context(0), only=only)}), flush=True)
         return
     if args.ensemble:
         args.indicator = args.indicator or args.ens_indicator
@@ -499,7 +571,7 @@ def main():
     model_b = W.bytes_per_node_stage(D, M, nnb)
 
     if args.timed_only:
-        args.no_ensemble = args.no_cpu_baseline = True
+        args.no_ensemble = args.no_cpu_baseline = args.no_reference_step = True
         args.run_steps = 0
     # the whole configured run from t = 0 (run_steps CFL steps), timed as one region
     st.upload(wl["U0"])
@@ -577,6 +649,8 @@ def main():
         ea = argparse.Namespace(**vars(args))
         ea.indicator = args.ens_indicator
         out["ensemble"] = ensemble_run(ea, ctx)
+    if not args.no_reference_step:
+        out["reference_step_benchmark"] = reference_step_run(ctx)
     if not args.no_cpu_baseline:
         # the oracle on the host cores of this process: all of them available to it, at most the 16 that a one-GPU
         # share of the box holds

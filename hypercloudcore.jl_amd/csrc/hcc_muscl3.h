@@ -114,6 +114,80 @@ __device__ __forceinline__ void chol_solve(const double (&U)[B][B], const double
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// neighbour sources: records (and gradient records, orders) through the caches, or the workgroup's halo staged in
+// LDS by stage_halo3 (the order-2 layout: [position, state | pad | gradient record | pad], orders as bytes after the
+// records, then the block's local-index table)
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M>
+struct HbmSrc3 {
+    using L = L3<D, M>;
+    const double* Y;
+    const double* G;
+    const int8_t* ord;
+    const int32_t* nbr;
+    __device__ __forceinline__ int32_t idx(int64_t s) const { return nbr[s]; }
+    template <int K>
+    __device__ __forceinline__ void head(int32_t j, double (&r)[K]) const { load_rec<K, L::PU>(Y, j, r); }
+    __device__ __forceinline__ void grad(int32_t j, double (&g)[L::NG]) const { load_rec<L::NG, L::GS>(G, j, g); }
+    __device__ __forceinline__ int order(int32_t j) const { return ord[j]; }
+};
+template <int D, int M, bool GRADS>
+struct LdsSrc3 {
+    using L = L3<D, M>;
+    static constexpr int GO = lds_grad_offset(L::NR);
+    static constexpr int ST = GRADS ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0);
+    const double* sh;
+    const int8_t* ord;
+    const uint16_t* nl;
+    int64_t s0;
+    __device__ __forceinline__ int32_t idx(int64_t s) const { return nl[s - s0]; }
+    template <int K>
+    __device__ __forceinline__ void head(int32_t l, double (&r)[K]) const { lds_read<K>(sh + l * ST, r); }
+    __device__ __forceinline__ void grad(int32_t l, double (&g)[L::NG]) const { lds_read<L::NG>(sh + l * ST + GO, g); }
+    __device__ __forceinline__ int order(int32_t l) const { return ord[l]; }
+};
+template <int D, int M>
+__device__ __forceinline__ HbmSrc3<D, M> hbm_src(const StepArgs& a, const double* Y) {
+    return HbmSrc3<D, M>{Y, a.G, a.orders, a.g.nbr};
+}
+
+// Bytes of LDS the staged passes need for halos of up to hm nodes and index tables of up to idx_bytes
+template <int D, int M>
+__host__ __device__ constexpr size_t lds_bytes3(int64_t hm, size_t idx_bytes, bool grads) {
+    using L = L3<D, M>;
+    return 2 * lds_index_offset(hm, grads ? lds_stride(L::NR, L::NG) : lds_stride(L::NR, 0), grads) + idx_bytes;
+}
+
+// Copies the halo of block lb (records; with GRADS gradient records and orders) and its index table into LDS
+template <int D, int M, bool GRADS>
+__device__ __forceinline__ LdsSrc3<D, M, GRADS> stage_halo3(const StepArgs& a, const double* __restrict__ Y,
+                                                            double* sh, int64_t lb) {
+    using L = L3<D, M>;
+    using S = LdsSrc3<D, M, GRADS>;
+    const int32_t h0 = a.g.halo_off[lb], nh = a.g.halo_off[lb + 1] - h0;
+    int8_t* ord = reinterpret_cast<int8_t*>(sh + (size_t)a.g.hmax * S::ST);
+    uint16_t* nl = reinterpret_cast<uint16_t*>(sh) + lds_index_offset(a.g.hmax, S::ST, GRADS);
+    const int64_t s0 = stage_index_table(a.g, nl, lb);
+    for (int t = threadIdx.x; t < nh; t += kBlock) {
+        const int32_t j = a.g.halo_idx[h0 + t];
+        double r[L::NR];
+        load_rec<L::NR, L::PU>(Y, j, r);
+        double* dst = sh + t * S::ST;
+        if constexpr (GRADS) {
+            double g[L::NG];
+            load_rec<L::NG, L::GS>(a.G, j, g);
+            ord[t] = a.orders[j];
+#pragma unroll
+            for (int q = 0; q < L::NG; ++q) dst[S::GO + q] = g[q];
+        }
+#pragma unroll
+        for (int q = 0; q < L::NR; ++q) dst[q] = r[q];
+    }
+    __syncthreads();
+    return S{sh, ord, nl, s0};
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // geometry caches (prepare_geometry!, MUSCL.jl:172-223): the degree-2 reconstruction inverse, the divergence inverse
 // of the degree each stencil carries at full order (_FullOrderDivDegree) padded to BD x BD, its derivative rows, and
 // (dw set) the per-slot derivative weights rows(inv) (w p~) invL with the full-degree intercept basis
@@ -208,8 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_moments3(MomentArgs a, int64_t nblk)
 // _limit_slopes (Limiter.jl:120-143) with the degree-2 interface basis; EXT stores the neighbourhood extrema for
 // MOODu1's checks
 // ------------------------------------------------------------------------------------------------------------
-template <int D, int M, int KIND, bool EXT>
-__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+template <int D, int M, int KIND, bool EXT, class Src>
+__device__ __forceinline__ void gather_node(const StepArgs& a, const double* __restrict__ Y, const Src& src, int64_t i) {
     using L = L3<D, M>;
     constexpr int NR = L::NR, BG = L::BG;
     const int64_t N = a.g.N;
@@ -243,7 +317,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         for (int c = 0; c < D; ++c) A[r][c] = 0.0;
     for (int k = 0; k < cnt; ++k) {
         double rj[NR];
-        load_rec<NR, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], rj);
+        src.template head<NR>(src.idx(base + (int64_t)k * kChunk), rj);
         if (cached || direct) {
             double dd[D], ds[D];
             const double d2 = rec_geom<D>(a.g, ri, rj, dd);
@@ -335,7 +409,7 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
         }
         for (int k = 0; k < cnt; ++k) {
             double pj[D], dd[D], h[D], pint[BG];
-            load_rec<D, L::PU>(Y, a.g.nbr[base + (int64_t)k * kChunk], pj);
+            src.template head<D>(src.idx(base + (int64_t)k * kChunk), pj);
             rec_geom<D>(a.g, ri, pj, dd);
 #pragma unroll
             for (int d = 0; d < D; ++d) h[d] = 0.5 * dd[d];
@@ -378,9 +452,23 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a, const double* __r
     const int64_t i = node_index(nblk);
     if (reset) step_reset3(a, i);
     if (i < a.g.N && !a.g.isb[i] && (full || a.changed[i])) {
-        if (full) gather_node<D, M, KIND, true>(a, Y, i);
-        else gather_node<D, M, KIND, false>(a, Y, i);
+        if (full) gather_node<D, M, KIND, true>(a, Y, hbm_src<D, M>(a, Y), i);
+        else gather_node<D, M, KIND, false>(a, Y, hbm_src<D, M>(a, Y), i);
     }
+}
+
+// the full gather with the block's halo staged in LDS
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const double* __restrict__ Y, int reset,
+                                                           int64_t nblk) {
+    extern __shared__ double sh[];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }
+    const int64_t lb = logical_block(nblk);
+    if (lb >= nblk) return;
+    const int64_t i = lb * kBlock + threadIdx.x;
+    if (reset) step_reset3(a, i);
+    const auto src = stage_halo3<D, M, false>(a, Y, sh, lb);
+    if (i < a.g.N && !a.g.isb[i]) gather_node<D, M, KIND, true>(a, Y, src, i);
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -438,9 +526,9 @@ __device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + 
     }
 }
 
-template <int D, int M, int KIND, bool CHECK>
+template <int D, int M, int KIND, bool CHECK, class Src>
 __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
-                                          double* __restrict__ K, int64_t i, double h) {
+                                          const Src& src, double* __restrict__ K, int64_t i, double h) {
     using L = L3<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = L::BD, B1 = L::B1, DM = D * M;
     const int64_t N = a.g.N;
@@ -474,11 +562,12 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             for (int q = 0; q < DM; ++q) b[c][q] = 0.0;
         if (okd) {
             for (int k = 0; k < cnt; ++k) {
-                const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+                const int32_t j = src.idx(base + (int64_t)k * kChunk);
                 double rj[NR], gj[NG], dd[D], w, dF[D][M];
-                load_rec<NR, L::PU>(Y, j, rj);
-                load_rec<NG, L::GS>(a.G, j, gj);
-                slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (a.orders[j] & 15) : a.orders[j], dd, w, dF);
+                src.template head<NR>(j, rj);
+                src.grad(j, gj);
+                const int oj = src.order(j);
+                slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
                 double ds[D], t[BD - 1];
 #pragma unroll
                 for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
@@ -520,11 +609,12 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
 #pragma unroll
             for (int q = 0; q < DM; ++q) b[c][q] = 0.0;
         for (int k = 0; k < cnt; ++k) {
-            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            const int32_t j = src.idx(base + (int64_t)k * kChunk);
             double rj[NR], gj[NG], dd[D], w, dF[D][M];
-            load_rec<NR, L::PU>(Y, j, rj);
-            load_rec<NG, L::GS>(a.G, j, gj);
-            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (a.orders[j] & 15) : a.orders[j], dd, w, dF);
+            src.template head<NR>(j, rj);
+            src.grad(j, gj);
+            const int oj = src.order(j);
+            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
             double p[B1];
             p[0] = 1.0;
 #pragma unroll
@@ -582,7 +672,34 @@ __global__ __launch_bounds__(kBlock) void k_flux(StepArgs a, PredArgs pred, cons
 #pragma unroll
                 for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
         } else if (full || a.dirty[i]) {
-            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, K, i, h);
+            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, hbm_src<D, M>(a, Y), K, i, h);
+            if (CHECK) bad = a.inadm[i];
+        }
+    }
+    if (CHECK) {
+        append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, bad);
+    }
+}
+
+// the full flux pass with the block's halo (records, gradient records, orders) staged in LDS
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                         double* __restrict__ K, int64_t nblk) {
+    extern __shared__ double sh[];
+    const int64_t lb = logical_block(nblk);
+    if (lb >= nblk) return;
+    const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, lb * kBlock) : 0.0;
+    const auto src = stage_halo3<D, M, true>(a, Y, sh, lb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    bool ch = false;
+    int bad = 0;
+    if (i < a.g.N) {
+        if (a.g.isb[i]) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+        } else {
+            ch = flux_node<D, M, KIND, CHECK>(a, pred, Y, src, K, i, h);
             if (CHECK) bad = a.inadm[i];
         }
     }
@@ -651,7 +768,7 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             nl += __popcll(ball);
         }
         flush();
-        if (cnt >= 0 && !a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, i);
+        if (cnt >= 0 && !a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, hbm_src<D, M>(a, Y), i);
     }
 }
 
@@ -671,10 +788,11 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
             if (!a.g.isb[i]) {
                 if constexpr (FP) {
                     const int before = a.inadm[i];
-                    ch = flux_node<D, M, KIND, true>(a, pred, Y, K, i, a.hratio * node_dt(a.dtp, a.inst, i));
+                    ch = flux_node<D, M, KIND, true>(a, pred, Y, hbm_src<D, M>(a, Y), K, i,
+                                                     a.hratio * node_dt(a.dtp, a.inst, i));
                     delta = (int)a.inadm[i] - before;
                 } else {
-                    flux_node<D, M, KIND, false>(a, pred, Y, K, i, 0.0);
+                    flux_node<D, M, KIND, false>(a, pred, Y, hbm_src<D, M>(a, Y), K, i, 0.0);
                 }
             }
         }

@@ -2647,7 +2647,9 @@ struct Variant3 : Variant<D, M, KIND> {
     }
     static void gather(const hcc_stepper* s, const StepArgs& a, const double* Y, int full, int reset) {
         const int64_t nb = nblocks(s->N);
-        m3::k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
+        if (full && s->lds_gather && s->tune.staged)
+            m3::k_gather_staged<D, M, KIND><<<grid_for(nb), kBlock, s->lds_gather, st(s)>>>(a, Y, reset, nb);
+        else m3::k_gather<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Y, full, reset, nb);
     }
     static void cfl(const hcc_stepper* s, const StepArgs& a, double* dtp, double cfl, double cap) {
         const int64_t nb = nblocks(s->N);
@@ -2671,6 +2673,9 @@ struct Variant3 : Variant<D, M, KIND> {
             k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
             k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
             m3::k_flux_list<D, M, KIND, false><<<list_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
+        } else if (full && s->lds_flux && s->tune.staged) {
+            if (check) m3::k_flux_staged<D, M, KIND, true><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
+            else m3::k_flux_staged<D, M, KIND, false><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, p, Y, K, nb);
         } else if (check) {
             m3::k_flux<D, M, KIND, true><<<grid_for(nb), kBlock, 0, st(s)>>>(a, p, Y, K, full, nb);
         } else {
@@ -2687,6 +2692,19 @@ static VariantOps make_ops3() {
     return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex,
                       &V::range, &V::cfl, &V::predict, &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack,
                       Layout<D, M>::PU, m3::L3<D, M>::GS};
+}
+// LDS bytes of the staged MUSCL3 gather and flux passes (0: the variant is not compiled)
+static void lds_sizes3(int D, int M, int kind, int64_t hm, size_t idx_bytes, size_t& gather, size_t& flux) {
+    gather = flux = 0;
+    auto set = [&](size_t g, size_t f) { gather = g; flux = f; };
+    if (kind == HCC_PDE_LINADV && D == 1 && M == 1)
+        set(m3::lds_bytes3<1, 1>(hm, idx_bytes, false), m3::lds_bytes3<1, 1>(hm, idx_bytes, true));
+    else if (kind == HCC_PDE_LINADV && D == 2 && M == 1)
+        set(m3::lds_bytes3<2, 1>(hm, idx_bytes, false), m3::lds_bytes3<2, 1>(hm, idx_bytes, true));
+    else if (kind == HCC_PDE_LINADV && D == 2 && M == 2)
+        set(m3::lds_bytes3<2, 2>(hm, idx_bytes, false), m3::lds_bytes3<2, 2>(hm, idx_bytes, true));
+    else if (kind == HCC_PDE_EULER && D == 2 && M == 4)
+        set(m3::lds_bytes3<2, 4>(hm, idx_bytes, false), m3::lds_bytes3<2, 4>(hm, idx_bytes, true));
 }
 // MUSCL3 variants: advection (1, 1), (2, 1), (2, 2) and 2-D Euler
 static bool find_ops3(int D, int M, int kind, VariantOps& ops) {
@@ -2872,6 +2890,14 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->G, 0, sizeof(double) * std::max<int64_t>(s->GS * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->scratch, 0, sizeof(double) * std::max<int64_t>(M * N, 1), st));
     HCC_TRY_HIP(ctx, hipMemsetAsync(s->counters, 0, 10 * sizeof(unsigned long long), st));
+    if (g->max_halo > 0 && s->order == 3) {   // the staged MUSCL3 gather and flux passes where the halos fit LDS
+        int lds_max = 0;
+        HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        size_t lg = 0, lf = 0;
+        lds_sizes3(D, (int)M, d->pde, g->max_halo, (size_t)g->max_block_slots * sizeof(uint16_t), lg, lf);
+        s->lds_gather = lg <= (size_t)lds_max ? lg : 0;
+        s->lds_flux = lf <= (size_t)lds_max ? lf : 0;
+    }
     if (g->max_halo > 0 && !s->moving && s->order == 2) {   // LDS staging of the full passes where the block halos fit a workgroup's LDS
         int lds_max = 0;
         HCC_TRY_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
@@ -2884,7 +2910,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         s->lds_cfl = fit(2 * lds_index_offset((int64_t)hm, (2 * D + 1) & ~1, false) + idx_bytes);
         s->lds_gcfl = fit(2 * lds_index_offset((int64_t)hm, lds_stride(NR, 0) + ((D + 1) & ~1), false) + idx_bytes);
     }
-    if (s->lds_flux && s->fv == 0 && !s->moving && N > 0) {   // flat blocks of the staged flux pass (k_block_flat)
+    if (s->lds_flux && s->fv == 0 && !s->moving && s->order == 2 && N > 0) {   // flat blocks of the staged flux pass (k_block_flat)
         HCC_TRY_HIP(ctx, dalloc(&s->cnb, N));
         HCC_TRY_HIP(ctx, dalloc(&s->bflat, nblocks(N)));
         HCC_TRY_HIP(ctx, hipMemset(s->cnb, 0, N));
