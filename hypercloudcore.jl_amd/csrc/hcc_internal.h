@@ -101,6 +101,8 @@ struct StepArgs {
     int limiter, mode, strategy, criterion;
     double delta;
     double plateau;              // MOODu2: a component whose local range is below plateau x range[m] is not checked
+    double ratio;                // MOODu2: ratio of the nearest curvature magnitudes (its override at MUSCL order 3)
+    double hmin;                 // minimum(pg.meta.dx) (the curvature override's spacing h)
     const double* range;         // MOODu2: global range of the previous stage per component (nullptr: plateau off)
     int streamed;                // divergence accumulated per slot with the derivative weights (flux variant 1)
     int aligned;                 // AlignedRusanovFlux: sort by the weights' signs below the full order (variant 1)

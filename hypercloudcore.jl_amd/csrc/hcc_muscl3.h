@@ -806,6 +806,133 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// the unfused check (_mood_flag! + _mood_drop!, MOOD.jl:289-309) with MOODu2's smooth-extremum override
+// (_u2_fails, MOOD.jl:223-271): a component that violates the DMP (and is not a plateau) is accepted when, at
+// order 3, every axis's curvature term (gradient record entry D + d) over the stencil has one sign
+// ((c_min h^2)(c_max h^2) > -d^2) and comparable magnitudes among the neighbours within 1.5 h
+// (|c|_min >= ratio |c|_max, ratio <= 0: off), or is negligible (max(|c_min|, |c_max|) h^2 < d); h = minimum(dx).
+// The gradient records are those of the last gather of the checked stage's state (the full pass, then the
+// recomputed particles), as the reference's cached gradients.
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M, int KIND>
+__device__ __forceinline__ bool curvature_fails(const StepArgs& a, const double* __restrict__ Yp, int64_t i,
+                                                const bool (&viol)[M]) {
+    using L = L3<D, M>;
+    if (a.orders[i] < 3) return true;
+    const double h = a.hmin, h2 = h * h, nr = 1.5 * h, near_sq = nr * nr, dl = a.delta;
+    const int cnt = a.g.cnt[i];
+    const int64_t base = sell_base(a.g, i);
+    double pi[D], gi[L::NG];
+    load_rec<D, L::PU>(Yp, i, pi);
+    load_rec<L::NG, L::GS>(a.G, i, gi);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double mini[M], maxi[M], mina[M], maxa[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            mini[m] = gi[(D + d) * M + m];
+            maxi[m] = mini[m];
+            mina[m] = fabs(mini[m]);
+            maxa[m] = mina[m];
+        }
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double pj[D], dd[D], gj[L::NG];
+            load_rec<D, L::PU>(Yp, j, pj);
+            const bool near = rec_geom<D>(a.g, pi, pj, dd) <= near_sq;
+            load_rec<L::NG, L::GS>(a.G, j, gj);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double c = gj[(D + d) * M + m];
+                mini[m] = jmin(mini[m], c);
+                maxi[m] = jmax(maxi[m], c);
+                if (near) { mina[m] = jmin(mina[m], fabs(c)); maxa[m] = jmax(maxa[m], fabs(c)); }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (!viol[m]) continue;
+            if (jmax(fabs(mini[m]), fabs(maxi[m])) * h2 < dl) continue;
+            const bool same_sign = (mini[m] * h2) * (maxi[m] * h2) > -(dl * dl);
+            const bool comparable = a.ratio <= 0.0 || mina[m] >= a.ratio * maxa[m];
+            if (!(same_sign && comparable)) return true;
+        }
+    }
+    return false;
+}
+
+template <int D, int M, int KIND>
+__global__ __launch_bounds__(kBlock) void k_mood_flag(StepArgs a, const double* __restrict__ Yp,
+                                                       const double* __restrict__ cand, const double* __restrict__ Kp,
+                                                       const double* __restrict__ KIp, int full, int64_t nblk) {
+    using P = Pde<D, M, KIND>;
+    using L = L3<D, M>;
+    const int64_t i = node_index(nblk);
+    const int64_t N = a.g.N;
+    bool ch = false;
+    if (i < N && !a.g.isb[i] && (full || a.dirty[i])) {
+        double r[L::NR], W[M];
+        if (Kp) {
+            const double h = a.hratio * node_dt(a.dtp, a.inst, i);
+            load_rec<L::NR, L::PU>(Yp, i, r);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                W[m] = KIp ? r[D + m] + h * (Kp[i * M + m] + KIp[i * M + m]) : r[D + m] + h * Kp[i * M + m];
+        } else {
+            load_rec<L::NR, L::PU>(cand, i, r);
+#pragma unroll
+            for (int m = 0; m < M; ++m) W[m] = r[D + m];
+        }
+        const bool adm = P::admissible(a.pde, W);
+        a.inadm[i] = !adm;
+        bool flag = !adm;
+        if (!flag) {
+            if (a.criterion == HCC_CRIT_ONLY) {
+                flag = true;
+            } else if (a.criterion == HCC_CRIT_U1 || a.criterion == HCC_CRIT_U2) {
+                double mn[M], mx[M];
+                if (a.ext_on) {   // the extrema the stage's full gather stored
+                    double e[2 * M];
+                    load_rec<2 * M, 2 * M>(a.ext, i, e);
+#pragma unroll
+                    for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+                } else {
+                    load_rec<L::NR, L::PU>(Yp, i, r);
+#pragma unroll
+                    for (int m = 0; m < M; ++m) { mn[m] = r[D + m]; mx[m] = r[D + m]; }
+                    const int cnt = a.g.cnt[i];
+                    const int64_t base = sell_base(a.g, i);
+                    for (int k = 0; k < cnt; ++k) {
+                        load_rec<L::NR, L::PU>(Yp, a.g.nbr[base + (int64_t)k * kChunk], r);
+#pragma unroll
+                        for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], r[D + m]); mx[m] = jmax(mx[m], r[D + m]); }
+                    }
+                }
+                const double dl = a.delta, d3 = dl * dl * dl;
+                const bool u2 = a.criterion == HCC_CRIT_U2;
+                bool viol[M], any = false;
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const bool v = fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl);
+                    const bool plateau = u2 && a.range && mx[m] - mn[m] < a.plateau * a.range[m];
+                    viol[m] = v && !plateau;
+                    any = any || viol[m];
+                }
+                flag = any && (!u2 || curvature_fails<D, M, KIND>(a, Yp, i, viol));
+            }
+        }
+        if (flag) {
+            const int o = a.orders[i];
+            if (o > 1) { a.orders[i] = (int8_t)(o - 1); a.trig[i] = 1; ch = true; }
+        }
+    }
+    if (i < N) a.changed[i] = ch;
+    const unsigned long long ball = __ballot(ch);
+    if (ball && ((threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1)))
+        atomicAdd(&a.counters[0], (unsigned long long)__popcll(ball));
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // CFL estimate with the cached weights (_cfl_weights_body!, ParticleGrids.jl:439-450): the per-slot derivative
 // weights of the fit in use recomputed from its derivative rows with the operations of _weights_body!
 // (MUSCL.jl:211-223, the full-degree intercept basis); a.lam holds the wave speeds (k_lambda)

@@ -2427,6 +2427,12 @@ StepArgs hcc_stepper::args() const {
     a.limiter = desc.limiter; a.mode = desc.limiter_mode; a.strategy = desc.mood_strategy;
     a.criterion = desc.mood_criterion; a.delta = desc.mood_delta;
     a.plateau = desc.mood_plateau;
+    a.ratio = desc.mood_ratio;
+    {
+        double h = grid->geom.desc.dx[0];
+        for (int d = 1; d < D; ++d) h = std::min(h, grid->geom.desc.dx[d]);
+        a.hmin = h;
+    }
     a.streamed = desc.muscl_streamed ? 1 : 0;
     a.src_mode = src_mode;
     a.src_rate = rate;
@@ -2685,6 +2691,12 @@ struct Variant3 : Variant<D, M, KIND> {
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
         m3::k_flux_list<D, M, KIND, true><<<list_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
     }
+    // the unfused check with MOODu2's curvature override
+    static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
+                     const double* KIp, int full) {
+        const int64_t nb = nblocks(s->N);
+        m3::k_mood_flag<D, M, KIND><<<grid_for(nb), kBlock, 0, st(s)>>>(a, Yp, cand, Kp, KIp, full, nb);
+    }
 };
 template <int D, int M, int KIND>
 static VariantOps make_ops3() {
@@ -2795,8 +2807,6 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         if (d->muscl_streamed || d->flux != HCC_FLUX_RUSANOV)
             return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL3 runs the buffered divergence with RusanovFlux");
         if (d->mover != HCC_MOVER_NONE) return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MUSCL3 on a moving grid");
-        if (d->mood_criterion == HCC_CRIT_U2)
-            return set_err(ctx, HCC_E_UNSUPPORTED, "hcc_stepper_create: MOODu2's curvature override with MUSCL3");
     }
     if (d->stages < 1 || d->stages > kMaxStages) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: 1..5 stages");
     if (d->limiter < HCC_LIM_NONE || d->limiter > HCC_LIM_VK) return set_err(ctx, HCC_E_ARG, "hcc_stepper_create: unknown limiter");

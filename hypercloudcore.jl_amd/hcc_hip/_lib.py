@@ -55,7 +55,8 @@ class StepperDesc(C.Structure):
                 ("substep_ratio", C.c_double), ("n_bc", C.c_int32), ("bc_tag", C.c_int32 * 16),
                 ("bc_kind", C.c_int32 * 16), ("mood_plateau", C.c_double), ("muscl_streamed", C.c_int32),
                 ("flux", C.c_int32), ("a_im", (C.c_double * 5) * 5), ("b_im", C.c_double * 5),
-                ("c_im", C.c_double * 5), ("mover", C.c_int32), ("mover_alpha", C.c_double)]
+                ("c_im", C.c_double * 5), ("mover", C.c_int32), ("mover_alpha", C.c_double),
+                ("mood_ratio", C.c_double)]
 
 
 _lib = None
@@ -415,6 +416,7 @@ def stepper_desc(scheme):
     sd.flux = scheme.get("flux", HCC_FLUX_RUSANOV)
     sd.mover = scheme.get("mover", HCC_MOVER_NONE)
     sd.mover_alpha = scheme.get("mover_alpha", 1.0)
+    sd.mood_ratio = scheme.get("ratio", 0.5)
     a_t, b_t, c_t = scheme["a_t"], scheme["b_t"], scheme["c_t"]
     s = len(b_t)
     sd.stages = s

@@ -102,10 +102,11 @@ def tableau(name):
 
 def scheme(M, pde, *, gamma=1.4, adv_speed=None, limiter=L.HCC_LIM_VK, mode=L.HCC_MODE_HARD, strategy=L.HCC_STRAT_EPD1,
            criterion=L.HCC_CRIT_U1, delta=0.0, plateau=0.0, tableau_name="RK3_SSP", bcs=(), mood_substeps=True,
-           streamed=False, flux=L.HCC_FLUX_RUSANOV, mover=L.HCC_MOVER_NONE, mover_alpha=1.0, order=2):
+           streamed=False, flux=L.HCC_FLUX_RUSANOV, mover=L.HCC_MOVER_NONE, mover_alpha=1.0, order=2, ratio=0.5):
     tb = tableau(tableau_name)
     return {"M": M, "pde": pde, "gamma": gamma, "adv_speed": adv_speed or [], "muscl_order": order, "limiter": limiter,
             "limiter_mode": mode, "strategy": strategy, "criterion": criterion, "delta": delta, "plateau": plateau,
+            "ratio": ratio,
             "streamed": streamed, "flux": flux, "a_t": tb["a_t"], "a_im": tb.get("a_im"), "b_im": tb.get("b_im"),
             "c_im": tb.get("c_im"), "mover": mover, "mover_alpha": mover_alpha,
             "b_t": tb["b_t"], "c_t": tb["c_t"], "substep_ratio": tb["substep_ratio"] if mood_substeps else 0.0,

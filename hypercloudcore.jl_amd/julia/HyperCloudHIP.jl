@@ -75,6 +75,7 @@ struct StepperDesc
     c_im::NTuple{5, Float64}
     mover::Int32             # 0 NoGridMover, 1 LagrangianMover, 2 ScaledLagrangianMover (hip_mover_code)
     mover_alpha::Float64
+    mood_ratio::Float64      # MOODu2's ratio (its curvature override at MUSCL order 3)
 end
 
 "`(mover, mover_alpha)` of the descriptor for `pg.mover`; PrescribedMover (a closure) has no compiled kernel."
@@ -173,13 +174,13 @@ hip_strategy(::HC.EPD1) = Int32(2)
 hip_strategy(::HC.EPD2) = Int32(3)
 hip_strategy(::HC.StrictEPD0) = Int32(4)
 
-"`(criterion, delta, plateau)` of a MOOD criterion (MOODu2's `ratio` is unused at MUSCL order 2)."
+"`(criterion, delta, plateau, ratio)` of a MOOD criterion (MOODu2's `ratio` acts at MUSCL order 3)."
 hip_criterion(c) = error("no compiled MOOD criterion for $(typeof(c))")
-hip_criterion(::HC.NoMOOD) = (Int32(0), 0.0, 0.0)
-hip_criterion(c::HC.MOODu1) = (Int32(1), Float64(c.d), 0.0)
-hip_criterion(::HC.MOODPAD) = (Int32(2), 0.0, 0.0)
-hip_criterion(::HC.OnlyMOOD) = (Int32(3), 0.0, 0.0)
-hip_criterion(c::HC.MOODu2) = (Int32(4), Float64(c.d), Float64(c.plateau))
+hip_criterion(::HC.NoMOOD) = (Int32(0), 0.0, 0.0, 0.5)
+hip_criterion(c::HC.MOODu1) = (Int32(1), Float64(c.d), 0.0, 0.5)
+hip_criterion(::HC.MOODPAD) = (Int32(2), 0.0, 0.0, 0.5)
+hip_criterion(::HC.OnlyMOOD) = (Int32(3), 0.0, 0.0, 0.5)
+hip_criterion(c::HC.MOODu2) = (Int32(4), Float64(c.d), Float64(c.plateau), Float64(c.ratio))
 
 hip_bc(bc) = error("no compiled boundary condition for $(typeof(bc)) (compiled: OutflowBC)")
 hip_bc(::HC.OutflowBC) = HCC_BC_OUTFLOW
@@ -216,7 +217,7 @@ function hip_stepper_desc(eq::HC.HyperbolicPDE{D, M}, pg, interp::HC.MUSCL, mood
                           substep_ratio::Real) where {D, M}
     kind, gamma, speeds = hip_descriptor(eq)
     lim, mode = hip_limiter(interp.limiter)
-    crit, delta, plateau = hip_criterion(mood.criterion)
+    crit, delta, plateau, ratio = hip_criterion(mood.criterion)
     s = length(tableau.b_t)
     s <= 5 || error("tableaus of up to 5 stages are compiled")
     bcm = pg.geometry.bc_map
@@ -228,7 +229,7 @@ function hip_stepper_desc(eq::HC.HyperbolicPDE{D, M}, pg, interp::HC.MUSCL, mood
                 crit, delta, Int32(s), _mat5(tableau.a_t), _vec5(tableau.b_t), _vec5(tableau.c_t), Float64(substep_ratio),
                 Int32(length(tags)), _pad(Int32.(tags), 16, Int32(0)), _pad([hip_bc(bcm[t]) for t in tags], 16, Int32(0)),
                 plateau, _streamed(interp), hip_flux(interp.flux), _mat5(tableau.a), _vec5(tableau.b), _vec5(tableau.c),
-                mover, malpha)
+                mover, malpha, ratio)
 end
 
 """

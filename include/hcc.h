@@ -118,7 +118,7 @@ typedef struct {
     double gamma;              /* Euler: ratio of specific heats */
     double adv_speed[3][8];    /* linear advection: speed of component m along axis d (diagonal system) */
     int32_t muscl_order;       /* MUSCL max_order: 2, or 3 (advection (1,1), (2,1), (2,2), 2-D Euler; buffered
-                                  RusanovFlux on static grids; MOODu2's curvature override not compiled) */
+                                  RusanovFlux on static grids) */
     int32_t limiter, limiter_mode;
     int32_t mood_strategy, mood_criterion;
     double mood_delta;         /* MOODu1 / MOODu2 d */
@@ -135,6 +135,8 @@ typedef struct {
     double a_im[5][5], b_im[5], c_im[5];
     int32_t mover;             /* HCC_MOVER_* (pg.mover of the ParticleGrid) */
     double mover_alpha;        /* ScaledLagrangianMover's alpha */
+    double mood_ratio;         /* MOODu2's ratio of the nearest curvature magnitudes (MOOD.jl:133, default 0.5; <= 0
+                                  off), used by its smooth-extremum override at MUSCL order 3 */
 } hcc_stepper_desc;
 
 /* Replaces GeneralRKTimeStepper(pde, MUSCL(...), MOOD(...), tableau) (TimeSteppers.jl:195-229) and the MUSCL

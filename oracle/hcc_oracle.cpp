@@ -599,6 +599,7 @@ struct Scheme {
     int strategy = STRAT_EPD1, criterion = CRIT_U1;
     double delta = 0.0;
     double plateau = 0.0;      // MOODu2 plateau fraction of the global range (0: off)
+    double ratio = 0.5;        // MOODu2 ratio of the nearest curvature magnitudes (<= 0: off)
     int streamed = 0;          // MUSCL(...; streamed = true): divergence accumulated per slot with dweights
     int aligned = 0;           // AlignedRusanovFlux: sort by sign(dweights) on pairs below the full order
     int s = 3;
@@ -643,6 +644,8 @@ struct SimBase {
     virtual void force_of(const uint8_t* trig, const int32_t* snap, int32_t* o, uint8_t* ch) = 0;
     virtual void expand_of(uint8_t* mask, int hops) = 0;
     virtual int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) = 0;
+    virtual int u2_flags3_of(const double* U, int64_t p, const double* cand, double delta, double plateau, double ratio,
+                             const double* g, const int32_t* o) = 0;
     virtual void volumes_of(int sweeps, double* out) = 0;
     virtual void set_source(int mode, const double* rates) = 0;
     virtual void get_stage(int which, int i, double* out) const = 0;
@@ -664,7 +667,7 @@ struct Sim : SimBase {
     using P = Phys<D, M>;
 
     Domain g;
-    double alpha = 1.0, inv_range_sq = 0.0, R_sq = 0.0, invL = 1.0;
+    double alpha = 1.0, inv_range_sq = 0.0, R_sq = 0.0, invL = 1.0, hmin = 1.0;
     std::vector<std::array<double, D>> pos;
     std::vector<uint8_t> isb;
     std::vector<int32_t> tags;
@@ -714,6 +717,7 @@ struct Sim : SimBase {
         double mindx = g.dx[0];
         for (int d = 1; d < D; ++d) mindx = std::min(mindx, g.dx[d]);
         invL = 1.0 / mindx;
+        hmin = mindx;
         search();
         prepare_geometry();
         update_volumes(1);   // the ParticleGrid constructor's ShepardVolumes() (ParticleGrids.jl:275)
@@ -1430,9 +1434,12 @@ struct Sim : SimBase {
         return false;
     }
 
-    // _u2_fails (MOOD.jl:215-270) for MUSCL order 2: the DMP test of MOODu1 on the components that are not a
-    // plateau (max - min below plateau x the global range, when the plateau test is on); a violating component
-    // flags the particle, since the curvature override needs consistency order >= 3
+    // _u2_fails (MOOD.jl:223-271): the DMP test of MOODu1 on the components that are not a plateau (max - min below
+    // plateau x the global range, when the plateau test is on); a violating component flags the particle unless the
+    // scheme has curvature coefficients (MUSCL order >= 3) and the particle is at order >= 3, in which case, per axis
+    // d, the curvature terms (gradient record entry D + d) over the stencil must have one sign ((c_min h^2)(c_max h^2)
+    // > -delta^2) and comparable magnitudes among the neighbours within 1.5 h (|c|_min >= ratio |c|_max), or be
+    // negligible (max |c| h^2 < delta); h = minimum(dx)
     bool u2_fails(int64_t p, const std::vector<State>& Up, const State& cand, const State* range) const {
         State mn = Up[p], mx = Up[p];
         for (int64_t k = rstart[p]; k < rstart[p + 1]; ++k) {
@@ -1440,10 +1447,35 @@ struct Sim : SimBase {
             for (int m = 0; m < M; ++m) { mn[m] = jmin(mn[m], fj[m]); mx[m] = jmax(mx[m], fj[m]); }
         }
         double d = sc.delta, d3 = d * d * d;
+        bool viol[M], any = false;
         for (int m = 0; m < M; ++m) {
             const bool violates = std::fabs(mx[m] - mn[m]) >= d3 && (cand[m] < mn[m] - d || cand[m] > mx[m] + d);
             const bool plateau = range && mx[m] - mn[m] < sc.plateau * (*range)[m];
-            if (violates && !plateau) return true;
+            viol[m] = violates && !plateau;
+            any = any || viol[m];
+        }
+        if (!any) return false;
+        if (O < 3 || orders[p] < 3) return true;
+        const double h2 = hmin * hmin, nr = 1.5 * hmin, near_sq = nr * nr;   // _U2_RATIO_RADIUS = 1.5
+        for (int dd = 0; dd < D; ++dd) {
+            const int ci = D + dd;
+            State mini = grads[p][ci], maxi = grads[p][ci], mina, maxa;
+            for (int m = 0; m < M; ++m) { mina[m] = std::fabs(mini[m]); maxa[m] = mina[m]; }
+            for (int64_t k = rstart[p]; k < rstart[p + 1]; ++k) {
+                const State& cj = grads[nbr[k]][ci];
+                const bool near = sumsq(dist[k]) <= near_sq;
+                for (int m = 0; m < M; ++m) {
+                    mini[m] = jmin(mini[m], cj[m]); maxi[m] = jmax(maxi[m], cj[m]);
+                    if (near) { mina[m] = jmin(mina[m], std::fabs(cj[m])); maxa[m] = jmax(maxa[m], std::fabs(cj[m])); }
+                }
+            }
+            for (int m = 0; m < M; ++m) {
+                if (!viol[m]) continue;
+                if (jmax(std::fabs(mini[m]), std::fabs(maxi[m])) * h2 < d) continue;
+                const bool same_sign = (mini[m] * h2) * (maxi[m] * h2) > -(d * d);
+                const bool comparable = sc.ratio <= 0.0 || mina[m] >= sc.ratio * maxa[m];
+                if (!(same_sign && comparable)) return true;
+            }
         }
         return false;
     }
@@ -1609,6 +1641,18 @@ struct Sim : SimBase {
         for (int h = 0; h < hops; ++h) expand_dirty();
         std::copy(dirty.begin(), dirty.end(), mask);
     }
+    // the MOODu2 check of particle p with given gradient records (N x BG x M) and orders
+    int u2_flags3_of(const double* U, int64_t p, const double* cand, double delta, double plateau, double ratio,
+                     const double* g, const int32_t* o) override {
+        for (int64_t i = 0; i < N; ++i)
+            for (int a = 0; a < BG; ++a) for (int m = 0; m < M; ++m) grads[i][a][m] = g[(i * BG + a) * M + m];
+        orders.assign(o, o + N);
+        const double kr = sc.ratio;
+        sc.ratio = ratio;
+        const int r = u2_flags_of(U, p, cand, delta, plateau);
+        sc.ratio = kr;
+        return r;
+    }
     int u2_flags_of(const double* U, int64_t p, const double* cand, double delta, double plateau) override {
         std::vector<State> Uv(N);
         for (int64_t i = 0; i < N; ++i) for (int m = 0; m < M; ++m) Uv[i][m] = U[i * M + m];
@@ -1739,7 +1783,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
                         int criterion, double delta, int stages, const double* a_t, const double* b_t, const double* c_t,
                         double substep_ratio, int n_bc, const int32_t* bc_tags, const int32_t* bc_kinds,
                         double plateau, int streamed, int aligned, const double* a_im, const double* b_im,
-                        const double* c_im, int mover, double mover_alpha) {
+                        const double* c_im, int mover, double mover_alpha, double ratio) {
     SimBase* sb = (SimBase*)s;
     PDE e; e.kind = pde_kind; e.gamma = gamma;
     if (pde_kind == PDE_LINADV) {
@@ -1751,6 +1795,7 @@ void orc_sim_set_scheme(void* s, int pde_kind, double gamma, const double* c, in
     }
     Scheme sc; sc.limiter = limiter; sc.mode = mode; sc.strategy = strategy; sc.criterion = criterion; sc.delta = delta;
     sc.plateau = plateau;
+    sc.ratio = ratio;
     sc.streamed = streamed; sc.aligned = aligned;
     sc.mover = mover; sc.mover_alpha = mover_alpha;
     if (a_im) for (int i = 0; i < stages; ++i) { sc.b_im[i] = b_im[i]; sc.c_im[i] = c_im[i]; for (int j = 0; j < stages; ++j) sc.a_im[i][j] = a_im[i * stages + j]; }
@@ -1791,6 +1836,10 @@ void orc_sim_set_source(void* s, int mode, const double* rates) { ((SimBase*)s)-
 void orc_sim_get_stage(void* s, int which, int i, double* out) { ((SimBase*)s)->get_stage(which, i, out); }
 void orc_sim_get_positions(void* s, double* out) { ((SimBase*)s)->get_positions(out); }
 void orc_sim_set_positions(void* s, const double* x, int full_search) { ((SimBase*)s)->set_positions(x, full_search); }
+int orc_sim_u2_flags3(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau,
+                      double ratio, const double* g, const int32_t* o) {
+    return ((SimBase*)s)->u2_flags3_of(U, p, cand, delta, plateau, ratio, g, o);
+}
 int orc_sim_u2_flags(void* s, const double* U, int64_t p, const double* cand, double delta, double plateau) {
     return ((SimBase*)s)->u2_flags_of(U, p, cand, delta, plateau);
 }

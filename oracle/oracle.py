@@ -27,7 +27,7 @@ def lib():
             "orc_sim_free": (None, [vp]),
             "orc_sim_set_threads": (None, [vp, C.c_int]),
             "orc_sim_set_scheme": (None, [vp, C.c_int, d, vp, C.c_int, C.c_int, C.c_int, C.c_int, d, C.c_int, vp, vp, vp,
-                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int, vp, vp, vp, C.c_int, d]),
+                                          d, C.c_int, vp, vp, d, C.c_int, C.c_int, vp, vp, vp, C.c_int, d, d]),
             "orc_sim_set_state": (None, [vp, vp]),
             "orc_sim_get_state": (None, [vp, vp]),
             "orc_sim_time_step": (d, [vp]),
@@ -47,6 +47,7 @@ def lib():
             "orc_sim_force": (None, [vp, vp, vp, vp, vp]),
             "orc_sim_expand": (None, [vp, vp, C.c_int]),
             "orc_sim_u2_flags": (C.c_int, [vp, vp, i64, vp, d, d]),
+            "orc_sim_u2_flags3": (C.c_int, [vp, vp, i64, vp, d, d, d, vp, vp]),
             "orc_sim_volumes": (None, [vp, C.c_int, vp]),
             "orc_sim_mass": (None, [vp, C.c_int, vp]),
             "orc_sim_set_source": (None, [vp, C.c_int, vp]),
@@ -162,7 +163,8 @@ class OracleSim:
                                  _p(a_t), _p(b_t), _p(c_t), sch.get("substep_ratio", 0.0), len(bcs), _p(tags), _p(kinds),
                                  sch.get("plateau", 0.0), int(bool(sch.get("streamed", False))),
                                  int(sch.get("flux", FLUX_RUSANOV) == FLUX_ALIGNED_RUSANOV), _p(a_im), _p(b_im),
-                                 _p(c_im), int(sch.get("mover", 0)), float(sch.get("mover_alpha", 1.0)))
+                                 _p(c_im), int(sch.get("mover", 0)), float(sch.get("mover_alpha", 1.0)),
+                                 float(sch.get("ratio", 0.5)))
 
     def set_threads(self, n):
         lib().orc_sim_set_threads(self.h, int(n))
@@ -267,6 +269,15 @@ class OracleSim:
         U = np.ascontiguousarray(U, np.float64).reshape(self.N, self.M)
         c = np.ascontiguousarray(cand, np.float64)
         return bool(lib().orc_sim_u2_flags(self.h, _p(U), p, _p(c), delta, plateau))
+
+    def u2_flags3(self, U, p, cand, delta, plateau, ratio, grads, orders):
+        """The MOODu2 check of particle p with the given gradient records (N x BG x M) and particle orders."""
+        U = np.ascontiguousarray(U, np.float64)
+        cand = np.ascontiguousarray(cand, np.float64)
+        g = np.ascontiguousarray(grads, np.float64)
+        o = np.ascontiguousarray(orders, np.int32)
+        return bool(lib().orc_sim_u2_flags3(self.h, _p(U), int(p), _p(cand), float(delta), float(plateau),
+                                            float(ratio), _p(g), _p(o)))
 
     def set_source(self, mode, rates):
         """The decay source S_i(U) = -rates_i U: mode 1 explicit, 2 implicit, 0 none."""

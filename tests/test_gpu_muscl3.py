@@ -116,6 +116,11 @@ STEP_CASES = {
     "2d_none_epd0_u1_rk3_box": (2, 48, 2.2, 0.25, BOX, L.HCC_LIM_NONE, L.HCC_STRAT_EPD0, L.HCC_CRIT_U1, "RK3_SSP", 8),
     "2d_none_epd2_u1_rk3_box": (2, 48, 2.5, 0.25, BOX, L.HCC_LIM_NONE, L.HCC_STRAT_EPD2, L.HCC_CRIT_U1, "RK3_SSP", 8),
     "2d_bj_soft_pad_rk3_box": (2, 48, 2.5, 0.25, BOX, L.HCC_LIM_BJ, L.HCC_STRAT_EPD1, L.HCC_CRIT_PAD, "RK3_SSP", 6),
+    # MOODu2 with its smooth-extremum override (gpu_tests.jl:20: MUSCL3 MOODu2 EPD1 RK3 box)
+    "2d_none_epd1_u2_rk3_box": (2, 48, 2.5, 0.25, BOX, L.HCC_LIM_NONE, L.HCC_STRAT_EPD1, L.HCC_CRIT_U2, "RK3_SSP", 8),
+    "2d_none_epd1_u2plateau_rk3_gauss": (2, 48, 2.5, 0.25, GAUSS, L.HCC_LIM_NONE, L.HCC_STRAT_EPD1, L.HCC_CRIT_U2,
+                                         "RK3_SSP", 8),
+    "2d_vk_sepd0_u2_rk2_box": (2, 48, 2.5, 0.25, BOX, L.HCC_LIM_VK, L.HCC_STRAT_STRICT_EPD0, L.HCC_CRIT_U2, "RK2_SSP", 6),
 }
 
 
@@ -124,10 +129,12 @@ def test_steps_bit_exact(ctx, case):
     """Multi-step runs: dt, states, orders (three levels), MOOD flags and the inadmissible count after every step."""
     D, n, R, jit, f, lim, strat, crit, tab, steps = STEP_CASES[case]
     mode = L.HCC_MODE_SOFT if "soft" in case else L.HCC_MODE_HARD
-    wl = _adv(D, n, R, jit, f, limiter=lim, mode=mode, strategy=strat, criterion=crit, delta=1e-6, tableau_name=tab)
+    plateau = 1e-3 if "plateau" in case else 0.0
+    wl = _adv(D, n, R, jit, f, limiter=lim, mode=mode, strategy=strat, criterion=crit, delta=1e-6, tableau_name=tab,
+              plateau=plateau)
     g, st, o = build_pair(ctx, wl)
     log = _run_steps(st, o, steps, cfl=0.4)
-    if crit == L.HCC_CRIT_U1 and "box" in case:   # the box makes MOOD drop orders (the smooth Gaussian need not)
+    if crit in (L.HCC_CRIT_U1, L.HCC_CRIT_U2) and "box" in case:   # the box makes MOOD drop orders
         assert sum(fl for _, fl, _, _ in log) > 0, "MOOD never fired"
         assert max(o1 + o2 for _, _, o1, o2 in log) > 0
 
@@ -185,11 +192,25 @@ def test_unsupported_muscl3_setups(ctx):
     wl = _adv(2, 24, 2.5, 0.25, GAUSS)
     c = wl["cloud"]
     g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
-    for kw in ({"streamed": True}, {"flux": L.HCC_FLUX_ALIGNED_RUSANOV}, {"criterion": L.HCC_CRIT_U2},
-               {"mover": L.HCC_MOVER_LAGRANGIAN}):
+    for kw in ({"streamed": True}, {"flux": L.HCC_FLUX_ALIGNED_RUSANOV}, {"mover": L.HCC_MOVER_LAGRANGIAN}):
         sch = W.scheme(1, L.HCC_PDE_LINADV, adv_speed=[[1.0], [0.5]], order=3, **kw)
         with pytest.raises(L.HccError):
             Stepper(ctx, g, sch)
     sch = W.scheme(1, L.HCC_PDE_LINADV, adv_speed=[[1.0], [0.5]], order=4)
     with pytest.raises(L.HccError):
         Stepper(ctx, g, sch)
+
+
+def test_u2_override_accepts_smooth_extrema(ctx):
+    """On a smooth Gaussian the DMP undershoots/overshoots at the extremum are accepted by MOODu2's curvature
+    override at order 3: fewer MOOD drops than MOODu1 over the same steps (both bit-exact with the oracle)."""
+    drops = {}
+    for crit in (L.HCC_CRIT_U1, L.HCC_CRIT_U2):
+        wl = _adv(2, 48, 2.5, 0.0, lambda x: np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.05),
+                  limiter=L.HCC_LIM_NONE, strategy=L.HCC_STRAT_EPD1, criterion=crit, delta=1e-12)
+        g, st, o = build_pair(ctx, wl)
+        log = _run_steps(st, o, 8, cfl=0.4)
+        drops[crit] = sum(fl for _, fl, _, _ in log)
+    assert drops[L.HCC_CRIT_U1] > 0
+    assert drops[L.HCC_CRIT_U2] < drops[L.HCC_CRIT_U1]
+
