@@ -307,7 +307,7 @@ def ensemble_main(args):
     print(json.dumps(line), flush=True)
 
 
-VARIANTS = ("moving", "imex", "epd2_u2", "strict_epd0", "streamed", "aligned", "solve")
+VARIANTS = ("moving", "imex", "epd2_u2", "strict_epd0", "streamed", "aligned", "solve", "euler3d", "muscl3")
 
 
 def variant_workload(W, L, name, n, seed):
@@ -327,6 +327,13 @@ def variant_workload(W, L, name, n, seed):
               "U0": W.euler_conserved(rho_q[q], np.column_stack([u_q[q], v_q[q]]), p_q[q], 1.4),
               "scheme": W.scheme(4, L.HCC_PDE_EULER, delta=1e-4 * h * h, mover=L.HCC_MOVER_LAGRANGIAN)}
         return wl, "LagrangianMover on the periodic unit square (same four quadrants)"
+    if name == "euler3d":   # the 3-D cube: 160^3 jittered periodic nodes (4.1 M), a pressure pulse, MUSCL2
+        wl = W.euler_3d(n=160, seed=seed)
+        return wl, ("3-D Euler pressure pulse on the jittered periodic 160^3 cube (4.1 M nodes, R = 2.2 h), "
+                    "MUSCL2+VK(hard)+Rusanov+MOOD(EPD1,u1), SSP-RK3, CFL 0.3 (neighbours through the caches)")
+    if name == "muscl3":   # the headline problem with MUSCL order 3
+        wl["scheme"] = dict(wl["scheme"], muscl_order=3)
+        return wl, "MUSCL3+VK(hard)+Rusanov+MOOD(EPD1,u1) (degree-2 reconstruction, degree-3 divergence)"
     if name == "imex":
         kw.update(tableau_name="IMEX_ARS233")
         desc = "IMEX ARS(2,3,3) with the implicit decay source, rate 2 (1 + 0.5 sin 2 pi x)"
@@ -379,11 +386,12 @@ def variant_main(args):
     U = st.download()
     assert np.all(np.isfinite(U)), "non-finite state"
     stages = len(wl["scheme"]["b_t"])
-    out = {"metric": f"node-stage updates/s, 2D Euler MUSCL+MOOD @1M nodes, variant {args.variant}",
+    out = {"metric": f"node-stage updates/s, Euler MUSCL+MOOD, variant {args.variant}",
            "value": g.N * stages * args.steps / wall, "unit": "node-stage updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"2-D Euler four-quadrant, h=1/{args.n}, " + desc, "nodes": g.N, "stages": stages,
+           "config": {"workload": desc if args.variant == "euler3d" else f"2-D Euler four-quadrant, h=1/{args.n}, " + desc,
+                      "nodes": g.N, "stages": stages,
                       "variant": args.variant},
            "detail": {"step_event_ms": tot_ms / max(n_steps, 1), "mood_iterations": iters}}
     print(json.dumps(out), flush=True)
