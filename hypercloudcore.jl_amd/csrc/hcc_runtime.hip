@@ -10,7 +10,7 @@
 // One thread per node, a sequential loop over its neighbour slots in slot order: every per-node sum is
 // formed in the reference's order, so results are bit-identical to the Float64 CPU path.
 #include "hcc_device.h"
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -304,12 +304,13 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(SearchArgs a, int64_t 
     }
 }
 
-// queued on the context's stream; the temporary storage is the grid's, grown when a scan needs more
+// rocPRIM's device-wide exclusive sum of integer counts (exact, so any association gives the same result), queued on
+// the context's stream; the temporary storage is the grid's, grown when a scan needs more
 template <class T>
 static hipError_t exclusive_scan(hcc_grid* g, const T* in, T* out, int64_t n) {
     hipStream_t st = g->ctx->stream;
     size_t bytes = 0;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st);
+    hipError_t e = rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), st);
     if (e != hipSuccess) return e;
     if (bytes > g->scan_tmp_bytes) {
         if (g->d_scan_tmp) {
@@ -321,7 +322,7 @@ static hipError_t exclusive_scan(hcc_grid* g, const T* in, T* out, int64_t n) {
         if ((e = hipMalloc(&g->d_scan_tmp, bytes)) != hipSuccess) return e;
         g->scan_tmp_bytes = bytes;
     }
-    return hipcub::DeviceScan::ExclusiveSum(g->d_scan_tmp, bytes, in, out, (int)n, st);
+    return rocprim::exclusive_scan(g->d_scan_tmp, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), st);
 }
 
 hcc::GeomArgs hcc_grid::args() const {
