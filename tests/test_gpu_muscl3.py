@@ -177,15 +177,16 @@ def test_step_cfl_matches_host_loop(ctx):
     assert np.array_equal(a.download(), b.download())
 
 
-def test_step_benchmark_workload_bit_exact(ctx):
-    """The reference's step benchmark (benchmark/step.jl, stab = both) at 250 000 particles: dt = 0.3 get_time_step,
-    fixed, 4 steps; everything bit-exact after each step."""
-    wl = W.step_benchmark(n=500)
+@pytest.mark.parametrize("n,steps", [(500, 4), (1000, 3)])
+def test_step_benchmark_workload_bit_exact(ctx, n, steps):
+    """The reference's step benchmark (benchmark/step.jl, stab = both) at 250 000 and at its published 1 000 000
+    particles: dt = 0.3 get_time_step, fixed; everything bit-exact after each step."""
+    wl = W.step_benchmark(n=n)
     g, st, o = build_pair(ctx, wl, threads=16)
     dt_g, dt_o = st.time_step(), o.time_step()
     assert dt_g == dt_o
-    log = _run_steps(st, o, 4, dt=0.3 * dt_g)
-    assert len(log) == 4
+    log = _run_steps(st, o, steps, dt=0.3 * dt_g)
+    assert len(log) == steps
 
 
 def test_unsupported_muscl3_setups(ctx):
