@@ -51,6 +51,56 @@ HCC_HD double vk_phi(double r) {
 }
 HCC_HD double bj_phi(double r) { return jmin(1.0, r); }
 
+// The limiter's phi of one component without a quotient per interface. _bounding_phi (Limiter.jl:93-107) folds
+// phi = min(phi, limiter_phi(r_k)) from phi = 1 over the limited slots, r_k = (bound - U_i) / rec_k with the bound on
+// rec_k's side: dup = u_max - U_i >= 0 for rec_k > 0, ddn = u_min - U_i <= 0 for rec_k < 0. Within one side the
+// rounded quotient is monotone in rec_k (the numerator is fixed and division rounds monotonically), so the smallest
+// ratio is that of the largest increment p1 (resp. the most negative n1), and every other slot's ratio is at least that
+// of the second largest distinct increment p2 (n2). A slot scan keeps these four numbers (0: none; a limited
+// increment exceeds its tolerance, so it is non-zero and not NaN) and phi_scan_finish forms phi from them:
+// - Barth-Jespersen: min(1, r) is monotone, so phi = min(1, r*), r* = min(r(p1), r(n1)) (signed zeros included:
+//   a zero bound difference gives the same zero ratio for its whole side).
+// - Venkatakrishnan: phi = min(1, vk(r*)) when every other ratio r is r*, >= r* + kPhiGap, or >= kPhiFlat:
+//   * vk(r) >= 1 exactly for r >= 2, and the rounded large branch keeps that for r >= kPhiFlat: there
+//     s = fl(1/r) <= 0.5 - 2e-10, the margin (1 + 2s) - (1 + s + 2s^2) = s (1 - 2s) >= 1e-10 exceeds the rounding of
+//     numerator and denominator (< 1e-15), so num >= den and the quotient rounds to >= 1.
+//   * on [0, kPhiFlat] vk rises with slope >= 0.124 (vk' = (4 + 4r - r^2) / (r^2 + r + 2)^2, smallest at the right
+//     end), so ratios kPhiGap apart differ by >= 1.2e-13 in vk, against a rounding error of the computed vk below
+//     1e-15 (a handful of correctly rounded operations on values <= 1.21): the farther ratio's rounded phi is not
+//     smaller.
+//   Otherwise (a near tie) it returns false and the caller folds every slot as the reference does.
+constexpr double kPhiFlat = 2.000000001, kPhiGap = 1e-12;
+struct PhiScan {
+    double p1, p2, n1, n2;
+};
+HCC_HD void phi_scan_init(PhiScan& t) { t.p1 = t.p2 = t.n1 = t.n2 = 0.0; }
+// v: a limited increment (|v| above its side's tolerance)
+HCC_HD void phi_scan_add(PhiScan& t, double v) {
+    const bool pos = v > 0.0;
+    const bool ptop = pos && v > t.p1, pmid = pos && v < t.p1 && v > t.p2;
+    const bool ntop = !pos && v < t.n1, nmid = !pos && v > t.n1 && v < t.n2;
+    t.p2 = ptop ? t.p1 : (pmid ? v : t.p2);
+    t.p1 = ptop ? v : t.p1;
+    t.n2 = ntop ? t.n1 : (nmid ? v : t.n2);
+    t.n1 = ntop ? v : t.n1;
+}
+HCC_HD bool phi_scan_finish(const PhiScan& t, double dup, double ddn, bool bj, double& phi) {
+    const double inf = __builtin_inf();
+    const double rp = t.p1 > 0.0 ? dup / t.p1 : inf, rn = t.n1 < 0.0 ? ddn / t.n1 : inf;
+    const double rs = jmin(rp, rn);
+    if (bj) { phi = jmin(1.0, rs); return true; }
+    if (rs != rs) return false;
+    if (!(rs < kPhiFlat)) { phi = 1.0; return true; }
+    const double lo = rs + kPhiGap;
+    auto safe = [&](double x) { return x >= lo || x >= kPhiFlat; };
+    const double rp2 = t.p2 > 0.0 ? dup / t.p2 : inf, rn2 = t.n2 < 0.0 ? ddn / t.n2 : inf;
+    // a zero bound difference gives its whole side the same zero ratio
+    if (!((rp == rs || safe(rp)) && (rn == rs || safe(rn)) && (dup == 0.0 || safe(rp2)) && (ddn == 0.0 || safe(rn2))))
+        return false;
+    phi = jmin(1.0, vk_phi(rs));
+    return true;
+}
+
 // Compressible Euler (build-defined; the reference ships no Euler PDE): state (rho, m_1..m_D, E),
 // u = m * (1/rho), p = (gamma - 1) (E - 0.5 sum m_d u_d), c = sqrt((gamma p) * (1/rho)).
 template <int D>

@@ -407,25 +407,52 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
             tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
             tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
         }
-        for (int k = 0; k < cnt; ++k) {
-            double pj[D], dd[D], h[D], pint[BG];
-            src.template head<D>(src.idx(base + (int64_t)k * kChunk), pj);
-            rec_geom<D>(a.g, ri, pj, dd);
+        // walks the slots in order: each one's increments at the degree-2 interface basis and their limiting
+        auto walk = [&](auto&& fn) {
+            for (int k = 0; k < cnt; ++k) {
+                double pj[D], dd[D], h[D], pint[BG];
+                src.template head<D>(src.idx(base + (int64_t)k * kChunk), pj);
+                rec_geom<D>(a.g, ri, pj, dd);
 #pragma unroll
-            for (int d = 0; d < D; ++d) h[d] = 0.5 * dd[d];
-            taylor<D, BG>(h, pint);
+                for (int d = 0; d < D; ++d) h[d] = 0.5 * dd[d];
+                taylor<D, BG>(h, pint);
+                double rec[M];
+                bool lim[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                double v = 0.0;
+                for (int m = 0; m < M; ++m) {
+                    double v = 0.0;
 #pragma unroll
-                for (int q = 0; q < BG; ++q) v = v + g[q * M + m] * pint[q];
-                const bool lim = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
-                if (lim) {
-                    const double r = (v > 0.0 ? dup[m] : ddn[m]) / v;
-                    const double pk = a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r);
-                    phi[m] = jmin(phi[m], pk);
+                    for (int q = 0; q < BG; ++q) v = v + g[q * M + m] * pint[q];
+                    rec[m] = v;
+                    lim[m] = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
                 }
+                fn(rec, lim);
             }
+        };
+        // phi from the extreme increments of each side (phi_scan_finish), the full fold after a near tie
+        const bool bj = a.limiter == HCC_LIM_BJ;
+        PhiScan sc[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) phi_scan_init(sc[m]);
+        walk([&](const double (&rec)[M], const bool (&lim)[M]) {
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (lim[m]) phi_scan_add(sc[m], rec[m]);
+        });
+        bool decided = true;
+#pragma unroll
+        for (int m = 0; m < M; ++m) decided = phi_scan_finish(sc[m], dup[m], ddn[m], bj, phi[m]) && decided;
+        if (!decided) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) phi[m] = 1.0;
+            walk([&](const double (&rec)[M], const bool (&lim)[M]) {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (lim[m]) {
+                        const double r = (rec[m] > 0.0 ? dup[m] : ddn[m]) / rec[m];
+                        phi[m] = jmin(phi[m], bj ? bj_phi(r) : vk_phi(r));
+                    }
+            });
         }
         const bool hard = a.mode == HCC_MODE_HARD;
 #pragma unroll

@@ -577,49 +577,75 @@ __device__ __forceinline__ void gather_node(const StepArgs& a, const double* __r
                 tup[m] = 16 * kEps * jmax(fabs(ui), fabs(umax[m]));
                 tdn[m] = 16 * kEps * jmax(fabs(ui), fabs(umin[m]));
             }
-            int32_t k1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
-            double pn[D];
-            if constexpr (Src::kPrefetch) src.template head<D>(src_slot(src, base, 0, cnt), pn);
-            for (int k = 0; k < cnt; ++k) {
-                double pj[D];
-                if constexpr (Src::kPrefetch) {
+            // walks the slots in order and hands each one's interface increments rec[m] = sum_q g[q, m] (d_q / 2)
+            // and whether the tolerance limits them
+            auto walk = [&](auto&& fn) {
+                int32_t k1 = src_slot(src, base, Src::kPrefetch ? 1 : 0, cnt);
+                double pn[D];
+                if constexpr (Src::kPrefetch) src.template head<D>(src_slot(src, base, 0, cnt), pn);
+                for (int k = 0; k < cnt; ++k) {
+                    double pj[D];
+                    if constexpr (Src::kPrefetch) {
 #pragma unroll
-                    for (int d = 0; d < D; ++d) pj[d] = pn[d];
-                    const int32_t jn = k1;
-                    k1 = src_slot(src, base, k + 2, cnt);
-                    src.template head<D>(jn, pn);
-                } else {
-                    const int32_t j = k1;
-                    k1 = src_slot(src, base, k + 1, cnt);
-                    src.template head<D>(j, pj);
-                }
-                double dd[D];
-                rec_geom<D>(a.g, ri, pj, dd);
-                double pint[D];
+                        for (int d = 0; d < D; ++d) pj[d] = pn[d];
+                        const int32_t jn = k1;
+                        k1 = src_slot(src, base, k + 2, cnt);
+                        src.template head<D>(jn, pn);
+                    } else {
+                        const int32_t j = k1;
+                        k1 = src_slot(src, base, k + 1, cnt);
+                        src.template head<D>(j, pj);
+                    }
+                    double dd[D];
+                    rec_geom<D>(a.g, ri, pj, dd);
+                    double pint[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) pint[d] = 0.5 * dd[d];
-                // The components are evaluated without branches (a skipped slot's quotient is computed and discarded;
-                // a zero reconstruction divides to an unused inf or NaN), so that the four division chains of a slot
-                // sit in one basic block and interleave; a slot no lane limits is skipped as a whole.
-                double rec[M];
-                bool lim[M], any = false;
-#pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int q = 0; q < D; ++q) v = v + g[q * M + m] * pint[q];
-                    rec[m] = v;
-                    lim[m] = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
-                    any = any || lim[m];
-                }
-                if (any) {
+                    for (int d = 0; d < D; ++d) pint[d] = 0.5 * dd[d];
+                    double rec[M];
+                    bool lim[M];
 #pragma unroll
                     for (int m = 0; m < M; ++m) {
-                        const double r = (rec[m] > 0.0 ? dup[m] : ddn[m]) / rec[m];
-                        const double pk = a.limiter == HCC_LIM_BJ ? bj_phi(r) : vk_phi(r);
-                        phi[m] = lim[m] ? jmin(phi[m], pk) : phi[m];
+                        double v = 0.0;
+#pragma unroll
+                        for (int q = 0; q < D; ++q) v = v + g[q * M + m] * pint[q];
+                        rec[m] = v;
+                        lim[m] = fabs(v) > (v > 0.0 ? tup[m] : tdn[m]);
                     }
+                    fn(rec, lim);
                 }
+            };
+            // phi from the extreme increments of each side (phi_scan_finish): no quotient per slot
+            const bool bj = a.limiter == HCC_LIM_BJ;
+            PhiScan sc[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) phi_scan_init(sc[m]);
+            walk([&](const double (&rec)[M], const bool (&lim)[M]) {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (lim[m]) phi_scan_add(sc[m], rec[m]);
+            });
+            bool decided = true;
+#pragma unroll
+            for (int m = 0; m < M; ++m) decided = phi_scan_finish(sc[m], dup[m], ddn[m], bj, phi[m]) && decided;
+            if (!decided) {
+                // a near tie of ratios: the reference's fold over every slot. The components are evaluated without
+                // branches (a skipped slot's quotient is computed and discarded), so that the division chains of a
+                // slot sit in one basic block and interleave.
+#pragma unroll
+                for (int m = 0; m < M; ++m) phi[m] = 1.0;
+                walk([&](const double (&rec)[M], const bool (&lim)[M]) {
+                    bool any = false;
+#pragma unroll
+                    for (int m = 0; m < M; ++m) any = any || lim[m];
+                    if (any) {
+#pragma unroll
+                        for (int m = 0; m < M; ++m) {
+                            const double r = (rec[m] > 0.0 ? dup[m] : ddn[m]) / rec[m];
+                            const double pk = bj ? bj_phi(r) : vk_phi(r);
+                            phi[m] = lim[m] ? jmin(phi[m], pk) : phi[m];
+                        }
+                    }
+                });
             }
 #pragma unroll
             for (int q = 0; q < D; ++q)

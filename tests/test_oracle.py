@@ -41,6 +41,109 @@ def test_venkatakrishnan_matches_textbook():
     assert O.limiter_phi(1, 1e17) == 1.0 and O.limiter_phi(1, 0.25) == 0.25
 
 
+PHI_FLAT, PHI_GAP = 2.000000001, 1e-12   # kPhiFlat, kPhiGap of hcc_math.h
+
+
+def _phi_scan(rec, lim, dup, ddn, bj):
+    """The kernels' phi without a quotient per slot (phi_scan_add / phi_scan_finish in hcc_math.h), restated:
+    None where a near tie leaves it to the full fold."""
+    p1 = p2 = n1 = n2 = 0.0
+    for v, l in zip(rec, lim):
+        if not l:
+            continue
+        if v > 0:
+            if v > p1:
+                p1, p2 = v, p1
+            elif p2 < v < p1:
+                p2 = v
+        else:
+            if v < n1:
+                n1, n2 = v, n1
+            elif n1 < v < n2:
+                n2 = v
+    inf = math.inf
+    rp = dup / p1 if p1 > 0 else inf
+    rn = ddn / n1 if n1 < 0 else inf
+    rs = O.lib().orc_jmin(rp, rn)
+    if bj:
+        return O.lib().orc_jmin(1.0, rs)
+    if rs != rs:
+        return None
+    if not rs < PHI_FLAT:
+        return 1.0
+    safe = lambda x: x >= rs + PHI_GAP or x >= PHI_FLAT
+    rp2 = dup / p2 if p2 > 0 else inf
+    rn2 = ddn / n2 if n2 < 0 else inf
+    if not ((rp == rs or safe(rp)) and (rn == rs or safe(rn)) and (dup == 0 or safe(rp2)) and (ddn == 0 or safe(rn2))):
+        return None
+    return O.lib().orc_jmin(1.0, O.limiter_phi(2, rs))
+
+
+def _phi_fold(rec, lim, dup, ddn, bj):
+    """_bounding_phi (Limiter.jl:93-107): phi = min(phi, limiter_phi(r)) over the limited slots, from 1."""
+    phi = 1.0
+    for v, l in zip(rec, lim):
+        if l:
+            r = (dup if v > 0 else ddn) / v
+            phi = O.lib().orc_jmin(phi, O.limiter_phi(1 if bj else 2, r))
+    return phi
+
+
+def test_limiter_phi_from_extreme_increments():
+    """The kernels form phi from the largest increments of each side instead of a quotient per slot; the
+    restatement here agrees with the reference's fold on random stencils with near ties, ties, signed zeros, huge
+    ratios and limited and unlimited slots, and reports the near ties it leaves to the fold."""
+    rng = np.random.default_rng(11)
+    undecided = 0
+    for trial in range(4000):
+        n = int(rng.integers(1, 24))
+        rec = rng.standard_normal(n) * 10.0 ** rng.uniform(-3, 3)
+        if trial % 5 == 0:   # ties and near ties
+            rec[: n // 2] = rec[0] * (1.0 + rng.integers(0, 3, n // 2) * 1e-15)
+        lim = rng.random(n) < 0.8
+        umax = float(np.max(np.abs(rec))) * rng.choice([0.0, 0.3, 1.9, 2.0, 2.2, 50.0, 1e300])
+        dup, ddn = umax, -umax * rng.choice([0.0, 0.5, 1.0, 3.0])
+        for bj in (False, True):
+            fast, ref = _phi_scan(rec, lim, dup, ddn, bj), _phi_fold(rec, lim, dup, ddn, bj)
+            if fast is None:
+                undecided += 1
+                continue
+            assert fast == ref and math.copysign(1, fast) == math.copysign(1, ref), (rec, lim, dup, ddn, bj)
+    assert 0 < undecided < 1000   # of the 4000 Venkatakrishnan stencils, a fifth of them built around ties
+
+
+def test_venkatakrishnan_rounding_facts_behind_the_scan():
+    """The two facts phi_scan_finish rests on, checked on the oracle's limiter_phi (Limiter.jl:39-45): the rounded
+    phi is >= 1 for every ratio >= kPhiFlat (dense next to it, up to 1e308 and inf), and ratios kPhiGap apart on
+    [0, kPhiFlat] never give a smaller rounded phi to the larger ratio (its computed value is within 1e-15 of the
+    exact rational). Adjacent ratios can invert by an ulp, which is why a near tie goes to the full fold."""
+    rng = np.random.default_rng(7)
+    rs = [PHI_FLAT, math.inf]
+    x = PHI_FLAT
+    for _ in range(3000):
+        x = float(np.nextafter(x, np.inf))
+        rs.append(x)
+    rs += list(rng.uniform(PHI_FLAT, 10.0, 20000)) + list(10.0 ** rng.uniform(1.0, 308.0, 20000))
+    rs += list(10.0 ** rng.uniform(13.0, 17.5, 20000))   # s = 1/r where fl(1 + s) and fl(1 + 2s) meet
+    for r in rs:
+        assert O.limiter_phi(2, r) >= 1.0, r
+    for r in rng.uniform(0.0, PHI_FLAT, 3000):
+        fr = Fraction(float(r))
+        exact = (fr * fr + 2 * fr) / (fr * fr + fr + 2)
+        assert abs(Fraction(O.limiter_phi(2, float(r))) - exact) < Fraction(1, 10 ** 15)
+    xs = rng.uniform(0.0, PHI_FLAT - PHI_GAP, 40000)
+    ys = np.minimum(xs + PHI_GAP * (1.0 + rng.uniform(0.0, 1.0, xs.size)), PHI_FLAT)
+    for x, y in zip(xs, ys):
+        if y - x >= PHI_GAP:
+            assert O.limiter_phi(2, float(y)) >= O.limiter_phi(2, float(x)), (x, y)
+    inversions = 0
+    for x in rng.uniform(0.5, 2.0, 20000):
+        y = float(np.nextafter(x, np.inf))
+        inversions += O.limiter_phi(2, y) < O.limiter_phi(2, float(x))
+    assert inversions > 0   # without the gap, the smallest ratio would not always give the smallest phi
+    assert O.limiter_phi(2, 1.9) < 0.996
+
+
 def test_julia_min_max_mod_semantics():
     """Utils.jl:84-87 (exact min/max, NaN-propagating, unit_core.jl:90-91) and Base.mod for floats."""
     assert O.lib().orc_jmin(0.1, 0.3) == 0.1 and O.lib().orc_jmax(0.1, 0.3) == 0.3
