@@ -2743,15 +2743,21 @@ static void lds_sizes3(int D, int M, int kind, int64_t hm, size_t idx_bytes, siz
         set(m3::lds_bytes3<2, 2>(hm, idx_bytes, false), m3::lds_bytes3<2, 2>(hm, idx_bytes, true));
     else if (kind == HCC_PDE_EULER && D == 2 && M == 4)
         set(m3::lds_bytes3<2, 4>(hm, idx_bytes, false), m3::lds_bytes3<2, 4>(hm, idx_bytes, true));
+    else if (kind == HCC_PDE_LINADV && D == 3 && M == 1)
+        set(m3::lds_bytes3<3, 1>(hm, idx_bytes, false), m3::lds_bytes3<3, 1>(hm, idx_bytes, true));
+    else if (kind == HCC_PDE_EULER && D == 3 && M == 5)
+        set(m3::lds_bytes3<3, 5>(hm, idx_bytes, false), m3::lds_bytes3<3, 5>(hm, idx_bytes, true));
 }
-// MUSCL3 variants: advection (1, 1), (2, 1), (2, 2) and 2-D Euler
+// MUSCL3 variants: advection (1, 1), (2, 1), (2, 2), (3, 1), 2-D and 3-D Euler
 static bool find_ops3(int D, int M, int kind, VariantOps& ops) {
     if (kind == HCC_PDE_LINADV) {
         if (D == 1 && M == 1) { ops = make_ops3<1, 1, HCC_PDE_LINADV>(); return true; }
         if (D == 2 && M == 1) { ops = make_ops3<2, 1, HCC_PDE_LINADV>(); return true; }
         if (D == 2 && M == 2) { ops = make_ops3<2, 2, HCC_PDE_LINADV>(); return true; }
+        if (D == 3 && M == 1) { ops = make_ops3<3, 1, HCC_PDE_LINADV>(); return true; }
     } else if (kind == HCC_PDE_EULER) {
         if (D == 2 && M == 4) { ops = make_ops3<2, 4, HCC_PDE_EULER>(); return true; }
+        if (D == 3 && M == 5) { ops = make_ops3<3, 5, HCC_PDE_EULER>(); return true; }
     }
     return false;
 }
@@ -2784,7 +2790,8 @@ static void launch_moments(hcc_stepper* s, double R_cut, double* dw = nullptr) {
     hipStream_t st = s->ctx->stream;
     if (s->order == 3) {   // the degree-2 reconstruction and the safeguarded degree-3 divergence
         if (s->D == 1) m3::k_moments3<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
-        else m3::k_moments3<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        else if (s->D == 2) m3::k_moments3<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+        else m3::k_moments3<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
         return;
     }
     if (s->D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);

@@ -1772,6 +1772,7 @@ void* orc_sim_new(int D, int M, int64_t N, const double* pos, const uint8_t* isb
         if (D == 2 && M == 2) return new Sim<2, 2, 3>(g, N, pos, isb, tags, alpha);
         if (D == 2 && M == 4) return new Sim<2, 4, 3>(g, N, pos, isb, tags, alpha);
         if (D == 3 && M == 1) return new Sim<3, 1, 3>(g, N, pos, isb, tags, alpha);
+        if (D == 3 && M == 5) return new Sim<3, 5, 3>(g, N, pos, isb, tags, alpha);
     }
     return nullptr;
 }

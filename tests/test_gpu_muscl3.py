@@ -44,7 +44,7 @@ def _run_steps(st, o, nsteps, cfl=None, dt=None, check_each=True):
 def _adv(D, n, R, jitter, U0fn, periodic=True, M=1, **kw):
     h = 1.0 / n
     dom = W.domain(D, (0.0,) * D, (1.0,) * D, (periodic,) * D, h, R, jitter, 3)
-    speeds = [[1.0, -0.6][:M], [0.5, 0.8][:M]][:D]
+    speeds = [[1.0, -0.6][:M], [0.5, 0.8][:M], [-0.7, 0.3][:M]][:D]
     sch = W.scheme(M, L.HCC_PDE_LINADV, adv_speed=speeds, order=3, **kw)
     cloud = L.cloud_generate(W.domain_desc(dom))
     U0 = U0fn(cloud["pos"])
@@ -56,11 +56,11 @@ GAUSS = lambda x: np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
 BOX = lambda x: np.all((x > 0.3) & (x < 0.6), axis=1).astype(float)
 
 
-@pytest.mark.parametrize("D,R,jitter", [(1, 2.5, 0.0), (1, 3.5, 0.2), (2, 2.2, 0.25), (2, 2.5, 0.0)])
+@pytest.mark.parametrize("D,R,jitter", [(1, 2.5, 0.0), (1, 3.5, 0.2), (2, 2.2, 0.25), (2, 2.5, 0.0), (3, 2.0, 0.25)])
 def test_geometry_caches_bit_exact(ctx, D, R, jitter):
     """k_moments3: the degree-2 reconstruction inverse, the divergence inverse of each stencil's degree (3, or 2 where
     the safeguard lowers it; zero-padded), its derivative rows and the per-slot derivative weights."""
-    wl = _adv(D, 48 if D == 2 else 64, R, jitter, GAUSS)
+    wl = _adv(D, {1: 64, 2: 48, 3: 12}[D], R, jitter, GAUSS)
     g, st, o = build_pair(ctx, wl)
     ig, og, idv, od, dw = st.moments()
     ig_o, og_o, idv_o, od_o, dw_o = o.moments()
@@ -215,3 +215,49 @@ def test_u2_override_accepts_smooth_extrema(ctx):
     assert drops[L.HCC_CRIT_U1] > 0
     assert drops[L.HCC_CRIT_U2] < drops[L.HCC_CRIT_U1]
 
+
+
+def _euler3d_muscl3(n, seed=1):
+    """config[2]'s cube at a small size: periodic, R = 2.2 h (a mix of degree-3 and safeguarded degree-2 divergence
+    fits), rest state with a Gaussian pressure pulse of width 0.05, MUSCL3 + VK(hard) + MOOD(EPD1, u1), RK3."""
+    wl = W.euler_3d(n=n, seed=seed)
+    wl["scheme"] = dict(wl["scheme"], muscl_order=3)
+    return wl
+
+
+def test_rhs_3d_advection_and_euler(ctx):
+    """MUSCL3 in 3-D: the degree-2 reconstruction with its nine terms (x, y, z, x^2/2, y^2/2, z^2/2, xy, xz, yz),
+    the 20-term degree-3 divergence fit where the stencil carries it (R = 2 h: a quarter of the particles have fewer
+    than 30 neighbours and take the safeguarded degree 2), for a limited box and the Euler pulse."""
+    wl = _adv(3, 12, 2.0, 0.25, BOX, limiter=L.HCC_LIM_VK)
+    g, st, o = build_pair(ctx, wl)
+    K, G = st.rhs(wl["U0"])
+    K_o, G_o = o.rhs(wl["U0"])
+    assert np.array_equal(G, G_o), first_diff(G, G_o)
+    assert np.array_equal(K, K_o), first_diff(K, K_o)
+    counts = np.diff(g.neighbors(weights=False, distances=False)[0])
+    assert np.any(counts < 30) and np.any(counts >= 30)   # both divergence degrees occur (2 nb >= 3 x 20 for degree 3)
+    e = _euler3d_muscl3(10)
+    g, st, o = build_pair(ctx, e)
+    K, G = st.rhs(e["U0"])
+    K_o, G_o = o.rhs(e["U0"])
+    assert np.array_equal(G, G_o), first_diff(G, G_o)
+    assert np.array_equal(K, K_o), first_diff(K, K_o)
+
+
+def test_steps_3d_advection_box_mood(ctx):
+    """3-D advection of a box, MUSCL3 + VK(hard) + MOOD(EPD1, u1), RK3: MOOD drops particles through three orders;
+    every step bit-exact."""
+    wl = _adv(3, 14, 2.2, 0.25, BOX, limiter=L.HCC_LIM_VK, strategy=L.HCC_STRAT_EPD1, criterion=L.HCC_CRIT_U1,
+              delta=1e-6)
+    g, st, o = build_pair(ctx, wl)
+    log = _run_steps(st, o, 4, cfl=0.4)
+    assert sum(fl for _, fl, _, _ in log) > 0, "MOOD never fired"
+
+
+def test_steps_3d_euler_pulse(ctx):
+    """The 3-D Euler pulse with MUSCL3, CFL 0.3: dt, states, orders, flags and the inadmissible count bit-exact after
+    every step."""
+    wl = _euler3d_muscl3(12)
+    g, st, o = build_pair(ctx, wl)
+    _run_steps(st, o, 4, cfl=wl["cfl"])
