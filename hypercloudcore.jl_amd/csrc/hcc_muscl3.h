@@ -833,6 +833,252 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// The flux pass of wide fits (3-D Euler: the divergence fit has BD = 20 terms for D M = 15 flux columns, 300
+// accumulators, far beyond a lane's registers): one particle per group of G = 16 lanes. In each chunk of G slots, lane
+// gl runs slot gl of the chunk (slot3: the interface states, fluxes, Rusanov, dF) and stores w, the divergence basis
+// and dF in LDS; then lane q < D M adds the chunk's terms of fit column q in slot order (BD terms for the cached fit,
+// the B1 = D + 1 terms of the direct degree-1 fit, whose moment matrix lane G - 1 accumulates). Every sum keeps the
+// order of flux_node, so the results are its bits. Workgroups of one wave run GPW = 4 particles; the chunk loop runs
+// to the wave's largest count, so every barrier is uniform.
+// ------------------------------------------------------------------------------------------------------------
+template <int D, int M>
+struct Split3 {
+    static constexpr int G = 16, GPW = 64 / G;
+    static constexpr int BD = L3<D, M>::BD, DM = D * M, B1 = D + 1;
+    static constexpr int SR = BD + DM;        // per slot: w, the divergence basis (BD - 1 terms), dF (axis major)
+    static constexpr int XS = B1 * D + 1;     // the direct fit's solved derivative columns, and whether it solved
+    static_assert(DM < G, "one fit column per lane, and a lane for the moment matrix");
+};
+// the wide variants (register-resident accumulators of flux_node would spill)
+template <int D, int M>
+constexpr bool kWide3 = L3<D, M>::BD * D * M > 160;
+
+template <int D, int M, int KIND, bool CHECK>
+__device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
+                                            double* __restrict__ K, int64_t i, bool take, double h,
+                                            double (*rows)[Split3<D, M>::SR], double* xs, double* part) {
+    using L = L3<D, M>;
+    using S = Split3<D, M>;
+    constexpr int NR = L::NR, NG = L::NG, BD = L::BD, DM = S::DM, B1 = S::B1, G = S::G;
+    const int gl = threadIdx.x % G;
+    const int64_t N = a.g.N;
+    const double invL = a.g.invL;
+    int cnt = 0, mode = 0;   // mode 1: the cached fit over the slots; 2: the direct degree-1 fit; 0: no slot loop
+    int oi = 0;
+    double ri[NR], fi[M], gi[NG], Fi[D][M];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) ri[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) gi[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int m = 0; m < M; ++m) Fi[d][m] = 0.0;
+    int64_t base = 0;
+    if (take) {
+        cnt = a.g.cnt[i];
+        load_rec<NR, L::PU>(Y, i, ri);
+        if (cnt > 0) {
+            load_rec<NG, L::GS>(a.G, i, gi);
+            base = sell_base(a.g, i);
+            oi = a.orders[i];
+            int oi_t = oi;
+            if (a.packed) { oi_t = oi >> 4; oi &= 15; }
+            const bool okd = a.okd[i];
+            const bool cached = okd && div_index(D, oi_t, cnt) == div_index(D, 3, cnt);
+            mode = (cached || oi_t == 3) ? (okd ? 1 : 0) : 2;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
+    if (mode) Pde<D, M, KIND>::flux(a.pde, fi, Fi);
+    int mc = mode ? cnt : 0;   // the wave's largest slot count: a uniform chunk loop
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+    const int nb = mode == 1 ? BD : B1;
+    double b[BD], A[B1][B1];
+#pragma unroll
+    for (int c = 0; c < BD; ++c) b[c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < B1; ++r)
+#pragma unroll
+        for (int c = 0; c < B1; ++c) A[r][c] = 0.0;
+    for (int k0 = 0; k0 < mc; k0 += G) {
+        const int k = k0 + gl;
+        if (mode && k < cnt) {
+            const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
+            double rj[NR], gj[NG], dd[D], w, dF[D][M];
+            load_rec<NR, L::PU>(Y, j, rj);
+            load_rec<NG, L::GS>(a.G, j, gj);
+            const int oj = a.orders[j];
+            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
+            double ds[D], t[BD - 1];
+#pragma unroll
+            for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
+            taylor<D, BD - 1>(ds, t);   // its first D terms are ds itself: the direct fit's basis [1, d invL]
+            double* row = rows[gl];
+            row[0] = w;
+#pragma unroll
+            for (int c = 1; c < BD; ++c) row[c] = t[c - 1];
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int m = 0; m < M; ++m) row[BD + d * M + m] = dF[d][m];
+        }
+        __syncthreads();
+        if (mode) {
+            const int ns = min(G, cnt - k0);
+            if (gl < DM) {
+                for (int s = 0; s < ns; ++s) {
+                    const double* row = rows[s];
+                    const double w = row[0], dFv = row[BD + gl];
+                    b[0] = b[0] + w * (1.0 * dFv);
+#pragma unroll
+                    for (int c = 1; c < BD; ++c)
+                        if (c < nb) b[c] = b[c] + w * (row[c] * dFv);
+                }
+            } else if (gl == G - 1 && mode == 2) {
+                for (int s = 0; s < ns; ++s) {
+                    const double* row = rows[s];
+                    const double w = row[0];
+                    double pb[B1];
+                    pb[0] = 1.0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pb[1 + d] = row[1 + d];
+#pragma unroll
+                    for (int r = 0; r < B1; ++r)
+#pragma unroll
+                        for (int c = 0; c < B1; ++c) A[r][c] = A[r][c] + w * (pb[r] * pb[c]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (mode == 2 && gl == G - 1) {   // _mls_divergence: (C \ E)' with E the derivative rows of the identity
+        double U[B1][B1];
+        const bool ok = chol_moment<B1>(A, U);
+        if (ok) {
+            double E[B1][D], X[B1][D];
+#pragma unroll
+            for (int r = 0; r < B1; ++r)
+#pragma unroll
+                for (int d = 0; d < D; ++d) E[r][d] = r == 1 + d ? 1.0 : 0.0;
+            chol_solve<B1, D>(U, E, X);
+#pragma unroll
+            for (int r = 0; r < B1; ++r)
+#pragma unroll
+                for (int d = 0; d < D; ++d) xs[r * D + d] = X[r][d];
+        }
+        xs[B1 * D] = ok ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (mode && gl < DM) {   // the derivative row of fit column gl = d M + m
+        const int d = gl / M;
+        double r = 0.0;
+        if (mode == 1) {
+            r = a.invd[(d * BD + 0) * N + i] * b[0];
+#pragma unroll
+            for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c];
+        } else if (xs[B1 * D] != 0.0) {
+            r = xs[0 * D + d] * b[0];
+#pragma unroll
+            for (int c = 1; c < B1; ++c) r = r + xs[c * D + d] * b[c];
+        }
+        part[gl] = r;
+    }
+    __syncthreads();
+    bool ch = false;
+    if (take && gl == 0) {
+        double Kout[M];
+        const bool fit = mode == 1 || (mode == 2 && xs[B1 * D] != 0.0);
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (cnt == 0) {
+                Kout[m] = -0.0 + explicit_source<M>(a, i, fi, m);
+            } else if (fit) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) acc = acc + part[d * M + m];
+                Kout[m] = -(2.0 * (acc * invL)) + explicit_source<M>(a, i, fi, m);
+            } else {
+                Kout[m] = -(2.0 * 0.0) + explicit_source<M>(a, i, fi, m);
+            }
+        }
+        ch = flux_tail<D, M, KIND, CHECK>(a, pred, K, i, fi, Kout, h);
+    }
+    __syncthreads();   // the group's LDS is reused by the next particle
+    return ch;
+}
+
+// the full (or dirty-only) flux pass of the wide variants: four particles a workgroup, XCD-aware order of the quads
+template <int D, int M, int KIND, bool CHECK>
+__global__ __launch_bounds__(64) void k_flux_wide(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                 double* __restrict__ K, int full, int64_t nquad) {
+    using S = Split3<D, M>;
+    __shared__ double rows[S::GPW][S::G][S::SR];
+    __shared__ double xs[S::GPW][S::XS];
+    __shared__ double part[S::GPW][S::DM];
+    const int64_t lq = xcd_block(blockIdx.x, nquad);
+    if (lq >= nquad) return;
+    const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
+    const int64_t i = lq * S::GPW + grp;
+    bool take = false;
+    double h = 0.0;
+    if (i < a.g.N) {
+        if (a.g.isb[i]) {
+            if (full && gl == 0)
+#pragma unroll
+                for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+        } else {
+            take = full || a.dirty[i];
+            if (CHECK) h = a.hratio * node_dt(a.dtp, a.inst, i);
+        }
+    }
+    const bool ch = flux_group3<D, M, KIND, CHECK>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp]);
+    if (CHECK) {
+        const int bad = (take && gl == 0) ? a.inadm[i] : 0;
+        append_changed(a, ch, i);
+        if (pred.count) add_inadmissible(a, bad);
+    }
+}
+
+// the fixed point's flux pass of the wide variants over its work list (the bookkeeping of k_flux_list)
+template <int D, int M, int KIND, bool FP = true>
+__global__ __launch_bounds__(64) void k_flux_wide_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                      double* __restrict__ K, int p) {
+    using S = Split3<D, M>;
+    __shared__ double rows[S::GPW][S::G][S::SR];
+    __shared__ double xs[S::GPW][S::XS];
+    __shared__ double part[S::GPW][S::DM];
+    const int64_t n = (int64_t)*work_count(a, p);
+    if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
+    const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
+    for (int64_t t0 = (int64_t)blockIdx.x * S::GPW; t0 < n; t0 += (int64_t)gridDim.x * S::GPW) {
+        const int64_t t = t0 + grp;
+        int64_t i = 0;
+        bool take = false;
+        int before = 0;
+        if (t < n) {
+            i = a.dlist[t];
+            take = !a.g.isb[i];
+            if (FP && gl == 0) {
+                a.dmark[i] = 0u;   // each dirty particle is listed once (k_mark_gather); cleared for the next
+                if (take) before = a.inadm[i];
+            }
+        }
+        const double h = FP && take ? a.hratio * node_dt(a.dtp, a.inst, i) : 0.0;
+        const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp]);
+        if constexpr (!FP) continue;
+        const int delta = (take && gl == 0) ? (int)a.inadm[i] - before : 0;
+        if (t < n && gl == 0) {
+            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+            if (ch) *changed_any(a, 1 - p) = 1;
+        }
+        if (pred.count) add_inadmissible(a, delta);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // the unfused check (_mood_flag! + _mood_drop!, MOOD.jl:289-309) with MOODu2's smooth-extremum override
 // (_u2_fails, MOOD.jl:223-271): a component that violates the DMP (and is not a plateau) is accepted when, at
 // order 3, every axis's curvature term (gradient record entry D + d) over the stencil has one sign

@@ -2701,7 +2701,18 @@ struct Variant3 : Variant<D, M, KIND> {
     static void flux(const hcc_stepper* s, const StepArgs& a, const PredArgs& p, const double* Y, double* K, int full,
                      bool check) {
         const int64_t nb = nblocks(s->N);
-        if (full < 0) {
+        if constexpr (m3::kWide3<D, M>) {   // one particle per 16 lanes, the fit's columns spread over the lanes
+            const int64_t nq = (s->N + m3::Split3<D, M>::GPW - 1) / m3::Split3<D, M>::GPW;
+            if (full < 0) {
+                k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
+                k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
+                m3::k_flux_wide_list<D, M, KIND, false><<<list_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
+            } else if (check) {
+                m3::k_flux_wide<D, M, KIND, true><<<grid_for(nq), 64, 0, st(s)>>>(a, p, Y, K, full, nq);
+            } else {
+                m3::k_flux_wide<D, M, KIND, false><<<grid_for(nq), 64, 0, st(s)>>>(a, p, Y, K, full, nq);
+            }
+        } else if (full < 0) {
             k_zero_u64<<<1, 64, 0, st(s)>>>(s->counters + 5);
             k_compact_dirty<<<grid_for(nb), kBlock, 0, st(s)>>>(a, nb);
             m3::k_flux_list<D, M, KIND, false><<<list_grid(s), 64, 0, st(s)>>>(a, p, Y, K, 0);
@@ -2715,7 +2726,8 @@ struct Variant3 : Variant<D, M, KIND> {
         }
     }
     static void flux_list(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K, int p) {
-        m3::k_flux_list<D, M, KIND, true><<<list_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+        if constexpr (m3::kWide3<D, M>) m3::k_flux_wide_list<D, M, KIND, true><<<list_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
+        else m3::k_flux_list<D, M, KIND, true><<<list_grid(s), 64, 0, st(s)>>>(a, pr, Y, K, p);
     }
     // the unfused check with MOODu2's curvature override
     static void flag(const hcc_stepper* s, const StepArgs& a, const double* Yp, const double* cand, const double* Kp,
