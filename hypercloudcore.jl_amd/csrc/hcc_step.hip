@@ -786,8 +786,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
                                                            int64_t nblk) {
     extern __shared__ double sh[];
     if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
-    const int64_t lb = logical_block(nblk);
-    if (lb >= nblk) return;
+    const int64_t pos = logical_block(nblk);
+    if (pos >= nblk) return;
+    const int64_t lb = (!CFL && !FEAT && a.gflat && a.gorder) ? a.gorder[pos] : pos;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
     if constexpr (!CFL && !FEAT) {
@@ -841,6 +842,35 @@ __global__ __launch_bounds__(kBlock) void k_block_gflat(GeomArgs g, const uint8_
     }
     ok = __syncthreads_and(ok);
     if (threadIdx.x == 0) gflat[lb] = (uint8_t)ok;
+}
+
+// The launch order of a staged pass's blocks from their flat flags (bflat or gflat): the non-flat blocks, whose
+// workgroups stage a halo and run slot loops, first, then the flat ones, each group in index order. The hardware
+// hands out workgroups in launch order as CUs free up, so the long workgroups start first and the short flat ones
+// fill the gaps at the end (longest first), instead of a long one starting late and finishing alone. The XCD-aware
+// mapping then applies to positions in this list. One workgroup: each thread counts a contiguous range, an exclusive
+// scan of the counts places every block. Results do not depend on the order (each block writes its own particles).
+__global__ __launch_bounds__(1024) void k_block_order(const uint8_t* __restrict__ flat, int32_t* __restrict__ order,
+                                                      int64_t nblk) {
+    __shared__ int32_t cnt[1024];
+    const int64_t per = (nblk + 1023) / 1024, b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+    int32_t c = 0;
+    for (int64_t b = b0; b < b1; ++b) c += flat[b] ? 0 : 1;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {   // inclusive scan (Hillis-Steele)
+        const int32_t v = threadIdx.x >= (unsigned)o ? cnt[threadIdx.x - o] : 0;
+        __syncthreads();
+        cnt[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int32_t total = cnt[1023];
+    int64_t nf = cnt[threadIdx.x] - c;             // non-flat blocks before this range
+    int64_t fl = total + (b0 < nblk ? b0 : nblk) - nf;   // flat blocks go after all non-flat ones
+    for (int64_t b = b0; b < b1; ++b) {
+        if (flat[b]) order[fl++] = (int32_t)b;
+        else order[nf++] = (int32_t)b;
+    }
 }
 
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
@@ -1365,8 +1395,9 @@ template <int D, int M, int KIND, bool CHECK, int FV>
 __global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux_staged(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                          double* __restrict__ K, int64_t nblk) {
     extern __shared__ double sh[];
-    const int64_t lb = logical_block(nblk);
-    if (lb >= nblk) return;
+    const int64_t pos = logical_block(nblk);
+    if (pos >= nblk) return;
+    const int64_t lb = a.border ? a.border[pos] : pos;
     const double h = CHECK ? a.hratio * node_dt(a.dtp, a.inst, lb * kBlock) : 0.0;
     // a flat block (FV = 0) needs neither its halo nor its slot loops (uniform per workgroup: no barrier is skipped
     // by some threads only)
@@ -2369,6 +2400,8 @@ struct hcc_stepper {
     uint8_t* bflat = nullptr;
     uint8_t* kz = nullptr;     // per particle: which stages' K are exactly zero (N, boundary particles 0xFF) ...
     uint8_t* gflat = nullptr;  // ... and the flat blocks of the gathers of stages 2 .. s
+    int32_t* border = nullptr; // launch orders of the staged flux pass and of those gathers (k_block_order)
+    int32_t* gorder = nullptr;
     bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
@@ -2474,7 +2507,7 @@ StepArgs hcc_stepper::args() const {
     // fused check in the flux pass, or k_mood_flag): the checkpoint's state is the one the stage's gather read
     a.ext_on = (desc.mood_criterion == HCC_CRIT_U1 || desc.mood_criterion == HCC_CRIT_U2) ? 1 : 0;
     a.dtp = dtp;
-    if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; }
+    if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; a.border = border; }
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.feat = feat;
     a.ind_list = ind_list;
@@ -2517,6 +2550,7 @@ struct Variant {
         if (!full || !a.bflat) return;
         const int64_t nb = nblocks(s->N);
         k_block_flat<<<(unsigned)nb, kBlock, 0, st(s)>>>(a.g, a.cnb, a.bflat, nb);
+        if (a.border) k_block_order<<<1, 1024, 0, st(s)>>>(a.bflat, const_cast<int32_t*>(a.border), nb);
     }
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count;
     // up to 65536 / GPW workgroups (32 waves a CU for the first, largest lists; a cap of 16384 / GPW measured about
@@ -2821,7 +2855,8 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
                     (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n,
-                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat})
+                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat, (void*)s->border,
+                    (void*)s->gorder})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
         (void)hipFree(s->Y[k]); (void)hipFree(s->K[k]); (void)hipFree(s->KI[k]); (void)hipFree(s->KX[k]);
@@ -2974,6 +3009,12 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->gflat, nblocks(N)));
         HCC_TRY_HIP(ctx, hipMemset(s->kz, 0xFF, N));   // boundary particles without a root never change
         HCC_TRY_HIP(ctx, hipMemset(s->gflat, 0, nblocks(N)));
+        HCC_TRY_HIP(ctx, dalloc(&s->border, nblocks(N)));
+        HCC_TRY_HIP(ctx, dalloc(&s->gorder, nblocks(N)));
+        // no block is flat yet: both orders start as the identity
+        k_block_order<<<1, 1024, 0, st>>>(s->bflat, s->border, nblocks(N));
+        k_block_order<<<1, 1024, 0, st>>>(s->gflat, s->gorder, nblocks(N));
+        HCC_TRY_HIP(ctx, hipGetLastError());
         s->flat_blocks = true;
     }
     for (auto& r : s->ring) {
@@ -3373,8 +3414,10 @@ struct StepRunner {
             if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
+                k_block_order<<<1, 1024, 0, str>>>(s->gflat, s->gorder, nb);
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
+                ag.gorder = s->gorder;
                 ops.gather(s, ag, Y, 1, 0);
             } else ops.gather(s, a, Y, 1, i == 1);
             if (s->feat_on) {
