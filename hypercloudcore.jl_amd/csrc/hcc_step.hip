@@ -788,7 +788,11 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (blockIdx.x == 0 && threadIdx.x == 0) { a.counters[0] = 0; a.counters[5] = 0; a.counters[8] = 0; }   // lists 0 of the flux pass's fixed point
     const int64_t pos = logical_block(nblk);
     if (pos >= nblk) return;
-    const int64_t lb = (!CFL && !FEAT && a.gflat && a.gorder) ? a.gorder[pos] : pos;
+    // launch order: the stage's flat-block order for the gathers of stages 2 .. s; otherwise that of the last flux
+    // pass, whose non-flat blocks are those of a state one stage earlier (a guess at the long workgroups: any order
+    // gives the same results)
+    const int32_t* ord = (!CFL && !FEAT && a.gflat) ? a.gorder : a.border;
+    const int64_t lb = ord ? ord[pos] : pos;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
     if constexpr (!CFL && !FEAT) {
