@@ -809,6 +809,48 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (CFL) block_min(a, dtc, lb);
 }
 
+// The launch order of a staged pass's blocks from their flat flags (bflat or gflat, 0 or 1): the non-flat blocks,
+// whose workgroups stage a halo and run slot loops, first, then the flat ones, each group in index order. The hardware
+// hands out workgroups in launch order as CUs free up, so the long workgroups start first and the short flat ones
+// fill the gaps at the end (longest first), instead of a long one starting late and finishing alone; the XCD-aware
+// mapping then applies to positions in this list. One workgroup: each thread counts a contiguous range of flags
+// (16 per load), a scan of the counts (shuffles in each wave, then the four wave totals) places every block. Results
+// do not depend on the order (each block writes its own particles).
+__global__ __launch_bounds__(kBlock) void k_block_order(const uint8_t* __restrict__ flat, int32_t* __restrict__ order,
+                                                        int64_t nblk) {
+    __shared__ int32_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t per = (((nblk + kBlock - 1) / kBlock) + 15) & ~(int64_t)15;   // a multiple of 16 flags a thread
+    const int64_t b0 = min(nblk, (int64_t)threadIdx.x * per), b1 = min(nblk, b0 + per);
+    int32_t c = 0;
+    int64_t b = b0;
+    for (; b + 16 <= b1; b += 16) {   // b0 is a multiple of 16: aligned 16-byte loads
+        const uint4 v = *reinterpret_cast<const uint4*>(flat + b);
+        c += 16 - __popc(v.x) - __popc(v.y) - __popc(v.z) - __popc(v.w);
+    }
+    for (; b < b1; ++b) c += flat[b] ? 0 : 1;
+    int32_t x = c;   // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        before += w < wave ? wsum[w] : 0;
+        total += wsum[w];
+    }
+    int64_t nf = before + x - c;      // non-flat blocks before this range
+    int64_t fl = total + b0 - nf;     // flat blocks go after all non-flat ones
+    for (int64_t q = b0; q < b1; ++q) {
+        if (flat[q]) order[fl++] = (int32_t)q;
+        else order[nf++] = (int32_t)q;
+    }
+}
+
 // Whether block lb of the next flux pass is flat: every particle of its halo (its own particles and their
 // neighbours) is a boundary particle or has cnb from the full gather just run. Then for every particle i of the block
 // and every neighbour j: U_j == U_i (i's neighbourhood is constant), g_i = g_j = 0 (a constant neighbourhood fits an
@@ -848,33 +890,10 @@ __global__ __launch_bounds__(kBlock) void k_block_gflat(GeomArgs g, const uint8_
     if (threadIdx.x == 0) gflat[lb] = (uint8_t)ok;
 }
 
-// The launch order of a staged pass's blocks from their flat flags (bflat or gflat): the non-flat blocks, whose
-// workgroups stage a halo and run slot loops, first, then the flat ones, each group in index order. The hardware
-// hands out workgroups in launch order as CUs free up, so the long workgroups start first and the short flat ones
-// fill the gaps at the end (longest first), instead of a long one starting late and finishing alone. The XCD-aware
-// mapping then applies to positions in this list. One workgroup: each thread counts a contiguous range, an exclusive
-// scan of the counts places every block. Results do not depend on the order (each block writes its own particles).
-__global__ __launch_bounds__(1024) void k_block_order(const uint8_t* __restrict__ flat, int32_t* __restrict__ order,
-                                                      int64_t nblk) {
-    __shared__ int32_t cnt[1024];
-    const int64_t per = (nblk + 1023) / 1024, b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
-    int32_t c = 0;
-    for (int64_t b = b0; b < b1; ++b) c += flat[b] ? 0 : 1;
-    cnt[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {   // inclusive scan (Hillis-Steele)
-        const int32_t v = threadIdx.x >= (unsigned)o ? cnt[threadIdx.x - o] : 0;
-        __syncthreads();
-        cnt[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const int32_t total = cnt[1023];
-    int64_t nf = cnt[threadIdx.x] - c;             // non-flat blocks before this range
-    int64_t fl = total + (b0 < nblk ? b0 : nblk) - nf;   // flat blocks go after all non-flat ones
-    for (int64_t b = b0; b < b1; ++b) {
-        if (flat[b]) order[fl++] = (int32_t)b;
-        else order[nf++] = (int32_t)b;
-    }
+// The identity order (no block known to be flat yet)
+__global__ __launch_bounds__(kBlock) void k_iota(int32_t* __restrict__ order, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) order[i] = (int32_t)i;
 }
 
 // The stage predictor fused into the flux passes (form_candidate!, TimeSteppers.jl:365-397; _stage_predictor!,
@@ -2554,7 +2573,7 @@ struct Variant {
         if (!full || !a.bflat) return;
         const int64_t nb = nblocks(s->N);
         k_block_flat<<<(unsigned)nb, kBlock, 0, st(s)>>>(a.g, a.cnb, a.bflat, nb);
-        if (a.border) k_block_order<<<1, 1024, 0, st(s)>>>(a.bflat, const_cast<int32_t*>(a.border), nb);
+        if (a.border) k_block_order<<<1, kBlock, 0, st(s)>>>(a.bflat, const_cast<int32_t*>(a.border), nb);
     }
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count;
     // up to 65536 / GPW workgroups (32 waves a CU for the first, largest lists; a cap of 16384 / GPW measured about
@@ -3016,8 +3035,8 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->border, nblocks(N)));
         HCC_TRY_HIP(ctx, dalloc(&s->gorder, nblocks(N)));
         // no block is flat yet: both orders start as the identity
-        k_block_order<<<1, 1024, 0, st>>>(s->bflat, s->border, nblocks(N));
-        k_block_order<<<1, 1024, 0, st>>>(s->gflat, s->gorder, nblocks(N));
+        k_iota<<<(unsigned)nblocks(nblocks(N)), kBlock, 0, st>>>(s->border, nblocks(N));
+        k_iota<<<(unsigned)nblocks(nblocks(N)), kBlock, 0, st>>>(s->gorder, nblocks(N));
         HCC_TRY_HIP(ctx, hipGetLastError());
         s->flat_blocks = true;
     }
@@ -3418,7 +3437,7 @@ struct StepRunner {
             if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
-                k_block_order<<<1, 1024, 0, str>>>(s->gflat, s->gorder, nb);
+                k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
                 ag.gorder = s->gorder;
