@@ -791,12 +791,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     // launch order: the stage's flat-block order for the gathers of stages 2 .. s; otherwise that of the last flux
     // pass, whose non-flat blocks are those of a state one stage earlier (a guess at the long workgroups: any order
     // gives the same results)
-    const int32_t* ord = (!CFL && !FEAT && a.gflat) ? a.gorder : a.border;
+    const int32_t* ord = (!FEAT && a.gflat) ? a.gorder : a.border;
     const int64_t lb = ord ? ord[pos] : pos;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
-    if constexpr (!CFL && !FEAT) {
-        if (a.gflat && a.gflat[lb]) {   // a flat gather block (uniform per workgroup): no halo, no slot loops
+    if constexpr (!FEAT) {
+        // a flat gather block (uniform per workgroup): no halo, no slot loops; with CFL its minimum in bmin is kept
+        if (a.gflat && a.gflat[lb]) {
             if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M, true>(a, Y, i);
             return;
         }
@@ -2438,6 +2439,10 @@ struct hcc_stepper {
     // hcc_step_cfl on a state whose stage 1 is U: the CFL estimate runs inside the stage-1 gather (cfl, cap)
     bool cfl_fused = false;
     double cfl_num = 0, cfl_cap = 0;
+    // the last step was an uncaptured fused CFL step with the zero-K bits tracked, and nothing has touched the state
+    // since: its kz bits, its last gather's cnb and the CFL block minima describe U, so the next stage-1 gather may
+    // keep a block's minimum where the block is flat (StepRunner::run)
+    bool cfl_chain = false;
     // statistics of queued steps, read when a getter asks or the ring is full (a new step does not wait)
     struct StepStats {
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -3434,7 +3439,17 @@ struct StepRunner {
             const double* Y = stage_state(i);
             if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
             if (s->feat_on) k_zero_u64<<<1, 64, 0, str>>>(s->ind_n);
-            if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
+            if (i == 1 && s->cfl_fused && a.kz && s->cfl_chain && !graph && !s->feat_on) {
+                // U = Y_s of the last step plus dt sum b_j K_j: on a block whose halo had every K of that step +-0
+                // and constant neighbourhoods in Y_s, U equals Y_s (and the last step's U) in value, so the block's
+                // gather is flat and its CFL minimum is the one the last stage-1 gather stored
+                k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << st) - 1u, s->gflat, nb);
+                k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                StepArgs ag = a;
+                ag.gflat = s->gflat;
+                ag.gorder = s->gorder;
+                ops.gather_cfl(s, ag, Y, 1, s->cfl_num, s->cfl_cap);
+            } else if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
                 k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
@@ -3531,6 +3546,7 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     r.evp = slot.evp;
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
     hipError_t e = r.run();
+    s->cfl_chain = s->cfl_fused && r.a.kz != nullptr && e == hipSuccess;
     if (e != hipSuccess) { ctx->err = std::string(who) + ": " + hipGetErrorString(e); return HCC_E_HIP; }
     if (s->moving)
         if (int rc = moving_step_end(s, r.ops)) return rc;
@@ -3591,6 +3607,7 @@ static int graph_settle(hcc_stepper* s) {
 static int graph_step(hcc_stepper* s) {
     hcc_ctx* ctx = s->ctx;
     const int st = s->desc.stages;
+    s->cfl_chain = false;
     bool stale = s->gcfl != s->cfl_num || s->gcap != s->cfl_cap;
     for (int i = 2; i <= st + 1; ++i) stale = stale || s->gbound[i] < s->gseen[i] + 1;
     if (stale) {
@@ -3761,6 +3778,7 @@ extern "C" int hcc_stepper_autotune(hcc_stepper* s, int32_t steps, double cfl, i
         HCC_TRY_HIP(ctx, hipMemcpyAsync(s->U, save, sizeof(double) * nrec, hipMemcpyDeviceToDevice, ctx->stream));
         HCC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         s->fresh = fresh0;
+        s->cfl_chain = false;   // the trial steps' zero-K bits and minima do not describe the restored state
         return HCC_OK;
     };
     const int g0 = split_base(s);
@@ -3965,6 +3983,7 @@ extern "C" int hcc_rhs(hcc_stepper* s, const double* Y, double* K, double* gradi
     const int64_t N = s->N;
     const int M = s->M, D = s->D;
     if (!N) return HCC_OK;
+    s->cfl_chain = false;   // its passes rewrite cnb for Y
     VariantOps ops;
     find_ops(s, ops);
     hipStream_t st = ctx->stream;
