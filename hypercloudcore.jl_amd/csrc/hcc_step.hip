@@ -823,13 +823,26 @@ __global__ __launch_bounds__(kBlock) void k_block_order(const uint8_t* __restric
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t per = (((nblk + kBlock - 1) / kBlock) + 15) & ~(int64_t)15;   // a multiple of 16 flags a thread
     const int64_t b0 = min(nblk, (int64_t)threadIdx.x * per), b1 = min(nblk, b0 + per);
+    // 16 flags per aligned 16-byte load (b0 is a multiple of 16; flags past nblk read as flat and are dropped)
+    auto chunk = [&](int64_t b) {
+        uint4 v = make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
+        if (b + 16 <= nblk) {
+            v = *reinterpret_cast<const uint4*>(flat + b);
+        } else {
+            uint32_t w[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
+            for (int k = 0; b + k < nblk; ++k) {
+                w[k >> 2] &= ~(0xffu << (8 * (k & 3)));
+                w[k >> 2] |= (uint32_t)flat[b + k] << (8 * (k & 3));
+            }
+            v = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        return v;
+    };
     int32_t c = 0;
-    int64_t b = b0;
-    for (; b + 16 <= b1; b += 16) {   // b0 is a multiple of 16: aligned 16-byte loads
-        const uint4 v = *reinterpret_cast<const uint4*>(flat + b);
+    for (int64_t b = b0; b < b1; b += 16) {
+        const uint4 v = chunk(b);
         c += 16 - __popc(v.x) - __popc(v.y) - __popc(v.z) - __popc(v.w);
     }
-    for (; b < b1; ++b) c += flat[b] ? 0 : 1;
     int32_t x = c;   // inclusive scan over the wave
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -846,9 +859,15 @@ __global__ __launch_bounds__(kBlock) void k_block_order(const uint8_t* __restric
     }
     int64_t nf = before + x - c;      // non-flat blocks before this range
     int64_t fl = total + b0 - nf;     // flat blocks go after all non-flat ones
-    for (int64_t q = b0; q < b1; ++q) {
-        if (flat[q]) order[fl++] = (int32_t)q;
-        else order[nf++] = (int32_t)q;
+    for (int64_t b = b0; b < b1; b += 16) {
+        const uint4 v = chunk(b);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (b + k >= b1) break;
+            if ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) order[fl++] = (int32_t)(b + k);
+            else order[nf++] = (int32_t)(b + k);
+        }
     }
 }
 
