@@ -149,9 +149,9 @@ struct StepArgs {
     uint8_t* kz;                 // bit s: the particle's K of stage s (0-based) is exactly +-0 in every component (the
                                  // fused predictor's passes write it, ghosts copy their root's; nullptr: not tracked)
     const uint8_t* gflat;        // per kStage block: the stage's full gather is flat (k_block_gflat; nullptr: not used)
-    const int32_t* border;       // the staged flux pass's blocks in launch order: non-flat ones first (k_block_order;
-                                 // nullptr: the XCD-aware order of the block indices)
-    const int32_t* gorder;       // the same for the staged gathers of stages 2 .. s (non-flat gather blocks first)
+    const int32_t* border;       // the staged flux pass's and stage-1 gather's blocks in launch order: the order of the
+                                 // stage's gather (nullptr: the XCD-aware order of the block indices)
+    const int32_t* gorder;       // the staged gathers' launch order after k_block_gflat: non-flat blocks first
 };
 
 }  // namespace hcc

@@ -2443,8 +2443,7 @@ struct hcc_stepper {
     uint8_t* bflat = nullptr;
     uint8_t* kz = nullptr;     // per particle: which stages' K are exactly zero (N, boundary particles 0xFF) ...
     uint8_t* gflat = nullptr;  // ... and the flat blocks of the gathers of stages 2 .. s
-    int32_t* border = nullptr; // launch orders of the staged flux pass and of those gathers (k_block_order)
-    int32_t* gorder = nullptr;
+    int32_t* gorder = nullptr; // launch order of the staged gathers and flux passes (k_block_order)
     bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
@@ -2554,7 +2553,9 @@ StepArgs hcc_stepper::args() const {
     // fused check in the flux pass, or k_mood_flag): the checkpoint's state is the one the stage's gather read
     a.ext_on = (desc.mood_criterion == HCC_CRIT_U1 || desc.mood_criterion == HCC_CRIT_U2) ? 1 : 0;
     a.dtp = dtp;
-    if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; a.border = border; }
+    // the flux pass launches in the order of its stage's gather (k_block_order after k_block_gflat): the blocks whose
+    // gather is not flat are, but for a few, those whose flux pass is not flat either
+    if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; a.border = gorder; }
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.feat = feat;
     a.ind_list = ind_list;
@@ -2597,7 +2598,6 @@ struct Variant {
         if (!full || !a.bflat) return;
         const int64_t nb = nblocks(s->N);
         k_block_flat<<<(unsigned)nb, kBlock, 0, st(s)>>>(a.g, a.cnb, a.bflat, nb);
-        if (a.border) k_block_order<<<1, kBlock, 0, st(s)>>>(a.bflat, const_cast<int32_t*>(a.border), nb);
     }
     // the fixed point's kernels: 64 / kSplit nodes per 64-lane workgroup, grid-strided over the device-side count;
     // up to 65536 / GPW workgroups (32 waves a CU for the first, largest lists; a cap of 16384 / GPW measured about
@@ -2902,7 +2902,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
                     (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n,
-                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat, (void*)s->border,
+                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat,
                     (void*)s->gorder})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
@@ -3056,10 +3056,8 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, dalloc(&s->gflat, nblocks(N)));
         HCC_TRY_HIP(ctx, hipMemset(s->kz, 0xFF, N));   // boundary particles without a root never change
         HCC_TRY_HIP(ctx, hipMemset(s->gflat, 0, nblocks(N)));
-        HCC_TRY_HIP(ctx, dalloc(&s->border, nblocks(N)));
         HCC_TRY_HIP(ctx, dalloc(&s->gorder, nblocks(N)));
-        // no block is flat yet: both orders start as the identity
-        k_iota<<<(unsigned)nblocks(nblocks(N)), kBlock, 0, st>>>(s->border, nblocks(N));
+        // no block is known to be flat yet: the order starts as the identity
         k_iota<<<(unsigned)nblocks(nblocks(N)), kBlock, 0, st>>>(s->gorder, nblocks(N));
         HCC_TRY_HIP(ctx, hipGetLastError());
         s->flat_blocks = true;
