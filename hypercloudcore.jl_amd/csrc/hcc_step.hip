@@ -3197,6 +3197,7 @@ struct StepRunner {
     // capture: the fused fixed point of checkpoint i runs gbound[i] iterations queued without host decisions, then
     // k_fixed_point_overflow; no pass events. Uncaptured: nonempty[i] counts the iterations that had work.
     bool graph = false;
+    bool ordered = false;   // this step has listed its launch order (from the first gather-flag kernel it ran)
     int nonempty[kMaxStages + 2] = {};
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
     // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK); not with an implicit
@@ -3462,6 +3463,7 @@ struct StepRunner {
                 // gather is flat and its CFL minimum is the one the last stage-1 gather stored
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << st) - 1u, s->gflat, nb);
                 k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                ordered = true;
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
                 ag.gorder = s->gorder;
@@ -3469,7 +3471,9 @@ struct StepRunner {
             } else if (i == 1 && s->cfl_fused) ops.gather_cfl(s, a, Y, 1, s->cfl_num, s->cfl_cap);
             else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
-                k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                // one launch order a step: the non-flat region moves by about a spacing a step
+                if (!ordered) k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                ordered = true;
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
                 ag.gorder = s->gorder;
