@@ -791,16 +791,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     // launch order: the stage's flat-block order for the gathers of stages 2 .. s; otherwise that of the last flux
     // pass, whose non-flat blocks are those of a state one stage earlier (a guess at the long workgroups: any order
     // gives the same results)
-    const int32_t* ord = (!FEAT && a.gflat) ? a.gorder : a.border;
+    const int32_t* ord = a.gflat ? a.gorder : a.border;
     const int64_t lb = ord ? ord[pos] : pos;
     const int64_t i = lb * kBlock + threadIdx.x;
     if (reset) step_reset(a, i);
-    if constexpr (!FEAT) {
-        // a flat gather block (uniform per workgroup): no halo, no slot loops; with CFL its minimum in bmin is kept
-        if (a.gflat && a.gflat[lb]) {
-            if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M, true>(a, Y, i);
-            return;
-        }
+    // a flat gather block (uniform per workgroup): no halo, no slot loops; with CFL its minimum in bmin is kept. With
+    // FEAT (the indicator on, no collection) its particles are no candidates (every neighbourhood is constant), so
+    // no feature of theirs is read and none is stored.
+    if (a.gflat && a.gflat[lb]) {
+        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M, true>(a, Y, i);
+        return;
     }
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
     double dtc = INFINITY;
@@ -2513,7 +2513,8 @@ static bool mood_fused(const hcc_stepper_desc& d) {
 static StepArgs step_args(const hcc_stepper* s) {
     StepArgs a = s->args();
     const bool fused_pred = !s->imex && !s->moving && (s->desc.mood_criterion == HCC_CRIT_NONE || mood_fused(s->desc));
-    if (s->flat_blocks && s->tune.staged && s->lds_gather && !s->feat_on && fused_pred) a.kz = s->kz;
+    // zero-K bits (the flat gather blocks): not while training data is collected (its copies take every feature)
+    if (s->flat_blocks && s->tune.staged && s->lds_gather && !s->collect && fused_pred) a.kz = s->kz;
     return a;
 }
 
@@ -3457,7 +3458,7 @@ struct StepRunner {
             const double* Y = stage_state(i);
             if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
             if (s->feat_on) k_zero_u64<<<1, 64, 0, str>>>(s->ind_n);
-            if (i == 1 && s->cfl_fused && a.kz && s->cfl_chain && !graph && !s->feat_on) {
+            if (i == 1 && s->cfl_fused && a.kz && s->cfl_chain && !graph) {
                 // U = Y_s of the last step plus dt sum b_j K_j: on a block whose halo had every K of that step +-0
                 // and constant neighbourhoods in Y_s, U equals Y_s (and the last step's U) in value, so the block's
                 // gather is flat and its CFL minimum is the one the last stage-1 gather stored
