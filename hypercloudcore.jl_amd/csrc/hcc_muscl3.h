@@ -505,8 +505,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
 // or the direct degree-1 fit with intercept (compute_dynamic_divergence -> _mls_divergence, :308-346); K_E = -2 div
 // ------------------------------------------------------------------------------------------------------------
 template <int D, int M, int KIND>
-__device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + M], const double (&fi)[M],
-                                      const double (&gi)[L3<D, M>::NG], const double (&Fi)[D][M], int oi,
+// The particle's own state fi, gradient record gi and flux Fi (flat, axis major) come from registers or, in the wide
+// kernels, from the group's LDS copy.
+__device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + M], const double* fi,
+                                      const double* gi, const double* Fi, int oi,
                                       const double (&rj)[D + M], const double (&gj)[L3<D, M>::NG], int oj,
                                       double (&dd)[D], double& w, double (&dF)[D][M]) {
     using P = Pde<D, M, KIND>;
@@ -548,7 +550,7 @@ __device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + 
         for (int m = 0; m < M; ++m) {
             const double x = fji[m] - fij[m];
             const double Fnum = 0.5 * (Fij[d][m] + Fji[d][m] - sds * x);
-            dF[d][m] = Fnum - Fi[d][m];
+            dF[d][m] = Fnum - Fi[d * M + m];
         }
     }
 }
@@ -594,7 +596,7 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
                 src.template head<NR>(j, rj);
                 src.grad(j, gj);
                 const int oj = src.order(j);
-                slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
+                slot3<D, M, KIND>(a, ri, fi, gi, &Fi[0][0], oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
                 double ds[D], t[BD - 1];
 #pragma unroll
                 for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
@@ -641,7 +643,7 @@ __device__ __forceinline__ bool flux_node(const StepArgs& a, const PredArgs& pre
             src.template head<NR>(j, rj);
             src.grad(j, gj);
             const int oj = src.order(j);
-            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
+            slot3<D, M, KIND>(a, ri, fi, gi, &Fi[0][0], oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
             double p[B1];
             p[0] = 1.0;
 #pragma unroll
@@ -847,6 +849,7 @@ struct Split3 {
     static constexpr int BD = L3<D, M>::BD, DM = D * M, B1 = D + 1;
     static constexpr int SR = BD + DM;        // per slot: w, the divergence basis (BD - 1 terms), dF (axis major)
     static constexpr int XS = B1 * D + 1;     // the direct fit's solved derivative columns, and whether it solved
+    static constexpr int NS = L3<D, M>::NG + D * M;   // the particle's gradient record and flux
     static_assert(DM < G, "one fit column per lane, and a lane for the moment matrix");
 };
 // the wide variants (register-resident accumulators of flux_node would spill)
@@ -856,7 +859,7 @@ constexpr bool kWide3 = L3<D, M>::BD * D * M > 160;
 template <int D, int M, int KIND, bool CHECK>
 __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
                                             double* __restrict__ K, int64_t i, bool take, double h,
-                                            double (*rows)[Split3<D, M>::SR], double* xs, double* part) {
+                                            double (*rows)[Split3<D, M>::SR], double* xs, double* part, double* nsh) {
     using L = L3<D, M>;
     using S = Split3<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = L::BD, DM = S::DM, B1 = S::B1, G = S::G;
@@ -865,21 +868,16 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
     const double invL = a.g.invL;
     int cnt = 0, mode = 0;   // mode 1: the cached fit over the slots; 2: the direct degree-1 fit; 0: no slot loop
     int oi = 0;
-    double ri[NR], fi[M], gi[NG], Fi[D][M];
+    double ri[NR], fi[M];
 #pragma unroll
     for (int q = 0; q < NR; ++q) ri[q] = 0.0;
-#pragma unroll
-    for (int q = 0; q < NG; ++q) gi[q] = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-#pragma unroll
-        for (int m = 0; m < M; ++m) Fi[d][m] = 0.0;
     int64_t base = 0;
+    // the particle's gradient record and flux, shared by the group's lanes: nsh[0 .. NG) and nsh[NG .. NG + D M)
     if (take) {
         cnt = a.g.cnt[i];
         load_rec<NR, L::PU>(Y, i, ri);
         if (cnt > 0) {
-            load_rec<NG, L::GS>(a.G, i, gi);
+            for (int q = gl; q < NG; q += G) nsh[q] = a.G[i * L::GS + q];
             base = sell_base(a.g, i);
             oi = a.orders[i];
             int oi_t = oi;
@@ -891,7 +889,15 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
     }
 #pragma unroll
     for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
-    if (mode) Pde<D, M, KIND>::flux(a.pde, fi, Fi);
+    if (mode && gl == 0) {
+        double Fi[D][M];
+        Pde<D, M, KIND>::flux(a.pde, fi, Fi);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int m = 0; m < M; ++m) nsh[NG + d * M + m] = Fi[d][m];
+    }
+    __syncthreads();
     int mc = mode ? cnt : 0;   // the wave's largest slot count: a uniform chunk loop
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
@@ -911,7 +917,7 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
             load_rec<NR, L::PU>(Y, j, rj);
             load_rec<NG, L::GS>(a.G, j, gj);
             const int oj = a.orders[j];
-            slot3<D, M, KIND>(a, ri, fi, gi, Fi, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
+            slot3<D, M, KIND>(a, ri, fi, nsh, nsh + NG, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
             double ds[D], t[BD - 1];
 #pragma unroll
             for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
@@ -1012,12 +1018,13 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
 
 // the full (or dirty-only) flux pass of the wide variants: four particles a workgroup, XCD-aware order of the quads
 template <int D, int M, int KIND, bool CHECK>
-__global__ __launch_bounds__(64) void k_flux_wide(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+__global__ __launch_bounds__(64, 2) void k_flux_wide(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                  double* __restrict__ K, int full, int64_t nquad) {
     using S = Split3<D, M>;
     __shared__ double rows[S::GPW][S::G][S::SR];
     __shared__ double xs[S::GPW][S::XS];
     __shared__ double part[S::GPW][S::DM];
+    __shared__ double nsh[S::GPW][S::NS];
     const int64_t lq = xcd_block(blockIdx.x, nquad);
     if (lq >= nquad) return;
     const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
@@ -1034,7 +1041,7 @@ __global__ __launch_bounds__(64) void k_flux_wide(StepArgs a, PredArgs pred, con
             if (CHECK) h = a.hratio * node_dt(a.dtp, a.inst, i);
         }
     }
-    const bool ch = flux_group3<D, M, KIND, CHECK>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp]);
+    const bool ch = flux_group3<D, M, KIND, CHECK>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp]);
     if (CHECK) {
         const int bad = (take && gl == 0) ? a.inadm[i] : 0;
         append_changed(a, ch, i);
@@ -1044,12 +1051,13 @@ __global__ __launch_bounds__(64) void k_flux_wide(StepArgs a, PredArgs pred, con
 
 // the fixed point's flux pass of the wide variants over its work list (the bookkeeping of k_flux_list)
 template <int D, int M, int KIND, bool FP = true>
-__global__ __launch_bounds__(64) void k_flux_wide_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+__global__ __launch_bounds__(64, 2) void k_flux_wide_list(StepArgs a, PredArgs pred, const double* __restrict__ Y,
                                                       double* __restrict__ K, int p) {
     using S = Split3<D, M>;
     __shared__ double rows[S::GPW][S::G][S::SR];
     __shared__ double xs[S::GPW][S::XS];
     __shared__ double part[S::GPW][S::DM];
+    __shared__ double nsh[S::GPW][S::NS];
     const int64_t n = (int64_t)*work_count(a, p);
     if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
@@ -1067,7 +1075,7 @@ __global__ __launch_bounds__(64) void k_flux_wide_list(StepArgs a, PredArgs pred
             }
         }
         const double h = FP && take ? a.hratio * node_dt(a.dtp, a.inst, i) : 0.0;
-        const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp]);
+        const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp]);
         if constexpr (!FP) continue;
         const int delta = (take && gl == 0) ? (int)a.inadm[i] - before : 0;
         if (t < n && gl == 0) {
