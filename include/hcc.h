@@ -117,8 +117,8 @@ typedef struct {
     int32_t pde;               /* HCC_PDE_* */
     double gamma;              /* Euler: ratio of specific heats */
     double adv_speed[3][8];    /* linear advection: speed of component m along axis d (diagonal system) */
-    int32_t muscl_order;       /* MUSCL max_order: 2, or 3 (advection (1,1), (2,1), (2,2), 2-D Euler; buffered
-                                  RusanovFlux on static grids) */
+    int32_t muscl_order;       /* MUSCL max_order: 2, or 3 (advection (1,1), (2,1), (2,2), (3,1), 2-D and 3-D
+                                  Euler; buffered RusanovFlux on static grids) */
     int32_t limiter, limiter_mode;
     int32_t mood_strategy, mood_criterion;
     double mood_delta;         /* MOODu1 / MOODu2 d */
