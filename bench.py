@@ -259,6 +259,42 @@ def reference_step_run(ctx, steps=30, warmup=3, only=None):
             "cases": cases}
 
 
+def cube_run(ctx, steps=5, warmup=1):
+    """The 3-D Euler cube with MLS order 3 (the pressure pulse on the jittered periodic 160^3 cube, R = 2.2 h,
+    MUSCL3 + VK(hard) + Rusanov + MOOD(EPD1, u1), SSP-RK3, CFL 0.3): warm-up and timed CFL steps from t = 0."""
+    import hcc_hip
+    from hcc_hip import Grid, Stepper
+    from hcc_hip import workloads as W
+    sync = lambda: hcc_hip.lib().hcc_ctx_synchronize(ctx.h)   # noqa: E731
+    t_setup = time.perf_counter()
+    wl = W.euler_3d(n=160, seed=0)
+    wl["scheme"] = dict(wl["scheme"], muscl_order=3)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    sync()
+    setup_s = time.perf_counter() - t_setup
+    for _ in range(warmup):
+        st.step_cfl(wl["cfl"])
+    sync()
+    st.totals(reset=True)
+    wall = timed_steps(st, steps, wl["cfl"], sync)
+    n_steps, iters, tot_ms, tot_g, tot_f = st.totals(reset=True)
+    U = st.download()
+    out = {"workload": "3-D Euler pressure pulse (rest state, amplitude 0.5, width 0.05) on the jittered periodic "
+                       "160^3 cube, R = 2.2 h; MUSCL3 (degree-2 reconstruction, degree-3 divergence) + VK(hard) + "
+                       f"Rusanov + MOOD(EPD1, u1), SSP-RK3, CFL 0.3; {warmup} warm-up and {steps} timed CFL steps "
+                       "from t = 0 (MLS order 3 of config[2] with MUSCL in place of meshless WENO)",
+           "nodes": g.N, "mean_neighbours": round(g.S / g.N, 1), "setup_s": round(setup_s, 1),
+           "value": g.N * 3 * steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "flux_avg_ms": tot_f / (3 * n_steps), "gather_avg_ms": tot_g / (3 * n_steps), "mood_iterations": iters,
+           "finite": bool(np.all(np.isfinite(U)))}
+    st.close()
+    g.close()
+    return out
+
+
 def graph_compare(args):
     """hcc_step_cfl uncaptured against hipGraph-captured steps (hcc_stepper_set_graph) on the four-quadrant problem at
     each h = 1/n: the same 5 warm-up and `steps` timed CFL steps from t = 0, and a bit-for-bit comparison of the final
@@ -481,6 +517,7 @@ def main():
     ap.add_argument("--reference-step", action="store_true",
                     help="only the reference's own step benchmark (benchmark/step.jl, MUSCL3 advection)")
     ap.add_argument("--no-reference-step", action="store_true", help="leave out the reference step-benchmark block")
+    ap.add_argument("--no-cube", action="store_true", help="leave out the 3-D MLS-order-3 cube block")
     ap.add_argument("--reference-case", default=None, metavar="N,M,STAB",
                     help="with --reference-step: only this case (e.g. 1000,1,both)")
     ap.add_argument("--graph-compare", type=int, nargs="+", default=None, metavar="N",
@@ -587,12 +624,11 @@ def main():
     model_b = W.bytes_per_node_stage(D, M, nnb)
 
     if args.timed_only:
-        args.no_ensemble = args.no_cpu_baseline = args.no_reference_step = True
-        args.run_steps = 0
+        args.no_ensemble = args.no_cpu_baseline = args.no_reference_step = args.no_cube = True
     # the whole configured run from t = 0 (run_steps CFL steps), timed as one region
     st.upload(wl["U0"])
     sync()
-    wall_run = timed_steps(st, args.run_steps, cfl, sync) if args.run_steps else float("nan")
+    wall_run = timed_steps(st, args.run_steps, cfl, sync) if not args.timed_only else float("nan")
     # the dense state: the four quadrants times a smooth factor, so that no neighbourhood is constant and every
     # slot of every pass does its full arithmetic (the same cloud and scheme)
     x = c["pos"]
@@ -667,6 +703,8 @@ def main():
         out["ensemble"] = ensemble_run(ea, ctx)
     if not args.no_reference_step:
         out["reference_step_benchmark"] = reference_step_run(ctx)
+    if not args.no_cube:
+        out["cube_3d_mls3"] = cube_run(ctx)
     if not args.no_cpu_baseline:
         # the oracle on the host cores of this process: all of them available to it, at most the 16 that a one-GPU
         # share of the box holds
