@@ -159,10 +159,13 @@ struct alignas(16) FwdLds {
 // over groups of four tiles, so every wave reaches each barrier (a tile past the end computes on zeros, unstored).
 // With idx, sample s is feature row idx[s] and n is read from *n_dev.
 constexpr int kFwdWaves = 4;
+// apply (not null): the indicator's decision fused in: a listed particle whose logit exceeds thr takes order 1, and
+// the count of such particles is added to apply_count (one atomic per wave with a hit)
 __global__ __launch_bounds__(64 * kFwdWaves) void k_mlp_forward(const MlpW* __restrict__ gw, const __bf16* __restrict__ X,
                                                                 int64_t n_host, const int32_t* __restrict__ idx,
                                                                 const unsigned long long* __restrict__ n_dev,
-                                                                float* __restrict__ z) {
+                                                                float* __restrict__ z, int8_t* __restrict__ apply,
+                                                                float thr, unsigned long long* __restrict__ apply_count) {
     __shared__ MlpW w;
     __shared__ FwdLds tiles[kFwdWaves];
     load_weights(w, gw);
@@ -176,7 +179,19 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_mlp_forward(const MlpW* __re
         load_x(t, nullptr, X, n, tile, l, idx);
         float a1[2][16], a2[2][16], h2[2][16];
         forward_tile(w, t, nullptr, l, a1, a2, h2);
-        if (l < T && tile * T + l < n) z[tile * T + l] = t.z[l];
+        bool hit = false;
+        if (l < T && tile * T + l < n) {
+            const float zl = t.z[l];
+            z[tile * T + l] = zl;
+            if (apply && zl > thr) {
+                apply[idx[tile * T + l]] = 1;
+                hit = true;
+            }
+        }
+        if (apply) {
+            const unsigned long long b = __ballot(hit);
+            if (b && l == (int)(__ffsll((unsigned long long)b) - 1)) atomicAdd(apply_count, (unsigned long long)__popcll(b));
+        }
         __syncthreads();
     }
 }
@@ -353,14 +368,17 @@ void mlp_forward_device(const hcc_mlp* m, const uint16_t* X, int64_t n, float* z
     const int64_t ngroup = ((n + T - 1) / T + kFwdWaves - 1) / kFwdWaves;
     const unsigned nb = (unsigned)std::min<int64_t>(ngroup, 4096);
     k_mlp_forward<<<nb, 64 * kFwdWaves, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), n, nullptr,
-                                                             nullptr, z);
+                                                             nullptr, z, nullptr, 0.0f, nullptr);
 }
 void mlp_forward_list(const hcc_mlp* m, const uint16_t* X, const int32_t* idx, const unsigned long long* n_dev,
-                      int64_t n_max, float* z) {
+                      int64_t n_max, float* z, int8_t* apply, float thr, unsigned long long* apply_count) {
     if (n_max <= 0) return;
+    // the list's length is on the device: up to 1024 workgroups looping over its tiles (a quarter of that measured
+    // slower: a tile's forward pass is a chain of LDS round trips, so the tiles want waves in flight)
     const int64_t ngroup = ((n_max + T - 1) / T + kFwdWaves - 1) / kFwdWaves;
     const unsigned nb = (unsigned)std::min<int64_t>(ngroup, 1024);
-    k_mlp_forward<<<nb, 64 * kFwdWaves, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), 0, idx, n_dev, z);
+    k_mlp_forward<<<nb, 64 * kFwdWaves, 0, m->ctx->stream>>>(m->w, reinterpret_cast<const __bf16*>(X), 0, idx, n_dev, z,
+                                                             apply, thr, apply_count);
 }
 }  // namespace hcc
 

@@ -2212,24 +2212,6 @@ __global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ b
     }
 }
 
-// The indicator's decision before a stage's flux pass: the candidates (interior particles at the full order with a
-// non-constant neighbourhood) whose logit exceeds the threshold take first order now (MOOD's check of the stage
-// candidate still follows). z[t] is the logit of candidate t; grid-strided over the device-side count.
-__global__ __launch_bounds__(kBlock) void k_apply_indicator(int8_t* __restrict__ orders, const int32_t* __restrict__ list,
-                                                             const unsigned long long* __restrict__ list_n,
-                                                             const float* __restrict__ z, float thr,
-                                                             unsigned long long* count) {
-    const int64_t n = (int64_t)*list_n;
-    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < n; t0 += (int64_t)gridDim.x * kBlock) {
-        const int64_t t = t0 + threadIdx.x;
-        const bool hit = t < n && z[t] > thr;
-        if (hit) orders[list[t]] = 1;
-        const unsigned long long b = __ballot(hit);
-        if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)b) - 1))
-            atomicAdd(count, (unsigned long long)__popcll(b));
-    }
-}
-
 __global__ void k_set_dt(double* dtp, double dt, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) dtp[k] = dt;
 }
@@ -3295,9 +3277,8 @@ struct StepRunner {
             snap_orders(i - 1);
         }
         if (s->ind) {   // the network over the candidates only (their count stays on the device)
-            mlp_forward_list(s->ind, s->feat, s->ind_list, s->ind_n, s->N, s->ind_z);
-            k_apply_indicator<<<(unsigned)std::min<int64_t>(nblocks(s->N), 2048), kBlock, 0, str>>>(
-                s->orders, s->ind_list, s->ind_n, s->ind_z, s->ind_thr, s->ind_cnt);
+            // logits, and the decision applied to the orders in the same kernel
+            mlp_forward_list(s->ind, s->feat, s->ind_list, s->ind_n, s->N, s->ind_z, s->orders, s->ind_thr, s->ind_cnt);
         }
         return hipGetLastError();
     }
