@@ -810,8 +810,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     if (CFL) block_min(a, dtc, lb);
 }
 
-// The launch order of a staged pass's blocks from their flat flags (bflat or gflat, 0 or 1): the non-flat blocks,
-// whose workgroups stage a halo and run slot loops, first, then the flat ones, each group in index order. The hardware
+// The launch order of the staged passes' blocks from the gather's flat flags (gflat, 0 or 1): the non-flat blocks,
+// whose workgroups stage a halo and run slot loops, first, then the flat ones, each group in index order. One list a
+// step serves its gathers and flux passes (a block whose gather is not flat is, but for a few, one whose flux pass is
+// not flat either). The hardware
 // hands out workgroups in launch order as CUs free up, so the long workgroups start first and the short flat ones
 // fill the gaps at the end (longest first), instead of a long one starting late and finishing alone; the XCD-aware
 // mapping then applies to positions in this list. One workgroup: each thread counts a contiguous range of flags
