@@ -152,7 +152,13 @@ struct StepArgs {
     const int32_t* border;       // the staged flux pass's and stage-1 gather's blocks in launch order: the order of the
                                  // stage's gather (nullptr: the XCD-aware order of the block indices)
     const int32_t* gorder;       // the staged gathers' launch order after k_block_gflat: non-flat blocks first
+    const uint8_t* zfit;         // per particle, geometry only (k_zero_fits): what its fits give for all-zero
+                                 // accumulators, so that the flat paths need not read the inverses (kZ* bits)
 };
+// bits of StepArgs::zfit
+constexpr unsigned kZDiv = 1u;    // the divergence fit of zero accumulators is -0 in every component (before sources)
+constexpr unsigned kZGrad = 2u;   // the gradient fit of zero accumulators is +-0 in every row: bit kZSign0 + q is
+constexpr unsigned kZSign0 = 2;   // row q's sign (the same for every component)
 
 }  // namespace hcc
 

@@ -711,7 +711,13 @@ __device__ __forceinline__ void gather_node_flat(const StepArgs& a, const double
     double g[L::NG];
 #pragma unroll
     for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
-    if (order == 2 && a.okg[i]) {
+    const unsigned zf = a.zfit ? a.zfit[i] : 0u;
+    if (order == 2 && (zf & kZGrad)) {   // the rows' zeros as the fit below forms them (k_zero_fits)
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) g[q * M + m] = ((zf >> (kZSign0 + q)) & 1u) ? -0.0 : 0.0;
+    } else if (order == 2 && a.okg[i]) {
         const double invL = a.g.invL;
 #pragma unroll
         for (int q = 0; q < D; ++q) {
@@ -910,6 +916,45 @@ __global__ __launch_bounds__(kBlock) void k_block_gflat(GeomArgs g, const uint8_
     }
     ok = __syncthreads_and(ok);
     if (threadIdx.x == 0) gflat[lb] = (uint8_t)ok;
+}
+
+// What each particle's fits give for all-zero accumulators (geometry only; order-2 flat paths): the divergence
+// fit's K before sources, -(2 (acc invL)) with acc = 0 + sum_d (sum_c invd * 0) (-(2.0 * 0.0) without a cached
+// fit), and each gradient row (sum_r invg * 0) invL, formed with the operations of fit_divergence and gather_node.
+// kZDiv when K is -0 in every component; kZGrad (with each row's sign bit) when every row is a zero: a non-finite
+// inverse or invL leaves the bit clear and the flat paths form the fit.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_zero_fits(const double* __restrict__ invg, const uint8_t* __restrict__ okg,
+                                                       const double* __restrict__ invd, const uint8_t* __restrict__ okd,
+                                                       double invL, uint8_t* __restrict__ zfit, int64_t N) {
+    constexpr int BD = D + 1;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    unsigned z = 0;
+    double kd = -(2.0 * 0.0);
+    if (okd[i]) {
+        double acc = 0.0;
+        for (int d = 0; d < D; ++d) {
+            double r = invd[(d * BD + 0) * N + i] * 0.0;
+            for (int c = 1; c < BD; ++c) r = r + invd[(d * BD + c) * N + i] * 0.0;
+            acc = acc + r;
+        }
+        kd = -(2.0 * (acc * invL));
+    }
+    if (kd == 0.0 && signbit(kd)) z |= kZDiv;
+    bool rows = true;
+    for (int q = 0; q < D; ++q) {
+        double c = 0.0;
+        if (okg[i]) {
+            c = invg[(q * D + 0) * N + i] * 0.0;
+            for (int r = 1; r < D; ++r) c = c + invg[(q * D + r) * N + i] * 0.0;
+            c = c * invL;
+        }
+        rows = rows && c == 0.0;
+        if (signbit(c)) z |= 1u << (kZSign0 + q);
+    }
+    if (rows) z |= kZGrad;
+    zfit[i] = (uint8_t)z;
 }
 
 // The identity order (no block known to be flat yet)
@@ -1211,8 +1256,10 @@ __device__ __forceinline__ void fit_divergence(const StepArgs& a, int64_t i, con
 }
 
 // After K_E of node i: the store, the fused stage predictor and (CHECK) MOOD's candidate check with the decrement
-// deferred to k_mark_gather. Returns whether the node's order is to drop.
-template <int D, int M, int KIND, bool CHECK>
+// deferred to k_mark_gather. Returns whether the node's order is to drop. CNB: node i has a constant neighbourhood in
+// the checked state (cnb), so its stored extrema equal U_i in value and the DMP bounds are taken from fi (the
+// comparisons see values only; a NaN U_i has no constant neighbourhood).
+template <int D, int M, int KIND, bool CHECK, bool CNB = false>
 __device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pred, double* __restrict__ K, int64_t i,
                                           const double (&fi)[M], const double (&Kout)[M], double h) {
     using P = Pde<D, M, KIND>;
@@ -1243,10 +1290,16 @@ __device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pre
         } else if (a.criterion == HCC_CRIT_U1) {
             // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
             // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
-            double e[2 * M], mx[M], mn[M];
-            load_rec<2 * M, 2 * M>(a.ext, i, e);
+            double mx[M], mn[M];
+            if constexpr (CNB) {
 #pragma unroll
-            for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+                for (int m = 0; m < M; ++m) { mx[m] = fi[m]; mn[m] = fi[m]; }
+            } else {
+                double e[2 * M];
+                load_rec<2 * M, 2 * M>(a.ext, i, e);
+#pragma unroll
+                for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+            }
             const double dl = a.delta, d3 = dl * dl * dl;
 #pragma unroll
             for (int m = 0; m < M; ++m)
@@ -1359,7 +1412,8 @@ __device__ __forceinline__ bool flux_node_flat(const StepArgs& a, const PredArgs
     load_rec<L::NR, L::PU>(Y, i, ri);
 #pragma unroll
     for (int m = 0; m < M; ++m) fi[m] = ri[D + m];
-    if (a.g.cnt[i] == 0) {
+    // the fit of the zero accumulators is -0 before sources (kZDiv) or there is no fit: no inverse is read
+    if ((a.zfit && (a.zfit[i] & kZDiv)) || a.g.cnt[i] == 0) {
 #pragma unroll
         for (int m = 0; m < M; ++m) Kout[m] = -0.0 + explicit_source<M>(a, i, fi, m);
     } else {
@@ -1370,7 +1424,7 @@ __device__ __forceinline__ bool flux_node_flat(const StepArgs& a, const PredArgs
             for (int q = 0; q < D * M; ++q) b[c][q] = 0.0;
         fit_divergence<D, M>(a, i, b, fi, Kout);
     }
-    return flux_tail<D, M, KIND, CHECK>(a, pred, K, i, fi, Kout, h);
+    return flux_tail<D, M, KIND, CHECK, true>(a, pred, K, i, fi, Kout, h);
 }
 
 // The lists of the fused fixed point alternate between two copies, so that no kernel resets a count another
@@ -2428,6 +2482,7 @@ struct hcc_stepper {
     uint8_t* kz = nullptr;     // per particle: which stages' K are exactly zero (N, boundary particles 0xFF) ...
     uint8_t* gflat = nullptr;  // ... and the flat blocks of the gathers of stages 2 .. s
     int32_t* gorder = nullptr; // launch order of the staged gathers and flux passes (k_block_order)
+    uint8_t* zfit = nullptr;   // what the fits give for zero accumulators (k_zero_fits, with the moments)
     bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
     size_t lds_gather = 0, lds_flux = 0, lds_cfl = 0, lds_gcfl = 0;
@@ -2541,6 +2596,7 @@ StepArgs hcc_stepper::args() const {
     // the flux pass launches in the order of its stage's gather (k_block_order after k_block_gflat): the blocks whose
     // gather is not flat are, but for a few, those whose flux pass is not flat either
     if (flat_blocks && tune.staged) { a.cnb = cnb; a.bflat = bflat; a.border = gorder; }
+    a.zfit = zfit;
     a.inst = grid->n_inst > 1 ? grid->d_blk_inst : nullptr;
     a.feat = feat;
     a.ind_list = ind_list;
@@ -2875,6 +2931,12 @@ static void launch_moments(hcc_stepper* s, double R_cut, double* dw = nullptr) {
     if (s->D == 1) k_moments<1><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
     else if (s->D == 2) k_moments<2><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
     else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
+    if (s->zfit) {
+        const double invL = ma.g.invL;
+        if (s->D == 1) k_zero_fits<1><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
+        else if (s->D == 2) k_zero_fits<2><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
+        else k_zero_fits<3><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
+    }
 }
 
 extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
@@ -2887,7 +2949,7 @@ extern "C" int hcc_stepper_destroy(hcc_stepper* s) {
                     (void*)s->lam, (void*)s->bmin, (void*)s->ext, (void*)s->snap, (void*)s->nbmin, (void*)s->mask2,
                     (void*)s->rpart, (void*)s->range, (void*)s->dtp, (void*)s->counters, (void*)s->feat, (void*)s->ind_z,
                     (void*)s->ind_cnt, (void*)s->cfeat, (void*)s->cord, (void*)s->ind_list, (void*)s->ind_n,
-                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat,
+                    (void*)s->cnb, (void*)s->bflat, (void*)s->kz, (void*)s->gflat, (void*)s->zfit,
                     (void*)s->gorder})
         (void)hipFree(p);
     for (int k = 0; k < kMaxStages; ++k) {
@@ -3042,6 +3104,7 @@ extern "C" int hcc_stepper_create(hcc_ctx* ctx, hcc_grid* g, const hcc_stepper_d
         HCC_TRY_HIP(ctx, hipMemset(s->kz, 0xFF, N));   // boundary particles without a root never change
         HCC_TRY_HIP(ctx, hipMemset(s->gflat, 0, nblocks(N)));
         HCC_TRY_HIP(ctx, dalloc(&s->gorder, nblocks(N)));
+        HCC_TRY_HIP(ctx, dalloc(&s->zfit, N));   // formed with the moments
         // no block is known to be flat yet: the order starts as the identity
         k_iota<<<(unsigned)nblocks(nblocks(N)), kBlock, 0, st>>>(s->gorder, nblocks(N));
         HCC_TRY_HIP(ctx, hipGetLastError());
