@@ -159,6 +159,7 @@ struct StepArgs {
 constexpr unsigned kZDiv = 1u;    // the divergence fit of zero accumulators is -0 in every component (before sources)
 constexpr unsigned kZGrad = 2u;   // the gradient fit of zero accumulators is +-0 in every row: bit kZSign0 + q is
 constexpr unsigned kZSign0 = 2;   // row q's sign (the same for every component)
+constexpr unsigned kZStarved = 32u;   // fewer than D neighbours
 
 }  // namespace hcc
 

@@ -698,20 +698,23 @@ __device__ __forceinline__ void append_cand(const StepArgs& a, bool c, int64_t i
 }
 
 // gather_node for a particle of a flat gather block (k_block_gflat): every neighbour's state equals U_i, so the fit
-// accumulators stay +0 and the gradient is the inverse times zero (g = 0 without a fit), which no slot limits; the
-// extrema are U_i's state (value-equal to what the neighbour loop folds; only the signs of zeros could differ, which the
-// DMP comparisons cannot see). The starved-order rule and the stores are those of gather_node.
-template <int D, int M, bool EXT>
-__device__ __forceinline__ void gather_node_flat(const StepArgs& a, const double* __restrict__ Y, int64_t i) {
+// accumulators stay +0 and the gradient is the inverse times zero (g = 0 without a fit; the zeros' signs from zfit),
+// which no slot limits. The extrema are not stored again: the particle had a constant neighbourhood (cnb) in the state
+// of the full gather that last stored its cnb and its extrema, and this state repeats that state's values (every K
+// since then is +-0), so the stored extrema are U_i's value in both halves, value-equal to what the neighbour loop
+// would fold (only signs of zeros could differ, which the DMP comparisons cannot see). The starved-order rule and the
+// gradient store are those of gather_node.
+template <int D, int M>
+__device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i) {
     using L = Layout<D, M>;
     const int64_t N = a.g.N;
-    const int cnt = a.g.cnt[i];
+    const unsigned zf = a.zfit ? a.zfit[i] : 0u;
+    const bool starved = a.zfit ? (zf & kZStarved) != 0u : a.g.cnt[i] < D;
     int order = a.orders[i];
-    if (order > 1 && cnt < D) { order = 1; a.orders[i] = 1; }   // starved neighbourhood (basis_count(D, 1) = D)
+    if (order > 1 && starved) { order = 1; a.orders[i] = 1; }   // starved neighbourhood (basis_count(D, 1) = D)
     double g[L::NG];
 #pragma unroll
     for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
-    const unsigned zf = a.zfit ? a.zfit[i] : 0u;
     if (order == 2 && (zf & kZGrad)) {   // the rows' zeros as the fit below forms them (k_zero_fits)
 #pragma unroll
         for (int q = 0; q < D; ++q)
@@ -734,13 +737,6 @@ __device__ __forceinline__ void gather_node_flat(const StepArgs& a, const double
         }
     }
     store_rec<L::NG, L::GS>(a.G, i, g);
-    if (EXT && a.ext_on) {
-        double ri[L::NR], e[2 * M];
-        load_rec<L::NR, L::PU>(Y, i, ri);
-#pragma unroll
-        for (int m = 0; m < M; ++m) { e[m] = ri[D + m]; e[M + m] = ri[D + m]; }
-        store_rec<2 * M, 2 * M>(a.ext, i, e);
-    }
 }
 
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
@@ -805,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     // FEAT (the indicator on, no collection) its particles are no candidates (every neighbourhood is constant), so
     // no feature of theirs is read and none is stored.
     if (a.gflat && a.gflat[lb]) {
-        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M, true>(a, Y, i);
+        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M>(a, i);
         return;
     }
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
@@ -922,11 +918,13 @@ __global__ __launch_bounds__(kBlock) void k_block_gflat(GeomArgs g, const uint8_
 // fit's K before sources, -(2 (acc invL)) with acc = 0 + sum_d (sum_c invd * 0) (-(2.0 * 0.0) without a cached
 // fit), and each gradient row (sum_r invg * 0) invL, formed with the operations of fit_divergence and gather_node.
 // kZDiv when K is -0 in every component; kZGrad (with each row's sign bit) when every row is a zero: a non-finite
-// inverse or invL leaves the bit clear and the flat paths form the fit.
+// inverse or invL leaves the bit clear and the flat paths form the fit. kZStarved: fewer than D neighbours (the
+// starved-order rule).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_zero_fits(const double* __restrict__ invg, const uint8_t* __restrict__ okg,
                                                        const double* __restrict__ invd, const uint8_t* __restrict__ okd,
-                                                       double invL, uint8_t* __restrict__ zfit, int64_t N) {
+                                                       const int32_t* __restrict__ cnt, double invL,
+                                                       uint8_t* __restrict__ zfit, int64_t N) {
     constexpr int BD = D + 1;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= N) return;
@@ -954,6 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_zero_fits(const double* __restrict__
         if (signbit(c)) z |= 1u << (kZSign0 + q);
     }
     if (rows) z |= kZGrad;
+    if (cnt[i] < D) z |= kZStarved;
     zfit[i] = (uint8_t)z;
 }
 
@@ -2933,9 +2932,9 @@ static void launch_moments(hcc_stepper* s, double R_cut, double* dw = nullptr) {
     else k_moments<3><<<grid_for(nb), kBlock, 0, st>>>(ma, nb);
     if (s->zfit) {
         const double invL = ma.g.invL;
-        if (s->D == 1) k_zero_fits<1><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
-        else if (s->D == 2) k_zero_fits<2><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
-        else k_zero_fits<3><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, invL, s->zfit, s->N);
+        if (s->D == 1) k_zero_fits<1><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, ma.g.cnt, invL, s->zfit, s->N);
+        else if (s->D == 2) k_zero_fits<2><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, ma.g.cnt, invL, s->zfit, s->N);
+        else k_zero_fits<3><<<(unsigned)nb, kBlock, 0, st>>>(s->invg, s->okg, s->invd, s->okd, ma.g.cnt, invL, s->zfit, s->N);
     }
 }
 
