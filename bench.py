@@ -117,7 +117,7 @@ def ensemble_run(args, ctx=None):
         st_i.upload(U0)
         st_i.set_indicator(net, IND.DEFAULT_THRESHOLD)
         if tuning:
-            st_i.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"])
+            st_i.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"], tuning["dense"])
     nodes = int(ens.sizes.sum())
     t_setup = time.perf_counter() - t_setup
     print(f"# ensemble setup {t_setup:.1f} s: {len(wls)} members, {nodes} nodes (ensemble grid {ens.N} with padding)",
@@ -147,7 +147,7 @@ def ensemble_run(args, ctx=None):
         st_g = Stepper(ctx, ens, wls[0]["scheme"])
         st_g.upload(U0)
         if tuning:
-            st_g.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"])
+            st_g.set_tuning(tuning["staged"], tuning["split"], tuning["sweep"], tuning["fuse_cfl"], tuning["dense"])
         st_g.set_graph(True)
         wall_g = timed([st_g])
         assert np.array_equal(st_g.download(), st_e.download()), "captured batched steps differ"

@@ -758,6 +758,18 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
         __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     const int lane = threadIdx.x;
+    int nl = 0;   // entries in lbuf over all the workgroup's items (uniform), appended when it fills and at the end
+    auto flush = [&]() {
+        __syncthreads();
+        if (nl > 0) {
+            unsigned long long wb = 0;
+            if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
+            wb = __shfl(wb, 0);
+            for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
+        }
+        __syncthreads();
+        nl = 0;
+    };
     for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < n; t0 += (int64_t)gridDim.x * 64) {
         const int64_t t = t0 + lane;
         int32_t i = 0;
@@ -771,18 +783,6 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
         const int64_t base = cnt >= 0 ? sell_base(a.g, i) : 0;
-        int nl = 0;
-        auto flush = [&]() {
-            __syncthreads();
-            if (nl > 0) {
-                unsigned long long wb = 0;
-                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
-                wb = __shfl(wb, 0);
-                for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
-            }
-            __syncthreads();
-            nl = 0;
-        };
         for (int k = -1; k < mc; ++k) {
             if (nl > kListCap - 64) flush();
             int32_t j = 0;
@@ -796,9 +796,9 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             if (first) lbuf[nl + __popcll(below)] = j;
             nl += __popcll(ball);
         }
-        flush();
         if (cnt >= 0 && !a.g.isb[i]) gather_node<D, M, KIND, false>(a, Y, hbm_src<D, M>(a, Y), i);
     }
+    flush();
 }
 
 template <int D, int M, int KIND, bool FP = true>
@@ -806,6 +806,8 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
                                                   double* __restrict__ K, int p) {
     const int64_t n = (int64_t)*work_count(a, p);
     if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
+    int dsum = 0;         // this lane's inadmissible delta over its items
+    bool anych = false;   // this lane listed a change
     for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < n; t0 += (int64_t)gridDim.x * 64) {
         const int64_t t = t0 + threadIdx.x;
         bool ch = false;
@@ -826,11 +828,13 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
             }
         }
         if constexpr (!FP) continue;
-        if (t < n) {
-            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
-            if (ch) *changed_any(a, 1 - p) = 1;
-        }
-        if (pred.count) add_inadmissible(a, delta);
+        if (t < n) changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+        anych = anych || ch;
+        dsum += delta;
+    }
+    if constexpr (FP) {
+        set_changed_any(a, anych, 1 - p);
+        if (pred.count) add_inadmissible_sum(a, dsum);
     }
 }
 
@@ -1061,6 +1065,8 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide_list(StepArgs a, PredArgs p
     const int64_t n = (int64_t)*work_count(a, p);
     if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
+    int dsum = 0;         // this lane's inadmissible delta over its items
+    bool anych = false;   // this lane listed a change
     for (int64_t t0 = (int64_t)blockIdx.x * S::GPW; t0 < n; t0 += (int64_t)gridDim.x * S::GPW) {
         const int64_t t = t0 + grp;
         int64_t i = 0;
@@ -1078,11 +1084,13 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide_list(StepArgs a, PredArgs p
         const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp]);
         if constexpr (!FP) continue;
         const int delta = (take && gl == 0) ? (int)a.inadm[i] - before : 0;
-        if (t < n && gl == 0) {
-            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
-            if (ch) *changed_any(a, 1 - p) = 1;
-        }
-        if (pred.count) add_inadmissible(a, delta);
+        if (t < n && gl == 0) changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+        anych = anych || (ch && gl == 0);
+        dsum += delta;
+    }
+    if constexpr (FP) {
+        set_changed_any(a, anych, 1 - p);
+        if (pred.count) add_inadmissible_sum(a, dsum);
     }
 }
 

@@ -1034,6 +1034,14 @@ __device__ __forceinline__ void add_inadmissible(const StepArgs& a, int delta) {
     if ((up | dn) && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1))
         atomicAdd(&a.counters[1], (unsigned long long)((long long)__popcll(up) - (long long)__popcll(dn)));
 }
+// The same count from per-lane sums a grid-strided kernel kept over all its items: one atomic per wavefront at its
+// end (called by every lane of the wavefront). A wavefront that took an atomic per item queued one contended atomic
+// on this single counter per item group, which serialised the list kernels when most particles changed.
+__device__ __forceinline__ void add_inadmissible_sum(const StepArgs& a, int d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+    if (d != 0 && (threadIdx.x & 63) == 0) atomicAdd(&a.counters[1], (unsigned long long)(long long)d);
+}
 
 // ============================================================================================================
 // flux pass: MUSCL functor + _flux_loop (MUSCL.jl:390-508), Rusanov (FluxFunctions.jl:16-76), sort_flux
@@ -1444,6 +1452,11 @@ __device__ __forceinline__ unsigned long long* work_count(const StepArgs& a, int
 __device__ __forceinline__ unsigned long long* changed_any(const StepArgs& a, int p) {
     return p ? &a.counters[9] : &a.counters[8];
 }
+// Sets changed_any(p) if any lane of the wavefront saw a change over all its items (every lane calls it): one store
+// a wavefront instead of one per item group to the same word.
+__device__ __forceinline__ void set_changed_any(const StepArgs& a, bool any, int p) {
+    if (__ballot(any) && (threadIdx.x & 63) == 0) *changed_any(a, p) = 1;
+}
 
 // Appends the flagged particles of a wavefront to changed list p (one atomic per wavefront) and counts them.
 __device__ __forceinline__ void append_changed(const StepArgs& a, bool ch, int64_t i, int p = 0) {
@@ -1573,6 +1586,8 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
     const int grp = threadIdx.x / G, gl = threadIdx.x % G;
     if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const bool streamed = FV == 1 && a.streamed;
+    int dsum = 0;         // this lane's inadmissible delta over its items
+    bool anych = false;   // this lane listed a change
     for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
         const int64_t t = t0 + grp;
         int64_t i = 0;
@@ -1682,11 +1697,13 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
         __syncthreads();
         if constexpr (!FP) continue;
         // work item t's entry of changed list 1 - p: the node if its order is to drop, else -1 (no atomics)
-        if (t < n && gl == 0) {
-            changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
-            if (ch) *changed_any(a, 1 - p) = 1;
-        }
-        if (pred.count) add_inadmissible(a, delta);
+        if (t < n && gl == 0) changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
+        anych = anych || ch;
+        dsum += delta;
+    }
+    if constexpr (FP) {
+        set_changed_any(a, anych, 1 - p);
+        if (pred.count) add_inadmissible_sum(a, dsum);
     }
 }
 
@@ -1845,6 +1862,22 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
         __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     const int lane = threadIdx.x, grp = lane / G, gl = lane % G;
+    // the particle and its neighbours are marked dirty; the lane whose atomicExch sets a node's mark lists the node
+    // (once per iteration): collected in LDS over all the workgroup's items, then appended to the work list with one
+    // atomicAdd whenever the buffer fills and once at the end (an append per item group queued one contended atomic
+    // on the list length per group, which serialised the kernel when most particles changed)
+    int nl = 0;   // entries in lbuf (uniform)
+    auto flush = [&]() {
+        __syncthreads();
+        if (nl > 0) {
+            unsigned long long wb = 0;
+            if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
+            wb = __shfl(wb, 0);
+            for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
+        }
+        __syncthreads();
+        nl = 0;
+    };
     for (int64_t t0 = (int64_t)blockIdx.x * GPW; t0 < n; t0 += (int64_t)gridDim.x * GPW) {
         const int64_t t = t0 + grp;
         int32_t i = 0;
@@ -1858,24 +1891,10 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             }
         }
         order = __shfl(order, grp * G);
-        // the particle and its neighbours are marked dirty; the lane whose atomicExch sets a node's mark lists the node
-        // (once per iteration): collected in LDS, then appended to the work list with one atomicAdd per pass
         int mc = cnt;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
         const int64_t base = cnt >= 0 ? sell_base(a.g, i) : 0;
-        int nl = 0;   // entries in lbuf (uniform)
-        auto flush = [&]() {
-            __syncthreads();
-            if (nl > 0) {
-                unsigned long long wb = 0;
-                if (lane == 0) wb = atomicAdd(work_count(a, p), (unsigned long long)nl);
-                wb = __shfl(wb, 0);
-                for (int e = lane; e < nl; e += 64) a.dlist[wb + e] = lbuf[e];
-            }
-            __syncthreads();
-            nl = 0;
-        };
         for (int k0 = -1; k0 < mc; k0 += G) {
             if (nl > kListCap - 64) flush();   // a step adds at most 64 entries
             const int k = k0 + gl;
@@ -1890,9 +1909,93 @@ __global__ __launch_bounds__(64) void k_mark_gather(StepArgs a, const double* __
             if (first) lbuf[nl + __popcll(below)] = j;
             nl += __popcll(ball);
         }
-        flush();
         gather_split<D, M, KIND, G>(a, Y, i, cnt >= 0 && !a.g.isb[i], cnt, order, gl, rows[grp], bsh[grp]);
     }
+    flush();
+}
+
+// The mark pass of a dense fixed-point iteration with parity p (MUSCL order 2; k_flux_marked follows), one lane per
+// entry of changed list p: the particle drops one order, to 1 (the only order below 2), so its new gradient is the
+// zero record gather_split stores for a particle without a fit; it and its neighbours are marked for the flux pass
+// with plain stores (a mark set twice is the same mark; nothing is listed). The bookkeeping is k_mark_gather's.
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_mark_dense(StepArgs a, unsigned long long* pub, int p) {
+    using L = Layout<D, M>;
+    const int64_t n = *changed_any(a, p) ? (int64_t)*changed_count(a, p) : 0;
+    const int32_t* cl = changed_list(a, p);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *changed_count(a, 1 - p) = 0;
+        *work_count(a, 1 - p) = 0;
+        *changed_any(a, 1 - p) = 0;
+        if (n > 0) a.counters[4] += 1;
+        __hip_atomic_store(pub, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        const int32_t i = cl[t];
+        if (i < 0) continue;
+        a.orders[i] = (int8_t)(a.orders[i] - 1);
+        if (!a.g.isb[i]) {
+            double g[L::NG];
+#pragma unroll
+            for (int q = 0; q < L::NG; ++q) g[q] = 0.0;
+            store_rec<L::NG, L::GS>(a.G, i, g);
+        }
+        a.dmark[i] = 1u;
+        const int cnt = a.g.cnt[i];
+        const int64_t base = sell_base(a.g, i);
+        for (int k = 0; k < cnt; ++k) a.dmark[a.g.nbr[base + (int64_t)k * kChunk]] = 1u;
+    }
+}
+
+// append_changed with one atomic per workgroup of kBlock threads (every thread calls it): the wavefronts' counts are
+// added in LDS, thread 0 takes the workgroup's range of changed list p, each wavefront writes its part in lane order
+__device__ __forceinline__ void append_changed_block(const StepArgs& a, bool ch, int64_t i, int p) {
+    constexpr int NW = kBlock / 64;
+    __shared__ unsigned long long wbase[NW + 1];
+    const unsigned long long ball = __ballot(ch);
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    if (lane == 0) wbase[w] = (unsigned long long)__popcll(ball);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int q = 0; q < NW; ++q) { const unsigned long long c = wbase[q]; wbase[q] = tot; tot += c; }
+        wbase[NW] = tot ? atomicAdd(changed_count(a, p), tot) : 0ull;
+        if (tot) *changed_any(a, p) = 1;
+    }
+    __syncthreads();
+    if (ch) {
+        const unsigned long long below = ball & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        changed_list(a, p)[wbase[NW] + wbase[w] + (unsigned long long)__popcll(below)] = (int32_t)i;
+    }
+}
+
+// The flux pass of a dense fixed-point iteration with parity p (after k_mark_dense): the staged full
+// pass restricted to the marked particles (dmark, cleared here), each recomputed and re-checked as k_flux_list does,
+// the changed ones appended to changed list 1 - p (compact). A workgroup without a marked particle ends before
+// staging. The set recomputed and the values written are those of the list form, so its bits are too; it is the
+// faster form when most particles are marked (one staged halo per block instead of kSplit lanes per particle reading
+// through the caches, and no list appends).
+template <int D, int M, int KIND, int FV>
+__global__ __launch_bounds__(kBlock, D < 3 ? 2 : 1) void k_flux_marked(StepArgs a, PredArgs pred, const double* __restrict__ Y,
+                                                         double* __restrict__ K, int p, int64_t nblk) {
+    extern __shared__ double sh[];
+    const int64_t pos = logical_block(nblk);
+    if (pos >= nblk) return;
+    const int64_t lb = a.border ? a.border[pos] : pos;
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const bool mk = i < a.g.N && a.dmark[i] != 0u;
+    if (!__syncthreads_or(mk)) return;   // uniform per workgroup
+    if (mk) a.dmark[i] = 0u;
+    const LdsSrc<D, M, true> src = stage_halo<D, M, true>(a, Y, sh, lb);
+    bool ch = false;
+    int delta = 0;
+    if (mk && !a.g.isb[i]) {
+        const int before = a.inadm[i];
+        ch = flux_node<D, M, KIND, true, FV>(a, pred, Y, src, K, i, a.hratio * node_dt(a.dtp, a.inst, i));
+        delta = (int)a.inadm[i] - before;
+    }
+    append_changed_block(a, ch, i, 1 - p);
+    if (pred.count) add_inadmissible(a, delta);
 }
 
 // ============================================================================================================
@@ -2488,9 +2591,11 @@ struct hcc_stepper {
     // launch choices (hcc_stepper_set_tuning / hcc_stepper_autotune; results are the same bits under every choice):
     // LDS-staged full passes or neighbour reads through the caches, the lanes per particle of the fixed point's
     // kernels (kSplit or twice it), the particles one sweep of their grid covers, and the CFL estimate fused into the
-    // stage-1 gather or run as its own pass
+    // stage-1 gather or run as its own pass, and when the first iteration of a fused fixed point runs in the dense form
+    // (k_flux_marked): dense > 0 when the same checkpoint of the last step started from at least N / dense changed
+    // particles, 0 never, -1 always
     struct Tuning {
-        int staged = 1, split = 0, sweep = 65536, fuse_cfl = 1;
+        int staged = 1, split = 0, sweep = 65536, fuse_cfl = 1, dense = 64;
     } tune;
     // hcc_step_cfl on a state whose stage 1 is U: the CFL estimate runs inside the stage-1 gather (cfl, cap)
     bool cfl_fused = false;
@@ -2524,6 +2629,9 @@ struct hcc_stepper {
     double gcfl = 0, gcap = 0;
     int gbound[kMaxStages + 2] = {};   // iterations captured per checkpoint (index: checkpoint 2 .. stages + 1)
     int gseen[kMaxStages + 2] = {};    // the most non-empty iterations an uncaptured checkpoint needed
+    // the count the first fused fixed-point iteration of each checkpoint started from in the last uncaptured step
+    // (the dense form's launch choice, tune.dense)
+    int64_t fp_first[kMaxStages + 2] = {};
     int64_t g_launches = 0, g_captures = 0, g_reruns = 0;
     double last_ms = 0, last_gather_ms = 0, last_flux_ms = 0;
     int64_t inadmissible = 0, mood_iters = 0;
@@ -2674,6 +2782,17 @@ struct Variant {
             k_mark_gather<D, M, KIND, G><<<split_grid<G>(s), 64, 0, st(s)>>>(a, Y, pub, p);
         });
     }
+    // a dense fixed-point iteration with parity p (plain marks, then the staged flux pass of the marked particles);
+    // false where the flux pass is not staged (the caller runs the list form)
+    static bool mood_dense(const hcc_stepper* s, const StepArgs& a, const PredArgs& pr, const double* Y, double* K,
+                           unsigned long long* pub, int p) {
+        if (!(s->lds_flux && s->tune.staged)) return false;
+        const int64_t nb = nblocks(s->N);
+        k_mark_dense<D, M><<<(unsigned)std::min<int64_t>(nb, 4096), kBlock, 0, st(s)>>>(a, pub, p);
+        if (s->fv) k_flux_marked<D, M, KIND, 1><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, pr, Y, K, p, nb);
+        else k_flux_marked<D, M, KIND, 0><<<grid_for(nb), kBlock, s->lds_flux, st(s)>>>(a, pr, Y, K, p, nb);
+        return true;
+    }
     template <int FV>
     // full: 1 every particle, 0 the dirty ones (a pass over all that skips the clean), -1 the dirty ones of an
     // unfused recompute listed first (kSplit lanes a particle)
@@ -2771,6 +2890,8 @@ struct VariantOps {
     void (*mark_gather)(const hcc_stepper*, const StepArgs&, const double*, unsigned long long*, int);
     void (*flux)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int, bool);
     void (*flux_list)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*, int);
+    bool (*mood_dense)(const hcc_stepper*, const StepArgs&, const PredArgs&, const double*, double*,
+                       unsigned long long*, int);   // nullptr: the variant has no dense iteration
     void (*flag)(const hcc_stepper*, const StepArgs&, const double*, const double*, const double*, const double*, int);
     void (*predict_imex)(const hcc_stepper*, const StepArgs&, const ImexTerms&, double*);
     void (*range)(const hcc_stepper*, const double*);
@@ -2786,7 +2907,8 @@ struct VariantOps {
 template <int D, int M, int KIND>
 static VariantOps make_ops() {
     using V = Variant<D, M, KIND>;
-    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex, &V::range, &V::cfl, &V::predict,
+    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::mood_dense, &V::flag,
+                      &V::predict_imex, &V::range, &V::cfl, &V::predict,
                       &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack, Layout<D, M>::PU, Layout<D, M>::GS};
 }
 // MUSCL order 3: the passes of hcc_muscl3.h (neighbours through the caches, one lane a particle in the fixed point);
@@ -2860,7 +2982,8 @@ struct Variant3 : Variant<D, M, KIND> {
 template <int D, int M, int KIND>
 static VariantOps make_ops3() {
     using V = Variant3<D, M, KIND>;
-    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, &V::flag, &V::predict_imex,
+    return VariantOps{&V::gather, &V::gather_cfl, &V::mark_gather, &V::flux, &V::flux_list, nullptr, &V::flag,
+                      &V::predict_imex,
                       &V::range, &V::cfl, &V::predict, &V::stage_pos, &V::rec_pos, &V::mass, &V::pack, &V::unpack,
                       Layout<D, M>::PU, m3::L3<D, M>::GS};
 }
@@ -3378,11 +3501,22 @@ struct StepRunner {
     // kPending): the host polls the slot instead of waiting for an event, which would also hold the stream between the
     // two kernels.
     static constexpr unsigned long long kPending = ~0ull;
-    hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, int q) {
+    hipError_t mood_iteration(const PredArgs& p, const double* Yp, double* Kp, int q, bool dense = false) {
         *(volatile unsigned long long*)&s->h_pub[q] = kPending;   // its last writer has published (waited on)
-        ops.mark_gather(s, a, Yp, s->d_pub + q, q);
-        ops.flux_list(s, a, p, Yp, Kp, q);
+        if (!(dense && ops.mood_dense && ops.mood_dense(s, a, p, Yp, Kp, s->d_pub + q, q))) {
+            ops.mark_gather(s, a, Yp, s->d_pub + q, q);
+            ops.flux_list(s, a, p, Yp, Kp, q);
+        }
         return hipGetLastError();
+    }
+    // The first iteration of a fixed point runs in the dense form (k_flux_marked) when the same checkpoint of the last
+    // step started from at least N / tune.dense changed particles: then nearly every particle is recomputed, which
+    // the staged pass does at a fraction of the list kernels' cost. The host does not know the count before it queues
+    // the iteration; the last step's count is a good guess (the flagged set moves little from step to step), and
+    // either form gives the same bits.
+    bool dense_first(int i) const {
+        const int d = s->tune.dense;
+        return d < 0 || (d > 0 && s->fp_first[i] * d >= s->N);
     }
     // The count published in slot q. Polls for up to a second; then waits for the stream, so that a failed launch
     // surfaces as its error instead of a hang. The first polls spin (the usual wait is a few to a few tens of
@@ -3427,12 +3561,13 @@ struct StepRunner {
                 k_fixed_point_overflow<<<1, 64, 0, str>>>(s->counters, s->gbound[i] & 1, s->d_pub + 2);
                 return hipGetLastError();
             }
-            hipError_t e = mood_iteration(p, Yp, Kp, 0);
+            hipError_t e = mood_iteration(p, Yp, Kp, 0, dense_first(i));
             if (e != hipSuccess) return e;
             for (int k = 0;; ++k) {
                 const int q = k & 1;
                 unsigned long long n = 0;
                 if ((e = wait_published(q, n)) != hipSuccess) return e;
+                if (k == 0) s->fp_first[i] = (int64_t)n;
                 if (n == 0) break;   // n_k = 0: iteration k + 1 does nothing
                 ++nonempty[i];
                 if ((e = mood_iteration(p, Yp, Kp, q ^ 1)) != hipSuccess) return e;
@@ -3753,7 +3888,8 @@ extern "C" int hcc_stepper_graph_stats(hcc_stepper* s, int64_t* launches, int64_
 // ---- tuning -------------------------------------------------------------------------------------------------
 // knobs: [0] staged full passes (1) or neighbour reads through the caches (0); [1] lanes per particle of the fixed
 // point's kernels (kSplit or twice it; 0: kSplit); [2] particles one sweep of their grid covers (16384 or 65536);
-// [3] the CFL estimate fused into the stage-1 gather (1) or its own pass (0)
+// [3] the CFL estimate fused into the stage-1 gather (1) or its own pass (0); [4] the dense form of a fixed point's
+// first iteration: when the last step's count at the checkpoint reached N / knobs[4] (> 0), never (0), always (-1)
 static int split_base(const hcc_stepper* s) { return s->D * s->M <= 8 ? 8 : 16; }
 
 extern "C" int hcc_stepper_set_tuning(hcc_stepper* s, const int32_t* knobs) {
@@ -3762,7 +3898,7 @@ extern "C" int hcc_stepper_set_tuning(hcc_stepper* s, const int32_t* knobs) {
     const int g0 = split_base(s);
     const int split = knobs[1] == 0 ? g0 : knobs[1];
     if ((knobs[0] != 0 && knobs[0] != 1) || (split != g0 && split != 2 * g0) ||
-        (knobs[2] != 16384 && knobs[2] != 65536) || (knobs[3] != 0 && knobs[3] != 1))
+        (knobs[2] != 16384 && knobs[2] != 65536) || (knobs[3] != 0 && knobs[3] != 1) || knobs[4] < -1)
         return set_err(ctx, HCC_E_ARG, "hcc_stepper_set_tuning: knobs out of range");
     HCC_TRY_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = finish_pending(s)) return rc;
@@ -3771,6 +3907,7 @@ extern "C" int hcc_stepper_set_tuning(hcc_stepper* s, const int32_t* knobs) {
     s->tune.split = split;
     s->tune.sweep = knobs[2];
     s->tune.fuse_cfl = knobs[3];
+    s->tune.dense = knobs[4];
     return HCC_OK;
 }
 
@@ -3780,6 +3917,7 @@ extern "C" int hcc_stepper_get_tuning(hcc_stepper* s, int32_t* knobs) {
     knobs[1] = s->tune.split ? s->tune.split : split_base(s);
     knobs[2] = s->tune.sweep;
     knobs[3] = s->tune.fuse_cfl;
+    knobs[4] = s->tune.dense;
     return HCC_OK;
 }
 

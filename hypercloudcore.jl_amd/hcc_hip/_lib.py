@@ -496,23 +496,24 @@ class Stepper:
               "hcc_stepper_graph_stats")
         return a.value, b.value, c.value, bounds[:len(self.scheme["b_t"])].copy()
 
-    def set_tuning(self, staged=1, split=0, sweep=65536, fuse_cfl=1):
+    def set_tuning(self, staged=1, split=0, sweep=65536, fuse_cfl=1, dense=64):
         """Launch choices (hcc_stepper_set_tuning); every choice gives the same results."""
-        k = np.array([staged, split, sweep, fuse_cfl], np.int32)
+        k = np.array([staged, split, sweep, fuse_cfl, dense], np.int32)
         check(lib().hcc_stepper_set_tuning(self.h, ptr(k)), self.ctx.h, "hcc_stepper_set_tuning")
 
     def tuning(self):
-        k = np.zeros(4, np.int32)
+        k = np.zeros(5, np.int32)
         check(lib().hcc_stepper_get_tuning(self.h, ptr(k)), self.ctx.h, "hcc_stepper_get_tuning")
-        return {"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3])}
+        return {"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3]), "dense": int(k[4])}
 
     def autotune(self, steps=3, cfl=0.4):
         """hcc_stepper_autotune: (chosen knobs, best ms/step, default ms/step, timed)."""
-        k = np.zeros(4, np.int32)
+        k = np.zeros(5, np.int32)
         best, dflt, timed = C.c_double(), C.c_double(), C.c_int32()
         check(lib().hcc_stepper_autotune(self.h, int(steps), float(cfl), ptr(k), C.byref(best), C.byref(dflt),
                                          C.byref(timed)), self.ctx.h, "hcc_stepper_autotune")
-        return ({"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3])}, best.value,
+        return ({"staged": int(k[0]), "split": int(k[1]), "sweep": int(k[2]), "fuse_cfl": int(k[3]), "dense": int(k[4])},
+                best.value,
                 dflt.value, bool(timed.value))
 
     def set_indicator(self, mlp, threshold=0.0):

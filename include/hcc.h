@@ -205,14 +205,17 @@ int hcc_stepper_graph_stats(hcc_stepper* st, int64_t* launches, int64_t* capture
  * neighbourhood in LDS (1) or read neighbours through the caches (0); knobs[1] lanes per particle in the MOOD fixed
  * point's kernels (kSplit = 8 for D*M <= 8, else 16; or twice it; 0 = kSplit); knobs[2] particles one sweep of the
  * fixed point's grid covers (16384 or 65536); knobs[3] the CFL estimate of hcc_step_cfl fused into the stage-1 gather
- * (1) or its own pass (0). Defaults: staged where the LDS tables exist, kSplit, 65536, fused. */
+ * (1) or its own pass (0); knobs[4] when the first iteration of a fused MOOD fixed point runs in its dense form (plain
+ * marks, then the staged flux pass of the marked particles instead of the list kernels): when the same checkpoint of
+ * the last step started from at least N / knobs[4] changed particles (knobs[4] > 0), never (0), always (-1). Defaults:
+ * staged where the LDS tables exist, kSplit, 65536, fused, 64. */
 int hcc_stepper_set_tuning(hcc_stepper* st, const int32_t* knobs);
 int hcc_stepper_get_tuning(hcc_stepper* st, int32_t* knobs);
 /* The autotuner: times `steps` CFL steps (CFL number cfl) of every launch choice from the stepper's current state,
  * restored before each trial and afterwards, and keeps the fastest if it beats the defaults by more than 2 %. The choice
  * is stored per workload shape (D, M, PDE, flux variant, LDS tables present, log2 of the node count, mean neighbour
  * count, ensemble) for the process: a stepper of a tuned shape takes it without timing (*timed = 0). Outputs (may be
- * NULL): knobs[4] as hcc_stepper_get_tuning, the best and the default milliseconds per step (0 when not timed). The
+ * NULL): knobs[5] as hcc_stepper_get_tuning, the best and the default milliseconds per step (0 when not timed). The
  * step statistics (hcc_step_totals and the last step's diagnostics) are left as they were. */
 int hcc_stepper_autotune(hcc_stepper* st, int32_t steps, double cfl, int32_t* knobs, double* ms_best, double* ms_default,
                          int32_t* timed);
