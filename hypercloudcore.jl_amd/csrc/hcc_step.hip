@@ -2613,6 +2613,7 @@ struct hcc_stepper {
         int64_t host_iters = 0;
         bool dt_from_device = false;
         bool graph = false;                   // a captured step: no pass events (gather and flux times read 0)
+        bool passes = false;                  // the pass events were recorded (hcc_stepper_set_pass_timing)
     };
     static constexpr int kRing = 4;
     StepStats ring[kRing];
@@ -2638,6 +2639,10 @@ struct hcc_stepper {
     // hcc_step returns without waiting for the device; the statistics of its queued steps (ring slots) are read when
     // a getter asks or the ring is full, and summed into the totals
     double last_dt = 0;
+    // pass timing: the events around each stage's gather and flux pass are recorded every pass_every-th uncaptured
+    // step (0: never); pass_serial counts the uncaptured steps, tot_timed the steps of the totals that were timed
+    int pass_every = 1;
+    int64_t pass_serial = 0, tot_timed = 0;
     int64_t tot_steps = 0, tot_iters = 0;
     double tot_ms = 0, tot_gather_ms = 0, tot_flux_ms = 0;
     StepArgs args() const;
@@ -3361,7 +3366,7 @@ struct StepRunner {
     hcc_stepper* s;
     VariantOps ops;
     StepArgs a;
-    hipEvent_t* evp = nullptr;   // the pass events of this step's statistics slot
+    hipEvent_t* evp = nullptr;   // the pass events of this step's statistics slot (nullptr: passes not timed)
     int64_t nb = 0;
     int iters = 0;
     // capture: the fused fixed point of checkpoint i runs gbound[i] iterations queued without host decisions, then
@@ -3636,7 +3641,7 @@ struct StepRunner {
                 if (s->moving) apply_stage_geometry(stage_state(i));
             }
             const double* Y = stage_state(i);
-            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
+            if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 0], str);
             if (s->feat_on) k_zero_u64<<<1, 64, 0, str>>>(s->ind_n);
             if (i == 1 && s->cfl_fused && a.kz && s->cfl_chain && !graph) {
                 // U = Y_s of the last step plus dt sum b_j K_j: on a block whose halo had every K of that step +-0
@@ -3664,10 +3669,10 @@ struct StepRunner {
                 hipError_t e = after_gather(i);
                 if (e != hipSuccess) return e;
             }
-            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
+            if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
             ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
-            if (!graph) (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
+            if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
         hipError_t e = checkpoint(st + 1);
         if (e != hipSuccess) return e;
@@ -3696,7 +3701,7 @@ static int fold_oldest(hcc_stepper* s) {
     HCC_TRY_HIP(ctx, hipEventElapsedTime(&ms, r.ev0, r.ev1));
     s->last_ms = ms;
     s->last_gather_ms = s->last_flux_ms = 0;
-    for (int i = 0; i < s->desc.stages && !r.graph; ++i) {
+    for (int i = 0; i < s->desc.stages && r.passes; ++i) {
         float gm = 0, fm = 0;
         HCC_TRY_HIP(ctx, hipEventElapsedTime(&gm, r.evp[4 * i + 0], r.evp[4 * i + 1]));
         HCC_TRY_HIP(ctx, hipEventElapsedTime(&fm, r.evp[4 * i + 1], r.evp[4 * i + 3]));
@@ -3704,6 +3709,7 @@ static int fold_oldest(hcc_stepper* s) {
         s->last_flux_ms += fm;
     }
     s->tot_steps += 1;
+    s->tot_timed += r.passes ? 1 : 0;
     s->tot_iters += s->mood_iters;
     s->tot_ms += s->last_ms;
     s->tot_gather_ms += s->last_gather_ms;
@@ -3745,7 +3751,8 @@ static int launch_step(hcc_stepper* s, const char* who, bool dt_from_device) {
     r.s = s;
     find_ops(s, r.ops);
     r.a = step_args(s);
-    r.evp = slot.evp;
+    slot.passes = s->pass_every > 0 && s->pass_serial++ % s->pass_every == 0;
+    r.evp = slot.passes ? slot.evp : nullptr;
     HCC_TRY_HIP(ctx, hipEventRecord(slot.ev0, ctx->stream));
     hipError_t e = r.run();
     s->cfl_chain = s->cfl_fused && r.a.kz != nullptr && e == hipSuccess;
@@ -3854,6 +3861,7 @@ static int graph_step(hcc_stepper* s) {
     slot.host_iters = 0;
     slot.dt_from_device = true;
     slot.graph = true;
+    slot.passes = false;
     s->ring_head = (s->ring_head + 1) % hcc_stepper::kRing;
     ++s->ring_count;
     std::swap(s->U, s->cand);   // pg.rhos <- rho_cand
@@ -4230,7 +4238,21 @@ extern "C" int hcc_step_totals(hcc_stepper* s, int reset, int64_t* steps, int64_
     if (ms_total) *ms_total = s->tot_ms;
     if (ms_gather) *ms_gather = s->tot_gather_ms;
     if (ms_flux) *ms_flux = s->tot_flux_ms;
-    if (reset) { s->tot_steps = s->tot_iters = 0; s->tot_ms = s->tot_gather_ms = s->tot_flux_ms = 0; }
+    if (reset) { s->tot_steps = s->tot_iters = s->tot_timed = 0; s->tot_ms = s->tot_gather_ms = s->tot_flux_ms = 0; }
+    return HCC_OK;
+}
+
+extern "C" int hcc_stepper_set_pass_timing(hcc_stepper* s, int32_t every) {
+    if (!s || every < 0) return HCC_E_ARG;
+    s->pass_every = every;
+    s->pass_serial = 0;
+    return HCC_OK;
+}
+
+extern "C" int hcc_step_timed_passes(hcc_stepper* s, int64_t* steps) {
+    if (!s || !steps) return HCC_E_ARG;
+    if (int rc = finish_pending(s)) return rc;
+    *steps = s->tot_timed;
     return HCC_OK;
 }
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "hcc_mlp_get_params", "hcc_mlp_forward", "hcc_mlp_gradients", "hcc_mlp_gradients_noalloc", "hcc_mlp_train",
     "hcc_set_indicator", "hcc_indicator_collect", "hcc_get_indicator_samples", "hcc_get_indicator_count",
     "hcc_stepper_set_graph", "hcc_stepper_graph_stats", "hcc_stepper_set_tuning", "hcc_stepper_get_tuning",
-    "hcc_stepper_autotune",
+    "hcc_stepper_autotune", "hcc_stepper_set_pass_timing", "hcc_step_timed_passes",
 ]
 MLP_PARAMS = 5313
 
@@ -100,6 +100,8 @@ def lib():
         "hcc_rhs": (C.c_int, [vp, vp, vp, vp]),
         "hcc_last_step_timing": (C.c_int, [vp, P(d), P(d), P(d)]),
         "hcc_step_totals": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(d), P(d), P(d)]),
+        "hcc_stepper_set_pass_timing": (C.c_int, [vp, C.c_int32]),
+        "hcc_step_timed_passes": (C.c_int, [vp, P(i64)]),
         "hcc_step_cfl": (C.c_int, [vp, C.c_double, C.c_double]),
         "hcc_last_dt": (C.c_int, [vp, P(d)]),
         "hcc_grid_volumes": (C.c_int, [vp, C.c_int32, vp]),
@@ -624,6 +626,16 @@ class Stepper:
         check(lib().hcc_step_totals(self.h, int(reset), C.byref(n), C.byref(it), C.byref(a), C.byref(b), C.byref(c)),
               self.ctx.h, "hcc_step_totals")
         return n.value, it.value, a.value, b.value, c.value
+
+    def set_pass_timing(self, every=1):
+        """Record the gather / flux pass events every `every`-th step (0: never; hcc_stepper_set_pass_timing)."""
+        check(lib().hcc_stepper_set_pass_timing(self.h, int(every)), self.ctx.h, "hcc_stepper_set_pass_timing")
+
+    def timed_passes(self):
+        """The steps of the current totals whose passes were timed (hcc_step_timed_passes); read before a reset."""
+        n = C.c_int64()
+        check(lib().hcc_step_timed_passes(self.h, C.byref(n)), self.ctx.h, "hcc_step_timed_passes")
+        return n.value
 
     def close(self):
         if getattr(self, "h", None):

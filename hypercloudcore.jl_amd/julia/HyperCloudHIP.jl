@@ -329,14 +329,23 @@ function mood_inadmissible(ts::HIPRKTimeStepper)
     return Int(n[])
 end
 
-# Steps, MOOD iterations and HIP-event milliseconds (step, gather passes, flux passes) since the last reset.
+# Steps, MOOD iterations and HIP-event milliseconds (step, gather passes, flux passes) since the last reset, and the
+# steps among them whose passes were timed (set_pass_timing!).
 function step_totals(ts::HIPRKTimeStepper; reset::Bool = true)
-    n, it = Ref{Int64}(0), Ref{Int64}(0)
+    n, it, nt = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
     a, b, c = Ref{Float64}(0), Ref{Float64}(0), Ref{Float64}(0)
+    checkrc(ccall((:hcc_step_timed_passes, libhcc), Cint, (Ptr{Cvoid}, Ref{Int64}), ts.st, nt), ts.ctx)
     checkrc(ccall((:hcc_step_totals, libhcc), Cint,
                   (Ptr{Cvoid}, Cint, Ref{Int64}, Ref{Int64}, Ref{Float64}, Ref{Float64}, Ref{Float64}),
                   ts.st, Cint(reset), n, it, a, b, c), ts.ctx)
-    return (steps = Int(n[]), mood_iterations = Int(it[]), ms_step = a[], ms_gather = b[], ms_flux = c[])
+    return (steps = Int(n[]), mood_iterations = Int(it[]), ms_step = a[], ms_gather = b[], ms_flux = c[],
+            timed_pass_steps = Int(nt[]))
+end
+
+# The gather / flux pass events recorded every `every`-th step from now on (1: every step, the default; 0: never).
+function set_pass_timing!(ts::HIPRKTimeStepper, every::Integer)
+    checkrc(ccall((:hcc_stepper_set_pass_timing, libhcc), Cint, (Ptr{Cvoid}, Int32), ts.st, Int32(every)), ts.ctx)
+    return ts
 end
 
 "`pg.core.volumes` of the device grid (`update_volumes!(pg, ShepardVolumes(sweeps = sweeps))` first if `sweeps >= 0`)."
@@ -480,6 +489,6 @@ end
 export hip_set_source!, hip_mover_code, download_positions!, HIPEnsembleStepper, hip_step_cfl!, hip_time_steps,
        hip_set_indicator!
 export HIPRKTimeStepper, HIPDivergence, hip_time_step, hip_descriptor, upload!, download!, DomainDesc, StepperDesc,
-       step_totals, hip_volumes, hip_total_mass, hip_solve_equation, hip_domain_desc, hip_stepper_desc
+       step_totals, set_pass_timing!, hip_volumes, hip_total_mass, hip_solve_equation, hip_domain_desc, hip_stepper_desc
 
 end # module

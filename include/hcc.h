@@ -183,6 +183,14 @@ int hcc_last_step_timing(hcc_stepper* st, double* ms_total, double* ms_gather, d
  * driver time a run of steps without a host round trip per step (benchmark/step.jl times its loop as a whole). */
 int hcc_step_totals(hcc_stepper* st, int reset, int64_t* steps, int64_t* mood_iterations, double* ms_total,
                     double* ms_gather, double* ms_flux);
+/* No counterpart in the reference. Pass timing: the HIP events around each stage's full gather and flux pass (the
+ * gather and flux milliseconds above) are recorded every `every`-th step queued from now on (1, the default: every
+ * step; 0: never). Each event between two kernels is a marker packet that leaves the device idle for a few
+ * microseconds (the nine of a 2-D Euler RK3 step cost 3.7 % of it at 1 M nodes), so a driver that times a long run
+ * may sample. A step without pass events adds 0 to the gather and flux sums. */
+int hcc_stepper_set_pass_timing(hcc_stepper* st, int32_t every);
+/* The steps of the hcc_step_totals sums whose passes were timed (cleared with them). */
+int hcc_step_timed_passes(hcc_stepper* st, int64_t* steps);
 
 /* No counterpart in the reference (its step is a host loop over kernel launches). With on != 0, hcc_step_cfl runs
  * each step as one hipGraph launch: the whole step -- the CFL estimate fused into the stage-1 gather, the gather and

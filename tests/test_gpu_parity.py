@@ -318,6 +318,34 @@ def test_step_totals_match_per_step_values(ctx):
     assert st.totals(reset=False)[0] == 0
 
 
+def test_sampled_pass_timing(ctx):
+    """hcc_stepper_set_pass_timing: the pass events of every k-th step (k = 3: steps 0, 3, 6 of 7), none with k = 0;
+    the steps that were timed are counted with the totals, and the states do not depend on the timing."""
+    wl = W.euler_2d(n=64, seed=2)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    a, b = Stepper(ctx, g, wl["scheme"]), Stepper(ctx, g, wl["scheme"])
+    a.upload(wl["U0"])
+    b.upload(wl["U0"])
+    a.set_pass_timing(3)
+    b.set_pass_timing(0)
+    a.totals(reset=True)
+    b.totals(reset=True)
+    for k in range(7):
+        a.step_cfl(wl["cfl"])
+        b.step_cfl(wl["cfl"])
+        assert np.array_equal(a.download(), b.download()), f"step {k}"
+    assert a.timed_passes() == 3 and b.timed_passes() == 0
+    n, _, tot_ms, g_ms, f_ms = a.totals(reset=True)
+    assert n == 7 and 0 < g_ms < tot_ms and 0 < f_ms < tot_ms
+    assert a.timed_passes() == 0, "cleared with the totals"
+    n, _, _, g_ms, f_ms = b.totals(reset=True)
+    assert n == 7 and g_ms == 0 and f_ms == 0
+    for bad in (-1,):
+        with pytest.raises(Exception):
+            a.set_pass_timing(bad)
+
+
 def test_step_cfl_matches_host_driver_loop(ctx):
     """hcc_step_cfl (dt = min(CFL * CFL step, cap) on the device) gives the same dt and states, bit for bit, as the
     driver loop with the host-side dt (time_step, then step); the cap binds in the last step."""
