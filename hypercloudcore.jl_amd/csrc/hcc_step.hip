@@ -1610,14 +1610,40 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
         double b[BD], sacc = 0.0;   // owner lane gl: accumulators of pair gl = d M + m (streamed: component gl)
 #pragma unroll
         for (int c = 0; c < BD; ++c) b[c] = 0.0;
+        // the owner lane's derivative row of the divergence inverse, loaded before the slots (used after them)
+        const bool own = take && !streamed && gl < NP;
+        double xr[BD];
+#pragma unroll
+        for (int c = 0; c < BD; ++c) xr[c] = own ? a.invd[((gl / M) * BD + c) * N + i] : 0.0;
+        // The lane's slots k = k0 + gl, one per chunk, software-pipelined: the records of the next chunk's slot are
+        // loaded before this chunk's arithmetic, its neighbour index one chunk earlier still, so that a chunk waits
+        // on no load of its own (a short list's kernel is a chain of dependent loads, not arithmetic).
+        int32_t jn = (take && gl < cnt) ? a.g.nbr[base + (int64_t)gl * kChunk] : 0;   // index of the next slot
+        double rn[NR], gn[NG];   // records of the next slot
+        int on = 0;
+        if (take && gl < cnt) {
+            load_rec<NR, L::PU>(Y, jn, rn);
+            load_rec<NG, L::GS>(a.G, jn, gn);
+            on = a.orders[jn];
+        }
+        jn = (take && gl + G < cnt) ? a.g.nbr[base + (int64_t)(gl + G) * kChunk] : 0;
         for (int k0 = 0; k0 < mc; k0 += G) {
             const int k = k0 + gl;
+            double rj[NR], gj[NG];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) rj[q] = rn[q];
+#pragma unroll
+            for (int q = 0; q < NG; ++q) gj[q] = gn[q];
+            const int oj = on;
+            if (take && k + G < cnt) {
+                load_rec<NR, L::PU>(Y, jn, rn);
+                load_rec<NG, L::GS>(a.G, jn, gn);
+                on = a.orders[jn];
+            }
+            jn = (take && k + 2 * G < cnt) ? a.g.nbr[base + (int64_t)(k + 2 * G) * kChunk] : 0;
             if (take && k < cnt) {
-                const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
-                double rj[NR], gj[NG], dF[D][M], w, pt[BD], ctr[M];
-                load_rec<NR, L::PU>(Y, j, rj);
-                load_rec<NG, L::GS>(a.G, j, gj);
-                slot_flux<D, M, KIND, FV>(a, nd, rj, gj, a.orders[j], dF, w, pt, ctr);
+                double dF[D][M], w, pt[BD], ctr[M];
+                slot_flux<D, M, KIND, FV>(a, nd, rj, gj, oj, dF, w, pt, ctr);
                 double* row = rows[grp][gl];
                 if (streamed) {
 #pragma unroll
@@ -1655,11 +1681,10 @@ __global__ __launch_bounds__(64) void k_flux_list(StepArgs a, PredArgs pred, con
         }
         // the derivative row of each pair (apply_weights_divergence), then K_E and the check by the group's first lane
         if (take && streamed && gl < M) part[grp][gl] = sacc;
-        if (take && !streamed && gl < NP && a.okd[i]) {
-            const int d = gl / M;
-            double r = a.invd[(d * BD + 0) * N + i] * b[0];
+        if (own && a.okd[i]) {
+            double r = xr[0] * b[0];
 #pragma unroll
-            for (int c = 1; c < BD; ++c) r = r + a.invd[(d * BD + c) * N + i] * b[c];
+            for (int c = 1; c < BD; ++c) r = r + xr[c] * b[c];
             part[grp][gl] = r;
         }
         __syncthreads();
