@@ -296,6 +296,45 @@ def test_euler2d_full_size_cfl_run_bit_exact(ctx):
     assert fired > 0, "MOOD never fired on the checked steps"
 
 
+def test_euler2d_full_size_dense_state_bit_exact(ctx):
+    """The bench's dense state at full size (the four quadrants times 1 + 0.01 sin(6 pi x) sin(10 pi y) on the
+    1 013 475-node cloud): no constant neighbourhood, and the first MOOD check of stage 1 flags most particles, so
+    from the second step on the first fixed-point iteration runs in its dense form (k_mark_dense, k_flux_marked).
+    Five hcc_step_cfl steps beside the oracle: dt every step, states, orders, flags and the inadmissible count bit for
+    bit after each."""
+    import os
+    wl = W.euler_2d(h=1e-3, seed=0)
+    c = wl["cloud"]
+    x = c["pos"]
+    U0 = wl["U0"].copy()
+    fac = 1.0 + 0.01 * np.sin(2 * np.pi * 3 * x[:, 0]) * np.sin(2 * np.pi * 5 * x[:, 1])
+    U0[:, 0] *= fac
+    U0[:, 3] *= fac
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    assert st.tuning()["dense"] == 64
+    st.upload(U0)
+    from oracle import oracle as O
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    o = O.OracleSim(wl["domain"], c["pos"], c["is_boundary"], c["tags"], wl["scheme"], threads=threads)
+    o.set_state(U0)
+    cfl, t = wl["cfl"], 0.0
+    for k in range(1, 6):
+        st.step_cfl(cfl, float("inf"))
+        dt = cfl * o.time_step()
+        assert st.last_dt() == dt, f"dt differs at step {k}: {st.last_dt()!r} vs {dt!r}"
+        o.step(t, dt)
+        t += dt
+        f, f_o = st.mood_flags(), o.mood_flags()
+        assert np.array_equal(f, f_o), f"MOOD flags differ at step {k}: {first_diff(f, f_o)}"
+        assert int(f.sum()) * 64 >= g.N, f"step {k}: {int(f.sum())} flags do not select the dense form"
+        assert np.array_equal(st.orders(), o.orders()), f"orders differ at step {k}"
+        assert st.inadmissible() == o.inadmissible(), f"inadmissible count differs at step {k}"
+        U, U_o = st.download(), o.state()
+        assert np.array_equal(U, U_o), f"states differ at step {k}: {first_diff(U, U_o)}"
+        print(f"dense state step {k}: dt = {dt:.6e}, MOOD flags {int(f.sum())}, bit-exact", flush=True)
+
+
 def test_step_totals_match_per_step_values(ctx):
     """hcc_step returns once its work is queued; the per-step getters and the summed totals see the same steps."""
     wl = W.euler_2d(n=64, seed=2)
