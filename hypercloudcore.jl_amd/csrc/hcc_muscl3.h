@@ -863,7 +863,8 @@ constexpr bool kWide3 = L3<D, M>::BD * D * M > 160;
 template <int D, int M, int KIND, bool CHECK>
 __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
                                             double* __restrict__ K, int64_t i, bool take, double h,
-                                            double (*rows)[Split3<D, M>::SR], double* xs, double* part, double* nsh) {
+                                            double (*rows)[Split3<D, M>::SR], double* xs, double* part, double* nsh,
+                                            double* amat) {
     using L = L3<D, M>;
     using S = Split3<D, M>;
     constexpr int NR = L::NR, NG = L::NG, BD = L::BD, DM = S::DM, B1 = S::B1, G = S::G;
@@ -906,13 +907,14 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
     const int nb = mode == 1 ? BD : B1;
-    double b[BD], A[B1][B1];
+    // the direct fit's moment matrix lives in the group's LDS (amat, B1 x B1, touched by lane G - 1 only): in
+    // registers its 32 VGPRs sat beside every lane's slot arithmetic and pushed it into scratch spills
+    double b[BD];
 #pragma unroll
     for (int c = 0; c < BD; ++c) b[c] = 0.0;
+    if (gl == G - 1)
 #pragma unroll
-    for (int r = 0; r < B1; ++r)
-#pragma unroll
-        for (int c = 0; c < B1; ++c) A[r][c] = 0.0;
+        for (int q = 0; q < B1 * B1; ++q) amat[q] = 0.0;
     for (int k0 = 0; k0 < mc; k0 += G) {
         const int k = k0 + gl;
         if (mode && k < cnt) {
@@ -958,14 +960,18 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
 #pragma unroll
                     for (int r = 0; r < B1; ++r)
 #pragma unroll
-                        for (int c = 0; c < B1; ++c) A[r][c] = A[r][c] + w * (pb[r] * pb[c]);
+                        for (int c = 0; c < B1; ++c) amat[r * B1 + c] = amat[r * B1 + c] + w * (pb[r] * pb[c]);
                 }
             }
         }
         __syncthreads();
     }
     if (mode == 2 && gl == G - 1) {   // _mls_divergence: (C \ E)' with E the derivative rows of the identity
-        double U[B1][B1];
+        double A[B1][B1], U[B1][B1];
+#pragma unroll
+        for (int r = 0; r < B1; ++r)
+#pragma unroll
+            for (int c = 0; c < B1; ++c) A[r][c] = amat[r * B1 + c];
         const bool ok = chol_moment<B1>(A, U);
         if (ok) {
             double E[B1][D], X[B1][D];
@@ -1029,6 +1035,7 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide(StepArgs a, PredArgs pred, 
     __shared__ double xs[S::GPW][S::XS];
     __shared__ double part[S::GPW][S::DM];
     __shared__ double nsh[S::GPW][S::NS];
+    __shared__ double amat[S::GPW][S::B1 * S::B1];
     const int64_t lq = xcd_block(blockIdx.x, nquad);
     if (lq >= nquad) return;
     const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
@@ -1045,7 +1052,8 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide(StepArgs a, PredArgs pred, 
             if (CHECK) h = a.hratio * node_dt(a.dtp, a.inst, i);
         }
     }
-    const bool ch = flux_group3<D, M, KIND, CHECK>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp]);
+    const bool ch = flux_group3<D, M, KIND, CHECK>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp],
+                                                   amat[grp]);
     if (CHECK) {
         const int bad = (take && gl == 0) ? a.inadm[i] : 0;
         append_changed(a, ch, i);
@@ -1062,6 +1070,7 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide_list(StepArgs a, PredArgs p
     __shared__ double xs[S::GPW][S::XS];
     __shared__ double part[S::GPW][S::DM];
     __shared__ double nsh[S::GPW][S::NS];
+    __shared__ double amat[S::GPW][S::B1 * S::B1];
     const int64_t n = (int64_t)*work_count(a, p);
     if (FP && blockIdx.x == 0 && threadIdx.x == 0) *changed_count(a, 1 - p) = (unsigned long long)n;
     const int grp = threadIdx.x / S::G, gl = threadIdx.x % S::G;
@@ -1081,7 +1090,8 @@ __global__ __launch_bounds__(64, 2) void k_flux_wide_list(StepArgs a, PredArgs p
             }
         }
         const double h = FP && take ? a.hratio * node_dt(a.dtp, a.inst, i) : 0.0;
-        const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp]);
+        const bool ch = flux_group3<D, M, KIND, FP>(a, pred, Y, K, i, take, h, rows[grp], xs[grp], part[grp], nsh[grp],
+                                                    amat[grp]);
         if constexpr (!FP) continue;
         const int delta = (take && gl == 0) ? (int)a.inadm[i] - before : 0;
         if (t < n && gl == 0) changed_list(a, 1 - p)[t] = ch ? (int32_t)i : -1;
