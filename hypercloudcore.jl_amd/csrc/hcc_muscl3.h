@@ -509,7 +509,7 @@ template <int D, int M, int KIND>
 // kernels, from the group's LDS copy.
 __device__ __forceinline__ void slot3(const StepArgs& a, const double (&ri)[D + M], const double* fi,
                                       const double* gi, const double* Fi, int oi,
-                                      const double (&rj)[D + M], const double (&gj)[L3<D, M>::NG], int oj,
+                                      const double (&rj)[D + M], const double* __restrict__ gj, int oj,
                                       double (&dd)[D], double& w, double (&dF)[D][M]) {
     using P = Pde<D, M, KIND>;
     constexpr int BG = L3<D, M>::BG;
@@ -919,11 +919,13 @@ __device__ __forceinline__ bool flux_group3(const StepArgs& a, const PredArgs& p
         const int k = k0 + gl;
         if (mode && k < cnt) {
             const int32_t j = a.g.nbr[base + (int64_t)k * kChunk];
-            double rj[NR], gj[NG], dd[D], w, dF[D][M];
+            double rj[NR], dd[D], w, dF[D][M];
             load_rec<NR, L::PU>(Y, j, rj);
-            load_rec<NG, L::GS>(a.G, j, gj);
             const int oj = a.orders[j];
-            slot3<D, M, KIND>(a, ri, fi, nsh, nsh + NG, oi, rj, gj, a.packed ? (oj & 15) : oj, dd, w, dF);
+            // the neighbour's gradient record is read term by term where the reconstruction needs it (a copy of all
+            // its terms in registers beside the fit accumulators spilled)
+            slot3<D, M, KIND>(a, ri, fi, nsh, nsh + NG, oi, rj, a.G + (int64_t)j * L::GS, a.packed ? (oj & 15) : oj,
+                              dd, w, dF);
             double ds[D], t[BD - 1];
 #pragma unroll
             for (int d = 0; d < D; ++d) ds[d] = dd[d] * invL;
