@@ -996,6 +996,15 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     for (int j = 0; j < kMaxStages; ++j) {
         if (j < p.nterm && ((p.use >> j) & 1u)) {
             const double cj = dt * p.at[j];
+            // K_j(n) exactly +-0 in every component (zero-K bit j): the term adds cj (+-0) = +-0, which leaves a
+            // component that is not a zero with its bits (x + (+-0) = x), so where no component is a zero the term and
+            // the load of its K row are skipped; a zero component takes the term (the sign of the sum depends on it)
+            if (kz && j < p.nterm - 1 && ((kzv >> j) & 1u) && isfinite(cj)) {
+                bool nz = true;
+#pragma unroll
+                for (int m = 0; m < M; ++m) nz = nz && (r[O + m] != 0.0);
+                if (nz) continue;
+            }
             double kv[M];
             if (j == p.nterm - 1) {
 #pragma unroll
