@@ -980,7 +980,13 @@ struct PredArgs {
     const double* dtp;
     const uint16_t* inst;      // ensemble grid: the instance of each kStage block (node_dt)
     int on, count;
+    // first: the first full flux pass of its stage in a step that continues an unbroken chain of steps (every state
+    // buffer holds what the last step left there, and the zero-K bits describe its K's); kmask: the bits of all stages
+    int first;
+    unsigned kmask;
 };
+// zero-K bit 7: every K of the last step was exactly +-0 (set by the stage-1 flux pass from the last step's bits)
+constexpr unsigned kZLastStep = 0x80u;
 
 template <int D, int M>
 __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, const double (&Kn)[M],
@@ -988,6 +994,7 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     using L = Layout<D, M>;
     constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
     double r[2 * NP];
+    bool skip_own = false;
     const double dt = node_dt(p.dtp, p.inst, n);
     // Every term exactly +-0 (the zero-K bits of the used terms, this pass's K included) and the stage state fi = Y_s(n)
     // of the pass free of zeros and finite: Y_s(n) is U^n(n) plus +-0 terms, so its non-zero components are U^n's,
@@ -1007,6 +1014,12 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
     if (same) {
 #pragma unroll
         for (int m = 0; m < M; ++m) r[m] = fi[m];
+        // and the last step's K's all exactly zero too: the target, written with U^{n-1}'s values in the last step
+        // (or skipped there by this same rule), holds U^{n-1}(n) = U^n(n), these very bits, so nothing is stored
+        // (a store of part of a record costs the memory a read-modify-write of the 64 B)
+        if (p.first && (kzv & kZLastStep)) {
+            skip_own = true;
+        }
     } else {
         const double2* own = reinterpret_cast<const double2*>(p.Un + n * L::PU) + P0;
 #pragma unroll
@@ -1039,9 +1052,11 @@ __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, cons
             for (int m = 0; m < M; ++m) r[O + m] = r[O + m] + cj * kv[m];
         }
     }
-    double2* dst = reinterpret_cast<double2*>(p.target + n * L::PU) + P0;
+    if (!skip_own) {
+        double2* dst = reinterpret_cast<double2*>(p.target + n * L::PU) + P0;
 #pragma unroll
-    for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+        for (int q = 0; q < NP; ++q) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    }
     for (int32_t e = p.goff[n], e1 = p.goff[n + 1]; e < e1; ++e) {
         const int64_t g = p.gidx[e];
         if (kz) kz[g] = kzv;   // a ghost's stage states are its root's
@@ -1308,7 +1323,14 @@ __device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pre
 #pragma unroll
             for (int m = 0; m < M; ++m) z = z && Kout[m] == 0.0;
             const int st = pred.nterm - 1;
-            kzv = st == 0 ? (uint8_t)z : (uint8_t)((a.kz[i] & ~(1u << st)) | ((unsigned)z << st));
+            const unsigned old = a.kz[i];
+            if (st == 0) {   // bit 7 from the last step's bits before this step's first pass overwrites them
+                const unsigned last = pred.first ? (((old & pred.kmask) == pred.kmask) ? kZLastStep : 0u)
+                                                 : (old & kZLastStep);
+                kzv = (uint8_t)((unsigned)z | last);
+            } else {
+                kzv = (uint8_t)((old & ~(1u << st)) | ((unsigned)z << st));
+            }
             a.kz[i] = kzv;
         }
         fused_predict<D, M>(pred, i, Kout, a.kz, kzv, fi);
@@ -3427,6 +3449,7 @@ struct StepRunner {
     // k_fixed_point_overflow; no pass events. Uncaptured: nonempty[i] counts the iterations that had work.
     bool graph = false;
     bool ordered = false;   // this step has listed its launch order (from the first gather-flag kernel it ran)
+    bool chain0 = false;    // the step continues an unbroken chain of steps (PredArgs::first of its full flux passes)
     int nonempty[kMaxStages + 2] = {};
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
     // MOOD checks the Euler substep inside the flux pass that produces K (flux_node with CHECK); not with an implicit
@@ -3544,6 +3567,8 @@ struct StepRunner {
         p.inst = a.inst;
         p.on = 1;
         p.count = count ? 1 : 0;
+        p.first = 0;
+        p.kmask = (1u << s->desc.stages) - 1u;
         return p;
     }
     // the unfused check's count of changed particles (counters[0]), published into pinned slot 0 and polled
@@ -3681,6 +3706,9 @@ struct StepRunner {
         hipStream_t str = ctx->stream;
         nb = nblocks(s->N);
         y1_in_U = !s->fresh && !s->imex;
+        // this uncaptured fused CFL step continues an unbroken chain (cfl_chain: the last step was one too, with the
+        // zero-K bits tracked, and nothing has touched the state since; an upload makes the next step unfused)
+        chain0 = s->cfl_chain && s->cfl_fused && a.kz != nullptr && !graph;
         // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
         // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
         // particles). The full gather also clears the changed count for the check of the flux pass.
@@ -3724,7 +3752,8 @@ struct StepRunner {
                 if (e != hipSuccess) return e;
             }
             if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
-            const PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
+            PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
+            p.first = chain0 ? 1 : 0;
             ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
             if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }

@@ -335,6 +335,37 @@ def test_euler2d_full_size_dense_state_bit_exact(ctx):
         print(f"dense state step {k}: dt = {dt:.6e}, MOOD flags {int(f.sum())}, bit-exact", flush=True)
 
 
+def test_upload_breaks_the_step_chain(ctx):
+    """Consecutive CFL steps form a chain in which a particle whose K's stay exactly zero keeps the stage records the
+    last step wrote (its predictor stores nothing). An upload in the middle of a chain must end it: after five steps
+    of the four-quadrant state, the same state with density and energy scaled by 1.1 (the same constant regions, other
+    values in every record) is uploaded, and the steps after it match the oracle started from it, bit for bit."""
+    from oracle import oracle as O
+    wl = W.euler_2d(n=64, seed=0)
+    c = wl["cloud"]
+    g = Grid(ctx, W.domain_desc(wl["domain"]), c["pos"], c["is_boundary"], c["tags"])
+    st = Stepper(ctx, g, wl["scheme"])
+    st.upload(wl["U0"])
+    for _ in range(5):   # the chain is on from the third step
+        st.step_cfl(wl["cfl"])
+    U_b = wl["U0"].copy()
+    U_b[:, 0] *= 1.1
+    U_b[:, 3] *= 1.1
+    st.upload(U_b)
+    o = O.OracleSim(wl["domain"], c["pos"], c["is_boundary"], c["tags"], wl["scheme"], threads=8)
+    o.set_state(U_b)
+    t = 0.0
+    for k in range(1, 7):
+        st.step_cfl(wl["cfl"])
+        dt = wl["cfl"] * o.time_step()
+        assert st.last_dt() == dt, f"step {k} after the upload"
+        o.step(t, dt)
+        t += dt
+        U, U_o = st.download(), o.state()
+        assert np.array_equal(U, U_o), f"step {k} after the upload: {first_diff(U, U_o)}"
+        assert np.array_equal(st.orders(), o.orders()), f"step {k} after the upload"
+
+
 def test_step_totals_match_per_step_values(ctx):
     """hcc_step returns once its work is queued; the per-step getters and the summed totals see the same steps."""
     wl = W.euler_2d(n=64, seed=2)
