@@ -705,7 +705,12 @@ __device__ __forceinline__ void append_cand(const StepArgs& a, bool c, int64_t i
 // would fold (only signs of zeros could differ, which the DMP comparisons cannot see). The starved-order rule and the
 // gradient store are those of gather_node.
 template <int D, int M>
-__device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i) {
+// store: the gradient record is written. A flat gather block of stage s >= 2 (every halo particle had a constant
+// neighbourhood at stage s - 1) may leave it: the stage s - 1 gather wrote this particle the same fit of zero
+// accumulators at the same order (its slots were all flat), and a MOOD drop in between (2 -> 1) wrote the zero record of
+// order 1 (k_mark_gather, k_mark_dense), which is what this gather would store; the indicator never lowers such a
+// particle (no candidate). At stage 1 the orders were reset to the maximum, so the record is stored.
+__device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i, bool store = true) {
     using L = Layout<D, M>;
     const int64_t N = a.g.N;
     const unsigned zf = a.zfit ? a.zfit[i] : 0u;
@@ -736,7 +741,7 @@ __device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i) {
             }
         }
     }
-    store_rec<L::NG, L::GS>(a.G, i, g);
+    if (store) store_rec<L::NG, L::GS>(a.G, i, g);
 }
 
 // The gather pass over all interior particles (full) or the `changed` mask (TimeSteppers.jl:329-334).
@@ -800,8 +805,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     // a flat gather block (uniform per workgroup): no halo, no slot loops; with CFL its minimum in bmin is kept. With
     // FEAT (the indicator on, no collection) its particles are no candidates (every neighbourhood is constant), so
     // no feature of theirs is read and none is stored.
-    if (a.gflat && a.gflat[lb]) {
-        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M>(a, i);
+    if (a.gflat && a.gflat[lb]) {   // without CFL: stages 2 .. s (the stage-1 gather of a chain runs with CFL)
+        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M>(a, i, CFL);
         return;
     }
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
