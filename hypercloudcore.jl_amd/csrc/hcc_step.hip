@@ -989,33 +989,43 @@ struct PredArgs {
     // buffer holds what the last step left there, and the zero-K bits describe its K's); kmask: the bits of all stages
     int first;
     unsigned kmask;
+    int pass1;   // the first full flux pass of its stage (0: a MOOD recompute)
 };
-// zero-K bit 7: every K of the last step was exactly +-0 (set by the stage-1 flux pass from the last step's bits)
-constexpr unsigned kZLastStep = 0x80u;
+// Zero-K byte, above the stage bits (bit s: K_s exactly +-0): bit 5, this step at every stage so far the stage state
+// stayed (shortcut of fused_predict: every term zero, no zero component) and MOOD's check passed; bit 6, the last
+// step's bit 5 (taken at this step's first stage-1 pass); bit 7, every K of the last step exactly +-0 (likewise).
+constexpr unsigned kZStableNow = 0x20u, kZStableLast = 0x40u, kZLastStep = 0x80u;
 
+// Every term of the stage predictor exactly +-0 (the zero-K bits kzv of the used terms, this pass's K included) and
+// the stage state fi = Y_s(n) of the pass free of zeros and finite: Y_s(n) is U^n(n) plus +-0 terms, so its non-zero
+// components are U^n's, and so is the new stage state; it is fi itself (layouts whose written pairs hold the state
+// only: no position or padding to fill).
+template <int D, int M>
+__device__ __forceinline__ bool stage_stays(const PredArgs& p, int64_t n, unsigned kzv, const double (&fi)[M]) {
+    constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
+    if constexpr (O == 0 && 2 * NP == M) {
+        const double dt = node_dt(p.dtp, p.inst, n);
+        bool same = true;
+#pragma unroll
+        for (int j = 0; j < kMaxStages; ++j)
+            if (j < p.nterm && ((p.use >> j) & 1u)) same = same && ((kzv >> j) & 1u) && isfinite(dt * p.at[j]);
+#pragma unroll
+        for (int m = 0; m < M; ++m) same = same && fi[m] != 0.0 && isfinite(fi[m]);
+        return same;
+    }
+    return false;
+}
+
+// same: stage_stays held (the new stage state is fi, and U^n's record is not read)
 template <int D, int M>
 __device__ __forceinline__ void fused_predict(const PredArgs& p, int64_t n, const double (&Kn)[M],
-                                              uint8_t* kz = nullptr, uint8_t kzv = 0, const double* fi = nullptr) {
+                                              uint8_t* kz = nullptr, uint8_t kzv = 0, const double* fi = nullptr,
+                                              bool same = false) {
     using L = Layout<D, M>;
     constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
     double r[2 * NP];
     bool skip_own = false;
     const double dt = node_dt(p.dtp, p.inst, n);
-    // Every term exactly +-0 (the zero-K bits of the used terms, this pass's K included) and the stage state fi = Y_s(n)
-    // of the pass free of zeros and finite: Y_s(n) is U^n(n) plus +-0 terms, so its non-zero components are U^n's,
-    // and so is the new stage state; it is fi itself, and U^n's record is not read. (Layouts whose written pairs hold
-    // the state only: no position or padding to fill.)
-    bool same = false;
-    if constexpr (O == 0 && 2 * NP == M) {
-        if (kz && fi) {
-            same = true;
-#pragma unroll
-            for (int j = 0; j < kMaxStages; ++j)
-                if (j < p.nterm && ((p.use >> j) & 1u)) same = same && ((kzv >> j) & 1u) && isfinite(dt * p.at[j]);
-#pragma unroll
-            for (int m = 0; m < M; ++m) same = same && fi[m] != 0.0 && isfinite(fi[m]);
-        }
-    }
     if (same) {
 #pragma unroll
         for (int m = 0; m < M; ++m) r[m] = fi[m];
@@ -1321,53 +1331,62 @@ __device__ __forceinline__ bool flux_tail(const StepArgs& a, const PredArgs& pre
     using P = Pde<D, M, KIND>;
 #pragma unroll
     for (int m = 0; m < M; ++m) K[i * M + m] = Kout[m];
+    unsigned v = 0;       // the zero-K byte (stored after the check, which decides bit 5)
+    bool same = false;    // stage_stays
     if (pred.on) {
-        uint8_t kzv = 0;
         if (a.kz) {   // bit s of kz: K_s(i) is +-0 (s: this pass's K, the last term of the predictor)
             bool z = true;
 #pragma unroll
             for (int m = 0; m < M; ++m) z = z && Kout[m] == 0.0;
             const int st = pred.nterm - 1;
             const unsigned old = a.kz[i];
-            if (st == 0) {   // bit 7 from the last step's bits before this step's first pass overwrites them
-                const unsigned last = pred.first ? (((old & pred.kmask) == pred.kmask) ? kZLastStep : 0u)
-                                                 : (old & kZLastStep);
-                kzv = (uint8_t)((unsigned)z | last);
+            if (st == 0) {   // bits 6 and 7 from the last step's byte before this step's first pass overwrites it
+                const unsigned last = pred.first ? ((((old & pred.kmask) == pred.kmask) ? kZLastStep : 0u) |
+                                                    ((old & kZStableNow) ? kZStableLast : 0u))
+                                                 : (pred.pass1 ? 0u : (old & (kZLastStep | kZStableLast)));
+                v = (unsigned)z | last | (pred.pass1 ? kZStableNow : (old & kZStableNow));
             } else {
-                kzv = (uint8_t)((old & ~(1u << st)) | ((unsigned)z << st));
+                v = (old & ~(1u << st)) | ((unsigned)z << st);
             }
-            a.kz[i] = kzv;
+            same = stage_stays<D, M>(pred, i, v, fi);
         }
-        fused_predict<D, M>(pred, i, Kout, a.kz, kzv, fi);
+        fused_predict<D, M>(pred, i, Kout, a.kz, (uint8_t)v, fi, same);
     }
-    if (!CHECK) return false;
-    double W[M];
+    // MOOD's check of the Euler substep
+    bool flag = false, adm = true;
+    if constexpr (CHECK) {
+        double W[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
-    const bool adm = P::admissible(a.pde, W);
-    a.inadm[i] = !adm;
-    bool flag = !adm;
-    if (!flag) {
-        if (a.criterion == HCC_CRIT_ONLY) {
-            flag = true;
-        } else if (a.criterion == HCC_CRIT_U1) {
-            // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
-            // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
-            double mx[M], mn[M];
-            if constexpr (CNB) {
+        for (int m = 0; m < M; ++m) W[m] = fi[m] + h * Kout[m];
+        adm = P::admissible(a.pde, W);
+        a.inadm[i] = !adm;
+        flag = !adm;
+        if (!flag) {
+            if (a.criterion == HCC_CRIT_ONLY) {
+                flag = true;
+            } else if (a.criterion == HCC_CRIT_U1) {
+                // the DMP bounds: extrema of Y_{i-1} over {i} and its neighbours, stored by this stage's full gather
+                // pass (the limiter's local extrema over the same set; the fused check only runs with them stored)
+                double mx[M], mn[M];
+                if constexpr (CNB) {
 #pragma unroll
-                for (int m = 0; m < M; ++m) { mx[m] = fi[m]; mn[m] = fi[m]; }
-            } else {
-                double e[2 * M];
-                load_rec<2 * M, 2 * M>(a.ext, i, e);
+                    for (int m = 0; m < M; ++m) { mx[m] = fi[m]; mn[m] = fi[m]; }
+                } else {
+                    double e[2 * M];
+                    load_rec<2 * M, 2 * M>(a.ext, i, e);
 #pragma unroll
-                for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+                    for (int m = 0; m < M; ++m) { mx[m] = e[m]; mn[m] = e[M + m]; }
+                }
+                const double dl = a.delta, d3 = dl * dl * dl;
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
             }
-            const double dl = a.delta, d3 = dl * dl * dl;
-#pragma unroll
-            for (int m = 0; m < M; ++m)
-                if (fabs(mx[m] - mn[m]) >= d3 && (W[m] < mn[m] - dl || W[m] > mx[m] + dl)) flag = true;
         }
+    }
+    if (pred.on && a.kz) {
+        if (!(same && (!CHECK || (adm && !flag)))) v &= ~kZStableNow;
+        a.kz[i] = (uint8_t)v;
     }
     if (flag && a.orders[i] > 1) { a.trig[i] = 1; return true; }
     return false;
@@ -1471,6 +1490,31 @@ template <int D, int M, int KIND, bool CHECK>
 __device__ __forceinline__ bool flux_node_flat(const StepArgs& a, const PredArgs& pred, const double* __restrict__ Y,
                                                double* __restrict__ K, int64_t i, double h) {
     using L = Layout<D, M>;
+    constexpr int P0 = D / 2, P1 = (D + M - 1) / 2, NP = P1 - P0 + 1, O = D - 2 * P0;
+    // A particle that stayed through the whole last step (every K exactly zero, its state free of zeros and finite,
+    // every check passed: bits 7 and 6, or at stage 1 the last step's stage bits and bit 5) and through this step's
+    // earlier stages (their bits and bit 5), in the first pass of a chain, without a source and without outflow
+    // ghosts. Its stage state is U^n = U^{n-1} and the last step's stage state, bit for bit; its K here is the zero fit,
+    // +0 in every component; the check sees last step's Euler substep (the state plus h (+0)) and passes again; the
+    // predictor's target already holds the state (bit 7). The pass stores the K row, the byte and the check's
+    // inadmissible flag as the general path would, and reads nothing of the particle's record.
+    if constexpr (O == 0 && 2 * NP == M) {
+        if (a.kz && pred.on && pred.first && a.src_mode == HCC_SRC_NONE) {
+            const unsigned old = a.kz[i];
+            const int st = pred.nterm - 1;
+            const unsigned low = (1u << st) - 1u;
+            const bool last = st == 0 ? ((old & pred.kmask) == pred.kmask && (old & kZStableNow))
+                                      : ((old & kZLastStep) && (old & kZStableLast));
+            const bool now = st == 0 || ((old & low) == low && (old & kZStableNow));
+            if (last && now && pred.goff[i + 1] == pred.goff[i]) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) K[i * M + m] = 0.0;
+                a.kz[i] = (uint8_t)(st == 0 ? (1u | kZStableNow | kZStableLast | kZLastStep) : (old | (1u << st)));
+                if constexpr (CHECK) a.inadm[i] = 0;
+                return false;
+            }
+        }
+    }
     double ri[L::NR], fi[M], Kout[M];
     load_rec<L::NR, L::PU>(Y, i, ri);
 #pragma unroll
@@ -3574,6 +3618,7 @@ struct StepRunner {
         p.count = count ? 1 : 0;
         p.first = 0;
         p.kmask = (1u << s->desc.stages) - 1u;
+        p.pass1 = 0;
         return p;
     }
     // the unfused check's count of changed particles (counters[0]), published into pinned slot 0 and polled
@@ -3759,6 +3804,7 @@ struct StepRunner {
             if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 1], str);   // end of the gather = start of the flux pass
             PredArgs p = fuse_pred() ? pred_args(i + 1, fused()) : PredArgs{};
             p.first = chain0 ? 1 : 0;
+            p.pass1 = 1;
             ops.flux(s, flux_args(i - 1), p, Y, s->K[i - 1], 1, fused());
             if (evp) (void)hipEventRecord(evp[4 * (i - 1) + 3], str);
         }
