@@ -148,6 +148,7 @@ struct StepArgs {
                                  // slot of the block's flux pass is flat (k_block_flat; nullptr: not used)
     uint8_t* kz;                 // bit s: the particle's K of stage s (0-based) is exactly +-0 in every component (the
                                  // fused predictor's passes write it, ghosts copy their root's; nullptr: not tracked)
+    unsigned kzmask;             // the stage bits of kz: (1 << stages) - 1
     const uint8_t* gflat;        // per kStage block: the stage's full gather is flat (k_block_gflat; nullptr: not used)
     const int32_t* border;       // the staged flux pass's and stage-1 gather's blocks in launch order: the order of the
                                  // stage's gather (nullptr: the XCD-aware order of the block indices)

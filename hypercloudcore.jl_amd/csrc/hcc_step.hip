@@ -697,6 +697,11 @@ __device__ __forceinline__ void append_cand(const StepArgs& a, bool c, int64_t i
     }
 }
 
+// Zero-K byte, above the stage bits (bit s: K_s exactly +-0): bit 5, this step at every stage so far the stage state
+// stayed (shortcut of fused_predict: every term zero, no zero component) and MOOD's check passed; bit 6, the last
+// step's bit 5 (taken at this step's first stage-1 pass); bit 7, every K of the last step exactly +-0 (likewise).
+constexpr unsigned kZStableNow = 0x20u, kZStableLast = 0x40u, kZLastStep = 0x80u;
+
 // gather_node for a particle of a flat gather block (k_block_gflat): every neighbour's state equals U_i, so the fit
 // accumulators stay +0 and the gradient is the inverse times zero (g = 0 without a fit; the zeros' signs from zfit),
 // which no slot limits. The extrema are not stored again: the particle had a constant neighbourhood (cnb) in the state
@@ -713,6 +718,14 @@ template <int D, int M>
 __device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i, bool store = true) {
     using L = Layout<D, M>;
     const int64_t N = a.g.N;
+    // The stage-1 flat gather of a chained step (the last step's K's all zero on the halo and its last gather's
+    // constant neighbourhoods): a particle that stayed through the whole last step (every K zero, every check passed:
+    // never dropped) kept order 2 or the starved order 1 there, and each of its gathers wrote, or as a flat gather of a
+    // later stage left, the zero fit at that order; the reset order is the same, so is the record, and it is not stored.
+    if (store && a.kz) {
+        const unsigned old = a.kz[i];
+        if ((old & a.kzmask) == a.kzmask && (old & kZStableNow)) store = false;
+    }
     const unsigned zf = a.zfit ? a.zfit[i] : 0u;
     const bool starved = a.zfit ? (zf & kZStarved) != 0u : a.g.cnt[i] < D;
     int order = a.orders[i];
@@ -991,10 +1004,7 @@ struct PredArgs {
     unsigned kmask;
     int pass1;   // the first full flux pass of its stage (0: a MOOD recompute)
 };
-// Zero-K byte, above the stage bits (bit s: K_s exactly +-0): bit 5, this step at every stage so far the stage state
-// stayed (shortcut of fused_predict: every term zero, no zero component) and MOOD's check passed; bit 6, the last
-// step's bit 5 (taken at this step's first stage-1 pass); bit 7, every K of the last step exactly +-0 (likewise).
-constexpr unsigned kZStableNow = 0x20u, kZStableLast = 0x40u, kZLastStep = 0x80u;
+
 
 // Every term of the stage predictor exactly +-0 (the zero-K bits kzv of the used terms, this pass's K included) and
 // the stage state fi = Y_s(n) of the pass free of zeros and finite: Y_s(n) is U^n(n) plus +-0 terms, so its non-zero
@@ -2791,6 +2801,7 @@ static StepArgs step_args(const hcc_stepper* s) {
     const bool fused_pred = !s->imex && !s->moving && (s->desc.mood_criterion == HCC_CRIT_NONE || mood_fused(s->desc));
     // zero-K bits (the flat gather blocks): not while training data is collected (its copies take every feature)
     if (s->flat_blocks && s->tune.staged && s->lds_gather && !s->collect && fused_pred) a.kz = s->kz;
+    a.kzmask = (1u << s->desc.stages) - 1u;
     return a;
 }
 
