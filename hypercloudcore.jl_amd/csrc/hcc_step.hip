@@ -715,14 +715,16 @@ template <int D, int M>
 // accumulators at the same order (its slots were all flat), and a MOOD drop in between (2 -> 1) wrote the zero record of
 // order 1 (k_mark_gather, k_mark_dense), which is what this gather would store; the indicator never lowers such a
 // particle (no candidate). At stage 1 the orders were reset to the maximum, so the record is stored.
-__device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i, bool store = true) {
+__device__ __forceinline__ void gather_node_flat(const StepArgs& a, int64_t i, bool store = true,
+                                                 bool keep_stable = false) {
     using L = Layout<D, M>;
     const int64_t N = a.g.N;
-    // The stage-1 flat gather of a chained step (the last step's K's all zero on the halo and its last gather's
-    // constant neighbourhoods): a particle that stayed through the whole last step (every K zero, every check passed:
-    // never dropped) kept order 2 or the starved order 1 there, and each of its gathers wrote, or as a flat gather of a
-    // later stage left, the zero fit at that order; the reset order is the same, so is the record, and it is not stored.
-    if (store && a.kz) {
+    // keep_stable: the stage-1 flat gather of a chained step (the last step's K's all zero on the halo and its last
+    // gather's constant neighbourhoods) without the learned indicator, which lowers orders without a gather. A
+    // particle that stayed through the whole last step (every K zero, every check passed: never dropped) kept order 2
+    // or the starved order 1 there, and each of its gathers wrote, or as a flat gather of a later stage left, the zero
+    // fit at that order; the reset order is the same, so is the record, and it is not stored.
+    if (store && keep_stable && a.kz) {
         const unsigned old = a.kz[i];
         if ((old & a.kzmask) == a.kzmask && (old & kZStableNow)) store = false;
     }
@@ -819,7 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
     // FEAT (the indicator on, no collection) its particles are no candidates (every neighbourhood is constant), so
     // no feature of theirs is read and none is stored.
     if (a.gflat && a.gflat[lb]) {   // without CFL: stages 2 .. s (the stage-1 gather of a chain runs with CFL)
-        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M>(a, i, CFL);
+        if (i < a.g.N && !a.g.isb[i]) gather_node_flat<D, M>(a, i, CFL, CFL && !FEAT);
         return;
     }
     const auto src = stage_halo<D, M, false, CFL, KIND>(a, Y, sh, lb);
