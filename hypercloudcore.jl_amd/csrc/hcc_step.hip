@@ -841,6 +841,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_staged(StepArgs a, const doub
 // mapping then applies to positions in this list. One workgroup: each thread counts a contiguous range of flags
 // (16 per load), a scan of the counts (shuffles in each wave, then the four wave totals) places every block. Results
 // do not depend on the order (each block writes its own particles).
+constexpr uint32_t kOrderEvery = 4;   // uncaptured steps between new launch orders
 __global__ __launch_bounds__(kBlock) void k_block_order(const uint8_t* __restrict__ flat, int32_t* __restrict__ order,
                                                         int64_t nblk) {
     __shared__ int32_t wsum[kBlock / 64];
@@ -2721,6 +2722,11 @@ struct hcc_stepper {
     uint8_t* kz = nullptr;     // per particle: which stages' K are exactly zero (N, boundary particles 0xFF) ...
     uint8_t* gflat = nullptr;  // ... and the flat blocks of the gathers of stages 2 .. s
     int32_t* gorder = nullptr; // launch order of the staged gathers and flux passes (k_block_order)
+    // Uncaptured steps list a new launch order every kOrderEvery steps: the non-flat region moves by about a spacing
+    // a step and a block spans 16 spacings, so an order a few steps old still starts nearly every long workgroup
+    // first, and any order gives the same results (every list is a permutation of the blocks)
+    bool order_ok = false;      // gorder holds a listed order (set up as the identity; written by an uncaptured step)
+    uint32_t order_serial = 0;  // uncaptured steps run
     uint8_t* zfit = nullptr;   // what the fits give for zero accumulators (k_zero_fits, with the moments)
     bool flat_blocks = false;  // the full gathers record cnb and k_block_flat marks the staged flux pass's flat blocks
     // dynamic LDS of the staged full passes (0: the pass reads neighbours from HBM)
@@ -3511,6 +3517,7 @@ struct StepRunner {
     // k_fixed_point_overflow; no pass events. Uncaptured: nonempty[i] counts the iterations that had work.
     bool graph = false;
     bool ordered = false;   // this step has listed its launch order (from the first gather-flag kernel it ran)
+    bool reorder = true;    // this step lists a new launch order (captured steps always do)
     bool chain0 = false;    // the step continues an unbroken chain of steps (PredArgs::first of its full flux passes)
     int nonempty[kMaxStages + 2] = {};
     bool substeps() const { return s->desc.substep_ratio > 0.0 && s->desc.mood_criterion != HCC_CRIT_NONE; }
@@ -3763,6 +3770,11 @@ struct StepRunner {
         k_count_inadm<<<grid_for(nb), kBlock, 0, str>>>(a, nb);
         return hipGetLastError();
     }
+    // a new launch order from this stage's flat flags
+    void list_order(int64_t nb) {
+        k_block_order<<<1, kBlock, 0, s->ctx->stream>>>(s->gflat, s->gorder, nb);
+        if (!graph) s->order_ok = true;
+    }
     hipError_t run() {
         hcc_ctx* ctx = s->ctx;
         const int st = s->desc.stages;
@@ -3772,6 +3784,8 @@ struct StepRunner {
         // this uncaptured fused CFL step continues an unbroken chain (cfl_chain: the last step was one too, with the
         // zero-K bits tracked, and nothing has touched the state since; an upload makes the next step unfused)
         chain0 = s->cfl_chain && s->cfl_fused && a.kz != nullptr && !graph;
+        reorder = graph || !s->order_ok || s->order_serial % kOrderEvery == 0;
+        if (!graph) ++s->order_serial;
         // The stage-1 gather resets orders, flags and counters. Gradients: every full gather pass writes each interior
         // particle's record; ghosts keep the zeros written at set-up (the reference's gather skips boundary
         // particles). The full gather also clears the changed count for the check of the flux pass.
@@ -3793,7 +3807,7 @@ struct StepRunner {
                 // and constant neighbourhoods in Y_s, U equals Y_s (and the last step's U) in value, so the block's
                 // gather is flat and its CFL minimum is the one the last stage-1 gather stored
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << st) - 1u, s->gflat, nb);
-                k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                if (reorder) list_order(nb);
                 ordered = true;
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
@@ -3803,7 +3817,7 @@ struct StepRunner {
             else if (i >= 2 && a.kz) {   // the flat blocks of this gather: K_1 .. K_{i-1} zero on their halos
                 k_block_gflat<<<(unsigned)nb, kBlock, 0, str>>>(a.g, s->cnb, s->kz, (1u << (i - 1)) - 1u, s->gflat, nb);
                 // one launch order a step: the non-flat region moves by about a spacing a step
-                if (!ordered) k_block_order<<<1, kBlock, 0, str>>>(s->gflat, s->gorder, nb);
+                if (!ordered && reorder) list_order(nb);
                 ordered = true;
                 StepArgs ag = a;
                 ag.gflat = s->gflat;
